@@ -1,0 +1,218 @@
+/*
+ * ompl_amd.h -- C-ABI of libompl_amd, the MI355X (gfx950) implementation of
+ * OMPL's data-parallel planning inner loop.  Plain pointers and sizes only.
+ *
+ * Every function returns OMPL_AMD_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is ompl_amd_last_error().  The C++
+ * adapters (ompl_amd/include/ompl_amd/ompl_adapters.hpp) rethrow failures as
+ * ompl::Exception, the reference's error type (util/Exception.h:46).
+ *
+ * Pointer conventions: parameters named *_dev are device pointers (HBM);
+ * all others are host pointers.  `stream` is a hipStream_t (NULL = HIP's
+ * null stream).  *_dev entry points are asynchronous on `stream`;
+ * host-pointer entry points return after the result is in host memory.
+ *
+ * State layouts: joint configurations are structure-of-arrays, dof-major:
+ * q[j * n + i] is joint j of state i (coalesced loads: lane i reads q[j*n+i]).
+ * Nearest-neighbour points are row-major [n][dim] FP32.
+ */
+#ifndef OMPL_AMD_H
+#define OMPL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMPL_AMD_OK 0
+#define OMPL_AMD_ERR_ARG -1       /* invalid argument (null pointer, bad size) */
+#define OMPL_AMD_ERR_HIP -2       /* HIP runtime error */
+#define OMPL_AMD_ERR_NOMEM -3     /* device or host allocation failed */
+#define OMPL_AMD_ERR_EMPTY -4     /* nearest() on an empty structure */
+#define OMPL_AMD_ERR_ADMISSION -5 /* serving: request exceeds the HBM budget */
+#define OMPL_AMD_ERR_DEADLINE -6  /* serving: request deadline expired */
+#define OMPL_AMD_ERR_CANCELLED -7 /* serving: request cancelled */
+#define OMPL_AMD_ERR_INTERNAL -8
+
+/* ------------------------------------------------------------------ */
+/* Library                                                             */
+/* ------------------------------------------------------------------ */
+int ompl_amd_init(int device);
+const char *ompl_amd_last_error(void);
+int ompl_amd_version(void); /* major*10000 + minor*100 + patch */
+int ompl_amd_synchronize(void *stream);
+
+/* Device memory helpers (thin wrappers over hipMalloc/hipMemcpy) for callers
+ * that do not own a HIP runtime binding. */
+int ompl_amd_malloc(void **ptr_dev, size_t bytes);
+int ompl_amd_free(void *ptr_dev);
+int ompl_amd_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
+int ompl_amd_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+
+/* ------------------------------------------------------------------ */
+/* Environments (obstacle sets, device resident)                       */
+/* ------------------------------------------------------------------ */
+typedef struct ompl_amd_env_s *ompl_amd_env;
+
+/* spheres: n_spheres x (cx, cy, cz, radius); boxes: n_boxes x (cx, cy, cz,
+ * hx, hy, hz) axis-aligned, half-extents.  Host pointers, copied to HBM. */
+int ompl_amd_env_create(const float *spheres, int n_spheres, const float *boxes, int n_boxes,
+                        ompl_amd_env *out);
+int ompl_amd_env_destroy(ompl_amd_env env);
+/* device bytes held by the environment */
+int ompl_amd_env_bytes(ompl_amd_env env, size_t *bytes);
+
+/* ------------------------------------------------------------------ */
+/* State validity -- replaces ompl::base::StateValidityChecker::isValid */
+/* (base/StateValidityChecker.h:111) for the 7-DOF sphere arm; the      */
+/* FP64->FP32 cast follows vamp/Utils.h:26.                             */
+/* ------------------------------------------------------------------ */
+#define OMPL_AMD_ARM_DOF 7
+#define OMPL_AMD_ARM_SPHERES 28
+
+/* Batched: q_soa_dev FP32 [7][n]; out_dev[i] = 1 if state i is collision free. */
+int ompl_amd_validate_states_dev(ompl_amd_env env, const float *q_soa_dev, int64_t n, uint8_t *out_dev,
+                                 void *stream);
+/* FP64 states (OMPL's StateType::values), cast per joint to FP32 on device. */
+int ompl_amd_validate_states_f64_dev(ompl_amd_env env, const double *q_soa_dev, int64_t n, uint8_t *out_dev,
+                                     void *stream);
+/* Plain single call: one state (7 doubles), one launch, blocking. */
+int ompl_amd_is_valid(ompl_amd_env env, const double *q, int *valid);
+/* Robot sphere centres for FP32 states (debug/inspection): out_dev [n][28][3]. */
+int ompl_amd_arm_spheres_dev(const float *q_soa_dev, int64_t n, float *out_dev, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Motion validation -- replaces DiscreteMotionValidator::checkMotion   */
+/* (base/src/DiscreteMotionValidator.cpp:93-145, and :48-91 for the     */
+/* lastValid overload).  nd = factor * ceil(|b - a| / lvs) in FP64      */
+/* (base/src/StateSpace.cpp:851-854); interpolation in FP64             */
+/* (base/spaces/src/RealVectorStateSpace.cpp:257-265), then FP32.       */
+/* ------------------------------------------------------------------ */
+/* a_soa_dev, b_soa_dev: FP64 [7][n] edge endpoints (a assumed valid).
+ * out_dev[i] = 1 if the motion is valid.  checked_dev (optional) receives
+ * the number of states the reference checks for that edge: s2, then the
+ * bisection positions up to and including the first invalid one. */
+int ompl_amd_check_motions_dev(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
+                               double longest_valid_segment, unsigned factor, uint8_t *out_dev,
+                               int32_t *checked_dev, void *stream);
+/* lastValid variant: last_t_dev[i] = (j-1)/nd for the first invalid j of the
+ * sequential scan, (nd-1)/nd when only s2 is invalid; untouched when valid. */
+int ompl_amd_check_motions_last_valid_dev(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev,
+                                          int64_t n, double longest_valid_segment, unsigned factor,
+                                          uint8_t *out_dev, double *last_t_dev, void *stream);
+/* Plain single call (host pointers, one launch, blocking): the GPU path a
+ * planner gets when every si->checkMotion(s1, s2) becomes its own launch. */
+int ompl_amd_check_motion(ompl_amd_env env, const double *a, const double *b, double longest_valid_segment,
+                          unsigned factor, int *valid);
+int ompl_amd_check_motion_last_valid(ompl_amd_env env, const double *a, const double *b,
+                                     double longest_valid_segment, unsigned factor, int *valid, double *last_t);
+
+/* Launch-shape knobs used by the batched kernels (set by the autotuner).
+ * lanes_per_state in {1,2,4,8}; block in {64,128,256,512,1024}. */
+typedef struct {
+    int validity_lanes_per_state;
+    int validity_block;
+    int motion_lanes_per_state;
+    int motion_block;
+    int motion_waves_per_cu; /* persistent interior kernel occupancy */
+    int motion_two_phase;    /* 1: s2 pass over all edges, then walk the survivors;
+                                0: one wave walks each edge from s2 */
+} ompl_amd_tuning;
+int ompl_amd_get_tuning(ompl_amd_tuning *t);
+int ompl_amd_set_tuning(const ompl_amd_tuning *t);
+
+/* Work a scalar checker performs on the same inputs in the reference order
+ * (s2, then bisection positions up to the first invalid state; per state the
+ * robot spheres in order against sphere then box obstacles, stopping at the
+ * first hit): out[0] states, out[1] sphere-pair tests, out[2] box-pair tests.
+ * Used for roofline accounting; blocking; no counterpart in the reference. */
+int ompl_amd_motion_work(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
+                         double longest_valid_segment, unsigned factor, int64_t out[3]);
+int ompl_amd_validity_work(ompl_amd_env env, const float *q_soa_dev, int64_t n, int64_t out[3]);
+
+/* Per-kernel timing with HIP events recorded around each launch on its own
+ * stream (off by default).  Names: "validate_states", "motion_endpoint",
+ * "motion_walk", "nn_knn", "nn_radius", "sample_uniform", ... */
+int ompl_amd_profile_enable(int on);
+int ompl_amd_profile_reset(void);
+int ompl_amd_profile_query(const char *kernel, int64_t *launches, double *total_ms);
+
+/* ------------------------------------------------------------------ */
+/* State sampling -- RealVectorStateSampler::sampleUniform              */
+/* (base/spaces/src/RealVectorStateSpace.cpp:45-52) over RNG::uniformReal */
+/* (util/RandomNumbers.h:72-76): std::mt19937 + generate_canonical.     */
+/* ------------------------------------------------------------------ */
+/* n_streams independent mt19937 generators, stream s seeded with seeds[s];
+ * stream s writes n_per_stream states of `dim` doubles, row-major, to
+ * out_dev + s*n_per_stream*dim, exactly as n_per_stream successive
+ * sampleUniform calls of an RNG with that local seed.  Also writes the FP32
+ * dof-major copy to out_f32_soa_dev ([dim][n_streams*n_per_stream]) when
+ * non-null. */
+int ompl_amd_sample_uniform_dev(const uint32_t *seeds, int n_streams, int64_t n_per_stream, int dim,
+                                const double *lo, const double *hi, double *out_dev, float *out_f32_soa_dev,
+                                void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Nearest neighbours -- ompl::NearestNeighbors<_T>                      */
+/* (datastructures/NearestNeighbors.h:46-115), exact results in FP32,   */
+/* ordered by (distance, index), radius inclusive (<=) as               */
+/* NearestNeighborsLinear.h:135-142.                                    */
+/* ------------------------------------------------------------------ */
+#define OMPL_AMD_METRIC_L2 0  /* RealVectorStateSpace::distance */
+#define OMPL_AMD_METRIC_SE3 1 /* (x,y,z,qx,qy,qz,qw): R3 + SO3 arc, weights 1,1 */
+
+typedef struct ompl_amd_nn_s *ompl_amd_nn;
+
+int ompl_amd_nn_create(int metric, int dim, ompl_amd_nn *out);
+int ompl_amd_nn_destroy(ompl_amd_nn nn);
+int ompl_amd_nn_clear(ompl_amd_nn nn);
+/* Append n points (host row-major [n][dim]); ids are assigned 0,1,2,... in
+ * insertion order (first id returned in *first_id). */
+int ompl_amd_nn_add(ompl_amd_nn nn, const float *pts, int64_t n, int64_t *first_id);
+int ompl_amd_nn_add_dev(ompl_amd_nn nn, const float *pts_dev, int64_t n, int64_t *first_id, void *stream);
+/* Remove by id (lazy: the id is excluded from every later query). */
+int ompl_amd_nn_remove(ompl_amd_nn nn, int64_t id, int *removed);
+int ompl_amd_nn_size(ompl_amd_nn nn, int64_t *size);
+/* Rebuild the acceleration structure after adds (called implicitly by queries). */
+int ompl_amd_nn_build(ompl_amd_nn nn, void *stream);
+/* k nearest of nq queries (device row-major [nq][dim]); out_idx_dev [nq][k]
+ * (-1 past the end), out_dist_dev [nq][k] (+inf past the end). */
+int ompl_amd_nn_knn_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, int k, int32_t *out_idx_dev,
+                        float *out_dist_dev, void *stream);
+/* Radius search.  Two-pass CSR: out_offsets_dev [nq+1] is always written; if
+ * out_idx_dev is non-null it must hold out_offsets[nq] entries (call once
+ * with nulls to size it, or pass capacity >= total). */
+int ompl_amd_nn_radius_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, float radius,
+                           int64_t *out_offsets_dev, int32_t *out_idx_dev, float *out_dist_dev, int64_t capacity,
+                           int64_t *total, void *stream);
+/* Host convenience (blocking). */
+int ompl_amd_nn_knn(ompl_amd_nn nn, const float *q, int64_t nq, int k, int32_t *out_idx, float *out_dist);
+int ompl_amd_nn_nearest(ompl_amd_nn nn, const float *q, int64_t *out_id);
+
+/* RRT* neighbourhood radius (geometric/planners/rrt/src/RRTstar.cpp:602-617,
+ * 1147-1160): min(max_distance, r_rrt * (log(n+1)/(n+1))^(1/d)),
+ * r_rrt = rewire * (2 (1 + 1/d) measure / unitNBallMeasure(d))^(1/d). */
+double ompl_amd_rrtstar_radius(int dim, double measure, double rewire_factor, double max_distance, int64_t n);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic workloads (std::mt19937 + OMPL RNG semantics, host side)   */
+/* ------------------------------------------------------------------ */
+/* Obstacles: centres U[-1,1]^3 rejected while |c| < min_base_dist; sphere
+ * radii U[r_lo, r_hi]; box half-extents U[h_lo, h_hi] per axis. */
+int ompl_amd_workload_obstacles(uint32_t seed, int n_spheres, int n_boxes, double r_lo, double r_hi, double h_lo,
+                                double h_hi, double min_base_dist, float *spheres_out, float *boxes_out);
+/* n joint configs U[-pi,pi]^7, FP64 row-major [n][7] (sampleUniform order). */
+int ompl_amd_workload_configs(uint32_t seed, int64_t n, double *out);
+/* n edges: a U[-pi,pi]^7, direction uniform on S^6, length exp(N(mu, sigma))
+ * clipped to [len_lo, len_hi], b = clamp(a + len*dir, -pi, pi); row-major. */
+int ompl_amd_workload_edges(uint32_t seed, int64_t n, double mu, double sigma, double len_lo, double len_hi,
+                            double *a_out, double *b_out);
+/* n SE3 states: translation U[lo,hi]^3 then RNG::quaternion (x,y,z,w). */
+int ompl_amd_workload_se3(uint32_t seed, int64_t n, double lo, double hi, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

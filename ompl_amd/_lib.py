@@ -1,0 +1,128 @@
+"""ctypes binding of libompl_amd.so (the C-ABI declared in include/ompl_amd.h).
+
+The library is loaded from the package directory only; there is no fallback:
+if libompl_amd.so is missing or does not load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libompl_amd.so"
+
+_lib = None
+
+c_int_p = C.POINTER(C.c_int)
+c_i64_p = C.POINTER(C.c_int64)
+c_size_p = C.POINTER(C.c_size_t)
+vp = C.c_void_p
+
+# name -> (restype, argtypes); mirrors include/ompl_amd.h
+SIGNATURES = {
+    "ompl_amd_init": (C.c_int, [C.c_int]),
+    "ompl_amd_last_error": (C.c_char_p, []),
+    "ompl_amd_version": (C.c_int, []),
+    "ompl_amd_synchronize": (C.c_int, [vp]),
+    "ompl_amd_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
+    "ompl_amd_free": (C.c_int, [vp]),
+    "ompl_amd_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
+    "ompl_amd_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+    "ompl_amd_env_create": (C.c_int, [vp, C.c_int, vp, C.c_int, C.POINTER(vp)]),
+    "ompl_amd_env_destroy": (C.c_int, [vp]),
+    "ompl_amd_env_bytes": (C.c_int, [vp, c_size_p]),
+    "ompl_amd_validate_states_dev": (C.c_int, [vp, vp, C.c_int64, vp, vp]),
+    "ompl_amd_validate_states_f64_dev": (C.c_int, [vp, vp, C.c_int64, vp, vp]),
+    "ompl_amd_is_valid": (C.c_int, [vp, vp, c_int_p]),
+    "ompl_amd_arm_spheres_dev": (C.c_int, [vp, C.c_int64, vp, vp]),
+    "ompl_amd_check_motions_dev": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, vp, vp]),
+    "ompl_amd_check_motions_last_valid_dev": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, vp,
+                                                        vp]),
+    "ompl_amd_check_motion": (C.c_int, [vp, vp, vp, C.c_double, C.c_uint, c_int_p]),
+    "ompl_amd_check_motion_last_valid": (C.c_int, [vp, vp, vp, C.c_double, C.c_uint, c_int_p,
+                                                   C.POINTER(C.c_double)]),
+    "ompl_amd_get_tuning": (C.c_int, [vp]),
+    "ompl_amd_set_tuning": (C.c_int, [vp]),
+    "ompl_amd_motion_work": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp]),
+    "ompl_amd_validity_work": (C.c_int, [vp, vp, C.c_int64, vp]),
+    "ompl_amd_profile_enable": (C.c_int, [C.c_int]),
+    "ompl_amd_profile_reset": (C.c_int, []),
+    "ompl_amd_profile_query": (C.c_int, [C.c_char_p, c_i64_p, C.POINTER(C.c_double)]),
+    "ompl_amd_sample_uniform_dev": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int, vp, vp, vp, vp, vp]),
+    "ompl_amd_nn_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
+    "ompl_amd_nn_destroy": (C.c_int, [vp]),
+    "ompl_amd_nn_clear": (C.c_int, [vp]),
+    "ompl_amd_nn_add": (C.c_int, [vp, vp, C.c_int64, c_i64_p]),
+    "ompl_amd_nn_add_dev": (C.c_int, [vp, vp, C.c_int64, c_i64_p, vp]),
+    "ompl_amd_nn_remove": (C.c_int, [vp, C.c_int64, c_int_p]),
+    "ompl_amd_nn_size": (C.c_int, [vp, c_i64_p]),
+    "ompl_amd_nn_build": (C.c_int, [vp, vp]),
+    "ompl_amd_nn_knn_dev": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp, vp]),
+    "ompl_amd_nn_radius_dev": (C.c_int, [vp, vp, C.c_int64, C.c_float, vp, vp, vp, C.c_int64, c_i64_p, vp]),
+    "ompl_amd_nn_knn": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp]),
+    "ompl_amd_nn_nearest": (C.c_int, [vp, vp, c_i64_p]),
+    "ompl_amd_rrtstar_radius": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64]),
+    "ompl_amd_workload_obstacles": (C.c_int, [C.c_uint32, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, vp, vp]),
+    "ompl_amd_workload_configs": (C.c_int, [C.c_uint32, C.c_int64, vp]),
+    "ompl_amd_workload_edges": (C.c_int, [C.c_uint32, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, vp, vp]),
+    "ompl_amd_workload_se3": (C.c_int, [C.c_uint32, C.c_int64, C.c_double, C.c_double, vp]),
+}
+
+
+class OmplAmdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("validity_lanes_per_state", C.c_int),
+        ("validity_block", C.c_int),
+        ("motion_lanes_per_state", C.c_int),
+        ("motion_block", C.c_int),
+        ("motion_waves_per_cu", C.c_int),
+        ("motion_two_phase", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def load(path: str | os.PathLike | None = None):
+    """Loads libompl_amd.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise OmplAmdError(-8, f"{p} is not built; run `python -m ompl_amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_symbols(lib=None) -> dict[str, bool]:
+    lib = lib or load()
+    return {name: hasattr(lib, name) for name in SIGNATURES}
+
+
+def check(code: int) -> None:
+    if code != 0:
+        msg = load().ompl_amd_last_error()
+        raise OmplAmdError(code, msg.decode() if msg else "unknown error")
+
+
+def call(name: str, *args):
+    fn = getattr(load(), name)
+    check(fn(*args))

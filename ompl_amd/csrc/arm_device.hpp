@@ -1,0 +1,211 @@
+// Device-side arm model: FP32 forward kinematics of the 7-DOF sphere arm and
+// the sphere/sphere, sphere/box collision predicates, with the obstacle set
+// staged in LDS.  Operation order (fixed; -ffp-contract=off):
+//
+//   joint i (axis z for even i, y for odd i), (s, c) = sincos_pl(q_i):
+//     z joint: X' = X*c + Y*s,  Y' = Y*c - X*s   (per component)
+//     y joint: X' = X*c - Z*s,  Z' = X*s + Z*c
+//   sphere j of link i: centre = p_i + Z_i * off_j, off = {0.0375, 0.1125,
+//     0.1875, 0.2625} (link 0.3 m, fractions 1/8, 3/8, 5/8, 7/8), r = 0.05
+//   p_{i+1} = p_i + Z_i * 0.3
+//   sphere obstacle (o, R): d = o - centre, d2 = (dx*dx + dy*dy) + dz*dz,
+//     collision iff d2 < rr, rr = (R + 0.05f) * (R + 0.05f)
+//   box obstacle (lo = b - h, hi = b + h): p = min(max(centre, lo), hi),
+//     d = centre - p, d2 = (dx*dx + dy*dy) + dz*dz, collision iff d2 < 0.05f*0.05f
+//
+// The reference's own FP32 checker (VAMP, vamp/VampStateValidityChecker.h:37-41)
+// is an absent submodule; this arm and these predicates are the project's
+// specification of the checker behind StateValidityChecker::isValid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fp32_math.hpp"
+
+namespace ompl_amd {
+
+constexpr int kDof = 7;
+constexpr int kSpheres = 28;
+constexpr float kLink = 0.3f;
+constexpr float kRobotR = 0.05f;
+constexpr float kFar = 1e30f; // centre of an unused sphere slot: never collides
+
+__device__ __forceinline__ float sphere_offset(int j)
+{
+    return j == 0 ? 0.0375f : (j == 1 ? 0.1125f : (j == 2 ? 0.1875f : 0.2625f));
+}
+
+// Obstacles in LDS: [0, ns) spheres (x, y, z, rr); then nb boxes as pairs
+// (lo.xyz, 0), (hi.xyz, 0).
+struct EnvView {
+    const float4 *obs; // device (HBM) copy in the same layout
+    int ns;
+    int nb;
+};
+
+__device__ __forceinline__ int env_words(const EnvView &e)
+{
+    return e.ns + 2 * e.nb;
+}
+
+// Cooperative copy of the environment into LDS by the whole block.
+__device__ __forceinline__ void stage_env(const EnvView &e, float4 *lds)
+{
+    const int nw = env_words(e);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x)
+        lds[i] = e.obs[i];
+    __syncthreads();
+}
+
+// Sphere centres held by one lane: the lane is member g of a group of LPS
+// lanes that together hold the 28 spheres of one state; slot m holds sphere
+// m*LPS + g (link (m*LPS+g)/4, sphere (m*LPS+g)%4).
+template <int LPS>
+struct LaneSpheres {
+    static constexpr int N = (kSpheres + LPS - 1) / LPS;
+    float x[N], y[N], z[N];
+};
+
+template <int LPS>
+__device__ __forceinline__ void arm_fk(const float q[kDof], int g, LaneSpheres<LPS> &S)
+{
+    float X0 = 1.0f, X1 = 0.0f, X2 = 0.0f;
+    float Y0 = 0.0f, Y1 = 1.0f, Y2 = 0.0f;
+    float Z0 = 0.0f, Z1 = 0.0f, Z2 = 1.0f;
+    float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+#pragma unroll
+    for (int m = 0; m < LaneSpheres<LPS>::N; ++m) {
+        S.x[m] = kFar;
+        S.y[m] = kFar;
+        S.z[m] = kFar;
+    }
+#pragma unroll
+    for (int i = 0; i < kDof; ++i) {
+        float s, c;
+        sincos_pl(q[i], s, c);
+        if ((i & 1) == 0) {
+            float nx0 = X0 * c + Y0 * s, ny0 = Y0 * c - X0 * s;
+            float nx1 = X1 * c + Y1 * s, ny1 = Y1 * c - X1 * s;
+            float nx2 = X2 * c + Y2 * s, ny2 = Y2 * c - X2 * s;
+            X0 = nx0; X1 = nx1; X2 = nx2;
+            Y0 = ny0; Y1 = ny1; Y2 = ny2;
+        } else {
+            float nx0 = X0 * c - Z0 * s, nz0 = X0 * s + Z0 * c;
+            float nx1 = X1 * c - Z1 * s, nz1 = X1 * s + Z1 * c;
+            float nx2 = X2 * c - Z2 * s, nz2 = X2 * s + Z2 * c;
+            X0 = nx0; X1 = nx1; X2 = nx2;
+            Z0 = nz0; Z1 = nz1; Z2 = nz2;
+        }
+#pragma unroll
+        for (int m = 0; m < LaneSpheres<LPS>::N; ++m) {
+            const int s_lo = m * LPS, s_hi = m * LPS + LPS - 1;
+            if (s_hi < 4 * i || s_lo > 4 * i + 3) continue; // resolved at compile time
+            const int sph = m * LPS + g;
+            if (sph < kSpheres && (sph >> 2) == i) {
+                const float off = sphere_offset(sph & 3);
+                S.x[m] = p0 + Z0 * off;
+                S.y[m] = p1 + Z1 * off;
+                S.z[m] = p2 + Z2 * off;
+            }
+        }
+        p0 = p0 + Z0 * kLink;
+        p1 = p1 + Z1 * kLink;
+        p2 = p2 + Z2 * kLink;
+    }
+}
+
+template <int LPS>
+__device__ __forceinline__ bool hits_sphere(const LaneSpheres<LPS> &S, float4 o)
+{
+    bool hit = false;
+#pragma unroll
+    for (int m = 0; m < LaneSpheres<LPS>::N; ++m) {
+        float dx = o.x - S.x[m], dy = o.y - S.y[m], dz = o.z - S.z[m];
+        float d2 = (dx * dx + dy * dy) + dz * dz;
+        hit |= d2 < o.w;
+    }
+    return hit;
+}
+
+template <int LPS>
+__device__ __forceinline__ bool hits_box(const LaneSpheres<LPS> &S, float4 lo, float4 hi)
+{
+    bool hit = false;
+#pragma unroll
+    for (int m = 0; m < LaneSpheres<LPS>::N; ++m) {
+        float px = fminf(fmaxf(S.x[m], lo.x), hi.x);
+        float py = fminf(fmaxf(S.y[m], lo.y), hi.y);
+        float pz = fminf(fmaxf(S.z[m], lo.z), hi.z);
+        float dx = S.x[m] - px, dy = S.y[m] - py, dz = S.z[m] - pz;
+        float d2 = (dx * dx + dy * dy) + dz * dz;
+        hit |= d2 < kRobotR * kRobotR;
+    }
+    return hit;
+}
+
+// Wave-level helpers (wave64).
+__device__ __forceinline__ uint64_t wave_ballot(bool p)
+{
+    return __ballot(p);
+}
+
+// Bits set at the first lane of every LPS-lane group.
+template <int LPS>
+__host__ __device__ constexpr uint64_t leader_mask()
+{
+    uint64_t m = 0;
+    for (int i = 0; i < 64; i += LPS) m |= (uint64_t)1 << i;
+    return m;
+}
+
+// Folds a per-lane mask onto group leaders: bit (g*LPS) set iff any lane of
+// group g is set.
+template <int LPS>
+__device__ __forceinline__ uint64_t fold_groups(uint64_t m)
+{
+#pragma unroll
+    for (int sh = 1; sh < LPS; sh <<= 1) m |= m >> sh;
+    return m & leader_mask<LPS>();
+}
+
+// Exit policies of the obstacle loop.
+//   kAllGroups: independent states per group; stop when every group has hit
+//               (its state is invalid) -- batched isValid.
+//   kAnyGroup:  all groups belong to one edge; stop at the first hit.
+//   kLowestGroup: all groups belong to one edge; stop once the lowest group
+//               with a hit has no unresolved group below it (exact position
+//               of the first invalid state in check order).
+enum class Exit { kAllGroups, kAnyGroup, kLowestGroup };
+
+// Checks the lane's spheres against the LDS environment.  `live` lanes take
+// part; dead lanes count as resolved.  Returns this lane's hit flag (the
+// group result is the OR over its lanes).
+template <int LPS, Exit E>
+__device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live)
+{
+    bool hit = false;
+    const uint64_t live_leaders = fold_groups<LPS>(wave_ballot(live));
+    auto done = [&](bool h) -> bool {
+        const uint64_t H = fold_groups<LPS>(wave_ballot(h && live));
+        if (E == Exit::kAllGroups) return (H & live_leaders) == live_leaders;
+        if (E == Exit::kAnyGroup) return H != 0;
+        if (H == 0) return false;
+        const uint64_t low = H & (~H + 1);
+        return ((low - 1) & live_leaders) == 0;
+    };
+    if (live_leaders == 0) return hit;
+    for (int o = 0; o < ns; ++o) {
+        hit |= hits_sphere<LPS>(S, lds[o]);
+        if (done(hit)) return hit;
+    }
+    const float4 *boxes = lds + ns;
+    for (int o = 0; o < nb; ++o) {
+        hit |= hits_box<LPS>(S, boxes[2 * o], boxes[2 * o + 1]);
+        if (done(hit)) return hit;
+    }
+    return hit;
+}
+
+} // namespace ompl_amd

@@ -1,0 +1,240 @@
+// C-ABI of environments, state validity and motion validation.
+#include <cmath>
+#include <vector>
+
+#include "runtime.hpp"
+
+using namespace ompl_amd;
+
+namespace {
+
+Env &env_of(ompl_amd_env e)
+{
+    OA_REQUIRE(e, OMPL_AMD_ERR_ARG, "null environment");
+    return e->env;
+}
+
+// Batched DiscreteMotionValidator::checkMotion(s1, s2) over device arrays.
+void check_motions(const Env &env, const double *a, const double *b, int64_t n, double lvs, unsigned factor,
+                   uint8_t *out, int32_t *checked, hipStream_t s)
+{
+    OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative edge count");
+    if (n == 0) return;
+    OA_REQUIRE(a && b && out, OMPL_AMD_ERR_ARG, "null edge or output pointer");
+    OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "longest valid segment must be > 0 and factor >= 1");
+    OA_REQUIRE(n < (int64_t)1 << 31, OMPL_AMD_ERR_ARG, "at most 2^31-1 edges per call");
+    const Tuning t = get_tuning();
+    MotionArgs A{};
+    A.env = env.view();
+    A.a = a;
+    A.b = b;
+    A.n = n;
+    A.lvs = lvs;
+    A.factor = factor;
+    A.out = out;
+    A.checked = checked;
+    const bool exact = checked != nullptr;
+    if (t.motion_two_phase) {
+        Workspace &w = workspace(s, (size_t)n);
+        A.list = w.list;
+        A.list_count = w.counters;
+        A.queue = w.counters + 1;
+        A.start = 1;
+        launch_zero(w.counters, 2, s);
+        launch_motion_endpoint(A, t.validity_lanes_per_state, t.validity_block, s);
+        const int64_t waves = (int64_t)num_cus() * t.motion_waves_per_cu;
+        launch_motion_walk(A, exact, t.motion_lanes_per_state, t.motion_block, waves, s);
+    } else {
+        A.start = 0;
+        launch_motion_walk(A, exact, t.motion_lanes_per_state, t.motion_block, n, s);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int ompl_amd_env_create(const float *spheres, int n_spheres, const float *boxes, int n_boxes, ompl_amd_env *out)
+{
+    return guarded([&] {
+        OA_REQUIRE(out, OMPL_AMD_ERR_ARG, "null output handle");
+        OA_REQUIRE(n_spheres >= 0 && n_boxes >= 0, OMPL_AMD_ERR_ARG, "negative obstacle count");
+        OA_REQUIRE(n_spheres == 0 || spheres, OMPL_AMD_ERR_ARG, "null sphere array");
+        OA_REQUIRE(n_boxes == 0 || boxes, OMPL_AMD_ERR_ARG, "null box array");
+        OA_REQUIRE((int64_t)n_spheres + 2 * (int64_t)n_boxes <= kMaxEnvWords, OMPL_AMD_ERR_ARG,
+                   "environment too large: spheres + 2*boxes must be <= %d", kMaxEnvWords);
+        ensure_init();
+        std::vector<float4> h((size_t)n_spheres + 2 * (size_t)n_boxes + 1);
+        for (int i = 0; i < n_spheres; ++i) {
+            const float *o = spheres + 4 * i;
+            OA_REQUIRE(o[3] >= 0.0f, OMPL_AMD_ERR_ARG, "negative sphere radius");
+            const float rs = o[3] + kRobotR;
+            h[i] = make_float4(o[0], o[1], o[2], rs * rs);
+        }
+        for (int i = 0; i < n_boxes; ++i) {
+            const float *o = boxes + 6 * i;
+            OA_REQUIRE(o[3] >= 0.0f && o[4] >= 0.0f && o[5] >= 0.0f, OMPL_AMD_ERR_ARG, "negative box extent");
+            h[n_spheres + 2 * i] = make_float4(o[0] - o[3], o[1] - o[4], o[2] - o[5], 0.0f);
+            h[n_spheres + 2 * i + 1] = make_float4(o[0] + o[3], o[1] + o[4], o[2] + o[5], 0.0f);
+        }
+        auto *e = new ompl_amd_env_s();
+        e->env.ns = n_spheres;
+        e->env.nb = n_boxes;
+        e->env.bytes = h.size() * sizeof(float4);
+        hipError_t err = hipMalloc(&e->env.obs, e->env.bytes);
+        if (err != hipSuccess) {
+            delete e;
+            fail(OMPL_AMD_ERR_NOMEM, "environment allocation failed: %s", hipGetErrorString(err));
+        }
+        OA_HIP(hipMemcpy(e->env.obs, h.data(), e->env.bytes, hipMemcpyHostToDevice));
+        *out = e;
+    });
+}
+
+int ompl_amd_env_destroy(ompl_amd_env e)
+{
+    return guarded([&] {
+        if (!e) return;
+        if (e->env.obs) OA_HIP(hipFree(e->env.obs));
+        delete e;
+    });
+}
+
+int ompl_amd_env_bytes(ompl_amd_env e, size_t *bytes)
+{
+    return guarded([&] {
+        OA_REQUIRE(bytes, OMPL_AMD_ERR_ARG, "null output");
+        *bytes = env_of(e).bytes;
+    });
+}
+
+int ompl_amd_validate_states_dev(ompl_amd_env e, const float *q, int64_t n, uint8_t *out, void *stream)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative state count");
+        if (n == 0) return;
+        OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
+        const Tuning t = get_tuning();
+        launch_validate<float>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_block,
+                               resolve_stream(stream));
+    });
+}
+
+int ompl_amd_validate_states_f64_dev(ompl_amd_env e, const double *q, int64_t n, uint8_t *out, void *stream)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative state count");
+        if (n == 0) return;
+        OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
+        const Tuning t = get_tuning();
+        launch_validate<double>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_block,
+                                resolve_stream(stream));
+    });
+}
+
+int ompl_amd_is_valid(ompl_amd_env e, const double *q, int *valid)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(q && valid, OMPL_AMD_ERR_ARG, "null state or output pointer");
+        ThreadResources &r = thread_resources();
+        State1 st;
+        for (int j = 0; j < kDof; ++j) st.q[j] = q[j];
+        launch_validate_single(env.view(), st, r.pinned_dev, r.stream);
+        OA_HIP(hipStreamSynchronize(r.stream));
+        *valid = r.pinned[0];
+    });
+}
+
+int ompl_amd_arm_spheres_dev(const float *q, int64_t n, float *out, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative state count");
+        if (n == 0) return;
+        OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null pointer");
+        launch_arm_spheres(q, n, out, resolve_stream(stream));
+    });
+}
+
+int ompl_amd_check_motions_dev(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
+                               unsigned factor, uint8_t *out, int32_t *checked, void *stream)
+{
+    return guarded([&] { check_motions(env_of(e), a, b, n, lvs, factor, out, checked, resolve_stream(stream)); });
+}
+
+int ompl_amd_check_motions_last_valid_dev(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
+                                          unsigned factor, uint8_t *out, double *last_t, void *stream)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative edge count");
+        if (n == 0) return;
+        OA_REQUIRE(a && b && out && last_t, OMPL_AMD_ERR_ARG, "null pointer");
+        OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "longest valid segment must be > 0 and factor >= 1");
+        const Tuning t = get_tuning();
+        MotionArgs A{};
+        A.env = env.view();
+        A.a = a;
+        A.b = b;
+        A.n = n;
+        A.lvs = lvs;
+        A.factor = factor;
+        A.out = out;
+        launch_motion_last_valid(A, last_t, t.motion_lanes_per_state, t.motion_block, n, resolve_stream(stream));
+    });
+}
+
+int ompl_amd_check_motion(ompl_amd_env e, const double *a, const double *b, double lvs, unsigned factor, int *valid)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(a && b && valid, OMPL_AMD_ERR_ARG, "null pointer");
+        OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "longest valid segment must be > 0 and factor >= 1");
+        ThreadResources &r = thread_resources();
+        Edge1 E;
+        for (int j = 0; j < kDof; ++j) {
+            E.a[j] = a[j];
+            E.b[j] = b[j];
+        }
+        launch_motion_single(env.view(), E, lvs, factor, r.pinned_dev, r.stream);
+        OA_HIP(hipStreamSynchronize(r.stream));
+        *valid = r.pinned[0];
+    });
+}
+
+int ompl_amd_check_motion_last_valid(ompl_amd_env e, const double *a, const double *b, double lvs, unsigned factor,
+                                     int *valid, double *last_t)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(a && b && valid && last_t, OMPL_AMD_ERR_ARG, "null pointer");
+        OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "longest valid segment must be > 0 and factor >= 1");
+        ThreadResources &r = thread_resources();
+        // endpoints staged through the pinned slot: [a(7), b(7), last_t] doubles
+        double *pd = reinterpret_cast<double *>(r.pinned + 16);
+        for (int j = 0; j < kDof; ++j) {
+            pd[j] = a[j];
+            pd[kDof + j] = b[j];
+        }
+        pd[2 * kDof] = *last_t;
+        // the kernel reads dof-major [7][1] arrays: identical to a flat vector
+        double *dev_pd = reinterpret_cast<double *>(r.pinned_dev + 16);
+        uint8_t *dev_out = reinterpret_cast<uint8_t *>(r.pinned_dev);
+        MotionArgs A{};
+        A.env = env.view();
+        A.a = dev_pd;
+        A.b = dev_pd + kDof;
+        A.n = 1;
+        A.lvs = lvs;
+        A.factor = factor;
+        A.out = dev_out;
+        launch_motion_last_valid(A, dev_pd + 2 * kDof, 1, 64, 1, r.stream);
+        OA_HIP(hipStreamSynchronize(r.stream));
+        *valid = reinterpret_cast<uint8_t *>(r.pinned)[0];
+        *last_t = pd[2 * kDof];
+    });
+}
+
+} // extern "C"

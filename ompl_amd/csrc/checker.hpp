@@ -1,0 +1,49 @@
+// Host-visible argument blocks and launchers of the checker kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "arm_device.hpp"
+#include "common.hpp"
+
+namespace ompl_amd {
+
+struct MotionArgs {
+    EnvView env;
+    const double *a; // FP64 [7][n] dof-major
+    const double *b;
+    int64_t n;
+    double lvs;
+    unsigned factor;
+    uint8_t *out;
+    int32_t *checked;        // optional
+    int32_t *list;           // survivor list (endpoint output / walk input), optional
+    int32_t *list_count;     // number of entries in list
+    int32_t *queue;          // dynamic work counter for the walk, optional
+    int start;               // first walk position (0 = s2, 1 = skip s2)
+};
+
+struct Edge1 {
+    double a[kDof];
+    double b[kDof];
+};
+
+struct State1 {
+    double q[kDof];
+};
+
+template <typename T>
+void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, int lps, int block, hipStream_t s);
+void launch_zero(int *p, int n, hipStream_t s);
+void launch_motion_endpoint(const MotionArgs &A, int lps, int block, hipStream_t s);
+void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int block, int64_t waves, hipStream_t s);
+void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result,
+                          hipStream_t s);
+void launch_validate_single(const EnvView &env, const State1 &st, int *result, hipStream_t s);
+void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int block, int64_t waves,
+                              hipStream_t s);
+void launch_arm_spheres(const float *q, int64_t n, float *out, hipStream_t s);
+
+} // namespace ompl_amd

@@ -1,0 +1,827 @@
+// Exact k-nearest and radius neighbour search for ompl::NearestNeighbors<_T>
+// (datastructures/NearestNeighbors.h:46-115), brute force over a device-resident
+// point set, with the semantics of NearestNeighborsLinear
+// (datastructures/NearestNeighborsLinear.h:98-142): results sorted by distance,
+// radius inclusive (d <= r), ties broken by lower index (a project rule: the
+// reference leaves equal keys in unspecified order).
+//
+// Distances (FP32, the oracle's operation order, element minus query):
+//   L2:  sqrt(sum_i (e_i - q_i)^2) accumulated from 0 in index order
+//        (base/spaces/src/RealVectorStateSpace.cpp:230-242)
+//   SE3: L2 over (x,y,z) + acos(|q_e . q_q|) (0 when |dot| > 1), weights 1, 1
+//        (base/src/StateSpace.cpp:1074-1082, base/spaces/src/SO3StateSpace.cpp:253-262)
+//
+// Scheme (one wave = 32 queries, one DB tile = 32 points):
+//   1. MFMA v_mfma_f32_32x32x2f32 on augmented vectors gives an approximate
+//      squared distance for all 32x32 pairs:  x_aug = (x, 1, |x|^2),
+//      q_aug = (-2q, |q|^2, 1)  ->  x_aug . q_aug = |q|^2 + |x|^2 - 2 q.x
+//      (SE3: the translation part only, a lower bound of the full distance).
+//   2. Each lane owns one query (column) and 16 of the tile's points (rows);
+//      a value passes if approx <= thr, thr = kth^2 (1 + 8e-6) + M with the
+//      rigorous FP32 bound M = 64 u (|q|^2 + max|x|^2) (u = 2^-24) on the
+//      cancellation error, so no point of the exact result is ever dropped.
+//   3. Passing points get the exact FP32 distance (the oracle's formula) and
+//      enter the lane's sorted register top-k; the two lanes of a query merge
+//      their lists at the end (wave-level merge), DB splits merge in a second
+//      kernel.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "fp32_math.hpp"
+#include "runtime.hpp"
+
+namespace ompl_amd {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr float kPadNorm = 1e30f;
+constexpr float kUnit = 5.9604644775390625e-8f; // 2^-24
+
+struct NNIndex {
+    int metric = 0;
+    int dim = 0;     // coordinates per point
+    int tdim = 0;    // coordinates in the MFMA part (dim for L2, 3 for SE3)
+    int ksteps = 0;  // MFMA K steps of 2 (augmented length tdim + 2, padded)
+    int64_t n = 0, cap = 0;
+    float *pts = nullptr;      // [cap][dim]
+    float *xt = nullptr;       // [cap/32][ksteps][64] tiled augmented points
+    uint8_t *removed = nullptr;
+    uint32_t *xn_max = nullptr; // max |x|^2 as FP32 bits (non-negative)
+    int64_t n_removed = 0;
+    std::vector<uint8_t> removed_host;
+    // per-call scratch (grow-only)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+} // namespace ompl_amd
+
+struct ompl_amd_nn_s {
+    ompl_amd::NNIndex idx;
+};
+
+namespace ompl_amd {
+
+__device__ __forceinline__ float l2_exact(const float *__restrict__ e, const float *q, int dim)
+{
+    float dist = 0.0f;
+    for (int i = 0; i < dim; ++i) {
+        const float diff = e[i] - q[i];
+        dist += diff * diff;
+    }
+    return __builtin_sqrtf(dist);
+}
+
+template <int METRIC, int DIM>
+__device__ __forceinline__ float exact_dist(const float *__restrict__ e, const float q[DIM])
+{
+    if (METRIC == OMPL_AMD_METRIC_L2) {
+        float dist = 0.0f;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            const float diff = e[i] - q[i];
+            dist += diff * diff;
+        }
+        return __builtin_sqrtf(dist);
+    } else {
+        float t = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float diff = e[i] - q[i];
+            t += diff * diff;
+        }
+        const float dt = __builtin_sqrtf(t);
+        const float dq = fabsf(((e[3] * q[3] + e[4] * q[4]) + e[5] * q[5]) + e[6] * q[6]);
+        const float dr = dq > 1.0f ? 0.0f : acos_as(dq);
+        float dist = 0.0f;
+        dist += 1.0f * dt;
+        dist += 1.0f * dr;
+        return dist;
+    }
+}
+
+struct Key {
+    float d;
+    int i;
+};
+
+__device__ __forceinline__ bool key_less(float da, int ia, float db, int ib)
+{
+    return da < db || (da == db && ia < ib);
+}
+
+template <int KMAX>
+struct TopK {
+    float d[KMAX];
+    int i[KMAX];
+    __device__ __forceinline__ void init()
+    {
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) {
+            d[s] = INFINITY;
+            i[s] = INT_MAX;
+        }
+    }
+    __device__ __forceinline__ void insert(float kd, int ki)
+    {
+#pragma unroll
+        for (int s = KMAX - 1; s >= 0; --s) {
+            const bool lt = key_less(kd, ki, d[s], i[s]);
+            const bool lt_prev = s > 0 ? key_less(kd, ki, d[s > 0 ? s - 1 : 0], i[s > 0 ? s - 1 : 0]) : false;
+            if (lt) {
+                d[s] = lt_prev ? d[s > 0 ? s - 1 : 0] : kd;
+                i[s] = lt_prev ? i[s > 0 ? s - 1 : 0] : ki;
+            }
+        }
+    }
+    __device__ __forceinline__ void kth(int k, float &kd, int &ki) const
+    {
+        kd = d[0];
+        ki = i[0];
+#pragma unroll
+        for (int s = 1; s < KMAX; ++s)
+            if (s == k - 1) {
+                kd = d[s];
+                ki = i[s];
+            }
+    }
+};
+
+struct KnnArgs {
+    const float *pts;
+    const float *xt;
+    const uint8_t *removed;
+    const uint32_t *xn_max;
+    int64_t n;
+    int64_t n_tiles;
+    const float *q; // [nq][dim]
+    int64_t nq;
+    int k;
+    int splits;
+    int64_t tiles_per_split;
+    float *out_d;    // final [nq][k] or partial [split][nq][k]
+    int32_t *out_i;
+    float radius;    // radius search
+    int64_t *counts; // radius pass 1: per-query counts; pass 2: per-query cursors
+    const int64_t *offsets;
+};
+
+// Row of the 32x32 MFMA output held in element j of a lane in half h.
+__device__ __forceinline__ int mfma_row(int j, int h)
+{
+    return ((j >> 2) << 3) + (h << 2) + (j & 3);
+}
+
+// Shared prologue: the lane's query and its MFMA operand registers.
+template <int METRIC, int DIM, int KSTEPS>
+struct QueryLane {
+    float q[DIM];
+    float b[KSTEPS];
+    float margin; // M
+    bool live;
+    int64_t qi;
+    __device__ __forceinline__ void load(const KnnArgs &A, int64_t qtile, int lane)
+    {
+        const int col = lane & 31, h = lane >> 5;
+        qi = qtile * 32 + col;
+        live = qi < A.nq;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) q[i] = live ? A.q[qi * DIM + i] : 0.0f;
+        constexpr int TD = METRIC == OMPL_AMD_METRIC_L2 ? DIM : 3;
+        float qn = 0.0f;
+#pragma unroll
+        for (int i = 0; i < TD; ++i) qn += q[i] * q[i];
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            const int kk = 2 * s + h;
+            float v = 0.0f;
+            if (kk < TD) v = -2.0f * q[kk];
+            else if (kk == TD) v = qn;
+            else if (kk == TD + 1) v = 1.0f;
+            b[s] = v;
+        }
+        const float xm = __uint_as_float(*A.xn_max);
+        margin = 64.0f * kUnit * (qn + xm);
+    }
+};
+
+template <int KSTEPS>
+__device__ __forceinline__ f32x16 tile_mma(const float *__restrict__ xt, int64_t t, int lane, const float *b)
+{
+    f32x16 acc = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    const float *base = xt + t * (KSTEPS * 64) + lane;
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(base[s * 64], b[s], acc, 0, 0, 0);
+    return acc;
+}
+
+template <int METRIC, int DIM, int KSTEPS, int KMAX>
+__global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
+{
+    const int lane = threadIdx.x & 63;
+    const int h = lane >> 5;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t qtiles = (A.nq + 31) / 32;
+    const int64_t qtile = wave % qtiles;
+    const int split = (int)(wave / qtiles);
+    if (split >= A.splits) return;
+    QueryLane<METRIC, DIM, KSTEPS> Q;
+    Q.load(A, qtile, lane);
+    TopK<KMAX> top;
+    top.init();
+    float thr = INFINITY;
+    const int64_t t0 = (int64_t)split * A.tiles_per_split;
+    const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
+    for (int64_t t = t0; t < t1; ++t) {
+        const f32x16 acc = tile_mma<KSTEPS>(A.xt, t, lane, Q.b);
+        uint32_t mask = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) mask |= (acc[j] <= thr ? 1u : 0u) << j;
+        if (!Q.live) mask = 0;
+        while (mask) {
+            const int j = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const int64_t p = t * 32 + mfma_row(j, h);
+            if (p >= A.n || A.removed[p]) continue;
+            const float d = exact_dist<METRIC, DIM>(A.pts + p * DIM, Q.q);
+            float kd;
+            int ki;
+            top.kth(A.k, kd, ki);
+            if (key_less(d, (int)p, kd, ki)) {
+                top.insert(d, (int)p);
+                top.kth(A.k, kd, ki);
+                if (kd < INFINITY) thr = kd * kd * (1.0f + 8e-6f) + Q.margin;
+            }
+        }
+    }
+    // wave-level merge: lane l (< 32) absorbs the list of lane l + 32
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+        const float od = __shfl_xor(top.d[s], 32);
+        const int oi = __shfl_xor(top.i[s], 32);
+        if (h == 0 && s < A.k) top.insert(od, oi);
+    }
+    if (h != 0 || !Q.live) return;
+    const int64_t base = ((int64_t)split * A.nq + Q.qi) * A.k;
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+        if (s < A.k) {
+            A.out_d[base + s] = top.d[s];
+            A.out_i[base + s] = top.i[s] == INT_MAX ? -1 : top.i[s];
+        }
+    }
+}
+
+// Merges the per-split partial lists of each query (one lane per query).
+template <int KMAX>
+__global__ void knn_merge_kernel(const float *__restrict__ pd, const int32_t *__restrict__ pi, int64_t nq, int k,
+                                 int splits, float *__restrict__ out_d, int32_t *__restrict__ out_i)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    TopK<KMAX> top;
+    top.init();
+    for (int s = 0; s < splits; ++s) {
+        const int64_t base = ((int64_t)s * nq + q) * k;
+        for (int j = 0; j < k; ++j) {
+            const int id = pi[base + j];
+            if (id < 0) break;
+            top.insert(pd[base + j], id);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+        if (s < k) {
+            out_d[q * k + s] = top.d[s];
+            out_i[q * k + s] = top.i[s] == INT_MAX ? -1 : top.i[s];
+        }
+    }
+}
+
+// Radius search: pass 0 counts, pass 1 writes (idx, dist) into CSR slots.
+template <int METRIC, int DIM, int KSTEPS, int PASS>
+__global__ void __launch_bounds__(256) radius_kernel(KnnArgs A, uint64_t *__restrict__ keys)
+{
+    const int lane = threadIdx.x & 63;
+    const int h = lane >> 5;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t qtiles = (A.nq + 31) / 32;
+    const int64_t qtile = wave % qtiles;
+    const int split = (int)(wave / qtiles);
+    if (split >= A.splits) return;
+    QueryLane<METRIC, DIM, KSTEPS> Q;
+    Q.load(A, qtile, lane);
+    const float r = A.radius;
+    const float thr = r * r * (1.0f + 8e-6f) + Q.margin;
+    const int64_t t0 = (int64_t)split * A.tiles_per_split;
+    const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
+    int64_t found = 0;
+    for (int64_t t = t0; t < t1; ++t) {
+        const f32x16 acc = tile_mma<KSTEPS>(A.xt, t, lane, Q.b);
+        uint32_t mask = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) mask |= (acc[j] <= thr ? 1u : 0u) << j;
+        if (!Q.live) mask = 0;
+        while (mask) {
+            const int j = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const int64_t p = t * 32 + mfma_row(j, h);
+            if (p >= A.n || A.removed[p]) continue;
+            const float d = exact_dist<METRIC, DIM>(A.pts + p * DIM, Q.q);
+            if (d <= r) {
+                if (PASS == 0) {
+                    ++found;
+                } else {
+                    const int64_t slot = A.offsets[Q.qi] + atomicAdd((unsigned long long *)&A.counts[Q.qi], 1ull);
+                    keys[slot] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)p;
+                }
+            }
+        }
+    }
+    if (PASS == 0 && Q.live && found) atomicAdd((unsigned long long *)&A.counts[Q.qi], (unsigned long long)found);
+}
+
+__global__ void radius_unpack_kernel(const uint64_t *__restrict__ keys, int64_t total, int32_t *__restrict__ idx,
+                                     float *__restrict__ dist)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t k = keys[i];
+    if (idx) idx[i] = (int32_t)(uint32_t)(k & 0xffffffffu);
+    if (dist) dist[i] = __uint_as_float((uint32_t)(k >> 32));
+}
+
+// Writes tiled augmented rows for points [from, to) and the padding of the
+// last tile; tracks max |x|^2.
+__global__ void build_xt_kernel(const float *__restrict__ pts, int dim, int tdim, int ksteps, int64_t from,
+                                int64_t to, int64_t n_total, float *__restrict__ xt, uint32_t *__restrict__ xn_max)
+{
+    const int64_t p = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= to) return;
+    const bool real = p < n_total;
+    float xn = 0.0f;
+    if (real)
+        for (int i = 0; i < tdim; ++i) xn += pts[p * dim + i] * pts[p * dim + i];
+    const int64_t t = p >> 5;
+    const int row = (int)(p & 31);
+    for (int kk = 0; kk < 2 * ksteps; ++kk) {
+        float v;
+        if (kk < tdim) v = real ? pts[p * dim + kk] : 0.0f;
+        else if (kk == tdim) v = 1.0f;
+        else if (kk == tdim + 1) v = real ? xn : kPadNorm;
+        else v = 0.0f;
+        xt[t * (ksteps * 64) + (kk >> 1) * 64 + (kk & 1) * 32 + row] = v;
+    }
+    if (real) atomicMax(xn_max, __float_as_uint(xn));
+}
+
+__global__ void fill_i64_kernel(int64_t *p, int64_t n, int64_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static void *scratch(NNIndex &x, size_t bytes)
+{
+    if (x.scratch_bytes < bytes) {
+        if (x.scratch) OA_HIP(hipFree(x.scratch));
+        x.scratch = nullptr;
+        hipError_t e = hipMalloc(&x.scratch, bytes);
+        if (e != hipSuccess) fail(OMPL_AMD_ERR_NOMEM, "nn scratch allocation of %zu bytes failed", bytes);
+        x.scratch_bytes = bytes;
+    }
+    return x.scratch;
+}
+
+static void reserve(NNIndex &x, int64_t need, hipStream_t s)
+{
+    if (need <= x.cap) return;
+    int64_t cap = x.cap ? x.cap : 1024;
+    while (cap < need) cap *= 2;
+    cap = (cap + 31) / 32 * 32;
+    float *pts = nullptr, *xt = nullptr;
+    uint8_t *rem = nullptr;
+    OA_HIP(hipMalloc(&pts, (size_t)cap * x.dim * sizeof(float)));
+    OA_HIP(hipMalloc(&xt, (size_t)cap * x.ksteps * 2 * sizeof(float)));
+    OA_HIP(hipMalloc(&rem, (size_t)cap));
+    OA_HIP(hipMemsetAsync(rem, 0, (size_t)cap, s));
+    if (x.n) {
+        OA_HIP(hipMemcpyAsync(pts, x.pts, (size_t)x.n * x.dim * sizeof(float), hipMemcpyDeviceToDevice, s));
+        OA_HIP(hipMemcpyAsync(xt, x.xt, (size_t)((x.n + 31) / 32) * 32 * x.ksteps * 2 * sizeof(float),
+                              hipMemcpyDeviceToDevice, s));
+        OA_HIP(hipMemcpyAsync(rem, x.removed, (size_t)x.n, hipMemcpyDeviceToDevice, s));
+    }
+    OA_HIP(hipStreamSynchronize(s));
+    if (x.pts) OA_HIP(hipFree(x.pts));
+    if (x.xt) OA_HIP(hipFree(x.xt));
+    if (x.removed) OA_HIP(hipFree(x.removed));
+    x.pts = pts;
+    x.xt = xt;
+    x.removed = rem;
+    x.cap = cap;
+    x.removed_host.resize((size_t)cap, 0);
+}
+
+static void append(NNIndex &x, const float *src, int64_t n, bool src_dev, hipStream_t s)
+{
+    if (n <= 0) return;
+    reserve(x, x.n + n, s);
+    OA_HIP(hipMemcpyAsync(x.pts + x.n * x.dim, src, (size_t)n * x.dim * sizeof(float),
+                          src_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    const int64_t from = x.n, to_real = x.n + n;
+    const int64_t to_pad = (to_real + 31) / 32 * 32;
+    x.n = to_real;
+    const int64_t cnt = to_pad - from;
+    hipLaunchKernelGGL(build_xt_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, x.pts, x.dim, x.tdim,
+                       x.ksteps, from, to_pad, x.n, x.xt, x.xn_max);
+    OA_HIP(hipGetLastError());
+    if (!src_dev) OA_HIP(hipStreamSynchronize(s)); // host source may be freed after return
+}
+
+template <int METRIC, int DIM, int KSTEPS>
+static void launch_knn_t(const KnnArgs &A, int kmax, int64_t waves, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    switch (kmax) {
+    case 16: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 16>), dim3(blocks), dim3(256), 0, s, A); break;
+    case 32: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 32>), dim3(blocks), dim3(256), 0, s, A); break;
+    default: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 64>), dim3(blocks), dim3(256), 0, s, A); break;
+    }
+}
+
+template <int METRIC, int DIM, int KSTEPS, int PASS>
+static void launch_radius_t(const KnnArgs &A, uint64_t *keys, int64_t waves, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    hipLaunchKernelGGL((radius_kernel<METRIC, DIM, KSTEPS, PASS>), dim3(blocks), dim3(256), 0, s, A, keys);
+}
+
+// Dispatch over the supported (metric, dim) shapes: L2 with dim 1..14, SE3.
+template <int PASS>
+static void dispatch_search(const NNIndex &x, const KnnArgs &A, int kmax, uint64_t *keys, int64_t waves,
+                            hipStream_t s, bool radius)
+{
+#define OA_CASE(D, KS)                                                                                              \
+    case D:                                                                                                         \
+        if (radius) launch_radius_t<OMPL_AMD_METRIC_L2, D, KS, PASS>(A, keys, waves, s);                          \
+        else launch_knn_t<OMPL_AMD_METRIC_L2, D, KS>(A, kmax, waves, s);                                          \
+        break;
+    if (x.metric == OMPL_AMD_METRIC_SE3) {
+        if (radius) launch_radius_t<OMPL_AMD_METRIC_SE3, 7, 3, PASS>(A, keys, waves, s);
+        else launch_knn_t<OMPL_AMD_METRIC_SE3, 7, 3>(A, kmax, waves, s);
+        return;
+    }
+    switch (x.dim) {
+        OA_CASE(1, 2)
+        OA_CASE(2, 2)
+        OA_CASE(3, 3)
+        OA_CASE(4, 3)
+        OA_CASE(5, 4)
+        OA_CASE(6, 4)
+        OA_CASE(7, 5)
+        OA_CASE(8, 5)
+        OA_CASE(9, 6)
+        OA_CASE(10, 6)
+        OA_CASE(11, 7)
+        OA_CASE(12, 7)
+        OA_CASE(13, 8)
+        OA_CASE(14, 8)
+    default: fail(OMPL_AMD_ERR_ARG, "L2 dimension %d not supported (1..14)", x.dim);
+    }
+#undef OA_CASE
+}
+
+static int64_t choose_splits(int64_t nq, int64_t n_tiles)
+{
+    const int64_t qtiles = (nq + 31) / 32;
+    const int64_t target = (int64_t)num_cus() * 16;
+    int64_t s = (target + qtiles - 1) / qtiles;
+    const int64_t max_s = n_tiles / 8 > 1 ? n_tiles / 8 : 1; // >= 8 tiles per split
+    if (s > max_s) s = max_s;
+    if (s < 1) s = 1;
+    if (s > 256) s = 256;
+    return s;
+}
+
+static KnnArgs base_args(const NNIndex &x, const float *q, int64_t nq)
+{
+    KnnArgs A{};
+    A.pts = x.pts;
+    A.xt = x.xt;
+    A.removed = x.removed;
+    A.xn_max = x.xn_max;
+    A.n = x.n;
+    A.n_tiles = (x.n + 31) / 32;
+    A.q = q;
+    A.nq = nq;
+    return A;
+}
+
+static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, float *out_d, hipStream_t s)
+{
+    OA_REQUIRE(k >= 1 && k <= 64, OMPL_AMD_ERR_ARG, "k must be in [1, 64] (got %d)", k);
+    OA_REQUIRE(nq >= 0, OMPL_AMD_ERR_ARG, "negative query count");
+    if (nq == 0) return;
+    OA_REQUIRE(q && out_i && out_d, OMPL_AMD_ERR_ARG, "null pointer");
+    OA_REQUIRE(x.n < INT32_MAX, OMPL_AMD_ERR_ARG, "too many points for int32 ids");
+    const int kmax = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
+    KnnArgs A = base_args(x, q, nq);
+    A.k = k;
+    if (x.n == 0) {
+        // empty structure: every slot is (-1, +inf)
+        A.n_tiles = 0;
+        A.splits = 1;
+        A.tiles_per_split = 0;
+        A.out_d = out_d;
+        A.out_i = out_i;
+        const int64_t qtiles = (nq + 31) / 32;
+        dispatch_search<0>(x, A, kmax, nullptr, qtiles, s, false);
+        OA_HIP(hipGetLastError());
+        return;
+    }
+    const int64_t splits = choose_splits(nq, A.n_tiles);
+    A.splits = (int)splits;
+    A.tiles_per_split = (A.n_tiles + splits - 1) / splits;
+    const int64_t qtiles = (nq + 31) / 32;
+    ProfScope ps("nn_knn", s);
+    if (splits == 1) {
+        A.out_d = out_d;
+        A.out_i = out_i;
+        dispatch_search<0>(x, A, kmax, nullptr, qtiles, s, false);
+    } else {
+        const size_t per = (size_t)splits * nq * k;
+        char *buf = (char *)scratch(x, per * (sizeof(float) + sizeof(int32_t)));
+        A.out_d = (float *)buf;
+        A.out_i = (int32_t *)(buf + per * sizeof(float));
+        dispatch_search<0>(x, A, kmax, nullptr, qtiles * splits, s, false);
+        const unsigned blocks = (unsigned)((nq + 255) / 256);
+        switch (kmax) {
+        case 16:
+            hipLaunchKernelGGL(knn_merge_kernel<16>, dim3(blocks), dim3(256), 0, s, A.out_d, A.out_i, nq, k,
+                               (int)splits, out_d, out_i);
+            break;
+        case 32:
+            hipLaunchKernelGGL(knn_merge_kernel<32>, dim3(blocks), dim3(256), 0, s, A.out_d, A.out_i, nq, k,
+                               (int)splits, out_d, out_i);
+            break;
+        default:
+            hipLaunchKernelGGL(knn_merge_kernel<64>, dim3(blocks), dim3(256), 0, s, A.out_d, A.out_i, nq, k,
+                               (int)splits, out_d, out_i);
+            break;
+        }
+    }
+    OA_HIP(hipGetLastError());
+}
+
+static void radius(NNIndex &x, const float *q, int64_t nq, float r, int64_t *offsets, int32_t *out_i, float *out_d,
+                   int64_t capacity, int64_t *total, hipStream_t s)
+{
+    OA_REQUIRE(nq >= 0 && offsets && total, OMPL_AMD_ERR_ARG, "bad radius arguments");
+    OA_REQUIRE(r >= 0.0f, OMPL_AMD_ERR_ARG, "negative radius");
+    *total = 0;
+    if (nq == 0) {
+        OA_HIP(hipMemsetAsync(offsets, 0, sizeof(int64_t), s));
+        return;
+    }
+    OA_REQUIRE(q, OMPL_AMD_ERR_ARG, "null queries");
+    KnnArgs A = base_args(x, q, nq);
+    A.radius = r;
+    const int64_t n_tiles = A.n_tiles;
+    const int64_t splits = n_tiles ? choose_splits(nq, n_tiles) : 1;
+    A.splits = (int)splits;
+    A.tiles_per_split = n_tiles ? (n_tiles + splits - 1) / splits : 0;
+    const int64_t qtiles = (nq + 31) / 32;
+    // scratch: counts [nq], cub temp, keys (pass 2)
+    size_t cub_bytes = 0;
+    OA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, (int64_t *)nullptr, (int64_t *)nullptr,
+                                            (int)(nq + 1), s));
+    const size_t counts_bytes = (size_t)(nq + 1) * sizeof(int64_t);
+    char *buf = (char *)scratch(x, counts_bytes + cub_bytes + 256);
+    int64_t *counts = (int64_t *)buf;
+    void *cub_tmp = buf + ((counts_bytes + 255) / 256) * 256;
+    ProfScope ps("nn_radius", s);
+    hipLaunchKernelGGL(fill_i64_kernel, dim3((unsigned)((nq + 1 + 255) / 256)), dim3(256), 0, s, counts, nq + 1,
+                       (int64_t)0);
+    A.counts = counts;
+    if (n_tiles) dispatch_search<0>(x, A, 16, nullptr, qtiles * splits, s, true);
+    OA_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp, cub_bytes, counts, offsets, (int)(nq + 1), s));
+    int64_t tot = 0;
+    OA_HIP(hipMemcpyAsync(&tot, offsets + nq, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipStreamSynchronize(s));
+    *total = tot;
+    if ((!out_i && !out_d) || tot == 0) return;
+    OA_REQUIRE(capacity >= tot, OMPL_AMD_ERR_ARG, "radius output capacity %lld < %lld results", (long long)capacity,
+               (long long)tot);
+    OA_REQUIRE(tot < INT32_MAX, OMPL_AMD_ERR_ARG, "too many radius results for one call");
+    // pass 2: keys (dist bits << 32 | idx), per-query segmented sort
+    size_t sort_bytes = 0;
+    OA_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                                      (int)tot, (int)nq, (int64_t *)nullptr, (int64_t *)nullptr, 0,
+                                                      64, s));
+    const size_t keys_bytes = (size_t)tot * sizeof(uint64_t);
+    const size_t need = counts_bytes + 256 + keys_bytes * 2 + sort_bytes + 512;
+    buf = (char *)scratch(x, need > counts_bytes + cub_bytes + 256 ? need : counts_bytes + cub_bytes + 256);
+    counts = (int64_t *)buf;
+    char *p = buf + ((counts_bytes + 255) / 256) * 256;
+    uint64_t *keys = (uint64_t *)p;
+    p += ((keys_bytes + 255) / 256) * 256;
+    uint64_t *keys_sorted = (uint64_t *)p;
+    p += ((keys_bytes + 255) / 256) * 256;
+    void *sort_tmp = p;
+    hipLaunchKernelGGL(fill_i64_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, counts, nq, (int64_t)0);
+    A.counts = counts;
+    A.offsets = offsets;
+    dispatch_search<1>(x, A, 16, keys, qtiles * splits, s, true);
+    OA_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(sort_tmp, sort_bytes, keys, keys_sorted, (int)tot, (int)nq,
+                                                      offsets, offsets + 1, 0, 64, s));
+    hipLaunchKernelGGL(radius_unpack_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, keys_sorted, tot,
+                       out_i, out_d);
+    OA_HIP(hipGetLastError());
+}
+
+static NNIndex &nn_of(ompl_amd_nn h)
+{
+    OA_REQUIRE(h, OMPL_AMD_ERR_ARG, "null nearest-neighbour handle");
+    return h->idx;
+}
+
+} // namespace ompl_amd
+
+using namespace ompl_amd;
+
+extern "C" {
+
+int ompl_amd_nn_create(int metric, int dim, ompl_amd_nn *out)
+{
+    return guarded([&] {
+        OA_REQUIRE(out, OMPL_AMD_ERR_ARG, "null output handle");
+        OA_REQUIRE(metric == OMPL_AMD_METRIC_L2 || metric == OMPL_AMD_METRIC_SE3, OMPL_AMD_ERR_ARG, "unknown metric");
+        if (metric == OMPL_AMD_METRIC_SE3)
+            OA_REQUIRE(dim == 7, OMPL_AMD_ERR_ARG, "SE3 points have 7 coordinates (x,y,z,qx,qy,qz,qw)");
+        else
+            OA_REQUIRE(dim >= 1 && dim <= 14, OMPL_AMD_ERR_ARG, "L2 dimension must be in [1, 14]");
+        ensure_init();
+        auto *h = new ompl_amd_nn_s();
+        NNIndex &x = h->idx;
+        x.metric = metric;
+        x.dim = dim;
+        x.tdim = metric == OMPL_AMD_METRIC_SE3 ? 3 : dim;
+        x.ksteps = (x.tdim + 2 + 1) / 2;
+        OA_HIP(hipMalloc(&x.xn_max, sizeof(uint32_t)));
+        OA_HIP(hipMemset(x.xn_max, 0, sizeof(uint32_t)));
+        *out = h;
+    });
+}
+
+int ompl_amd_nn_destroy(ompl_amd_nn h)
+{
+    return guarded([&] {
+        if (!h) return;
+        NNIndex &x = h->idx;
+        if (x.pts) OA_HIP(hipFree(x.pts));
+        if (x.xt) OA_HIP(hipFree(x.xt));
+        if (x.removed) OA_HIP(hipFree(x.removed));
+        if (x.xn_max) OA_HIP(hipFree(x.xn_max));
+        if (x.scratch) OA_HIP(hipFree(x.scratch));
+        delete h;
+    });
+}
+
+int ompl_amd_nn_clear(ompl_amd_nn h)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        hipStream_t s = default_stream();
+        if (x.n && x.removed) OA_HIP(hipMemsetAsync(x.removed, 0, (size_t)x.n, s));
+        OA_HIP(hipMemsetAsync(x.xn_max, 0, sizeof(uint32_t), s));
+        OA_HIP(hipStreamSynchronize(s));
+        std::fill(x.removed_host.begin(), x.removed_host.end(), 0);
+        x.n = 0;
+        x.n_removed = 0;
+    });
+}
+
+int ompl_amd_nn_add(ompl_amd_nn h, const float *pts, int64_t n, int64_t *first_id)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(n >= 0 && (n == 0 || pts), OMPL_AMD_ERR_ARG, "bad points");
+        if (first_id) *first_id = x.n;
+        append(x, pts, n, false, default_stream());
+    });
+}
+
+int ompl_amd_nn_add_dev(ompl_amd_nn h, const float *pts, int64_t n, int64_t *first_id, void *stream)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(n >= 0 && (n == 0 || pts), OMPL_AMD_ERR_ARG, "bad points");
+        if (first_id) *first_id = x.n;
+        append(x, pts, n, true, resolve_stream(stream));
+    });
+}
+
+int ompl_amd_nn_remove(ompl_amd_nn h, int64_t id, int *removed)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        int ok = 0;
+        if (id >= 0 && id < x.n && !x.removed_host[(size_t)id]) {
+            x.removed_host[(size_t)id] = 1;
+            const uint8_t one = 1;
+            hipStream_t s = default_stream();
+            OA_HIP(hipMemcpyAsync(x.removed + id, &one, 1, hipMemcpyHostToDevice, s));
+            OA_HIP(hipStreamSynchronize(s));
+            ++x.n_removed;
+            ok = 1;
+        }
+        if (removed) *removed = ok;
+    });
+}
+
+int ompl_amd_nn_size(ompl_amd_nn h, int64_t *size)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(size, OMPL_AMD_ERR_ARG, "null output");
+        *size = x.n - x.n_removed;
+    });
+}
+
+int ompl_amd_nn_build(ompl_amd_nn h, void *stream)
+{
+    return guarded([&] {
+        (void)nn_of(h);
+        (void)stream; // tiles are written by add(); nothing is deferred
+    });
+}
+
+int ompl_amd_nn_knn_dev(ompl_amd_nn h, const float *q, int64_t nq, int k, int32_t *out_i, float *out_d, void *stream)
+{
+    return guarded([&] { knn(nn_of(h), q, nq, k, out_i, out_d, resolve_stream(stream)); });
+}
+
+int ompl_amd_nn_radius_dev(ompl_amd_nn h, const float *q, int64_t nq, float r, int64_t *offsets, int32_t *out_i,
+                           float *out_d, int64_t capacity, int64_t *total, void *stream)
+{
+    return guarded(
+        [&] { radius(nn_of(h), q, nq, r, offsets, out_i, out_d, capacity, total, resolve_stream(stream)); });
+}
+
+int ompl_amd_nn_knn(ompl_amd_nn h, const float *q, int64_t nq, int k, int32_t *out_i, float *out_d)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(nq >= 0 && (nq == 0 || (q && out_i && out_d)), OMPL_AMD_ERR_ARG, "bad arguments");
+        if (nq == 0) return;
+        hipStream_t s = default_stream();
+        float *dq = nullptr, *dd = nullptr;
+        int32_t *di = nullptr;
+        OA_HIP(hipMallocAsync((void **)&dq, (size_t)nq * x.dim * sizeof(float), s));
+        OA_HIP(hipMallocAsync((void **)&dd, (size_t)nq * k * sizeof(float), s));
+        OA_HIP(hipMallocAsync((void **)&di, (size_t)nq * k * sizeof(int32_t), s));
+        OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
+        knn(x, dq, nq, k, di, dd, s);
+        OA_HIP(hipMemcpyAsync(out_d, dd, (size_t)nq * k * sizeof(float), hipMemcpyDeviceToHost, s));
+        OA_HIP(hipMemcpyAsync(out_i, di, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        OA_HIP(hipFreeAsync(dq, s));
+        OA_HIP(hipFreeAsync(dd, s));
+        OA_HIP(hipFreeAsync(di, s));
+        OA_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int ompl_amd_nn_nearest(ompl_amd_nn h, const float *q, int64_t *out_id)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(q && out_id, OMPL_AMD_ERR_ARG, "null pointer");
+        if (x.n - x.n_removed <= 0) fail(OMPL_AMD_ERR_EMPTY, "No elements found in nearest neighbors data structure");
+        int32_t id = -1;
+        float d = 0.0f;
+        int rc = ompl_amd_nn_knn(h, q, 1, 1, &id, &d);
+        if (rc) fail(rc, "%s", last_error());
+        *out_id = id;
+    });
+}
+
+double ompl_amd_rrtstar_radius(int dim, double measure, double rewire_factor, double max_distance, int64_t n)
+{
+    // unitNBallMeasure (util/src/GeometricEquations.cpp:55-60): pi^(d/2) / Gamma(d/2 + 1)
+    const double d = (double)dim;
+    const double ball = std::pow(std::sqrt(M_PI), d) / std::tgamma(d / 2.0 + 1.0);
+    const double r_rrt = rewire_factor * std::pow(2 * (1.0 + 1.0 / d) * (measure / ball), 1.0 / d);
+    const double cardDbl = (double)(n + 1);
+    const double r = r_rrt * std::pow(std::log(cardDbl) / cardDbl, 1.0 / d);
+    return r < max_distance ? r : max_distance;
+}
+
+} // extern "C"

@@ -1,0 +1,60 @@
+// Runtime state shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "checker.hpp"
+#include "common.hpp"
+
+namespace ompl_amd {
+
+void init_device(int dev);
+
+// Per-stream scratch of the batched motion path: survivor list + counters.
+struct Workspace {
+    int32_t *list = nullptr;
+    size_t list_cap = 0;
+    int32_t *counters = nullptr; // [0] survivor count, [1] walk queue
+};
+Workspace &workspace(hipStream_t s, size_t list_entries);
+
+// Per-thread stream and pinned result slot of the blocking single-call entry
+// points (OMPL checkers must be thread-safe: base/StateValidityChecker.h:87-89).
+struct ThreadResources {
+    hipStream_t stream = nullptr;
+    int32_t *pinned = nullptr;     // 4 KiB, host-coherent, device-mapped
+    int32_t *pinned_dev = nullptr; // device address of the same memory
+};
+ThreadResources &thread_resources();
+
+struct Tuning {
+    int validity_lanes_per_state;
+    int validity_block;
+    int motion_lanes_per_state;
+    int motion_block;
+    int motion_waves_per_cu;
+    int motion_two_phase;
+};
+Tuning get_tuning();
+void set_tuning(const Tuning &t);
+
+// Device-resident obstacle set.
+struct Env {
+    float4 *obs = nullptr; // ns spheres (x,y,z,(R+r)^2), then nb boxes as (lo),(hi)
+    int ns = 0;
+    int nb = 0;
+    size_t bytes = 0;
+    EnvView view() const { return EnvView{obs, ns, nb}; }
+};
+
+// Largest obstacle set staged in LDS by one block (float4 words).
+constexpr int kMaxEnvWords = 4096; // 64 KiB of dynamic LDS
+
+} // namespace ompl_amd
+
+struct ompl_amd_env_s {
+    ompl_amd::Env env;
+};

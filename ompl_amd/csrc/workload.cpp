@@ -1,0 +1,131 @@
+// Synthetic workload generators of the benchmark configurations, written on
+// the same engine and distributions as ompl::RNG (util/RandomNumbers.h:66-76,
+// 190-191): std::mt19937 seeded with the local seed,
+// uniformReal(lo, hi) = (hi - lo) * uniform_real_distribution<>(0,1) + lo,
+// gaussian(mean, sd) = normal_distribution<>(0,1) * sd + mean, and
+// RNG::quaternion (util/src/RandomNumbers.cpp:266-279) for SO3 samples.
+#include <algorithm>
+#include <cmath>
+#include <random>
+
+#include "runtime.hpp"
+
+namespace {
+
+class LocalRng {
+public:
+    explicit LocalRng(uint32_t seed) : gen_(seed) {}
+    double uniformReal(double lo, double hi) { return (hi - lo) * uni_(gen_) + lo; }
+    double uniform01() { return uni_(gen_); }
+    double gaussian(double mean, double sd) { return normal_(gen_) * sd + mean; }
+    void quaternion(double v[4])
+    {
+        const double pi = 3.141592653589793238462643383279502884;
+        double x0 = uni_(gen_);
+        double r1 = std::sqrt(1.0 - x0), r2 = std::sqrt(x0);
+        double t1 = 2.0 * pi * uni_(gen_), t2 = 2.0 * pi * uni_(gen_);
+        double c1 = std::cos(t1), s1 = std::sin(t1);
+        double c2 = std::cos(t2), s2 = std::sin(t2);
+        v[0] = s1 * r1;
+        v[1] = c1 * r1;
+        v[2] = s2 * r2;
+        v[3] = c2 * r2;
+    }
+
+private:
+    std::mt19937 gen_;
+    std::uniform_real_distribution<> uni_{0, 1};
+    std::normal_distribution<> normal_{0, 1};
+};
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+} // namespace
+
+using namespace ompl_amd;
+
+extern "C" {
+
+int ompl_amd_workload_obstacles(uint32_t seed, int n_spheres, int n_boxes, double r_lo, double r_hi, double h_lo,
+                                double h_hi, double min_base_dist, float *spheres_out, float *boxes_out)
+{
+    return guarded([&] {
+        OA_REQUIRE(n_spheres >= 0 && n_boxes >= 0, OMPL_AMD_ERR_ARG, "negative count");
+        OA_REQUIRE((n_spheres == 0 || spheres_out) && (n_boxes == 0 || boxes_out), OMPL_AMD_ERR_ARG,
+                   "null output");
+        OA_REQUIRE(min_base_dist < 1.5, OMPL_AMD_ERR_ARG, "min_base_dist leaves no room in [-1,1]^3");
+        LocalRng rng(seed);
+        auto centre = [&](double c[3]) {
+            do {
+                for (int k = 0; k < 3; ++k) c[k] = rng.uniformReal(-1.0, 1.0);
+            } while (std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) < min_base_dist);
+        };
+        for (int i = 0; i < n_spheres; ++i) {
+            double c[3];
+            centre(c);
+            double r = rng.uniformReal(r_lo, r_hi);
+            for (int k = 0; k < 3; ++k) spheres_out[4 * i + k] = (float)c[k];
+            spheres_out[4 * i + 3] = (float)r;
+        }
+        for (int i = 0; i < n_boxes; ++i) {
+            double c[3];
+            centre(c);
+            for (int k = 0; k < 3; ++k) boxes_out[6 * i + k] = (float)c[k];
+            for (int k = 0; k < 3; ++k) boxes_out[6 * i + 3 + k] = (float)rng.uniformReal(h_lo, h_hi);
+        }
+    });
+}
+
+int ompl_amd_workload_configs(uint32_t seed, int64_t n, double *out)
+{
+    return guarded([&] {
+        OA_REQUIRE(n >= 0 && (n == 0 || out), OMPL_AMD_ERR_ARG, "bad arguments");
+        LocalRng rng(seed);
+        for (int64_t i = 0; i < n * kDof; ++i) out[i] = rng.uniformReal(-kPi, kPi);
+    });
+}
+
+int ompl_amd_workload_edges(uint32_t seed, int64_t n, double mu, double sigma, double len_lo, double len_hi,
+                            double *a_out, double *b_out)
+{
+    return guarded([&] {
+        OA_REQUIRE(n >= 0 && (n == 0 || (a_out && b_out)), OMPL_AMD_ERR_ARG, "bad arguments");
+        LocalRng rng(seed);
+        for (int64_t i = 0; i < n; ++i) {
+            double *a = a_out + i * kDof, *b = b_out + i * kDof;
+            for (int j = 0; j < kDof; ++j) a[j] = rng.uniformReal(-kPi, kPi);
+            double dir[kDof], norm = 0.0;
+            do {
+                norm = 0.0;
+                for (int j = 0; j < kDof; ++j) {
+                    dir[j] = rng.gaussian(0.0, 1.0);
+                    norm += dir[j] * dir[j];
+                }
+            } while (norm == 0.0);
+            norm = std::sqrt(norm);
+            double len = std::exp(rng.gaussian(mu, sigma));
+            len = std::min(std::max(len, len_lo), len_hi);
+            for (int j = 0; j < kDof; ++j) {
+                double v = a[j] + len * (dir[j] / norm);
+                b[j] = std::min(std::max(v, -kPi), kPi);
+            }
+        }
+    });
+}
+
+int ompl_amd_workload_se3(uint32_t seed, int64_t n, double lo, double hi, float *out)
+{
+    return guarded([&] {
+        OA_REQUIRE(n >= 0 && (n == 0 || out), OMPL_AMD_ERR_ARG, "bad arguments");
+        LocalRng rng(seed);
+        for (int64_t i = 0; i < n; ++i) {
+            float *s = out + i * 7;
+            for (int k = 0; k < 3; ++k) s[k] = (float)rng.uniformReal(lo, hi);
+            double q[4];
+            rng.quaternion(q);
+            for (int k = 0; k < 4; ++k) s[3 + k] = (float)q[k];
+        }
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,113 @@
+/*
+ * oracle.h -- CPU restatement of the planning inner loop, used ONLY as the
+ * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * Nothing in the product (ompl_amd/) includes, links or calls this file.
+ *
+ * What it restates (reference citations are relative to the OMPL 2.0.2 tree):
+ *   - DiscreteMotionValidator::checkMotion, both overloads
+ *       base/src/DiscreteMotionValidator.cpp:48-91 and :93-145
+ *   - StateSpace::validSegmentCount            base/src/StateSpace.cpp:851-854
+ *   - StateSpace::setup (longest valid segment) base/src/StateSpace.cpp:237-241
+ *   - RealVectorStateSpace distance/interpolate/maxExtent
+ *       base/spaces/src/RealVectorStateSpace.cpp:167-176, 230-242, 257-265
+ *   - RealVectorStateSampler::sampleUniform    base/spaces/src/RealVectorStateSpace.cpp:45-52
+ *   - RNG::uniformReal / uniform01 over std::mt19937
+ *       util/RandomNumbers.h:66-76, 190-191 (libstdc++ generate_canonical<double,53>)
+ *   - RNG::quaternion (Shoemake)               util/src/RandomNumbers.cpp:266-279
+ *   - SO3 arcLength / SE3 compound distance    base/spaces/src/SO3StateSpace.cpp:253-262,
+ *       base/src/StateSpace.cpp:1074-1082, base/spaces/SE3StateSpace.h:119-120
+ *   - NearestNeighborsLinear nearest/nearestK/nearestR
+ *       datastructures/NearestNeighborsLinear.h:98-142
+ *   - the FP64->FP32 cast at the checker boundary  vamp/Utils.h:26
+ *   - the FP32 arm forward kinematics and sphere/sphere, sphere/box collision
+ *     checker.  The reference's own FP32 checker (VAMP) is an absent
+ *     submodule, so this part restates the project's arm specification (see the
+ *     comment block above orc_arm_spheres) and is pinned by closed forms.
+ *
+ * Everything is compiled with -ffp-contract=off so every FP32/FP64 expression
+ * is evaluated in exactly the order written.
+ */
+#ifndef OMPL_AMD_ORACLE_H
+#define OMPL_AMD_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- project-owned FP32 transcendentals (identical method on the GPU) ---- */
+void orc_sincosf(float x, float *s, float *c);
+float orc_acosf(float x); /* domain [0,1] */
+
+/* ---- arm specification ---- */
+#define ORC_ARM_DOF 7
+#define ORC_ARM_SPHERES 28
+void orc_arm_spheres(const float q[ORC_ARM_DOF], float centres[ORC_ARM_SPHERES][3]);
+
+/* Obstacles: spheres as (x,y,z,R), boxes as (cx,cy,cz,hx,hy,hz). */
+typedef struct {
+    int n_spheres;
+    const float *spheres; /* 4 floats each */
+    int n_boxes;
+    const float *boxes;   /* 6 floats each */
+} orc_env;
+
+/* 1 = collision free, 0 = in collision.  *pairs (optional) receives the number
+ * of (robot sphere, obstacle) pair tests performed before the answer was known,
+ * in the spec's loop order (spheres outer, obstacles inner; spheres before boxes). */
+int orc_state_valid(const orc_env *env, const float q[ORC_ARM_DOF], long *pairs);
+/* FP64 state -> FP32 cast per joint (vamp/Utils.h:26) then orc_state_valid. */
+int orc_state_valid_d(const orc_env *env, const double q[ORC_ARM_DOF], long *pairs);
+void orc_validate_states(const orc_env *env, const float *q_soa, long n, uint8_t *out);
+
+/* ---- DiscreteMotionValidator ---- */
+double orc_max_extent(const double *lo, const double *hi, int dim);
+double orc_longest_valid_segment(double max_extent, double fraction);
+double orc_distance_d(const double *a, const double *b, int dim);
+void orc_interpolate_d(const double *a, const double *b, double t, int dim, double *out);
+unsigned orc_valid_segment_count(const double *a, const double *b, int dim, double lvs, unsigned factor);
+/* BFS bisection order of DiscreteMotionValidator.cpp:106-134 for a given nd:
+ * writes the nd-1 interpolation indices (each in [1, nd-1]) in check order. */
+int orc_bfs_order(int nd, int *out);
+/* checkMotion(s1,s2): returns 1/0; *checked = states tested (s2 first, then
+ * BFS mids up to and including the first invalid one). */
+int orc_check_motion(const orc_env *env, const double a[ORC_ARM_DOF], const double b[ORC_ARM_DOF],
+                     double lvs, unsigned factor, int *checked);
+/* checkMotion(s1,s2,lastValid): sequential j=1..nd-1 then s2; on failure
+ * *last_t = (j-1)/nd (or (nd-1)/nd when s2 fails); *last_t untouched if valid. */
+int orc_check_motion_last_valid(const orc_env *env, const double a[ORC_ARM_DOF], const double b[ORC_ARM_DOF],
+                                double lvs, unsigned factor, double *last_t, int *checked);
+
+/* ---- std::mt19937 + libstdc++ generate_canonical<double,53> ---- */
+typedef struct {
+    uint32_t mt[624];
+    int idx;
+} orc_mt19937;
+void orc_mt_seed(orc_mt19937 *g, uint32_t seed);
+uint32_t orc_mt_next(orc_mt19937 *g);
+double orc_canonical(orc_mt19937 *g);
+double orc_uniform_real(orc_mt19937 *g, double lo, double hi);
+/* n states of dim doubles, row-major, exactly n*dim sequential uniformReal calls */
+void orc_sample_uniform(orc_mt19937 *g, const double *lo, const double *hi, int dim, long n, double *out);
+void orc_quaternion(orc_mt19937 *g, double q[4]);
+
+/* ---- nearest neighbours (FP32 metric, ties broken by lower index) ---- */
+float orc_l2_dist(const float *elem, const float *query, int dim);
+/* SE3 state: (x,y,z, qx,qy,qz,qw) */
+float orc_se3_dist(const float *elem, const float *query);
+/* metric: 0 = L2(dim), 1 = SE3 (dim must be 7). data/query are row-major. */
+void orc_knn(int metric, const float *data, long n, int dim, const float *query, int k, int *out_idx,
+             float *out_dist);
+/* Radius search (inclusive dist <= r).  Writes at most cap results sorted by
+ * (dist, index); returns the total count found. */
+long orc_radius(int metric, const float *data, long n, int dim, const float *query, float r, int *out_idx,
+                float *out_dist, long cap);
+/* First strict minimum, as NearestNeighborsLinear::nearest; -1 if empty. */
+long orc_nearest(int metric, const float *data, long n, int dim, const float *query);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,282 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+REF_LINEAR = HERE / "_ref" / "ref_linear"
+REFERENCE_SRC = Path("/root/reference/src")
+
+vp = C.c_void_p
+
+
+def build(force: bool = False) -> Path:
+    """Builds the oracle (and the reference Linear driver when the reference tree exists)."""
+    if force or not LIB.exists():
+        subprocess.run(["make", "-C", str(HERE), "all"], check=True, capture_output=True)
+    if REFERENCE_SRC.exists() and (force or not REF_LINEAR.exists()):
+        subprocess.run(["make", "-C", str(HERE), "ref"], check=True, capture_output=True)
+    return LIB
+
+
+class Env(C.Structure):
+    _fields_ = [("n_spheres", C.c_int), ("spheres", vp), ("n_boxes", C.c_int), ("boxes", vp)]
+
+
+class MT(C.Structure):
+    _fields_ = [("mt", C.c_uint32 * 624), ("idx", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        L.orc_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_acosf.argtypes = [C.c_float]
+        L.orc_acosf.restype = C.c_float
+        L.orc_arm_spheres.argtypes = [vp, vp]
+        L.orc_state_valid.argtypes = [C.POINTER(Env), vp, C.POINTER(C.c_long)]
+        L.orc_state_valid_d.argtypes = [C.POINTER(Env), vp, C.POINTER(C.c_long)]
+        L.orc_validate_states.argtypes = [C.POINTER(Env), vp, C.c_long, vp]
+        L.orc_max_extent.argtypes = [vp, vp, C.c_int]
+        L.orc_max_extent.restype = C.c_double
+        L.orc_longest_valid_segment.argtypes = [C.c_double, C.c_double]
+        L.orc_longest_valid_segment.restype = C.c_double
+        L.orc_distance_d.argtypes = [vp, vp, C.c_int]
+        L.orc_distance_d.restype = C.c_double
+        L.orc_valid_segment_count.argtypes = [vp, vp, C.c_int, C.c_double, C.c_uint]
+        L.orc_valid_segment_count.restype = C.c_uint
+        L.orc_bfs_order.argtypes = [C.c_int, vp]
+        L.orc_check_motion.argtypes = [C.POINTER(Env), vp, vp, C.c_double, C.c_uint, C.POINTER(C.c_int)]
+        L.orc_check_motion_last_valid.argtypes = [C.POINTER(Env), vp, vp, C.c_double, C.c_uint,
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.orc_mt_seed.argtypes = [C.POINTER(MT), C.c_uint32]
+        L.orc_mt_next.argtypes = [C.POINTER(MT)]
+        L.orc_mt_next.restype = C.c_uint32
+        L.orc_canonical.argtypes = [C.POINTER(MT)]
+        L.orc_canonical.restype = C.c_double
+        L.orc_uniform_real.argtypes = [C.POINTER(MT), C.c_double, C.c_double]
+        L.orc_uniform_real.restype = C.c_double
+        L.orc_sample_uniform.argtypes = [C.POINTER(MT), vp, vp, C.c_int, C.c_long, vp]
+        L.orc_quaternion.argtypes = [C.POINTER(MT), vp]
+        L.orc_l2_dist.argtypes = [vp, vp, C.c_int]
+        L.orc_l2_dist.restype = C.c_float
+        L.orc_se3_dist.argtypes = [vp, vp]
+        L.orc_se3_dist.restype = C.c_float
+        L.orc_knn.argtypes = [C.c_int, vp, C.c_long, C.c_int, vp, C.c_int, vp, vp]
+        L.orc_radius.argtypes = [C.c_int, vp, C.c_long, C.c_int, vp, C.c_float, vp, vp, C.c_long]
+        L.orc_radius.restype = C.c_long
+        L.orc_nearest.argtypes = [C.c_int, vp, C.c_long, C.c_int, vp]
+        L.orc_nearest.restype = C.c_long
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(vp)
+
+
+class OracleEnv:
+    def __init__(self, spheres, boxes):
+        self.s = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4)
+        self.b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        self.c = Env(self.s.shape[0], _p(self.s), self.b.shape[0], _p(self.b))
+
+    def ref(self):
+        return C.byref(self.c)
+
+
+def sincosf(x: float):
+    s, c = C.c_float(), C.c_float()
+    lib().orc_sincosf(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def acosf(x: float) -> float:
+    return lib().orc_acosf(x)
+
+
+def arm_spheres(q) -> np.ndarray:
+    qa = np.ascontiguousarray(q, np.float32)
+    out = np.zeros((28, 3), np.float32)
+    lib().orc_arm_spheres(_p(qa), _p(out))
+    return out
+
+
+def state_valid(env: OracleEnv, q) -> bool:
+    qa = np.ascontiguousarray(q, np.float32)
+    return bool(lib().orc_state_valid(env.ref(), _p(qa), None))
+
+
+def validate_states(env: OracleEnv, q_soa: np.ndarray) -> np.ndarray:
+    qa = np.ascontiguousarray(q_soa, np.float32)
+    n = qa.shape[1]
+    out = np.zeros(n, np.uint8)
+    lib().orc_validate_states(env.ref(), _p(qa), n, _p(out))
+    return out
+
+
+def max_extent(lo, hi) -> float:
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    return lib().orc_max_extent(_p(lo), _p(hi), lo.shape[0])
+
+
+def longest_valid_segment(max_ext: float, fraction: float) -> float:
+    return lib().orc_longest_valid_segment(max_ext, fraction)
+
+
+def valid_segment_count(a, b, lvs, factor=1) -> int:
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return lib().orc_valid_segment_count(_p(a), _p(b), a.shape[0], lvs, factor)
+
+
+def bfs_order(nd: int) -> np.ndarray:
+    out = np.zeros(max(nd, 1), np.int32)
+    n = lib().orc_bfs_order(nd, _p(out))
+    return out[:n].copy()
+
+
+def check_motion(env: OracleEnv, a, b, lvs, factor=1):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    cnt = C.c_int()
+    v = lib().orc_check_motion(env.ref(), _p(a), _p(b), lvs, factor, C.byref(cnt))
+    return bool(v), cnt.value
+
+
+def check_motion_last_valid(env: OracleEnv, a, b, lvs, factor=1, last_t=float("nan")):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    cnt = C.c_int()
+    t = C.c_double(last_t)
+    v = lib().orc_check_motion_last_valid(env.ref(), _p(a), _p(b), lvs, factor, C.byref(t), C.byref(cnt))
+    return bool(v), t.value, cnt.value
+
+
+def check_motions(env: OracleEnv, a_rows: np.ndarray, b_rows: np.ndarray, lvs, factor=1):
+    """Row-major [n][7] endpoints -> (valid u8[n], checked i32[n])."""
+    n = a_rows.shape[0]
+    out = np.zeros(n, np.uint8)
+    chk = np.zeros(n, np.int32)
+    for i in range(n):
+        v, c = check_motion(env, a_rows[i], b_rows[i], lvs, factor)
+        out[i] = v
+        chk[i] = c
+    return out, chk
+
+
+class Mt19937:
+    def __init__(self, seed: int):
+        self.g = MT()
+        lib().orc_mt_seed(C.byref(self.g), seed)
+
+    def next(self) -> int:
+        return lib().orc_mt_next(C.byref(self.g))
+
+    def canonical(self) -> float:
+        return lib().orc_canonical(C.byref(self.g))
+
+    def uniform_real(self, lo, hi) -> float:
+        return lib().orc_uniform_real(C.byref(self.g), lo, hi)
+
+    def sample_uniform(self, lo, hi, n) -> np.ndarray:
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        out = np.zeros((n, lo.shape[0]), np.float64)
+        lib().orc_sample_uniform(C.byref(self.g), _p(lo), _p(hi), lo.shape[0], n, _p(out))
+        return out
+
+    def quaternion(self) -> np.ndarray:
+        q = np.zeros(4, np.float64)
+        lib().orc_quaternion(C.byref(self.g), _p(q))
+        return q
+
+
+def l2_dist(e, q) -> float:
+    e = np.ascontiguousarray(e, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    return lib().orc_l2_dist(_p(e), _p(q), e.shape[0])
+
+
+def se3_dist(e, q) -> float:
+    e = np.ascontiguousarray(e, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    return lib().orc_se3_dist(_p(e), _p(q))
+
+
+def knn(metric: int, data: np.ndarray, queries: np.ndarray, k: int):
+    data = np.ascontiguousarray(data, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    n, dim = data.shape
+    nq = queries.shape[0]
+    idx = np.zeros((nq, k), np.int32)
+    dist = np.zeros((nq, k), np.float32)
+    for j in range(nq):
+        lib().orc_knn(metric, _p(data), n, dim, _p(queries[j]), k, _p(idx[j]), _p(dist[j]))
+    return idx, dist
+
+
+def radius(metric: int, data: np.ndarray, queries: np.ndarray, r: float):
+    data = np.ascontiguousarray(data, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    n, dim = data.shape
+    res = []
+    cap = 4096
+    for j in range(queries.shape[0]):
+        while True:
+            idx = np.zeros(cap, np.int32)
+            dist = np.zeros(cap, np.float32)
+            cnt = lib().orc_radius(metric, _p(data), n, dim, _p(queries[j]), r, _p(idx), _p(dist), cap)
+            if cnt <= cap:
+                break
+            cap = int(cnt)
+        res.append((idx[:cnt].copy(), dist[:cnt].copy()))
+    return res
+
+
+def nearest(metric: int, data: np.ndarray, query) -> int:
+    data = np.ascontiguousarray(data, np.float32)
+    q = np.ascontiguousarray(query, np.float32)
+    return lib().orc_nearest(metric, _p(data), data.shape[0], data.shape[1], _p(q))
+
+
+def ref_linear(metric: int, data: np.ndarray, queries: np.ndarray, k: int, r: float, workdir: Path):
+    """Runs the reference NearestNeighborsLinear driver (FP64 distances)."""
+    if not REF_LINEAR.exists():
+        raise FileNotFoundError(REF_LINEAR)
+    workdir = Path(workdir)
+    d = workdir / "data.f32"
+    q = workdir / "queries.f32"
+    o = workdir / "out.txt"
+    np.ascontiguousarray(data, np.float32).tofile(d)
+    np.ascontiguousarray(queries, np.float32).tofile(q)
+    subprocess.run([str(REF_LINEAR), str(metric), str(data.shape[1]), str(d), str(data.shape[0]), str(q),
+                    str(queries.shape[0]), str(k), repr(float(r)), str(o)], check=True)
+    knn_res, rad_res, nearest_res = [], [], []
+    for line in o.read_text().splitlines():
+        parts = line.split()
+        if parts[0] == "K":
+            vals = parts[1:]
+            knn_res.append((np.array(vals[0::2], np.int64), np.array(vals[1::2], np.float64)))
+        elif parts[0] == "R":
+            vals = parts[2:]
+            rad_res.append((np.array(vals[0::2], np.int64), np.array(vals[1::2], np.float64)))
+        else:
+            nearest_res.append(int(parts[1]))
+    return knn_res, rad_res, nearest_res

@@ -1,0 +1,171 @@
+"""GPU parity of state validity and motion validation against the CPU oracle.
+Bit-exact: validity and motion booleans, checked-state counts and lastValid
+fractions must equal the oracle's on the same seeded inputs."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+PI = math.pi
+LVS = 0.01  # 0.01 rad resolution: 2*pi*sqrt(7) * (0.01 / (2*pi*sqrt(7)))
+
+
+def lvs_config():
+    ext = 2 * PI * math.sqrt(7)
+    return ext * (0.01 / ext)
+
+
+def to_dev(x, dtype=None):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda") if dtype is None else \
+        torch.from_numpy(np.ascontiguousarray(x)).to("cuda", dtype)
+
+
+def env_pair(gpu, oracle, seed, ns, nb):
+    s, b = gpu.workload.obstacles(seed, ns, nb)
+    return gpu.Env(s, b), oracle.OracleEnv(s, b), (s, b)
+
+
+def test_fk_bit_exact(gpu, oracle):
+    rng = np.random.default_rng(0)
+    q = rng.uniform(-PI, PI, (2000, 7)).astype(np.float32)
+    q[:10] = 0.0
+    q[10:20] = rng.uniform(-20, 20, (10, 7))
+    got = gpu.arm_spheres(to_dev(q.T.copy())).cpu().numpy()
+    for i in range(q.shape[0]):
+        assert np.array_equal(got[i], oracle.arm_spheres(q[i])), i
+
+
+@pytest.mark.parametrize("lps", [1, 2, 4, 8])
+@pytest.mark.parametrize("ns,nb", [(64, 0), (128, 128), (0, 40), (0, 0)])
+def test_validate_states_bit_exact(gpu, oracle, lps, ns, nb):
+    print(f"validate lps={lps} ns={ns} nb={nb}", flush=True)
+    gpu.set_tuning(validity_lanes_per_state=lps)
+    env, oenv, _ = env_pair(gpu, oracle, 1000 + ns + nb, ns, nb)
+    q = gpu.workload.configs(1, 65536 if (ns, nb) == (64, 0) else 8192)
+    qf = q.astype(np.float32).T.copy()
+    got = gpu.validate_states(env, to_dev(qf)).cpu().numpy()
+    ref = oracle.validate_states(oenv, qf)
+    assert np.array_equal(got, ref)
+    # FP64 input: cast per joint on device
+    got64 = gpu.validate_states(env, to_dev(q.T.copy())).cpu().numpy()
+    assert np.array_equal(got64, ref)
+    if ns + nb == 0:
+        assert ref.all()
+    gpu.set_tuning(validity_lanes_per_state=8)
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 129, 1000])
+def test_validate_ragged_sizes(gpu, oracle, n):
+    env, oenv, _ = env_pair(gpu, oracle, 7, 32, 32)
+    q = gpu.workload.configs(n, n).astype(np.float32).T.copy()
+    for lps in (1, 8):
+        gpu.set_tuning(validity_lanes_per_state=lps)
+        assert np.array_equal(gpu.validate_states(env, to_dev(q)).cpu().numpy(), oracle.validate_states(oenv, q))
+    gpu.set_tuning(validity_lanes_per_state=8)
+
+
+def test_is_valid_single_call(gpu, oracle):
+    env, oenv, _ = env_pair(gpu, oracle, 1000, 64, 0)
+    q = gpu.workload.configs(5, 200)
+    for i in range(q.shape[0]):
+        assert gpu.is_valid(env, q[i]) == oracle.state_valid(oenv, q[i].astype(np.float32))
+
+
+def motion_case(gpu, seed, n, mu=-1.0, sigma=1.0, lo=0.05, hi=5.0):
+    a, b = gpu.workload.edges(seed, n, mu, sigma, lo, hi)
+    return a, b
+
+
+@pytest.mark.parametrize("two_phase", [1, 0])
+@pytest.mark.parametrize("lps", [1, 4, 8])
+def test_check_motions_config2_bit_exact(gpu, oracle, two_phase, lps):
+    """Config: 16,384 edges, 128 spheres + 128 boxes, 0.01 rad resolution."""
+    print(f"motions two_phase={two_phase} lps={lps}", flush=True)
+    gpu.set_tuning(motion_two_phase=two_phase, motion_lanes_per_state=lps, validity_lanes_per_state=lps)
+    env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
+    a, b = motion_case(gpu, 4, 16384)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs)
+    A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
+    v, c = gpu.check_motions(env, A, B, lvs, exact_count=True)
+    assert np.array_equal(v.cpu().numpy(), ref_v)
+    assert np.array_equal(c.cpu().numpy(), ref_c)
+    v2 = gpu.check_motions(env, A, B, lvs)  # no count: early exit at any hit
+    assert np.array_equal(v2.cpu().numpy(), ref_v)
+    gpu.set_tuning(motion_two_phase=1, motion_lanes_per_state=1, validity_lanes_per_state=8)
+
+
+@pytest.mark.parametrize("two_phase", [1, 0])
+def test_check_motions_sparse_env_deep_walks(gpu, oracle, two_phase):
+    """Few obstacles: most edges are valid, so every bisection position runs."""
+    gpu.set_tuning(motion_two_phase=two_phase)
+    env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
+    a, b = motion_case(gpu, 9, 3000, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs)
+    assert 0.2 < ref_v.mean() < 0.95
+    v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, exact_count=True)
+    assert np.array_equal(v.cpu().numpy(), ref_v)
+    assert np.array_equal(c.cpu().numpy(), ref_c)
+    gpu.set_tuning(motion_two_phase=1)
+
+
+def test_check_motions_edge_cases(gpu, oracle):
+    env, oenv, _ = env_pair(gpu, oracle, 1003, 8, 8)
+    rng = np.random.default_rng(4)
+    a = rng.uniform(-PI, PI, (40, 7))
+    b = a.copy()
+    b[10:20] += 0.004          # nd = 1
+    b[20:30] += rng.normal(size=(10, 7)) * 2.0   # long edges (nd in the thousands)
+    b[30:40] = np.clip(b[30:40] + 0.9, -PI, PI)
+    lvs = lvs_config()
+    for factor in (1, 2):
+        ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs, factor)
+        for tp in (1, 0):
+            gpu.set_tuning(motion_two_phase=tp)
+            v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, factor, exact_count=True)
+            assert np.array_equal(v.cpu().numpy(), ref_v)
+            assert np.array_equal(c.cpu().numpy(), ref_c)
+    gpu.set_tuning(motion_two_phase=1)
+
+
+def test_check_motion_plain_single_call(gpu, oracle):
+    env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
+    a, b = motion_case(gpu, 12, 300, mu=-0.5)
+    lvs = lvs_config()
+    for i in range(a.shape[0]):
+        assert gpu.check_motion(env, a[i], b[i], lvs) == oracle.check_motion(oenv, a[i], b[i], lvs)[0]
+
+
+@pytest.mark.parametrize("lps", [1, 8])
+def test_check_motions_last_valid(gpu, oracle, lps):
+    gpu.set_tuning(motion_lanes_per_state=lps)
+    env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
+    a, b = motion_case(gpu, 13, 2000, mu=-0.5)
+    lvs = lvs_config()
+    v, t = gpu.check_motions_last_valid(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs)
+    v, t = v.cpu().numpy(), t.cpu().numpy()
+    for i in range(a.shape[0]):
+        ov, ot, _ = oracle.check_motion_last_valid(oenv, a[i], b[i], lvs)
+        assert v[i] == ov
+        if not ov:
+            assert t[i] == ot, (i, t[i], ot)
+        else:
+            assert math.isnan(t[i])
+    for i in range(0, 200, 7):
+        ok, tt = gpu.check_motion_last_valid(env, a[i], b[i], lvs)
+        ov, ot, _ = oracle.check_motion_last_valid(oenv, a[i], b[i], lvs)
+        assert ok == ov and (ov or tt == ot)
+    gpu.set_tuning(motion_lanes_per_state=1)
+
+
+def test_empty_batches(gpu):
+    import torch
+
+    env = gpu.Env(np.zeros((1, 4), np.float32), None)
+    z = torch.empty((7, 0), dtype=torch.float64, device="cuda")
+    assert gpu.check_motions(env, z, z, 0.01).numel() == 0
+    assert gpu.validate_states(env, z.float()).numel() == 0

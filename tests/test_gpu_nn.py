@@ -1,0 +1,155 @@
+"""GPU parity of the nearest-neighbour index against the CPU oracle (exact:
+identical index lists and bit-identical FP32 distances, ties by lower index)
+and against the reference's NearestNeighborsLinear golden data (index sets
+equal except at FP32 near-ties)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def make_nn(gpu, metric, dim, data, batches=1):
+    from ompl_amd.nn import NearestNeighborsGPU
+
+    nn = NearestNeighborsGPU(metric, dim)
+    for part in np.array_split(data, batches):
+        nn.add(part)
+    return nn
+
+
+def se3_points(gpu, seed, n, lo=-10.0, hi=10.0):
+    return gpu.workload.se3(seed, n, lo, hi)
+
+
+@pytest.mark.parametrize("name,metric,dim", [("nn_linear_l2.npz", 0, 7), ("nn_linear_se3.npz", 1, 7)])
+def test_knn_golden_reference_linear(gpu, oracle, name, metric, dim):
+    f = np.load(GOLDEN / name)
+    data, queries, k = f["data"], f["queries"], int(f["k"])
+    nn = make_nn(gpu, metric, dim, data)
+    gi, gd = nn.knn_batch(queries, k)
+    oi, od = oracle.knn(metric, data, queries, k)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    ties = 0
+    for j in range(queries.shape[0]):
+        if set(gi[j].tolist()) != set(f["knn_idx"][j].tolist()):
+            ties += 1  # only allowed at an FP32 near-tie of the k-th distance
+            assert abs(float(gd[j][-1]) - float(f["knn_dist"][j][-1])) < 2e-5
+    assert ties <= 2
+
+
+@pytest.mark.parametrize("name,metric,dim", [("nn_linear_l2.npz", 0, 7), ("nn_linear_se3.npz", 1, 7)])
+def test_radius_golden_reference_linear(gpu, oracle, name, metric, dim):
+    f = np.load(GOLDEN / name)
+    data, queries, r = f["data"], f["queries"], float(f["radius"])
+    nn = make_nn(gpu, metric, dim, data, batches=3)
+    off, gi, gd = nn.radius_batch(queries, r)
+    ores = oracle.radius(metric, data, queries, r)
+    for j, (oi, od) in enumerate(ores):
+        assert np.array_equal(gi[off[j]:off[j + 1]], oi)
+        assert np.array_equal(gd[off[j]:off[j + 1]], od)
+
+
+def test_knn_config1_r7(gpu, oracle):
+    """100k states U[-pi,pi]^7, k=10 (4096 queries on the GPU; 384 checked
+    against the oracle's full scan)."""
+    rng = np.random.default_rng(2)
+    data = rng.uniform(-np.pi, np.pi, (100_000, 7)).astype(np.float32)
+    queries = rng.uniform(-np.pi, np.pi, (4096, 7)).astype(np.float32)
+    nn = make_nn(gpu, 0, 7, data, batches=2)
+    gi, gd = nn.knn_batch(queries, 10)
+    assert np.all(np.diff(gd, axis=1) >= 0)
+    sel = np.arange(0, 4096, 4096 // 384)[:384]
+    oi, od = oracle.knn(0, data, queries[sel], 10)
+    assert np.array_equal(gi[sel], oi) and np.array_equal(gd[sel], od)
+
+
+def test_knn_config3_se3_sample(gpu, oracle):
+    """1M SE3 states, 65,536 queries, k=16; 48 queries checked exactly."""
+    data = se3_points(gpu, 5, 1_000_000)
+    queries = se3_points(gpu, 6, 65_536)
+    nn = make_nn(gpu, 1, 7, data)
+    gi, gd = nn.knn_batch(queries, 16)
+    assert np.all(np.diff(gd, axis=1) >= 0) and np.all(gi >= 0)
+    sel = np.linspace(0, 65_535, 48).astype(int)
+    oi, od = oracle.knn(1, data, queries[sel], 16)
+    assert np.array_equal(gi[sel], oi) and np.array_equal(gd[sel], od)
+
+
+def test_radius_config3_se3_sample(gpu, oracle):
+    data = se3_points(gpu, 5, 200_000)
+    queries = se3_points(gpu, 6, 2048)
+    r = gpu.rrtstar_radius(6, 8000.0 * np.pi ** 2, 1_000_000)
+    assert abs(r - 0.97753) < 1e-4  # RRT* schedule at n = 1e6
+    nn = make_nn(gpu, 1, 7, data)
+    off, gi, gd = nn.radius_batch(queries, r)
+    sel = list(range(0, 2048, 64))
+    ores = oracle.radius(1, data, queries[sel], r)
+    for j, (oi, od) in zip(sel, ores):
+        assert np.array_equal(gi[off[j]:off[j + 1]], oi)
+        assert np.array_equal(gd[off[j]:off[j + 1]], od)
+
+
+def test_ties_duplicates_and_k_edge_cases(gpu, oracle):
+    rng = np.random.default_rng(9)
+    base = rng.uniform(-1, 1, (50, 7)).astype(np.float32)
+    data = np.concatenate([base, base, base[:10]])  # exact duplicates: ties by lower index
+    nn = make_nn(gpu, 0, 7, data, batches=4)
+    q = np.concatenate([base[:8], rng.uniform(-1, 1, (8, 7)).astype(np.float32)])
+    for k in (1, 3, 16, 17, 64):
+        gi, gd = nn.knn_batch(q, k)
+        oi, od = oracle.knn(0, data, q, k)
+        assert np.array_equal(gi, oi) and np.array_equal(gd, od), k
+    # k larger than the structure: -1 / +inf padding
+    small = make_nn(gpu, 0, 7, base[:5])
+    gi, gd = small.knn_batch(q[:2], 8)
+    assert np.all(gi[:, 5:] == -1) and np.all(np.isinf(gd[:, 5:]))
+    assert set(gi[0, :5].tolist()) == set(range(5))
+
+
+def test_empty_nearest_raises_and_removal(gpu, oracle):
+    from ompl_amd._lib import OmplAmdError
+
+    nn = make_nn(gpu, 0, 3, np.zeros((0, 3), np.float32))
+    with pytest.raises(OmplAmdError, match="No elements found"):
+        nn.nearest([0, 0, 0])
+    gi, gd = nn.knn_batch(np.zeros((2, 3), np.float32), 4)
+    assert np.all(gi == -1)
+    pts = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [3, 0, 0]], np.float32)
+    nn.add(pts)
+    assert nn.nearest([0.9, 0, 0]) == 1
+    assert nn.remove(1) and not nn.remove(1) and nn.size() == 3
+    assert nn.nearest([0.9, 0, 0]) == 0  # equal distance to 0 and 2: lower id
+    i, d = nn.nearestK([0.9, 0, 0], 3)
+    assert list(i) == [0, 2, 3]
+    i, d = nn.nearestR([1.0, 0, 0], 1.0)  # inclusive radius
+    assert list(i) == [0, 2]
+    nn.clear()
+    assert nn.size() == 0
+
+
+def test_incremental_adds_match_oracle(gpu, oracle):
+    rng = np.random.default_rng(21)
+    nn = make_nn(gpu, 0, 7, np.zeros((0, 7), np.float32))
+    data = np.zeros((0, 7), np.float32)
+    for step in range(6):
+        new = rng.uniform(-np.pi, np.pi, (int(rng.integers(1, 700)), 7)).astype(np.float32)
+        nn.add(new)
+        data = np.concatenate([data, new])
+        q = rng.uniform(-np.pi, np.pi, (40, 7)).astype(np.float32)
+        gi, gd = nn.knn_batch(q, 10)
+        oi, od = oracle.knn(0, data, q, 10)
+        assert np.array_equal(gi, oi) and np.array_equal(gd, od), step
+
+
+@pytest.mark.parametrize("dim", [2, 3, 6, 11, 14])
+def test_other_dimensions(gpu, oracle, dim):
+    rng = np.random.default_rng(dim)
+    data = rng.uniform(-5, 5, (3000, dim)).astype(np.float32)
+    q = rng.uniform(-5, 5, (64, dim)).astype(np.float32)
+    nn = make_nn(gpu, 0, dim, data)
+    gi, gd = nn.knn_batch(q, 12)
+    oi, od = oracle.knn(0, data, q, 12)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
