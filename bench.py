@@ -56,7 +56,8 @@ def main(argv=None):
     ap.add_argument("--plain-edges", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tuning", type=str, default=None, help="JSON dict of launch knobs")
+    ap.add_argument("--tuning", type=str, default=None, help="JSON dict of launch knobs (skips the autotuner)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="all-cores CPU run (0 = os.cpu_count, max 16)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -69,14 +70,6 @@ def main(argv=None):
     import ompl_amd as oa
 
     oa.init(0)
-    tuning_path = ROOT / "ompl_amd" / "tuning.json"
-    if args.tuning:
-        oa.set_tuning(**json.loads(args.tuning))
-    elif tuning_path.exists():
-        cfg = json.loads(tuning_path.read_text()).get("motion_config2")
-        if cfg:
-            oa.set_tuning(**cfg)
-
     spheres, boxes = oa.workload.obstacles(ENV_SEED, N_SPHERES, N_BOXES)
     env = oa.Env(spheres, boxes)
     a_rows, b_rows = oa.workload.edges(EDGE_SEED, N_EDGES)
@@ -86,6 +79,14 @@ def main(argv=None):
     B = torch.from_numpy(b_rows.T.copy()).to(dev)
     out = torch.empty(N_EDGES, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
+
+    # ---- launch shape: the project autotuner (or a fixed shape from --tuning)
+    if args.tuning:
+        oa.set_tuning(**json.loads(args.tuning))
+        tuned_us = None
+    else:
+        _, tuned_us = oa.autotune_motions(env, A, B, lvs, reps=20)
+    t = oa.get_tuning()
 
     # ---- warmup
     for _ in range(args.warmup):
@@ -115,34 +116,31 @@ def main(argv=None):
         torch.cuda.synchronize()
         step_ms.append((time.perf_counter() - s0) * 1e3)
 
-    # ---- per-kernel durations with HIP events on the launch stream
+    # ---- per-kernel durations: HIP events recorded by the library on the
+    #      launch stream around every launch of the same K steps
     oa.profile.reset()
     oa.profile.enable(True)
     for _ in range(args.steps):
         oa.check_motions(env, A, B, lvs, out=out)
     torch.cuda.synchronize()
     kernels = {}
-    for name in ("motion_endpoint", "motion_walk"):
+    for name in ("motion_fused", "motion_endpoint", "motion_walk"):
         n, ms = oa.profile.query(name)
         if n:
             kernels[name] = {"launches": n, "avg_us": 1e3 * ms / n}
     oa.profile.enable(False)
     oa.profile.reset()
 
-    # ---- algorithmic work (reference order) for the roofline
+    # ---- algorithmic work in the reference order, for the roofline
     work = oa.motion_work(env, A, B, lvs)
     flops_step = oa.work_flops(work)
-    # s2 checks of every edge are the endpoint kernel's share
-    Bf = B.float().contiguous()
-    w_end = oa.validity_work(env, Bf)
+    w_end = oa.validity_work(env, B.float().contiguous())  # the s2 checks of every edge
     flops_end = oa.work_flops(w_end)
-    t = oa.get_tuning()
-    if t["motion_two_phase"] and "motion_endpoint" in kernels:
-        dom, dom_flops = "motion_endpoint", flops_end
-        if kernels.get("motion_walk", {}).get("avg_us", 0) > kernels["motion_endpoint"]["avg_us"]:
-            dom, dom_flops = "motion_walk", flops_step - flops_end
-    else:
-        dom, dom_flops = "motion_walk", flops_step
+    share = {"motion_fused": flops_step, "motion_endpoint": flops_end, "motion_walk": flops_step - flops_end}
+    if t["motion_strategy"] == 0:
+        share["motion_walk"] = flops_step
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    dom_flops = share[dom]
     dom_us = kernels[dom]["avg_us"]
     achieved = dom_flops / (dom_us * 1e-6) / 1e12
 
@@ -156,8 +154,9 @@ def main(argv=None):
         plain_lat.append((time.perf_counter() - s0) * 1e3)
     plain_rate = P / (sum(plain_lat) / 1e3)
 
-    # ---- CPU restatement (single thread) on a bounded sample + parity
+    # ---- CPU restatement on a bounded sample (repeated full passes) + parity
     cpu = None
+    cpu_all = None
     parity = None
     if not args.no_cpu:
         from oracle import oracle_py
@@ -165,19 +164,22 @@ def main(argv=None):
         oenv = oracle_py.OracleEnv(spheres, boxes)
         ref_v, _ = oracle_py.check_motions(oenv, a_rows, b_rows, lvs)
         parity = bool(np.array_equal(ref_v, valid)) and bool(np.array_equal(ref_v[:P], plain_valid))
-        done = 0
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            for i in range(0, N_EDGES, 64):
-                for j in range(i, min(i + 64, N_EDGES)):
-                    oracle_py.check_motion(oenv, a_rows[j], b_rows[j], lvs)
-                done += min(64, N_EDGES - i)
-                if time.perf_counter() - t0 >= args.cpu_seconds:
-                    break
-        cpu_s = time.perf_counter() - t0
+
+        def cpu_rate(threads, seconds):
+            done, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < seconds:
+                oracle_py.check_motions(oenv, a_rows, b_rows, lvs, threads=threads)
+                done += N_EDGES
+            return done, time.perf_counter() - t0
+
+        done, cpu_s = cpu_rate(1, args.cpu_seconds)
         cpu = {"value": done / cpu_s, "unit": "edge-checks/s", "cores": 1, "kind": "port",
-               "sample": f"{done} edges of the same batch (repeated passes), {cpu_s:.1f} s, "
-                         "C restatement via ctypes per edge, -O2 -ffp-contract=off"}
+               "sample": f"{done // N_EDGES} full passes over the 16384-edge batch ({cpu_s:.1f} s), C restatement "
+                         "(-O2 -ffp-contract=off), one thread"}
+        nthr = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        done, cpu_s = cpu_rate(nthr, max(2.0, args.cpu_seconds / 3))
+        cpu_all = {"value": done / cpu_s, "unit": "edge-checks/s", "cores": nthr, "kind": "port",
+                   "sample": f"{done // N_EDGES} full passes, {cpu_s:.1f} s, POSIX threads over edges"}
 
     res = {
         "metric": "edge-checks/s (batched DiscreteMotionValidator, bit-exact)",
@@ -195,17 +197,20 @@ def main(argv=None):
         "config": {"workload": "uneven motion validation: 7-DOF arm, 128 spheres + 128 boxes, 16384 edges "
                                "lognormal(-1,1) length, 0.01 rad, bisection + first-collision exit",
                    "edges": N_EDGES, "obstacles": N_SPHERES + N_BOXES, "tuning": t},
+        "kernels": kernels,
         "valid_fraction": float(valid.mean()),
         "step_latency_ms": {"p50": percentile(step_ms, 50), "p99": percentile(step_ms, 99)},
         "plain": {"edge_checks_per_s": plain_rate, "edges": P,
                   "call_latency_ms": {"p50": percentile(plain_lat, 50), "p99": percentile(plain_lat, 99)}},
         "batched_over_plain": value / plain_rate,
-        "kernels": kernels,
         "work_per_step": work,
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "algorithmic_flops_per_launch": dom_flops},
+                     "kernel": dom, "kernel_avg_us": dom_us, "algorithmic_flops_per_launch": dom_flops,
+                     "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
         "cpu_baseline": cpu,
+        "cpu_all_cores": cpu_all,
+        "autotuned_us_per_batch": tuned_us,
         "parity_bit_exact": parity,
         "wall_s": wall,
     }

@@ -117,11 +117,21 @@ typedef struct {
     int motion_lanes_per_state;
     int motion_block;
     int motion_waves_per_cu; /* persistent interior kernel occupancy */
-    int motion_two_phase;    /* 1: s2 pass over all edges, then walk the survivors;
-                                0: one wave walks each edge from s2 */
+    int motion_strategy;     /* 0: one wave walks each edge from s2;
+                                1: s2 pass over all edges, then a persistent
+                                   queue walks the survivors (3 launches);
+                                2: fused: each wave checks s2 of its edges and
+                                   walks its own survivors (1 launch) */
 } ompl_amd_tuning;
 int ompl_amd_get_tuning(ompl_amd_tuning *t);
 int ompl_amd_set_tuning(const ompl_amd_tuning *t);
+/* Autotuner: times every candidate launch shape (strategy x lanes x block x
+ * occupancy) on the given device-resident batch with HIP events, installs the
+ * fastest as the current tuning and returns it with its time per batch.  No
+ * counterpart in the reference. */
+int ompl_amd_autotune_motions(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
+                              double longest_valid_segment, unsigned factor, int reps, ompl_amd_tuning *best,
+                              double *best_us);
 
 /* Work a scalar checker performs on the same inputs in the reference order
  * (s2, then bisection positions up to the first invalid state; per state the
@@ -189,6 +199,8 @@ int ompl_amd_nn_radius_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, float
                            int64_t *total, void *stream);
 /* Host convenience (blocking). */
 int ompl_amd_nn_knn(ompl_amd_nn nn, const float *q, int64_t nq, int k, int32_t *out_idx, float *out_dist);
+int ompl_amd_nn_radius(ompl_amd_nn nn, const float *q, int64_t nq, float radius, int64_t *out_offsets,
+                       int32_t *out_idx, float *out_dist, int64_t capacity, int64_t *total);
 int ompl_amd_nn_nearest(ompl_amd_nn nn, const float *q, int64_t *out_id);
 
 /* RRT* neighbourhood radius (geometric/planners/rrt/src/RRTstar.cpp:602-617,
@@ -199,8 +211,10 @@ double ompl_amd_rrtstar_radius(int dim, double measure, double rewire_factor, do
 /* ------------------------------------------------------------------ */
 /* Synthetic workloads (std::mt19937 + OMPL RNG semantics, host side)   */
 /* ------------------------------------------------------------------ */
-/* Obstacles: centres U[-1,1]^3 rejected while |c| < min_base_dist; sphere
- * radii U[r_lo, r_hi]; box half-extents U[h_lo, h_hi] per axis. */
+/* Obstacles: centre U[-1,1]^3 then sphere radius U[r_lo, r_hi] (or box
+ * half-extents U[h_lo, h_hi] per axis); a draw is rejected while its centre is
+ * within min_base_dist of the origin or it overlaps the arm's fixed base link
+ * (link 0's spheres), so the base region stays free. */
 int ompl_amd_workload_obstacles(uint32_t seed, int n_spheres, int n_boxes, double r_lo, double r_hi, double h_lo,
                                 double h_hi, double min_base_dist, float *spheres_out, float *boxes_out);
 /* n joint configs U[-pi,pi]^7, FP64 row-major [n][7] (sampleUniform order). */

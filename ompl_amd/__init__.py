@@ -111,6 +111,16 @@ def work_flops(w: dict) -> int:
     return w["states"] * FK_FLOPS + w["sphere_pairs"] * SPHERE_PAIR_FLOPS + w["box_pairs"] * BOX_PAIR_FLOPS
 
 
+def autotune_motions(env: "Env", a_soa, b_soa, lvs: float, factor: int = 1, reps: int = 20):
+    """Runs the library's launch-shape autotuner on a device-resident batch and
+    installs the fastest shape; returns (tuning dict, microseconds per batch)."""
+    t = Tuning()
+    us = C.c_double()
+    call("ompl_amd_autotune_motions", env.handle, _ptr(a_soa), _ptr(b_soa), a_soa.shape[1], float(lvs),
+         int(factor), int(reps), C.byref(t), C.byref(us))
+    return t.as_dict(), us.value
+
+
 def longest_valid_segment(lo, hi, resolution: float) -> float:
     """StateSpace::setup: maxExtent * fraction (base/src/StateSpace.cpp:237-241),
     with fraction = resolution / maxExtent as SpaceInformation::

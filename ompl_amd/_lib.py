@@ -44,6 +44,8 @@ SIGNATURES = {
                                                    C.POINTER(C.c_double)]),
     "ompl_amd_get_tuning": (C.c_int, [vp]),
     "ompl_amd_set_tuning": (C.c_int, [vp]),
+    "ompl_amd_autotune_motions": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, C.c_int, vp,
+                                            C.POINTER(C.c_double)]),
     "ompl_amd_motion_work": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp]),
     "ompl_amd_validity_work": (C.c_int, [vp, vp, C.c_int64, vp]),
     "ompl_amd_profile_enable": (C.c_int, [C.c_int]),
@@ -61,6 +63,7 @@ SIGNATURES = {
     "ompl_amd_nn_knn_dev": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp, vp]),
     "ompl_amd_nn_radius_dev": (C.c_int, [vp, vp, C.c_int64, C.c_float, vp, vp, vp, C.c_int64, c_i64_p, vp]),
     "ompl_amd_nn_knn": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp]),
+    "ompl_amd_nn_radius": (C.c_int, [vp, vp, C.c_int64, C.c_float, vp, vp, vp, C.c_int64, c_i64_p]),
     "ompl_amd_nn_nearest": (C.c_int, [vp, vp, c_i64_p]),
     "ompl_amd_rrtstar_radius": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64]),
     "ompl_amd_workload_obstacles": (C.c_int, [C.c_uint32, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -85,7 +88,7 @@ class Tuning(C.Structure):
         ("motion_lanes_per_state", C.c_int),
         ("motion_block", C.c_int),
         ("motion_waves_per_cu", C.c_int),
-        ("motion_two_phase", C.c_int),
+        ("motion_strategy", C.c_int),
     ]
 
     def as_dict(self):

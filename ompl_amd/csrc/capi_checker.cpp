@@ -15,15 +15,14 @@ Env &env_of(ompl_amd_env e)
 }
 
 // Batched DiscreteMotionValidator::checkMotion(s1, s2) over device arrays.
-void check_motions(const Env &env, const double *a, const double *b, int64_t n, double lvs, unsigned factor,
-                   uint8_t *out, int32_t *checked, hipStream_t s)
+void check_motions_t(const Tuning &t, const Env &env, const double *a, const double *b, int64_t n, double lvs,
+                     unsigned factor, uint8_t *out, int32_t *checked, hipStream_t s)
 {
     OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative edge count");
     if (n == 0) return;
     OA_REQUIRE(a && b && out, OMPL_AMD_ERR_ARG, "null edge or output pointer");
     OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "longest valid segment must be > 0 and factor >= 1");
     OA_REQUIRE(n < (int64_t)1 << 31, OMPL_AMD_ERR_ARG, "at most 2^31-1 edges per call");
-    const Tuning t = get_tuning();
     MotionArgs A{};
     A.env = env.view();
     A.a = a;
@@ -34,7 +33,10 @@ void check_motions(const Env &env, const double *a, const double *b, int64_t n, 
     A.out = out;
     A.checked = checked;
     const bool exact = checked != nullptr;
-    if (t.motion_two_phase) {
+    if (t.motion_strategy == 2) {
+        A.start = 1;
+        launch_motion_fused(A, exact, t.validity_lanes_per_state, t.motion_lanes_per_state, t.motion_block, s);
+    } else if (t.motion_strategy == 1) {
         Workspace &w = workspace(s, (size_t)n);
         A.list = w.list;
         A.list_count = w.counters;
@@ -48,6 +50,32 @@ void check_motions(const Env &env, const double *a, const double *b, int64_t n, 
         A.start = 0;
         launch_motion_walk(A, exact, t.motion_lanes_per_state, t.motion_block, n, s);
     }
+}
+
+void check_motions(const Env &env, const double *a, const double *b, int64_t n, double lvs, unsigned factor,
+                   uint8_t *out, int32_t *checked, hipStream_t s)
+{
+    check_motions_t(get_tuning(), env, a, b, n, lvs, factor, out, checked, s);
+}
+
+// Candidate launch shapes of the motion autotuner.
+std::vector<Tuning> motion_candidates(const Tuning &base)
+{
+    std::vector<Tuning> c;
+    const int ends[] = {8, 4, 2, 1};
+    for (int E : ends)
+        for (int W : {1, 2, 4})
+            for (int block : {64, 256, 1024}) {
+                // fused: one launch
+                const bool built = (E == 8 && W <= 4) || (E == 4 && W <= 2) || (E == 2 && W == 1) || (E == 1 && W == 1);
+                if (built) c.push_back(Tuning{E, block, W, block, base.motion_waves_per_cu, 2});
+                // two-phase queue
+                for (int wpc : {4, 8, 16})
+                    if (block == 256) c.push_back(Tuning{E, block, W, 256, wpc, 1});
+            }
+    for (int W : {1, 2, 4, 8})
+        for (int block : {64, 256}) c.push_back(Tuning{base.validity_lanes_per_state, 256, W, block, 8, 0});
+    return c;
 }
 
 } // namespace
@@ -238,3 +266,48 @@ int ompl_amd_check_motion_last_valid(ompl_amd_env e, const double *a, const doub
 }
 
 } // extern "C"
+
+extern "C" int ompl_amd_autotune_motions(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
+                                         unsigned factor, int reps, ompl_amd_tuning *best, double *best_us)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(n > 0 && a && b && reps >= 1, OMPL_AMD_ERR_ARG, "bad autotune arguments");
+        ThreadResources &r = thread_resources();
+        hipStream_t s = r.stream;
+        uint8_t *out = nullptr;
+        OA_HIP(hipMalloc(&out, (size_t)n));
+        hipEvent_t e0, e1;
+        OA_HIP(hipEventCreate(&e0));
+        OA_HIP(hipEventCreate(&e1));
+        const Tuning base = get_tuning();
+        Tuning win = base;
+        float win_ms = INFINITY;
+        for (const Tuning &t : motion_candidates(base)) {
+            for (int w = 0; w < 2; ++w) check_motions_t(t, env, a, b, n, lvs, factor, out, nullptr, s);
+            OA_HIP(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; ++i) check_motions_t(t, env, a, b, n, lvs, factor, out, nullptr, s);
+            OA_HIP(hipEventRecord(e1, s));
+            OA_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            OA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < win_ms) {
+                win_ms = ms;
+                win = t;
+            }
+        }
+        OA_HIP(hipEventDestroy(e0));
+        OA_HIP(hipEventDestroy(e1));
+        OA_HIP(hipFree(out));
+        set_tuning(win);
+        if (best) {
+            best->validity_lanes_per_state = win.validity_lanes_per_state;
+            best->validity_block = win.validity_block;
+            best->motion_lanes_per_state = win.motion_lanes_per_state;
+            best->motion_block = win.motion_block;
+            best->motion_waves_per_cu = win.motion_waves_per_cu;
+            best->motion_strategy = win.motion_strategy;
+        }
+        if (best_us) *best_us = 1e3 * (double)win_ms / reps;
+    });
+}

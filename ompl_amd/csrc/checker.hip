@@ -162,14 +162,10 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
     const int lane = lane_id();
     const int waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t count = A.list ? (int64_t)(*A.list_count) : A.n;
-    // dynamic queue when a counter is given, otherwise a static wave-stride
-    int64_t k = A.queue ? -1 : (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    // first item by wave id; then a dynamic queue when a counter is given
+    // (items waves_total, waves_total+1, ...), otherwise a static wave-stride
+    int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     for (;;) {
-        if (A.queue) {
-            int t = 0;
-            if (lane == 0) t = atomicAdd(A.queue, 1);
-            k = __shfl(t, 0);
-        }
         if (k >= count) break;
         const int64_t e = A.list ? (int64_t)A.list[k] : k;
         double a[kDof], b[kDof];
@@ -184,7 +180,76 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
             A.out[e] = bad == INT_MAX ? 1 : 0;
             if (A.checked) A.checked[e] = bad == INT_MAX ? total : (bad < 0 ? 0 : bad + 1);
         }
-        if (!A.queue) k += waves_total;
+        if (A.queue) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(A.queue, 1);
+            k = waves_total + (int64_t)__shfl(t, 0);
+        } else {
+            k += waves_total;
+        }
+    }
+}
+
+// Fused batch: each wave checks s2 of 64/LPS_E edges (LPS_E lanes per edge),
+// then walks its own survivors one after another with LPS_W lanes per state.
+// One launch, no queue and no counters.
+template <int LPS_E, int LPS_W, bool EXACT>
+__global__ void __launch_bounds__(1024) motion_fused_kernel(MotionArgs A)
+{
+    extern __shared__ float4 lds[];
+    stage_env(A.env, lds);
+    constexpr int G = 64 / LPS_E;
+    const int lane = lane_id();
+    const int g = lane % LPS_E;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t e = wave * G + lane / LPS_E;
+    const bool live = e < A.n;
+    double a[kDof], b[kDof];
+    float q[kDof];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) {
+        a[j] = live ? A.a[(int64_t)j * A.n + e] : 0.0;
+        b[j] = live ? A.b[(int64_t)j * A.n + e] : 0.0;
+        q[j] = (float)b[j];
+    }
+    LaneSpheres<LPS_E> S;
+    arm_fk<LPS_E>(q, g, S);
+    const bool hit = collide_env<LPS_E, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live);
+    const uint64_t H = fold_groups<LPS_E>(wave_ballot(hit && live));
+    const bool s2_valid = !((H >> lane) & 1);
+    bool survivor = false;
+    if (live && g == 0) {
+        const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
+        if (!s2_valid) {
+            A.out[e] = 0;
+            if (A.checked) A.checked[e] = 1;
+        } else if (nd > kMaxSegments) {
+            A.out[e] = 0;
+            if (A.checked) A.checked[e] = 0;
+        } else if (nd < 2) {
+            A.out[e] = 1;
+            if (A.checked) A.checked[e] = 1;
+        } else {
+            survivor = true;
+        }
+    }
+    uint64_t sm = wave_ballot(survivor);
+    while (sm) {
+        const int src = lowest_bit(sm);
+        sm &= sm - 1;
+        const int64_t es = wave * G + src / LPS_E;
+        double ea[kDof], eb[kDof];
+#pragma unroll
+        for (int j = 0; j < kDof; ++j) {
+            ea[j] = A.a[(int64_t)j * A.n + es];
+            eb[j] = A.b[(int64_t)j * A.n + es];
+        }
+        int total = 0;
+        const int bad = walk_edge<LPS_W, EXACT>(ea, eb, A.lvs, A.factor, lds, A.env.ns, A.env.nb, 1, total);
+        if (lane == 0) {
+            A.out[es] = bad == INT_MAX ? 1 : 0;
+            if (A.checked) A.checked[es] = bad == INT_MAX ? total : bad + 1;
+        }
     }
 }
 
@@ -367,6 +432,40 @@ void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int block, int
     default: launch_walk_t<8>(A, exact, block, waves, s); break;
     }
     OA_HIP(hipGetLastError());
+}
+
+template <int LPS_E, int LPS_W>
+static void launch_fused_t(const MotionArgs &A, bool exact, int block, hipStream_t s)
+{
+    const int64_t per_block = (int64_t)(block / 64) * (64 / LPS_E);
+    const unsigned blocks = (unsigned)((A.n + per_block - 1) / per_block);
+    if (exact)
+        hipLaunchKernelGGL((motion_fused_kernel<LPS_E, LPS_W, true>), dim3(blocks), dim3(block), lds_bytes(A.env), s,
+                           A);
+    else
+        hipLaunchKernelGGL((motion_fused_kernel<LPS_E, LPS_W, false>), dim3(blocks), dim3(block), lds_bytes(A.env),
+                           s, A);
+}
+
+void launch_motion_fused(const MotionArgs &A, bool exact, int lps_end, int lps_walk, int block, hipStream_t s)
+{
+    if (A.n <= 0) return;
+    ProfScope ps("motion_fused", s);
+#define OA_FUSED(E, W)                                                                                             \
+    if (lps_end == E && lps_walk == W) {                                                                            \
+        launch_fused_t<E, W>(A, exact, block, s);                                                                   \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+    OA_FUSED(8, 1)
+    OA_FUSED(8, 2)
+    OA_FUSED(8, 4)
+    OA_FUSED(4, 1)
+    OA_FUSED(4, 2)
+    OA_FUSED(2, 1)
+    OA_FUSED(1, 1)
+#undef OA_FUSED
+    fail(OMPL_AMD_ERR_ARG, "fused motion kernel not built for lanes (%d, %d)", lps_end, lps_walk);
 }
 
 void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result, hipStream_t s)

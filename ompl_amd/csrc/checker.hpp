@@ -39,6 +39,7 @@ void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, in
 void launch_zero(int *p, int n, hipStream_t s);
 void launch_motion_endpoint(const MotionArgs &A, int lps, int block, hipStream_t s);
 void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int block, int64_t waves, hipStream_t s);
+void launch_motion_fused(const MotionArgs &A, bool exact, int lps_end, int lps_walk, int block, hipStream_t s);
 void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result,
                           hipStream_t s);
 void launch_validate_single(const EnvView &env, const State1 &st, int *result, hipStream_t s);

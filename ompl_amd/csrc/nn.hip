@@ -799,6 +799,41 @@ int ompl_amd_nn_knn(ompl_amd_nn h, const float *q, int64_t nq, int k, int32_t *o
     });
 }
 
+int ompl_amd_nn_radius(ompl_amd_nn h, const float *q, int64_t nq, float r, int64_t *offsets, int32_t *out_i,
+                       float *out_d, int64_t capacity, int64_t *total)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(nq >= 0 && offsets && total && (nq == 0 || q), OMPL_AMD_ERR_ARG, "bad arguments");
+        hipStream_t s = default_stream();
+        float *dq = nullptr;
+        int64_t *doff = nullptr;
+        int32_t *di = nullptr;
+        float *dd = nullptr;
+        const bool want = out_i || out_d;
+        OA_HIP(hipMallocAsync((void **)&dq, (size_t)(nq ? nq : 1) * x.dim * sizeof(float), s));
+        OA_HIP(hipMallocAsync((void **)&doff, (size_t)(nq + 1) * sizeof(int64_t), s));
+        if (want && capacity > 0) {
+            OA_HIP(hipMallocAsync((void **)&di, (size_t)capacity * sizeof(int32_t), s));
+            OA_HIP(hipMallocAsync((void **)&dd, (size_t)capacity * sizeof(float), s));
+        }
+        if (nq) OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
+        int64_t tot = 0;
+        radius(x, dq, nq, r, doff, want ? di : nullptr, want ? dd : nullptr, capacity, &tot, s);
+        OA_HIP(hipMemcpyAsync(offsets, doff, (size_t)(nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        if (want && tot > 0) {
+            if (out_i) OA_HIP(hipMemcpyAsync(out_i, di, (size_t)tot * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            if (out_d) OA_HIP(hipMemcpyAsync(out_d, dd, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+        OA_HIP(hipFreeAsync(dq, s));
+        OA_HIP(hipFreeAsync(doff, s));
+        if (di) OA_HIP(hipFreeAsync(di, s));
+        if (dd) OA_HIP(hipFreeAsync(dd, s));
+        OA_HIP(hipStreamSynchronize(s));
+        *total = tot;
+    });
+}
+
 int ompl_amd_nn_nearest(ompl_amd_nn h, const float *q, int64_t *out_id)
 {
     return guarded([&] {

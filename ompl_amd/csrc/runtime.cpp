@@ -184,6 +184,13 @@ void set_tuning(const Tuning &t)
                "block must be a power of two in [64, 1024]");
     OA_REQUIRE(t.motion_waves_per_cu >= 1 && t.motion_waves_per_cu <= 32, OMPL_AMD_ERR_ARG,
                "motion_waves_per_cu must be in [1, 32]");
+    OA_REQUIRE(t.motion_strategy >= 0 && t.motion_strategy <= 2, OMPL_AMD_ERR_ARG, "motion_strategy must be 0, 1 or 2");
+    if (t.motion_strategy == 2) {
+        // instantiated (endpoint lanes, walk lanes) pairs of motion_fused_kernel
+        const int E = t.validity_lanes_per_state, W = t.motion_lanes_per_state;
+        const bool ok = (E == 8 && W <= 4) || (E == 4 && W <= 2) || (E == 2 && W == 1) || (E == 1 && W == 1);
+        OA_REQUIRE(ok, OMPL_AMD_ERR_ARG, "fused strategy: (endpoint, walk) lanes (%d, %d) not built", E, W);
+    }
     std::lock_guard<std::mutex> lk(g_tuning_mutex);
     g_tuning = t;
 }
@@ -300,7 +307,7 @@ int ompl_amd_get_tuning(ompl_amd_tuning *t)
         t->motion_lanes_per_state = x.motion_lanes_per_state;
         t->motion_block = x.motion_block;
         t->motion_waves_per_cu = x.motion_waves_per_cu;
-        t->motion_two_phase = x.motion_two_phase;
+        t->motion_strategy = x.motion_strategy;
     });
 }
 
@@ -309,7 +316,7 @@ int ompl_amd_set_tuning(const ompl_amd_tuning *t)
     return guarded([&] {
         OA_REQUIRE(t, OMPL_AMD_ERR_ARG, "null tuning");
         set_tuning(Tuning{t->validity_lanes_per_state, t->validity_block, t->motion_lanes_per_state,
-                          t->motion_block, t->motion_waves_per_cu, t->motion_two_phase ? 1 : 0});
+                          t->motion_block, t->motion_waves_per_cu, t->motion_strategy});
     });
 }
 

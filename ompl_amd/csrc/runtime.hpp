@@ -36,7 +36,7 @@ struct Tuning {
     int motion_lanes_per_state;
     int motion_block;
     int motion_waves_per_cu;
-    int motion_two_phase;
+    int motion_strategy;
 };
 Tuning get_tuning();
 void set_tuning(const Tuning &t);

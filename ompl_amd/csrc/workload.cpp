@@ -55,23 +55,50 @@ int ompl_amd_workload_obstacles(uint32_t seed, int n_spheres, int n_boxes, doubl
                    "null output");
         OA_REQUIRE(min_base_dist < 1.5, OMPL_AMD_ERR_ARG, "min_base_dist leaves no room in [-1,1]^3");
         LocalRng rng(seed);
-        auto centre = [&](double c[3]) {
-            do {
-                for (int k = 0; k < 3; ++k) c[k] = rng.uniformReal(-1.0, 1.0);
-            } while (std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) < min_base_dist);
+        // The base link (link 0, joint axis z) never moves: its spheres sit at
+        // (0, 0, 0.3 f_j).  An obstacle touching them would make every state
+        // invalid, so "outside 0.2 m of the base" rejects a draw whose centre is
+        // within min_base_dist of the origin or which overlaps a base-link sphere
+        // (with a 1 mm margin for the FP32 evaluation).
+        const double base_z[4] = {0.0375, 0.1125, 0.1875, 0.2625};
+        const double rr = 0.05 + 1e-3;
+        auto near_base = [&](const double c[3]) {
+            return std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) < min_base_dist;
         };
         for (int i = 0; i < n_spheres; ++i) {
-            double c[3];
-            centre(c);
-            double r = rng.uniformReal(r_lo, r_hi);
+            double c[3], r;
+            bool bad;
+            do {
+                for (int k = 0; k < 3; ++k) c[k] = rng.uniformReal(-1.0, 1.0);
+                r = rng.uniformReal(r_lo, r_hi);
+                bad = near_base(c);
+                for (double z : base_z) {
+                    const double d = std::sqrt(c[0] * c[0] + c[1] * c[1] + (c[2] - z) * (c[2] - z));
+                    bad = bad || d < r + rr;
+                }
+            } while (bad);
             for (int k = 0; k < 3; ++k) spheres_out[4 * i + k] = (float)c[k];
             spheres_out[4 * i + 3] = (float)r;
         }
         for (int i = 0; i < n_boxes; ++i) {
-            double c[3];
-            centre(c);
+            double c[3], h[3];
+            bool bad;
+            do {
+                for (int k = 0; k < 3; ++k) c[k] = rng.uniformReal(-1.0, 1.0);
+                for (int k = 0; k < 3; ++k) h[k] = rng.uniformReal(h_lo, h_hi);
+                bad = near_base(c);
+                for (double z : base_z) {
+                    const double p[3] = {0.0, 0.0, z};
+                    double d2 = 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        const double q = std::min(std::max(p[k], c[k] - h[k]), c[k] + h[k]);
+                        d2 += (p[k] - q) * (p[k] - q);
+                    }
+                    bad = bad || std::sqrt(d2) < rr;
+                }
+            } while (bad);
             for (int k = 0; k < 3; ++k) boxes_out[6 * i + k] = (float)c[k];
-            for (int k = 0; k < 3; ++k) boxes_out[6 * i + 3 + k] = (float)rng.uniformReal(h_lo, h_hi);
+            for (int k = 0; k < 3; ++k) boxes_out[6 * i + 3 + k] = (float)h[k];
         }
     });
 }

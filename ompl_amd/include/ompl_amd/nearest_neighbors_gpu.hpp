@@ -1,0 +1,187 @@
+// NearestNeighborsGPU<_T>: an ompl::NearestNeighbors<_T> (datastructures/
+// NearestNeighbors.h:46-115 of the reference) backed by the exact GPU index of
+// libompl_amd.  Drop-in beside NearestNeighborsLinear and NearestNeighborsGNAT:
+//
+//   auto nn = std::make_shared<ompl_amd::NearestNeighborsGPU<Motion *>>(
+//       OMPL_AMD_METRIC_L2, 7, [](Motion *const &m, float *out) { ... 7 coords ... });
+//   nn->setDistanceFunction(...);   // kept for API compatibility; the metric
+//                                   // descriptor above is what the GPU evaluates
+//
+// Semantics follow NearestNeighborsLinear (NearestNeighborsLinear.h:98-142):
+// sorted results, inclusive radius, nearest() throws ompl::Exception when
+// empty, remove() drops the last element equal to the argument.  Ties in
+// distance are ordered by insertion order.
+#pragma once
+
+#include <ompl/datastructures/NearestNeighbors.h>
+#include <ompl/util/Exception.h>
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../../include/ompl_amd.h"
+
+namespace ompl_amd {
+
+inline void throw_on_error(int rc)
+{
+    if (rc != OMPL_AMD_OK) throw ompl::Exception(std::string("ompl_amd: ") + ompl_amd_last_error());
+}
+
+template <typename _T>
+class NearestNeighborsGPU : public ompl::NearestNeighbors<_T>
+{
+public:
+    using Coordinates = std::function<void(const _T &, float *)>;
+
+    NearestNeighborsGPU(int metric, int dim, Coordinates coords) : metric_(metric), dim_(dim), coords_(std::move(coords))
+    {
+        throw_on_error(ompl_amd_nn_create(metric, dim, &nn_));
+    }
+
+    ~NearestNeighborsGPU() override
+    {
+        ompl_amd_nn_destroy(nn_);
+    }
+
+    NearestNeighborsGPU(const NearestNeighborsGPU &) = delete;
+    NearestNeighborsGPU &operator=(const NearestNeighborsGPU &) = delete;
+
+    bool reportsSortedResults() const override
+    {
+        return true;
+    }
+
+    void clear() override
+    {
+        throw_on_error(ompl_amd_nn_clear(nn_));
+        data_.clear();
+        alive_.clear();
+    }
+
+    void add(const _T &data) override
+    {
+        std::vector<float> c(dim_);
+        coords_(data, c.data());
+        int64_t first = 0;
+        throw_on_error(ompl_amd_nn_add(nn_, c.data(), 1, &first));
+        data_.push_back(data);
+        alive_.push_back(1);
+    }
+
+    void add(const std::vector<_T> &data) override
+    {
+        if (data.empty()) return;
+        std::vector<float> c(data.size() * dim_);
+        for (std::size_t i = 0; i < data.size(); ++i) coords_(data[i], c.data() + i * dim_);
+        int64_t first = 0;
+        throw_on_error(ompl_amd_nn_add(nn_, c.data(), (int64_t)data.size(), &first));
+        data_.insert(data_.end(), data.begin(), data.end());
+        alive_.insert(alive_.end(), data.size(), 1);
+    }
+
+    bool remove(const _T &data) override
+    {
+        for (int64_t i = (int64_t)data_.size() - 1; i >= 0; --i)
+            if (alive_[i] && data_[i] == data) {
+                int ok = 0;
+                throw_on_error(ompl_amd_nn_remove(nn_, i, &ok));
+                alive_[i] = 0;
+                return ok != 0;
+            }
+        return false;
+    }
+
+    _T nearest(const _T &data) const override
+    {
+        std::vector<float> c(dim_);
+        coords_(data, c.data());
+        int64_t id = -1;
+        int rc = ompl_amd_nn_nearest(nn_, c.data(), &id);
+        if (rc == OMPL_AMD_ERR_EMPTY) throw ompl::Exception("No elements found in nearest neighbors data structure");
+        throw_on_error(rc);
+        return data_[(std::size_t)id];
+    }
+
+    void nearestK(const _T &data, std::size_t k, std::vector<_T> &nbh) const override
+    {
+        nbh.clear();
+        if (k == 0) return;
+        std::vector<std::vector<_T>> out;
+        nearestKBatch(std::vector<_T>{data}, k, out);
+        nbh = std::move(out[0]);
+    }
+
+    void nearestR(const _T &data, double radius, std::vector<_T> &nbh) const override
+    {
+        std::vector<std::vector<_T>> out;
+        nearestRBatch(std::vector<_T>{data}, radius, out);
+        nbh = std::move(out[0]);
+    }
+
+    std::size_t size() const override
+    {
+        int64_t n = 0;
+        throw_on_error(ompl_amd_nn_size(nn_, &n));
+        return (std::size_t)n;
+    }
+
+    void list(std::vector<_T> &data) const override
+    {
+        data.clear();
+        for (std::size_t i = 0; i < data_.size(); ++i)
+            if (alive_[i]) data.push_back(data_[i]);
+    }
+
+    // ---- batched forms: one launch for many queries
+    void nearestKBatch(const std::vector<_T> &queries, std::size_t k, std::vector<std::vector<_T>> &out) const
+    {
+        out.assign(queries.size(), {});
+        if (queries.empty() || k == 0) return;
+        const std::size_t kk = k > 64 ? 64 : k;
+        std::vector<float> q(queries.size() * dim_);
+        for (std::size_t i = 0; i < queries.size(); ++i) coords_(queries[i], q.data() + i * dim_);
+        std::vector<int32_t> idx(queries.size() * kk);
+        std::vector<float> dist(queries.size() * kk);
+        throw_on_error(ompl_amd_nn_knn(nn_, q.data(), (int64_t)queries.size(), (int)kk, idx.data(), dist.data()));
+        for (std::size_t i = 0; i < queries.size(); ++i)
+            for (std::size_t j = 0; j < kk; ++j) {
+                const int32_t id = idx[i * kk + j];
+                if (id >= 0) out[i].push_back(data_[(std::size_t)id]);
+            }
+    }
+
+    void nearestRBatch(const std::vector<_T> &queries, double radius, std::vector<std::vector<_T>> &out) const
+    {
+        out.assign(queries.size(), {});
+        if (queries.empty()) return;
+        std::vector<float> q(queries.size() * dim_);
+        for (std::size_t i = 0; i < queries.size(); ++i) coords_(queries[i], q.data() + i * dim_);
+        std::vector<int64_t> off(queries.size() + 1);
+        int64_t total = 0;
+        throw_on_error(ompl_amd_nn_radius(nn_, q.data(), (int64_t)queries.size(), (float)radius, off.data(), nullptr,
+                                          nullptr, 0, &total));
+        std::vector<int32_t> idx((std::size_t)total);
+        std::vector<float> dist((std::size_t)total);
+        throw_on_error(ompl_amd_nn_radius(nn_, q.data(), (int64_t)queries.size(), (float)radius, off.data(),
+                                          idx.data(), dist.data(), total, &total));
+        for (std::size_t i = 0; i < queries.size(); ++i)
+            for (int64_t j = off[i]; j < off[i + 1]; ++j) out[i].push_back(data_[(std::size_t)idx[(std::size_t)j]]);
+    }
+
+    ompl_amd_nn handle() const
+    {
+        return nn_;
+    }
+
+private:
+    int metric_;
+    int dim_;
+    Coordinates coords_;
+    ompl_amd_nn nn_ = nullptr;
+    std::vector<_T> data_;
+    std::vector<uint8_t> alive_;
+};
+
+} // namespace ompl_amd

@@ -492,3 +492,116 @@ long orc_nearest(int metric, const float *data, long n, int dim, const float *qu
     }
     return pos;
 }
+
+/* ======================================================================
+ * Batch drivers over POSIX threads (CPU baseline timing)
+ * ====================================================================== */
+#include <pthread.h>
+
+typedef struct {
+    int kind;
+    const orc_env *env;
+    const double *a, *b;
+    const float *q;
+    long n, lo, hi;
+    double lvs;
+    unsigned factor;
+    uint8_t *out;
+    int32_t *checked;
+    int metric, dim, k;
+    const float *data;
+    long ndata;
+    int *idx;
+    float *dist;
+} orc_job;
+
+static void *orc_worker(void *p)
+{
+    orc_job *j = (orc_job *)p;
+    for (long i = j->lo; i < j->hi; ++i) {
+        if (j->kind == 0) {
+            int c = 0;
+            j->out[i] = (uint8_t)orc_check_motion(j->env, j->a + i * ORC_ARM_DOF, j->b + i * ORC_ARM_DOF, j->lvs,
+                                                  j->factor, &c);
+            if (j->checked) j->checked[i] = c;
+        } else if (j->kind == 1) {
+            float q[ORC_ARM_DOF];
+            for (int d = 0; d < ORC_ARM_DOF; ++d) q[d] = j->q[(long)d * j->n + i];
+            j->out[i] = (uint8_t)orc_state_valid(j->env, q, NULL);
+        } else {
+            orc_knn(j->metric, j->data, j->ndata, j->dim, j->q + i * j->dim, j->k, j->idx + i * j->k,
+                    j->dist + i * j->k);
+        }
+    }
+    return NULL;
+}
+
+static void orc_run(orc_job proto, long n, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    orc_job jobs[256];
+    long chunk = (n + nthreads - 1) / nthreads;
+    int started = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        jobs[t] = proto;
+        jobs[t].lo = t * chunk;
+        jobs[t].hi = (t + 1) * chunk < n ? (t + 1) * chunk : n;
+        if (jobs[t].lo >= jobs[t].hi) break;
+        if (nthreads == 1) {
+            orc_worker(&jobs[t]);
+        } else {
+            pthread_create(&th[t], NULL, orc_worker, &jobs[t]);
+            ++started;
+        }
+    }
+    for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+}
+
+void orc_check_motions_batch(const orc_env *env, const double *a_rows, const double *b_rows, long n, double lvs,
+                             unsigned factor, uint8_t *out, int32_t *checked, int nthreads)
+{
+    orc_job p;
+    memset(&p, 0, sizeof(p));
+    p.kind = 0;
+    p.env = env;
+    p.a = a_rows;
+    p.b = b_rows;
+    p.n = n;
+    p.lvs = lvs;
+    p.factor = factor;
+    p.out = out;
+    p.checked = checked;
+    orc_run(p, n, nthreads);
+}
+
+void orc_validate_states_batch(const orc_env *env, const float *q_soa, long n, uint8_t *out, int nthreads)
+{
+    orc_job p;
+    memset(&p, 0, sizeof(p));
+    p.kind = 1;
+    p.env = env;
+    p.q = q_soa;
+    p.n = n;
+    p.out = out;
+    orc_run(p, n, nthreads);
+}
+
+void orc_knn_batch(int metric, const float *data, long n, int dim, const float *queries, long nq, int k,
+                   int *out_idx, float *out_dist, int nthreads)
+{
+    orc_job p;
+    memset(&p, 0, sizeof(p));
+    p.kind = 2;
+    p.metric = metric;
+    p.data = data;
+    p.ndata = n;
+    p.dim = dim;
+    p.q = queries;
+    p.n = nq;
+    p.k = k;
+    p.idx = out_idx;
+    p.dist = out_dist;
+    orc_run(p, nq, nthreads);
+}

@@ -107,6 +107,15 @@ long orc_radius(int metric, const float *data, long n, int dim, const float *que
 /* First strict minimum, as NearestNeighborsLinear::nearest; -1 if empty. */
 long orc_nearest(int metric, const float *data, long n, int dim, const float *query);
 
+/* ---- batch drivers (CPU baseline timing): independent edges / queries split
+ * over nthreads POSIX threads, as a thread-safe checker allows
+ * (base/StateValidityChecker.h:87-89). */
+void orc_check_motions_batch(const orc_env *env, const double *a_rows, const double *b_rows, long n, double lvs,
+                             unsigned factor, uint8_t *out, int32_t *checked, int nthreads);
+void orc_validate_states_batch(const orc_env *env, const float *q_soa, long n, uint8_t *out, int nthreads);
+void orc_knn_batch(int metric, const float *data, long n, int dim, const float *queries, long nq, int k,
+                   int *out_idx, float *out_dist, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,10 @@ def lib():
         L.orc_radius.restype = C.c_long
         L.orc_nearest.argtypes = [C.c_int, vp, C.c_long, C.c_int, vp]
         L.orc_nearest.restype = C.c_long
+        L.orc_check_motions_batch.argtypes = [C.POINTER(Env), vp, vp, C.c_long, C.c_double, C.c_uint, vp, vp,
+                                              C.c_int]
+        L.orc_validate_states_batch.argtypes = [C.POINTER(Env), vp, C.c_long, vp, C.c_int]
+        L.orc_knn_batch.argtypes = [C.c_int, vp, C.c_long, C.c_int, vp, C.c_long, C.c_int, vp, vp, C.c_int]
         _lib = L
     return _lib
 
@@ -169,16 +173,33 @@ def check_motion_last_valid(env: OracleEnv, a, b, lvs, factor=1, last_t=float("n
     return bool(v), t.value, cnt.value
 
 
-def check_motions(env: OracleEnv, a_rows: np.ndarray, b_rows: np.ndarray, lvs, factor=1):
+def check_motions(env: OracleEnv, a_rows: np.ndarray, b_rows: np.ndarray, lvs, factor=1, threads=1):
     """Row-major [n][7] endpoints -> (valid u8[n], checked i32[n])."""
-    n = a_rows.shape[0]
+    a = np.ascontiguousarray(a_rows, np.float64)
+    b = np.ascontiguousarray(b_rows, np.float64)
+    n = a.shape[0]
     out = np.zeros(n, np.uint8)
     chk = np.zeros(n, np.int32)
-    for i in range(n):
-        v, c = check_motion(env, a_rows[i], b_rows[i], lvs, factor)
-        out[i] = v
-        chk[i] = c
+    lib().orc_check_motions_batch(env.ref(), _p(a), _p(b), n, lvs, factor, _p(out), _p(chk), threads)
     return out, chk
+
+
+def validate_states_mt(env: OracleEnv, q_soa: np.ndarray, threads: int) -> np.ndarray:
+    qa = np.ascontiguousarray(q_soa, np.float32)
+    n = qa.shape[1]
+    out = np.zeros(n, np.uint8)
+    lib().orc_validate_states_batch(env.ref(), _p(qa), n, _p(out), threads)
+    return out
+
+
+def knn_mt(metric: int, data: np.ndarray, queries: np.ndarray, k: int, threads: int):
+    data = np.ascontiguousarray(data, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    idx = np.zeros((queries.shape[0], k), np.int32)
+    dist = np.zeros((queries.shape[0], k), np.float32)
+    lib().orc_knn_batch(metric, _p(data), data.shape[0], data.shape[1], _p(queries), queries.shape[0], k, _p(idx),
+                        _p(dist), threads)
+    return idx, dist
 
 
 class Mt19937:

@@ -1,0 +1,73 @@
+// Compile-and-run check of ompl_amd::NearestNeighborsGPU<_T> against the
+// reference's own ompl::NearestNeighbors<_T> interface header.  Without a GPU
+// it only exercises construction failure paths; with one it compares the
+// adapter with NearestNeighborsLinear on a small set.
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <vector>
+
+#include <ompl/datastructures/NearestNeighborsLinear.h>
+
+#include "ompl_amd/nearest_neighbors_gpu.hpp"
+
+struct Pt {
+    float c[3];
+    int id;
+    bool operator==(const Pt &o) const { return id == o.id; }
+};
+
+int main(int argc, char **argv)
+{
+    const bool run_gpu = argc > 1;
+    std::unique_ptr<ompl::NearestNeighbors<Pt>> base;
+    try {
+        base = std::make_unique<ompl_amd::NearestNeighborsGPU<Pt>>(OMPL_AMD_METRIC_L2, 3, [](const Pt &p, float *o) {
+            o[0] = p.c[0];
+            o[1] = p.c[1];
+            o[2] = p.c[2];
+        });
+    } catch (const ompl::Exception &e) {
+        std::printf("construction failed as expected without a device: %s\n", e.what());
+        return run_gpu ? 1 : 0;
+    }
+    if (!run_gpu) return 0;
+    auto dist = [](const Pt &a, const Pt &b) {
+        double s = 0;
+        for (int i = 0; i < 3; ++i) s += ((double)a.c[i] - b.c[i]) * ((double)a.c[i] - b.c[i]);
+        return std::sqrt(s);
+    };
+    base->setDistanceFunction(dist);
+    ompl::NearestNeighborsLinear<Pt> lin;
+    lin.setDistanceFunction(dist);
+    std::vector<Pt> pts;
+    unsigned s = 12345;
+    for (int i = 0; i < 500; ++i) {
+        Pt p;
+        for (float &v : p.c) {
+            s = s * 1664525u + 1013904223u;
+            v = (float)((s >> 8) % 10000) / 1000.0f;
+        }
+        p.id = i;
+        pts.push_back(p);
+    }
+    base->add(pts);
+    lin.add(pts);
+    int bad = 0;
+    for (int q = 0; q < 50; ++q) {
+        const Pt &qp = pts[(size_t)q * 7];
+        std::vector<Pt> a, b;
+        base->nearestK(qp, 8, a);
+        lin.nearestK(qp, 8, b);
+        for (size_t j = 0; j < a.size(); ++j)
+            if (std::fabs(dist(a[j], qp) - dist(b[j], qp)) > 1e-5) ++bad;
+        base->nearestR(qp, 1.0, a);
+        lin.nearestR(qp, 1.0, b);
+        if (a.size() != b.size()) ++bad;
+        if (base->nearest(qp).id != lin.nearest(qp).id) ++bad;
+    }
+    base->remove(pts[3]);
+    if (base->size() != 499) ++bad;
+    std::printf("adapter check: %d mismatches\n", bad);
+    return bad ? 1 : 0;
+}

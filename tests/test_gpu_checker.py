@@ -79,12 +79,14 @@ def motion_case(gpu, seed, n, mu=-1.0, sigma=1.0, lo=0.05, hi=5.0):
     return a, b
 
 
-@pytest.mark.parametrize("two_phase", [1, 0])
-@pytest.mark.parametrize("lps", [1, 4, 8])
-def test_check_motions_config2_bit_exact(gpu, oracle, two_phase, lps):
+STRATEGIES = [(1, 8, 1), (1, 4, 4), (0, 8, 1), (0, 8, 8), (2, 8, 1), (2, 8, 4), (2, 4, 2), (2, 1, 1)]
+
+
+@pytest.mark.parametrize("strategy,lps_end,lps_walk", STRATEGIES)
+def test_check_motions_config2_bit_exact(gpu, oracle, strategy, lps_end, lps_walk):
     """Config: 16,384 edges, 128 spheres + 128 boxes, 0.01 rad resolution."""
-    print(f"motions two_phase={two_phase} lps={lps}", flush=True)
-    gpu.set_tuning(motion_two_phase=two_phase, motion_lanes_per_state=lps, validity_lanes_per_state=lps)
+    print(f"motions strategy={strategy} lanes=({lps_end},{lps_walk})", flush=True)
+    gpu.set_tuning(motion_strategy=strategy, motion_lanes_per_state=lps_walk, validity_lanes_per_state=lps_end)
     env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
     a, b = motion_case(gpu, 4, 16384)
     lvs = lvs_config()
@@ -95,13 +97,13 @@ def test_check_motions_config2_bit_exact(gpu, oracle, two_phase, lps):
     assert np.array_equal(c.cpu().numpy(), ref_c)
     v2 = gpu.check_motions(env, A, B, lvs)  # no count: early exit at any hit
     assert np.array_equal(v2.cpu().numpy(), ref_v)
-    gpu.set_tuning(motion_two_phase=1, motion_lanes_per_state=1, validity_lanes_per_state=8)
+    gpu.set_tuning(motion_strategy=1, motion_lanes_per_state=1, validity_lanes_per_state=8)
 
 
-@pytest.mark.parametrize("two_phase", [1, 0])
-def test_check_motions_sparse_env_deep_walks(gpu, oracle, two_phase):
+@pytest.mark.parametrize("strategy,lps_end,lps_walk", STRATEGIES)
+def test_check_motions_sparse_env_deep_walks(gpu, oracle, strategy, lps_end, lps_walk):
     """Few obstacles: most edges are valid, so every bisection position runs."""
-    gpu.set_tuning(motion_two_phase=two_phase)
+    gpu.set_tuning(motion_strategy=strategy, motion_lanes_per_state=lps_walk, validity_lanes_per_state=lps_end)
     env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
     a, b = motion_case(gpu, 9, 3000, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
     lvs = lvs_config()
@@ -110,7 +112,7 @@ def test_check_motions_sparse_env_deep_walks(gpu, oracle, two_phase):
     v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, exact_count=True)
     assert np.array_equal(v.cpu().numpy(), ref_v)
     assert np.array_equal(c.cpu().numpy(), ref_c)
-    gpu.set_tuning(motion_two_phase=1)
+    gpu.set_tuning(motion_strategy=1, motion_lanes_per_state=1, validity_lanes_per_state=8)
 
 
 def test_check_motions_edge_cases(gpu, oracle):
@@ -124,12 +126,12 @@ def test_check_motions_edge_cases(gpu, oracle):
     lvs = lvs_config()
     for factor in (1, 2):
         ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs, factor)
-        for tp in (1, 0):
-            gpu.set_tuning(motion_two_phase=tp)
+        for tp in (1, 0, 2):
+            gpu.set_tuning(motion_strategy=tp)
             v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, factor, exact_count=True)
             assert np.array_equal(v.cpu().numpy(), ref_v)
             assert np.array_equal(c.cpu().numpy(), ref_c)
-    gpu.set_tuning(motion_two_phase=1)
+    gpu.set_tuning(motion_strategy=1)
 
 
 def test_check_motion_plain_single_call(gpu, oracle):
