@@ -110,18 +110,26 @@ int ompl_amd_check_motion_last_valid(ompl_amd_env env, const double *a, const do
                                      double longest_valid_segment, unsigned factor, int *valid, double *last_t);
 
 /* Launch-shape knobs used by the batched kernels (set by the autotuner).
- * lanes_per_state in {1,2,4,8}; block in {64,128,256,512,1024}. */
+ * A state is checked by lanes_per_state x obstacle_parts lanes: the first
+ * splits the arm's 28 spheres, the second the obstacle list.  Built shapes
+ * (lanes, parts): (1,1) (2,1) (4,1) (8,1) (8,2) (8,4) (8,8); block in
+ * {64,128,256,512,1024}. */
 typedef struct {
-    int validity_lanes_per_state;
+    int validity_lanes_per_state; /* validity and s2 (endpoint) checks */
+    int validity_obstacle_parts;
     int validity_block;
-    int motion_lanes_per_state;
+    int motion_lanes_per_state;   /* interior (bisection) checks */
+    int motion_obstacle_parts;
     int motion_block;
-    int motion_waves_per_cu; /* persistent interior kernel occupancy */
-    int motion_strategy;     /* 0: one wave walks each edge from s2;
-                                1: s2 pass over all edges, then a persistent
-                                   queue walks the survivors (3 launches);
-                                2: fused: each wave checks s2 of its edges and
-                                   walks its own survivors (1 launch) */
+    int motion_waves_per_cu;      /* persistent kernels' occupancy */
+    int motion_strategy;          /* 0: one wave walks each edge from s2;
+                                     1: s2 pass, then a persistent queue walks
+                                        each survivor with one wave;
+                                     2: fused: each wave checks s2 of its edges
+                                        and walks its own survivors (1 launch);
+                                     3: s2 pass, then the survivors' bisection
+                                        chunks run in parallel across the GPU,
+                                        chunk-major, with early exit */
 } ompl_amd_tuning;
 int ompl_amd_get_tuning(ompl_amd_tuning *t);
 int ompl_amd_set_tuning(const ompl_amd_tuning *t);

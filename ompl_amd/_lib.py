@@ -84,8 +84,10 @@ class OmplAmdError(RuntimeError):
 class Tuning(C.Structure):
     _fields_ = [
         ("validity_lanes_per_state", C.c_int),
+        ("validity_obstacle_parts", C.c_int),
         ("validity_block", C.c_int),
         ("motion_lanes_per_state", C.c_int),
+        ("motion_obstacle_parts", C.c_int),
         ("motion_block", C.c_int),
         ("motion_waves_per_cu", C.c_int),
         ("motion_strategy", C.c_int),

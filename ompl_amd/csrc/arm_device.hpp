@@ -179,16 +179,38 @@ __device__ __forceinline__ uint64_t fold_groups(uint64_t m)
 //               of the first invalid state in check order).
 enum class Exit { kAllGroups, kAnyGroup, kLowestGroup };
 
-// Checks the lane's spheres against the LDS environment.  `live` lanes take
-// part; dead lanes count as resolved.  Returns this lane's hit flag (the
-// group result is the OR over its lanes).
-template <int LPS, Exit E>
-__device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live)
+// Lane roles inside a state group of LPS*OPS lanes: the lane holds sphere slot
+// `slot` (LaneSpheres<LPS>) and visits obstacle partition `part` (obstacles
+// part, part + OPS, part + 2*OPS, ...).
+template <int LPS, int OPS>
+struct GroupShape {
+    static constexpr int GS = LPS * OPS;    // lanes per state
+    static constexpr int STATES = 64 / GS;  // states per wave
+    int slot, part, group;
+    __device__ __forceinline__ explicit GroupShape(int lane)
+    {
+        const int sub = lane % GS;
+        slot = sub % LPS;
+        part = sub / LPS;
+        group = lane / GS;
+    }
+};
+
+// Checks the lane's spheres against its obstacle partition of the LDS
+// environment.  `live` lanes take part; dead lanes count as resolved.  The
+// obstacle loop runs a wave-uniform trip count (two obstacles of the lane's
+// partition per iteration, indices clamped to the last obstacle, whose
+// repeated test is harmless) and ends each iteration with a ballot.  Returns
+// this lane's hit flag (the state's result is the OR over its group).
+template <int LPS, int OPS, Exit E>
+__device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live,
+                                            int part)
 {
+    constexpr int GS = LPS * OPS;
     bool hit = false;
-    const uint64_t live_leaders = fold_groups<LPS>(wave_ballot(live));
+    const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));
     auto done = [&](bool h) -> bool {
-        const uint64_t H = fold_groups<LPS>(wave_ballot(h && live));
+        const uint64_t H = fold_groups<GS>(wave_ballot(h && live));
         if (E == Exit::kAllGroups) return (H & live_leaders) == live_leaders;
         if (E == Exit::kAnyGroup) return H != 0;
         if (H == 0) return false;
@@ -196,13 +218,18 @@ __device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const flo
         return ((low - 1) & live_leaders) == 0;
     };
     if (live_leaders == 0) return hit;
-    for (int o = 0; o < ns; ++o) {
-        hit |= hits_sphere<LPS>(S, lds[o]);
+    for (int ob = 0; ob < ns; ob += 2 * OPS) {
+        const int o0 = min(ob + part, ns - 1), o1 = min(ob + OPS + part, ns - 1);
+        const float4 s0 = lds[o0], s1 = lds[o1];
+        hit |= hits_sphere<LPS>(S, s0) | hits_sphere<LPS>(S, s1);
         if (done(hit)) return hit;
     }
     const float4 *boxes = lds + ns;
-    for (int o = 0; o < nb; ++o) {
-        hit |= hits_box<LPS>(S, boxes[2 * o], boxes[2 * o + 1]);
+    for (int ob = 0; ob < nb; ob += 2 * OPS) {
+        const int o0 = min(ob + part, nb - 1), o1 = min(ob + OPS + part, nb - 1);
+        const float4 l0 = boxes[2 * o0], h0 = boxes[2 * o0 + 1];
+        const float4 l1 = boxes[2 * o1], h1 = boxes[2 * o1 + 1];
+        hit |= hits_box<LPS>(S, l0, h0) | hits_box<LPS>(S, l1, h1);
         if (done(hit)) return hit;
     }
     return hit;

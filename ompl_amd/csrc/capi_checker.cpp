@@ -33,22 +33,35 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
     A.out = out;
     A.checked = checked;
     const bool exact = checked != nullptr;
-    if (t.motion_strategy == 2) {
+    const int LE = t.validity_lanes_per_state, OE = t.validity_obstacle_parts;
+    const int LW = t.motion_lanes_per_state, OW = t.motion_obstacle_parts;
+    const int64_t persistent_waves = (int64_t)num_cus() * t.motion_waves_per_cu;
+    if (t.motion_strategy == 3) {
+        Workspace &w = workspace(s, (size_t)n);
+        A.list = w.list;
+        A.list_count = w.counters;
+        A.max_m = w.counters + 2;
+        A.first_bad = w.first_bad;
         A.start = 1;
-        launch_motion_fused(A, exact, t.validity_lanes_per_state, t.motion_lanes_per_state, t.motion_block, s);
+        launch_zero(w.counters, 4, s);
+        launch_motion_endpoint(A, LE, OE, t.validity_block, s);
+        launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s);
+        launch_motion_finalize(A, n, s);
+    } else if (t.motion_strategy == 2) {
+        A.start = 1;
+        launch_motion_fused(A, exact, LE, OE, LW, OW, t.motion_block, s);
     } else if (t.motion_strategy == 1) {
         Workspace &w = workspace(s, (size_t)n);
         A.list = w.list;
         A.list_count = w.counters;
         A.queue = w.counters + 1;
         A.start = 1;
-        launch_zero(w.counters, 2, s);
-        launch_motion_endpoint(A, t.validity_lanes_per_state, t.validity_block, s);
-        const int64_t waves = (int64_t)num_cus() * t.motion_waves_per_cu;
-        launch_motion_walk(A, exact, t.motion_lanes_per_state, t.motion_block, waves, s);
+        launch_zero(w.counters, 4, s);
+        launch_motion_endpoint(A, LE, OE, t.validity_block, s);
+        launch_motion_walk(A, exact, LW, OW, t.motion_block, persistent_waves, s);
     } else {
         A.start = 0;
-        launch_motion_walk(A, exact, t.motion_lanes_per_state, t.motion_block, n, s);
+        launch_motion_walk(A, exact, LW, OW, t.motion_block, n, s);
     }
 }
 
@@ -61,20 +74,22 @@ void check_motions(const Env &env, const double *a, const double *b, int64_t n, 
 // Candidate launch shapes of the motion autotuner.
 std::vector<Tuning> motion_candidates(const Tuning &base)
 {
+    (void)base;
     std::vector<Tuning> c;
-    const int ends[] = {8, 4, 2, 1};
-    for (int E : ends)
-        for (int W : {1, 2, 4})
-            for (int block : {64, 256, 1024}) {
-                // fused: one launch
-                const bool built = (E == 8 && W <= 4) || (E == 4 && W <= 2) || (E == 2 && W == 1) || (E == 1 && W == 1);
-                if (built) c.push_back(Tuning{E, block, W, block, base.motion_waves_per_cu, 2});
-                // two-phase queue
-                for (int wpc : {4, 8, 16})
-                    if (block == 256) c.push_back(Tuning{E, block, W, 256, wpc, 1});
-            }
-    for (int W : {1, 2, 4, 8})
-        for (int block : {64, 256}) c.push_back(Tuning{base.validity_lanes_per_state, 256, W, block, 8, 0});
+    const int shapes[][2] = {{8, 1}, {8, 2}, {8, 4}, {8, 8}, {4, 1}, {1, 1}};
+    for (const auto &e : shapes)
+        for (const auto &w : shapes) {
+            const int LE = e[0], OE = e[1], LW = w[0], OW = w[1];
+            // chunk-parallel walk of the survivors
+            for (int wpc : {8, 16, 32})
+                c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 3});
+            // fused one-launch form
+            if (fused_supported(LE, OE, LW, OW))
+                for (int block : {64, 256}) c.push_back(Tuning{LE, OE, block, LW, OW, block, 8, 2});
+            // persistent queue, one wave per survivor
+            c.push_back(Tuning{LE, OE, 256, LW, OW, 256, 16, 1});
+        }
+    for (const auto &w : shapes) c.push_back(Tuning{8, 1, 256, w[0], w[1], 256, 8, 0});
     return c;
 }
 
@@ -144,7 +159,8 @@ int ompl_amd_validate_states_dev(ompl_amd_env e, const float *q, int64_t n, uint
         if (n == 0) return;
         OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
         const Tuning t = get_tuning();
-        launch_validate<float>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_block,
+        launch_validate<float>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_obstacle_parts,
+                               t.validity_block,
                                resolve_stream(stream));
     });
 }
@@ -157,7 +173,8 @@ int ompl_amd_validate_states_f64_dev(ompl_amd_env e, const double *q, int64_t n,
         if (n == 0) return;
         OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
         const Tuning t = get_tuning();
-        launch_validate<double>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_block,
+        launch_validate<double>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_obstacle_parts,
+                                t.validity_block,
                                 resolve_stream(stream));
     });
 }
@@ -210,7 +227,8 @@ int ompl_amd_check_motions_last_valid_dev(ompl_amd_env e, const double *a, const
         A.lvs = lvs;
         A.factor = factor;
         A.out = out;
-        launch_motion_last_valid(A, last_t, t.motion_lanes_per_state, t.motion_block, n, resolve_stream(stream));
+        launch_motion_last_valid(A, last_t, t.motion_lanes_per_state, t.motion_obstacle_parts, t.motion_block, n,
+                                 resolve_stream(stream));
     });
 }
 
@@ -258,7 +276,7 @@ int ompl_amd_check_motion_last_valid(ompl_amd_env e, const double *a, const doub
         A.lvs = lvs;
         A.factor = factor;
         A.out = dev_out;
-        launch_motion_last_valid(A, dev_pd + 2 * kDof, 1, 64, 1, r.stream);
+        launch_motion_last_valid(A, dev_pd + 2 * kDof, 1, 1, 64, 1, r.stream);
         OA_HIP(hipStreamSynchronize(r.stream));
         *valid = reinterpret_cast<uint8_t *>(r.pinned)[0];
         *last_t = pd[2 * kDof];
@@ -301,12 +319,7 @@ extern "C" int ompl_amd_autotune_motions(ompl_amd_env e, const double *a, const 
         OA_HIP(hipFree(out));
         set_tuning(win);
         if (best) {
-            best->validity_lanes_per_state = win.validity_lanes_per_state;
-            best->validity_block = win.validity_block;
-            best->motion_lanes_per_state = win.motion_lanes_per_state;
-            best->motion_block = win.motion_block;
-            best->motion_waves_per_cu = win.motion_waves_per_cu;
-            best->motion_strategy = win.motion_strategy;
+            *best = tuning_to_c(win);
         }
         if (best_us) *best_us = 1e3 * (double)win_ms / reps;
     });

@@ -1,21 +1,27 @@
 // Batched state validity and motion validation kernels for gfx950.
 //
-// Mapping (wave64): a state is checked by a group of LPS lanes (LPS in
-// {1,2,4,8}); each lane of the group holds 28/LPS of the arm's sphere centres
-// (FK is recomputed per lane: ~0.6k FP32 ops against >= 10k for collision at
-// the benchmark sizes).  Obstacles are staged once per block in LDS and read
-// as wave-uniform float4 broadcasts; every obstacle iteration ends with a
-// ballot so a wave leaves the loop as soon as its exit policy is met.
+// Mapping (wave64): a state is checked by a group of LPS*OPS lanes: LPS lanes
+// split the arm's 28 sphere centres (28/LPS each; FK is recomputed per lane)
+// and OPS lane sets split the obstacles (partition p visits obstacles p,
+// p+OPS, ...).  Obstacles are staged once per block in LDS and read as float4
+// broadcasts; every obstacle iteration ends with a ballot so a wave leaves the
+// loop as soon as its exit policy is met.  Shapes (LPS, OPS) built:
+// (1,1) (2,1) (4,1) (8,1) (8,2) (8,4) (8,8).
 //
-//   validate_kernel       batched StateValidityChecker::isValid
-//   motion_endpoint_kernel  check s2 of every edge (DiscreteMotionValidator.cpp:96),
-//                         emit survivors (s2 valid and nd >= 2) to a list
-//   motion_walk_kernel    one wave per edge: positions 0 (s2) and 1..nd-1 (the
-//                         bisection order of DiscreteMotionValidator.cpp:106-134)
-//                         in chunks of 64/LPS, stop at the first chunk with a
-//                         collision; optional exact first-invalid position
-//   motion_single_kernel  the same walk for one edge passed by value (the plain
-//                         one-launch-per-call path)
+//   validate_kernel         batched StateValidityChecker::isValid
+//   motion_endpoint_kernel  s2 of every edge (DiscreteMotionValidator.cpp:96);
+//                           survivors (s2 valid, nd >= 2) go to a list
+//   motion_walk_kernel      one wave per edge: positions 0 (s2) and 1..nd-1 (the
+//                           bisection order of DiscreteMotionValidator.cpp:106-134)
+//                           in chunks, stop at the first chunk with a collision
+//   motion_chunk_kernel     chunk-parallel walk of the survivors: work item
+//                           (chunk b, survivor s) in chunk-major order over a
+//                           persistent grid; a chunk is skipped once an earlier
+//                           position of its edge is known invalid (atomicMin)
+//   motion_finalize_kernel  survivors' results from their first invalid position
+//   motion_fused_kernel     s2 of the wave's edges, then the wave walks its own
+//                           survivors (one launch)
+//   motion_single_kernel    one edge passed by value (the plain per-call path)
 //   motion_last_valid_kernel  the sequential lastValid overload (:48-91)
 #include <hip/hip_runtime.h>
 
@@ -25,91 +31,53 @@
 #include "arm_device.hpp"
 #include "checker.hpp"
 #include "motion_device.hpp"
+#include "walk_device.hpp"
 
 namespace ompl_amd {
 
-__device__ __forceinline__ int lane_id()
+// (LPS, OPS) shapes instantiated for every batched kernel.
+#define OA_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(8, 2) X(8, 4) X(8, 8)
+
+bool shape_supported(int lps, int ops)
 {
-    return threadIdx.x & 63;
+#define OA_SHAPE_OK(L, O)                                                                                           \
+    if (lps == L && ops == O) return true;
+    OA_SHAPES(OA_SHAPE_OK)
+#undef OA_SHAPE_OK
+    return false;
 }
 
-__device__ __forceinline__ int lowest_bit(uint64_t m)
-{
-    return __builtin_ctzll(m);
-}
-
-template <int LPS, typename T>
+template <int LPS, int OPS, typename T>
 __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__restrict__ q_soa, int64_t n,
                                                         uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
     stage_env(env, lds);
-    constexpr int G = 64 / LPS;
+    using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
-    const int g = lane % LPS;
+    const Shape sh(lane);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t st = wave * G + lane / LPS;
+    const int64_t st = wave * Shape::STATES + sh.group;
     const bool live = st < n;
     float q[kDof];
 #pragma unroll
-    for (int j = 0; j < kDof; ++j)
-        q[j] = live ? (float)q_soa[(int64_t)j * n + st] : 0.0f;
+    for (int j = 0; j < kDof; ++j) q[j] = live ? (float)q_soa[(int64_t)j * n + st] : 0.0f;
     LaneSpheres<LPS> S;
-    arm_fk<LPS>(q, g, S);
-    const bool hit = collide_env<LPS, Exit::kAllGroups>(S, lds, env.ns, env.nb, live);
-    const uint64_t H = fold_groups<LPS>(wave_ballot(hit && live));
-    if (live && g == 0) out[st] = ((H >> lane) & 1) ? 0 : 1;
+    arm_fk<LPS>(q, sh.slot, S);
+    const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, env.ns, env.nb, live, sh.part);
+    const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+    if (live && lane % Shape::GS == 0) out[st] = ((H >> lane) & 1) ? 0 : 1;
 }
 
-// Walks one edge from position `start` (0 = s2).  Returns the position of the
-// first invalid state found (INT_MAX if none) and the total position count.
-template <int LPS, bool EXACT>
-__device__ __forceinline__ int walk_edge(const double a[kDof], const double b[kDof], double lvs, unsigned factor,
-                                         const float4 *lds, int ns, int nb, int start, int &total)
+// s2 check of one edge per lane group; returns true for the group leader of a
+// survivor (s2 valid, 2 <= nd <= kMaxSegments) and writes the final result of
+// every other edge.
+template <int LPS, int OPS>
+__device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4 *lds, int64_t e, bool live, int &m)
 {
-    constexpr int G = 64 / LPS;
+    using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
-    const int g = lane % LPS;
-    const int grp = lane / LPS;
-    const int nd = valid_segment_count<kDof>(a, b, lvs, factor);
-    if (nd > kMaxSegments) {
-        total = 0;
-        return -1;
-    }
-    const int m = nd >= 2 ? nd - 1 : 0;
-    total = 1 + m;
-    for (int base = start; base < total; base += G) {
-        const int p = base + grp;
-        const bool live = p < total;
-        float q[kDof];
-        if (p == 0 || !live) {
-#pragma unroll
-            for (int j = 0; j < kDof; ++j) q[j] = (float)b[j];
-        } else {
-            const int idx = bisection_index(p - 1, nd);
-            const double t = (double)idx / (double)nd;
-            interpolate_f32<kDof>(a, b, t, q);
-        }
-        LaneSpheres<LPS> S;
-        arm_fk<LPS>(q, g, S);
-        const bool hit = collide_env<LPS, EXACT ? Exit::kLowestGroup : Exit::kAnyGroup>(S, lds, ns, nb, live);
-        const uint64_t H = fold_groups<LPS>(wave_ballot(hit && live));
-        if (H) return base + lowest_bit(H) / LPS;
-    }
-    return INT_MAX;
-}
-
-template <int LPS>
-__global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
-{
-    extern __shared__ float4 lds[];
-    stage_env(A.env, lds);
-    constexpr int G = 64 / LPS;
-    const int lane = lane_id();
-    const int g = lane % LPS;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t e = wave * G + lane / LPS;
-    const bool live = e < A.n;
+    const Shape sh(lane);
     double a[kDof], b[kDof];
     float q[kDof];
 #pragma unroll
@@ -119,28 +87,43 @@ __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
         q[j] = (float)b[j];
     }
     LaneSpheres<LPS> S;
-    arm_fk<LPS>(q, g, S);
-    const bool hit = collide_env<LPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live);
-    const uint64_t H = fold_groups<LPS>(wave_ballot(hit && live));
+    arm_fk<LPS>(q, sh.slot, S);
+    const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
+    const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+    m = 0;
+    if (!live || lane % Shape::GS != 0) return false;
     const bool s2_valid = !((H >> lane) & 1);
-    bool survivor = false;
-    if (live && g == 0) {
-        if (!s2_valid) {
-            A.out[e] = 0;
-            if (A.checked) A.checked[e] = 1;
-        } else {
-            const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
-            if (nd > kMaxSegments) {
-                A.out[e] = 0;
-                if (A.checked) A.checked[e] = 0;
-            } else if (nd < 2) {
-                A.out[e] = 1;
-                if (A.checked) A.checked[e] = 1;
-            } else {
-                survivor = true;
-            }
-        }
+    if (!s2_valid) {
+        A.out[e] = 0;
+        if (A.checked) A.checked[e] = 1;
+        return false;
     }
+    const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
+    if (nd > kMaxSegments) {
+        A.out[e] = 0;
+        if (A.checked) A.checked[e] = 0;
+        return false;
+    }
+    if (nd < 2) {
+        A.out[e] = 1;
+        if (A.checked) A.checked[e] = 1;
+        return false;
+    }
+    m = nd - 1;
+    return true;
+}
+
+template <int LPS, int OPS>
+__global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
+{
+    extern __shared__ float4 lds[];
+    stage_env(A.env, lds);
+    using Shape = GroupShape<LPS, OPS>;
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t e = wave * Shape::STATES + lane / Shape::GS;
+    int m = 0;
+    const bool survivor = endpoint_check<LPS, OPS>(A, lds, e, e < A.n, m);
     // wave-aggregated append of the survivors
     const uint64_t sm = wave_ballot(survivor);
     if (sm) {
@@ -148,13 +131,15 @@ __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
         if (lane == lowest_bit(sm)) base = atomicAdd(A.list_count, __popcll(sm));
         base = __shfl(base, lowest_bit(sm));
         if (survivor) {
-            const uint64_t below = sm & (((uint64_t)1 << lane) - 1);
-            A.list[base + __popcll(below)] = (int32_t)e;
+            const int slot = base + __popcll(sm & (((uint64_t)1 << lane) - 1));
+            A.list[slot] = (int32_t)e;
+            if (A.first_bad) A.first_bad[slot] = INT_MAX;
+            if (A.max_m) atomicMax(A.max_m, m);
         }
     }
 }
 
-template <int LPS, bool EXACT>
+template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
@@ -175,7 +160,7 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
             b[j] = A.b[(int64_t)j * A.n + e];
         }
         int total = 0;
-        const int bad = walk_edge<LPS, EXACT>(a, b, A.lvs, A.factor, lds, A.env.ns, A.env.nb, A.start, total);
+        const int bad = walk_edge<LPS, OPS, EXACT>(a, b, A.lvs, A.factor, lds, A.env.ns, A.env.nb, A.start, total);
         if (lane == 0) {
             A.out[e] = bad == INT_MAX ? 1 : 0;
             if (A.checked) A.checked[e] = bad == INT_MAX ? total : (bad < 0 ? 0 : bad + 1);
@@ -190,54 +175,71 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
     }
 }
 
-// Fused batch: each wave checks s2 of 64/LPS_E edges (LPS_E lanes per edge),
-// then walks its own survivors one after another with LPS_W lanes per state.
-// One launch, no queue and no counters.
-template <int LPS_E, int LPS_W, bool EXACT>
+template <int LPS, int OPS, bool EXACT>
+__global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
+{
+    extern __shared__ float4 lds[];
+    stage_env(A.env, lds);
+    constexpr int G = GroupShape<LPS, OPS>::STATES;
+    const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
+    const int64_t n_surv = *A.list_count;
+    if (n_surv == 0) return;
+    const int64_t nblk = ((int64_t)*A.max_m + G - 1) / G;
+    const int64_t items = n_surv * nblk;
+    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < items; k += waves_total) {
+        const int64_t blk = k / n_surv, s = k - blk * n_surv;
+        const int base = 1 + (int)(blk * G);
+        if (*(volatile int32_t *)&A.first_bad[s] < base) continue; // an earlier position failed
+        const int64_t e = A.list[s];
+        double a[kDof], b[kDof];
+#pragma unroll
+        for (int j = 0; j < kDof; ++j) {
+            a[j] = A.a[(int64_t)j * A.n + e];
+            b[j] = A.b[(int64_t)j * A.n + e];
+        }
+        const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
+        const int total = nd; // positions 0..nd-1 (nd >= 2 for survivors)
+        if (base >= total) continue;
+        const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, nd, total, base, lds, A.env.ns, A.env.nb);
+        if (bad != INT_MAX && lane_id() == 0) atomicMin(&A.first_bad[s], bad);
+    }
+}
+
+__global__ void motion_finalize_kernel(MotionArgs A)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= *A.list_count) return;
+    const int64_t e = A.list[s];
+    double a[kDof], b[kDof];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) {
+        a[j] = A.a[(int64_t)j * A.n + e];
+        b[j] = A.b[(int64_t)j * A.n + e];
+    }
+    const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
+    const int fb = A.first_bad[s];
+    A.out[e] = fb == INT_MAX ? 1 : 0;
+    if (A.checked) A.checked[e] = fb == INT_MAX ? nd : fb + 1;
+}
+
+// Fused batch: each wave checks s2 of its edges (endpoint shape LE x OE), then
+// walks its own survivors one after another (walk shape LW x OW).
+template <int LE, int OE, int LW, int OW, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_fused_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
-    constexpr int G = 64 / LPS_E;
+    using Shape = GroupShape<LE, OE>;
     const int lane = lane_id();
-    const int g = lane % LPS_E;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t e = wave * G + lane / LPS_E;
-    const bool live = e < A.n;
-    double a[kDof], b[kDof];
-    float q[kDof];
-#pragma unroll
-    for (int j = 0; j < kDof; ++j) {
-        a[j] = live ? A.a[(int64_t)j * A.n + e] : 0.0;
-        b[j] = live ? A.b[(int64_t)j * A.n + e] : 0.0;
-        q[j] = (float)b[j];
-    }
-    LaneSpheres<LPS_E> S;
-    arm_fk<LPS_E>(q, g, S);
-    const bool hit = collide_env<LPS_E, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live);
-    const uint64_t H = fold_groups<LPS_E>(wave_ballot(hit && live));
-    const bool s2_valid = !((H >> lane) & 1);
-    bool survivor = false;
-    if (live && g == 0) {
-        const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
-        if (!s2_valid) {
-            A.out[e] = 0;
-            if (A.checked) A.checked[e] = 1;
-        } else if (nd > kMaxSegments) {
-            A.out[e] = 0;
-            if (A.checked) A.checked[e] = 0;
-        } else if (nd < 2) {
-            A.out[e] = 1;
-            if (A.checked) A.checked[e] = 1;
-        } else {
-            survivor = true;
-        }
-    }
+    const int64_t e = wave * Shape::STATES + lane / Shape::GS;
+    int m = 0;
+    const bool survivor = endpoint_check<LE, OE>(A, lds, e, e < A.n, m);
     uint64_t sm = wave_ballot(survivor);
     while (sm) {
         const int src = lowest_bit(sm);
         sm &= sm - 1;
-        const int64_t es = wave * G + src / LPS_E;
+        const int64_t es = wave * Shape::STATES + src / Shape::GS;
         double ea[kDof], eb[kDof];
 #pragma unroll
         for (int j = 0; j < kDof; ++j) {
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(1024) motion_fused_kernel(MotionArgs A)
             eb[j] = A.b[(int64_t)j * A.n + es];
         }
         int total = 0;
-        const int bad = walk_edge<LPS_W, EXACT>(ea, eb, A.lvs, A.factor, lds, A.env.ns, A.env.nb, 1, total);
+        const int bad = walk_edge<LW, OW, EXACT>(ea, eb, A.lvs, A.factor, lds, A.env.ns, A.env.nb, 1, total);
         if (lane == 0) {
             A.out[es] = bad == INT_MAX ? 1 : 0;
             if (A.checked) A.checked[es] = bad == INT_MAX ? total : bad + 1;
@@ -253,14 +255,14 @@ __global__ void __launch_bounds__(1024) motion_fused_kernel(MotionArgs A)
     }
 }
 
-template <int LPS>
+template <int LPS, int OPS>
 __global__ void __launch_bounds__(64) motion_single_kernel(EnvView env, Edge1 E, double lvs, unsigned factor,
                                                            int *result)
 {
     extern __shared__ float4 lds[];
     stage_env(env, lds);
     int total = 0;
-    const int bad = walk_edge<LPS, false>(E.a, E.b, lvs, factor, lds, env.ns, env.nb, 0, total);
+    const int bad = walk_edge<LPS, OPS, false>(E.a, E.b, lvs, factor, lds, env.ns, env.nb, 0, total);
     if (lane_id() == 0) *result = bad == INT_MAX ? 1 : 0; // bad < 0: too many segments, invalid
 }
 
@@ -268,29 +270,30 @@ __global__ void __launch_bounds__(64) validate_single_kernel(EnvView env, State1
 {
     extern __shared__ float4 lds[];
     stage_env(env, lds);
-    // 8 lanes per state; lane groups 1..7 idle
+    // one state over the whole wave: 8 sphere slots x 8 obstacle partitions
+    using Shape = GroupShape<8, 8>;
     const int lane = lane_id();
-    const bool live = lane < 8;
+    const Shape sh(lane);
     float q[kDof];
 #pragma unroll
     for (int j = 0; j < kDof; ++j) q[j] = (float)s.q[j];
     LaneSpheres<8> S;
-    arm_fk<8>(q, lane & 7, S);
-    const bool hit = collide_env<8, Exit::kAllGroups>(S, lds, env.ns, env.nb, live);
-    const uint64_t H = wave_ballot(hit && live);
+    arm_fk<8>(q, sh.slot, S);
+    const bool hit = collide_env<8, 8, Exit::kAllGroups>(S, lds, env.ns, env.nb, true, sh.part);
+    const uint64_t H = wave_ballot(hit);
     if (lane == 0) *result = H ? 0 : 1;
 }
 
 // Sequential lastValid overload: positions j = 1..nd-1 in order, then s2.
-template <int LPS>
+template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, double *__restrict__ last_t)
 {
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
-    constexpr int G = 64 / LPS;
+    using Shape = GroupShape<LPS, OPS>;
+    constexpr int G = Shape::STATES;
     const int lane = lane_id();
-    const int g = lane % LPS;
-    const int grp = lane / LPS;
+    const Shape sh(lane);
     const int waves_total = (gridDim.x * blockDim.x) >> 6;
     for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; e < A.n; e += waves_total) {
         double a[kDof], b[kDof];
@@ -311,7 +314,7 @@ __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, d
         const int last = nd > 1 ? nd : 1;
         int bad = INT_MAX;
         for (int base = 1; base <= last; base += G) {
-            const int p = base + grp;
+            const int p = base + sh.group;
             const bool live = p <= last;
             float q[kDof];
             if (p >= last) {
@@ -321,11 +324,11 @@ __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, d
                 interpolate_f32<kDof>(a, b, (double)p / (double)nd, q);
             }
             LaneSpheres<LPS> S;
-            arm_fk<LPS>(q, g, S);
-            const bool hit = collide_env<LPS, Exit::kLowestGroup>(S, lds, A.env.ns, A.env.nb, live);
-            const uint64_t H = fold_groups<LPS>(wave_ballot(hit && live));
+            arm_fk<LPS>(q, sh.slot, S);
+            const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, A.env.ns, A.env.nb, live, sh.part);
+            const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
             if (H) {
-                bad = base + lowest_bit(H) / LPS;
+                bad = base + lowest_bit(H) / Shape::GS;
                 break;
             }
         }
@@ -348,169 +351,6 @@ __global__ void zero_ints_kernel(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
-// ---------------------------------------------------------------------------
-// host launchers
-// ---------------------------------------------------------------------------
-static size_t lds_bytes(const EnvView &e)
-{
-    return (size_t)(e.ns + 2 * e.nb) * sizeof(float4);
-}
-
-template <int LPS, typename T>
-static void launch_validate_t(const EnvView &env, const T *q, int64_t n, uint8_t *out, int block, hipStream_t s)
-{
-    const int64_t states_per_block = (int64_t)(block / 64) * (64 / LPS);
-    const int64_t blocks = (n + states_per_block - 1) / states_per_block;
-    hipLaunchKernelGGL((validate_kernel<LPS, T>), dim3((unsigned)blocks), dim3(block), lds_bytes(env), s, env, q, n,
-                       out);
-}
-
-template <typename T>
-void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, int lps, int block, hipStream_t s)
-{
-    if (n <= 0) return;
-    ProfScope ps("validate_states", s);
-    switch (lps) {
-    case 1: launch_validate_t<1>(env, q, n, out, block, s); break;
-    case 2: launch_validate_t<2>(env, q, n, out, block, s); break;
-    case 4: launch_validate_t<4>(env, q, n, out, block, s); break;
-    default: launch_validate_t<8>(env, q, n, out, block, s); break;
-    }
-    OA_HIP(hipGetLastError());
-}
-template void launch_validate<float>(const EnvView &, const float *, int64_t, uint8_t *, int, int, hipStream_t);
-template void launch_validate<double>(const EnvView &, const double *, int64_t, uint8_t *, int, int, hipStream_t);
-
-void launch_zero(int *p, int n, hipStream_t s)
-{
-    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
-    OA_HIP(hipGetLastError());
-}
-
-template <int LPS>
-static void launch_endpoint_t(const MotionArgs &A, int block, hipStream_t s)
-{
-    const int64_t per_block = (int64_t)(block / 64) * (64 / LPS);
-    const int64_t blocks = (A.n + per_block - 1) / per_block;
-    hipLaunchKernelGGL((motion_endpoint_kernel<LPS>), dim3((unsigned)blocks), dim3(block), lds_bytes(A.env), s, A);
-}
-
-void launch_motion_endpoint(const MotionArgs &A, int lps, int block, hipStream_t s)
-{
-    if (A.n <= 0) return;
-    ProfScope ps("motion_endpoint", s);
-    switch (lps) {
-    case 1: launch_endpoint_t<1>(A, block, s); break;
-    case 2: launch_endpoint_t<2>(A, block, s); break;
-    case 4: launch_endpoint_t<4>(A, block, s); break;
-    default: launch_endpoint_t<8>(A, block, s); break;
-    }
-    OA_HIP(hipGetLastError());
-}
-
-template <int LPS>
-static void launch_walk_t(const MotionArgs &A, bool exact, int block, int64_t waves, hipStream_t s)
-{
-    const int wpb = block / 64;
-    const int64_t blocks = (waves + wpb - 1) / wpb;
-    if (exact)
-        hipLaunchKernelGGL((motion_walk_kernel<LPS, true>), dim3((unsigned)blocks), dim3(block), lds_bytes(A.env), s,
-                           A);
-    else
-        hipLaunchKernelGGL((motion_walk_kernel<LPS, false>), dim3((unsigned)blocks), dim3(block), lds_bytes(A.env), s,
-                           A);
-}
-
-void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int block, int64_t waves, hipStream_t s)
-{
-    if (waves <= 0) return;
-    ProfScope ps("motion_walk", s);
-    switch (lps) {
-    case 1: launch_walk_t<1>(A, exact, block, waves, s); break;
-    case 2: launch_walk_t<2>(A, exact, block, waves, s); break;
-    case 4: launch_walk_t<4>(A, exact, block, waves, s); break;
-    default: launch_walk_t<8>(A, exact, block, waves, s); break;
-    }
-    OA_HIP(hipGetLastError());
-}
-
-template <int LPS_E, int LPS_W>
-static void launch_fused_t(const MotionArgs &A, bool exact, int block, hipStream_t s)
-{
-    const int64_t per_block = (int64_t)(block / 64) * (64 / LPS_E);
-    const unsigned blocks = (unsigned)((A.n + per_block - 1) / per_block);
-    if (exact)
-        hipLaunchKernelGGL((motion_fused_kernel<LPS_E, LPS_W, true>), dim3(blocks), dim3(block), lds_bytes(A.env), s,
-                           A);
-    else
-        hipLaunchKernelGGL((motion_fused_kernel<LPS_E, LPS_W, false>), dim3(blocks), dim3(block), lds_bytes(A.env),
-                           s, A);
-}
-
-void launch_motion_fused(const MotionArgs &A, bool exact, int lps_end, int lps_walk, int block, hipStream_t s)
-{
-    if (A.n <= 0) return;
-    ProfScope ps("motion_fused", s);
-#define OA_FUSED(E, W)                                                                                             \
-    if (lps_end == E && lps_walk == W) {                                                                            \
-        launch_fused_t<E, W>(A, exact, block, s);                                                                   \
-        OA_HIP(hipGetLastError());                                                                                  \
-        return;                                                                                                     \
-    }
-    OA_FUSED(8, 1)
-    OA_FUSED(8, 2)
-    OA_FUSED(8, 4)
-    OA_FUSED(4, 1)
-    OA_FUSED(4, 2)
-    OA_FUSED(2, 1)
-    OA_FUSED(1, 1)
-#undef OA_FUSED
-    fail(OMPL_AMD_ERR_ARG, "fused motion kernel not built for lanes (%d, %d)", lps_end, lps_walk);
-}
-
-void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result, hipStream_t s)
-{
-    hipLaunchKernelGGL((motion_single_kernel<1>), dim3(1), dim3(64), lds_bytes(env), s, env, E, lvs, factor, result);
-    OA_HIP(hipGetLastError());
-}
-
-void launch_validate_single(const EnvView &env, const State1 &st, int *result, hipStream_t s)
-{
-    hipLaunchKernelGGL(validate_single_kernel, dim3(1), dim3(64), lds_bytes(env), s, env, st, result);
-    OA_HIP(hipGetLastError());
-}
-
-void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int block, int64_t waves,
-                              hipStream_t s)
-{
-    if (A.n <= 0) return;
-    const int wpb = block / 64;
-    const int64_t blocks = (waves + wpb - 1) / wpb;
-    switch (lps) {
-    case 1:
-        hipLaunchKernelGGL((motion_last_valid_kernel<1>), dim3((unsigned)blocks), dim3(block), lds_bytes(A.env), s,
-                           A, last_t);
-        break;
-    case 2:
-        hipLaunchKernelGGL((motion_last_valid_kernel<2>), dim3((unsigned)blocks), dim3(block), lds_bytes(A.env), s,
-                           A, last_t);
-        break;
-    case 4:
-        hipLaunchKernelGGL((motion_last_valid_kernel<4>), dim3((unsigned)blocks), dim3(block), lds_bytes(A.env), s,
-                           A, last_t);
-        break;
-    default:
-        hipLaunchKernelGGL((motion_last_valid_kernel<8>), dim3((unsigned)blocks), dim3(block), lds_bytes(A.env), s,
-                           A, last_t);
-        break;
-    }
-    OA_HIP(hipGetLastError());
-}
-
-} // namespace ompl_amd
-
-namespace ompl_amd {
-
 __global__ void arm_spheres_kernel(const float *__restrict__ q_soa, int64_t n, float *__restrict__ out)
 {
     const int64_t st = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,10 +368,177 @@ __global__ void arm_spheres_kernel(const float *__restrict__ q_soa, int64_t n, f
     }
 }
 
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static size_t lds_bytes(const EnvView &e)
+{
+    return (size_t)(e.ns + 2 * e.nb) * sizeof(float4);
+}
+
+static unsigned grid_for(int64_t units, int64_t units_per_block)
+{
+    return (unsigned)((units + units_per_block - 1) / units_per_block);
+}
+
+template <typename T>
+void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, int lps, int ops, int block,
+                     hipStream_t s)
+{
+    if (n <= 0) return;
+    ProfScope ps("validate_states", s);
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
+        hipLaunchKernelGGL((validate_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env), s, env, \
+                           q, n, out);                                                                              \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+    OA_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    fail(OMPL_AMD_ERR_ARG, "validity shape (%d, %d) not built", lps, ops);
+}
+template void launch_validate<float>(const EnvView &, const float *, int64_t, uint8_t *, int, int, int,
+                                     hipStream_t);
+template void launch_validate<double>(const EnvView &, const double *, int64_t, uint8_t *, int, int, int,
+                                      hipStream_t);
+
+void launch_zero(int *p, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    OA_HIP(hipGetLastError());
+}
+
+void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hipStream_t s)
+{
+    if (A.n <= 0) return;
+    ProfScope ps("motion_endpoint", s);
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
+        hipLaunchKernelGGL((motion_endpoint_kernel<L, O>), dim3(grid_for(A.n, per)), dim3(block),                 \
+                           lds_bytes(A.env), s, A);                                                                \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+    OA_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    fail(OMPL_AMD_ERR_ARG, "endpoint shape (%d, %d) not built", lps, ops);
+}
+
+void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves, hipStream_t s)
+{
+    if (waves <= 0) return;
+    ProfScope ps("motion_walk", s);
+    const unsigned g = grid_for(waves, block / 64);
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        if (exact)                                                                                                  \
+            hipLaunchKernelGGL((motion_walk_kernel<L, O, true>), dim3(g), dim3(block), lds_bytes(A.env), s, A);   \
+        else                                                                                                        \
+            hipLaunchKernelGGL((motion_walk_kernel<L, O, false>), dim3(g), dim3(block), lds_bytes(A.env), s, A);  \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+    OA_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    fail(OMPL_AMD_ERR_ARG, "walk shape (%d, %d) not built", lps, ops);
+}
+
+void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
+                          hipStream_t s)
+{
+    ProfScope ps("motion_chunk", s);
+    const unsigned g = grid_for(waves, block / 64);
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        if (exact)                                                                                                  \
+            hipLaunchKernelGGL((motion_chunk_kernel<L, O, true>), dim3(g), dim3(block), lds_bytes(A.env), s, A);  \
+        else                                                                                                        \
+            hipLaunchKernelGGL((motion_chunk_kernel<L, O, false>), dim3(g), dim3(block), lds_bytes(A.env), s, A); \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+    OA_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    fail(OMPL_AMD_ERR_ARG, "chunk shape (%d, %d) not built", lps, ops);
+}
+
+void launch_motion_finalize(const MotionArgs &A, int64_t n_max, hipStream_t s)
+{
+    if (n_max <= 0) return;
+    hipLaunchKernelGGL(motion_finalize_kernel, dim3(grid_for(n_max, 256)), dim3(256), 0, s, A);
+    OA_HIP(hipGetLastError());
+}
+
+bool fused_supported(int le, int oe, int lw, int ow)
+{
+    const bool e_ok = le == 8 && (oe == 1 || oe == 2 || oe == 4);
+    const bool w_ok = (lw == 1 && ow == 1) || (lw == 8 && (ow == 1 || ow == 8));
+    return e_ok && w_ok;
+}
+
+void launch_motion_fused(const MotionArgs &A, bool exact, int le, int oe, int lw, int ow, int block, hipStream_t s)
+{
+    if (A.n <= 0) return;
+    ProfScope ps("motion_fused", s);
+#define OA_FUSED_W(LE, OE, LW, OW)                                                                                  \
+    if (oe == OE && lw == LW && ow == OW) {                                                                         \
+        const int64_t per = (int64_t)(block / 64) * GroupShape<LE, OE>::STATES;                                   \
+        if (exact)                                                                                                  \
+            hipLaunchKernelGGL((motion_fused_kernel<LE, OE, LW, OW, true>), dim3(grid_for(A.n, per)), dim3(block), \
+                               lds_bytes(A.env), s, A);                                                             \
+        else                                                                                                        \
+            hipLaunchKernelGGL((motion_fused_kernel<LE, OE, LW, OW, false>), dim3(grid_for(A.n, per)),            \
+                               dim3(block), lds_bytes(A.env), s, A);                                                \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+#define OA_FUSED_E(OE) OA_FUSED_W(8, OE, 1, 1) OA_FUSED_W(8, OE, 8, 1) OA_FUSED_W(8, OE, 8, 8)
+    if (le == 8) {
+        OA_FUSED_E(1)
+        OA_FUSED_E(2)
+        OA_FUSED_E(4)
+    }
+#undef OA_FUSED_E
+#undef OA_FUSED_W
+    fail(OMPL_AMD_ERR_ARG, "fused shape (%d,%d | %d,%d) not built", le, oe, lw, ow);
+}
+
+void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result, hipStream_t s)
+{
+    hipLaunchKernelGGL((motion_single_kernel<1, 1>), dim3(1), dim3(64), lds_bytes(env), s, env, E, lvs, factor,
+                       result);
+    OA_HIP(hipGetLastError());
+}
+
+void launch_validate_single(const EnvView &env, const State1 &st, int *result, hipStream_t s)
+{
+    hipLaunchKernelGGL(validate_single_kernel, dim3(1), dim3(64), lds_bytes(env), s, env, st, result);
+    OA_HIP(hipGetLastError());
+}
+
+void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int ops, int block, int64_t waves,
+                              hipStream_t s)
+{
+    if (A.n <= 0) return;
+    const unsigned g = grid_for(waves, block / 64);
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        hipLaunchKernelGGL((motion_last_valid_kernel<L, O>), dim3(g), dim3(block), lds_bytes(A.env), s, A, last_t); \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+    OA_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    fail(OMPL_AMD_ERR_ARG, "lastValid shape (%d, %d) not built", lps, ops);
+}
+
 void launch_arm_spheres(const float *q, int64_t n, float *out, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(arm_spheres_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, q, n, out);
+    hipLaunchKernelGGL(arm_spheres_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, q, n, out);
     OA_HIP(hipGetLastError());
 }
 

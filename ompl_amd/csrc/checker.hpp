@@ -18,11 +18,13 @@ struct MotionArgs {
     double lvs;
     unsigned factor;
     uint8_t *out;
-    int32_t *checked;        // optional
-    int32_t *list;           // survivor list (endpoint output / walk input), optional
-    int32_t *list_count;     // number of entries in list
-    int32_t *queue;          // dynamic work counter for the walk, optional
-    int start;               // first walk position (0 = s2, 1 = skip s2)
+    int32_t *checked;    // optional
+    int32_t *list;       // survivor list (endpoint output / walk input), optional
+    int32_t *list_count; // number of entries in list
+    int32_t *queue;      // dynamic work counter for the walk, optional
+    int32_t *first_bad;  // per survivor: first invalid position (chunk-parallel walk)
+    int32_t *max_m;      // largest bisection position count among survivors
+    int start;           // first walk position (0 = s2, 1 = skip s2)
 };
 
 struct Edge1 {
@@ -34,16 +36,23 @@ struct State1 {
     double q[kDof];
 };
 
+bool shape_supported(int lps, int ops);
+bool fused_supported(int le, int oe, int lw, int ow);
+
 template <typename T>
-void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, int lps, int block, hipStream_t s);
+void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, int lps, int ops, int block,
+                     hipStream_t s);
 void launch_zero(int *p, int n, hipStream_t s);
-void launch_motion_endpoint(const MotionArgs &A, int lps, int block, hipStream_t s);
-void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int block, int64_t waves, hipStream_t s);
-void launch_motion_fused(const MotionArgs &A, bool exact, int lps_end, int lps_walk, int block, hipStream_t s);
+void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hipStream_t s);
+void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves, hipStream_t s);
+void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
+                          hipStream_t s);
+void launch_motion_finalize(const MotionArgs &A, int64_t n_max, hipStream_t s);
+void launch_motion_fused(const MotionArgs &A, bool exact, int le, int oe, int lw, int ow, int block, hipStream_t s);
 void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result,
                           hipStream_t s);
 void launch_validate_single(const EnvView &env, const State1 &st, int *result, hipStream_t s);
-void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int block, int64_t waves,
+void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int ops, int block, int64_t waves,
                               hipStream_t s);
 void launch_arm_spheres(const float *q, int64_t n, float *out, hipStream_t s);
 

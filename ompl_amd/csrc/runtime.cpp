@@ -19,7 +19,7 @@ int g_cus = 256;
 hipStream_t g_stream = nullptr;
 std::mutex g_ws_mutex;
 std::map<hipStream_t, Workspace *> g_ws;
-Tuning g_tuning{8, 256, 1, 256, 8, 1};
+Tuning g_tuning{8, 2, 256, 8, 1, 256, 8, 3};
 std::mutex g_tuning_mutex;
 } // namespace
 
@@ -110,8 +110,10 @@ Workspace &workspace(hipStream_t s, size_t list_entries)
     }
     if (w->list_cap < list_entries) {
         if (w->list) OA_HIP(hipFree(w->list));
+        if (w->first_bad) OA_HIP(hipFree(w->first_bad));
         size_t cap = list_entries < 1024 ? 1024 : list_entries;
         OA_HIP(hipMalloc(&w->list, cap * sizeof(int32_t)));
+        OA_HIP(hipMalloc(&w->first_bad, cap * sizeof(int32_t)));
         w->list_cap = cap;
     }
     return *w;
@@ -178,21 +180,44 @@ Tuning get_tuning()
 void set_tuning(const Tuning &t)
 {
     auto pow2 = [](int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; };
-    OA_REQUIRE(pow2(t.validity_lanes_per_state, 1, 8) && pow2(t.motion_lanes_per_state, 1, 8), OMPL_AMD_ERR_ARG,
-               "lanes_per_state must be 1, 2, 4 or 8");
+    OA_REQUIRE(shape_supported(t.validity_lanes_per_state, t.validity_obstacle_parts), OMPL_AMD_ERR_ARG,
+               "validity shape (lanes %d, obstacle parts %d) not built", t.validity_lanes_per_state,
+               t.validity_obstacle_parts);
+    OA_REQUIRE(shape_supported(t.motion_lanes_per_state, t.motion_obstacle_parts), OMPL_AMD_ERR_ARG,
+               "motion shape (lanes %d, obstacle parts %d) not built", t.motion_lanes_per_state,
+               t.motion_obstacle_parts);
     OA_REQUIRE(pow2(t.validity_block, 64, 1024) && pow2(t.motion_block, 64, 1024), OMPL_AMD_ERR_ARG,
                "block must be a power of two in [64, 1024]");
     OA_REQUIRE(t.motion_waves_per_cu >= 1 && t.motion_waves_per_cu <= 32, OMPL_AMD_ERR_ARG,
                "motion_waves_per_cu must be in [1, 32]");
-    OA_REQUIRE(t.motion_strategy >= 0 && t.motion_strategy <= 2, OMPL_AMD_ERR_ARG, "motion_strategy must be 0, 1 or 2");
-    if (t.motion_strategy == 2) {
-        // instantiated (endpoint lanes, walk lanes) pairs of motion_fused_kernel
-        const int E = t.validity_lanes_per_state, W = t.motion_lanes_per_state;
-        const bool ok = (E == 8 && W <= 4) || (E == 4 && W <= 2) || (E == 2 && W == 1) || (E == 1 && W == 1);
-        OA_REQUIRE(ok, OMPL_AMD_ERR_ARG, "fused strategy: (endpoint, walk) lanes (%d, %d) not built", E, W);
-    }
+    OA_REQUIRE(t.motion_strategy >= 0 && t.motion_strategy <= 3, OMPL_AMD_ERR_ARG,
+               "motion_strategy must be 0, 1, 2 or 3");
+    if (t.motion_strategy == 2)
+        OA_REQUIRE(fused_supported(t.validity_lanes_per_state, t.validity_obstacle_parts, t.motion_lanes_per_state,
+                                   t.motion_obstacle_parts),
+                   OMPL_AMD_ERR_ARG, "fused strategy: shape pair not built");
     std::lock_guard<std::mutex> lk(g_tuning_mutex);
     g_tuning = t;
+}
+
+ompl_amd_tuning tuning_to_c(const Tuning &x)
+{
+    ompl_amd_tuning t;
+    t.validity_lanes_per_state = x.validity_lanes_per_state;
+    t.validity_obstacle_parts = x.validity_obstacle_parts;
+    t.validity_block = x.validity_block;
+    t.motion_lanes_per_state = x.motion_lanes_per_state;
+    t.motion_obstacle_parts = x.motion_obstacle_parts;
+    t.motion_block = x.motion_block;
+    t.motion_waves_per_cu = x.motion_waves_per_cu;
+    t.motion_strategy = x.motion_strategy;
+    return t;
+}
+
+Tuning tuning_from_c(const ompl_amd_tuning &t)
+{
+    return Tuning{t.validity_lanes_per_state, t.validity_obstacle_parts, t.validity_block, t.motion_lanes_per_state,
+                  t.motion_obstacle_parts,    t.motion_block,            t.motion_waves_per_cu, t.motion_strategy};
 }
 
 } // namespace ompl_amd
@@ -301,13 +326,7 @@ int ompl_amd_get_tuning(ompl_amd_tuning *t)
 {
     return guarded([&] {
         OA_REQUIRE(t, OMPL_AMD_ERR_ARG, "null tuning");
-        Tuning x = get_tuning();
-        t->validity_lanes_per_state = x.validity_lanes_per_state;
-        t->validity_block = x.validity_block;
-        t->motion_lanes_per_state = x.motion_lanes_per_state;
-        t->motion_block = x.motion_block;
-        t->motion_waves_per_cu = x.motion_waves_per_cu;
-        t->motion_strategy = x.motion_strategy;
+        *t = tuning_to_c(get_tuning());
     });
 }
 
@@ -315,8 +334,7 @@ int ompl_amd_set_tuning(const ompl_amd_tuning *t)
 {
     return guarded([&] {
         OA_REQUIRE(t, OMPL_AMD_ERR_ARG, "null tuning");
-        set_tuning(Tuning{t->validity_lanes_per_state, t->validity_block, t->motion_lanes_per_state,
-                          t->motion_block, t->motion_waves_per_cu, t->motion_strategy});
+        set_tuning(tuning_from_c(*t));
     });
 }
 

@@ -16,8 +16,9 @@ void init_device(int dev);
 // Per-stream scratch of the batched motion path: survivor list + counters.
 struct Workspace {
     int32_t *list = nullptr;
+    int32_t *first_bad = nullptr; // per survivor (chunk-parallel walk)
     size_t list_cap = 0;
-    int32_t *counters = nullptr; // [0] survivor count, [1] walk queue
+    int32_t *counters = nullptr; // [0] survivor count, [1] walk queue, [2] max positions
 };
 Workspace &workspace(hipStream_t s, size_t list_entries);
 
@@ -30,16 +31,22 @@ struct ThreadResources {
 };
 ThreadResources &thread_resources();
 
+// Launch shapes (mirrors ompl_amd_tuning).
 struct Tuning {
     int validity_lanes_per_state;
+    int validity_obstacle_parts;
     int validity_block;
     int motion_lanes_per_state;
+    int motion_obstacle_parts;
     int motion_block;
     int motion_waves_per_cu;
     int motion_strategy;
 };
+
 Tuning get_tuning();
 void set_tuning(const Tuning &t);
+ompl_amd_tuning tuning_to_c(const Tuning &x);
+Tuning tuning_from_c(const ompl_amd_tuning &t);
 
 // Device-resident obstacle set.
 struct Env {

@@ -1,0 +1,82 @@
+// Device helpers shared by the batched motion kernels and the planning
+// server: wave utilities and the walk of one edge in DiscreteMotionValidator
+// order (position 0 = s2, positions p >= 1 = the p-th bisection check,
+// base/src/DiscreteMotionValidator.cpp:93-145).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "arm_device.hpp"
+#include "motion_device.hpp"
+
+namespace ompl_amd {
+
+__device__ __forceinline__ int lane_id()
+{
+    return threadIdx.x & 63;
+}
+
+__device__ __forceinline__ int lowest_bit(uint64_t m)
+{
+    return __builtin_ctzll(m);
+}
+
+// State of walk position p (0 = b, else bisection index p-1) as FP32.
+__device__ __forceinline__ void walk_state(const double a[kDof], const double b[kDof], int nd, int p, float q[kDof])
+{
+    if (p == 0) {
+#pragma unroll
+        for (int j = 0; j < kDof; ++j) q[j] = (float)b[j];
+    } else {
+        const int idx = bisection_index(p - 1, nd);
+        interpolate_f32<kDof>(a, b, (double)idx / (double)nd, q);
+    }
+}
+
+// Checks walk positions [base, base + STATES) of one edge (one state per lane
+// group); returns the lowest position found invalid, or INT_MAX.  With EXACT
+// the returned position is the first invalid one of the chunk.
+template <int LPS, int OPS, bool EXACT>
+__device__ __forceinline__ int walk_chunk(const double a[kDof], const double b[kDof], int nd, int total, int base,
+                                          const float4 *lds, int ns, int nb)
+{
+    using Shape = GroupShape<LPS, OPS>;
+    const Shape sh(lane_id());
+    const int p = base + sh.group;
+    const bool live = p < total;
+    float q[kDof];
+    walk_state(a, b, nd, live ? p : 0, q);
+    LaneSpheres<LPS> S;
+    arm_fk<LPS>(q, sh.slot, S);
+    const bool hit =
+        collide_env<LPS, OPS, EXACT ? Exit::kLowestGroup : Exit::kAnyGroup>(S, lds, ns, nb, live, sh.part);
+    const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+    return H ? base + lowest_bit(H) / Shape::GS : INT_MAX;
+}
+
+// Walks one edge from position `start` (0 = s2) chunk by chunk.  Returns the
+// position of the first invalid state found (INT_MAX if none, -1 when the
+// edge has more than kMaxSegments segments) and the total position count.
+template <int LPS, int OPS, bool EXACT>
+__device__ __forceinline__ int walk_edge(const double a[kDof], const double b[kDof], double lvs, unsigned factor,
+                                         const float4 *lds, int ns, int nb, int start, int &total)
+{
+    constexpr int G = GroupShape<LPS, OPS>::STATES;
+    const int nd = valid_segment_count<kDof>(a, b, lvs, factor);
+    if (nd > kMaxSegments) {
+        total = 0;
+        return -1;
+    }
+    const int m = nd >= 2 ? nd - 1 : 0;
+    total = 1 + m;
+    for (int base = start; base < total; base += G) {
+        const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, nd, total, base, lds, ns, nb);
+        if (bad != INT_MAX) return bad;
+    }
+    return INT_MAX;
+}
+
+} // namespace ompl_amd

@@ -605,3 +605,120 @@ void orc_knn_batch(int metric, const float *data, long n, int dim, const float *
     p.dist = out_dist;
     orc_run(p, nq, nthreads);
 }
+
+/* ======================================================================
+ * RRT-Connect restatement (see oracle.h)
+ * ====================================================================== */
+typedef struct {
+    double *s;  /* [cap][7] */
+    int *parent;
+    int n, cap;
+} orc_tree;
+
+static void tree_push(orc_tree *t, const double *st, int parent)
+{
+    if (t->n == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 256;
+        t->s = (double *)realloc(t->s, sizeof(double) * ORC_ARM_DOF * (size_t)t->cap);
+        t->parent = (int *)realloc(t->parent, sizeof(int) * (size_t)t->cap);
+    }
+    memcpy(t->s + (size_t)t->n * ORC_ARM_DOF, st, sizeof(double) * ORC_ARM_DOF);
+    t->parent[t->n] = parent;
+    ++t->n;
+}
+
+enum { ORC_TRAPPED = 0, ORC_ADVANCED = 1, ORC_REACHED = 2 };
+
+/* growTree: nearest (first strict minimum of distance(elem, target)), steer
+ * by max_distance, check the motion in the direction of tree growth. */
+static int grow_tree(const orc_env *env, orc_tree *t, int start_tree, const double *target, double maxd, double lvs,
+                     unsigned factor, int *xmotion)
+{
+    int nm = 0;
+    double dmin = 0.0;
+    for (int i = 0; i < t->n; ++i) {
+        double d = orc_distance_d(t->s + (size_t)i * ORC_ARM_DOF, target, ORC_ARM_DOF);
+        if (i == 0 || dmin > d) {
+            nm = i;
+            dmin = d;
+        }
+    }
+    const double *ns = t->s + (size_t)nm * ORC_ARM_DOF;
+    int reach = 1;
+    double x[ORC_ARM_DOF];
+    const double *dst = target;
+    double d = orc_distance_d(ns, target, ORC_ARM_DOF);
+    if (d > maxd) {
+        orc_interpolate_d(ns, target, maxd / d, ORC_ARM_DOF, x);
+        int equal = 1;
+        for (int k = 0; k < ORC_ARM_DOF; ++k)
+            if (fabs(ns[k] - x[k]) > 2.220446049250313080847e-16 * 2.0) equal = 0;
+        if (equal) return ORC_TRAPPED;
+        dst = x;
+        reach = 0;
+    }
+    int valid;
+    if (start_tree)
+        valid = orc_check_motion(env, ns, dst, lvs, factor, NULL);
+    else
+        valid = orc_state_valid_d(env, dst, NULL) && orc_check_motion(env, dst, ns, lvs, factor, NULL);
+    if (!valid) return ORC_TRAPPED;
+    tree_push(t, dst, nm);
+    *xmotion = t->n - 1;
+    return reach ? ORC_REACHED : ORC_ADVANCED;
+}
+
+int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const double goal[ORC_ARM_DOF],
+                    uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters, double *path,
+                    int path_cap, int *path_len, int *iterations, long *tree_nodes)
+{
+    const double pi = 3.141592653589793238462643383279502884;
+    orc_tree ts = {0}, tg = {0};
+    tree_push(&ts, start, -1);
+    tree_push(&tg, goal, -1);
+    orc_mt19937 rng;
+    orc_mt_seed(&rng, seed);
+    int start_tree = 1, solved = 0, it = 0;
+    double rstate[ORC_ARM_DOF];
+    *path_len = 0;
+    for (it = 0; it < max_iters && !solved; ++it) {
+        orc_tree *tree = start_tree ? &ts : &tg;
+        int tgi_start = start_tree;
+        start_tree = !start_tree;
+        orc_tree *other = start_tree ? &ts : &tg;
+        for (int k = 0; k < ORC_ARM_DOF; ++k) rstate[k] = orc_uniform_real(&rng, -pi, pi);
+        int xmotion = -1;
+        int gs = grow_tree(env, tree, tgi_start, rstate, max_distance, lvs, factor, &xmotion);
+        if (gs == ORC_TRAPPED) continue;
+        int added = xmotion;
+        if (gs != ORC_REACHED) memcpy(rstate, tree->s + (size_t)added * ORC_ARM_DOF, sizeof(rstate));
+        tgi_start = start_tree;
+        int gsc = grow_tree(env, other, tgi_start, rstate, max_distance, lvs, factor, &xmotion);
+        if (gsc == ORC_TRAPPED) tgi_start = !tgi_start;
+        while (gsc == ORC_ADVANCED) gsc = grow_tree(env, other, tgi_start, rstate, max_distance, lvs, factor, &xmotion);
+        if (gsc != ORC_REACHED) continue;
+        /* tgi.start tells which tree xmotion belongs to */
+        int sm = tgi_start ? xmotion : added;
+        int gm = tgi_start ? added : xmotion;
+        if (ts.parent[sm] >= 0) sm = ts.parent[sm];
+        else gm = tg.parent[gm];
+        int n1 = 0, n2 = 0;
+        for (int m = sm; m >= 0; m = ts.parent[m]) ++n1;
+        for (int m = gm; m >= 0; m = tg.parent[m]) ++n2;
+        *path_len = n1 + n2;
+        int pos = n1 - 1;
+        for (int m = sm; m >= 0; m = ts.parent[m], --pos)
+            if (pos < path_cap) memcpy(path + (size_t)pos * ORC_ARM_DOF, ts.s + (size_t)m * ORC_ARM_DOF, sizeof(rstate));
+        pos = n1;
+        for (int m = gm; m >= 0; m = tg.parent[m], ++pos)
+            if (pos < path_cap) memcpy(path + (size_t)pos * ORC_ARM_DOF, tg.s + (size_t)m * ORC_ARM_DOF, sizeof(rstate));
+        solved = 1;
+    }
+    *iterations = it;
+    if (tree_nodes) *tree_nodes = (long)ts.n + tg.n;
+    free(ts.s);
+    free(ts.parent);
+    free(tg.s);
+    free(tg.parent);
+    return solved;
+}

@@ -116,6 +116,16 @@ void orc_validate_states_batch(const orc_env *env, const float *q_soa, long n, u
 void orc_knn_batch(int metric, const float *data, long n, int dim, const float *queries, long nq, int k,
                    int *out_idx, float *out_dist, int nthreads);
 
+/* ---- RRT-Connect (geometric/planners/rrt/src/RRTConnect.cpp:116-196 growTree,
+ * :245-330 solve) on the 7-DOF arm: one start, one goal state, samples from
+ * RNG(seed)::uniformReal(-pi, pi) per joint, Linear nearest neighbours in FP64
+ * (first strict minimum), DiscreteMotionValidator checks, addIntermediateStates
+ * off.  Returns 1 when solved within max_iters iterations; the path (states,
+ * row-major) goes to path (up to path_cap states, *path_len = full length). */
+int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const double goal[ORC_ARM_DOF],
+                    uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters, double *path,
+                    int path_cap, int *path_len, int *iterations, long *tree_nodes);
+
 #ifdef __cplusplus
 }
 #endif

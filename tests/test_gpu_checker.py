@@ -38,11 +38,14 @@ def test_fk_bit_exact(gpu, oracle):
         assert np.array_equal(got[i], oracle.arm_spheres(q[i])), i
 
 
-@pytest.mark.parametrize("lps", [1, 2, 4, 8])
-@pytest.mark.parametrize("ns,nb", [(64, 0), (128, 128), (0, 40), (0, 0)])
-def test_validate_states_bit_exact(gpu, oracle, lps, ns, nb):
-    print(f"validate lps={lps} ns={ns} nb={nb}", flush=True)
-    gpu.set_tuning(validity_lanes_per_state=lps)
+SHAPES = [(1, 1), (2, 1), (4, 1), (8, 1), (8, 2), (8, 4), (8, 8)]
+
+
+@pytest.mark.parametrize("lps,ops", SHAPES)
+@pytest.mark.parametrize("ns,nb", [(64, 0), (128, 128), (0, 40), (0, 0), (3, 1)])
+def test_validate_states_bit_exact(gpu, oracle, lps, ops, ns, nb):
+    print(f"validate shape=({lps},{ops}) ns={ns} nb={nb}", flush=True)
+    gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops)
     env, oenv, _ = env_pair(gpu, oracle, 1000 + ns + nb, ns, nb)
     q = gpu.workload.configs(1, 65536 if (ns, nb) == (64, 0) else 8192)
     qf = q.astype(np.float32).T.copy()
@@ -54,17 +57,17 @@ def test_validate_states_bit_exact(gpu, oracle, lps, ns, nb):
     assert np.array_equal(got64, ref)
     if ns + nb == 0:
         assert ref.all()
-    gpu.set_tuning(validity_lanes_per_state=8)
+    gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2)
 
 
 @pytest.mark.parametrize("n", [1, 63, 65, 129, 1000])
 def test_validate_ragged_sizes(gpu, oracle, n):
     env, oenv, _ = env_pair(gpu, oracle, 7, 32, 32)
     q = gpu.workload.configs(n, n).astype(np.float32).T.copy()
-    for lps in (1, 8):
-        gpu.set_tuning(validity_lanes_per_state=lps)
+    for lps, ops in ((1, 1), (8, 1), (8, 8)):
+        gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops)
         assert np.array_equal(gpu.validate_states(env, to_dev(q)).cpu().numpy(), oracle.validate_states(oenv, q))
-    gpu.set_tuning(validity_lanes_per_state=8)
+    gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2)
 
 
 def test_is_valid_single_call(gpu, oracle):
@@ -79,14 +82,24 @@ def motion_case(gpu, seed, n, mu=-1.0, sigma=1.0, lo=0.05, hi=5.0):
     return a, b
 
 
-STRATEGIES = [(1, 8, 1), (1, 4, 4), (0, 8, 1), (0, 8, 8), (2, 8, 1), (2, 8, 4), (2, 4, 2), (2, 1, 1)]
+# (strategy, endpoint shape, walk shape)
+STRATEGIES = [(3, (8, 2), (8, 1)), (3, (8, 8), (8, 8)), (3, (1, 1), (1, 1)), (3, (4, 1), (8, 4)),
+              (1, (8, 1), (1, 1)), (1, (4, 1), (8, 2)), (0, (8, 1), (1, 1)), (0, (8, 1), (8, 8)),
+              (2, (8, 1), (1, 1)), (2, (8, 2), (8, 8)), (2, (8, 4), (8, 1))]
+DEFAULT = dict(motion_strategy=3, validity_lanes_per_state=8, validity_obstacle_parts=2,
+               motion_lanes_per_state=8, motion_obstacle_parts=1)
 
 
-@pytest.mark.parametrize("strategy,lps_end,lps_walk", STRATEGIES)
-def test_check_motions_config2_bit_exact(gpu, oracle, strategy, lps_end, lps_walk):
+def set_strategy(gpu, strategy, e, w):
+    gpu.set_tuning(motion_strategy=strategy, validity_lanes_per_state=e[0], validity_obstacle_parts=e[1],
+                   motion_lanes_per_state=w[0], motion_obstacle_parts=w[1])
+
+
+@pytest.mark.parametrize("strategy,eshape,wshape", STRATEGIES)
+def test_check_motions_config2_bit_exact(gpu, oracle, strategy, eshape, wshape):
     """Config: 16,384 edges, 128 spheres + 128 boxes, 0.01 rad resolution."""
-    print(f"motions strategy={strategy} lanes=({lps_end},{lps_walk})", flush=True)
-    gpu.set_tuning(motion_strategy=strategy, motion_lanes_per_state=lps_walk, validity_lanes_per_state=lps_end)
+    print(f"motions strategy={strategy} shapes={eshape},{wshape}", flush=True)
+    set_strategy(gpu, strategy, eshape, wshape)
     env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
     a, b = motion_case(gpu, 4, 16384)
     lvs = lvs_config()
@@ -97,13 +110,13 @@ def test_check_motions_config2_bit_exact(gpu, oracle, strategy, lps_end, lps_wal
     assert np.array_equal(c.cpu().numpy(), ref_c)
     v2 = gpu.check_motions(env, A, B, lvs)  # no count: early exit at any hit
     assert np.array_equal(v2.cpu().numpy(), ref_v)
-    gpu.set_tuning(motion_strategy=1, motion_lanes_per_state=1, validity_lanes_per_state=8)
+    gpu.set_tuning(**DEFAULT)
 
 
-@pytest.mark.parametrize("strategy,lps_end,lps_walk", STRATEGIES)
-def test_check_motions_sparse_env_deep_walks(gpu, oracle, strategy, lps_end, lps_walk):
+@pytest.mark.parametrize("strategy,eshape,wshape", STRATEGIES)
+def test_check_motions_sparse_env_deep_walks(gpu, oracle, strategy, eshape, wshape):
     """Few obstacles: most edges are valid, so every bisection position runs."""
-    gpu.set_tuning(motion_strategy=strategy, motion_lanes_per_state=lps_walk, validity_lanes_per_state=lps_end)
+    set_strategy(gpu, strategy, eshape, wshape)
     env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
     a, b = motion_case(gpu, 9, 3000, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
     lvs = lvs_config()
@@ -112,7 +125,7 @@ def test_check_motions_sparse_env_deep_walks(gpu, oracle, strategy, lps_end, lps
     v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, exact_count=True)
     assert np.array_equal(v.cpu().numpy(), ref_v)
     assert np.array_equal(c.cpu().numpy(), ref_c)
-    gpu.set_tuning(motion_strategy=1, motion_lanes_per_state=1, validity_lanes_per_state=8)
+    gpu.set_tuning(**DEFAULT)
 
 
 def test_check_motions_edge_cases(gpu, oracle):
@@ -126,12 +139,13 @@ def test_check_motions_edge_cases(gpu, oracle):
     lvs = lvs_config()
     for factor in (1, 2):
         ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs, factor)
-        for tp in (1, 0, 2):
-            gpu.set_tuning(motion_strategy=tp)
+        for tp in (3, 1, 0, 2):
+            gpu.set_tuning(**{**DEFAULT, "motion_strategy": tp, "motion_lanes_per_state": 8,
+                              "motion_obstacle_parts": 1, "validity_obstacle_parts": 1})
             v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, factor, exact_count=True)
             assert np.array_equal(v.cpu().numpy(), ref_v)
             assert np.array_equal(c.cpu().numpy(), ref_c)
-    gpu.set_tuning(motion_strategy=1)
+    gpu.set_tuning(**DEFAULT)
 
 
 def test_check_motion_plain_single_call(gpu, oracle):
@@ -142,9 +156,9 @@ def test_check_motion_plain_single_call(gpu, oracle):
         assert gpu.check_motion(env, a[i], b[i], lvs) == oracle.check_motion(oenv, a[i], b[i], lvs)[0]
 
 
-@pytest.mark.parametrize("lps", [1, 8])
-def test_check_motions_last_valid(gpu, oracle, lps):
-    gpu.set_tuning(motion_lanes_per_state=lps)
+@pytest.mark.parametrize("lps,ops", [(1, 1), (8, 1), (8, 8)])
+def test_check_motions_last_valid(gpu, oracle, lps, ops):
+    gpu.set_tuning(motion_lanes_per_state=lps, motion_obstacle_parts=ops)
     env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
     a, b = motion_case(gpu, 13, 2000, mu=-0.5)
     lvs = lvs_config()
@@ -161,7 +175,7 @@ def test_check_motions_last_valid(gpu, oracle, lps):
         ok, tt = gpu.check_motion_last_valid(env, a[i], b[i], lvs)
         ov, ot, _ = oracle.check_motion_last_valid(oenv, a[i], b[i], lvs)
         assert ok == ov and (ov or tt == ot)
-    gpu.set_tuning(motion_lanes_per_state=1)
+    gpu.set_tuning(**DEFAULT)
 
 
 def test_empty_batches(gpu):
