@@ -48,6 +48,97 @@ def percentile(xs, p):
     return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
 
 
+def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000):
+    """Poisson stream of RRT-Connect requests over seeded environments (16-1024
+    obstacles, log-uniform, half spheres / half boxes), served (a) by the
+    continuous-batching server and (b) by the plain path (host RRT-Connect, one
+    GPU launch + sync per check) on `plain_workers` threads.  Latency =
+    completion - arrival.  Returns a dict of p50/p99 latency and throughput."""
+    import concurrent.futures as cf
+    import threading
+
+    import torch
+
+    from ompl_amd import serve
+
+    rng = np.random.default_rng(seed)
+    lvs = lvs_0p01()
+    envs, starts = [], []
+    for e in range(n_envs):
+        n_obs = int(round(math.exp(rng.uniform(math.log(16), math.log(1024)))))
+        sph, box = oa.workload.obstacles(1000 + e, n_obs - n_obs // 2, n_obs // 2)
+        env = oa.Env(sph, box)
+        q = oa.workload.configs(2000 + e, 4096)
+        v = oa.validate_states(env, torch.from_numpy(q.T.copy()).cuda()).cpu().numpy()
+        ok = q[v == 1]
+        if len(ok) >= 2:
+            envs.append(env)
+            starts.append(ok)
+    reqs = []
+    for i in range(n_requests):
+        k = i % len(envs)
+        cand = starts[k]
+        a, b = cand[rng.integers(len(cand))], cand[rng.integers(len(cand))]
+        reqs.append(serve.make_request(envs[k], a, b, seed=10_000 + i, lvs=lvs, max_iters=max_iters))
+    gaps = rng.exponential(1.0 / qps, n_requests)
+    arrivals = np.cumsum(gaps)
+
+    def pct(xs):
+        return {"p50": float(np.percentile(xs, 50)), "p99": float(np.percentile(xs, 99)),
+                "mean": float(np.mean(xs))}
+
+    # (a) served
+    srv = serve.Server(hbm_budget_bytes=4 << 30, max_active=1024, max_nodes=4096)
+    ids = [None] * n_requests
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        dt = arrivals[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids[i] = srv.submit(r)
+    res_s = [srv.wait(rid, 600_000) for rid in ids]
+    span_s = time.perf_counter() - t0
+    stats = srv.stats()
+    lat_s = [r.latency_ms for r in res_s]
+    paths_s = [srv.path(rid) for rid in ids]
+    srv.close()
+
+    # (b) plain: FIFO queue drained by plain_workers threads
+    lat_p = [0.0] * n_requests
+    res_p = [None] * n_requests
+    lock = threading.Lock()
+
+    def work(i, arrive):
+        res, path = serve.plan_plain(reqs[i])
+        with lock:
+            lat_p[i] = (time.perf_counter() - arrive) * 1e3
+            res_p[i] = (res, path)
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=plain_workers) as ex:
+        futs = []
+        for i in range(n_requests):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(ex.submit(work, i, t0 + arrivals[i]))
+        for f in futs:
+            f.result()
+    span_p = time.perf_counter() - t0
+    same = all(res_p[i][0].status == res_s[i].status and np.array_equal(res_p[i][1], paths_s[i])
+               for i in range(n_requests))
+    solved = sum(r.status == serve.SOLVED for r in res_s)
+    return {
+        "requests": n_requests, "qps": qps, "envs_used": len(envs), "envs_drawn": n_envs,
+        "solved": solved, "max_iters": max_iters,
+        "served": {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "ticks": stats["ticks"],
+                   "grow_steps": stats["steps"]},
+        "plain": {"latency_ms": pct(lat_p), "throughput_rps": n_requests / span_p, "workers": plain_workers},
+        "p99_plain_over_served": float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)),
+        "identical_plans": bool(same),
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +149,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tuning", type=str, default=None, help="JSON dict of launch knobs (skips the autotuner)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="all-cores CPU run (0 = os.cpu_count, max 16)")
+    ap.add_argument("--stream-requests", type=int, default=256, help="served-stream requests (0 = skip)")
+    ap.add_argument("--stream-qps", type=float, default=200.0)
+    ap.add_argument("--stream-envs", type=int, default=32)
+    ap.add_argument("--plain-workers", type=int, default=8)
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -124,7 +219,7 @@ def main(argv=None):
         oa.check_motions(env, A, B, lvs, out=out)
     torch.cuda.synchronize()
     kernels = {}
-    for name in ("motion_fused", "motion_endpoint", "motion_walk"):
+    for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk"):
         n, ms = oa.profile.query(name)
         if n:
             kernels[name] = {"launches": n, "avg_us": 1e3 * ms / n}
@@ -136,7 +231,8 @@ def main(argv=None):
     flops_step = oa.work_flops(work)
     w_end = oa.validity_work(env, B.float().contiguous())  # the s2 checks of every edge
     flops_end = oa.work_flops(w_end)
-    share = {"motion_fused": flops_step, "motion_endpoint": flops_end, "motion_walk": flops_step - flops_end}
+    share = {"motion_fused": flops_step, "motion_endpoint": flops_end, "motion_walk": flops_step - flops_end,
+             "motion_chunk": flops_step - flops_end}
     if t["motion_strategy"] == 0:
         share["motion_walk"] = flops_step
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
@@ -181,8 +277,12 @@ def main(argv=None):
         cpu_all = {"value": done / cpu_s, "unit": "edge-checks/s", "cores": nthr, "kind": "port",
                    "sample": f"{done // N_EDGES} full passes, {cpu_s:.1f} s, POSIX threads over edges"}
 
+    stream = None
+    if args.stream_requests > 0:
+        stream = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers)
+
     res = {
-        "metric": "edge-checks/s (batched DiscreteMotionValidator, bit-exact)",
+        "metric": "edge-checks/s + p50/p99 plan latency, served vs plain GPU path, bit-exact",
         "value": value,
         "unit": "edge-checks/s",
         "n_gpus": 1,
@@ -212,6 +312,7 @@ def main(argv=None):
         "cpu_all_cores": cpu_all,
         "autotuned_us_per_batch": tuned_us,
         "parity_bit_exact": parity,
+        "served_stream": stream,
         "wall_s": wall,
     }
     print(json.dumps(res))

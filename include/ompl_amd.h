@@ -217,6 +217,74 @@ int ompl_amd_nn_nearest(ompl_amd_nn nn, const float *q, int64_t *out_id);
 double ompl_amd_rrtstar_radius(int dim, double measure, double rewire_factor, double max_distance, int64_t n);
 
 /* ------------------------------------------------------------------ */
+/* Planning server (continuous batching) -- RRT-Connect requests        */
+/* (geometric/planners/rrt/src/RRTConnect.cpp:116-196, 245-330) on the  */
+/* 7-DOF arm with device-resident trees.  Every tick advances each      */
+/* active request by one growTree step in a single launch; requests are */
+/* admitted between ticks while their tree memory fits the HBM budget.  */
+/* A request's result does not depend on the other requests it shares   */
+/* ticks with.  No counterpart in the reference (it plans one query per */
+/* thread).                                                             */
+/* ------------------------------------------------------------------ */
+#define OMPL_AMD_PLAN_QUEUED 0
+#define OMPL_AMD_PLAN_RUNNING 1
+#define OMPL_AMD_PLAN_SOLVED 2
+#define OMPL_AMD_PLAN_FAILED 3    /* max_iters reached without a solution */
+#define OMPL_AMD_PLAN_DEADLINE 4  /* deadline_ms expired (queued or running) */
+#define OMPL_AMD_PLAN_CANCELLED 5
+#define OMPL_AMD_PLAN_REJECTED 6  /* footprint larger than the whole budget */
+#define OMPL_AMD_PLAN_TREE_FULL 7 /* a tree reached max_nodes */
+
+typedef struct ompl_amd_server_s *ompl_amd_server;
+
+typedef struct {
+    size_t hbm_budget_bytes; /* admission limit on the sum of active requests' tree memory */
+    int max_active;          /* concurrent requests (tree slots preallocated) */
+    int max_nodes;           /* nodes per tree per request */
+} ompl_amd_server_config;
+
+typedef struct {
+    ompl_amd_env env;
+    double start[OMPL_AMD_ARM_DOF];
+    double goal[OMPL_AMD_ARM_DOF];
+    uint32_t seed;                /* local seed of the request's state sampler RNG */
+    int max_iters;                /* RRT-Connect iterations (the termination condition) */
+    double range;                 /* maxDistance_; <= 0: 0.2 * maximum extent (SelfConfig) */
+    double longest_valid_segment; /* DiscreteMotionValidator resolution */
+    unsigned factor;
+    double deadline_ms;           /* <= 0: none */
+    int max_nodes;                /* <= 0: the server's max_nodes; footprint = 2*max_nodes*60 B */
+} ompl_amd_plan_request;
+
+typedef struct {
+    int status;
+    int iterations;
+    int path_len;       /* states in the solution path */
+    int64_t tree_nodes; /* nodes in both trees */
+    double queued_ms;   /* submit -> admission */
+    double run_ms;      /* admission -> finish */
+    double latency_ms;  /* submit -> finish */
+} ompl_amd_plan_result;
+
+int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *out);
+int ompl_amd_server_destroy(ompl_amd_server server);
+int ompl_amd_server_submit(ompl_amd_server server, const ompl_amd_plan_request *req, int64_t *request_id);
+int ompl_amd_server_poll(ompl_amd_server server, int64_t request_id, ompl_amd_plan_result *res);
+/* Blocks until the request finishes or timeout_ms passes (< 0: no timeout). */
+int ompl_amd_server_wait(ompl_amd_server server, int64_t request_id, double timeout_ms, ompl_amd_plan_result *res);
+/* Solution path, row-major states (up to cap states); *n = full length. */
+int ompl_amd_server_path(ompl_amd_server server, int64_t request_id, double *out, int cap, int *n);
+int ompl_amd_server_cancel(ompl_amd_server server, int64_t request_id);
+/* Frees a finished request's record. */
+int ompl_amd_server_release(ompl_amd_server server, int64_t request_id);
+/* out: ticks, growTree steps, active, queued, used bytes, budget bytes */
+int ompl_amd_server_stats(ompl_amd_server server, int64_t out[6]);
+/* The same RRT-Connect on the host: host nearest neighbours, one GPU launch
+ * and synchronisation per isValid / checkMotion call (the plain path). */
+int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *res, double *path, int cap,
+                        int *path_len);
+
+/* ------------------------------------------------------------------ */
 /* Synthetic workloads (std::mt19937 + OMPL RNG semantics, host side)   */
 /* ------------------------------------------------------------------ */
 /* Obstacles: centre U[-1,1]^3 then sphere radius U[r_lo, r_hi] (or box

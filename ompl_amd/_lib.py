@@ -66,6 +66,16 @@ SIGNATURES = {
     "ompl_amd_nn_radius": (C.c_int, [vp, vp, C.c_int64, C.c_float, vp, vp, vp, C.c_int64, c_i64_p]),
     "ompl_amd_nn_nearest": (C.c_int, [vp, vp, c_i64_p]),
     "ompl_amd_rrtstar_radius": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64]),
+    "ompl_amd_server_create": (C.c_int, [vp, C.POINTER(vp)]),
+    "ompl_amd_server_destroy": (C.c_int, [vp]),
+    "ompl_amd_server_submit": (C.c_int, [vp, vp, c_i64_p]),
+    "ompl_amd_server_poll": (C.c_int, [vp, C.c_int64, vp]),
+    "ompl_amd_server_wait": (C.c_int, [vp, C.c_int64, C.c_double, vp]),
+    "ompl_amd_server_path": (C.c_int, [vp, C.c_int64, vp, C.c_int, c_int_p]),
+    "ompl_amd_server_cancel": (C.c_int, [vp, C.c_int64]),
+    "ompl_amd_server_release": (C.c_int, [vp, C.c_int64]),
+    "ompl_amd_server_stats": (C.c_int, [vp, vp]),
+    "ompl_amd_plan_plain": (C.c_int, [vp, vp, vp, C.c_int, c_int_p]),
     "ompl_amd_workload_obstacles": (C.c_int, [C.c_uint32, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                               C.c_double, C.c_double, vp, vp]),
     "ompl_amd_workload_configs": (C.c_int, [C.c_uint32, C.c_int64, vp]),

@@ -18,7 +18,6 @@
 //                           (chunk b, survivor s) in chunk-major order over a
 //                           persistent grid; a chunk is skipped once an earlier
 //                           position of its edge is known invalid (atomicMin)
-//   motion_finalize_kernel  survivors' results from their first invalid position
 //   motion_fused_kernel     s2 of the wave's edges, then the wave walks its own
 //                           survivors (one launch)
 //   motion_single_kernel    one edge passed by value (the plain per-call path)
@@ -134,6 +133,7 @@ __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
             const int slot = base + __popcll(sm & (((uint64_t)1 << lane) - 1));
             A.list[slot] = (int32_t)e;
             if (A.first_bad) A.first_bad[slot] = INT_MAX;
+            if (A.surv_nd) A.surv_nd[slot] = m + 1;
             if (A.max_m) atomicMax(A.max_m, m);
         }
     }
@@ -175,51 +175,72 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
     }
 }
 
+// Chunk-parallel walk of the survivors.  Virtual work items k = (chunk blk,
+// survivor s) in chunk-major order; a wave examines 64 consecutive items at
+// once (one per lane: is the chunk inside the edge and no earlier position of
+// that edge known invalid?), then walks the live ones with all its lanes.
+// The wave that finishes last derives every survivor's result and resets the
+// counters for the next batch (no separate finalize or clear launch).
 template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
     constexpr int G = GroupShape<LPS, OPS>::STATES;
+    const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
-    const int64_t n_surv = *A.list_count;
-    if (n_surv == 0) return;
-    const int64_t nblk = ((int64_t)*A.max_m + G - 1) / G;
-    const int64_t items = n_surv * nblk;
-    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < items; k += waves_total) {
-        const int64_t blk = k / n_surv, s = k - blk * n_surv;
-        const int base = 1 + (int)(blk * G);
-        if (*(volatile int32_t *)&A.first_bad[s] < base) continue; // an earlier position failed
-        const int64_t e = A.list[s];
-        double a[kDof], b[kDof];
-#pragma unroll
-        for (int j = 0; j < kDof; ++j) {
-            a[j] = A.a[(int64_t)j * A.n + e];
-            b[j] = A.b[(int64_t)j * A.n + e];
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n_surv = *A.list_count;
+    const int nblk = (*A.max_m + G - 1) / G;
+    const int64_t items = (int64_t)n_surv * nblk;
+    for (int64_t g0 = wave * 64; g0 < items; g0 += waves_total * 64) {
+        const int64_t k = g0 + lane;
+        int s = 0, base = 0;
+        bool live = k < items;
+        if (live) {
+            const int blk = (int)(k / n_surv);
+            s = (int)(k - (int64_t)blk * n_surv);
+            base = 1 + blk * G;
+            live = base < A.surv_nd[s] &&
+                   __hip_atomic_load(&A.first_bad[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= base;
         }
-        const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
-        const int total = nd; // positions 0..nd-1 (nd >= 2 for survivors)
-        if (base >= total) continue;
-        const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, nd, total, base, lds, A.env.ns, A.env.nb);
-        if (bad != INT_MAX && lane_id() == 0) atomicMin(&A.first_bad[s], bad);
-    }
-}
-
-__global__ void motion_finalize_kernel(MotionArgs A)
-{
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= *A.list_count) return;
-    const int64_t e = A.list[s];
-    double a[kDof], b[kDof];
+        uint64_t mask = wave_ballot(live);
+        while (mask) {
+            const int j = lowest_bit(mask);
+            mask &= mask - 1;
+            const int sj = __shfl(s, j), bj = __shfl(base, j);
+            if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) continue;
+            const int64_t e = A.list[sj];
+            const int nd = A.surv_nd[sj];
+            double a[kDof], b[kDof];
 #pragma unroll
-    for (int j = 0; j < kDof; ++j) {
-        a[j] = A.a[(int64_t)j * A.n + e];
-        b[j] = A.b[(int64_t)j * A.n + e];
+            for (int d = 0; d < kDof; ++d) {
+                a[d] = A.a[(int64_t)d * A.n + e];
+                b[d] = A.b[(int64_t)d * A.n + e];
+            }
+            const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, nd, nd, bj, lds, A.env.ns, A.env.nb);
+            if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[sj], bad);
+        }
     }
-    const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
-    const int fb = A.first_bad[s];
-    A.out[e] = fb == INT_MAX ? 1 : 0;
-    if (A.checked) A.checked[e] = fb == INT_MAX ? nd : fb + 1;
+    // last wave out: finalize the survivors and clear the counters
+    int last = 0;
+    if (lane == 0) {
+        __threadfence();
+        last = atomicAdd(A.done_waves, 1) == (int)waves_total - 1;
+    }
+    if (!__shfl(last, 0)) return;
+    __threadfence();
+    for (int s = lane; s < n_surv; s += 64) {
+        const int64_t e = A.list[s];
+        const int fb = __hip_atomic_load(&A.first_bad[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        A.out[e] = fb == INT_MAX ? 1 : 0;
+        if (A.checked) A.checked[e] = fb == INT_MAX ? A.surv_nd[s] : fb + 1;
+    }
+    if (lane == 0) {
+        *A.list_count = 0;
+        *A.max_m = 0;
+        *A.done_waves = 0;
+    }
 }
 
 // Fused batch: each wave checks s2 of its edges (endpoint shape LE x OE), then
@@ -463,13 +484,6 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
     OA_SHAPES(OA_LAUNCH)
 #undef OA_LAUNCH
     fail(OMPL_AMD_ERR_ARG, "chunk shape (%d, %d) not built", lps, ops);
-}
-
-void launch_motion_finalize(const MotionArgs &A, int64_t n_max, hipStream_t s)
-{
-    if (n_max <= 0) return;
-    hipLaunchKernelGGL(motion_finalize_kernel, dim3(grid_for(n_max, 256)), dim3(256), 0, s, A);
-    OA_HIP(hipGetLastError());
 }
 
 bool fused_supported(int le, int oe, int lw, int ow)

@@ -107,13 +107,19 @@ Workspace &workspace(hipStream_t s, size_t list_entries)
     if (!w) {
         w = new Workspace();
         OA_HIP(hipMalloc(&w->counters, 64 * sizeof(int32_t)));
+        OA_HIP(hipMemset(w->counters, 0, 64 * sizeof(int32_t)));
+        OA_HIP(hipDeviceSynchronize());
     }
     if (w->list_cap < list_entries) {
+        // the stream may still use the old arrays
+        OA_HIP(hipStreamSynchronize(s));
         if (w->list) OA_HIP(hipFree(w->list));
         if (w->first_bad) OA_HIP(hipFree(w->first_bad));
+        if (w->surv_nd) OA_HIP(hipFree(w->surv_nd));
         size_t cap = list_entries < 1024 ? 1024 : list_entries;
         OA_HIP(hipMalloc(&w->list, cap * sizeof(int32_t)));
         OA_HIP(hipMalloc(&w->first_bad, cap * sizeof(int32_t)));
+        OA_HIP(hipMalloc(&w->surv_nd, cap * sizeof(int32_t)));
         w->list_cap = cap;
     }
     return *w;

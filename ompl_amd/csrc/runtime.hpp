@@ -17,8 +17,12 @@ void init_device(int dev);
 struct Workspace {
     int32_t *list = nullptr;
     int32_t *first_bad = nullptr; // per survivor (chunk-parallel walk)
+    int32_t *surv_nd = nullptr;   // per survivor segment count (chunk-parallel walk)
     size_t list_cap = 0;
-    int32_t *counters = nullptr; // [0] survivor count, [1] walk queue, [2] max positions
+    // [0] survivor count, [1] walk queue (strategy 1, cleared per call);
+    // [8] survivor count, [9] max positions, [10] finished waves (strategy 3,
+    // zero at creation and reset by the kernel itself)
+    int32_t *counters = nullptr;
 };
 Workspace &workspace(hipStream_t s, size_t list_entries);
 

@@ -1,0 +1,127 @@
+"""Python client of the planning server (continuous batching of RRT-Connect
+requests, include/ompl_amd.h) and of the plain one-launch-per-call planner."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import call
+
+DOF = 7
+QUEUED, RUNNING, SOLVED, FAILED, DEADLINE, CANCELLED, REJECTED, TREE_FULL = range(8)
+STATUS = {0: "queued", 1: "running", 2: "solved", 3: "failed", 4: "deadline", 5: "cancelled", 6: "rejected",
+          7: "tree_full"}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("max_nodes", C.c_int)]
+
+
+class PlanRequest(C.Structure):
+    _fields_ = [
+        ("env", C.c_void_p),
+        ("start", C.c_double * DOF),
+        ("goal", C.c_double * DOF),
+        ("seed", C.c_uint32),
+        ("max_iters", C.c_int),
+        ("range", C.c_double),
+        ("longest_valid_segment", C.c_double),
+        ("factor", C.c_uint),
+        ("deadline_ms", C.c_double),
+        ("max_nodes", C.c_int),
+    ]
+
+
+class PlanResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int),
+        ("iterations", C.c_int),
+        ("path_len", C.c_int),
+        ("tree_nodes", C.c_int64),
+        ("queued_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("latency_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["status_name"] = STATUS.get(self.status, "?")
+        return d
+
+
+def make_request(env, start, goal, seed, lvs, max_iters=5000, rng=0.0, factor=1, deadline_ms=0.0, max_nodes=0):
+    r = PlanRequest()
+    r.env = env.handle.value if hasattr(env, "handle") else env
+    for k in range(DOF):
+        r.start[k] = float(start[k])
+        r.goal[k] = float(goal[k])
+    r.seed = int(seed)
+    r.max_iters = int(max_iters)
+    r.range = float(rng)
+    r.longest_valid_segment = float(lvs)
+    r.factor = int(factor)
+    r.deadline_ms = float(deadline_ms)
+    r.max_nodes = int(max_nodes)
+    return r
+
+
+class Server:
+    def __init__(self, hbm_budget_bytes=1 << 30, max_active=256, max_nodes=8192):
+        cfg = ServerConfig(hbm_budget_bytes, max_active, max_nodes)
+        h = C.c_void_p()
+        call("ompl_amd_server_create", C.byref(cfg), C.byref(h))
+        self.handle = h
+
+    def submit(self, req: PlanRequest) -> int:
+        rid = C.c_int64()
+        call("ompl_amd_server_submit", self.handle, C.byref(req), C.byref(rid))
+        return rid.value
+
+    def poll(self, rid: int) -> PlanResult:
+        r = PlanResult()
+        call("ompl_amd_server_poll", self.handle, rid, C.byref(r))
+        return r
+
+    def wait(self, rid: int, timeout_ms: float = -1.0) -> PlanResult:
+        r = PlanResult()
+        call("ompl_amd_server_wait", self.handle, rid, float(timeout_ms), C.byref(r))
+        return r
+
+    def path(self, rid: int) -> np.ndarray:
+        n = C.c_int()
+        call("ompl_amd_server_path", self.handle, rid, None, 0, C.byref(n))
+        out = np.zeros((max(n.value, 1), DOF), np.float64)
+        call("ompl_amd_server_path", self.handle, rid, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n))
+        return out[:n.value]
+
+    def cancel(self, rid: int) -> None:
+        call("ompl_amd_server_cancel", self.handle, rid)
+
+    def release(self, rid: int) -> None:
+        call("ompl_amd_server_release", self.handle, rid)
+
+    def stats(self) -> dict:
+        out = (C.c_int64 * 6)()
+        call("ompl_amd_server_stats", self.handle, out)
+        return dict(zip(("ticks", "steps", "active", "queued", "used_bytes", "budget_bytes"), list(out)))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            call("ompl_amd_server_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_plain(req: PlanRequest, cap: int = 4096):
+    """RRT-Connect with one GPU launch + sync per isValid / checkMotion call."""
+    res = PlanResult()
+    path = np.zeros((cap, DOF), np.float64)
+    n = C.c_int()
+    call("ompl_amd_plan_plain", C.byref(req), C.byref(res), path.ctypes.data_as(C.c_void_p), cap, C.byref(n))
+    return res, path[:min(n.value, cap)]

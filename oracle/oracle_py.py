@@ -86,6 +86,9 @@ def lib():
                                               C.c_int]
         L.orc_validate_states_batch.argtypes = [C.POINTER(Env), vp, C.c_long, vp, C.c_int]
         L.orc_knn_batch.argtypes = [C.c_int, vp, C.c_long, C.c_int, vp, C.c_long, C.c_int, vp, vp, C.c_int]
+        L.orc_rrt_connect.argtypes = [C.POINTER(Env), vp, vp, C.c_uint32, C.c_double, C.c_double, C.c_uint,
+                                      C.c_int, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -182,6 +185,18 @@ def check_motions(env: OracleEnv, a_rows: np.ndarray, b_rows: np.ndarray, lvs, f
     chk = np.zeros(n, np.int32)
     lib().orc_check_motions_batch(env.ref(), _p(a), _p(b), n, lvs, factor, _p(out), _p(chk), threads)
     return out, chk
+
+
+def rrt_connect(env: OracleEnv, start, goal, seed: int, max_distance: float, lvs: float, factor: int = 1,
+                max_iters: int = 5000, cap: int = 4096):
+    """RRT-Connect restatement -> (solved, iterations, path[n][7], tree_nodes)."""
+    s = np.ascontiguousarray(start, np.float64)
+    g = np.ascontiguousarray(goal, np.float64)
+    path = np.zeros((cap, 7), np.float64)
+    n, it, nodes = C.c_int(), C.c_int(), C.c_long()
+    ok = lib().orc_rrt_connect(env.ref(), _p(s), _p(g), seed, max_distance, lvs, factor, max_iters, _p(path), cap,
+                               C.byref(n), C.byref(it), C.byref(nodes))
+    return bool(ok), it.value, path[:min(n.value, cap)].copy(), nodes.value
 
 
 def validate_states_mt(env: OracleEnv, q_soa: np.ndarray, threads: int) -> np.ndarray:

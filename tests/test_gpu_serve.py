@@ -1,0 +1,125 @@
+"""GPU tests of the planning server (continuous batching) and the plain
+per-call planner: plans equal the CPU RRT-Connect restatement, each request's
+result is the same alone and in the mix, and the failure paths (deadline,
+admission, cancellation, tree capacity) are forced from the host."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+MAXD = 0.2 * 2 * math.pi * math.sqrt(7)
+
+
+def lvs_config():
+    ext = 2 * math.pi * math.sqrt(7)
+    return ext * (0.01 / ext)
+
+
+def valid_pairs(gpu, oracle, oenv, n, seed):
+    """Valid start/goal pairs drawn from seeded configurations."""
+    q = gpu.workload.configs(seed, 4000)
+    ok = [row for row in q if oracle.state_valid(oenv, row.astype(np.float32))]
+    assert len(ok) >= 2 * n
+    return [(ok[2 * i], ok[2 * i + 1]) for i in range(n)]
+
+
+@pytest.fixture(scope="module")
+def scene(gpu, oracle):
+    s, b = gpu.workload.obstacles(1020, 12, 12)
+    env = gpu.Env(s, b)
+    oenv = oracle.OracleEnv(s, b)
+    pairs = valid_pairs(gpu, oracle, oenv, 10, 77)
+    return env, oenv, pairs
+
+
+def reqs_for(gpu, env, pairs, max_iters=3000):
+    from ompl_amd.serve import make_request
+
+    return [make_request(env, a, b, seed=100 + i, lvs=lvs_config(), max_iters=max_iters) for i, (a, b) in
+            enumerate(pairs)]
+
+
+def test_plain_planner_matches_oracle(gpu, oracle, scene):
+    from ompl_amd.serve import SOLVED, plan_plain
+
+    env, oenv, pairs = scene
+    for i, r in enumerate(reqs_for(gpu, env, pairs[:5])):
+        res, path = plan_plain(r)
+        ok, it, opath, nodes = oracle.rrt_connect(oenv, pairs[i][0], pairs[i][1], 100 + i, MAXD, lvs_config(),
+                                                  max_iters=3000)
+        print(f"plain request {i}: status {res.status} iters {res.iterations} oracle {ok} {it}", flush=True)
+        assert (res.status == SOLVED) == ok
+        assert res.iterations == it and res.tree_nodes == nodes
+        assert np.array_equal(path, opath)
+
+
+def test_served_equals_oracle_alone_and_in_mix(gpu, oracle, scene):
+    from ompl_amd.serve import SOLVED, Server
+
+    env, oenv, pairs = scene
+    reqs = reqs_for(gpu, env, pairs)
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=64, max_nodes=8192)
+    alone = []
+    for r in reqs:  # one at a time
+        rid = srv.submit(r)
+        res = srv.wait(rid, 60000)
+        alone.append((res.status, res.iterations, res.tree_nodes, srv.path(rid)))
+        srv.release(rid)
+    ids = [srv.submit(r) for r in reqs]  # all together
+    mix = []
+    for rid in ids:
+        res = srv.wait(rid, 60000)
+        mix.append((res.status, res.iterations, res.tree_nodes, srv.path(rid)))
+    for i, (a, m) in enumerate(zip(alone, mix)):
+        assert a[0] == m[0] and a[1] == m[1] and a[2] == m[2] and np.array_equal(a[3], m[3]), i
+        ok, it, opath, nodes = oracle.rrt_connect(oenv, pairs[i][0], pairs[i][1], 100 + i, MAXD, lvs_config(),
+                                                  max_iters=3000)
+        assert (a[0] == SOLVED) == ok and a[1] == it and a[2] == nodes
+        assert np.array_equal(a[3], opath)
+    st = srv.stats()
+    assert st["active"] == 0 and st["queued"] == 0 and st["used_bytes"] == 0
+    assert sum(m[0] == SOLVED for m in mix) >= 5
+    srv.close()
+
+
+def test_continuous_batching_more_requests_than_slots(gpu, oracle, scene):
+    from ompl_amd.serve import SOLVED, FAILED, Server
+
+    env, oenv, pairs = scene
+    reqs = reqs_for(gpu, env, pairs * 3)
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=4, max_nodes=4096)
+    ids = [srv.submit(r) for r in reqs]
+    results = [srv.wait(rid, 120000) for rid in ids]
+    assert all(r.status in (SOLVED, FAILED) for r in results)
+    # identical requests (same pair and seed) give identical plans regardless of admission order
+    for i in range(len(pairs)):
+        a, b = ids[i], ids[i + len(pairs)]
+        assert np.array_equal(srv.path(a), srv.path(b)) and results[i].iterations == results[i + len(pairs)].iterations
+    srv.close()
+
+
+def test_failure_paths_forced_from_host(gpu, scene):
+    from ompl_amd.serve import CANCELLED, DEADLINE, REJECTED, TREE_FULL, Server, make_request
+
+    env, _, pairs = scene
+    a, b = pairs[0]
+    lvs = lvs_config()
+    # admission: a single request's footprint exceeds the whole budget
+    srv = Server(hbm_budget_bytes=1000, max_active=4, max_nodes=1024)
+    rid = srv.submit(make_request(env, a, b, 1, lvs))
+    assert srv.wait(rid, 1000).status == REJECTED
+    srv.close()
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=2, max_nodes=1024)
+    # deadline already expired
+    rid = srv.submit(make_request(env, a, b, 1, lvs, max_iters=10 ** 7, deadline_ms=1e-6))
+    assert srv.wait(rid, 5000).status == DEADLINE
+    # tree capacity: 3 nodes per tree cannot hold a plan
+    rid = srv.submit(make_request(env, a, b, 2, lvs, max_iters=10 ** 6, max_nodes=3))
+    assert srv.wait(rid, 5000).status == TREE_FULL
+    # cancellation of a long request
+    rid = srv.submit(make_request(env, a, np.array(a) + 1e-3, 3, lvs, max_iters=10 ** 7, rng=1e-3))
+    srv.cancel(rid)
+    assert srv.wait(rid, 5000).status == CANCELLED
+    assert srv.stats()["used_bytes"] == 0
+    srv.close()
