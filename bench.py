@@ -277,9 +277,9 @@ def main(argv=None):
         cpu_all = {"value": done / cpu_s, "unit": "edge-checks/s", "cores": nthr, "kind": "port",
                    "sample": f"{done // N_EDGES} full passes, {cpu_s:.1f} s, POSIX threads over edges"}
 
-    stream = None
+    stream_res = None
     if args.stream_requests > 0:
-        stream = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers)
+        stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers)
 
     res = {
         "metric": "edge-checks/s + p50/p99 plan latency, served vs plain GPU path, bit-exact",
@@ -312,7 +312,7 @@ def main(argv=None):
         "cpu_all_cores": cpu_all,
         "autotuned_us_per_batch": tuned_us,
         "parity_bit_exact": parity,
-        "served_stream": stream,
+        "served_stream": stream_res,
         "wall_s": wall,
     }
     print(json.dumps(res))

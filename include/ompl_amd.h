@@ -4,7 +4,8 @@
  *
  * Every function returns OMPL_AMD_OK (0) or a negative status; the message of
  * the last failure on the calling thread is ompl_amd_last_error().  The C++
- * adapters (ompl_amd/include/ompl_amd/ompl_adapters.hpp) rethrow failures as
+ * adapter ompl_amd/include/ompl_amd/nearest_neighbors_gpu.hpp (and checker /
+ * validator bindings written the same way) rethrows failures as
  * ompl::Exception, the reference's error type (util/Exception.h:46).
  *
  * Pointer conventions: parameters named *_dev are device pointers (HBM);

@@ -37,18 +37,18 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
     const int LW = t.motion_lanes_per_state, OW = t.motion_obstacle_parts;
     const int64_t persistent_waves = (int64_t)num_cus() * t.motion_waves_per_cu;
     if (t.motion_strategy == 3) {
-        // counters [8..10] belong to this strategy only: zero when the
-        // workspace is created, reset by the last wave of every chunk launch
+        // counters [8..9] belong to this strategy only: zero when the
+        // workspace is created, cleared by the finalize kernel of every batch
         Workspace &w = workspace(s, (size_t)n);
         A.list = w.list;
         A.list_count = w.counters + 8;
         A.max_m = w.counters + 9;
-        A.done_waves = w.counters + 10;
         A.first_bad = w.first_bad;
         A.surv_nd = w.surv_nd;
         A.start = 1;
         launch_motion_endpoint(A, LE, OE, t.validity_block, s);
         launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s);
+        launch_motion_finalize(A, s);
     } else if (t.motion_strategy == 2) {
         A.start = 1;
         launch_motion_fused(A, exact, LE, OE, LW, OW, t.motion_block, s);

@@ -18,6 +18,7 @@
 //                           (chunk b, survivor s) in chunk-major order over a
 //                           persistent grid; a chunk is skipped once an earlier
 //                           position of its edge is known invalid (atomicMin)
+//   motion_finalize_kernel  survivors' results; clears the counters
 //   motion_fused_kernel     s2 of the wave's edges, then the wave walks its own
 //                           survivors (one launch)
 //   motion_single_kernel    one edge passed by value (the plain per-call path)
@@ -179,8 +180,6 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 // survivor s) in chunk-major order; a wave examines 64 consecutive items at
 // once (one per lane: is the chunk inside the edge and no earlier position of
 // that edge known invalid?), then walks the live ones with all its lanes.
-// The wave that finishes last derives every survivor's result and resets the
-// counters for the next batch (no separate finalize or clear launch).
 template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
@@ -222,24 +221,23 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
             if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[sj], bad);
         }
     }
-    // last wave out: finalize the survivors and clear the counters
-    int last = 0;
-    if (lane == 0) {
-        __threadfence();
-        last = atomicAdd(A.done_waves, 1) == (int)waves_total - 1;
-    }
-    if (!__shfl(last, 0)) return;
-    __threadfence();
-    for (int s = lane; s < n_surv; s += 64) {
+}
+
+// Survivors' results from their first invalid position; one block, which then
+// clears the strategy's counters for the next batch (no memset launch).
+__global__ void __launch_bounds__(1024) motion_finalize_kernel(MotionArgs A)
+{
+    const int n_surv = *A.list_count;
+    for (int s = threadIdx.x; s < n_surv; s += blockDim.x) {
         const int64_t e = A.list[s];
-        const int fb = __hip_atomic_load(&A.first_bad[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int fb = A.first_bad[s];
         A.out[e] = fb == INT_MAX ? 1 : 0;
         if (A.checked) A.checked[e] = fb == INT_MAX ? A.surv_nd[s] : fb + 1;
     }
-    if (lane == 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
         *A.list_count = 0;
         *A.max_m = 0;
-        *A.done_waves = 0;
     }
 }
 
@@ -465,6 +463,12 @@ void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int b
     OA_SHAPES(OA_LAUNCH)
 #undef OA_LAUNCH
     fail(OMPL_AMD_ERR_ARG, "walk shape (%d, %d) not built", lps, ops);
+}
+
+void launch_motion_finalize(const MotionArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL(motion_finalize_kernel, dim3(1), dim3(1024), 0, s, A);
+    OA_HIP(hipGetLastError());
 }
 
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,

@@ -25,7 +25,6 @@ struct MotionArgs {
     int32_t *first_bad;  // per survivor: first invalid position (chunk-parallel walk)
     int32_t *max_m;      // largest bisection position count among survivors
     int32_t *surv_nd;    // per survivor: segment count nd
-    int32_t *done_waves; // chunk-parallel walk: waves finished (last one finalizes)
     int start;           // first walk position (0 = s2, 1 = skip s2)
 };
 
@@ -49,6 +48,7 @@ void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hi
 void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves, hipStream_t s);
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
                           hipStream_t s);
+void launch_motion_finalize(const MotionArgs &A, hipStream_t s);
 void launch_motion_fused(const MotionArgs &A, bool exact, int le, int oe, int lw, int ow, int block, hipStream_t s);
 void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result,
                           hipStream_t s);

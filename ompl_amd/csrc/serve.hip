@@ -6,9 +6,11 @@
 // states + parent indices).  A server thread runs ticks: every tick each
 // active request contributes exactly one growTree step -- nearest neighbour in
 // its tree, steering by the planner range, motion check in the direction of
-// growth, append -- and one kernel launch executes all of them (one wave per
-// request, the request's obstacles staged in that block's LDS).  The host then
-// advances each request's RRT-Connect state machine with the step's outcome.
+// growth, append -- executed for all requests by three launches: nearest +
+// steer (one wave per request), the motion checks of every request
+// chunk-parallel over the whole GPU (so one long valid motion in a dense
+// environment does not hold the tick for its single wave), and commit.  The
+// host then advances each request's RRT-Connect state machine.
 // New requests are admitted between ticks while their HBM footprint fits the
 // budget (continuous batching); a request's result does not depend on which
 // other requests share its ticks.
@@ -58,17 +60,36 @@ struct GrowRes {
     double xstate[kDof];
 };
 
-// growTree for one request per block (64 threads = one wave).
-__global__ void __launch_bounds__(64) grow_kernel(const GrowCmd *__restrict__ cmds, int ncmd,
-                                                  GrowRes *__restrict__ res)
+// Per-request scratch of one tick (device).
+struct GrowEdge {
+    double a[kDof], b[kDof]; // motion checked in the direction of growth
+    double dst[kDof];        // state to append
+    int32_t nd;              // segments of the motion
+    int32_t npos;            // positions to check (0 = none: trapped before the check)
+    int32_t extra;           // 1: the last position is isValid(dst) (goal tree)
+    int32_t nm;              // nearest node
+    int32_t reach;
+    int32_t first_bad;       // lowest invalid position found (INT_MAX: none)
+};
+
+struct TickArgs {
+    const GrowCmd *cmds;
+    GrowEdge *edges;
+    GrowRes *res;
+    int32_t *max_pos; // largest npos of the tick (reset by the commit kernel)
+    int ncmd;
+};
+
+// Tick kernel 1 (one wave per request): nearest neighbour in the request's
+// tree -- first strict minimum of distance(elem, target), lane-strided then
+// wave-reduced -- and steering by the planner range (RRTConnect.cpp:119-138).
+__global__ void __launch_bounds__(64) grow_prepare_kernel(TickArgs T)
 {
-    extern __shared__ float4 lds[];
     const int r = blockIdx.x;
-    if (r >= ncmd) return;
-    const GrowCmd &C = cmds[r];
-    stage_env(EnvView{C.obs, C.ns, C.nb}, lds);
+    if (r >= T.ncmd) return;
+    const GrowCmd &C = T.cmds[r];
+    GrowEdge &E = T.edges[r];
     const int lane = lane_id();
-    // nearest: first strict minimum of distance(elem, target) over the tree
     double best = INFINITY;
     int best_i = INT_MAX;
     for (int i = lane; i < C.tree_n; i += 64) {
@@ -80,7 +101,7 @@ __global__ void __launch_bounds__(64) grow_kernel(const GrowCmd *__restrict__ cm
             dist += diff * diff;
         }
         const double d = __builtin_sqrt(dist);
-        if (d < best) { // lanes visit increasing i: strict < keeps the lowest index
+        if (d < best) { // increasing i per lane: strict < keeps the lowest index
             best = d;
             best_i = i;
         }
@@ -94,56 +115,128 @@ __global__ void __launch_bounds__(64) grow_kernel(const GrowCmd *__restrict__ cm
             best_i = oi;
         }
     }
+    if (lane != 0) return;
     const int nm = best_i;
-    double ns_[kDof], target[kDof], x[kDof];
-#pragma unroll
-    for (int k = 0; k < kDof; ++k) {
-        ns_[k] = C.tree_s[(int64_t)nm * kDof + k];
-        target[k] = C.target[k];
-    }
-    // steer (RRTConnect.cpp:124-138)
+    double ns_[kDof], x[kDof];
+    for (int k = 0; k < kDof; ++k) ns_[k] = C.tree_s[(int64_t)nm * kDof + k];
+    E.nm = nm;
+    E.first_bad = INT_MAX;
     const double d = best;
     bool reach = true;
-    const double *dst = target;
+    for (int k = 0; k < kDof; ++k) x[k] = C.target[k];
     if (d > C.maxd) {
         const double t = C.maxd / d;
         bool equal = true;
-#pragma unroll
         for (int k = 0; k < kDof; ++k) {
-            x[k] = ns_[k] + (target[k] - ns_[k]) * t;
+            x[k] = ns_[k] + (C.target[k] - ns_[k]) * t;
             if (fabs(ns_[k] - x[k]) > 2.220446049250313080847e-16 * 2.0) equal = false;
         }
-        if (equal) {
-            if (lane == 0) res[r].status = kTrapped;
+        if (equal) { // no progress: TRAPPED without a check
+            E.npos = 0;
             return;
         }
-        dst = x;
         reach = false;
     }
-    // motion check in the direction of growth (RRTConnect.cpp:141-142)
-    bool valid;
-    int total = 0;
-    if (C.start_tree) {
-        valid = walk_edge<8, 1, false>(ns_, dst, C.lvs, C.factor, lds, C.ns, C.nb, 0, total) == INT_MAX;
-    } else {
-        valid = walk_edge<8, 8, false>(dst, dst, C.lvs, C.factor, lds, C.ns, C.nb, 0, total) == INT_MAX;
-        if (valid) valid = walk_edge<8, 1, false>(dst, ns_, C.lvs, C.factor, lds, C.ns, C.nb, 0, total) == INT_MAX;
+    E.reach = reach;
+    for (int k = 0; k < kDof; ++k) {
+        E.dst[k] = x[k];
+        E.a[k] = C.start_tree ? ns_[k] : x[k]; // start tree: checkMotion(nearest, dst)
+        E.b[k] = C.start_tree ? x[k] : ns_[k]; // goal tree: isValid(dst) && checkMotion(dst, nearest)
     }
-    if (lane == 0) {
-        if (!valid) {
-            res[r].status = kTrapped;
-        } else {
-            const int idx = C.tree_n;
+    const int nd = valid_segment_count<kDof>(E.a, E.b, C.lvs, C.factor);
+    if (nd > kMaxSegments) { // reported invalid
+        E.npos = 1;
+        E.first_bad = 0;
+        return;
+    }
+    E.nd = nd;
+    E.extra = C.start_tree ? 0 : 1;
+    E.npos = 1 + (nd >= 2 ? nd - 1 : 0) + E.extra;
+    atomicMax(T.max_pos, E.npos);
+}
+
+// Tick kernel 2: every request's positions, chunk-parallel over a persistent
+// grid (items (chunk, request) chunk-major; 64 items' liveness per ballot);
+// obstacles are read from each request's environment in HBM (wave-uniform
+// loads), so requests in different environments share the launch.
+template <int LPS, int OPS>
+__global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
+{
+    constexpr int G = GroupShape<LPS, OPS>::STATES;
+    const int lane = lane_id();
+    const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n = T.ncmd;
+    const int nblk = (*T.max_pos + G - 1) / G;
+    const int64_t items = (int64_t)n * nblk;
+    for (int64_t g0 = wave * 64; g0 < items; g0 += waves_total * 64) {
+        const int64_t k = g0 + lane;
+        int r = 0, base = 0;
+        bool live = k < items;
+        if (live) {
+            const int blk = (int)(k / n);
+            r = (int)(k - (int64_t)blk * n);
+            base = blk * G;
+            live = base < T.edges[r].npos &&
+                   __hip_atomic_load(&T.edges[r].first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == INT_MAX;
+        }
+        uint64_t mask = wave_ballot(live);
+        while (mask) {
+            const int j = lowest_bit(mask);
+            mask &= mask - 1;
+            const int rj = __shfl(r, j), bj = __shfl(base, j);
+            GrowEdge &E = T.edges[rj];
+            if (__hip_atomic_load(&E.first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != INT_MAX) continue;
+            const GrowCmd &C = T.cmds[rj];
+            using Shape = GroupShape<LPS, OPS>;
+            const Shape sh(lane);
+            const int p = bj + sh.group;
+            const int motion_pos = E.npos - E.extra;
+            const bool plive = p < E.npos;
+            float q[kDof];
+            if (plive && p >= motion_pos) {
 #pragma unroll
-            for (int k = 0; k < kDof; ++k) {
-                C.tree_s[(int64_t)idx * kDof + k] = dst[k];
-                res[r].xstate[k] = dst[k];
+                for (int d = 0; d < kDof; ++d) q[d] = (float)E.dst[d];
+            } else {
+                double a[kDof], b[kDof];
+#pragma unroll
+                for (int d = 0; d < kDof; ++d) {
+                    a[d] = E.a[d];
+                    b[d] = E.b[d];
+                }
+                walk_state(a, b, E.nd, plive ? p : 0, q);
             }
-            C.tree_parent[idx] = nm;
-            res[r].status = reach ? kReached : kAdvanced;
-            res[r].xmotion = idx;
+            LaneSpheres<LPS> S;
+            arm_fk<LPS>(q, sh.slot, S);
+            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, C.obs, C.ns, C.nb, plive, sh.part);
+            const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
+            if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);
         }
     }
+}
+
+// Tick kernel 3 (one thread per request): append the new node of a valid
+// motion, write the outcome, clear the tick counter.
+__global__ void grow_commit_kernel(TickArgs T)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) *T.max_pos = 0;
+    if (r >= T.ncmd) return;
+    const GrowCmd &C = T.cmds[r];
+    const GrowEdge &E = T.edges[r];
+    GrowRes &R = T.res[r];
+    if (E.npos == 0 || E.first_bad != INT_MAX) {
+        R.status = kTrapped;
+        return;
+    }
+    const int idx = C.tree_n;
+    for (int k = 0; k < kDof; ++k) {
+        C.tree_s[(int64_t)idx * kDof + k] = E.dst[k];
+        R.xstate[k] = E.dst[k];
+    }
+    C.tree_parent[idx] = E.nm;
+    R.status = E.reach ? kReached : kAdvanced;
+    R.xmotion = idx;
 }
 
 // ---------------------------------------------------------------------------
@@ -326,7 +419,9 @@ struct Server {
     char *pool = nullptr;
     GrowCmd *cmd_host = nullptr, *cmd_dev = nullptr;
     GrowRes *res_host = nullptr, *res_dev = nullptr;
-    int max_env_words = 0;
+    GrowEdge *edges_dev = nullptr;
+    int32_t *max_pos_dev = nullptr;
+    unsigned check_blocks = 0;
 
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -416,16 +511,14 @@ void Server::run()
             }
             if (batch.empty()) continue;
             const int n = (int)batch.size();
-            int words = 0;
-            for (int i = 0; i < n; ++i) {
-                next_command(*planners[i], cmd_host[i]);
-                words = std::max(words, cmd_host[i].ns + 2 * cmd_host[i].nb);
-            }
+            for (int i = 0; i < n; ++i) next_command(*planners[i], cmd_host[i]);
             OA_HIP(hipMemcpyAsync(cmd_dev, cmd_host, n * sizeof(GrowCmd), hipMemcpyHostToDevice, stream));
             {
-                ProfScope ps("serve_grow", stream);
-                hipLaunchKernelGGL(grow_kernel, dim3(n), dim3(64), (size_t)words * sizeof(float4), stream, cmd_dev, n,
-                                   res_dev);
+                ProfScope ps("serve_tick", stream);
+                TickArgs T{cmd_dev, edges_dev, res_dev, max_pos_dev, n};
+                hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(64), 0, stream, T);
+                hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
+                hipLaunchKernelGGL(grow_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, T);
                 OA_HIP(hipGetLastError());
             }
             OA_HIP(hipMemcpyAsync(res_host, res_dev, n * sizeof(GrowRes), hipMemcpyDeviceToHost, stream));
@@ -596,6 +689,10 @@ int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *o
         OA_HIP(hipHostMalloc((void **)&S.res_host, sizeof(GrowRes) * cfg->max_active, hipHostMallocDefault));
         OA_HIP(hipMalloc(&S.cmd_dev, sizeof(GrowCmd) * cfg->max_active));
         OA_HIP(hipMalloc(&S.res_dev, sizeof(GrowRes) * cfg->max_active));
+        OA_HIP(hipMalloc(&S.edges_dev, sizeof(GrowEdge) * cfg->max_active));
+        OA_HIP(hipMalloc(&S.max_pos_dev, sizeof(int32_t)));
+        OA_HIP(hipMemset(S.max_pos_dev, 0, sizeof(int32_t)));
+        S.check_blocks = (unsigned)num_cus() * 2; // 8 waves per CU
         S.th = std::thread([&S] { S.run(); });
         *out = h;
     });
@@ -616,6 +713,8 @@ int ompl_amd_server_destroy(ompl_amd_server h)
         hipFree(S.pool);
         hipFree(S.cmd_dev);
         hipFree(S.res_dev);
+        hipFree(S.edges_dev);
+        hipFree(S.max_pos_dev);
         hipHostFree(S.cmd_host);
         hipHostFree(S.res_host);
         hipStreamDestroy(S.stream);

@@ -33,11 +33,13 @@ def scene(gpu, oracle):
     return env, oenv, pairs
 
 
-def reqs_for(gpu, env, pairs, max_iters=3000):
+def reqs_for(gpu, env, pairs, max_iters=3000, period=None):
+    """Request i plans pair i with seed 100 + (i mod period)."""
     from ompl_amd.serve import make_request
 
-    return [make_request(env, a, b, seed=100 + i, lvs=lvs_config(), max_iters=max_iters) for i, (a, b) in
-            enumerate(pairs)]
+    period = period or len(pairs)
+    return [make_request(env, a, b, seed=100 + i % period, lvs=lvs_config(), max_iters=max_iters)
+            for i, (a, b) in enumerate(pairs)]
 
 
 def test_plain_planner_matches_oracle(gpu, oracle, scene):
@@ -87,7 +89,7 @@ def test_continuous_batching_more_requests_than_slots(gpu, oracle, scene):
     from ompl_amd.serve import SOLVED, FAILED, Server
 
     env, oenv, pairs = scene
-    reqs = reqs_for(gpu, env, pairs * 3)
+    reqs = reqs_for(gpu, env, pairs * 3, period=len(pairs))
     srv = Server(hbm_budget_bytes=1 << 30, max_active=4, max_nodes=4096)
     ids = [srv.submit(r) for r in reqs]
     results = [srv.wait(rid, 120000) for rid in ids]
