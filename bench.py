@@ -36,6 +36,7 @@ sys.path.insert(0, str(ROOT))
 N_EDGES = 16384
 N_SPHERES, N_BOXES = 128, 128
 ENV_SEED, EDGE_SEED = 1001, 4
+PRM_SEED, PRM_SAMPLES = 7, 200_000
 PEAK_FP32_TFLOPS = 157.3
 
 
@@ -139,6 +140,46 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     }
 
 
+def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
+    """Batched PRM roadmap (samples -> validity -> kNN k=10 -> candidate edges ->
+    motion checks -> components): the 200k-sample, 512-sphere + 512-box scene,
+    and the same pipeline over the 64-sphere scene of the validity batch, where
+    most samples are valid.  The CPU restatement runs on a bounded sample."""
+    from ompl_amd.prm import build_roadmap
+
+    out = {}
+    for name, env_seed, ns, nb in (("1024_obstacles", 1003, 512, 512), ("64_spheres", 1000, 64, 0)):
+        s, b = oa.workload.obstacles(env_seed, ns, nb)
+        env = oa.Env(s, b)
+        build_roadmap(env, PRM_SEED, 4096, n_streams=64, fetch=False)  # warm the pipeline
+        reps = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            st = build_roadmap(env, PRM_SEED, n_samples, n_streams=256, fetch=False)
+            reps.append(time.perf_counter() - t0)
+        best = min(reps)
+        row = {"samples": st["samples"], "milestones": st["milestones"], "candidate_edges": st["candidate_edges"],
+               "valid_edges": st["valid_edges"], "components": st["components"], "build_ms": 1e3 * best,
+               "stage_ms": {k[3:]: round(st[k], 3) for k in st if k.startswith("ms_")},
+               "edge_checks_per_s": st["candidate_edges"] / (st["ms_edges"] * 1e-3) if st["ms_edges"] > 0 else None}
+        if run_cpu:
+            from oracle import oracle_py
+
+            oenv = oracle_py.OracleEnv(s, b)
+            t0 = time.perf_counter()
+            r = oracle_py.prm_batch(oenv, PRM_SEED, cpu_samples, 10, lvs_0p01(), 1, 256, threads=cpu_threads)
+            cpu_s = time.perf_counter() - t0
+            g = build_roadmap(env, PRM_SEED, cpu_samples, n_streams=256)
+            same = (np.array_equal(g.sample_index, r["sample_index"]) and np.array_equal(g.edges, r["edges"])
+                    and np.array_equal(g.edge_valid, r["edge_valid"]) and np.array_equal(g.component, r["component"]))
+            row["cpu_baseline"] = {"samples_per_s": r["samples"].shape[0] / cpu_s, "cores": cpu_threads,
+                                   "kind": "port", "sample": f"{r['samples'].shape[0]} samples, {cpu_s:.1f} s"}
+            row["gpu_samples_per_s"] = st["samples"] / best
+            row["parity_bit_exact_on_sample"] = bool(same)
+        out[name] = row
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +194,8 @@ def main(argv=None):
     ap.add_argument("--stream-qps", type=float, default=200.0)
     ap.add_argument("--stream-envs", type=int, default=32)
     ap.add_argument("--plain-workers", type=int, default=8)
+    ap.add_argument("--prm-samples", type=int, default=PRM_SAMPLES, help="PRM roadmap batch size (0 = skip)")
+    ap.add_argument("--prm-cpu-samples", type=int, default=20000)
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -281,6 +324,11 @@ def main(argv=None):
     if args.stream_requests > 0:
         stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers)
 
+    prm_res = None
+    if args.prm_samples > 0:
+        prm_res = prm_batch(oa, args.prm_samples, args.prm_cpu_samples,
+                            args.cpu_threads or min(os.cpu_count() or 1, 16), not args.no_cpu)
+
     res = {
         "metric": "edge-checks/s + p50/p99 plan latency, served vs plain GPU path, bit-exact",
         "value": value,
@@ -313,6 +361,7 @@ def main(argv=None):
         "autotuned_us_per_batch": tuned_us,
         "parity_bit_exact": parity,
         "served_stream": stream_res,
+        "prm_batch": prm_res,
         "wall_s": wall,
     }
     print(json.dumps(res))

@@ -110,6 +110,22 @@ int ompl_amd_check_motion(ompl_amd_env env, const double *a, const double *b, do
 int ompl_amd_check_motion_last_valid(ompl_amd_env env, const double *a, const double *b,
                                      double longest_valid_segment, unsigned factor, int *valid, double *last_t);
 
+/* Precomputed motion states -- replaces SpaceInformation::checkMotion(
+ * states, count) and checkMotion(states, count, firstInvalidStateIndex)
+ * (base/src/SpaceInformation.cpp:288-341).  List i is the states
+ * [offsets_dev[i], offsets_dev[i+1]) of q_soa_dev (FP64 [7][n_states]);
+ * out_dev[i] = 1 when every state of the list is valid (the reference's
+ * bisection visit order does not change this result);
+ * first_invalid_dev[i] (optional) = index within the list of the first
+ * invalid state in sequential order, -1 when the list is valid.  Offsets are
+ * clamped to [0, n_states]. */
+int ompl_amd_check_state_lists_dev(ompl_amd_env env, const double *q_soa_dev, int64_t n_states,
+                                   const int64_t *offsets_dev, int64_t n_lists, uint8_t *out_dev,
+                                   int32_t *first_invalid_dev, void *stream);
+/* Plain single call: one list of `count` states (host row-major [count][7]);
+ * *first_invalid is written only when the list is invalid. */
+int ompl_amd_check_states(ompl_amd_env env, const double *states, int count, int *valid, int *first_invalid);
+
 /* Launch-shape knobs used by the batched kernels (set by the autotuner).
  * A state is checked by lanes_per_state x obstacle_parts lanes: the first
  * splits the arm's 28 spheres, the second the obstacle list.  Built shapes
@@ -172,6 +188,20 @@ int ompl_amd_profile_query(const char *kernel, int64_t *launches, double *total_
 int ompl_amd_sample_uniform_dev(const uint32_t *seeds, int n_streams, int64_t n_per_stream, int dim,
                                 const double *lo, const double *hi, double *out_dev, float *out_f32_soa_dev,
                                 void *stream);
+
+/* Valid-state sampling -- UniformValidStateSampler::sample
+ * (base/samplers/src/UniformValidStateSampler.cpp:46-57) batched over
+ * samplers.  Sampler s (generator seeded seeds[s], as above, bounds lo/hi)
+ * makes n_calls successive sample() calls; a call draws uniform states until
+ * one is valid for `env` or `attempts` draws were made (the reference
+ * default is magic::MAX_VALID_SAMPLE_ATTEMPTS = 100,
+ * tools/config/MagicConstants.h:90).  out_dev [n_streams][n_calls][7] FP64
+ * receives each call's final state (the last draw when none was valid),
+ * found_dev [n_streams][n_calls] = 1 when it is valid, draws_dev (optional)
+ * the number of draws the call made.  Blocking. */
+int ompl_amd_sample_valid_dev(ompl_amd_env env, const uint32_t *seeds, int n_streams, int64_t n_calls, int attempts,
+                              const double *lo, const double *hi, double *out_dev, uint8_t *found_dev,
+                              int32_t *draws_dev, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Nearest neighbours -- ompl::NearestNeighbors<_T>                      */
@@ -284,6 +314,47 @@ int ompl_amd_server_stats(ompl_amd_server server, int64_t out[6]);
  * and synchronisation per isValid / checkMotion call (the plain path). */
 int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *res, double *path, int cap,
                         int *path_len);
+
+/* ------------------------------------------------------------------ */
+/* Batched PRM roadmap -- the inner loop of PRM::growRoadmap /          */
+/* addMilestone (geometric/planners/prm/src/PRM.cpp:364-385, 570-604)   */
+/* with KStrategy (geometric/planners/prm/ConnectionStrategy.h:56-81)   */
+/* for a whole batch: samples -> validity -> k nearest milestones ->    */
+/* undirected candidate edges -> motion checks (lower index -> higher)  */
+/* -> connected components.  Batch semantics: each milestone connects   */
+/* to its k nearest among all milestones (the reference: among those    */
+/* added before it).                                                    */
+/* ------------------------------------------------------------------ */
+typedef struct ompl_amd_prm_s *ompl_amd_prm;
+
+typedef struct {
+    int64_t samples;
+    int64_t milestones;
+    int64_t candidate_edges;
+    int64_t valid_edges;
+    int64_t components;
+    double ms_sample, ms_validity, ms_knn, ms_edges, ms_graph, ms_total;
+} ompl_amd_prm_stats;
+
+/* Samples U[-pi,pi]^7 from n_streams mt19937 generators seeded seed,
+ * seed+1, ... (stream-major sample order; n_samples is rounded up to a
+ * multiple of n_streams), checks them with the arm checker of `env`, and
+ * builds the roadmap with k neighbours per milestone.  Milestone i is the
+ * i-th valid sample.  Requires 2 * (n_samples + n_streams) * k < 2^31. */
+int ompl_amd_prm_build(ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k,
+                       double longest_valid_segment, unsigned factor, ompl_amd_prm *out, ompl_amd_prm_stats *stats);
+int ompl_amd_prm_destroy(ompl_amd_prm prm);
+/* Milestones (row-major FP64) and their sample indices. */
+int ompl_amd_prm_milestones(ompl_amd_prm prm, double *nodes, int32_t *sample_index, int64_t cap);
+/* Candidate edges (milestone indices u < v) and their motion validity. */
+int ompl_amd_prm_edges(ompl_amd_prm prm, int32_t *u, int32_t *v, uint8_t *valid, int64_t cap);
+/* Component label (smallest milestone index of the component) per milestone. */
+int ompl_amd_prm_components(ompl_amd_prm prm, int32_t *label, int64_t cap);
+/* Roadmap graph (replaces the Boost.Graph adjacency list of PRM.h:113-121, 382):
+ * symmetric CSR of the valid edges, offsets [milestones + 1], neighbours
+ * ascending per row; adjacency needs 2 * valid_edges entries (may be NULL
+ * to read the offsets only). */
+int ompl_amd_prm_graph(ompl_amd_prm prm, int64_t *offsets, int32_t *adjacency, int64_t adjacency_cap);
 
 /* ------------------------------------------------------------------ */
 /* Synthetic workloads (std::mt19937 + OMPL RNG semantics, host side)   */

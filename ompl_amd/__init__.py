@@ -241,6 +241,31 @@ def check_motion_last_valid(env: Env, a, b, lvs: float, factor: int = 1, last_t:
     return bool(v.value), t.value
 
 
+def check_state_lists(env: Env, q_soa, offsets, stream=None):
+    """Batched SpaceInformation::checkMotion(states, count[, firstInvalidStateIndex])
+    (base/src/SpaceInformation.cpp:288-341) over lists of precomputed states.
+    q_soa: device [7, n_states] float64; offsets: device int64 [n_lists + 1].
+    Returns (valid u8 [n_lists], first_invalid i32 [n_lists], -1 when valid)."""
+    torch = _torch()
+    assert q_soa.dtype == torch.float64 and q_soa.shape[0] == ARM_DOF and q_soa.is_contiguous()
+    assert offsets.dtype == torch.int64 and offsets.is_contiguous()
+    n_lists = offsets.shape[0] - 1
+    out = torch.empty(max(n_lists, 0), dtype=torch.uint8, device=q_soa.device)
+    first = torch.empty(max(n_lists, 0), dtype=torch.int32, device=q_soa.device)
+    call("ompl_amd_check_state_lists_dev", env.handle, _ptr(q_soa), q_soa.shape[1], _ptr(offsets), n_lists,
+         _ptr(out), _ptr(first), _stream(stream))
+    return out, first
+
+
+def check_states(env: Env, states) -> tuple[bool, int]:
+    """Plain single-call path of SpaceInformation::checkMotion(states, count,
+    firstInvalidStateIndex): returns (valid, first invalid index or -1)."""
+    qa = np.ascontiguousarray(np.asarray(states, np.float64).reshape(-1, ARM_DOF))
+    v, f = C.c_int(), C.c_int(-1)
+    call("ompl_amd_check_states", env.handle, qa.ctypes.data_as(C.c_void_p), qa.shape[0], C.byref(v), C.byref(f))
+    return bool(v.value), f.value
+
+
 # ---------------------------------------------------------------------------
 # synthetic workloads (host, std::mt19937 with ompl::RNG semantics)
 # ---------------------------------------------------------------------------

@@ -42,6 +42,8 @@ SIGNATURES = {
     "ompl_amd_check_motion": (C.c_int, [vp, vp, vp, C.c_double, C.c_uint, c_int_p]),
     "ompl_amd_check_motion_last_valid": (C.c_int, [vp, vp, vp, C.c_double, C.c_uint, c_int_p,
                                                    C.POINTER(C.c_double)]),
+    "ompl_amd_check_state_lists_dev": (C.c_int, [vp, vp, C.c_int64, vp, C.c_int64, vp, vp, vp]),
+    "ompl_amd_check_states": (C.c_int, [vp, vp, C.c_int, c_int_p, c_int_p]),
     "ompl_amd_get_tuning": (C.c_int, [vp]),
     "ompl_amd_set_tuning": (C.c_int, [vp]),
     "ompl_amd_autotune_motions": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, C.c_int, vp,
@@ -52,6 +54,7 @@ SIGNATURES = {
     "ompl_amd_profile_reset": (C.c_int, []),
     "ompl_amd_profile_query": (C.c_int, [C.c_char_p, c_i64_p, C.POINTER(C.c_double)]),
     "ompl_amd_sample_uniform_dev": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int, vp, vp, vp, vp, vp]),
+    "ompl_amd_sample_valid_dev": (C.c_int, [vp, vp, C.c_int, C.c_int64, C.c_int, vp, vp, vp, vp, vp, vp]),
     "ompl_amd_nn_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "ompl_amd_nn_destroy": (C.c_int, [vp]),
     "ompl_amd_nn_clear": (C.c_int, [vp]),
@@ -76,6 +79,13 @@ SIGNATURES = {
     "ompl_amd_server_release": (C.c_int, [vp, C.c_int64]),
     "ompl_amd_server_stats": (C.c_int, [vp, vp]),
     "ompl_amd_plan_plain": (C.c_int, [vp, vp, vp, C.c_int, c_int_p]),
+    "ompl_amd_prm_build": (C.c_int, [vp, C.c_uint32, C.c_int, C.c_int64, C.c_int, C.c_double, C.c_uint,
+                                     C.POINTER(vp), vp]),
+    "ompl_amd_prm_destroy": (C.c_int, [vp]),
+    "ompl_amd_prm_milestones": (C.c_int, [vp, vp, vp, C.c_int64]),
+    "ompl_amd_prm_edges": (C.c_int, [vp, vp, vp, vp, C.c_int64]),
+    "ompl_amd_prm_components": (C.c_int, [vp, vp, C.c_int64]),
+    "ompl_amd_prm_graph": (C.c_int, [vp, vp, vp, C.c_int64]),
     "ompl_amd_workload_obstacles": (C.c_int, [C.c_uint32, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                               C.c_double, C.c_double, vp, vp]),
     "ompl_amd_workload_configs": (C.c_int, [C.c_uint32, C.c_int64, vp]),
@@ -101,6 +111,25 @@ class Tuning(C.Structure):
         ("motion_block", C.c_int),
         ("motion_waves_per_cu", C.c_int),
         ("motion_strategy", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PrmStats(C.Structure):
+    _fields_ = [
+        ("samples", C.c_int64),
+        ("milestones", C.c_int64),
+        ("candidate_edges", C.c_int64),
+        ("valid_edges", C.c_int64),
+        ("components", C.c_int64),
+        ("ms_sample", C.c_double),
+        ("ms_validity", C.c_double),
+        ("ms_knn", C.c_double),
+        ("ms_edges", C.c_double),
+        ("ms_graph", C.c_double),
+        ("ms_total", C.c_double),
     ]
 
     def as_dict(self):

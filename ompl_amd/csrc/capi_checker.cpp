@@ -252,6 +252,43 @@ int ompl_amd_check_motion(ompl_amd_env e, const double *a, const double *b, doub
     });
 }
 
+int ompl_amd_check_state_lists_dev(ompl_amd_env e, const double *q_soa, int64_t n_states, const int64_t *offsets,
+                                   int64_t n_lists, uint8_t *out, int32_t *first_invalid, void *stream)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(n_states >= 0 && n_lists >= 0, OMPL_AMD_ERR_ARG, "negative sizes");
+        if (n_lists == 0) return;
+        OA_REQUIRE(offsets && out && (n_states == 0 || q_soa), OMPL_AMD_ERR_ARG, "null pointer");
+        launch_state_lists(env.view(), q_soa, n_states, offsets, n_lists, out, first_invalid, resolve_stream(stream));
+    });
+}
+
+int ompl_amd_check_states(ompl_amd_env e, const double *states, int count, int *valid, int *first_invalid)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(count >= 0 && valid && (count == 0 || states), OMPL_AMD_ERR_ARG, "bad arguments");
+        ThreadResources &r = thread_resources();
+        // staging: offsets [2] int64, result u8 + i32, then the states dof-major
+        const size_t head = 64;
+        stage_reserve(r, head + (size_t)count * kDof * sizeof(double));
+        int64_t *off = reinterpret_cast<int64_t *>(r.stage);
+        off[0] = 0;
+        off[1] = count;
+        double *q = reinterpret_cast<double *>(r.stage + head);
+        for (int i = 0; i < count; ++i)
+            for (int j = 0; j < kDof; ++j) q[(size_t)j * count + i] = states[(size_t)i * kDof + j];
+        launch_state_lists(env.view(), reinterpret_cast<const double *>(r.stage_dev + head), count,
+                           reinterpret_cast<const int64_t *>(r.stage_dev), 1,
+                           reinterpret_cast<uint8_t *>(r.stage_dev + 16),
+                           reinterpret_cast<int32_t *>(r.stage_dev + 32), r.stream);
+        OA_HIP(hipStreamSynchronize(r.stream));
+        *valid = reinterpret_cast<uint8_t *>(r.stage)[16];
+        if (first_invalid && !*valid) *first_invalid = *reinterpret_cast<int32_t *>(r.stage + 32);
+    });
+}
+
 int ompl_amd_check_motion_last_valid(ompl_amd_env e, const double *a, const double *b, double lvs, unsigned factor,
                                      int *valid, double *last_t)
 {

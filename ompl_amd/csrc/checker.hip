@@ -177,49 +177,35 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 }
 
 // Chunk-parallel walk of the survivors.  Virtual work items k = (chunk blk,
-// survivor s) in chunk-major order; a wave examines 64 consecutive items at
-// once (one per lane: is the chunk inside the edge and no earlier position of
-// that edge known invalid?), then walks the live ones with all its lanes.
+// survivor s) in chunk-major order, one wave per item over a persistent grid;
+// an item is skipped when its chunk lies past the edge's last position or an
+// earlier position of the edge is already known invalid.
 template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
     constexpr int G = GroupShape<LPS, OPS>::STATES;
-    const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int n_surv = *A.list_count;
-    const int nblk = (*A.max_m + G - 1) / G;
-    const int64_t items = (int64_t)n_surv * nblk;
-    for (int64_t g0 = wave * 64; g0 < items; g0 += waves_total * 64) {
-        const int64_t k = g0 + lane;
-        int s = 0, base = 0;
-        bool live = k < items;
-        if (live) {
-            const int blk = (int)(k / n_surv);
-            s = (int)(k - (int64_t)blk * n_surv);
-            base = 1 + blk * G;
-            live = base < A.surv_nd[s] &&
-                   __hip_atomic_load(&A.first_bad[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= base;
-        }
-        uint64_t mask = wave_ballot(live);
-        while (mask) {
-            const int j = lowest_bit(mask);
-            mask &= mask - 1;
-            const int sj = __shfl(s, j), bj = __shfl(base, j);
-            if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) continue;
-            const int64_t e = A.list[sj];
-            const int nd = A.surv_nd[sj];
-            double a[kDof], b[kDof];
+    const int64_t n_surv = *A.list_count;
+    if (n_surv == 0) return;
+    const int64_t nblk = ((int64_t)*A.max_m + G - 1) / G;
+    const int64_t items = n_surv * nblk;
+    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < items; k += waves_total) {
+        const int64_t blk = k / n_surv, s = k - blk * n_surv;
+        const int base = 1 + (int)(blk * G);
+        const int nd = A.surv_nd[s];
+        if (base >= nd) continue; // past the edge's last position
+        if (__hip_atomic_load(&A.first_bad[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < base) continue;
+        const int64_t e = A.list[s];
+        double a[kDof], b[kDof];
 #pragma unroll
-            for (int d = 0; d < kDof; ++d) {
-                a[d] = A.a[(int64_t)d * A.n + e];
-                b[d] = A.b[(int64_t)d * A.n + e];
-            }
-            const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, nd, nd, bj, lds, A.env.ns, A.env.nb);
-            if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[sj], bad);
+        for (int d = 0; d < kDof; ++d) {
+            a[d] = A.a[(int64_t)d * A.n + e];
+            b[d] = A.b[(int64_t)d * A.n + e];
         }
+        const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, nd, nd, base, lds, A.env.ns, A.env.nb);
+        if (bad != INT_MAX && lane_id() == 0) atomicMin(&A.first_bad[s], bad);
     }
 }
 
@@ -301,6 +287,51 @@ __global__ void __launch_bounds__(64) validate_single_kernel(EnvView env, State1
     const bool hit = collide_env<8, 8, Exit::kAllGroups>(S, lds, env.ns, env.nb, true, sh.part);
     const uint64_t H = wave_ballot(hit);
     if (lane == 0) *result = H ? 0 : 1;
+}
+
+// Precomputed state lists (SpaceInformation::checkMotion(states, count[,
+// firstInvalidStateIndex]), base/src/SpaceInformation.cpp:288-341): one wave
+// per list walks its states in index order, G at a time, and stops at the
+// first chunk holding an invalid state; the exact lowest invalid index in that
+// chunk is the list's first invalid state.  Whether a list is valid does not
+// depend on the order the reference visits the states in (all must be valid).
+// Offsets are clamped to [0, n_states], so any offsets stay in bounds.
+template <int LPS, int OPS>
+__global__ void __launch_bounds__(256) state_lists_kernel(EnvView env, const double *__restrict__ q_soa,
+                                                          int64_t n_states, const int64_t *__restrict__ offsets,
+                                                          int64_t n_lists, uint8_t *__restrict__ out,
+                                                          int32_t *__restrict__ first_invalid)
+{
+    extern __shared__ float4 lds[];
+    stage_env(env, lds);
+    using Shape = GroupShape<LPS, OPS>;
+    constexpr int G = Shape::STATES;
+    const int lane = lane_id();
+    const Shape sh(lane);
+    const int64_t list = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (list >= n_lists) return;
+    const int64_t lo = min(max(offsets[list], (int64_t)0), n_states);
+    const int64_t hi = max(min(offsets[list + 1], n_states), lo);
+    int64_t bad = -1;
+    for (int64_t base = lo; base < hi; base += G) {
+        const int64_t st = base + sh.group;
+        const bool live = st < hi;
+        float q[kDof];
+#pragma unroll
+        for (int j = 0; j < kDof; ++j) q[j] = live ? (float)q_soa[(int64_t)j * n_states + st] : 0.0f;
+        LaneSpheres<LPS> S;
+        arm_fk<LPS>(q, sh.slot, S);
+        const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, env.ns, env.nb, live, sh.part);
+        const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+        if (H) {
+            bad = base + lowest_bit(H) / Shape::GS - lo;
+            break;
+        }
+    }
+    if (lane == 0) {
+        out[list] = bad < 0 ? 1 : 0;
+        if (first_invalid) first_invalid[list] = (int32_t)bad;
+    }
 }
 
 // Sequential lastValid overload: positions j = 1..nd-1 in order, then s2.
@@ -551,6 +582,16 @@ void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int 
     OA_SHAPES(OA_LAUNCH)
 #undef OA_LAUNCH
     fail(OMPL_AMD_ERR_ARG, "lastValid shape (%d, %d) not built", lps, ops);
+}
+
+void launch_state_lists(const EnvView &env, const double *q_soa, int64_t n_states, const int64_t *offsets,
+                        int64_t n_lists, uint8_t *out, int32_t *first_invalid, hipStream_t s)
+{
+    if (n_lists <= 0) return;
+    ProfScope ps("state_lists", s);
+    hipLaunchKernelGGL((state_lists_kernel<8, 1>), dim3(grid_for(n_lists, 4)), dim3(256), lds_bytes(env), s, env,
+                       q_soa, n_states, offsets, n_lists, out, first_invalid);
+    OA_HIP(hipGetLastError());
 }
 
 void launch_arm_spheres(const float *q, int64_t n, float *out, hipStream_t s)

@@ -1,0 +1,461 @@
+// Batched PRM roadmap construction on the device: the inner loop of
+// PRM::growRoadmap / PRM::addMilestone (geometric/planners/prm/src/PRM.cpp:
+// 364-385, 570-604) with the KStrategy connection rule
+// (geometric/planners/prm/ConnectionStrategy.h:56-81), run for a whole batch:
+//
+//   1. uniform samples (RealVectorStateSampler::sampleUniform per stream)
+//   2. state validity of every sample; valid samples become milestones
+//   3. k nearest milestones of every milestone (exact FP32 L2, ties by index)
+//   4. candidate edges = the undirected union of those k-NN pairs
+//   5. every candidate motion checked (DiscreteMotionValidator semantics,
+//      direction: lower milestone index -> higher)
+//   6. connected components of the valid edges (union-find: root hooking by
+//      atomicCAS + pointer jumping)
+//
+// The reference connects each new milestone to its k nearest *existing*
+// milestones as they are added (incremental); this batch form connects every
+// milestone to its k nearest among all milestones, so its roadmap does not
+// depend on the order of the milestones.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace ompl_amd {
+
+constexpr int kDim = OMPL_AMD_ARM_DOF;
+
+struct PrmRoadmap {
+    int64_t samples = 0, nv = 0, ne = 0, valid_edges = 0, components = 0;
+    double *samples_f64 = nullptr; // [n][7]
+    float *samples_f32 = nullptr;  // [7][n]
+    uint8_t *valid = nullptr;      // [n]
+    int32_t *milestone = nullptr;  // [nv] sample index of milestone i
+    double *nodes = nullptr;       // [nv][7]
+    float *nodes_f32 = nullptr;    // [nv][7]
+    int32_t *knn = nullptr;        // [nv][k+1]
+    float *knn_d = nullptr;
+    uint64_t *keys = nullptr;      // candidate (u << 32 | v), u < v
+    int32_t *eu = nullptr, *ev = nullptr;
+    uint8_t *evalid = nullptr;
+    int32_t *label = nullptr;      // component root per milestone
+    int64_t *csr_off = nullptr;    // [nv + 1] symmetric adjacency of the valid edges
+    int32_t *csr_adj = nullptr;    // [2 * valid_edges], ascending per row
+    ompl_amd_prm_stats stats{};
+    std::vector<void *> allocs;
+
+    template <class T>
+    T *alloc(size_t count)
+    {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, (count ? count : 1) * sizeof(T));
+        if (e != hipSuccess) fail(OMPL_AMD_ERR_NOMEM, "roadmap allocation (%zu bytes) failed", count * sizeof(T));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+    ~PrmRoadmap()
+    {
+        for (void *p : allocs) hipFree(p);
+    }
+};
+
+__global__ void gather_nodes_kernel(const int32_t *__restrict__ milestone, int64_t nv,
+                                    const double *__restrict__ samples, double *__restrict__ nodes,
+                                    float *__restrict__ nodes_f32)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const int64_t s = milestone[i];
+#pragma unroll
+    for (int k = 0; k < kDim; ++k) {
+        const double v = samples[s * kDim + k];
+        nodes[i * kDim + k] = v;
+        nodes_f32[i * kDim + k] = (float)v;
+    }
+}
+
+// Candidate keys: the first k neighbours of milestone i other than i itself.
+__global__ void edge_keys_kernel(const int32_t *__restrict__ knn, int64_t nv, int kk, int k,
+                                 uint64_t *__restrict__ keys)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    int taken = 0;
+    for (int j = 0; j < kk; ++j) {
+        const int nb = knn[i * kk + j];
+        if (taken < k && nb >= 0 && nb != (int)i) {
+            const uint64_t u = (uint64_t)min((int64_t)nb, i), v = (uint64_t)max((int64_t)nb, i);
+            keys[i * k + taken] = (u << 32) | v;
+            ++taken;
+        }
+    }
+    for (int t = taken; t < k; ++t) keys[i * k + t] = ~(uint64_t)0;
+}
+
+__global__ void edge_endpoints_kernel(const uint64_t *__restrict__ keys, int64_t ne, const double *__restrict__ nodes,
+                                      int32_t *__restrict__ eu, int32_t *__restrict__ ev, double *__restrict__ a_soa,
+                                      double *__restrict__ b_soa)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int32_t u = (int32_t)(keys[e] >> 32), v = (int32_t)(keys[e] & 0xffffffffu);
+    eu[e] = u;
+    ev[e] = v;
+#pragma unroll
+    for (int k = 0; k < kDim; ++k) {
+        a_soa[(int64_t)k * ne + e] = nodes[(int64_t)u * kDim + k];
+        b_soa[(int64_t)k * ne + e] = nodes[(int64_t)v * kDim + k];
+    }
+}
+
+__global__ void iota_kernel(int32_t *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+__device__ __forceinline__ int32_t find_root(const int32_t *parent, int32_t x)
+{
+    int32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return x;
+}
+
+// Hooks the higher root under the lower one for every valid edge whose ends
+// are in different trees; sets *changed when any hook succeeded.
+__global__ void hook_kernel(const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                            const uint8_t *__restrict__ evalid, int64_t ne, int32_t *parent, int32_t *changed)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne || !evalid[e]) return;
+    const int32_t ru = find_root(parent, eu[e]), rv = find_root(parent, ev[e]);
+    if (ru == rv) return;
+    const int32_t hi = max(ru, rv), lo = min(ru, rv);
+    if (atomicCAS(&parent[hi], hi, lo) == hi) *changed = 1;
+}
+
+__global__ void jump_kernel(int32_t *parent, int64_t nv)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    parent[i] = find_root(parent, (int32_t)i);
+}
+
+// Both directions of every valid edge as (row << 32 | col) keys, at the
+// positions given by the exclusive scan of evalid.
+__global__ void csr_keys_kernel(const int32_t *__restrict__ eu, const int32_t *__restrict__ ev,
+                                const uint8_t *__restrict__ evalid, const int32_t *__restrict__ pos, int64_t ne,
+                                uint64_t *__restrict__ keys, int32_t *__restrict__ degree)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne || !evalid[e]) return;
+    const uint64_t u = (uint32_t)eu[e], v = (uint32_t)ev[e];
+    keys[2 * (int64_t)pos[e]] = (u << 32) | v;
+    keys[2 * (int64_t)pos[e] + 1] = (v << 32) | u;
+    atomicAdd(&degree[u], 1);
+    atomicAdd(&degree[v], 1);
+}
+
+__global__ void csr_cols_kernel(const uint64_t *__restrict__ keys, int64_t n, int32_t *__restrict__ adj)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) adj[i] = (int32_t)(keys[i] & 0xffffffffu);
+}
+
+__global__ void widen_kernel(const int32_t *__restrict__ in, int64_t n, int64_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+__global__ void u8_to_i32_kernel(const uint8_t *__restrict__ in, int64_t n, int32_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+__global__ void count_roots_kernel(const int32_t *__restrict__ parent, int64_t nv, unsigned long long *count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool root = i < nv && parent[i] == (int32_t)i;
+    const uint64_t m = __ballot(root);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (unsigned long long)__popcll(m));
+}
+
+static unsigned blocks_for(int64_t n, int threads = 256)
+{
+    return (unsigned)((n + threads - 1) / threads);
+}
+
+static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k,
+                  double lvs, unsigned factor)
+{
+    using Clock = std::chrono::steady_clock;
+    auto ms = [](Clock::time_point a) {
+        return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+    };
+    hipStream_t s = default_stream();
+    const auto t_all = Clock::now();
+    auto check = [](int rc) {
+        if (rc != OMPL_AMD_OK) fail(rc, "%s", last_error());
+    };
+    // 1. samples
+    auto t0 = Clock::now();
+    const int64_t per = (n_samples + n_streams - 1) / n_streams;
+    const int64_t n = per * n_streams;
+    R.samples = n;
+    R.samples_f64 = R.alloc<double>((size_t)n * kDim);
+    R.samples_f32 = R.alloc<float>((size_t)n * kDim);
+    std::vector<uint32_t> seeds((size_t)n_streams);
+    for (int i = 0; i < n_streams; ++i) seeds[(size_t)i] = seed + (uint32_t)i;
+    const double pi = 3.141592653589793238462643383279502884;
+    const double lo[kDim] = {-pi, -pi, -pi, -pi, -pi, -pi, -pi}, hi[kDim] = {pi, pi, pi, pi, pi, pi, pi};
+    check(ompl_amd_sample_uniform_dev(seeds.data(), n_streams, per, kDim, lo, hi, R.samples_f64, R.samples_f32, s));
+    R.stats.ms_sample = ms(t0);
+    // 2. validity + milestones
+    t0 = Clock::now();
+    R.valid = R.alloc<uint8_t>((size_t)n);
+    check(ompl_amd_validate_states_dev(env, R.samples_f32, n, R.valid, s));
+    int32_t *iota = R.alloc<int32_t>((size_t)n);
+    hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n)), dim3(256), 0, s, iota, n);
+    R.milestone = R.alloc<int32_t>((size_t)n);
+    int64_t *d_nv = R.alloc<int64_t>(1);
+    size_t tmp_bytes = 0;
+    OA_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, iota, R.valid, R.milestone, d_nv, (int)n, s));
+    void *tmp = R.alloc<char>(tmp_bytes);
+    OA_HIP(hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, iota, R.valid, R.milestone, d_nv, (int)n, s));
+    OA_HIP(hipMemcpyAsync(&R.nv, d_nv, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipStreamSynchronize(s));
+    R.stats.ms_validity = ms(t0);
+    const int64_t nv = R.nv;
+    R.nodes = R.alloc<double>((size_t)nv * kDim);
+    R.nodes_f32 = R.alloc<float>((size_t)nv * kDim);
+    if (nv > 0)
+        hipLaunchKernelGGL(gather_nodes_kernel, dim3(blocks_for(nv)), dim3(256), 0, s, R.milestone, nv,
+                           R.samples_f64, R.nodes, R.nodes_f32);
+    R.label = R.alloc<int32_t>((size_t)nv);
+    if (nv < 2) {
+        if (nv) hipLaunchKernelGGL(iota_kernel, dim3(1), dim3(64), 0, s, R.label, nv);
+        R.csr_off = R.alloc<int64_t>((size_t)nv + 1);
+        OA_HIP(hipMemsetAsync(R.csr_off, 0, ((size_t)nv + 1) * sizeof(int64_t), s));
+        OA_HIP(hipStreamSynchronize(s));
+        R.components = nv;
+        R.stats.ms_total = ms(t_all);
+        return;
+    }
+    // 3. k nearest milestones (k + 1 to drop the milestone itself)
+    t0 = Clock::now();
+    const int kk = std::min<int64_t>(k + 1, nv);
+    ompl_amd_nn nn = nullptr;
+    check(ompl_amd_nn_create(OMPL_AMD_METRIC_L2, kDim, &nn));
+    R.knn = R.alloc<int32_t>((size_t)nv * kk);
+    R.knn_d = R.alloc<float>((size_t)nv * kk);
+    int rc = ompl_amd_nn_add_dev(nn, R.nodes_f32, nv, nullptr, s);
+    if (rc == OMPL_AMD_OK) rc = ompl_amd_nn_knn_dev(nn, R.nodes_f32, nv, kk, R.knn, R.knn_d, s);
+    if (rc == OMPL_AMD_OK) rc = ompl_amd_synchronize(s);
+    ompl_amd_nn_destroy(nn);
+    check(rc);
+    R.stats.ms_knn = ms(t0);
+    // 4. candidate edges: sort + unique of (min, max) keys
+    t0 = Clock::now();
+    const int64_t nk = nv * k;
+    uint64_t *keys = R.alloc<uint64_t>((size_t)nk);
+    uint64_t *sorted = R.alloc<uint64_t>((size_t)nk);
+    R.keys = R.alloc<uint64_t>((size_t)nk);
+    hipLaunchKernelGGL(edge_keys_kernel, dim3(blocks_for(nv)), dim3(256), 0, s, R.knn, nv, kk, k, keys);
+    size_t sort_bytes = 0, uniq_bytes = 0;
+    OA_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, sorted, (int)nk, 0, 64, s));
+    OA_HIP(hipcub::DeviceSelect::Unique(nullptr, uniq_bytes, sorted, R.keys, d_nv, (int)nk, s));
+    void *tmp2 = R.alloc<char>(std::max(sort_bytes, uniq_bytes));
+    OA_HIP(hipcub::DeviceRadixSort::SortKeys(tmp2, sort_bytes, keys, sorted, (int)nk, 0, 64, s));
+    OA_HIP(hipcub::DeviceSelect::Unique(tmp2, uniq_bytes, sorted, R.keys, d_nv, (int)nk, s));
+    int64_t nu = 0;
+    OA_HIP(hipMemcpyAsync(&nu, d_nv, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipStreamSynchronize(s));
+    if (nu > 0) {
+        uint64_t last = 0;
+        OA_HIP(hipMemcpy(&last, R.keys + nu - 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
+        if (last == ~(uint64_t)0) --nu; // padding key of milestones with fewer than k neighbours
+    }
+    R.ne = nu;
+    // 5. motion checks
+    R.eu = R.alloc<int32_t>((size_t)nu);
+    R.ev = R.alloc<int32_t>((size_t)nu);
+    R.evalid = R.alloc<uint8_t>((size_t)nu);
+    double *a_soa = R.alloc<double>((size_t)nu * kDim), *b_soa = R.alloc<double>((size_t)nu * kDim);
+    if (nu > 0) {
+        hipLaunchKernelGGL(edge_endpoints_kernel, dim3(blocks_for(nu)), dim3(256), 0, s, R.keys, nu, R.nodes, R.eu,
+                           R.ev, a_soa, b_soa);
+        check(ompl_amd_check_motions_dev(env, a_soa, b_soa, nu, lvs, factor, R.evalid, nullptr, s));
+    }
+    OA_HIP(hipStreamSynchronize(s));
+    R.stats.ms_edges = ms(t0);
+    // 6. components
+    t0 = Clock::now();
+    hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(nv)), dim3(256), 0, s, R.label, nv);
+    // Every round with a successful hook removes at least one root, so at
+    // most nv - 1 rounds change anything; the round after the last change
+    // sees every valid edge inside one tree and ends the loop.
+    int32_t *changed = R.alloc<int32_t>(1);
+    for (int64_t round = 0; nu > 0; ++round) {
+        if (round > nv) fail(OMPL_AMD_ERR_INTERNAL, "union-find did not converge");
+        OA_HIP(hipMemsetAsync(changed, 0, sizeof(int32_t), s));
+        hipLaunchKernelGGL(hook_kernel, dim3(blocks_for(nu)), dim3(256), 0, s, R.eu, R.ev, R.evalid, nu, R.label,
+                           changed);
+        hipLaunchKernelGGL(jump_kernel, dim3(blocks_for(nv)), dim3(256), 0, s, R.label, nv);
+        int32_t h = 0;
+        OA_HIP(hipMemcpyAsync(&h, changed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        OA_HIP(hipStreamSynchronize(s));
+        if (!h) break;
+    }
+    unsigned long long *cnt = R.alloc<unsigned long long>(1);
+    OA_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(count_roots_kernel, dim3(blocks_for(nv)), dim3(256), 0, s, R.label, nv, cnt);
+    // 7. symmetric CSR adjacency of the valid edges (the roadmap graph)
+    int32_t *pos = R.alloc<int32_t>((size_t)nu + 1);
+    int32_t *degree = R.alloc<int32_t>((size_t)nv + 1);
+    OA_HIP(hipMemsetAsync(degree, 0, ((size_t)nv + 1) * sizeof(int32_t), s));
+    int32_t *flags = R.alloc<int32_t>((size_t)nu);
+    size_t scan_bytes = 0;
+    OA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags, pos, (int)nu, s));
+    void *tmp3 = R.alloc<char>(scan_bytes);
+    int32_t last_pos = 0;
+    uint8_t last_valid = 0;
+    if (nu) {
+        hipLaunchKernelGGL(u8_to_i32_kernel, dim3(blocks_for(nu)), dim3(256), 0, s, R.evalid, nu, flags);
+        OA_HIP(hipcub::DeviceScan::ExclusiveSum(tmp3, scan_bytes, flags, pos, (int)nu, s));
+        OA_HIP(hipMemcpyAsync(&last_pos, pos + nu - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        OA_HIP(hipMemcpyAsync(&last_valid, R.evalid + nu - 1, 1, hipMemcpyDeviceToHost, s));
+    }
+    unsigned long long roots = 0;
+    OA_HIP(hipMemcpyAsync(&roots, cnt, sizeof(roots), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipStreamSynchronize(s));
+    R.components = (int64_t)roots;
+    R.valid_edges = nu ? (int64_t)last_pos + last_valid : 0;
+    const int64_t nd = 2 * R.valid_edges;
+    uint64_t *dkeys = R.alloc<uint64_t>((size_t)nd), *dsorted = R.alloc<uint64_t>((size_t)nd);
+    R.csr_adj = R.alloc<int32_t>((size_t)nd);
+    R.csr_off = R.alloc<int64_t>((size_t)nv + 1);
+    int32_t *off32 = R.alloc<int32_t>((size_t)nv + 1);
+    if (nd) {
+        hipLaunchKernelGGL(csr_keys_kernel, dim3(blocks_for(nu)), dim3(256), 0, s, R.eu, R.ev, R.evalid, pos, nu,
+                           dkeys, degree);
+        size_t b1 = 0;
+        OA_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b1, dkeys, dsorted, (int)nd, 0, 64, s));
+        void *tmp4 = R.alloc<char>(b1);
+        OA_HIP(hipcub::DeviceRadixSort::SortKeys(tmp4, b1, dkeys, dsorted, (int)nd, 0, 64, s));
+        hipLaunchKernelGGL(csr_cols_kernel, dim3(blocks_for(nd)), dim3(256), 0, s, dsorted, nd, R.csr_adj);
+    }
+    size_t b2 = 0;
+    OA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, degree, off32, (int)(nv + 1), s));
+    void *tmp5 = R.alloc<char>(b2);
+    OA_HIP(hipcub::DeviceScan::ExclusiveSum(tmp5, b2, degree, off32, (int)(nv + 1), s));
+    hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(nv + 1)), dim3(256), 0, s, off32, nv + 1, R.csr_off);
+    OA_HIP(hipStreamSynchronize(s));
+    OA_HIP(hipGetLastError());
+    R.stats.ms_graph = ms(t0);
+    R.stats.ms_total = ms(t_all);
+}
+
+} // namespace ompl_amd
+
+struct ompl_amd_prm_s {
+    ompl_amd::PrmRoadmap R;
+};
+
+using namespace ompl_amd;
+
+extern "C" {
+
+int ompl_amd_prm_build(ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k, double lvs,
+                       unsigned factor, ompl_amd_prm *out, ompl_amd_prm_stats *stats)
+{
+    return guarded([&] {
+        OA_REQUIRE(env && out, OMPL_AMD_ERR_ARG, "null argument");
+        OA_REQUIRE(n_streams >= 1 && n_samples >= 1 && n_samples < ((int64_t)1 << 31), OMPL_AMD_ERR_ARG,
+                   "bad sample counts");
+        OA_REQUIRE(k >= 1 && k <= 63, OMPL_AMD_ERR_ARG, "k must be in [1, 63]");
+        OA_REQUIRE(2 * (n_samples + n_streams) * (int64_t)k < ((int64_t)1 << 31), OMPL_AMD_ERR_ARG,
+                   "n_samples * k too large for one roadmap");
+        OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "bad segment settings");
+        auto *h = new ompl_amd_prm_s();
+        try {
+            build(h->R, env, seed, n_streams, n_samples, k, lvs, factor);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        PrmRoadmap &R = h->R;
+        R.stats.samples = R.samples;
+        R.stats.milestones = R.nv;
+        R.stats.candidate_edges = R.ne;
+        R.stats.valid_edges = R.valid_edges;
+        R.stats.components = R.components;
+        if (stats) *stats = R.stats;
+        *out = h;
+    });
+}
+
+int ompl_amd_prm_destroy(ompl_amd_prm h)
+{
+    return guarded([&] { delete h; });
+}
+
+int ompl_amd_prm_milestones(ompl_amd_prm h, double *nodes, int32_t *sample_index, int64_t cap)
+{
+    return guarded([&] {
+        OA_REQUIRE(h, OMPL_AMD_ERR_ARG, "null roadmap");
+        const PrmRoadmap &R = h->R;
+        const int64_t n = std::min(cap, R.nv);
+        if (nodes && n) OA_HIP(hipMemcpy(nodes, R.nodes, (size_t)n * kDim * sizeof(double), hipMemcpyDeviceToHost));
+        if (sample_index && n)
+            OA_HIP(hipMemcpy(sample_index, R.milestone, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    });
+}
+
+int ompl_amd_prm_edges(ompl_amd_prm h, int32_t *u, int32_t *v, uint8_t *valid, int64_t cap)
+{
+    return guarded([&] {
+        OA_REQUIRE(h, OMPL_AMD_ERR_ARG, "null roadmap");
+        const PrmRoadmap &R = h->R;
+        const int64_t n = std::min(cap, R.ne);
+        if (!n) return;
+        if (u) OA_HIP(hipMemcpy(u, R.eu, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (v) OA_HIP(hipMemcpy(v, R.ev, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (valid) OA_HIP(hipMemcpy(valid, R.evalid, (size_t)n, hipMemcpyDeviceToHost));
+    });
+}
+
+int ompl_amd_prm_components(ompl_amd_prm h, int32_t *label, int64_t cap)
+{
+    return guarded([&] {
+        OA_REQUIRE(h && label, OMPL_AMD_ERR_ARG, "null argument");
+        const PrmRoadmap &R = h->R;
+        const int64_t n = std::min(cap, R.nv);
+        if (n) OA_HIP(hipMemcpy(label, R.label, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    });
+}
+
+int ompl_amd_prm_graph(ompl_amd_prm h, int64_t *offsets, int32_t *adjacency, int64_t adjacency_cap)
+{
+    return guarded([&] {
+        OA_REQUIRE(h && offsets, OMPL_AMD_ERR_ARG, "null argument");
+        const PrmRoadmap &R = h->R;
+        OA_REQUIRE(!adjacency || adjacency_cap >= 2 * R.valid_edges, OMPL_AMD_ERR_ARG,
+                   "adjacency needs %lld entries", (long long)(2 * R.valid_edges));
+        OA_HIP(hipMemcpy(offsets, R.csr_off, ((size_t)R.nv + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (adjacency && R.valid_edges)
+            OA_HIP(hipMemcpy(adjacency, R.csr_adj, (size_t)(2 * R.valid_edges) * sizeof(int32_t),
+                             hipMemcpyDeviceToHost));
+    });
+}
+
+} // extern "C"

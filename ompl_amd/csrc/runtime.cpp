@@ -137,6 +137,19 @@ ThreadResources &thread_resources()
     return r;
 }
 
+void stage_reserve(ThreadResources &r, size_t bytes)
+{
+    if (bytes <= r.stage_cap) return;
+    size_t cap = r.stage_cap ? r.stage_cap : 4096;
+    while (cap < bytes) cap *= 2;
+    if (r.stage) OA_HIP(hipHostFree(r.stage));
+    r.stage = nullptr;
+    r.stage_cap = 0;
+    OA_HIP(hipHostMalloc((void **)&r.stage, cap, hipHostMallocCoherent | hipHostMallocMapped));
+    OA_HIP(hipHostGetDevicePointer((void **)&r.stage_dev, r.stage, 0));
+    r.stage_cap = cap;
+}
+
 namespace {
 struct ProfRec {
     std::string name;

@@ -32,8 +32,13 @@ struct ThreadResources {
     hipStream_t stream = nullptr;
     int32_t *pinned = nullptr;     // 4 KiB, host-coherent, device-mapped
     int32_t *pinned_dev = nullptr; // device address of the same memory
+    char *stage = nullptr;         // growable host-coherent, device-mapped staging
+    char *stage_dev = nullptr;
+    size_t stage_cap = 0;
 };
 ThreadResources &thread_resources();
+// Ensures r.stage holds `bytes` (callers synchronize r.stream before reuse).
+void stage_reserve(ThreadResources &r, size_t bytes);
 
 // Launch shapes (mirrors ompl_amd_tuning).
 struct Tuning {

@@ -16,11 +16,15 @@
 // each phase reading its inputs before any lane writes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "runtime.hpp"
 
 namespace ompl_amd {
 
 constexpr int kMtN = 624;
+constexpr int kDim7 = OMPL_AMD_ARM_DOF;
 constexpr int kMtM = 397;
 constexpr uint32_t kMtMatrix = 0x9908b0dfu;
 constexpr uint32_t kUpper = 0x80000000u, kLower = 0x7fffffffu;
@@ -137,6 +141,135 @@ __global__ void __launch_bounds__(64) sample_uniform_kernel(SampleArgs A)
     }
 }
 
+// ---------------------------------------------------------------------------
+// UniformValidStateSampler::sample (base/samplers/src/UniformValidStateSampler.cpp:
+// 46-57), batched over streams: each stream is one sampler making successive
+// sample() calls; a call draws uniform states until one is valid or `attempts`
+// draws were made.  The draws of a stream depend on the validity of earlier
+// ones, so the batch runs in rounds: every unfinished stream draws C more
+// states (its generator resumes where it stopped), all are checked in one
+// batched validity launch, and one thread per stream hands them out to its
+// calls in order.
+
+struct MtState {
+    uint32_t mt[kMtN];
+    int32_t idx; // next word; kMtN = twist before the next draw
+    int32_t pad[3];
+};
+
+struct ValidCursor {
+    int64_t call;     // next call of this stream
+    int32_t attempts; // draws made by the current call
+    int32_t pad;
+};
+
+__global__ void mt_seed_kernel(const uint32_t *__restrict__ seeds, int n, MtState *__restrict__ st)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t v = seeds[s];
+    st[s].mt[0] = v;
+    for (int i = 1; i < kMtN; ++i) {
+        v = 1812433253u * (v ^ (v >> 30)) + (uint32_t)i;
+        st[s].mt[i] = v;
+    }
+    st[s].idx = kMtN;
+}
+
+// One wave per stream: resumes the generator, writes C states of 7 doubles
+// ([stream][C][7]) and their FP32 copy ([7][n_active * C]).  Streams listed in
+// `active` only (compacted by the host from the previous round).
+__global__ void __launch_bounds__(64) resume_uniform_kernel(MtState *__restrict__ st, const int32_t *__restrict__ active,
+                                                            int n_active, int64_t C, const double *__restrict__ lo_g,
+                                                            const double *__restrict__ hi_g, double *__restrict__ out,
+                                                            float *__restrict__ out_f32)
+{
+    __shared__ uint32_t mt[kMtN];
+    __shared__ double lo[kDim7], span[kDim7];
+    const int a = blockIdx.x;
+    const int t = threadIdx.x;
+    MtState &S = st[active[a]];
+    for (int i = t; i < kMtN; i += 64) mt[i] = S.mt[i];
+    if (t < kDim7) {
+        lo[t] = lo_g[t];
+        span[t] = hi_g[t] - lo_g[t];
+    }
+    int idx = S.idx;
+    __syncthreads();
+    const int64_t total = C * kDim7;
+    const int64_t n_f32 = (int64_t)n_active * C;
+    double *o = out + (int64_t)a * total;
+    for (int64_t z = 0; z < total;) {
+        if (idx >= kMtN) {
+            mt_twist(mt);
+            idx = 0;
+        }
+        const int64_t cnt = min((int64_t)((kMtN - idx) / 2), total - z);
+        for (int p = t; p < cnt; p += 64) {
+            const uint32_t w1 = mt_temper(mt[idx + 2 * p]);
+            const uint32_t w2 = mt_temper(mt[idx + 2 * p + 1]);
+            const int64_t zz = z + p;
+            const int d = (int)(zz % kDim7);
+            const double v = span[d] * canonical53(w1, w2) + lo[d];
+            o[zz] = v;
+            out_f32[(int64_t)d * n_f32 + (int64_t)a * C + zz / kDim7] = (float)v;
+        }
+        __syncthreads();
+        idx += 2 * (int)cnt;
+        z += cnt;
+    }
+    for (int i = t; i < kMtN; i += 64) S.mt[i] = mt[i];
+    if (t == 0) S.idx = idx;
+}
+
+// One thread per active stream: hands the round's draws to its calls in order.
+__global__ void assign_valid_kernel(ValidCursor *__restrict__ cur, const int32_t *__restrict__ active, int n_active,
+                                    int64_t C, const uint8_t *__restrict__ valid, const double *__restrict__ cand,
+                                    int64_t n_calls, int attempts, double *__restrict__ out,
+                                    uint8_t *__restrict__ found, int32_t *__restrict__ draws,
+                                    unsigned long long *__restrict__ stats)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_active) return;
+    const int s = active[a];
+    ValidCursor c = cur[s];
+    int64_t used = 0, used_valid = 0;
+    for (int64_t j = 0; j < C && c.call < n_calls; ++j) {
+        const bool v = valid[(int64_t)a * C + j] != 0;
+        ++c.attempts;
+        ++used;
+        used_valid += v;
+        if (v || c.attempts >= attempts) {
+            const int64_t o = (int64_t)s * n_calls + c.call;
+#pragma unroll
+            for (int d = 0; d < kDim7; ++d) out[o * kDim7 + d] = cand[((int64_t)a * C + j) * kDim7 + d];
+            found[o] = v ? 1 : 0;
+            if (draws) draws[o] = c.attempts;
+            ++c.call;
+            c.attempts = 0;
+        }
+    }
+    cur[s] = c;
+    atomicAdd(&stats[0], (unsigned long long)used);
+    atomicAdd(&stats[1], (unsigned long long)used_valid);
+    if (c.call < n_calls) atomicMax(&stats[2], (unsigned long long)(n_calls - c.call));
+}
+
+__global__ void next_active_kernel(const ValidCursor *__restrict__ cur, int n_streams, int64_t n_calls,
+                                   int32_t *__restrict__ active, unsigned long long *__restrict__ stats)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool open = s < n_streams && cur[s].call < n_calls;
+    const uint64_t m = __ballot(open);
+    int base = 0;
+    const int lane = threadIdx.x & 63;
+    if (m) {
+        if (lane == __ffsll((unsigned long long)m) - 1) base = (int)atomicAdd(&stats[3], (unsigned long long)__popcll(m));
+        base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+    }
+    if (open) active[base + __popcll(m & (((uint64_t)1 << lane) - 1))] = s;
+}
+
 } // namespace ompl_amd
 
 using namespace ompl_amd;
@@ -168,5 +301,94 @@ extern "C" int ompl_amd_sample_uniform_dev(const uint32_t *seeds, int n_streams,
         }
         OA_HIP(hipFreeAsync(stage, s));
         OA_HIP(hipStreamSynchronize(s)); // host arrays were staged with async copies
+    });
+}
+
+extern "C" int ompl_amd_sample_valid_dev(ompl_amd_env env, const uint32_t *seeds, int n_streams, int64_t n_calls,
+                                         int attempts, const double *lo, const double *hi, double *out_dev,
+                                         uint8_t *found_dev, int32_t *draws_dev, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(env, OMPL_AMD_ERR_ARG, "null environment");
+        OA_REQUIRE(n_streams >= 0 && n_calls >= 0 && attempts >= 1, OMPL_AMD_ERR_ARG, "bad sizes");
+        if (n_streams == 0 || n_calls == 0) return;
+        OA_REQUIRE(seeds && lo && hi && out_dev && found_dev, OMPL_AMD_ERR_ARG, "null pointer");
+        hipStream_t s = resolve_stream(stream);
+        // device state: generators, cursors, active list, round statistics, bounds, seeds
+        const size_t head = 256;
+        const size_t bytes = head + (size_t)n_streams * (sizeof(MtState) + sizeof(ValidCursor) + 2 * sizeof(int32_t));
+        char *base = nullptr;
+        OA_HIP(hipMallocAsync((void **)&base, bytes, s));
+        unsigned long long *stats = (unsigned long long *)base; // [0] used [1] valid [2] max remaining [3] active
+        double *dlo = (double *)(base + 64), *dhi = dlo + kDim7;
+        MtState *gen = (MtState *)(base + head);
+        ValidCursor *cur = (ValidCursor *)(gen + n_streams);
+        int32_t *active = (int32_t *)(cur + n_streams);
+        uint32_t *dseeds = (uint32_t *)(active + n_streams);
+        OA_HIP(hipMemcpyAsync(dlo, lo, kDim7 * sizeof(double), hipMemcpyHostToDevice, s));
+        OA_HIP(hipMemcpyAsync(dhi, hi, kDim7 * sizeof(double), hipMemcpyHostToDevice, s));
+        OA_HIP(hipMemcpyAsync(dseeds, seeds, (size_t)n_streams * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        OA_HIP(hipMemsetAsync(cur, 0, (size_t)n_streams * sizeof(ValidCursor), s));
+        hipLaunchKernelGGL(mt_seed_kernel, dim3((n_streams + 63) / 64), dim3(64), 0, s, dseeds, n_streams, gen);
+        OA_HIP(hipGetLastError());
+        std::vector<int32_t> all((size_t)n_streams);
+        for (int i = 0; i < n_streams; ++i) all[(size_t)i] = i;
+        OA_HIP(hipMemcpyAsync(active, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        int n_active = n_streams;
+        int64_t remaining = n_calls;
+        double draws_per_call = 1.0; // refined from every round
+        double *cand = nullptr;
+        float *cand32 = nullptr;
+        uint8_t *valid = nullptr;
+        size_t cap = 0;
+        const int64_t max_round = ((int64_t)64 << 20) / ((int64_t)n_streams * kDim7 * 8) + 16; // ~64 MiB of draws
+        int64_t rounds = 0;
+        const int64_t round_limit = n_calls * (int64_t)attempts + 2;
+        while (n_active > 0) {
+            if (++rounds > round_limit) fail(OMPL_AMD_ERR_INTERNAL, "valid sampler did not finish");
+            const double want = (double)remaining * std::min((double)attempts, draws_per_call) * 1.25 + 16.0;
+            const int64_t C = std::max<int64_t>(16, std::min<int64_t>((int64_t)want, std::min<int64_t>(
+                                                        max_round, remaining * (int64_t)attempts)));
+            const size_t need = (size_t)n_active * (size_t)C;
+            if (need > cap) {
+                if (cand) {
+                    OA_HIP(hipFreeAsync(cand, s));
+                    OA_HIP(hipFreeAsync(cand32, s));
+                    OA_HIP(hipFreeAsync(valid, s));
+                }
+                cap = need;
+                OA_HIP(hipMallocAsync((void **)&cand, cap * kDim7 * sizeof(double), s));
+                OA_HIP(hipMallocAsync((void **)&cand32, cap * kDim7 * sizeof(float), s));
+                OA_HIP(hipMallocAsync((void **)&valid, cap, s));
+            }
+            {
+                ProfScope ps("sample_valid", s);
+                hipLaunchKernelGGL(resume_uniform_kernel, dim3(n_active), dim3(64), 0, s, gen, active, n_active, C,
+                                   dlo, dhi, cand, cand32);
+                OA_HIP(hipGetLastError());
+            }
+            const int rc = ompl_amd_validate_states_dev(env, cand32, (int64_t)need, valid, s);
+            if (rc != OMPL_AMD_OK) fail(rc, "%s", last_error());
+            OA_HIP(hipMemsetAsync(stats, 0, 4 * sizeof(unsigned long long), s));
+            hipLaunchKernelGGL(assign_valid_kernel, dim3((n_active + 63) / 64), dim3(64), 0, s, cur, active,
+                               n_active, C, valid, cand, n_calls, attempts, out_dev, found_dev, draws_dev, stats);
+            hipLaunchKernelGGL(next_active_kernel, dim3((n_streams + 255) / 256), dim3(256), 0, s, cur, n_streams,
+                               n_calls, active, stats);
+            OA_HIP(hipGetLastError());
+            unsigned long long h[4];
+            OA_HIP(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
+            OA_HIP(hipStreamSynchronize(s));
+            n_active = (int)h[3];
+            remaining = (int64_t)h[2];
+            if (h[1] > 0) draws_per_call = std::max(1.0, (double)h[0] / (double)h[1]);
+            else draws_per_call = (double)attempts;
+        }
+        if (cand) {
+            OA_HIP(hipFreeAsync(cand, s));
+            OA_HIP(hipFreeAsync(cand32, s));
+            OA_HIP(hipFreeAsync(valid, s));
+        }
+        OA_HIP(hipFreeAsync(base, s));
+        OA_HIP(hipStreamSynchronize(s));
     });
 }

@@ -273,6 +273,73 @@ int orc_check_motion(const orc_env *env, const double a[ORC_ARM_DOF], const doub
     return result;
 }
 
+/* SpaceInformation::checkMotion(states, count) -- base/src/SpaceInformation.cpp:301-341:
+ * count 1 -> isValid(front); count >= 2 -> front, back, then a FIFO of (first, last) index pairs
+ * seeded with (0, count-1): check mid = (first+last)/2, push (first, mid) if first < mid-1 and
+ * (mid, last) if last > mid+1.  *checked = number of isValid calls made. */
+int orc_check_states(const orc_env *env, const double *states, int count, int *checked)
+{
+    int cnt = 0, result = 1;
+    if (count == 1) {
+        cnt = 1;
+        result = orc_state_valid_d(env, states, NULL);
+    } else if (count > 1) {
+        ++cnt;
+        if (!orc_state_valid_d(env, states, NULL)) {
+            result = 0;
+            goto done;
+        }
+        ++cnt;
+        if (!orc_state_valid_d(env, states + (size_t)(count - 1) * ORC_ARM_DOF, NULL)) {
+            result = 0;
+            goto done;
+        }
+        if (count > 2) {
+            /* each index pair is pushed at most once per interior index: count slots suffice */
+            int *qa = (int *)malloc(sizeof(int) * 2 * (size_t)count);
+            int head = 0, tail = 0;
+            qa[2 * tail] = 0;
+            qa[2 * tail + 1] = count - 1;
+            ++tail;
+            while (head < tail) {
+                const int f = qa[2 * head], l = qa[2 * head + 1];
+                const int mid = (f + l) / 2;
+                ++cnt;
+                if (!orc_state_valid_d(env, states + (size_t)mid * ORC_ARM_DOF, NULL)) {
+                    result = 0;
+                    break;
+                }
+                ++head;
+                if (f < mid - 1) {
+                    qa[2 * tail] = f;
+                    qa[2 * tail + 1] = mid;
+                    ++tail;
+                }
+                if (l > mid + 1) {
+                    qa[2 * tail] = mid;
+                    qa[2 * tail + 1] = l;
+                    ++tail;
+                }
+            }
+            free(qa);
+        }
+    }
+done:
+    if (checked) *checked = cnt;
+    return result;
+}
+
+/* SpaceInformation::checkMotion(states, count, firstInvalidStateIndex) -- :288-299: sequential scan. */
+int orc_check_states_first_invalid(const orc_env *env, const double *states, int count, int *first_invalid)
+{
+    for (int i = 0; i < count; ++i)
+        if (!orc_state_valid_d(env, states + (size_t)i * ORC_ARM_DOF, NULL)) {
+            if (first_invalid) *first_invalid = i;
+            return 0;
+        }
+    return 1;
+}
+
 int orc_check_motion_last_valid(const orc_env *env, const double a[ORC_ARM_DOF], const double b[ORC_ARM_DOF],
                                 double lvs, unsigned factor, double *last_t, int *checked)
 {
@@ -362,6 +429,25 @@ void orc_sample_uniform(orc_mt19937 *g, const double *lo, const double *hi, int 
     for (long i = 0; i < n; ++i)
         for (int d = 0; d < dim; ++d)
             out[i * dim + d] = orc_uniform_real(g, lo[d], hi[d]);
+}
+
+/* UniformValidStateSampler::sample -- base/samplers/src/UniformValidStateSampler.cpp:46-57:
+ * do { sampleUniform(state); valid = isValid(state); ++attempts; } while (!valid && attempts < attempts_);
+ * n_calls successive calls on one generator; out [n_calls][7] = each call's final state. */
+void orc_sample_valid(const orc_env *env, orc_mt19937 *g, const double *lo, const double *hi, long n_calls,
+                      int attempts, double *out, uint8_t *found, int *draws)
+{
+    for (long c = 0; c < n_calls; ++c) {
+        double *st = out + c * ORC_ARM_DOF;
+        int a = 0, valid = 0;
+        do {
+            orc_sample_uniform(g, lo, hi, ORC_ARM_DOF, 1, st);
+            valid = orc_state_valid_d(env, st, NULL);
+            ++a;
+        } while (!valid && a < attempts);
+        found[c] = (uint8_t)valid;
+        if (draws) draws[c] = a;
+    }
 }
 
 void orc_quaternion(orc_mt19937 *g, double q[4])

@@ -77,6 +77,8 @@ int orc_check_motion(const orc_env *env, const double a[ORC_ARM_DOF], const doub
                      double lvs, unsigned factor, int *checked);
 /* checkMotion(s1,s2,lastValid): sequential j=1..nd-1 then s2; on failure
  * *last_t = (j-1)/nd (or (nd-1)/nd when s2 fails); *last_t untouched if valid. */
+int orc_check_states(const orc_env *env, const double *states, int count, int *checked);
+int orc_check_states_first_invalid(const orc_env *env, const double *states, int count, int *first_invalid);
 int orc_check_motion_last_valid(const orc_env *env, const double a[ORC_ARM_DOF], const double b[ORC_ARM_DOF],
                                 double lvs, unsigned factor, double *last_t, int *checked);
 
@@ -91,6 +93,8 @@ double orc_canonical(orc_mt19937 *g);
 double orc_uniform_real(orc_mt19937 *g, double lo, double hi);
 /* n states of dim doubles, row-major, exactly n*dim sequential uniformReal calls */
 void orc_sample_uniform(orc_mt19937 *g, const double *lo, const double *hi, int dim, long n, double *out);
+void orc_sample_valid(const orc_env *env, orc_mt19937 *g, const double *lo, const double *hi, long n_calls,
+                      int attempts, double *out, uint8_t *found, int *draws);
 void orc_quaternion(orc_mt19937 *g, double q[4]);
 
 /* ---- nearest neighbours (FP32 metric, ties broken by lower index) ---- */

@@ -64,6 +64,9 @@ def lib():
         L.orc_check_motion.argtypes = [C.POINTER(Env), vp, vp, C.c_double, C.c_uint, C.POINTER(C.c_int)]
         L.orc_check_motion_last_valid.argtypes = [C.POINTER(Env), vp, vp, C.c_double, C.c_uint,
                                                   C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.orc_check_states.argtypes = [C.POINTER(Env), vp, C.c_int, C.POINTER(C.c_int)]
+        L.orc_check_states_first_invalid.argtypes = [C.POINTER(Env), vp, C.c_int, C.POINTER(C.c_int)]
+        L.orc_sample_valid.argtypes = [C.POINTER(Env), C.POINTER(MT), vp, vp, C.c_long, C.c_int, vp, vp, vp]
         L.orc_mt_seed.argtypes = [C.POINTER(MT), C.c_uint32]
         L.orc_mt_next.argtypes = [C.POINTER(MT)]
         L.orc_mt_next.restype = C.c_uint32
@@ -167,6 +170,22 @@ def check_motion(env: OracleEnv, a, b, lvs, factor=1):
     return bool(v), cnt.value
 
 
+def check_states(env: OracleEnv, states):
+    """SpaceInformation::checkMotion(states, count) in the reference's visit order -> (valid, checks)."""
+    q = np.ascontiguousarray(np.asarray(states, np.float64).reshape(-1, 7))
+    cnt = C.c_int()
+    v = lib().orc_check_states(env.ref(), _p(q), q.shape[0], C.byref(cnt))
+    return bool(v), cnt.value
+
+
+def check_states_first_invalid(env: OracleEnv, states):
+    """checkMotion(states, count, firstInvalidStateIndex) -> (valid, first invalid index or -1)."""
+    q = np.ascontiguousarray(np.asarray(states, np.float64).reshape(-1, 7))
+    f = C.c_int(-1)
+    v = lib().orc_check_states_first_invalid(env.ref(), _p(q), q.shape[0], C.byref(f))
+    return bool(v), f.value
+
+
 def check_motion_last_valid(env: OracleEnv, a, b, lvs, factor=1, last_t=float("nan")):
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
@@ -237,6 +256,17 @@ class Mt19937:
         out = np.zeros((n, lo.shape[0]), np.float64)
         lib().orc_sample_uniform(C.byref(self.g), _p(lo), _p(hi), lo.shape[0], n, _p(out))
         return out
+
+    def sample_valid(self, env: "OracleEnv", n_calls: int, attempts: int = 100, lo=None, hi=None):
+        """UniformValidStateSampler::sample x n_calls -> (states [n][7], found u8 [n], draws i32 [n])."""
+        lo = np.ascontiguousarray(np.full(7, -np.pi) if lo is None else lo, np.float64)
+        hi = np.ascontiguousarray(np.full(7, np.pi) if hi is None else hi, np.float64)
+        out = np.zeros((n_calls, 7), np.float64)
+        found = np.zeros(n_calls, np.uint8)
+        draws = np.zeros(n_calls, np.int32)
+        lib().orc_sample_valid(env.ref(), C.byref(self.g), _p(lo), _p(hi), n_calls, attempts, _p(out), _p(found),
+                               _p(draws))
+        return out, found, draws
 
     def quaternion(self) -> np.ndarray:
         q = np.zeros(4, np.float64)
@@ -316,3 +346,60 @@ def ref_linear(metric: int, data: np.ndarray, queries: np.ndarray, k: int, r: fl
         else:
             nearest_res.append(int(parts[1]))
     return knn_res, rad_res, nearest_res
+
+
+def disjoint_set_labels(n: int, edges: np.ndarray, valid: np.ndarray) -> np.ndarray:
+    """PRM::uniteComponents over the valid edges (geometric/planners/prm/src/PRM.cpp:597, 606-609:
+    boost::disjoint_sets union_set), restated as union by smaller root with path halving; the label
+    of a component is its smallest element."""
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    for (u, v), ok in zip(edges.tolist(), valid.tolist()):
+        if ok:
+            ru, rv = find(u), find(v)
+            if ru != rv:
+                parent[max(ru, rv)] = min(ru, rv)
+    return np.array([find(i) for i in range(n)], np.int32)
+
+
+def prm_batch(env: OracleEnv, seed: int, n_samples: int, k: int, lvs: float, factor: int = 1, n_streams: int = 1,
+              threads: int = 1):
+    """Batch PRM roadmap: the steps of PRM::growRoadmap / addMilestone (geometric/planners/prm/src/
+    PRM.cpp:364-385, 570-604) with KStrategy (geometric/planners/prm/ConnectionStrategy.h:56-81) over a
+    fixed sample set.  Samples: stream s = mt19937(seed + s), n_samples rounded up to a multiple of
+    n_streams, stream-major; milestones = valid samples in order; candidates = undirected union of the
+    k nearest other milestones of each milestone (FP32 L2, ties to the lower index); each candidate
+    (u < v) checked from u to v (PRM.cpp:590 checks from the older milestone to the new one).
+    Returns a dict of numpy arrays."""
+    per = -(-n_samples // n_streams)
+    lo, hi = np.full(7, -np.pi), np.full(7, np.pi)
+    samples = np.concatenate([Mt19937(seed + s).sample_uniform(lo, hi, per) for s in range(n_streams)])
+    valid = validate_states_mt(env, samples.T.astype(np.float32), threads).astype(bool)
+    sidx = np.nonzero(valid)[0].astype(np.int32)
+    nodes = samples[sidx]
+    nv = nodes.shape[0]
+    out = dict(samples=samples, valid=valid, sample_index=sidx, milestones=nodes)
+    if nv < 2:
+        out.update(edges=np.zeros((0, 2), np.int32), edge_valid=np.zeros(0, bool), component=np.arange(nv,
+                   dtype=np.int32))
+        return out
+    kk = min(k + 1, nv)
+    f32 = nodes.astype(np.float32)
+    idx, _ = knn_mt(0, f32, f32, kk, threads)
+    keys = set()
+    for i in range(nv):
+        taken = 0
+        for j in idx[i].tolist():
+            if taken < k and j >= 0 and j != i:
+                keys.add((min(i, j), max(i, j)))
+                taken += 1
+    edges = np.array(sorted(keys), np.int32).reshape(-1, 2)
+    ok, _ = check_motions(env, nodes[edges[:, 0]], nodes[edges[:, 1]], lvs, factor, threads)
+    out.update(edges=edges, edge_valid=ok.astype(bool), component=disjoint_set_labels(nv, edges, ok.astype(bool)))
+    return out

@@ -44,3 +44,22 @@ def test_sampler_matches_oracle(gpu, oracle, dim, n):
         ref = oracle.Mt19937(seed).sample_uniform(lo, hi, n)
         assert np.array_equal(out[s], ref), (dim, n, seed)
     assert np.array_equal(f32, out.reshape(-1, dim).T.astype(np.float32))
+
+
+@pytest.mark.parametrize("n_obs,streams,calls,attempts", [(16, 1, 50, 100), (32, 7, 20, 100), (64, 300, 8, 3),
+                                                          (256, 4, 5, 5), (32, 64, 1, 1)])
+def test_sample_valid_matches_oracle(gpu, oracle, n_obs, streams, calls, attempts):
+    from ompl_amd.sampler import sample_valid
+
+    s, b = gpu.workload.obstacles(1002, n_obs, n_obs)
+    env = gpu.Env(s, b)
+    oenv = oracle.OracleEnv(s, b)
+    seeds = np.arange(streams, dtype=np.uint32) * 7919 + 5
+    print(f"sample_valid obstacles {2 * n_obs} streams {streams} calls {calls} attempts {attempts}", flush=True)
+    out, found, draws = sample_valid(env, seeds, calls, attempts=attempts)
+    out, found, draws = out.cpu().numpy(), found.cpu().numpy(), draws.cpu().numpy()
+    for i, sd in enumerate(seeds.tolist()):
+        o_states, o_found, o_draws = oracle.Mt19937(sd).sample_valid(oenv, calls, attempts)
+        assert np.array_equal(draws[i], o_draws), i
+        assert np.array_equal(found[i], o_found), i
+        assert np.array_equal(out[i], o_states), i
