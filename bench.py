@@ -151,7 +151,9 @@ def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
     for name, env_seed, ns, nb in (("1024_obstacles", 1003, 512, 512), ("64_spheres", 1000, 64, 0)):
         s, b = oa.workload.obstacles(env_seed, ns, nb)
         env = oa.Env(s, b)
-        build_roadmap(env, PRM_SEED, 4096, n_streams=64, fetch=False)  # warm the pipeline
+        # warm the pipeline and tune the motion launch shape on this roadmap's candidate edges
+        build_roadmap(env, PRM_SEED, n_samples, n_streams=256, fetch=False, autotune=True)
+        tuning = oa.get_tuning()
         reps = []
         for _ in range(3):
             t0 = time.perf_counter()
@@ -161,7 +163,8 @@ def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
         row = {"samples": st["samples"], "milestones": st["milestones"], "candidate_edges": st["candidate_edges"],
                "valid_edges": st["valid_edges"], "components": st["components"], "build_ms": 1e3 * best,
                "stage_ms": {k[3:]: round(st[k], 3) for k in st if k.startswith("ms_")},
-               "edge_checks_per_s": st["candidate_edges"] / (st["ms_edges"] * 1e-3) if st["ms_edges"] > 0 else None}
+               "edge_checks_per_s": st["candidate_edges"] / (st["ms_motions"] * 1e-3) if st["ms_motions"] > 0 else None,
+               "motion_tuning": tuning}
         if run_cpu:
             from oracle import oracle_py
 
@@ -180,6 +183,140 @@ def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
     return out
 
 
+def timed(fn, reps, warm=2):
+    """Average milliseconds of fn() over reps launches (HIP events on torch's
+    current stream, which is the stream the library launches on)."""
+    import torch
+
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def scope_rows(oa, run_cpu, cpu_threads):
+    """Throughput of the other rows of the hot path, each on its configuration,
+    with its algorithmic work and a bounded CPU-restatement baseline."""
+    import torch
+
+    from ompl_amd.nn import NearestNeighborsGPU
+    from ompl_amd.sampler import sample_uniform, sample_valid
+
+    rows = {}
+    oracle_py = None
+    if run_cpu:
+        from oracle import oracle_py
+
+    def roof(flops, ms, peak=PEAK_FP32_TFLOPS):
+        a = flops / (ms * 1e-3) / 1e12
+        return {"achieved_tflops": a, "frac": a / peak, "algorithmic_gflop": flops / 1e9}
+
+    # validity: 65,536 configurations U[-pi,pi]^7 against 64 spheres
+    s, b = oa.workload.obstacles(1000, 64, 0)
+    env = oa.Env(s, b)
+    q = oa.workload.configs(1, 65536)
+    qd = torch.from_numpy(q.T.astype(np.float32).copy()).cuda()
+    out = torch.empty(65536, dtype=torch.uint8, device="cuda")
+    ms = timed(lambda: oa.validate_states(env, qd, out=out), 50)
+    w = oa.validity_work(env, qd)
+    row = {"states": 65536, "obstacles": 64, "ms": ms, "states_per_s": 65536 / (ms * 1e-3),
+           "valid_fraction": float(out.float().mean().item()), "roofline": roof(oa.work_flops(w), ms)}
+    if oracle_py is not None:
+        oenv = oracle_py.OracleEnv(s, b)
+        t0 = time.perf_counter()
+        ref = oracle_py.validate_states_mt(oenv, q.T.astype(np.float32), cpu_threads)
+        row["cpu_baseline"] = {"states_per_s": 65536 / (time.perf_counter() - t0), "cores": cpu_threads,
+                               "kind": "port", "sample": "the full 65,536-state batch"}
+        row["parity_bit_exact"] = bool(np.array_equal(ref, out.cpu().numpy()))
+    rows["validity_64_spheres"] = row
+
+    # k-NN, R^7 L2: 100,000 points, 4,096 queries, k = 10
+    db = oa.workload.configs(2, 100_000).astype(np.float32)
+    qs = oa.workload.configs(3, 4096).astype(np.float32)
+    nn = NearestNeighborsGPU(oa.METRIC_L2, 7)
+    nn.add(torch.from_numpy(db).cuda())
+    qsd = torch.from_numpy(qs).cuda()
+    oi = torch.empty((4096, 10), dtype=torch.int32, device="cuda")
+    od = torch.empty((4096, 10), dtype=torch.float32, device="cuda")
+    ms = timed(lambda: nn.knn_dev(qsd, 10, oi, od), 20)
+    row = {"db": 100_000, "queries": 4096, "k": 10, "ms": ms, "queries_per_s": 4096 / (ms * 1e-3),
+           "roofline": roof(20.0 * 100_000 * 4096, ms)}
+    if oracle_py is not None:
+        t0 = time.perf_counter()
+        ri, _ = oracle_py.knn_mt(0, db, qs[:512], 10, cpu_threads)
+        dt = time.perf_counter() - t0
+        row["cpu_baseline"] = {"queries_per_s": 512 / dt, "cores": cpu_threads, "kind": "port",
+                               "sample": f"512 of the 4,096 queries, {dt:.1f} s"}
+        row["parity_bit_exact_on_sample"] = bool(np.array_equal(ri, oi[:512].cpu().numpy()))
+    nn.close()
+    rows["knn_l2_100k"] = row
+
+    # SE3 at scale: 1,000,000 points, 65,536 queries, k = 16 and the RRT* radius
+    db = oa.workload.se3(5, 1_000_000)
+    qs = oa.workload.se3(6, 65536)
+    nn = NearestNeighborsGPU(oa.METRIC_SE3, 7)
+    nn.add(torch.from_numpy(db).cuda())
+    qsd = torch.from_numpy(qs).cuda()
+    oi = torch.empty((65536, 16), dtype=torch.int32, device="cuda")
+    od = torch.empty((65536, 16), dtype=torch.float32, device="cuda")
+    ms_k = timed(lambda: nn.knn_dev(qsd, 16, oi, od), 3, warm=1)
+    r = oa.rrtstar_radius(6, 8000.0 * np.pi ** 2, 1_000_000)
+    ms_r = timed(lambda: nn.radius_dev(qsd, r), 3, warm=1)
+    off, _, _ = nn.radius_dev(qsd, r)
+    pairs = 65536.0 * 1_000_000
+    row = {"db": 1_000_000, "queries": 65536, "k": 16, "radius": r, "knn_ms": ms_k,
+           "knn_queries_per_s": 65536 / (ms_k * 1e-3), "radius_ms": ms_r,
+           "radius_queries_per_s": 65536 / (ms_r * 1e-3), "radius_mean_hits": float(off[-1].item()) / 65536,
+           "roofline_knn": roof(20.0 * pairs, ms_k), "roofline_radius": roof(20.0 * pairs, ms_r),
+           "work_note": "brute-force pairs x 20 FLOP-eq (acos and sqrt counted as 1)"}
+    if oracle_py is not None:
+        t0 = time.perf_counter()
+        ri, _ = oracle_py.knn_mt(1, db, qs[:64], 16, cpu_threads)
+        dt = time.perf_counter() - t0
+        row["cpu_baseline"] = {"queries_per_s": 64 / dt, "cores": cpu_threads, "kind": "port",
+                               "sample": f"64 of the 65,536 k-NN queries, {dt:.1f} s"}
+        row["parity_bit_exact_on_sample"] = bool(np.array_equal(ri, oi[:64].cpu().numpy()))
+    nn.close()
+    rows["se3_1m"] = row
+
+    # uniform sampling: 256 generators x 782 states (200,192 states)
+    lo, hi = np.full(7, -np.pi), np.full(7, np.pi)
+    seeds = np.arange(256, dtype=np.uint32) + PRM_SEED
+    ms = timed(lambda: sample_uniform(seeds, 782, lo, hi, f32_soa=True), 10)
+    n = 256 * 782
+    rows["sample_uniform"] = {"states": n, "ms": ms, "states_per_s": n / (ms * 1e-3),
+                              "bytes_written_per_state": 84,
+                              "hbm_gbps": n * 84 / (ms * 1e-3) / 1e9,
+                              "note": "host-staged seeds and bounds and a stream sync are inside the time"}
+
+    # valid sampling: 256 samplers x 64 calls in the 128 + 128 obstacle scene (1.2% valid)
+    s2, b2 = oa.workload.obstacles(ENV_SEED, N_SPHERES, N_BOXES)
+    env2 = oa.Env(s2, b2)
+    ms = timed(lambda: sample_valid(env2, seeds, 64), 3, warm=1)
+    _, found, draws = sample_valid(env2, seeds, 64)
+    rows["sample_valid"] = {"samplers": 256, "calls": 64, "attempts": 100, "ms": ms,
+                            "calls_per_s": 256 * 64 / (ms * 1e-3), "found_fraction": float(found.float().mean()),
+                            "draws_per_s": float(draws.sum().item()) / (ms * 1e-3)}
+
+    # precomputed state lists: the motion states of the 16,384 benchmark edges, first 64 each
+    a_rows, b_rows = oa.workload.edges(EDGE_SEED, 4096)
+    t = np.linspace(0.0, 1.0, 64)
+    st = (a_rows[:, None, :] + (b_rows - a_rows)[:, None, :] * t[None, :, None]).reshape(-1, 7)
+    std = torch.from_numpy(st.T.copy()).cuda()
+    offs = torch.arange(0, 4097 * 64, 64, dtype=torch.int64, device="cuda")
+    ms = timed(lambda: oa.check_state_lists(env2, std, offs), 20)
+    v, f = oa.check_state_lists(env2, std, offs)
+    rows["state_lists"] = {"lists": 4096, "states_per_list": 64, "ms": ms, "lists_per_s": 4096 / (ms * 1e-3),
+                           "valid_fraction": float(v.float().mean().item())}
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +333,9 @@ def main(argv=None):
     ap.add_argument("--plain-workers", type=int, default=8)
     ap.add_argument("--prm-samples", type=int, default=PRM_SAMPLES, help="PRM roadmap batch size (0 = skip)")
     ap.add_argument("--prm-cpu-samples", type=int, default=20000)
+    ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run 5 device copies of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -204,6 +344,16 @@ def main(argv=None):
         return 2
 
     import torch
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        return 0
 
     import ompl_amd as oa
 
@@ -329,6 +479,11 @@ def main(argv=None):
         prm_res = prm_batch(oa, args.prm_samples, args.prm_cpu_samples,
                             args.cpu_threads or min(os.cpu_count() or 1, 16), not args.no_cpu)
 
+    rows_res = None
+    if not args.no_rows:
+        oa.set_tuning(**t)  # the batch's tuned shapes (the PRM leg tunes its own)
+        rows_res = scope_rows(oa, not args.no_cpu, args.cpu_threads or min(os.cpu_count() or 1, 16))
+
     res = {
         "metric": "edge-checks/s + p50/p99 plan latency, served vs plain GPU path, bit-exact",
         "value": value,
@@ -362,6 +517,7 @@ def main(argv=None):
         "parity_bit_exact": parity,
         "served_stream": stream_res,
         "prm_batch": prm_res,
+        "rows": rows_res,
         "wall_s": wall,
     }
     print(json.dumps(res))

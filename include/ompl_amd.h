@@ -42,7 +42,10 @@ extern "C" {
 /* ------------------------------------------------------------------ */
 int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
-int ompl_amd_version(void); /* major*10000 + minor*100 + patch */
+/* major*10000 + minor*100 + patch; bumped with every change of a signature or
+ * struct layout, so a binding can refuse a library built from other sources. */
+#define OMPL_AMD_VERSION 300
+int ompl_amd_version(void);
 int ompl_amd_synchronize(void *stream);
 
 /* Device memory helpers (thin wrappers over hipMalloc/hipMemcpy) for callers
@@ -333,8 +336,12 @@ typedef struct {
     int64_t candidate_edges;
     int64_t valid_edges;
     int64_t components;
-    double ms_sample, ms_validity, ms_knn, ms_edges, ms_graph, ms_total;
+    double ms_sample, ms_validity, ms_knn, ms_candidates, ms_motions, ms_graph, ms_total;
 } ompl_amd_prm_stats;
+
+/* flags of ompl_amd_prm_build */
+#define OMPL_AMD_PRM_AUTOTUNE 1u /* run the motion autotuner on (up to 32,768 of) the candidate edges first and
+                                    install the winner, as ompl_amd_autotune_motions */
 
 /* Samples U[-pi,pi]^7 from n_streams mt19937 generators seeded seed,
  * seed+1, ... (stream-major sample order; n_samples is rounded up to a
@@ -342,7 +349,8 @@ typedef struct {
  * builds the roadmap with k neighbours per milestone.  Milestone i is the
  * i-th valid sample.  Requires 2 * (n_samples + n_streams) * k < 2^31. */
 int ompl_amd_prm_build(ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k,
-                       double longest_valid_segment, unsigned factor, ompl_amd_prm *out, ompl_amd_prm_stats *stats);
+                       double longest_valid_segment, unsigned factor, unsigned flags, ompl_amd_prm *out,
+                       ompl_amd_prm_stats *stats);
 int ompl_amd_prm_destroy(ompl_amd_prm prm);
 /* Milestones (row-major FP64) and their sample indices. */
 int ompl_amd_prm_milestones(ompl_amd_prm prm, double *nodes, int32_t *sample_index, int64_t cap);

@@ -13,6 +13,8 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
+# the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
+ABI_VERSION = 300
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -79,7 +81,7 @@ SIGNATURES = {
     "ompl_amd_server_release": (C.c_int, [vp, C.c_int64]),
     "ompl_amd_server_stats": (C.c_int, [vp, vp]),
     "ompl_amd_plan_plain": (C.c_int, [vp, vp, vp, C.c_int, c_int_p]),
-    "ompl_amd_prm_build": (C.c_int, [vp, C.c_uint32, C.c_int, C.c_int64, C.c_int, C.c_double, C.c_uint,
+    "ompl_amd_prm_build": (C.c_int, [vp, C.c_uint32, C.c_int, C.c_int64, C.c_int, C.c_double, C.c_uint, C.c_uint,
                                      C.POINTER(vp), vp]),
     "ompl_amd_prm_destroy": (C.c_int, [vp]),
     "ompl_amd_prm_milestones": (C.c_int, [vp, vp, vp, C.c_int64]),
@@ -127,7 +129,8 @@ class PrmStats(C.Structure):
         ("ms_sample", C.c_double),
         ("ms_validity", C.c_double),
         ("ms_knn", C.c_double),
-        ("ms_edges", C.c_double),
+        ("ms_candidates", C.c_double),
+        ("ms_motions", C.c_double),
         ("ms_graph", C.c_double),
         ("ms_total", C.c_double),
     ]
@@ -151,6 +154,10 @@ def load(path: str | os.PathLike | None = None):
             continue
         fn.restype = res
         fn.argtypes = args
+    got = lib.ompl_amd_version()
+    if got != ABI_VERSION:
+        raise OmplAmdError(-8, f"{p} has ABI version {got}, these bindings need {ABI_VERSION}: rebuild the library "
+                               "(python -m ompl_amd.build)")
     if path is None:
         _lib = lib
     return lib

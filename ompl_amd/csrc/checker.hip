@@ -177,35 +177,51 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 }
 
 // Chunk-parallel walk of the survivors.  Virtual work items k = (chunk blk,
-// survivor s) in chunk-major order, one wave per item over a persistent grid;
-// an item is skipped when its chunk lies past the edge's last position or an
-// earlier position of the edge is already known invalid.
+// survivor s) in chunk-major order, one wave per item over a persistent grid
+// (wave w takes items w, w + W, w + 2W, ...).  A wave tests the static
+// liveness of its next 64 items at once (one per lane: is the chunk inside
+// the edge?), then walks the live ones in item order, each after re-reading
+// whether an earlier position of its edge is already known invalid.
 template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
     constexpr int G = GroupShape<LPS, OPS>::STATES;
+    const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t n_surv = *A.list_count;
     if (n_surv == 0) return;
     const int64_t nblk = ((int64_t)*A.max_m + G - 1) / G;
     const int64_t items = n_surv * nblk;
-    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < items; k += waves_total) {
-        const int64_t blk = k / n_surv, s = k - blk * n_surv;
-        const int base = 1 + (int)(blk * G);
-        const int nd = A.surv_nd[s];
-        if (base >= nd) continue; // past the edge's last position
-        if (__hip_atomic_load(&A.first_bad[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < base) continue;
-        const int64_t e = A.list[s];
-        double a[kDof], b[kDof];
-#pragma unroll
-        for (int d = 0; d < kDof; ++d) {
-            a[d] = A.a[(int64_t)d * A.n + e];
-            b[d] = A.b[(int64_t)d * A.n + e];
+    for (int64_t k0 = wave; k0 < items; k0 += 64 * waves_total) {
+        const int64_t k = k0 + lane * waves_total;
+        int s = 0, base = 0, nd = 0;
+        bool live = false;
+        if (k < items) {
+            const int64_t blk = k / n_surv;
+            s = (int)(k - blk * n_surv);
+            base = 1 + (int)(blk * G);
+            nd = A.surv_nd[s];
+            live = base < nd;
         }
-        const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, nd, nd, base, lds, A.env.ns, A.env.nb);
-        if (bad != INT_MAX && lane_id() == 0) atomicMin(&A.first_bad[s], bad);
+        uint64_t mask = wave_ballot(live);
+        while (mask) {
+            const int j = lowest_bit(mask);
+            mask &= mask - 1;
+            const int sj = __shfl(s, j), bj = __shfl(base, j), ndj = __shfl(nd, j);
+            if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) continue;
+            const int64_t e = A.list[sj];
+            double a[kDof], b[kDof];
+#pragma unroll
+            for (int d = 0; d < kDof; ++d) {
+                a[d] = A.a[(int64_t)d * A.n + e];
+                b[d] = A.b[(int64_t)d * A.n + e];
+            }
+            const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
+            if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[sj], bad);
+        }
     }
 }
 

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -196,7 +197,7 @@ static unsigned blocks_for(int64_t n, int threads = 256)
 }
 
 static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k,
-                  double lvs, unsigned factor)
+                  double lvs, unsigned factor, unsigned flags)
 {
     using Clock = std::chrono::steady_clock;
     auto ms = [](Clock::time_point a) {
@@ -286,18 +287,30 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
         if (last == ~(uint64_t)0) --nu; // padding key of milestones with fewer than k neighbours
     }
     R.ne = nu;
-    // 5. motion checks
     R.eu = R.alloc<int32_t>((size_t)nu);
     R.ev = R.alloc<int32_t>((size_t)nu);
     R.evalid = R.alloc<uint8_t>((size_t)nu);
     double *a_soa = R.alloc<double>((size_t)nu * kDim), *b_soa = R.alloc<double>((size_t)nu * kDim);
-    if (nu > 0) {
+    if (nu > 0)
         hipLaunchKernelGGL(edge_endpoints_kernel, dim3(blocks_for(nu)), dim3(256), 0, s, R.keys, nu, R.nodes, R.eu,
                            R.ev, a_soa, b_soa);
-        check(ompl_amd_check_motions_dev(env, a_soa, b_soa, nu, lvs, factor, R.evalid, nullptr, s));
-    }
     OA_HIP(hipStreamSynchronize(s));
-    R.stats.ms_edges = ms(t0);
+    R.stats.ms_candidates = ms(t0);
+    if ((flags & OMPL_AMD_PRM_AUTOTUNE) && nu > 0) {
+        // launch shape tuned on (up to) the first 32,768 candidate edges
+        const int64_t nt = std::min<int64_t>(nu, 32768);
+        double *ta = R.alloc<double>((size_t)nt * kDim), *tb = R.alloc<double>((size_t)nt * kDim);
+        int32_t *teu = R.alloc<int32_t>((size_t)nt), *tev = R.alloc<int32_t>((size_t)nt);
+        hipLaunchKernelGGL(edge_endpoints_kernel, dim3(blocks_for(nt)), dim3(256), 0, s, R.keys, nt, R.nodes, teu,
+                           tev, ta, tb);
+        OA_HIP(hipStreamSynchronize(s));
+        check(ompl_amd_autotune_motions(env, ta, tb, nt, lvs, factor, 3, nullptr, nullptr));
+    }
+    // 5. motion checks
+    t0 = Clock::now();
+    if (nu > 0) check(ompl_amd_check_motions_dev(env, a_soa, b_soa, nu, lvs, factor, R.evalid, nullptr, s));
+    OA_HIP(hipStreamSynchronize(s));
+    R.stats.ms_motions = ms(t0);
     // 6. components
     t0 = Clock::now();
     hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(nv)), dim3(256), 0, s, R.label, nv);
@@ -323,15 +336,15 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
     int32_t *pos = R.alloc<int32_t>((size_t)nu + 1);
     int32_t *degree = R.alloc<int32_t>((size_t)nv + 1);
     OA_HIP(hipMemsetAsync(degree, 0, ((size_t)nv + 1) * sizeof(int32_t), s));
-    int32_t *flags = R.alloc<int32_t>((size_t)nu);
+    int32_t *valid32 = R.alloc<int32_t>((size_t)nu);
     size_t scan_bytes = 0;
-    OA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flags, pos, (int)nu, s));
+    OA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, valid32, pos, (int)nu, s));
     void *tmp3 = R.alloc<char>(scan_bytes);
     int32_t last_pos = 0;
     uint8_t last_valid = 0;
     if (nu) {
-        hipLaunchKernelGGL(u8_to_i32_kernel, dim3(blocks_for(nu)), dim3(256), 0, s, R.evalid, nu, flags);
-        OA_HIP(hipcub::DeviceScan::ExclusiveSum(tmp3, scan_bytes, flags, pos, (int)nu, s));
+        hipLaunchKernelGGL(u8_to_i32_kernel, dim3(blocks_for(nu)), dim3(256), 0, s, R.evalid, nu, valid32);
+        OA_HIP(hipcub::DeviceScan::ExclusiveSum(tmp3, scan_bytes, valid32, pos, (int)nu, s));
         OA_HIP(hipMemcpyAsync(&last_pos, pos + nu - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         OA_HIP(hipMemcpyAsync(&last_valid, R.evalid + nu - 1, 1, hipMemcpyDeviceToHost, s));
     }
@@ -376,7 +389,7 @@ using namespace ompl_amd;
 extern "C" {
 
 int ompl_amd_prm_build(ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k, double lvs,
-                       unsigned factor, ompl_amd_prm *out, ompl_amd_prm_stats *stats)
+                       unsigned factor, unsigned flags, ompl_amd_prm *out, ompl_amd_prm_stats *stats)
 {
     return guarded([&] {
         OA_REQUIRE(env && out, OMPL_AMD_ERR_ARG, "null argument");
@@ -388,7 +401,7 @@ int ompl_amd_prm_build(ompl_amd_env env, uint32_t seed, int n_streams, int64_t n
         OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "bad segment settings");
         auto *h = new ompl_amd_prm_s();
         try {
-            build(h->R, env, seed, n_streams, n_samples, k, lvs, factor);
+            build(h->R, env, seed, n_streams, n_samples, k, lvs, factor, flags);
         } catch (...) {
             delete h;
             throw;

@@ -257,7 +257,7 @@ const char *ompl_amd_last_error(void)
 
 int ompl_amd_version(void)
 {
-    return 100; // 0.1.0
+    return OMPL_AMD_VERSION;
 }
 
 int ompl_amd_synchronize(void *stream)

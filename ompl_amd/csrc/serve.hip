@@ -156,11 +156,12 @@ __global__ void __launch_bounds__(64) grow_prepare_kernel(TickArgs T)
 }
 
 // Tick kernel 2: every request's positions, chunk-parallel over a persistent
-// grid (items (chunk, request) chunk-major, one wave per item; an item is
-// skipped past the request's last position or once the motion is known
-// invalid); obstacles are read from each request's environment in HBM
-// (wave-uniform loads), so requests in different environments share the
-// launch.
+// grid: items (chunk, request) chunk-major, one wave per item (wave w takes
+// items w, w + W, ...); a wave tests whether its next 64 items lie inside
+// their motions at once, then checks the live ones in item order, skipping a
+// request whose motion is already known invalid.  Obstacles are read from
+// each request's environment in HBM (wave-uniform loads), so requests in
+// different environments share the launch.
 template <int LPS, int OPS>
 __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
 {
@@ -168,39 +169,51 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
     constexpr int G = Shape::STATES;
     const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int n = T.ncmd;
     const int64_t nblk = (*T.max_pos + G - 1) / G;
     const int64_t items = (int64_t)n * nblk;
     const Shape sh(lane);
-    for (int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < items; k += waves_total) {
-        const int64_t blk = k / n;
-        const int r = (int)(k - blk * n);
-        const int base = (int)(blk * G);
-        GrowEdge &E = T.edges[r];
-        if (base >= E.npos) continue;
-        if (__hip_atomic_load(&E.first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != INT_MAX) continue;
-        const GrowCmd &C = T.cmds[r];
-        const int p = base + sh.group;
-        const int motion_pos = E.npos - E.extra;
-        const bool plive = p < E.npos;
-        float q[kDof];
-        if (plive && p >= motion_pos) {
-#pragma unroll
-            for (int d = 0; d < kDof; ++d) q[d] = (float)E.dst[d];
-        } else {
-            double a[kDof], b[kDof];
-#pragma unroll
-            for (int d = 0; d < kDof; ++d) {
-                a[d] = E.a[d];
-                b[d] = E.b[d];
-            }
-            walk_state(a, b, E.nd, plive ? p : 0, q);
+    for (int64_t k0 = wave; k0 < items; k0 += 64 * waves_total) {
+        const int64_t k = k0 + lane * waves_total;
+        int r = 0, base = 0;
+        bool live = false;
+        if (k < items) {
+            const int64_t blk = k / n;
+            r = (int)(k - blk * n);
+            base = (int)(blk * G);
+            live = base < T.edges[r].npos;
         }
-        LaneSpheres<LPS> S;
-        arm_fk<LPS>(q, sh.slot, S);
-        const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, C.obs, C.ns, C.nb, plive, sh.part);
-        const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
-        if (H && lane == 0) atomicMin(&E.first_bad, base + lowest_bit(H) / Shape::GS);
+        uint64_t mask = wave_ballot(live);
+        while (mask) {
+            const int j = lowest_bit(mask);
+            mask &= mask - 1;
+            const int rj = __shfl(r, j), bj = __shfl(base, j);
+            GrowEdge &E = T.edges[rj];
+            if (__hip_atomic_load(&E.first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != INT_MAX) continue;
+            const GrowCmd &C = T.cmds[rj];
+            const int p = bj + sh.group;
+            const int motion_pos = E.npos - E.extra;
+            const bool plive = p < E.npos;
+            float q[kDof];
+            if (plive && p >= motion_pos) {
+#pragma unroll
+                for (int d = 0; d < kDof; ++d) q[d] = (float)E.dst[d];
+            } else {
+                double a[kDof], b[kDof];
+#pragma unroll
+                for (int d = 0; d < kDof; ++d) {
+                    a[d] = E.a[d];
+                    b[d] = E.b[d];
+                }
+                walk_state(a, b, E.nd, plive ? p : 0, q);
+            }
+            LaneSpheres<LPS> S;
+            arm_fk<LPS>(q, sh.slot, S);
+            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, C.obs, C.ns, C.nb, plive, sh.part);
+            const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
+            if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);
+        }
     }
 }
 

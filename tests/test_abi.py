@@ -30,6 +30,14 @@ def test_library_exports_every_declared_symbol():
     assert not [n for n in declared_functions() if n not in _lib.SIGNATURES]
 
 
+def test_abi_version_matches_header_and_bindings():
+    from ompl_amd import _lib
+
+    text = (ROOT / "include" / "ompl_amd.h").read_text()
+    header = int(re.search(r"#define OMPL_AMD_VERSION (\d+)", text).group(1))
+    assert header == _lib.ABI_VERSION == _lib.load().ompl_amd_version()
+
+
 def test_error_path_without_gpu():
     import ctypes as C
 

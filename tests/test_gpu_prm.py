@@ -86,7 +86,7 @@ def test_prm_components_against_scipy(gpu):
     env = gpu.Env(s, b)
     g = build_roadmap(env, 7, 60000, k=10, n_streams=256)
     nv = g.milestones.shape[0]
-    assert nv > 20000
+    assert nv > 10000
     e = g.edges[g.edge_valid]
     m = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(nv, nv))
     nc, lab = connected_components(m, directed=False)
