@@ -149,7 +149,11 @@ typedef struct {
                                         and walks its own survivors (1 launch);
                                      3: s2 pass, then the survivors' bisection
                                         chunks run in parallel across the GPU,
-                                        chunk-major, with early exit */
+                                        chunk-major, with early exit;
+                                     4: as 3, with each state's FK computed
+                                        once per wave and shared across the
+                                        obstacle partitions (shapes with
+                                        parts > 1; chunks of 64/lanes states) */
 } ompl_amd_tuning;
 int ompl_amd_get_tuning(ompl_amd_tuning *t);
 int ompl_amd_set_tuning(const ompl_amd_tuning *t);

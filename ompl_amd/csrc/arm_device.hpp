@@ -196,6 +196,25 @@ struct GroupShape {
     }
 };
 
+// Shared forward kinematics.  The FK layout computes 64/LPS states per wave
+// (state f = lane / LPS, sphere slot lane % LPS); the collision layout of
+// shape (LPS, OPS) checks 64/(LPS*OPS) of them at a time, so pass t of OPS
+// passes checks FK states t*STATES + group.  A collision lane takes the
+// centres of its (state, slot) from the FK lane that computed them, so each
+// state's FK is computed once instead of once per obstacle partition.
+template <int LPS, int OPS>
+__device__ __forceinline__ void spheres_from_fk(const LaneSpheres<LPS> &F, int t, LaneSpheres<LPS> &S)
+{
+    const GroupShape<LPS, OPS> sh(threadIdx.x & 63);
+    const int src = (t * GroupShape<LPS, OPS>::STATES + sh.group) * LPS + sh.slot;
+#pragma unroll
+    for (int m = 0; m < LaneSpheres<LPS>::N; ++m) {
+        S.x[m] = __shfl(F.x[m], src);
+        S.y[m] = __shfl(F.y[m], src);
+        S.z[m] = __shfl(F.z[m], src);
+    }
+}
+
 // Checks the lane's spheres against its obstacle partition of the LDS
 // environment.  `live` lanes take part; dead lanes count as resolved.  The
 // obstacle loop runs a wave-uniform trip count (two obstacles of the lane's

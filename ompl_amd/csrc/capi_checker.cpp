@@ -36,9 +36,12 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
     const int LE = t.validity_lanes_per_state, OE = t.validity_obstacle_parts;
     const int LW = t.motion_lanes_per_state, OW = t.motion_obstacle_parts;
     const int64_t persistent_waves = (int64_t)num_cus() * t.motion_waves_per_cu;
-    if (t.motion_strategy == 3) {
-        // counters [8..9] belong to this strategy only: zero when the
-        // workspace is created, cleared by the finalize kernel of every batch
+    if (t.motion_strategy == 3 || t.motion_strategy == 4) {
+        // counters [8..9] belong to these strategies only: zero when the
+        // workspace is created, cleared by the finalize kernel of every batch.
+        // Strategy 4 computes each state's FK once per wave and shares it
+        // across the obstacle partitions.
+        const bool shared = t.motion_strategy == 4;
         Workspace &w = workspace(s, (size_t)n);
         A.list = w.list;
         A.list_count = w.counters + 8;
@@ -46,8 +49,8 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         A.first_bad = w.first_bad;
         A.surv_nd = w.surv_nd;
         A.start = 1;
-        launch_motion_endpoint(A, LE, OE, t.validity_block, s);
-        launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s);
+        launch_motion_endpoint(A, LE, OE, t.validity_block, s, shared);
+        launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s, shared);
         launch_motion_finalize(A, s);
     } else if (t.motion_strategy == 2) {
         A.start = 1;
@@ -82,9 +85,11 @@ std::vector<Tuning> motion_candidates(const Tuning &base)
     for (const auto &e : shapes)
         for (const auto &w : shapes) {
             const int LE = e[0], OE = e[1], LW = w[0], OW = w[1];
-            // chunk-parallel walk of the survivors
-            for (int wpc : {8, 16, 32})
+            // chunk-parallel walk of the survivors (and its shared-FK form)
+            for (int wpc : {8, 16, 32}) {
                 c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 3});
+                if (OE > 1 || OW > 1) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 4});
+            }
             // fused one-launch form
             if (fused_supported(LE, OE, LW, OW))
                 for (int block : {64, 256}) c.push_back(Tuning{LE, OE, block, LW, OW, block, 8, 2});

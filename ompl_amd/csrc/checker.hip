@@ -37,6 +37,8 @@ namespace ompl_amd {
 
 // (LPS, OPS) shapes instantiated for every batched kernel.
 #define OA_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(8, 2) X(8, 4) X(8, 8)
+// shapes with more than one obstacle partition: the shared-FK kernels
+#define OA_SHARED_SHAPES(X) X(8, 2) X(8, 4) X(8, 8)
 
 bool shape_supported(int lps, int ops)
 {
@@ -72,31 +74,26 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
 // s2 check of one edge per lane group; returns true for the group leader of a
 // survivor (s2 valid, 2 <= nd <= kMaxSegments) and writes the final result of
 // every other edge.
-template <int LPS, int OPS>
-__device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4 *lds, int64_t e, bool live, int &m)
+// Result of an edge's s2 check (H: folded hit mask of the wave) on the group
+// leader: writes the final result of every edge that needs no walk and
+// returns true for a survivor (s2 valid, 2 <= nd <= kMaxSegments), m = nd - 1.
+template <int GS>
+__device__ __forceinline__ bool endpoint_finish(const MotionArgs &A, int64_t e, bool live, uint64_t H, int &m)
 {
-    using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
-    const Shape sh(lane);
-    double a[kDof], b[kDof];
-    float q[kDof];
-#pragma unroll
-    for (int j = 0; j < kDof; ++j) {
-        a[j] = live ? A.a[(int64_t)j * A.n + e] : 0.0;
-        b[j] = live ? A.b[(int64_t)j * A.n + e] : 0.0;
-        q[j] = (float)b[j];
-    }
-    LaneSpheres<LPS> S;
-    arm_fk<LPS>(q, sh.slot, S);
-    const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
-    const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
     m = 0;
-    if (!live || lane % Shape::GS != 0) return false;
+    if (!live || lane % GS != 0) return false;
     const bool s2_valid = !((H >> lane) & 1);
     if (!s2_valid) {
         A.out[e] = 0;
         if (A.checked) A.checked[e] = 1;
         return false;
+    }
+    double a[kDof], b[kDof];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) {
+        a[j] = A.a[(int64_t)j * A.n + e];
+        b[j] = A.b[(int64_t)j * A.n + e];
     }
     const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
     if (nd > kMaxSegments) {
@@ -113,30 +110,83 @@ __device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4
     return true;
 }
 
+// s2 check of one edge per lane group; returns true for the group leader of a
+// survivor and writes the final result of every other edge.
+template <int LPS, int OPS>
+__device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4 *lds, int64_t e, bool live, int &m)
+{
+    using Shape = GroupShape<LPS, OPS>;
+    const Shape sh(lane_id());
+    float q[kDof];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) q[j] = live ? (float)A.b[(int64_t)j * A.n + e] : 0.0f;
+    LaneSpheres<LPS> S;
+    arm_fk<LPS>(q, sh.slot, S);
+    const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
+    const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+    return endpoint_finish<Shape::GS>(A, e, live, H, m);
+}
+
+// Wave-aggregated append of the survivors to the list.
+__device__ __forceinline__ void append_survivor(const MotionArgs &A, int64_t e, bool survivor, int m)
+{
+    const int lane = lane_id();
+    const uint64_t sm = wave_ballot(survivor);
+    if (!sm) return;
+    int base = 0;
+    if (lane == lowest_bit(sm)) base = atomicAdd(A.list_count, __popcll(sm));
+    base = __shfl(base, lowest_bit(sm));
+    if (survivor) {
+        const int slot = base + __popcll(sm & (((uint64_t)1 << lane) - 1));
+        A.list[slot] = (int32_t)e;
+        if (A.first_bad) A.first_bad[slot] = INT_MAX;
+        if (A.surv_nd) A.surv_nd[slot] = m + 1;
+        if (A.max_m) atomicMax(A.max_m, m);
+    }
+}
+
 template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
-    const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t e = wave * Shape::STATES + lane / Shape::GS;
+    const int64_t e = wave * Shape::STATES + lane_id() / Shape::GS;
     int m = 0;
     const bool survivor = endpoint_check<LPS, OPS>(A, lds, e, e < A.n, m);
-    // wave-aggregated append of the survivors
-    const uint64_t sm = wave_ballot(survivor);
-    if (sm) {
-        int base = 0;
-        if (lane == lowest_bit(sm)) base = atomicAdd(A.list_count, __popcll(sm));
-        base = __shfl(base, lowest_bit(sm));
-        if (survivor) {
-            const int slot = base + __popcll(sm & (((uint64_t)1 << lane) - 1));
-            A.list[slot] = (int32_t)e;
-            if (A.first_bad) A.first_bad[slot] = INT_MAX;
-            if (A.surv_nd) A.surv_nd[slot] = m + 1;
-            if (A.max_m) atomicMax(A.max_m, m);
-        }
+    append_survivor(A, e, survivor, m);
+}
+
+// Shared-FK endpoint pass: a wave takes 64/LPS edges, computes the FK of their
+// s2 once (LPS lanes per state), then checks them OPS passes at a time.
+template <int LPS, int OPS>
+__global__ void __launch_bounds__(1024) motion_endpoint_shared_kernel(MotionArgs A)
+{
+    extern __shared__ float4 lds[];
+    stage_env(A.env, lds);
+    using Shape = GroupShape<LPS, OPS>;
+    constexpr int FK = 64 / LPS;
+    const int lane = lane_id();
+    const Shape sh(lane);
+    const int64_t e0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * FK;
+    const int64_t ef = e0 + lane / LPS;
+    float q[kDof];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) q[j] = ef < A.n ? (float)A.b[(int64_t)j * A.n + ef] : 0.0f;
+    LaneSpheres<LPS> F;
+    arm_fk<LPS>(q, lane % LPS, F);
+#pragma unroll 1
+    for (int t = 0; t < OPS; ++t) {
+        const int64_t e = e0 + t * Shape::STATES + sh.group;
+        const bool live = e < A.n;
+        LaneSpheres<LPS> S;
+        spheres_from_fk<LPS, OPS>(F, t, S);
+        const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
+        const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+        int m = 0;
+        const bool survivor = endpoint_finish<Shape::GS>(A, e, live, H, m);
+        append_survivor(A, e, survivor, m);
     }
 }
 
@@ -182,12 +232,12 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 // liveness of its next 64 items at once (one per lane: is the chunk inside
 // the edge?), then walks the live ones in item order, each after re-reading
 // whether an earlier position of its edge is already known invalid.
-template <int LPS, int OPS, bool EXACT>
+template <int LPS, int OPS, bool EXACT, bool SHARED>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
-    constexpr int G = GroupShape<LPS, OPS>::STATES;
+    constexpr int G = SHARED ? 64 / LPS : GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -219,7 +269,8 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
                 a[d] = A.a[(int64_t)d * A.n + e];
                 b[d] = A.b[(int64_t)d * A.n + e];
             }
-            const int bad = walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
+            const int bad = SHARED ? walk_chunk_shared<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb)
+                                   : walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
             if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[sj], bad);
         }
     }
@@ -476,10 +527,22 @@ void launch_zero(int *p, int n, hipStream_t s)
     OA_HIP(hipGetLastError());
 }
 
-void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hipStream_t s)
+void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hipStream_t s, bool shared)
 {
     if (A.n <= 0) return;
     ProfScope ps("motion_endpoint", s);
+    if (shared && ops > 1) {
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        const int64_t per = (int64_t)(block / 64) * (64 / L);                                                     \
+        hipLaunchKernelGGL((motion_endpoint_shared_kernel<L, O>), dim3(grid_for(A.n, per)), dim3(block),          \
+                           lds_bytes(A.env), s, A);                                                                \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+        OA_SHARED_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    }
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
@@ -519,16 +582,33 @@ void launch_motion_finalize(const MotionArgs &A, hipStream_t s)
 }
 
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
-                          hipStream_t s)
+                          hipStream_t s, bool shared)
 {
     ProfScope ps("motion_chunk", s);
     const unsigned g = grid_for(waves, block / 64);
+    if (shared && ops > 1) {
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         if (exact)                                                                                                  \
-            hipLaunchKernelGGL((motion_chunk_kernel<L, O, true>), dim3(g), dim3(block), lds_bytes(A.env), s, A);  \
+            hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, true>), dim3(g), dim3(block), lds_bytes(A.env), s, \
+                               A);                                                                                  \
         else                                                                                                        \
-            hipLaunchKernelGGL((motion_chunk_kernel<L, O, false>), dim3(g), dim3(block), lds_bytes(A.env), s, A); \
+            hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, true>), dim3(g), dim3(block), lds_bytes(A.env),   \
+                               s, A);                                                                               \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+        OA_SHARED_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    }
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        if (exact)                                                                                                  \
+            hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, false>), dim3(g), dim3(block), lds_bytes(A.env), s, \
+                               A);                                                                                  \
+        else                                                                                                        \
+            hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, false>), dim3(g), dim3(block), lds_bytes(A.env),  \
+                               s, A);                                                                               \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }

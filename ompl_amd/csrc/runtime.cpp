@@ -209,8 +209,8 @@ void set_tuning(const Tuning &t)
                "block must be a power of two in [64, 1024]");
     OA_REQUIRE(t.motion_waves_per_cu >= 1 && t.motion_waves_per_cu <= 32, OMPL_AMD_ERR_ARG,
                "motion_waves_per_cu must be in [1, 32]");
-    OA_REQUIRE(t.motion_strategy >= 0 && t.motion_strategy <= 3, OMPL_AMD_ERR_ARG,
-               "motion_strategy must be 0, 1, 2 or 3");
+    OA_REQUIRE(t.motion_strategy >= 0 && t.motion_strategy <= 4, OMPL_AMD_ERR_ARG,
+               "motion_strategy must be in [0, 4]");
     if (t.motion_strategy == 2)
         OA_REQUIRE(fused_supported(t.validity_lanes_per_state, t.validity_obstacle_parts, t.motion_lanes_per_state,
                                    t.motion_obstacle_parts),

@@ -57,6 +57,37 @@ __device__ __forceinline__ int walk_chunk(const double a[kDof], const double b[k
     return H ? base + lowest_bit(H) / Shape::GS : INT_MAX;
 }
 
+// Shared-FK form of walk_chunk: positions [base, base + 64/LPS) get their FK
+// in one pass (LPS lanes each), then are checked OPS passes of STATES
+// positions in increasing order; returns at the first pass with an invalid
+// position (its lowest one with EXACT), or INT_MAX.
+template <int LPS, int OPS, bool EXACT>
+__device__ __forceinline__ int walk_chunk_shared(const double a[kDof], const double b[kDof], int nd, int total,
+                                                 int base, const float4 *lds, int ns, int nb)
+{
+    using Shape = GroupShape<LPS, OPS>;
+    const int lane = lane_id();
+    const int pf = base + lane / LPS;
+    float q[kDof];
+    walk_state(a, b, nd, pf < total ? pf : 0, q);
+    LaneSpheres<LPS> F;
+    arm_fk<LPS>(q, lane % LPS, F);
+    const Shape sh(lane);
+#pragma unroll 1
+    for (int t = 0; t < OPS; ++t) {
+        const int p0 = base + t * Shape::STATES;
+        if (p0 >= total) break;
+        LaneSpheres<LPS> S;
+        spheres_from_fk<LPS, OPS>(F, t, S);
+        const bool live = p0 + sh.group < total;
+        const bool hit =
+            collide_env<LPS, OPS, EXACT ? Exit::kLowestGroup : Exit::kAnyGroup>(S, lds, ns, nb, live, sh.part);
+        const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+        if (H) return p0 + lowest_bit(H) / Shape::GS;
+    }
+    return INT_MAX;
+}
+
 // Walks one edge from position `start` (0 = s2) chunk by chunk.  Returns the
 // position of the first invalid state found (INT_MAX if none, -1 when the
 // edge has more than kMaxSegments segments) and the total position count.

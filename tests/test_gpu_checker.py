@@ -85,7 +85,8 @@ def motion_case(gpu, seed, n, mu=-1.0, sigma=1.0, lo=0.05, hi=5.0):
 # (strategy, endpoint shape, walk shape)
 STRATEGIES = [(3, (8, 2), (8, 1)), (3, (8, 8), (8, 8)), (3, (1, 1), (1, 1)), (3, (4, 1), (8, 4)),
               (1, (8, 1), (1, 1)), (1, (4, 1), (8, 2)), (0, (8, 1), (1, 1)), (0, (8, 1), (8, 8)),
-              (2, (8, 1), (1, 1)), (2, (8, 2), (8, 8)), (2, (8, 4), (8, 1))]
+              (2, (8, 1), (1, 1)), (2, (8, 2), (8, 8)), (2, (8, 4), (8, 1)),
+              (4, (8, 8), (8, 8)), (4, (8, 2), (8, 4)), (4, (8, 4), (8, 1)), (4, (1, 1), (8, 2))]
 DEFAULT = dict(motion_strategy=3, validity_lanes_per_state=8, validity_obstacle_parts=2,
                motion_lanes_per_state=8, motion_obstacle_parts=1)
 
@@ -139,9 +140,9 @@ def test_check_motions_edge_cases(gpu, oracle):
     lvs = lvs_config()
     for factor in (1, 2):
         ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs, factor)
-        for tp in (3, 1, 0, 2):
+        for tp, ops in ((3, 1), (1, 1), (0, 1), (2, 1), (4, 8), (4, 2)):
             gpu.set_tuning(**{**DEFAULT, "motion_strategy": tp, "motion_lanes_per_state": 8,
-                              "motion_obstacle_parts": 1, "validity_obstacle_parts": 1})
+                              "motion_obstacle_parts": ops, "validity_obstacle_parts": ops})
             v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, factor, exact_count=True)
             assert np.array_equal(v.cpu().numpy(), ref_v)
             assert np.array_equal(c.cpu().numpy(), ref_c)
