@@ -223,10 +223,19 @@ def scope_rows(oa, run_cpu, cpu_threads):
     q = oa.workload.configs(1, 65536)
     qd = torch.from_numpy(q.T.astype(np.float32).copy()).cuda()
     out = torch.empty(65536, dtype=torch.uint8, device="cuda")
-    ms = timed(lambda: oa.validate_states(env, qd, out=out), 50)
+    shapes = {}
+    saved = oa.get_tuning()
+    for lps, ops, shared in ((8, 1, 0), (8, 2, 0), (8, 8, 0), (8, 2, 1), (8, 4, 1), (8, 8, 1), (1, 1, 0)):
+        oa.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops, validity_shared_fk=shared)
+        shapes[f"{lps}x{ops}{'-shared' if shared else ''}"] = timed(lambda: oa.validate_states(env, qd, out=out), 50)
+    best = min(shapes, key=shapes.get)
+    ms = shapes[best]
+    oa.set_tuning(**saved)
+    oa.validate_states(env, qd, out=out)
     w = oa.validity_work(env, qd)
-    row = {"states": 65536, "obstacles": 64, "ms": ms, "states_per_s": 65536 / (ms * 1e-3),
-           "valid_fraction": float(out.float().mean().item()), "roofline": roof(oa.work_flops(w), ms)}
+    row = {"states": 65536, "obstacles": 64, "ms": ms, "states_per_s": 65536 / (ms * 1e-3), "shape": best,
+           "ms_by_shape": shapes, "valid_fraction": float(out.float().mean().item()),
+           "roofline": roof(oa.work_flops(w), ms)}
     if oracle_py is not None:
         oenv = oracle_py.OracleEnv(s, b)
         t0 = time.perf_counter()

@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 300
+#define OMPL_AMD_VERSION 301
 int ompl_amd_version(void);
 int ompl_amd_synchronize(void *stream);
 
@@ -154,6 +154,9 @@ typedef struct {
                                         once per wave and shared across the
                                         obstacle partitions (shapes with
                                         parts > 1; chunks of 64/lanes states) */
+    int validity_shared_fk;       /* 1: batched validity computes each
+                                     state's FK once per wave and shares it
+                                     across the obstacle partitions */
 } ompl_amd_tuning;
 int ompl_amd_get_tuning(ompl_amd_tuning *t);
 int ompl_amd_set_tuning(const ompl_amd_tuning *t);

@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 300
+ABI_VERSION = 301
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -113,6 +113,7 @@ class Tuning(C.Structure):
         ("motion_block", C.c_int),
         ("motion_waves_per_cu", C.c_int),
         ("motion_strategy", C.c_int),
+        ("validity_shared_fk", C.c_int),
     ]
 
     def as_dict(self):

@@ -79,7 +79,6 @@ void check_motions(const Env &env, const double *a, const double *b, int64_t n, 
 // Candidate launch shapes of the motion autotuner.
 std::vector<Tuning> motion_candidates(const Tuning &base)
 {
-    (void)base;
     std::vector<Tuning> c;
     const int shapes[][2] = {{8, 1}, {8, 2}, {8, 4}, {8, 8}, {4, 1}, {1, 1}};
     for (const auto &e : shapes)
@@ -97,6 +96,7 @@ std::vector<Tuning> motion_candidates(const Tuning &base)
             c.push_back(Tuning{LE, OE, 256, LW, OW, 256, 16, 1});
         }
     for (const auto &w : shapes) c.push_back(Tuning{8, 1, 256, w[0], w[1], 256, 8, 0});
+    for (Tuning &x : c) x.validity_shared_fk = base.validity_shared_fk; // not a motion knob
     return c;
 }
 
@@ -167,8 +167,7 @@ int ompl_amd_validate_states_dev(ompl_amd_env e, const float *q, int64_t n, uint
         OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
         const Tuning t = get_tuning();
         launch_validate<float>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_obstacle_parts,
-                               t.validity_block,
-                               resolve_stream(stream));
+                               t.validity_block, resolve_stream(stream), t.validity_shared_fk != 0);
     });
 }
 
@@ -181,8 +180,7 @@ int ompl_amd_validate_states_f64_dev(ompl_amd_env e, const double *q, int64_t n,
         OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
         const Tuning t = get_tuning();
         launch_validate<double>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_obstacle_parts,
-                                t.validity_block,
-                                resolve_stream(stream));
+                                t.validity_block, resolve_stream(stream), t.validity_shared_fk != 0);
     });
 }
 

@@ -74,6 +74,37 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
 // s2 check of one edge per lane group; returns true for the group leader of a
 // survivor (s2 valid, 2 <= nd <= kMaxSegments) and writes the final result of
 // every other edge.
+// Shared-FK batched validity: a wave takes 64/LPS states, computes their FK
+// once (LPS lanes per state), then checks them OPS passes at a time.
+template <int LPS, int OPS, typename T>
+__global__ void __launch_bounds__(1024) validate_shared_kernel(EnvView env, const T *__restrict__ q_soa, int64_t n,
+                                                               uint8_t *__restrict__ out)
+{
+    extern __shared__ float4 lds[];
+    stage_env(env, lds);
+    using Shape = GroupShape<LPS, OPS>;
+    constexpr int FK = 64 / LPS;
+    const int lane = lane_id();
+    const Shape sh(lane);
+    const int64_t s0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * FK;
+    const int64_t sf = s0 + lane / LPS;
+    float q[kDof];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) q[j] = sf < n ? (float)q_soa[(int64_t)j * n + sf] : 0.0f;
+    LaneSpheres<LPS> F;
+    arm_fk<LPS>(q, lane % LPS, F);
+#pragma unroll 1
+    for (int t = 0; t < OPS; ++t) {
+        const int64_t st = s0 + t * Shape::STATES + sh.group;
+        const bool live = st < n;
+        LaneSpheres<LPS> S;
+        spheres_from_fk<LPS, OPS>(F, t, S);
+        const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, env.ns, env.nb, live, sh.part);
+        const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+        if (live && lane % Shape::GS == 0) out[st] = ((H >> lane) & 1) ? 0 : 1;
+    }
+}
+
 // Result of an edge's s2 check (H: folded hit mask of the wave) on the group
 // leader: writes the final result of every edge that needs no walk and
 // returns true for a survivor (s2 valid, 2 <= nd <= kMaxSegments), m = nd - 1.
@@ -500,10 +531,22 @@ static unsigned grid_for(int64_t units, int64_t units_per_block)
 
 template <typename T>
 void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, int lps, int ops, int block,
-                     hipStream_t s)
+                     hipStream_t s, bool shared)
 {
     if (n <= 0) return;
     ProfScope ps("validate_states", s);
+    if (shared && ops > 1) {
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        const int64_t per = (int64_t)(block / 64) * (64 / L);                                                     \
+        hipLaunchKernelGGL((validate_shared_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env), \
+                           s, env, q, n, out);                                                                      \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+        OA_SHARED_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    }
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
@@ -517,9 +560,9 @@ void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, in
     fail(OMPL_AMD_ERR_ARG, "validity shape (%d, %d) not built", lps, ops);
 }
 template void launch_validate<float>(const EnvView &, const float *, int64_t, uint8_t *, int, int, int,
-                                     hipStream_t);
+                                     hipStream_t, bool);
 template void launch_validate<double>(const EnvView &, const double *, int64_t, uint8_t *, int, int, int,
-                                      hipStream_t);
+                                      hipStream_t, bool);
 
 void launch_zero(int *p, int n, hipStream_t s)
 {

@@ -42,7 +42,7 @@ bool fused_supported(int le, int oe, int lw, int ow);
 
 template <typename T>
 void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, int lps, int ops, int block,
-                     hipStream_t s);
+                     hipStream_t s, bool shared = false);
 void launch_zero(int *p, int n, hipStream_t s);
 void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hipStream_t s, bool shared = false);
 void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves, hipStream_t s);

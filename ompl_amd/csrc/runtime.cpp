@@ -211,6 +211,8 @@ void set_tuning(const Tuning &t)
                "motion_waves_per_cu must be in [1, 32]");
     OA_REQUIRE(t.motion_strategy >= 0 && t.motion_strategy <= 4, OMPL_AMD_ERR_ARG,
                "motion_strategy must be in [0, 4]");
+    OA_REQUIRE(t.validity_shared_fk == 0 || t.validity_shared_fk == 1, OMPL_AMD_ERR_ARG,
+               "validity_shared_fk must be 0 or 1");
     if (t.motion_strategy == 2)
         OA_REQUIRE(fused_supported(t.validity_lanes_per_state, t.validity_obstacle_parts, t.motion_lanes_per_state,
                                    t.motion_obstacle_parts),
@@ -230,13 +232,15 @@ ompl_amd_tuning tuning_to_c(const Tuning &x)
     t.motion_block = x.motion_block;
     t.motion_waves_per_cu = x.motion_waves_per_cu;
     t.motion_strategy = x.motion_strategy;
+    t.validity_shared_fk = x.validity_shared_fk;
     return t;
 }
 
 Tuning tuning_from_c(const ompl_amd_tuning &t)
 {
-    return Tuning{t.validity_lanes_per_state, t.validity_obstacle_parts, t.validity_block, t.motion_lanes_per_state,
-                  t.motion_obstacle_parts,    t.motion_block,            t.motion_waves_per_cu, t.motion_strategy};
+    return Tuning{t.validity_lanes_per_state, t.validity_obstacle_parts, t.validity_block,
+                  t.motion_lanes_per_state,   t.motion_obstacle_parts,   t.motion_block,
+                  t.motion_waves_per_cu,      t.motion_strategy,         t.validity_shared_fk};
 }
 
 } // namespace ompl_amd

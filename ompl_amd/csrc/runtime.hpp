@@ -50,6 +50,7 @@ struct Tuning {
     int motion_block;
     int motion_waves_per_cu;
     int motion_strategy;
+    int validity_shared_fk = 0;
 };
 
 Tuning get_tuning();

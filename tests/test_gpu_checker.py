@@ -39,13 +39,15 @@ def test_fk_bit_exact(gpu, oracle):
 
 
 SHAPES = [(1, 1), (2, 1), (4, 1), (8, 1), (8, 2), (8, 4), (8, 8)]
+# (lanes, parts, shared FK)
+VSHAPES = [(lps, ops, 0) for lps, ops in SHAPES] + [(8, 2, 1), (8, 4, 1), (8, 8, 1)]
 
 
-@pytest.mark.parametrize("lps,ops", SHAPES)
+@pytest.mark.parametrize("lps,ops,shared", VSHAPES)
 @pytest.mark.parametrize("ns,nb", [(64, 0), (128, 128), (0, 40), (0, 0), (3, 1)])
-def test_validate_states_bit_exact(gpu, oracle, lps, ops, ns, nb):
-    print(f"validate shape=({lps},{ops}) ns={ns} nb={nb}", flush=True)
-    gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops)
+def test_validate_states_bit_exact(gpu, oracle, lps, ops, shared, ns, nb):
+    print(f"validate shape=({lps},{ops}) shared={shared} ns={ns} nb={nb}", flush=True)
+    gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops, validity_shared_fk=shared)
     env, oenv, _ = env_pair(gpu, oracle, 1000 + ns + nb, ns, nb)
     q = gpu.workload.configs(1, 65536 if (ns, nb) == (64, 0) else 8192)
     qf = q.astype(np.float32).T.copy()
@@ -57,17 +59,17 @@ def test_validate_states_bit_exact(gpu, oracle, lps, ops, ns, nb):
     assert np.array_equal(got64, ref)
     if ns + nb == 0:
         assert ref.all()
-    gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2)
+    gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2, validity_shared_fk=0)
 
 
 @pytest.mark.parametrize("n", [1, 63, 65, 129, 1000])
 def test_validate_ragged_sizes(gpu, oracle, n):
     env, oenv, _ = env_pair(gpu, oracle, 7, 32, 32)
     q = gpu.workload.configs(n, n).astype(np.float32).T.copy()
-    for lps, ops in ((1, 1), (8, 1), (8, 8)):
-        gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops)
+    for lps, ops, shared in ((1, 1, 0), (8, 1, 0), (8, 8, 0), (8, 8, 1), (8, 2, 1)):
+        gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops, validity_shared_fk=shared)
         assert np.array_equal(gpu.validate_states(env, to_dev(q)).cpu().numpy(), oracle.validate_states(oenv, q))
-    gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2)
+    gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2, validity_shared_fk=0)
 
 
 def test_is_valid_single_call(gpu, oracle):
