@@ -49,7 +49,7 @@ def percentile(xs, p):
     return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
 
 
-def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000):
+def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000, log_path=None):
     """Poisson stream of RRT-Connect requests over seeded environments (16-1024
     obstacles, log-uniform, half spheres / half boxes), served (a) by the
     continuous-batching server and (b) by the plain path (host RRT-Connect, one
@@ -129,6 +129,16 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     same = all(res_p[i][0].status == res_s[i].status and np.array_equal(res_p[i][1], paths_s[i])
                for i in range(n_requests))
     solved = sum(r.status == serve.SOLVED for r in res_s)
+    if log_path:
+        from ompl_amd import benchlog
+
+        common = {"range": reqs[0].range, "longest valid segment": lvs, "max iterations": max_iters}
+        benchlog.write_log(log_path, "rrtconnect_served_vs_plain", [
+            {"name": "geometric_RRTConnect_served", "common": common, "runs": benchlog.plan_runs(res_s, paths_s)},
+            {"name": "geometric_RRTConnect_plain", "common": common,
+             "runs": benchlog.plan_runs([r for r, _ in res_p], [p for _, p in res_p])}],
+            seed=seed, total_duration=span_s + span_p,
+            properties={"qps REAL": float(qps), "requests INTEGER": n_requests, "environments INTEGER": len(envs)})
     return {
         "requests": n_requests, "qps": qps, "envs_used": len(envs), "envs_drawn": n_envs,
         "solved": solved, "max_iters": max_iters,
@@ -343,6 +353,8 @@ def main(argv=None):
     ap.add_argument("--prm-samples", type=int, default=PRM_SAMPLES, help="PRM roadmap batch size (0 = skip)")
     ap.add_argument("--prm-cpu-samples", type=int, default=20000)
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements")
+    ap.add_argument("--benchmark-log", type=str, default=None,
+                    help="also write the served stream's runs as an OMPL benchmark log file")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run 5 device copies of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -481,7 +493,8 @@ def main(argv=None):
 
     stream_res = None
     if args.stream_requests > 0:
-        stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers)
+        stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers,
+                                   log_path=args.benchmark_log)
 
     prm_res = None
     if args.prm_samples > 0:

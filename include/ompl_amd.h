@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 301
+#define OMPL_AMD_VERSION 302
 int ompl_amd_version(void);
 int ompl_amd_synchronize(void *stream);
 
@@ -343,12 +343,16 @@ typedef struct {
     int64_t candidate_edges;
     int64_t valid_edges;
     int64_t components;
+    int64_t k; /* neighbours per milestone used */
     double ms_sample, ms_validity, ms_knn, ms_candidates, ms_motions, ms_graph, ms_total;
 } ompl_amd_prm_stats;
 
 /* flags of ompl_amd_prm_build */
 #define OMPL_AMD_PRM_AUTOTUNE 1u /* run the motion autotuner on (up to 32,768 of) the candidate edges first and
                                     install the winner, as ompl_amd_autotune_motions */
+#define OMPL_AMD_PRM_KSTAR 2u    /* KStarStrategy (geometric/planners/prm/ConnectionStrategy.h:125-157, PRM*):
+                                    k = ceil((e + e/7) * ln(milestones)), capped at 63; the k argument is
+                                    ignored */
 
 /* Samples U[-pi,pi]^7 from n_streams mt19937 generators seeded seed,
  * seed+1, ... (stream-major sample order; n_samples is rounded up to a

@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 301
+ABI_VERSION = 302
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -127,6 +127,7 @@ class PrmStats(C.Structure):
         ("candidate_edges", C.c_int64),
         ("valid_edges", C.c_int64),
         ("components", C.c_int64),
+        ("k", C.c_int64),
         ("ms_sample", C.c_double),
         ("ms_validity", C.c_double),
         ("ms_knn", C.c_double),

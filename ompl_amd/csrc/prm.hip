@@ -243,6 +243,7 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
         hipLaunchKernelGGL(gather_nodes_kernel, dim3(blocks_for(nv)), dim3(256), 0, s, R.milestone, nv,
                            R.samples_f64, R.nodes, R.nodes_f32);
     R.label = R.alloc<int32_t>((size_t)nv);
+    R.stats.k = k;
     if (nv < 2) {
         if (nv) hipLaunchKernelGGL(iota_kernel, dim3(1), dim3(64), 0, s, R.label, nv);
         R.csr_off = R.alloc<int64_t>((size_t)nv + 1);
@@ -252,6 +253,14 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
         R.stats.ms_total = ms(t_all);
         return;
     }
+    if (flags & OMPL_AMD_PRM_KSTAR) {
+        // KStarStrategy: kPRMConstant = e + e/d, k = ceil(kPRMConstant * log(n)), n = milestones
+        const double e = 2.718281828459045235360287471352662498;
+        const double kc = e + e / (double)kDim;
+        k = (int)std::min<double>(63.0, std::ceil(kc * std::log((double)nv)));
+        if (k < 1) k = 1;
+    }
+    R.stats.k = k;
     // 3. k nearest milestones (k + 1 to drop the milestone itself)
     t0 = Clock::now();
     const int kk = std::min<int64_t>(k + 1, nv);
@@ -395,8 +404,11 @@ int ompl_amd_prm_build(ompl_amd_env env, uint32_t seed, int n_streams, int64_t n
         OA_REQUIRE(env && out, OMPL_AMD_ERR_ARG, "null argument");
         OA_REQUIRE(n_streams >= 1 && n_samples >= 1 && n_samples < ((int64_t)1 << 31), OMPL_AMD_ERR_ARG,
                    "bad sample counts");
-        OA_REQUIRE(k >= 1 && k <= 63, OMPL_AMD_ERR_ARG, "k must be in [1, 63]");
-        OA_REQUIRE(2 * (n_samples + n_streams) * (int64_t)k < ((int64_t)1 << 31), OMPL_AMD_ERR_ARG,
+        OA_REQUIRE((flags & ~(OMPL_AMD_PRM_AUTOTUNE | OMPL_AMD_PRM_KSTAR)) == 0, OMPL_AMD_ERR_ARG, "unknown flags");
+        const bool kstar = (flags & OMPL_AMD_PRM_KSTAR) != 0;
+        OA_REQUIRE(kstar || (k >= 1 && k <= 63), OMPL_AMD_ERR_ARG, "k must be in [1, 63]");
+        const int64_t k_max = kstar ? 63 : k;
+        OA_REQUIRE(2 * (n_samples + n_streams) * k_max < ((int64_t)1 << 31), OMPL_AMD_ERR_ARG,
                    "n_samples * k too large for one roadmap");
         OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "bad segment settings");
         auto *h = new ompl_amd_prm_s();

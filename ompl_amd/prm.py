@@ -53,16 +53,18 @@ class Roadmap:
 
 def build_roadmap(env, seed: int, n_samples: int, k: int = DEFAULT_NEAREST_NEIGHBORS, lvs: float = 0.01,
                   factor: int = 1, n_streams: int = 64, fetch: bool = True,
-                  autotune: bool = False) -> Roadmap | dict:
+                  autotune: bool = False, kstar: bool = False) -> Roadmap | dict:
     """Builds the roadmap on the device.  lvs is the longest valid segment in
     radians (the motion resolution); samples come from n_streams generators
     seeded seed, seed + 1, ...  autotune runs the motion autotuner on the
-    candidate edges first (the winner stays installed).  With fetch=False only
-    the statistics are returned."""
+    candidate edges first (the winner stays installed).  kstar uses PRM*'s
+    KStarStrategy, k = ceil((e + e/7) ln(milestones)) (ConnectionStrategy.h:
+    125-157), instead of k.  With fetch=False only the statistics are
+    returned."""
     h = C.c_void_p()
     st = PrmStats()
     call("ompl_amd_prm_build", env.handle, int(seed) & 0xffffffff, int(n_streams), int(n_samples), int(k),
-         float(lvs), int(factor), 1 if autotune else 0, C.byref(h), C.byref(st))
+         float(lvs), int(factor), (1 if autotune else 0) | (2 if kstar else 0), C.byref(h), C.byref(st))
     try:
         stats = st.as_dict()
         if not fetch:

@@ -368,8 +368,17 @@ def disjoint_set_labels(n: int, edges: np.ndarray, valid: np.ndarray) -> np.ndar
     return np.array([find(i) for i in range(n)], np.int32)
 
 
+def kstar_k(n_milestones: int, dim: int = 7) -> int:
+    """KStarStrategy's k (geometric/planners/prm/ConnectionStrategy.h:137-150): ceil((e + e/d) log n),
+    capped at 63 (the device k-NN bound)."""
+    import math
+
+    kc = math.e + math.e / dim
+    return max(1, min(63, math.ceil(kc * math.log(n_milestones))))
+
+
 def prm_batch(env: OracleEnv, seed: int, n_samples: int, k: int, lvs: float, factor: int = 1, n_streams: int = 1,
-              threads: int = 1):
+              threads: int = 1, kstar: bool = False):
     """Batch PRM roadmap: the steps of PRM::growRoadmap / addMilestone (geometric/planners/prm/src/
     PRM.cpp:364-385, 570-604) with KStrategy (geometric/planners/prm/ConnectionStrategy.h:56-81) over a
     fixed sample set.  Samples: stream s = mt19937(seed + s), n_samples rounded up to a multiple of
@@ -384,7 +393,9 @@ def prm_batch(env: OracleEnv, seed: int, n_samples: int, k: int, lvs: float, fac
     sidx = np.nonzero(valid)[0].astype(np.int32)
     nodes = samples[sidx]
     nv = nodes.shape[0]
-    out = dict(samples=samples, valid=valid, sample_index=sidx, milestones=nodes)
+    if kstar and nv >= 1:
+        k = kstar_k(nv)
+    out = dict(samples=samples, valid=valid, sample_index=sidx, milestones=nodes, k=k)
     if nv < 2:
         out.update(edges=np.zeros((0, 2), np.int32), edge_valid=np.zeros(0, bool), component=np.arange(nv,
                    dtype=np.int32))

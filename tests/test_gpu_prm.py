@@ -116,3 +116,14 @@ def test_prm_argument_errors(gpu):
     for kw in (dict(k=0), dict(k=64), dict(lvs=0.0), dict(n_streams=0)):
         with pytest.raises(OmplAmdError):
             build_roadmap(env, 1, 100, **kw)
+
+
+def test_prm_kstar_matches_oracle(gpu, oracle):
+    from ompl_amd.prm import build_roadmap
+
+    s, b = gpu.workload.obstacles(1007, 16, 16)
+    env = gpu.Env(s, b)
+    g = build_roadmap(env, 11, 2500, n_streams=16, kstar=True)
+    o = oracle.prm_batch(oracle.OracleEnv(s, b), 11, 2500, 10, 0.01, 1, 16, threads=8, kstar=True)
+    assert g.stats["k"] == o["k"] == oracle.kstar_k(g.milestones.shape[0])
+    assert_same(g, o)

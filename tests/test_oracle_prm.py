@@ -55,3 +55,13 @@ def test_prm_batch_structure(oracle):
         u, v = r["edges"][t]
         ok, _ = oracle.check_motion(env, nodes[u], nodes[v], 0.01)
         assert ok == r["edge_valid"][t]
+
+
+def test_kstar_k_known_values(oracle):
+    # k = ceil((e + e/7) ln n): e(1 + 1/7) = 3.1066078..., ln 1000 = 6.9077553 -> 21.46 -> 22;
+    # ln 135102 = 11.813786 -> 36.70 -> 37; ln 2 = 0.693 -> 2.15 -> 3; n = 1 -> 0, raised to 1
+    assert oracle.kstar_k(1000) == 22
+    assert oracle.kstar_k(135102) == 37
+    assert oracle.kstar_k(2) == 3
+    assert oracle.kstar_k(1) == 1
+    assert oracle.kstar_k(10 ** 12) == 63
