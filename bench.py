@@ -425,6 +425,26 @@ def main(argv=None):
         torch.cuda.synchronize()
         step_ms.append((time.perf_counter() - s0) * 1e3)
 
+    # ---- the same batch replayed from a HIP graph (launch cost removed)
+    g = oa.MotionGraph(env, A, B, lvs)
+    for _ in range(5):
+        g.launch()
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        g.launch()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    graph_ms = ev0.elapsed_time(ev1) / args.steps
+    graph_lat = []
+    for _ in range(min(args.steps, 200)):
+        s0 = time.perf_counter()
+        g.launch()
+        torch.cuda.synchronize()
+        graph_lat.append((time.perf_counter() - s0) * 1e3)
+    graph_same = bool(np.array_equal(g.out.cpu().numpy(), valid))
+    g.close()
+
     # ---- per-kernel durations: HIP events recorded by the library on the
     #      launch stream around every launch of the same K steps
     oa.profile.reset()
@@ -525,6 +545,9 @@ def main(argv=None):
         "kernels": kernels,
         "valid_fraction": float(valid.mean()),
         "step_latency_ms": {"p50": percentile(step_ms, 50), "p99": percentile(step_ms, 99)},
+        "graph": {"ms_per_step": graph_ms, "edge_checks_per_s": N_EDGES / (graph_ms * 1e-3),
+                  "step_latency_ms": {"p50": percentile(graph_lat, 50), "p99": percentile(graph_lat, 99)},
+                  "same_result": graph_same},
         "plain": {"edge_checks_per_s": plain_rate, "edges": P,
                   "call_latency_ms": {"p50": percentile(plain_lat, 50), "p99": percentile(plain_lat, 99)}},
         "batched_over_plain": value / plain_rate,

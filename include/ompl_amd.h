@@ -129,6 +129,20 @@ int ompl_amd_check_state_lists_dev(ompl_amd_env env, const double *q_soa_dev, in
  * *first_invalid is written only when the list is invalid. */
 int ompl_amd_check_states(ompl_amd_env env, const double *states, int count, int *valid, int *first_invalid);
 
+/* Captured batch (HIP graph): records the launches of one
+ * ompl_amd_check_motions_dev call with these exact pointers, sizes and the
+ * current launch shape, with its own survivor workspace; each launch replays
+ * the whole batch on `stream` (results as the eager call).  Replays of one
+ * graph must not overlap each other.  No counterpart in the reference: it
+ * removes the per-call launch cost for planners that re-check a fixed batch
+ * slot (e.g. a PRM expansion buffer). */
+typedef struct ompl_amd_motion_graph_s *ompl_amd_motion_graph;
+int ompl_amd_motion_graph_create(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
+                                 double longest_valid_segment, unsigned factor, uint8_t *out_dev,
+                                 int32_t *checked_dev, ompl_amd_motion_graph *graph);
+int ompl_amd_motion_graph_launch(ompl_amd_motion_graph graph, void *stream);
+int ompl_amd_motion_graph_destroy(ompl_amd_motion_graph graph);
+
 /* Launch-shape knobs used by the batched kernels (set by the autotuner).
  * A state is checked by lanes_per_state x obstacle_parts lanes: the first
  * splits the arm's 28 spheres, the second the obstacle list.  Built shapes

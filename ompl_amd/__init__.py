@@ -241,6 +241,38 @@ def check_motion_last_valid(env: Env, a, b, lvs: float, factor: int = 1, last_t:
     return bool(v.value), t.value
 
 
+class MotionGraph:
+    """A motion batch captured in a HIP graph: launch() replays exactly the
+    check_motions call it was made from (same device arrays, sizes and launch
+    shape) on the given stream.  Replays of one graph must not overlap."""
+
+    def __init__(self, env: Env, a_soa, b_soa, lvs: float, factor: int = 1, out=None, checked=None):
+        torch = _torch()
+        n = a_soa.shape[1]
+        self.out = out if out is not None else torch.empty(n, dtype=torch.uint8, device=a_soa.device)
+        self.checked = checked
+        self._keep = (a_soa, b_soa)  # the graph reads these arrays on every replay
+        h = C.c_void_p()
+        call("ompl_amd_motion_graph_create", env.handle, _ptr(a_soa), _ptr(b_soa), n, float(lvs), int(factor),
+             _ptr(self.out), _ptr(checked), C.byref(h))
+        self.handle = h
+
+    def launch(self, stream=None):
+        call("ompl_amd_motion_graph_launch", self.handle, _stream(stream))
+        return self.out
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            call("ompl_amd_motion_graph_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def check_state_lists(env: Env, q_soa, offsets, stream=None):
     """Batched SpaceInformation::checkMotion(states, count[, firstInvalidStateIndex])
     (base/src/SpaceInformation.cpp:288-341) over lists of precomputed states.

@@ -46,6 +46,9 @@ SIGNATURES = {
                                                    C.POINTER(C.c_double)]),
     "ompl_amd_check_state_lists_dev": (C.c_int, [vp, vp, C.c_int64, vp, C.c_int64, vp, vp, vp]),
     "ompl_amd_check_states": (C.c_int, [vp, vp, C.c_int, c_int_p, c_int_p]),
+    "ompl_amd_motion_graph_create": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, vp, C.POINTER(vp)]),
+    "ompl_amd_motion_graph_launch": (C.c_int, [vp, vp]),
+    "ompl_amd_motion_graph_destroy": (C.c_int, [vp]),
     "ompl_amd_get_tuning": (C.c_int, [vp]),
     "ompl_amd_set_tuning": (C.c_int, [vp]),
     "ompl_amd_autotune_motions": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, C.c_int, vp,

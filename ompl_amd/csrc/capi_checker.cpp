@@ -16,7 +16,7 @@ Env &env_of(ompl_amd_env e)
 
 // Batched DiscreteMotionValidator::checkMotion(s1, s2) over device arrays.
 void check_motions_t(const Tuning &t, const Env &env, const double *a, const double *b, int64_t n, double lvs,
-                     unsigned factor, uint8_t *out, int32_t *checked, hipStream_t s)
+                     unsigned factor, uint8_t *out, int32_t *checked, hipStream_t s, Workspace *own = nullptr)
 {
     OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative edge count");
     if (n == 0) return;
@@ -42,7 +42,7 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         // Strategy 4 computes each state's FK once per wave and shares it
         // across the obstacle partitions.
         const bool shared = t.motion_strategy == 4;
-        Workspace &w = workspace(s, (size_t)n);
+        Workspace &w = own ? *own : workspace(s, (size_t)n);
         A.list = w.list;
         A.list_count = w.counters + 8;
         A.max_m = w.counters + 9;
@@ -56,7 +56,7 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         A.start = 1;
         launch_motion_fused(A, exact, LE, OE, LW, OW, t.motion_block, s);
     } else if (t.motion_strategy == 1) {
-        Workspace &w = workspace(s, (size_t)n);
+        Workspace &w = own ? *own : workspace(s, (size_t)n);
         A.list = w.list;
         A.list_count = w.counters;
         A.queue = w.counters + 1;
@@ -101,6 +101,25 @@ std::vector<Tuning> motion_candidates(const Tuning &base)
 }
 
 } // namespace
+
+// A captured motion batch: the launches of one check_motions call recorded in
+// a HIP graph (pointers, sizes and launch shape fixed at capture) with its own
+// survivor workspace, so replays on any stream do not share state with eager
+// calls.  No memset nodes: the strategies clear their counters with kernels.
+struct ompl_amd_motion_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Workspace ws;
+    ~ompl_amd_motion_graph_s()
+    {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        if (ws.list) hipFree(ws.list);
+        if (ws.first_bad) hipFree(ws.first_bad);
+        if (ws.surv_nd) hipFree(ws.surv_nd);
+        if (ws.counters) hipFree(ws.counters);
+    }
+};
 
 extern "C" {
 
@@ -366,3 +385,62 @@ extern "C" int ompl_amd_autotune_motions(ompl_amd_env e, const double *a, const 
         if (best_us) *best_us = 1e3 * (double)win_ms / reps;
     });
 }
+
+extern "C" {
+
+int ompl_amd_motion_graph_create(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
+                                 unsigned factor, uint8_t *out, int32_t *checked, ompl_amd_motion_graph *graph)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(graph, OMPL_AMD_ERR_ARG, "null graph handle");
+        OA_REQUIRE(n >= 1, OMPL_AMD_ERR_ARG, "a captured batch needs at least one edge");
+        auto *g = new ompl_amd_motion_graph_s();
+        try {
+            const size_t cap = (size_t)n < 1024 ? 1024 : (size_t)n;
+            OA_HIP(hipMalloc(&g->ws.counters, 64 * sizeof(int32_t)));
+            OA_HIP(hipMemset(g->ws.counters, 0, 64 * sizeof(int32_t)));
+            OA_HIP(hipMalloc(&g->ws.list, cap * sizeof(int32_t)));
+            OA_HIP(hipMalloc(&g->ws.first_bad, cap * sizeof(int32_t)));
+            OA_HIP(hipMalloc(&g->ws.surv_nd, cap * sizeof(int32_t)));
+            g->ws.list_cap = cap;
+            OA_HIP(hipDeviceSynchronize());
+            // capture on the calling thread's own stream (the null stream cannot capture)
+            hipStream_t cs = thread_resources().stream;
+            const Tuning t = get_tuning();
+            OA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            try {
+                check_motions_t(t, env, a, b, n, lvs, factor, out, checked, cs, &g->ws);
+            } catch (...) {
+                hipGraph_t dropped = nullptr;
+                hipStreamEndCapture(cs, &dropped);
+                if (dropped) hipGraphDestroy(dropped);
+                throw;
+            }
+            OA_HIP(hipStreamEndCapture(cs, &g->graph));
+            OA_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *graph = g;
+    });
+}
+
+int ompl_amd_motion_graph_launch(ompl_amd_motion_graph graph, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(graph && graph->exec, OMPL_AMD_ERR_ARG, "null graph");
+        OA_HIP(hipGraphLaunch(graph->exec, resolve_stream(stream)));
+    });
+}
+
+int ompl_amd_motion_graph_destroy(ompl_amd_motion_graph graph)
+{
+    return guarded([&] {
+        if (graph) OA_HIP(hipDeviceSynchronize());
+        delete graph;
+    });
+}
+
+} // extern "C"

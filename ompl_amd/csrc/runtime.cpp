@@ -177,6 +177,8 @@ ProfScope::ProfScope(const char *name, hipStream_t s) : stream(s)
 {
     std::lock_guard<std::mutex> lk(g_prof_mutex);
     if (!g_prof_on) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return; // not in graphs
     ProfRec r{name, take_event(), take_event()};
     OA_HIP(hipEventRecord(r.start, s));
     idx = (int)g_prof.size();

@@ -188,3 +188,35 @@ def test_empty_batches(gpu):
     z = torch.empty((7, 0), dtype=torch.float64, device="cuda")
     assert gpu.check_motions(env, z, z, 0.01).numel() == 0
     assert gpu.validate_states(env, z.float()).numel() == 0
+
+
+@pytest.mark.parametrize("strategy,eshape,wshape", [(4, (8, 8), (8, 8)), (3, (8, 2), (8, 1)), (1, (8, 1), (1, 1)),
+                                                    (0, (8, 1), (8, 8)), (2, (8, 2), (8, 8))])
+def test_motion_graph_replays_match_eager(gpu, oracle, strategy, eshape, wshape):
+    """A captured batch replayed several times, with eager batches and host
+    synchronisations in between, gives the eager result on every replay."""
+    import torch
+
+    set_strategy(gpu, strategy, eshape, wshape)
+    env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
+    a, b = motion_case(gpu, 4, 4096)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs)
+    A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
+    checked = torch.empty(4096, dtype=torch.int32, device="cuda")
+    g = gpu.MotionGraph(env, A, B, lvs, checked=checked)
+    a2, b2 = motion_case(gpu, 5, 2048)
+    A2, B2 = to_dev(a2.T.copy()), to_dev(b2.T.copy())
+    ref2, _ = oracle.check_motions(oenv, a2, b2, lvs)
+    for rep in range(4):
+        g.out.fill_(7)
+        checked.fill_(-1)
+        g.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(g.out.cpu().numpy(), ref_v), rep
+        assert np.array_equal(checked.cpu().numpy(), ref_c), rep
+        v2 = gpu.check_motions(env, A2, B2, lvs)  # eager work between replays
+        torch.cuda.synchronize()
+        assert np.array_equal(v2.cpu().numpy(), ref2), rep
+    g.close()
+    gpu.set_tuning(**DEFAULT)
