@@ -227,6 +227,21 @@ int ompl_amd_sample_valid_dev(ompl_amd_env env, const uint32_t *seeds, int n_str
                               const double *lo, const double *hi, double *out_dev, uint8_t *found_dev,
                               int32_t *draws_dev, void *stream);
 
+/* Generators that persist between calls (the RNG of a long-lived sampler):
+ * stream s starts as mt19937 seeded seeds[s]; every call continues where the
+ * previous one stopped, exactly as successive calls on one reference
+ * sampler.  rng_sample_valid leaves each generator just after the last draw
+ * its calls consumed. */
+typedef struct ompl_amd_rng_s *ompl_amd_rng;
+int ompl_amd_rng_create(const uint32_t *seeds, int n_streams, ompl_amd_rng *out);
+int ompl_amd_rng_destroy(ompl_amd_rng rng);
+/* out_dev [n_streams][n_per_stream][dim]; out_f32_soa_dev (optional) [dim][n_streams*n_per_stream]. */
+int ompl_amd_rng_sample_uniform_dev(ompl_amd_rng rng, int64_t n_per_stream, int dim, const double *lo,
+                                    const double *hi, double *out_dev, float *out_f32_soa_dev, void *stream);
+int ompl_amd_rng_sample_valid_dev(ompl_amd_rng rng, ompl_amd_env env, int64_t n_calls, int attempts,
+                                  const double *lo, const double *hi, double *out_dev, uint8_t *found_dev,
+                                  int32_t *draws_dev, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Nearest neighbours -- ompl::NearestNeighbors<_T>                      */
 /* (datastructures/NearestNeighbors.h:46-115), exact results in FP32,   */

@@ -60,6 +60,10 @@ SIGNATURES = {
     "ompl_amd_profile_query": (C.c_int, [C.c_char_p, c_i64_p, C.POINTER(C.c_double)]),
     "ompl_amd_sample_uniform_dev": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int, vp, vp, vp, vp, vp]),
     "ompl_amd_sample_valid_dev": (C.c_int, [vp, vp, C.c_int, C.c_int64, C.c_int, vp, vp, vp, vp, vp, vp]),
+    "ompl_amd_rng_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+    "ompl_amd_rng_destroy": (C.c_int, [vp]),
+    "ompl_amd_rng_sample_uniform_dev": (C.c_int, [vp, C.c_int64, C.c_int, vp, vp, vp, vp, vp]),
+    "ompl_amd_rng_sample_valid_dev": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp, vp, vp, vp, vp]),
     "ompl_amd_nn_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "ompl_amd_nn_destroy": (C.c_int, [vp]),
     "ompl_amd_nn_clear": (C.c_int, [vp]),

@@ -159,6 +159,7 @@ struct MtState {
 
 struct ValidCursor {
     int64_t call;     // next call of this stream
+    int64_t used;     // draws consumed from the generator by the calls so far
     int32_t attempts; // draws made by the current call
     int32_t pad;
 };
@@ -176,27 +177,29 @@ __global__ void mt_seed_kernel(const uint32_t *__restrict__ seeds, int n, MtStat
     st[s].idx = kMtN;
 }
 
-// One wave per stream: resumes the generator, writes C states of 7 doubles
-// ([stream][C][7]) and their FP32 copy ([7][n_active * C]).  Streams listed in
-// `active` only (compacted by the host from the previous round).
+// One wave per listed stream: resumes the generator and writes C states of
+// `dim` doubles ([a][C][dim]) and, when out_f32 is given, their FP32 copy
+// ([dim][n_active * C]); `active` lists the streams (all of them, or the
+// unfinished ones of a valid-sampling round).
 __global__ void __launch_bounds__(64) resume_uniform_kernel(MtState *__restrict__ st, const int32_t *__restrict__ active,
-                                                            int n_active, int64_t C, const double *__restrict__ lo_g,
+                                                            int n_active, int64_t C, int dim,
+                                                            const double *__restrict__ lo_g,
                                                             const double *__restrict__ hi_g, double *__restrict__ out,
                                                             float *__restrict__ out_f32)
 {
     __shared__ uint32_t mt[kMtN];
-    __shared__ double lo[kDim7], span[kDim7];
+    __shared__ double lo[16], span[16];
     const int a = blockIdx.x;
     const int t = threadIdx.x;
-    MtState &S = st[active[a]];
+    MtState &S = st[active ? active[a] : a];
     for (int i = t; i < kMtN; i += 64) mt[i] = S.mt[i];
-    if (t < kDim7) {
+    if (t < dim) {
         lo[t] = lo_g[t];
         span[t] = hi_g[t] - lo_g[t];
     }
     int idx = S.idx;
     __syncthreads();
-    const int64_t total = C * kDim7;
+    const int64_t total = C * dim;
     const int64_t n_f32 = (int64_t)n_active * C;
     double *o = out + (int64_t)a * total;
     for (int64_t z = 0; z < total;) {
@@ -209,10 +212,10 @@ __global__ void __launch_bounds__(64) resume_uniform_kernel(MtState *__restrict_
             const uint32_t w1 = mt_temper(mt[idx + 2 * p]);
             const uint32_t w2 = mt_temper(mt[idx + 2 * p + 1]);
             const int64_t zz = z + p;
-            const int d = (int)(zz % kDim7);
+            const int d = (int)(zz % dim);
             const double v = span[d] * canonical53(w1, w2) + lo[d];
             o[zz] = v;
-            out_f32[(int64_t)d * n_f32 + (int64_t)a * C + zz / kDim7] = (float)v;
+            if (out_f32) out_f32[(int64_t)d * n_f32 + (int64_t)a * C + zz / dim] = (float)v;
         }
         __syncthreads();
         idx += 2 * (int)cnt;
@@ -220,6 +223,32 @@ __global__ void __launch_bounds__(64) resume_uniform_kernel(MtState *__restrict_
     }
     for (int i = t; i < kMtN; i += 64) S.mt[i] = mt[i];
     if (t == 0) S.idx = idx;
+}
+
+// One wave per stream: st = saved advanced by the cursor's consumed draws
+// (2 words per coordinate), i.e. the generator exactly after the last draw
+// the calls used; the draws of the last round past that point are dropped.
+__global__ void __launch_bounds__(64) mt_advance_kernel(MtState *__restrict__ st, const MtState *__restrict__ saved,
+                                                        const ValidCursor *__restrict__ cur, int dim)
+{
+    __shared__ uint32_t mt[kMtN];
+    const int s = blockIdx.x;
+    const int t = threadIdx.x;
+    for (int i = t; i < kMtN; i += 64) mt[i] = saved[s].mt[i];
+    int idx = saved[s].idx;
+    __syncthreads();
+    int64_t skip = cur[s].used * 2 * dim;
+    while (skip > 0) {
+        if (idx >= kMtN) {
+            mt_twist(mt);
+            idx = 0;
+        }
+        const int take = (int)min((int64_t)(kMtN - idx), skip);
+        idx += take;
+        skip -= take;
+    }
+    for (int i = t; i < kMtN; i += 64) st[s].mt[i] = mt[i];
+    if (t == 0) st[s].idx = idx;
 }
 
 // One thread per active stream: hands the round's draws to its calls in order.
@@ -249,6 +278,7 @@ __global__ void assign_valid_kernel(ValidCursor *__restrict__ cur, const int32_t
             c.attempts = 0;
         }
     }
+    c.used += used;
     cur[s] = c;
     atomicAdd(&stats[0], (unsigned long long)used);
     atomicAdd(&stats[1], (unsigned long long)used_valid);
@@ -268,6 +298,105 @@ __global__ void next_active_kernel(const ValidCursor *__restrict__ cur, int n_st
         base = __shfl(base, __ffsll((unsigned long long)m) - 1);
     }
     if (open) active[base + __popcll(m & (((uint64_t)1 << lane) - 1))] = s;
+}
+
+// Device bounds of a sampling call (host arrays staged per call).
+struct Bounds {
+    double *lo, *hi;
+};
+
+static Bounds stage_bounds(const double *lo, const double *hi, int dim, hipStream_t s)
+{
+    double *d = nullptr;
+    OA_HIP(hipMallocAsync((void **)&d, 32 * sizeof(double), s));
+    OA_HIP(hipMemcpyAsync(d, lo, dim * sizeof(double), hipMemcpyHostToDevice, s));
+    OA_HIP(hipMemcpyAsync(d + 16, hi, dim * sizeof(double), hipMemcpyHostToDevice, s));
+    return Bounds{d, d + 16};
+}
+
+// UniformValidStateSampler::sample x n_calls on each of n_streams generators
+// (gen, device), continuing them; on return every generator stands exactly
+// after the last draw its calls consumed.  Blocking.
+static void sample_valid_rounds(ompl_amd_env env, MtState *gen, int n_streams, int64_t n_calls, int attempts,
+                                const double *lo, const double *hi, double *out_dev, uint8_t *found_dev,
+                                int32_t *draws_dev, hipStream_t s)
+{
+    const size_t head = 256;
+    const size_t bytes = head + (size_t)n_streams * (sizeof(MtState) + sizeof(ValidCursor) + sizeof(int32_t));
+    char *base = nullptr;
+    OA_HIP(hipMallocAsync((void **)&base, bytes, s));
+    unsigned long long *stats = (unsigned long long *)base; // [0] used [1] valid [2] max remaining [3] active
+    MtState *saved = (MtState *)(base + head);
+    ValidCursor *cur = (ValidCursor *)(saved + n_streams);
+    int32_t *active = (int32_t *)(cur + n_streams);
+    const Bounds bd = stage_bounds(lo, hi, kDim7, s);
+    OA_HIP(hipMemcpyAsync(saved, gen, (size_t)n_streams * sizeof(MtState), hipMemcpyDeviceToDevice, s));
+    OA_HIP(hipMemsetAsync(cur, 0, (size_t)n_streams * sizeof(ValidCursor), s));
+    std::vector<int32_t> all((size_t)n_streams);
+    for (int i = 0; i < n_streams; ++i) all[(size_t)i] = i;
+    OA_HIP(hipMemcpyAsync(active, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    int n_active = n_streams;
+    int64_t remaining = n_calls;
+    double draws_per_call = 1.0; // refined from every round
+    double *cand = nullptr;
+    float *cand32 = nullptr;
+    uint8_t *valid = nullptr;
+    size_t cap = 0;
+    const int64_t max_round = ((int64_t)64 << 20) / ((int64_t)n_streams * kDim7 * 8) + 16; // ~64 MiB of draws
+    int64_t rounds = 0;
+    const int64_t round_limit = n_calls * (int64_t)attempts + 2;
+    while (n_active > 0) {
+        if (++rounds > round_limit) fail(OMPL_AMD_ERR_INTERNAL, "valid sampler did not finish");
+        const double want = (double)remaining * std::min((double)attempts, draws_per_call) * 1.25 + 16.0;
+        const int64_t C = std::max<int64_t>(
+            16, std::min<int64_t>((int64_t)want, std::min<int64_t>(max_round, remaining * (int64_t)attempts)));
+        const size_t need = (size_t)n_active * (size_t)C;
+        if (need > cap) {
+            if (cand) {
+                OA_HIP(hipFreeAsync(cand, s));
+                OA_HIP(hipFreeAsync(cand32, s));
+                OA_HIP(hipFreeAsync(valid, s));
+            }
+            cap = need;
+            OA_HIP(hipMallocAsync((void **)&cand, cap * kDim7 * sizeof(double), s));
+            OA_HIP(hipMallocAsync((void **)&cand32, cap * kDim7 * sizeof(float), s));
+            OA_HIP(hipMallocAsync((void **)&valid, cap, s));
+        }
+        {
+            ProfScope ps("sample_valid", s);
+            hipLaunchKernelGGL(resume_uniform_kernel, dim3(n_active), dim3(64), 0, s, gen, active, n_active, C, kDim7,
+                               bd.lo, bd.hi, cand, cand32);
+            OA_HIP(hipGetLastError());
+        }
+        const int rc = ompl_amd_validate_states_dev(env, cand32, (int64_t)need, valid, s);
+        if (rc != OMPL_AMD_OK) fail(rc, "%s", last_error());
+        OA_HIP(hipMemsetAsync(stats, 0, 4 * sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(assign_valid_kernel, dim3((n_active + 63) / 64), dim3(64), 0, s, cur, active, n_active, C,
+                           valid, cand, n_calls, attempts, out_dev, found_dev, draws_dev, stats);
+        hipLaunchKernelGGL(next_active_kernel, dim3((n_streams + 255) / 256), dim3(256), 0, s, cur, n_streams,
+                           n_calls, active, stats);
+        OA_HIP(hipGetLastError());
+        unsigned long long h[4];
+        OA_HIP(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
+        OA_HIP(hipStreamSynchronize(s));
+        n_active = (int)h[3];
+        remaining = (int64_t)h[2];
+        if (h[1] > 0)
+            draws_per_call = std::max(1.0, (double)h[0] / (double)h[1]);
+        else
+            draws_per_call = (double)attempts;
+    }
+    // drop the draws of the last round that no call used
+    hipLaunchKernelGGL(mt_advance_kernel, dim3(n_streams), dim3(64), 0, s, gen, saved, cur, kDim7);
+    OA_HIP(hipGetLastError());
+    if (cand) {
+        OA_HIP(hipFreeAsync(cand, s));
+        OA_HIP(hipFreeAsync(cand32, s));
+        OA_HIP(hipFreeAsync(valid, s));
+    }
+    OA_HIP(hipFreeAsync(bd.lo, s));
+    OA_HIP(hipFreeAsync(base, s));
+    OA_HIP(hipStreamSynchronize(s));
 }
 
 } // namespace ompl_amd
@@ -304,9 +433,36 @@ extern "C" int ompl_amd_sample_uniform_dev(const uint32_t *seeds, int n_streams,
     });
 }
 
-extern "C" int ompl_amd_sample_valid_dev(ompl_amd_env env, const uint32_t *seeds, int n_streams, int64_t n_calls,
-                                         int attempts, const double *lo, const double *hi, double *out_dev,
-                                         uint8_t *found_dev, int32_t *draws_dev, void *stream)
+// Generators kept on the device between calls: the exact continuation of a
+// long-lived sampler's RNG.
+struct ompl_amd_rng_s {
+    MtState *gen = nullptr;
+    int n = 0;
+    ~ompl_amd_rng_s()
+    {
+        if (gen) hipFree(gen);
+    }
+};
+
+static MtState *seed_generators(const uint32_t *seeds, int n, hipStream_t s)
+{
+    MtState *gen = nullptr;
+    uint32_t *dseeds = nullptr;
+    OA_HIP(hipMallocAsync((void **)&gen, (size_t)n * sizeof(MtState), s));
+    OA_HIP(hipMallocAsync((void **)&dseeds, (size_t)n * sizeof(uint32_t), s));
+    OA_HIP(hipMemcpyAsync(dseeds, seeds, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(mt_seed_kernel, dim3((n + 63) / 64), dim3(64), 0, s, dseeds, n, gen);
+    OA_HIP(hipGetLastError());
+    OA_HIP(hipFreeAsync(dseeds, s));
+    OA_HIP(hipStreamSynchronize(s)); // the seeds are a host array
+    return gen;
+}
+
+extern "C" {
+
+int ompl_amd_sample_valid_dev(ompl_amd_env env, const uint32_t *seeds, int n_streams, int64_t n_calls, int attempts,
+                              const double *lo, const double *hi, double *out_dev, uint8_t *found_dev,
+                              int32_t *draws_dev, void *stream)
 {
     return guarded([&] {
         OA_REQUIRE(env, OMPL_AMD_ERR_ARG, "null environment");
@@ -314,81 +470,77 @@ extern "C" int ompl_amd_sample_valid_dev(ompl_amd_env env, const uint32_t *seeds
         if (n_streams == 0 || n_calls == 0) return;
         OA_REQUIRE(seeds && lo && hi && out_dev && found_dev, OMPL_AMD_ERR_ARG, "null pointer");
         hipStream_t s = resolve_stream(stream);
-        // device state: generators, cursors, active list, round statistics, bounds, seeds
-        const size_t head = 256;
-        const size_t bytes = head + (size_t)n_streams * (sizeof(MtState) + sizeof(ValidCursor) + 2 * sizeof(int32_t));
-        char *base = nullptr;
-        OA_HIP(hipMallocAsync((void **)&base, bytes, s));
-        unsigned long long *stats = (unsigned long long *)base; // [0] used [1] valid [2] max remaining [3] active
-        double *dlo = (double *)(base + 64), *dhi = dlo + kDim7;
-        MtState *gen = (MtState *)(base + head);
-        ValidCursor *cur = (ValidCursor *)(gen + n_streams);
-        int32_t *active = (int32_t *)(cur + n_streams);
-        uint32_t *dseeds = (uint32_t *)(active + n_streams);
-        OA_HIP(hipMemcpyAsync(dlo, lo, kDim7 * sizeof(double), hipMemcpyHostToDevice, s));
-        OA_HIP(hipMemcpyAsync(dhi, hi, kDim7 * sizeof(double), hipMemcpyHostToDevice, s));
-        OA_HIP(hipMemcpyAsync(dseeds, seeds, (size_t)n_streams * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-        OA_HIP(hipMemsetAsync(cur, 0, (size_t)n_streams * sizeof(ValidCursor), s));
-        hipLaunchKernelGGL(mt_seed_kernel, dim3((n_streams + 63) / 64), dim3(64), 0, s, dseeds, n_streams, gen);
-        OA_HIP(hipGetLastError());
-        std::vector<int32_t> all((size_t)n_streams);
-        for (int i = 0; i < n_streams; ++i) all[(size_t)i] = i;
-        OA_HIP(hipMemcpyAsync(active, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        int n_active = n_streams;
-        int64_t remaining = n_calls;
-        double draws_per_call = 1.0; // refined from every round
-        double *cand = nullptr;
-        float *cand32 = nullptr;
-        uint8_t *valid = nullptr;
-        size_t cap = 0;
-        const int64_t max_round = ((int64_t)64 << 20) / ((int64_t)n_streams * kDim7 * 8) + 16; // ~64 MiB of draws
-        int64_t rounds = 0;
-        const int64_t round_limit = n_calls * (int64_t)attempts + 2;
-        while (n_active > 0) {
-            if (++rounds > round_limit) fail(OMPL_AMD_ERR_INTERNAL, "valid sampler did not finish");
-            const double want = (double)remaining * std::min((double)attempts, draws_per_call) * 1.25 + 16.0;
-            const int64_t C = std::max<int64_t>(16, std::min<int64_t>((int64_t)want, std::min<int64_t>(
-                                                        max_round, remaining * (int64_t)attempts)));
-            const size_t need = (size_t)n_active * (size_t)C;
-            if (need > cap) {
-                if (cand) {
-                    OA_HIP(hipFreeAsync(cand, s));
-                    OA_HIP(hipFreeAsync(cand32, s));
-                    OA_HIP(hipFreeAsync(valid, s));
-                }
-                cap = need;
-                OA_HIP(hipMallocAsync((void **)&cand, cap * kDim7 * sizeof(double), s));
-                OA_HIP(hipMallocAsync((void **)&cand32, cap * kDim7 * sizeof(float), s));
-                OA_HIP(hipMallocAsync((void **)&valid, cap, s));
-            }
-            {
-                ProfScope ps("sample_valid", s);
-                hipLaunchKernelGGL(resume_uniform_kernel, dim3(n_active), dim3(64), 0, s, gen, active, n_active, C,
-                                   dlo, dhi, cand, cand32);
-                OA_HIP(hipGetLastError());
-            }
-            const int rc = ompl_amd_validate_states_dev(env, cand32, (int64_t)need, valid, s);
-            if (rc != OMPL_AMD_OK) fail(rc, "%s", last_error());
-            OA_HIP(hipMemsetAsync(stats, 0, 4 * sizeof(unsigned long long), s));
-            hipLaunchKernelGGL(assign_valid_kernel, dim3((n_active + 63) / 64), dim3(64), 0, s, cur, active,
-                               n_active, C, valid, cand, n_calls, attempts, out_dev, found_dev, draws_dev, stats);
-            hipLaunchKernelGGL(next_active_kernel, dim3((n_streams + 255) / 256), dim3(256), 0, s, cur, n_streams,
-                               n_calls, active, stats);
-            OA_HIP(hipGetLastError());
-            unsigned long long h[4];
-            OA_HIP(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s));
-            OA_HIP(hipStreamSynchronize(s));
-            n_active = (int)h[3];
-            remaining = (int64_t)h[2];
-            if (h[1] > 0) draws_per_call = std::max(1.0, (double)h[0] / (double)h[1]);
-            else draws_per_call = (double)attempts;
+        MtState *gen = seed_generators(seeds, n_streams, s);
+        try {
+            sample_valid_rounds(env, gen, n_streams, n_calls, attempts, lo, hi, out_dev, found_dev, draws_dev, s);
+        } catch (...) {
+            hipStreamSynchronize(s);
+            hipFree(gen);
+            throw;
         }
-        if (cand) {
-            OA_HIP(hipFreeAsync(cand, s));
-            OA_HIP(hipFreeAsync(cand32, s));
-            OA_HIP(hipFreeAsync(valid, s));
-        }
-        OA_HIP(hipFreeAsync(base, s));
+        OA_HIP(hipFreeAsync(gen, s));
         OA_HIP(hipStreamSynchronize(s));
     });
 }
+
+int ompl_amd_rng_create(const uint32_t *seeds, int n_streams, ompl_amd_rng *out)
+{
+    return guarded([&] {
+        OA_REQUIRE(out && seeds && n_streams >= 1, OMPL_AMD_ERR_ARG, "bad arguments");
+        ensure_init();
+        auto *r = new ompl_amd_rng_s();
+        try {
+            r->gen = seed_generators(seeds, n_streams, default_stream());
+        } catch (...) {
+            delete r;
+            throw;
+        }
+        r->n = n_streams;
+        *out = r;
+    });
+}
+
+int ompl_amd_rng_destroy(ompl_amd_rng rng)
+{
+    return guarded([&] {
+        if (rng) OA_HIP(hipDeviceSynchronize());
+        delete rng;
+    });
+}
+
+int ompl_amd_rng_sample_uniform_dev(ompl_amd_rng rng, int64_t n_per_stream, int dim, const double *lo,
+                                    const double *hi, double *out_dev, float *out_f32_soa_dev, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(rng, OMPL_AMD_ERR_ARG, "null generator");
+        OA_REQUIRE(n_per_stream >= 0 && dim >= 1 && dim <= 16, OMPL_AMD_ERR_ARG, "bad sizes");
+        if (n_per_stream == 0) return;
+        OA_REQUIRE(lo && hi && out_dev, OMPL_AMD_ERR_ARG, "null pointer");
+        hipStream_t s = resolve_stream(stream);
+        const Bounds bd = stage_bounds(lo, hi, dim, s);
+        {
+            ProfScope ps("sample_uniform", s);
+            hipLaunchKernelGGL(resume_uniform_kernel, dim3(rng->n), dim3(64), 0, s, rng->gen, (const int32_t *)nullptr,
+                               rng->n, n_per_stream, dim, bd.lo, bd.hi, out_dev, out_f32_soa_dev);
+            OA_HIP(hipGetLastError());
+        }
+        OA_HIP(hipFreeAsync(bd.lo, s));
+        OA_HIP(hipStreamSynchronize(s)); // host bounds were staged with async copies
+    });
+}
+
+int ompl_amd_rng_sample_valid_dev(ompl_amd_rng rng, ompl_amd_env env, int64_t n_calls, int attempts,
+                                  const double *lo, const double *hi, double *out_dev, uint8_t *found_dev,
+                                  int32_t *draws_dev, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(rng && env, OMPL_AMD_ERR_ARG, "null generator or environment");
+        OA_REQUIRE(n_calls >= 0 && attempts >= 1, OMPL_AMD_ERR_ARG, "bad sizes");
+        if (n_calls == 0) return;
+        OA_REQUIRE(lo && hi && out_dev && found_dev, OMPL_AMD_ERR_ARG, "null pointer");
+        sample_valid_rounds(env, rng->gen, rng->n, n_calls, attempts, lo, hi, out_dev, found_dev, draws_dev,
+                            resolve_stream(stream));
+    });
+}
+
+} // extern "C"

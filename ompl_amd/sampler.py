@@ -17,7 +17,7 @@ import numpy as np
 from . import ARM_DOF, _ptr, _stream, _torch
 from ._lib import call
 
-__all__ = ["sample_uniform", "sample_valid", "MAX_VALID_SAMPLE_ATTEMPTS"]
+__all__ = ["sample_uniform", "sample_valid", "Rng", "MAX_VALID_SAMPLE_ATTEMPTS"]
 
 MAX_VALID_SAMPLE_ATTEMPTS = 100  # tools/config/MagicConstants.h:90
 
@@ -55,3 +55,47 @@ def sample_valid(env, seeds, n_calls: int, lo=None, hi=None, attempts: int = MAX
          lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), _ptr(out), _ptr(found), _ptr(draws),
          _stream(stream))
     return out, found, draws
+
+
+class Rng:
+    """Device-resident generators that persist between calls: stream s is
+    std::mt19937 seeded seeds[s], and every call continues it exactly as
+    successive calls of one reference sampler would."""
+
+    def __init__(self, seeds):
+        sd = _host(seeds, np.uint32)
+        self.n = int(sd.shape[0])
+        h = C.c_void_p()
+        call("ompl_amd_rng_create", sd.ctypes.data_as(C.c_void_p), self.n, C.byref(h))
+        self.handle = h
+
+    def uniform(self, n_per_stream: int, lo, hi, stream=None):
+        torch = _torch()
+        lo, hi = _host(lo, np.float64), _host(hi, np.float64)
+        out = torch.empty((self.n, n_per_stream, lo.shape[0]), dtype=torch.float64, device="cuda")
+        call("ompl_amd_rng_sample_uniform_dev", self.handle, int(n_per_stream), lo.shape[0],
+             lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), _ptr(out), None, _stream(stream))
+        return out
+
+    def valid(self, env, n_calls: int, lo=None, hi=None, attempts: int = MAX_VALID_SAMPLE_ATTEMPTS, stream=None):
+        torch = _torch()
+        lo = _host(np.full(ARM_DOF, -np.pi) if lo is None else lo, np.float64)
+        hi = _host(np.full(ARM_DOF, np.pi) if hi is None else hi, np.float64)
+        out = torch.empty((self.n, n_calls, ARM_DOF), dtype=torch.float64, device="cuda")
+        found = torch.empty((self.n, n_calls), dtype=torch.uint8, device="cuda")
+        draws = torch.empty((self.n, n_calls), dtype=torch.int32, device="cuda")
+        call("ompl_amd_rng_sample_valid_dev", self.handle, env.handle, int(n_calls), int(attempts),
+             lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), _ptr(out), _ptr(found), _ptr(draws),
+             _stream(stream))
+        return out, found, draws
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            call("ompl_amd_rng_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

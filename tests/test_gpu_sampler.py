@@ -63,3 +63,31 @@ def test_sample_valid_matches_oracle(gpu, oracle, n_obs, streams, calls, attempt
         assert np.array_equal(draws[i], o_draws), i
         assert np.array_equal(found[i], o_found), i
         assert np.array_equal(out[i], o_states), i
+
+
+def test_rng_handle_continues_like_one_sampler(gpu, oracle):
+    """Successive calls on persistent generators (uniform, valid, valid,
+    uniform) equal the same calls in order on one oracle generator per stream:
+    the valid sampler leaves each generator exactly after its last used draw."""
+    from ompl_amd.sampler import Rng
+
+    s, b = gpu.workload.obstacles(1002, 32, 32)
+    env = gpu.Env(s, b)
+    oenv = oracle.OracleEnv(s, b)
+    seeds = np.array([3, 99, 12345, 7], np.uint32)
+    lo, hi = np.full(7, -np.pi), np.full(7, np.pi)
+    lo3, hi3 = np.array([-1.0, -2.0, 0.5]), np.array([1.0, 2.0, 4.5])
+    rng = Rng(seeds)
+    u1 = rng.uniform(5, lo3, hi3).cpu().numpy()
+    v1 = [t.cpu().numpy() for t in rng.valid(env, 6, attempts=100)]
+    v2 = [t.cpu().numpy() for t in rng.valid(env, 3, attempts=2)]
+    u2 = rng.uniform(4, lo, hi).cpu().numpy()
+    rng.close()
+    for i, sd in enumerate(seeds.tolist()):
+        g = oracle.Mt19937(sd)
+        assert np.array_equal(u1[i], g.sample_uniform(lo3, hi3, 5)), i
+        st, fo, dr = g.sample_valid(oenv, 6, 100)
+        assert np.array_equal(v1[0][i], st) and np.array_equal(v1[1][i], fo) and np.array_equal(v1[2][i], dr), i
+        st, fo, dr = g.sample_valid(oenv, 3, 2)
+        assert np.array_equal(v2[0][i], st) and np.array_equal(v2[1][i], fo) and np.array_equal(v2[2][i], dr), i
+        assert np.array_equal(u2[i], g.sample_uniform(lo, hi, 4)), i
