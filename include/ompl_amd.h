@@ -238,6 +238,12 @@ int ompl_amd_rng_destroy(ompl_amd_rng rng);
 /* out_dev [n_streams][n_per_stream][dim]; out_f32_soa_dev (optional) [dim][n_streams*n_per_stream]. */
 int ompl_amd_rng_sample_uniform_dev(ompl_amd_rng rng, int64_t n_per_stream, int dim, const double *lo,
                                     const double *hi, double *out_dev, float *out_f32_soa_dev, void *stream);
+/* SE3StateSampler::sampleUniform (R^3 in [lo, hi] then RNG::quaternion,
+ * util/src/RandomNumbers.cpp:266-279): rows (x, y, z, qx, qy, qz, qw),
+ * out_dev FP64 [n_streams * n_per_stream][7], out_f32_dev (optional) FP32,
+ * the nearest-neighbour point layout.  sin/cos are the device's FP64 ones. */
+int ompl_amd_rng_sample_se3_dev(ompl_amd_rng rng, int64_t n_per_stream, const double *lo, const double *hi,
+                                double *out_dev, float *out_f32_dev, void *stream);
 int ompl_amd_rng_sample_valid_dev(ompl_amd_rng rng, ompl_amd_env env, int64_t n_calls, int attempts,
                                   const double *lo, const double *hi, double *out_dev, uint8_t *found_dev,
                                   int32_t *draws_dev, void *stream);

@@ -63,6 +63,7 @@ SIGNATURES = {
     "ompl_amd_rng_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
     "ompl_amd_rng_destroy": (C.c_int, [vp]),
     "ompl_amd_rng_sample_uniform_dev": (C.c_int, [vp, C.c_int64, C.c_int, vp, vp, vp, vp, vp]),
+    "ompl_amd_rng_sample_se3_dev": (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp]),
     "ompl_amd_rng_sample_valid_dev": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp, vp, vp, vp, vp]),
     "ompl_amd_nn_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
     "ompl_amd_nn_destroy": (C.c_int, [vp]),

@@ -300,6 +300,38 @@ __global__ void next_active_kernel(const ValidCursor *__restrict__ cur, int n_st
     if (open) active[base + __popcll(m & (((uint64_t)1 << lane) - 1))] = s;
 }
 
+// SE3StateSampler::sampleUniform (a compound sampler: the R^3 component's
+// RealVectorStateSampler, then SO3StateSampler::sampleUniform = RNG::quaternion,
+// util/src/RandomNumbers.cpp:266-279, Shoemake) from six canonical draws per
+// state u[0..5]: x_k = (hi_k - lo_k) u_k + lo_k; x0 = u3, t1 = 2 pi u4,
+// t2 = 2 pi u5, q = (sin t1 sqrt(1-x0), cos t1 sqrt(1-x0), sin t2 sqrt(x0),
+// cos t2 sqrt(x0)) in FP64 (the device's sin/cos are within an ulp or two of
+// the host's).  Output rows (x, y, z, qx, qy, qz, qw) as FP64 and FP32.
+__global__ void se3_from_uniform_kernel(const double *__restrict__ u, int64_t n, const double *__restrict__ lo,
+                                        const double *__restrict__ hi, double *__restrict__ out,
+                                        float *__restrict__ out_f32)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *v = u + i * 6;
+    double r[7];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = (hi[k] - lo[k]) * v[k] + lo[k];
+    const double pi = 3.141592653589793238462643383279502884;
+    const double x0 = v[3];
+    const double r1 = sqrt(1.0 - x0), r2 = sqrt(x0);
+    const double t1 = 2.0 * pi * v[4], t2 = 2.0 * pi * v[5];
+    r[3] = sin(t1) * r1;
+    r[4] = cos(t1) * r1;
+    r[5] = sin(t2) * r2;
+    r[6] = cos(t2) * r2;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        out[i * 7 + k] = r[k];
+        if (out_f32) out_f32[i * 7 + k] = (float)r[k];
+    }
+}
+
 // Device bounds of a sampling call (host arrays staged per call).
 struct Bounds {
     double *lo, *hi;
@@ -525,6 +557,36 @@ int ompl_amd_rng_sample_uniform_dev(ompl_amd_rng rng, int64_t n_per_stream, int 
             OA_HIP(hipGetLastError());
         }
         OA_HIP(hipFreeAsync(bd.lo, s));
+        OA_HIP(hipStreamSynchronize(s)); // host bounds were staged with async copies
+    });
+}
+
+int ompl_amd_rng_sample_se3_dev(ompl_amd_rng rng, int64_t n_per_stream, const double *lo, const double *hi,
+                                double *out_dev, float *out_f32_dev, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(rng, OMPL_AMD_ERR_ARG, "null generator");
+        OA_REQUIRE(n_per_stream >= 0, OMPL_AMD_ERR_ARG, "bad sizes");
+        if (n_per_stream == 0) return;
+        OA_REQUIRE(lo && hi && out_dev, OMPL_AMD_ERR_ARG, "null pointer");
+        hipStream_t s = resolve_stream(stream);
+        const double zero[6] = {0, 0, 0, 0, 0, 0}, one[6] = {1, 1, 1, 1, 1, 1};
+        const Bounds unit = stage_bounds(zero, one, 6, s);
+        const Bounds box = stage_bounds(lo, hi, 3, s);
+        const int64_t n = (int64_t)rng->n * n_per_stream;
+        double *u = nullptr;
+        OA_HIP(hipMallocAsync((void **)&u, (size_t)n * 6 * sizeof(double), s));
+        {
+            ProfScope ps("sample_se3", s);
+            hipLaunchKernelGGL(resume_uniform_kernel, dim3(rng->n), dim3(64), 0, s, rng->gen, (const int32_t *)nullptr,
+                               rng->n, n_per_stream, 6, unit.lo, unit.hi, u, (float *)nullptr);
+            hipLaunchKernelGGL(se3_from_uniform_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, u, n,
+                               box.lo, box.hi, out_dev, out_f32_dev);
+            OA_HIP(hipGetLastError());
+        }
+        OA_HIP(hipFreeAsync(u, s));
+        OA_HIP(hipFreeAsync(unit.lo, s));
+        OA_HIP(hipFreeAsync(box.lo, s));
         OA_HIP(hipStreamSynchronize(s)); // host bounds were staged with async copies
     });
 }

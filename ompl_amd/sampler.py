@@ -77,6 +77,17 @@ class Rng:
              lo.ctypes.data_as(C.c_void_p), hi.ctypes.data_as(C.c_void_p), _ptr(out), None, _stream(stream))
         return out
 
+    def se3(self, n_per_stream: int, lo, hi, stream=None):
+        """SE3StateSampler::sampleUniform: device float64 and float32 [n_streams * n_per_stream, 7]
+        rows (x, y, z, qx, qy, qz, qw)."""
+        torch = _torch()
+        lo, hi = _host(lo, np.float64), _host(hi, np.float64)
+        out = torch.empty((self.n * n_per_stream, 7), dtype=torch.float64, device="cuda")
+        out32 = torch.empty((self.n * n_per_stream, 7), dtype=torch.float32, device="cuda")
+        call("ompl_amd_rng_sample_se3_dev", self.handle, int(n_per_stream), lo.ctypes.data_as(C.c_void_p),
+             hi.ctypes.data_as(C.c_void_p), _ptr(out), _ptr(out32), _stream(stream))
+        return out, out32
+
     def valid(self, env, n_calls: int, lo=None, hi=None, attempts: int = MAX_VALID_SAMPLE_ATTEMPTS, stream=None):
         torch = _torch()
         lo = _host(np.full(ARM_DOF, -np.pi) if lo is None else lo, np.float64)

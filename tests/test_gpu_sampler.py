@@ -91,3 +91,32 @@ def test_rng_handle_continues_like_one_sampler(gpu, oracle):
         st, fo, dr = g.sample_valid(oenv, 3, 2)
         assert np.array_equal(v2[0][i], st) and np.array_equal(v2[1][i], fo) and np.array_equal(v2[2][i], dr), i
         assert np.array_equal(u2[i], g.sample_uniform(lo, hi, 4)), i
+
+
+def test_se3_sampler_matches_oracle(gpu, oracle):
+    """SE3StateSampler::sampleUniform on persistent generators: translations
+    bit-exact, quaternions within 4 ulp of the CPU's libm sin/cos (the device
+    evaluates its own FP64 sin/cos), unit norm; FP32 rows match the host
+    workload generator within one FP32 ulp."""
+    from ompl_amd.sampler import Rng
+
+    seeds = np.array([5, 17], np.uint32)
+    lo, hi = np.full(3, -10.0), np.full(3, 10.0)
+    rng = Rng(seeds)
+    out, out32 = rng.se3(3000, lo, hi)
+    out, out32 = out.cpu().numpy(), out32.cpu().numpy()
+    rng.close()
+    for i, sd in enumerate(seeds.tolist()):
+        g = oracle.Mt19937(sd)
+        ref = np.zeros((3000, 7))
+        for j in range(3000):
+            ref[j, :3] = g.sample_uniform(lo, hi, 1)[0]
+            ref[j, 3:] = g.quaternion()
+        got = out[i * 3000:(i + 1) * 3000]
+        assert np.array_equal(got[:, :3], ref[:, :3])
+        ulp = np.spacing(np.abs(ref[:, 3:])) + 1e-300
+        assert (np.abs(got[:, 3:] - ref[:, 3:]) <= 4 * ulp).all()
+        assert np.allclose((got[:, 3:] ** 2).sum(1), 1.0, atol=1e-14)
+        host = gpu.workload.se3(sd, 3000, -10.0, 10.0)
+        f32 = out32[i * 3000:(i + 1) * 3000]
+        assert (np.abs(f32 - host) <= np.spacing(np.abs(host))).all()
