@@ -50,6 +50,17 @@ __device__ __forceinline__ int env_words(const EnvView &e)
     return e.ns + 2 * e.nb;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  This permutation gives XCD x a contiguous run of logical blocks
+// (hardware block b = 8 i + x -> the i-th block of XCD x's run; the first
+// nb % 8 XCDs get one block more), so that neighbouring states, which share
+// cache lines of the dof-major arrays, are fetched into one L2 only.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb)
+{
+    const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // Cooperative copy of the environment into LDS by the whole block.
 __device__ __forceinline__ void stage_env(const EnvView &e, float4 *lds)
 {
@@ -68,9 +79,33 @@ struct LaneSpheres {
     float x[N], y[N], z[N];
 };
 
+// With LPS = 8 the eight lanes of an aligned lane octet always hold the same
+// state (every layout puts a state's sphere slots on lanes 8k .. 8k+7), so
+// lane 8k + j evaluates sincos of joint j once and the octet shares the seven
+// results by shuffles instead of each lane evaluating all seven; the values
+// are the same bits either way.
 template <int LPS>
 __device__ __forceinline__ void arm_fk(const float q[kDof], int g, LaneSpheres<LPS> &S)
 {
+    float sj[kDof], cj[kDof];
+    if constexpr (LPS == 8) {
+        const int lane = threadIdx.x & 63;
+        const int j = lane & 7;
+        float qj = q[0];
+#pragma unroll
+        for (int i = 1; i < kDof; ++i) qj = (j == i) ? q[i] : qj;
+        float sm, cm;
+        sincos_pl(qj, sm, cm);
+        const int base = lane & ~7;
+#pragma unroll
+        for (int i = 0; i < kDof; ++i) {
+            sj[i] = __shfl(sm, base + i);
+            cj[i] = __shfl(cm, base + i);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kDof; ++i) sincos_pl(q[i], sj[i], cj[i]);
+    }
     float X0 = 1.0f, X1 = 0.0f, X2 = 0.0f;
     float Y0 = 0.0f, Y1 = 1.0f, Y2 = 0.0f;
     float Z0 = 0.0f, Z1 = 0.0f, Z2 = 1.0f;
@@ -83,8 +118,7 @@ __device__ __forceinline__ void arm_fk(const float q[kDof], int g, LaneSpheres<L
     }
 #pragma unroll
     for (int i = 0; i < kDof; ++i) {
-        float s, c;
-        sincos_pl(q[i], s, c);
+        const float s = sj[i], c = cj[i];
         if ((i & 1) == 0) {
             float nx0 = X0 * c + Y0 * s, ny0 = Y0 * c - X0 * s;
             float nx1 = X1 * c + Y1 * s, ny1 = Y1 * c - X1 * s;

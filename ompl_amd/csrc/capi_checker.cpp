@@ -48,6 +48,7 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         A.max_m = w.counters + 9;
         A.first_bad = w.first_bad;
         A.surv_nd = w.surv_nd;
+        A.surv_ab = w.surv_ab;
         A.start = 1;
         launch_motion_endpoint(A, LE, OE, t.validity_block, s, shared);
         launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s, shared);
@@ -117,6 +118,7 @@ struct ompl_amd_motion_graph_s {
         if (ws.list) hipFree(ws.list);
         if (ws.first_bad) hipFree(ws.first_bad);
         if (ws.surv_nd) hipFree(ws.surv_nd);
+        if (ws.surv_ab) hipFree(ws.surv_ab);
         if (ws.counters) hipFree(ws.counters);
     }
 };
@@ -403,6 +405,7 @@ int ompl_amd_motion_graph_create(ompl_amd_env e, const double *a, const double *
             OA_HIP(hipMalloc(&g->ws.list, cap * sizeof(int32_t)));
             OA_HIP(hipMalloc(&g->ws.first_bad, cap * sizeof(int32_t)));
             OA_HIP(hipMalloc(&g->ws.surv_nd, cap * sizeof(int32_t)));
+            OA_HIP(hipMalloc(&g->ws.surv_ab, cap * 14 * sizeof(double)));
             g->ws.list_cap = cap;
             OA_HIP(hipDeviceSynchronize());
             // capture on the calling thread's own stream (the null stream cannot capture)

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
     using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
     const Shape sh(lane);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wave = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t st = wave * Shape::STATES + sh.group;
     const bool live = st < n;
     float q[kDof];
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(1024) validate_shared_kernel(EnvView env, cons
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
     const Shape sh(lane);
-    const int64_t s0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * FK;
+    const int64_t s0 = ((xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6) * FK;
     const int64_t sf = s0 + lane / LPS;
     float q[kDof];
 #pragma unroll
@@ -109,7 +109,8 @@ __global__ void __launch_bounds__(1024) validate_shared_kernel(EnvView env, cons
 // leader: writes the final result of every edge that needs no walk and
 // returns true for a survivor (s2 valid, 2 <= nd <= kMaxSegments), m = nd - 1.
 template <int GS>
-__device__ __forceinline__ bool endpoint_finish(const MotionArgs &A, int64_t e, bool live, uint64_t H, int &m)
+__device__ __forceinline__ bool endpoint_finish(const MotionArgs &A, int64_t e, bool live, uint64_t H, int &m,
+                                                double a[kDof], double b[kDof])
 {
     const int lane = lane_id();
     m = 0;
@@ -120,7 +121,6 @@ __device__ __forceinline__ bool endpoint_finish(const MotionArgs &A, int64_t e, 
         if (A.checked) A.checked[e] = 1;
         return false;
     }
-    double a[kDof], b[kDof];
 #pragma unroll
     for (int j = 0; j < kDof; ++j) {
         a[j] = A.a[(int64_t)j * A.n + e];
@@ -144,7 +144,8 @@ __device__ __forceinline__ bool endpoint_finish(const MotionArgs &A, int64_t e, 
 // s2 check of one edge per lane group; returns true for the group leader of a
 // survivor and writes the final result of every other edge.
 template <int LPS, int OPS>
-__device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4 *lds, int64_t e, bool live, int &m)
+__device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4 *lds, int64_t e, bool live, int &m,
+                                               double a[kDof], double b[kDof])
 {
     using Shape = GroupShape<LPS, OPS>;
     const Shape sh(lane_id());
@@ -155,11 +156,13 @@ __device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4
     arm_fk<LPS>(q, sh.slot, S);
     const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
     const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
-    return endpoint_finish<Shape::GS>(A, e, live, H, m);
+    return endpoint_finish<Shape::GS>(A, e, live, H, m, a, b);
 }
 
-// Wave-aggregated append of the survivors to the list.
-__device__ __forceinline__ void append_survivor(const MotionArgs &A, int64_t e, bool survivor, int m)
+// Wave-aggregated append of the survivors to the list (with their segment
+// count and, when kept, their endpoints contiguous for the chunk walk).
+__device__ __forceinline__ void append_survivor(const MotionArgs &A, int64_t e, bool survivor, int m,
+                                                const double a[kDof], const double b[kDof])
 {
     const int lane = lane_id();
     const uint64_t sm = wave_ballot(survivor);
@@ -172,6 +175,14 @@ __device__ __forceinline__ void append_survivor(const MotionArgs &A, int64_t e, 
         A.list[slot] = (int32_t)e;
         if (A.first_bad) A.first_bad[slot] = INT_MAX;
         if (A.surv_nd) A.surv_nd[slot] = m + 1;
+        if (A.surv_ab) {
+            double *ab = A.surv_ab + (int64_t)slot * 2 * kDof;
+#pragma unroll
+            for (int j = 0; j < kDof; ++j) {
+                ab[j] = a[j];
+                ab[kDof + j] = b[j];
+            }
+        }
         if (A.max_m) atomicMax(A.max_m, m);
     }
 }
@@ -182,11 +193,12 @@ __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
     extern __shared__ float4 lds[];
     stage_env(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wave = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t e = wave * Shape::STATES + lane_id() / Shape::GS;
     int m = 0;
-    const bool survivor = endpoint_check<LPS, OPS>(A, lds, e, e < A.n, m);
-    append_survivor(A, e, survivor, m);
+    double a[kDof], b[kDof];
+    const bool survivor = endpoint_check<LPS, OPS>(A, lds, e, e < A.n, m, a, b);
+    append_survivor(A, e, survivor, m, a, b);
 }
 
 // Shared-FK endpoint pass: a wave takes 64/LPS edges, computes the FK of their
@@ -200,7 +212,7 @@ __global__ void __launch_bounds__(1024) motion_endpoint_shared_kernel(MotionArgs
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
     const Shape sh(lane);
-    const int64_t e0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * FK;
+    const int64_t e0 = ((xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6) * FK;
     const int64_t ef = e0 + lane / LPS;
     float q[kDof];
 #pragma unroll
@@ -216,8 +228,9 @@ __global__ void __launch_bounds__(1024) motion_endpoint_shared_kernel(MotionArgs
         const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
         const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
         int m = 0;
-        const bool survivor = endpoint_finish<Shape::GS>(A, e, live, H, m);
-        append_survivor(A, e, survivor, m);
+        double a[kDof], b[kDof];
+        const bool survivor = endpoint_finish<Shape::GS>(A, e, live, H, m, a, b);
+        append_survivor(A, e, survivor, m, a, b);
     }
 }
 
@@ -293,12 +306,12 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
             mask &= mask - 1;
             const int sj = __shfl(s, j), bj = __shfl(base, j), ndj = __shfl(nd, j);
             if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) continue;
-            const int64_t e = A.list[sj];
+            const double *ab = A.surv_ab + (int64_t)sj * 2 * kDof;
             double a[kDof], b[kDof];
 #pragma unroll
             for (int d = 0; d < kDof; ++d) {
-                a[d] = A.a[(int64_t)d * A.n + e];
-                b[d] = A.b[(int64_t)d * A.n + e];
+                a[d] = ab[d];
+                b[d] = ab[kDof + d];
             }
             const int bad = SHARED ? walk_chunk_shared<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb)
                                    : walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
@@ -337,7 +350,8 @@ __global__ void __launch_bounds__(1024) motion_fused_kernel(MotionArgs A)
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t e = wave * Shape::STATES + lane / Shape::GS;
     int m = 0;
-    const bool survivor = endpoint_check<LE, OE>(A, lds, e, e < A.n, m);
+    double a_unused[kDof], b_unused[kDof];
+    const bool survivor = endpoint_check<LE, OE>(A, lds, e, e < A.n, m, a_unused, b_unused);
     uint64_t sm = wave_ballot(survivor);
     while (sm) {
         const int src = lowest_bit(sm);

@@ -25,6 +25,7 @@ struct MotionArgs {
     int32_t *first_bad;  // per survivor: first invalid position (chunk-parallel walk)
     int32_t *max_m;      // largest bisection position count among survivors
     int32_t *surv_nd;    // per survivor: segment count nd
+    double *surv_ab;     // per survivor: endpoints a then b (14 doubles, contiguous), optional
     int start;           // first walk position (0 = s2, 1 = skip s2)
 };
 

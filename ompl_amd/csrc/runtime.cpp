@@ -116,10 +116,12 @@ Workspace &workspace(hipStream_t s, size_t list_entries)
         if (w->list) OA_HIP(hipFree(w->list));
         if (w->first_bad) OA_HIP(hipFree(w->first_bad));
         if (w->surv_nd) OA_HIP(hipFree(w->surv_nd));
+        if (w->surv_ab) OA_HIP(hipFree(w->surv_ab));
         size_t cap = list_entries < 1024 ? 1024 : list_entries;
         OA_HIP(hipMalloc(&w->list, cap * sizeof(int32_t)));
         OA_HIP(hipMalloc(&w->first_bad, cap * sizeof(int32_t)));
         OA_HIP(hipMalloc(&w->surv_nd, cap * sizeof(int32_t)));
+        OA_HIP(hipMalloc(&w->surv_ab, cap * 14 * sizeof(double)));
         w->list_cap = cap;
     }
     return *w;

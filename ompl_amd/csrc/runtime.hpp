@@ -18,6 +18,7 @@ struct Workspace {
     int32_t *list = nullptr;
     int32_t *first_bad = nullptr; // per survivor (chunk-parallel walk)
     int32_t *surv_nd = nullptr;   // per survivor segment count (chunk-parallel walk)
+    double *surv_ab = nullptr;    // per survivor endpoints a, b (14 doubles)
     size_t list_cap = 0;
     // [0] survivor count, [1] walk queue (strategy 1, cleared per call);
     // [8] survivor count, [9] max positions, [10] finished waves (strategy 3,

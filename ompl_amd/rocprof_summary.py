@@ -1,0 +1,81 @@
+"""Text summaries of rocprofv3 csv output, for the profile records.
+
+    python -m ompl_amd.rocprof_summary stats DIR
+        per-kernel calls and average / min / max duration (kernel_stats.csv)
+    python -m ompl_amd.rocprof_summary traffic BENCH_FETCH BENCH_WRITE COPY_FETCH COPY_WRITE COPY_BYTES
+        per-kernel HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE runs, each
+        scaled by what the counter reads for a plain device copy of COPY_BYTES
+"""
+from __future__ import annotations
+
+import csv
+import glob
+import json
+import sys
+from collections import defaultdict
+
+
+def _find(d: str, suffix: str) -> str:
+    f = sorted(glob.glob(f"{d}/**/*{suffix}", recursive=True))
+    if not f:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    return f[0]
+
+
+def kernel_stats(d: str) -> list[dict]:
+    rows = list(csv.DictReader(open(_find(d, "kernel_stats.csv"))))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    return [{"kernel": r["Name"], "calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3,
+             "min_us": float(r["MinNs"]) / 1e3, "max_us": float(r["MaxNs"]) / 1e3,
+             "pct": float(r["Percentage"])} for r in rows]
+
+
+def counter_per_dispatch(d: str, counter: str) -> dict[str, float]:
+    acc = defaultdict(list)
+    for r in csv.DictReader(open(_find(d, "counter_collection.csv"))):
+        if r["Counter_Name"] == counter:
+            acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def traffic(bench_fetch: str, bench_write: str, copy_fetch: str, copy_write: str, copy_bytes: int) -> dict:
+    cf = counter_per_dispatch(copy_fetch, "FETCH_SIZE")
+    cw = counter_per_dispatch(copy_write, "WRITE_SIZE")
+    ck = [k for k in cf if "copyBuffer" in k or "copy" in k.lower()]
+    if not ck:
+        raise ValueError("no copy kernel in the calibration runs")
+    k = ck[0]
+    f_scale = copy_bytes / (cf[k] * 1024.0)  # bytes per counted KiB
+    w_scale = copy_bytes / (cw[k] * 1024.0)
+    bf = counter_per_dispatch(bench_fetch, "FETCH_SIZE")
+    bw = counter_per_dispatch(bench_write, "WRITE_SIZE")
+    out = {"copy_kernel": k, "copy_bytes": copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    for name in sorted(set(bf) | set(bw), key=lambda n: -(bf.get(n, 0) + bw.get(n, 0))):
+        rd = bf.get(name, 0.0) * 1024.0 * f_scale
+        wr = bw.get(name, 0.0) * 1024.0 * w_scale
+        out["kernels"][name] = {"read_bytes": rd, "write_bytes": wr, "bytes": rd + wr}
+    return out
+
+
+def main(argv: list[str]) -> int:
+    if argv[0] == "stats":
+        print(f"{'kernel':78s} {'calls':>6s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'%time':>6s}")
+        for r in kernel_stats(argv[1]):
+            print(f"{r['kernel'][:78]:78s} {r['calls']:6d} {r['avg_us']:9.2f} {r['min_us']:9.2f} {r['max_us']:9.2f} "
+                  f"{r['pct']:6.2f}")
+    elif argv[0] == "traffic":
+        t = traffic(argv[1], argv[2], argv[3], argv[4], int(argv[5]))
+        print(f"calibration: {t['copy_kernel']} of {t['copy_bytes']} B: FETCH_SIZE x {t['fetch_scale']:.3f}, "
+              f"WRITE_SIZE x {t['write_scale']:.3f} (bytes per counted KiB / 1024)")
+        print(f"{'kernel':78s} {'read_B':>12s} {'write_B':>12s} {'total_B':>12s}")
+        for k, v in t["kernels"].items():
+            print(f"{k[:78]:78s} {v['read_bytes']:12.0f} {v['write_bytes']:12.0f} {v['bytes']:12.0f}")
+        print("json: " + json.dumps(t))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
