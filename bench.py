@@ -193,6 +193,36 @@ def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
     return out
 
 
+TRAFFIC_RECORD = ROOT / "profiles" / "motion_batch_16384_edges_256_obstacles.hbm_traffic.json"
+
+
+def hbm_traffic(kernel: str, tuning: dict):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE record, when it was measured with this launch
+    shape; otherwise None."""
+    try:
+        rec = json.loads(TRAFFIC_RECORD.read_text())
+    except (OSError, ValueError):
+        return None, None
+    keys = ("validity_lanes_per_state", "validity_obstacle_parts", "motion_lanes_per_state",
+            "motion_obstacle_parts", "motion_strategy")
+    if any(rec["tuning"].get(k) != tuning.get(k) for k in keys):
+        return None, f"{TRAFFIC_RECORD.name}: recorded for another launch shape"
+    shared = "true" if tuning["motion_strategy"] == 4 else "false"
+    if kernel == "motion_chunk":
+        name = (f"motion_chunk_kernel<{tuning['motion_lanes_per_state']}, {tuning['motion_obstacle_parts']}, false, "
+                f"{shared}>")
+    elif kernel == "motion_endpoint":
+        kind = "motion_endpoint_shared_kernel" if shared == "true" else "motion_endpoint_kernel"
+        name = f"{kind}<{tuning['validity_lanes_per_state']}, {tuning['validity_obstacle_parts']}>"
+    else:
+        return None, None
+    for k, v in rec["bytes_per_dispatch"].items():
+        if name + "(" in k:
+            return int(v), f"{TRAFFIC_RECORD.name} ({k})"
+    return None, None
+
+
 def timed(fn, reps, warm=2):
     """Average milliseconds of fn() over reps launches (HIP events on torch's
     current stream, which is the stream the library launches on)."""
@@ -473,6 +503,7 @@ def main(argv=None):
     dom_flops = share[dom]
     dom_us = kernels[dom]["avg_us"]
     achieved = dom_flops / (dom_us * 1e-6) / 1e12
+    traffic, traffic_src = hbm_traffic(dom, t)
 
     # ---- plain one-launch-per-call path on a prefix of the same edges
     P = min(args.plain_edges, N_EDGES)
@@ -553,7 +584,8 @@ def main(argv=None):
         "batched_over_plain": value / plain_rate,
         "work_per_step": work,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": dom, "kernel_avg_us": dom_us, "algorithmic_flops_per_launch": dom_flops,
                      "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
         "cpu_baseline": cpu,
