@@ -164,12 +164,14 @@ typedef struct {
                                      3: s2 pass, then the survivors' bisection
                                         chunks run in parallel across the GPU,
                                         chunk-major, with early exit;
-                                     4: as 3, with each state's FK computed
-                                        once per wave and shared across the
-                                        obstacle partitions (shapes with
-                                        parts > 1; chunks of 64/lanes states) */
-    int validity_shared_fk;       /* 1: batched validity computes each
-                                     state's FK once per wave and shares it
+                                     4: as 3, the walk with each position's
+                                        FK computed once per wave and shared
+                                        across the obstacle partitions (shapes
+                                        with parts > 1; chunks of 64/lanes
+                                        positions) */
+    int validity_shared_fk;       /* 1: batched validity and the s2 pass of
+                                     the motion strategies compute each
+                                     state's FK once per wave and share it
                                      across the obstacle partitions */
 } ompl_amd_tuning;
 int ompl_amd_get_tuning(ompl_amd_tuning *t);

@@ -39,9 +39,11 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
     if (t.motion_strategy == 3 || t.motion_strategy == 4) {
         // counters [8..9] belong to these strategies only: zero when the
         // workspace is created, cleared by the finalize kernel of every batch.
-        // Strategy 4 computes each state's FK once per wave and shares it
-        // across the obstacle partitions.
+        // Strategy 4 walks chunks with each position's FK computed once per
+        // wave and shared across the obstacle partitions; the s2 pass shares
+        // its FK when validity_shared_fk is set.
         const bool shared = t.motion_strategy == 4;
+        const bool eshared = t.validity_shared_fk != 0;
         Workspace &w = own ? *own : workspace(s, (size_t)n);
         A.list = w.list;
         A.list_count = w.counters + 8;
@@ -50,7 +52,7 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         A.surv_nd = w.surv_nd;
         A.surv_ab = w.surv_ab;
         A.start = 1;
-        launch_motion_endpoint(A, LE, OE, t.validity_block, s, shared);
+        launch_motion_endpoint(A, LE, OE, t.validity_block, s, eshared);
         launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s, shared);
         launch_motion_finalize(A, s);
     } else if (t.motion_strategy == 2) {
@@ -63,7 +65,7 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         A.queue = w.counters + 1;
         A.start = 1;
         launch_zero(w.counters, 4, s);
-        launch_motion_endpoint(A, LE, OE, t.validity_block, s);
+        launch_motion_endpoint(A, LE, OE, t.validity_block, s, t.validity_shared_fk != 0);
         launch_motion_walk(A, exact, LW, OW, t.motion_block, persistent_waves, s);
     } else {
         A.start = 0;
@@ -78,26 +80,27 @@ void check_motions(const Env &env, const double *a, const double *b, int64_t n, 
 }
 
 // Candidate launch shapes of the motion autotuner.
-std::vector<Tuning> motion_candidates(const Tuning &base)
+std::vector<Tuning> motion_candidates()
 {
     std::vector<Tuning> c;
     const int shapes[][2] = {{8, 1}, {8, 2}, {8, 4}, {8, 8}, {4, 1}, {1, 1}};
     for (const auto &e : shapes)
         for (const auto &w : shapes) {
             const int LE = e[0], OE = e[1], LW = w[0], OW = w[1];
-            // chunk-parallel walk of the survivors (and its shared-FK form)
-            for (int wpc : {8, 16, 32}) {
-                c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 3});
-                if (OE > 1 || OW > 1) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 4});
-            }
+            // chunk-parallel walk of the survivors; shared-FK forms of the s2
+            // pass (validity_shared_fk) and of the walk (strategy 4)
+            for (int wpc : {8, 16, 32})
+                for (int es = 0; es <= (OE > 1 ? 1 : 0); ++es) {
+                    c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 3, es});
+                    if (OW > 1) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 4, es});
+                }
             // fused one-launch form
             if (fused_supported(LE, OE, LW, OW))
                 for (int block : {64, 256}) c.push_back(Tuning{LE, OE, block, LW, OW, block, 8, 2});
             // persistent queue, one wave per survivor
-            c.push_back(Tuning{LE, OE, 256, LW, OW, 256, 16, 1});
+            for (int es = 0; es <= (OE > 1 ? 1 : 0); ++es) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, 16, 1, es});
         }
     for (const auto &w : shapes) c.push_back(Tuning{8, 1, 256, w[0], w[1], 256, 8, 0});
-    for (Tuning &x : c) x.validity_shared_fk = base.validity_shared_fk; // not a motion knob
     return c;
 }
 
@@ -364,7 +367,7 @@ extern "C" int ompl_amd_autotune_motions(ompl_amd_env e, const double *a, const 
         const Tuning base = get_tuning();
         Tuning win = base;
         float win_ms = INFINITY;
-        for (const Tuning &t : motion_candidates(base)) {
+        for (const Tuning &t : motion_candidates()) {
             for (int w = 0; w < 2; ++w) check_motions_t(t, env, a, b, n, lvs, factor, out, nullptr, s);
             OA_HIP(hipEventRecord(e0, s));
             for (int i = 0; i < reps; ++i) check_motions_t(t, env, a, b, n, lvs, factor, out, nullptr, s);

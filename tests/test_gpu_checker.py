@@ -220,3 +220,19 @@ def test_motion_graph_replays_match_eager(gpu, oracle, strategy, eshape, wshape)
         assert np.array_equal(v2.cpu().numpy(), ref2), rep
     g.close()
     gpu.set_tuning(**DEFAULT)
+
+
+@pytest.mark.parametrize("strategy,eshape,wshape", [(3, (8, 8), (8, 4)), (4, (8, 2), (8, 8)), (1, (8, 4), (1, 1)),
+                                                    (3, (8, 4), (1, 1))])
+def test_check_motions_shared_s2_pass(gpu, oracle, strategy, eshape, wshape):
+    """The s2 pass with shared FK (validity_shared_fk) gives the same booleans and counts."""
+    set_strategy(gpu, strategy, eshape, wshape)
+    gpu.set_tuning(validity_shared_fk=1)
+    env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
+    a, b = motion_case(gpu, 4, 16384)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs)
+    v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, exact_count=True)
+    assert np.array_equal(v.cpu().numpy(), ref_v)
+    assert np.array_equal(c.cpu().numpy(), ref_c)
+    gpu.set_tuning(**DEFAULT, validity_shared_fk=0)
