@@ -455,6 +455,29 @@ def main(argv=None):
         torch.cuda.synchronize()
         step_ms.append((time.perf_counter() - s0) * 1e3)
 
+    # ---- a few fixed launch shapes beside the tuned one (per-kernel times)
+    sweep = {}
+    for name, kw in (("s3_e8x8_w8x8", dict(motion_strategy=3, validity_obstacle_parts=8, motion_obstacle_parts=8)),
+                     ("s3_e8x8sh_w8x4", dict(motion_strategy=3, validity_obstacle_parts=8, motion_obstacle_parts=4,
+                                             validity_shared_fk=1)),
+                     ("s4_e8x8_w8x8", dict(motion_strategy=4, validity_obstacle_parts=8, motion_obstacle_parts=8)),
+                     ("s4_e8x8sh_w8x2", dict(motion_strategy=4, validity_obstacle_parts=8, motion_obstacle_parts=2,
+                                             validity_shared_fk=1))):
+        oa.set_tuning(**{**t, "validity_lanes_per_state": 8, "motion_lanes_per_state": 8, "validity_shared_fk": 0,
+                         **kw})
+        oa.profile.reset()
+        oa.profile.enable(True)
+        ms = timed(lambda: oa.check_motions(env, A, B, lvs, out=out), 30)
+        oa.profile.enable(False)
+        per = {}
+        for kname in ("motion_endpoint", "motion_chunk"):
+            n, kms = oa.profile.query(kname)
+            if n:
+                per[kname] = 1e3 * kms / n
+        oa.profile.reset()
+        sweep[name] = {"ms_per_step": ms, "kernel_us": per}
+    oa.set_tuning(**t)
+
     # ---- the same batch replayed from a HIP graph (launch cost removed)
     g = oa.MotionGraph(env, A, B, lvs)
     for _ in range(5):
@@ -576,6 +599,7 @@ def main(argv=None):
         "kernels": kernels,
         "valid_fraction": float(valid.mean()),
         "step_latency_ms": {"p50": percentile(step_ms, 50), "p99": percentile(step_ms, 99)},
+        "shape_sweep": sweep,
         "graph": {"ms_per_step": graph_ms, "edge_checks_per_s": N_EDGES / (graph_ms * 1e-3),
                   "step_latency_ms": {"p50": percentile(graph_lat, 50), "p99": percentile(graph_lat, 99)},
                   "same_result": graph_same},

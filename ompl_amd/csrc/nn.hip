@@ -220,11 +220,23 @@ __device__ __forceinline__ f32x16 tile_mma(const float *__restrict__ xt, int64_t
     return acc;
 }
 
+// Orders the wave's LDS accesses: its lanes exchange data through the wave's
+// own LDS slice, so only the wave itself needs to be ordered.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int METRIC, int DIM, int KSTEPS, int KMAX>
 __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
 {
+    // the current tile's 32 points, per wave: candidates that pass the MFMA
+    // filter read their exact coordinates from here, not from HBM
+    __shared__ float tile_pts[4][32 * DIM];
     const int lane = threadIdx.x & 63;
     const int h = lane >> 5;
+    float *tp = tile_pts[(threadIdx.x >> 6) & 3];
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t qtiles = (A.nq + 31) / 32;
     const int64_t qtile = wave % qtiles;
@@ -235,23 +247,29 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
     TopK<KMAX> top;
     top.init();
     float thr = INFINITY;
+    float kd = INFINITY; // current k-th key of this lane's list
+    int ki = INT_MAX;
     const int64_t t0 = (int64_t)split * A.tiles_per_split;
     const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
+    const int64_t n_words = A.n * DIM;
     for (int64_t t = t0; t < t1; ++t) {
         const f32x16 acc = tile_mma<KSTEPS>(A.xt, t, lane, Q.b);
         uint32_t mask = 0;
 #pragma unroll
         for (int j = 0; j < 16; ++j) mask |= (acc[j] <= thr ? 1u : 0u) << j;
         if (!Q.live) mask = 0;
+        if (__ballot(mask != 0) == 0) continue; // wave-uniform
+        wave_lds_sync(); // every lane is done with the previous tile's points
+        const int64_t w0 = t * 32 * DIM;
+        for (int i = lane; i < 32 * DIM; i += 64) tp[i] = w0 + i < n_words ? A.pts[w0 + i] : 0.0f;
+        wave_lds_sync();
         while (mask) {
             const int j = __builtin_ctz(mask);
             mask &= mask - 1;
-            const int64_t p = t * 32 + mfma_row(j, h);
+            const int row = mfma_row(j, h);
+            const int64_t p = t * 32 + row;
             if (p >= A.n || A.removed[p]) continue;
-            const float d = exact_dist<METRIC, DIM>(A.pts + p * DIM, Q.q);
-            float kd;
-            int ki;
-            top.kth(A.k, kd, ki);
+            const float d = exact_dist<METRIC, DIM>(tp + row * DIM, Q.q);
             if (key_less(d, (int)p, kd, ki)) {
                 top.insert(d, (int)p);
                 top.kth(A.k, kd, ki);
