@@ -44,8 +44,24 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 302
+#define OMPL_AMD_VERSION 303
 int ompl_amd_version(void);
+
+/* Limits of this build and of the device it runs on. */
+typedef struct {
+    int arm_dof;              /* joints of the arm checker (7) */
+    int arm_spheres;          /* its collision spheres (28) */
+    int max_env_words;        /* spheres + 2 * boxes per environment */
+    int max_segments;         /* segments per motion (longer motions are invalid) */
+    int max_k;                /* k of nearest-neighbour queries */
+    int max_nn_dim;           /* L2 dimension of nearest-neighbour points */
+    int max_sample_dim;       /* dimension of uniform sampling */
+    int device;               /* HIP device in use */
+    int compute_units;
+    char device_name[64];
+} ompl_amd_caps;
+/* Initialises the device like ompl_amd_init(0) when needed. */
+int ompl_amd_capabilities(ompl_amd_caps *caps);
 int ompl_amd_synchronize(void *stream);
 
 /* Device memory helpers (thin wrappers over hipMalloc/hipMemcpy) for callers

@@ -48,6 +48,17 @@ def init(device: int = 0) -> None:
     call("ompl_amd_init", device)
 
 
+def capabilities() -> dict:
+    """Limits of the library and the device (initialises device 0 if needed)."""
+    from ._lib import Caps
+
+    c = Caps()
+    call("ompl_amd_capabilities", C.byref(c))
+    d = {k: getattr(c, k) for k, _ in c._fields_}
+    d["device_name"] = c.device_name.decode()
+    return d
+
+
 def get_tuning() -> dict:
     t = Tuning()
     call("ompl_amd_get_tuning", C.byref(t))

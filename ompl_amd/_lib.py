@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 302
+ABI_VERSION = 303
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -27,6 +27,7 @@ SIGNATURES = {
     "ompl_amd_last_error": (C.c_char_p, []),
     "ompl_amd_version": (C.c_int, []),
     "ompl_amd_synchronize": (C.c_int, [vp]),
+    "ompl_amd_capabilities": (C.c_int, [vp]),
     "ompl_amd_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
     "ompl_amd_free": (C.c_int, [vp]),
     "ompl_amd_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
@@ -126,6 +127,21 @@ class Tuning(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Caps(C.Structure):
+    _fields_ = [
+        ("arm_dof", C.c_int),
+        ("arm_spheres", C.c_int),
+        ("max_env_words", C.c_int),
+        ("max_segments", C.c_int),
+        ("max_k", C.c_int),
+        ("max_nn_dim", C.c_int),
+        ("max_sample_dim", C.c_int),
+        ("device", C.c_int),
+        ("compute_units", C.c_int),
+        ("device_name", C.c_char * 64),
+    ]
 
 
 class PrmStats(C.Structure):

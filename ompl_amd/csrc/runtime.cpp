@@ -268,6 +268,28 @@ int ompl_amd_version(void)
     return OMPL_AMD_VERSION;
 }
 
+int ompl_amd_capabilities(ompl_amd_caps *caps)
+{
+    return guarded([&] {
+        OA_REQUIRE(caps, OMPL_AMD_ERR_ARG, "null argument");
+        ensure_init();
+        ompl_amd_caps c{};
+        c.arm_dof = OMPL_AMD_ARM_DOF;
+        c.arm_spheres = OMPL_AMD_ARM_SPHERES;
+        c.max_env_words = kMaxEnvWords;
+        c.max_segments = 1 << 24;
+        c.max_k = 64;
+        c.max_nn_dim = 14;
+        c.max_sample_dim = 16;
+        c.device = device();
+        c.compute_units = num_cus();
+        hipDeviceProp_t prop;
+        OA_HIP(hipGetDeviceProperties(&prop, device()));
+        snprintf(c.device_name, sizeof(c.device_name), "%s", prop.gcnArchName);
+        *caps = c;
+    });
+}
+
 int ompl_amd_synchronize(void *stream)
 {
     return guarded([&] { OA_HIP(hipStreamSynchronize(resolve_stream(stream))); });

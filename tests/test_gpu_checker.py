@@ -236,3 +236,15 @@ def test_check_motions_shared_s2_pass(gpu, oracle, strategy, eshape, wshape):
     assert np.array_equal(v.cpu().numpy(), ref_v)
     assert np.array_equal(c.cpu().numpy(), ref_c)
     gpu.set_tuning(**DEFAULT, validity_shared_fk=0)
+
+
+def test_capabilities_match_enforced_limits(gpu):
+    from ompl_amd._lib import OmplAmdError
+
+    caps = gpu.capabilities()
+    assert caps["arm_dof"] == 7 and caps["arm_spheres"] == 28 and caps["compute_units"] >= 1
+    assert caps["device_name"].startswith("gfx950")
+    w = caps["max_env_words"]
+    gpu.Env(np.zeros((w - 2, 4), np.float32), np.zeros((1, 6), np.float32)).close()  # exactly at the limit
+    with pytest.raises(OmplAmdError):
+        gpu.Env(np.zeros((w - 1, 4), np.float32), np.zeros((1, 6), np.float32))
