@@ -343,6 +343,13 @@ def scope_rows(oa, run_cpu, cpu_threads):
                               "bytes_written_per_state": 84,
                               "hbm_gbps": n * 84 / (ms * 1e-3) / 1e9,
                               "note": "host-staged seeds and bounds and a stream sync are inside the time"}
+    if oracle_py is not None:
+        t0 = time.perf_counter()
+        for sd in seeds[:32].tolist():
+            oracle_py.Mt19937(sd).sample_uniform(lo, hi, 782)
+        dt = time.perf_counter() - t0
+        rows["sample_uniform"]["cpu_baseline"] = {"states_per_s": 32 * 782 / dt, "cores": 1, "kind": "port",
+                                                  "sample": f"32 of the 256 generators, {dt:.2f} s"}
 
     # valid sampling: 256 samplers x 64 calls in the 128 + 128 obstacle scene (1.2% valid)
     s2, b2 = oa.workload.obstacles(ENV_SEED, N_SPHERES, N_BOXES)
@@ -352,6 +359,14 @@ def scope_rows(oa, run_cpu, cpu_threads):
     rows["sample_valid"] = {"samplers": 256, "calls": 64, "attempts": 100, "ms": ms,
                             "calls_per_s": 256 * 64 / (ms * 1e-3), "found_fraction": float(found.float().mean()),
                             "draws_per_s": float(draws.sum().item()) / (ms * 1e-3)}
+    if oracle_py is not None:
+        oenv2 = oracle_py.OracleEnv(s2, b2)
+        t0 = time.perf_counter()
+        for sd in seeds[:8].tolist():
+            oracle_py.Mt19937(sd).sample_valid(oenv2, 64, 100)
+        dt = time.perf_counter() - t0
+        rows["sample_valid"]["cpu_baseline"] = {"calls_per_s": 8 * 64 / dt, "cores": 1, "kind": "port",
+                                                "sample": f"8 of the 256 samplers, {dt:.2f} s"}
 
     # precomputed state lists: the motion states of the 16,384 benchmark edges, first 64 each
     a_rows, b_rows = oa.workload.edges(EDGE_SEED, 4096)
@@ -363,6 +378,18 @@ def scope_rows(oa, run_cpu, cpu_threads):
     v, f = oa.check_state_lists(env2, std, offs)
     rows["state_lists"] = {"lists": 4096, "states_per_list": 64, "ms": ms, "lists_per_s": 4096 / (ms * 1e-3),
                            "valid_fraction": float(v.float().mean().item())}
+    if oracle_py is not None:
+        oenv2 = oracle_py.OracleEnv(s2, b2)
+        t0 = time.perf_counter()
+        same = True
+        vh, fh = v.cpu().numpy(), f.cpu().numpy()
+        for i in range(512):
+            ok, first = oracle_py.check_states_first_invalid(oenv2, st[64 * i:64 * (i + 1)])
+            same &= bool(ok) == bool(vh[i]) and (first if not ok else -1) == int(fh[i])
+        dt = time.perf_counter() - t0
+        rows["state_lists"]["cpu_baseline"] = {"lists_per_s": 512 / dt, "cores": 1, "kind": "port",
+                                               "sample": f"512 of the 4,096 lists, {dt:.2f} s"}
+        rows["state_lists"]["parity_bit_exact_on_sample"] = bool(same)
     return rows
 
 

@@ -17,8 +17,8 @@
 //   motion_chunk_kernel     chunk-parallel walk of the survivors: work item
 //                           (chunk b, survivor s) in chunk-major order over a
 //                           persistent grid; a chunk is skipped once an earlier
-//                           position of its edge is known invalid (atomicMin)
-//   motion_finalize_kernel  survivors' results; clears the counters
+//                           position of its edge is known invalid (atomicMin);
+//                           its last block writes the survivors' results
 //   motion_fused_kernel     s2 of the wave's edges, then the wave walks its own
 //                           survivors (one launch)
 //   motion_single_kernel    one edge passed by value (the plain per-call path)
@@ -318,23 +318,29 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
             if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[sj], bad);
         }
     }
-}
-
-// Survivors' results from their first invalid position; one block, which then
-// clears the strategy's counters for the next batch (no memset launch).
-__global__ void __launch_bounds__(1024) motion_finalize_kernel(MotionArgs A)
-{
-    const int n_surv = *A.list_count;
-    for (int s = threadIdx.x; s < n_surv; s += blockDim.x) {
-        const int64_t e = A.list[s];
-        const int fb = A.first_bad[s];
+    if (!A.done_blocks) return;
+    // the last block to finish writes the survivors' results and clears the
+    // counters for the next batch (one counter update per block)
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(A.done_blocks, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int64_t sv = threadIdx.x; sv < n_surv; sv += blockDim.x) {
+        const int64_t e = A.list[sv];
+        const int fb = __hip_atomic_load(&A.first_bad[sv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         A.out[e] = fb == INT_MAX ? 1 : 0;
-        if (A.checked) A.checked[e] = fb == INT_MAX ? A.surv_nd[s] : fb + 1;
+        if (A.checked) A.checked[e] = fb == INT_MAX ? A.surv_nd[sv] : fb + 1;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         *A.list_count = 0;
         *A.max_m = 0;
+        *A.done_blocks = 0;
     }
 }
 
@@ -630,12 +636,6 @@ void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int b
     OA_SHAPES(OA_LAUNCH)
 #undef OA_LAUNCH
     fail(OMPL_AMD_ERR_ARG, "walk shape (%d, %d) not built", lps, ops);
-}
-
-void launch_motion_finalize(const MotionArgs &A, hipStream_t s)
-{
-    hipLaunchKernelGGL(motion_finalize_kernel, dim3(1), dim3(1024), 0, s, A);
-    OA_HIP(hipGetLastError());
 }
 
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,

@@ -26,6 +26,7 @@ struct MotionArgs {
     int32_t *max_m;      // largest bisection position count among survivors
     int32_t *surv_nd;    // per survivor: segment count nd
     double *surv_ab;     // per survivor: endpoints a then b (14 doubles, contiguous), optional
+    int32_t *done_blocks; // chunk walk: finished blocks; the last one finalizes (optional)
     int start;           // first walk position (0 = s2, 1 = skip s2)
 };
 
@@ -49,7 +50,6 @@ void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hi
 void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves, hipStream_t s);
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
                           hipStream_t s, bool shared = false);
-void launch_motion_finalize(const MotionArgs &A, hipStream_t s);
 void launch_motion_fused(const MotionArgs &A, bool exact, int le, int oe, int lw, int ow, int block, hipStream_t s);
 void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result,
                           hipStream_t s);
