@@ -198,29 +198,30 @@ TRAFFIC_RECORD = ROOT / "profiles" / "motion_batch_16384_edges_256_obstacles.hbm
 
 def hbm_traffic(kernel: str, tuning: dict):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE record, when it was measured with this launch
+    FETCH_SIZE / WRITE_SIZE records, when one was taken with this launch
     shape; otherwise None."""
     try:
         rec = json.loads(TRAFFIC_RECORD.read_text())
     except (OSError, ValueError):
         return None, None
     keys = ("validity_lanes_per_state", "validity_obstacle_parts", "motion_lanes_per_state",
-            "motion_obstacle_parts", "motion_strategy")
-    if any(rec["tuning"].get(k) != tuning.get(k) for k in keys):
-        return None, f"{TRAFFIC_RECORD.name}: recorded for another launch shape"
+            "motion_obstacle_parts", "motion_strategy", "validity_shared_fk")
     shared = "true" if tuning["motion_strategy"] == 4 else "false"
     if kernel == "motion_chunk":
         name = (f"motion_chunk_kernel<{tuning['motion_lanes_per_state']}, {tuning['motion_obstacle_parts']}, false, "
                 f"{shared}>")
     elif kernel == "motion_endpoint":
-        kind = "motion_endpoint_shared_kernel" if shared == "true" else "motion_endpoint_kernel"
+        kind = "motion_endpoint_shared_kernel" if tuning.get("validity_shared_fk") else "motion_endpoint_kernel"
         name = f"{kind}<{tuning['validity_lanes_per_state']}, {tuning['validity_obstacle_parts']}>"
     else:
         return None, None
-    for k, v in rec["bytes_per_dispatch"].items():
-        if name + "(" in k:
-            return int(v), f"{TRAFFIC_RECORD.name} ({k})"
-    return None, None
+    for r in rec["records"]:
+        if any(r["tuning"].get(k) != tuning.get(k) for k in keys):
+            continue
+        for k, v in r["bytes_per_dispatch"].items():
+            if name + "(" in k:
+                return int(v), f"{TRAFFIC_RECORD.name} ({k})"
+    return None, f"{TRAFFIC_RECORD.name}: no record for this launch shape"
 
 
 def timed(fn, reps, warm=2):
