@@ -60,36 +60,24 @@ struct GrowRes {
     double xstate[kDof];
 };
 
-// Per-request scratch of one tick (device): the motion to check and the
-// command fields the later kernels need, so that only the first kernel reads
-// the command block (which lives in host memory).
+// Per-request scratch of one tick (device).
 struct GrowEdge {
     double a[kDof], b[kDof]; // motion checked in the direction of growth
     double dst[kDof];        // state to append
-    const float4 *obs;       // the request's environment
-    double *tree_s;          // its tree
-    int32_t *tree_parent;
-    int32_t ns, nb;
-    int32_t tree_n;
     int32_t nd;              // segments of the motion
     int32_t npos;            // positions to check (0 = none: trapped before the check)
     int32_t extra;           // 1: the last position is isValid(dst) (goal tree)
     int32_t nm;              // nearest node
     int32_t reach;
     int32_t first_bad;       // lowest invalid position found (INT_MAX: none)
-    int32_t pad;
 };
 
-// Arguments of the tick kernels.  cmds, res and ncmd are in host-coherent
-// mapped memory: the host writes a tick's commands and count, launches the
-// tick (one graph launch) and reads the results after the stream completes,
-// with no copies.  Grids are sized for the server's maximum active requests.
 struct TickArgs {
     const GrowCmd *cmds;
     GrowEdge *edges;
     GrowRes *res;
-    int32_t *max_pos;     // largest npos of the tick (reset by the commit kernel)
-    const int32_t *ncmd;  // requests in this tick
+    int32_t *max_pos; // largest npos of the tick (reset by the commit kernel)
+    int ncmd;
 };
 
 // Tick kernel 1 (one wave per request): nearest neighbour in the request's
@@ -98,16 +86,8 @@ struct TickArgs {
 __global__ void __launch_bounds__(64) grow_prepare_kernel(TickArgs T)
 {
     const int r = blockIdx.x;
-    if (r >= *T.ncmd) return;
-    // the command (host memory) is read once, one 8-byte word per lane, and
-    // shared through LDS
-    static_assert(sizeof(GrowCmd) % 8 == 0 && sizeof(GrowCmd) <= 64 * 8, "command layout");
-    constexpr int kWords = (int)(sizeof(GrowCmd) / 8);
-    __shared__ uint64_t cw[kWords];
-    if (threadIdx.x < kWords) cw[threadIdx.x] = reinterpret_cast<const uint64_t *>(T.cmds + r)[threadIdx.x];
-    __syncthreads();
-    GrowCmd C;
-    __builtin_memcpy(&C, cw, sizeof(GrowCmd));
+    if (r >= T.ncmd) return;
+    const GrowCmd &C = T.cmds[r];
     GrowEdge &E = T.edges[r];
     const int lane = lane_id();
     double best = INFINITY;
@@ -141,12 +121,6 @@ __global__ void __launch_bounds__(64) grow_prepare_kernel(TickArgs T)
     for (int k = 0; k < kDof; ++k) ns_[k] = C.tree_s[(int64_t)nm * kDof + k];
     E.nm = nm;
     E.first_bad = INT_MAX;
-    E.obs = C.obs;
-    E.ns = C.ns;
-    E.nb = C.nb;
-    E.tree_s = C.tree_s;
-    E.tree_parent = C.tree_parent;
-    E.tree_n = C.tree_n;
     const double d = best;
     bool reach = true;
     for (int k = 0; k < kDof; ++k) x[k] = C.target[k];
@@ -196,7 +170,7 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
     const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int n = *T.ncmd;
+    const int n = T.ncmd;
     const int64_t nblk = (*T.max_pos + G - 1) / G;
     const int64_t items = (int64_t)n * nblk;
     const Shape sh(lane);
@@ -217,6 +191,7 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
             const int rj = __shfl(r, j), bj = __shfl(base, j);
             GrowEdge &E = T.edges[rj];
             if (__hip_atomic_load(&E.first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != INT_MAX) continue;
+            const GrowCmd &C = T.cmds[rj];
             const int p = bj + sh.group;
             const int motion_pos = E.npos - E.extra;
             const bool plive = p < E.npos;
@@ -235,7 +210,7 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
             }
             LaneSpheres<LPS> S;
             arm_fk<LPS>(q, sh.slot, S);
-            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, E.obs, E.ns, E.nb, plive, sh.part);
+            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, C.obs, C.ns, C.nb, plive, sh.part);
             const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
             if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);
         }
@@ -248,19 +223,20 @@ __global__ void grow_commit_kernel(TickArgs T)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r == 0) *T.max_pos = 0;
-    if (r >= *T.ncmd) return;
+    if (r >= T.ncmd) return;
+    const GrowCmd &C = T.cmds[r];
     const GrowEdge &E = T.edges[r];
     GrowRes &R = T.res[r];
     if (E.npos == 0 || E.first_bad != INT_MAX) {
         R.status = kTrapped;
         return;
     }
-    const int idx = E.tree_n;
+    const int idx = C.tree_n;
     for (int k = 0; k < kDof; ++k) {
-        E.tree_s[(int64_t)idx * kDof + k] = E.dst[k];
+        C.tree_s[(int64_t)idx * kDof + k] = E.dst[k];
         R.xstate[k] = E.dst[k];
     }
-    E.tree_parent[idx] = E.nm;
+    C.tree_parent[idx] = E.nm;
     R.status = E.reach ? kReached : kAdvanced;
     R.xmotion = idx;
 }
@@ -443,14 +419,11 @@ struct Server {
     std::vector<Slot> slots;
     std::vector<int> free_slots;
     char *pool = nullptr;
-    GrowCmd *cmd_host = nullptr, *cmd_dev = nullptr; // one mapped host buffer, two addresses
+    GrowCmd *cmd_host = nullptr, *cmd_dev = nullptr;
     GrowRes *res_host = nullptr, *res_dev = nullptr;
-    int32_t *ncmd_host = nullptr, *ncmd_dev = nullptr;
     GrowEdge *edges_dev = nullptr;
     int32_t *max_pos_dev = nullptr;
     unsigned check_blocks = 0;
-    hipGraph_t tick_graph = nullptr;
-    hipGraphExec_t tick_exec = nullptr;
 
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -541,11 +514,16 @@ void Server::run()
             if (batch.empty()) continue;
             const int n = (int)batch.size();
             for (int i = 0; i < n; ++i) next_command(*planners[i], cmd_host[i]);
-            *ncmd_host = n;
+            OA_HIP(hipMemcpyAsync(cmd_dev, cmd_host, n * sizeof(GrowCmd), hipMemcpyHostToDevice, stream));
             {
                 ProfScope ps("serve_tick", stream);
-                OA_HIP(hipGraphLaunch(tick_exec, stream));
+                TickArgs T{cmd_dev, edges_dev, res_dev, max_pos_dev, n};
+                hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(64), 0, stream, T);
+                hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
+                hipLaunchKernelGGL(grow_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, T);
+                OA_HIP(hipGetLastError());
             }
+            OA_HIP(hipMemcpyAsync(res_host, res_dev, n * sizeof(GrowRes), hipMemcpyDeviceToHost, stream));
             OA_HIP(hipStreamSynchronize(stream));
             ++ticks;
             steps += n;
@@ -709,30 +687,14 @@ int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *o
             S.slots.push_back(sl);
             S.free_slots.push_back(cfg->max_active - 1 - i);
         }
-        const unsigned mapped = hipHostMallocCoherent | hipHostMallocMapped;
-        OA_HIP(hipHostMalloc((void **)&S.cmd_host, sizeof(GrowCmd) * cfg->max_active, mapped));
-        OA_HIP(hipHostMalloc((void **)&S.res_host, sizeof(GrowRes) * cfg->max_active, mapped));
-        OA_HIP(hipHostMalloc((void **)&S.ncmd_host, sizeof(int32_t), mapped));
-        OA_HIP(hipHostGetDevicePointer((void **)&S.cmd_dev, S.cmd_host, 0));
-        OA_HIP(hipHostGetDevicePointer((void **)&S.res_dev, S.res_host, 0));
-        OA_HIP(hipHostGetDevicePointer((void **)&S.ncmd_dev, S.ncmd_host, 0));
-        *S.ncmd_host = 0;
+        OA_HIP(hipHostMalloc((void **)&S.cmd_host, sizeof(GrowCmd) * cfg->max_active, hipHostMallocDefault));
+        OA_HIP(hipHostMalloc((void **)&S.res_host, sizeof(GrowRes) * cfg->max_active, hipHostMallocDefault));
+        OA_HIP(hipMalloc(&S.cmd_dev, sizeof(GrowCmd) * cfg->max_active));
+        OA_HIP(hipMalloc(&S.res_dev, sizeof(GrowRes) * cfg->max_active));
         OA_HIP(hipMalloc(&S.edges_dev, sizeof(GrowEdge) * cfg->max_active));
         OA_HIP(hipMalloc(&S.max_pos_dev, sizeof(int32_t)));
         OA_HIP(hipMemset(S.max_pos_dev, 0, sizeof(int32_t)));
-        OA_HIP(hipDeviceSynchronize());
         S.check_blocks = (unsigned)num_cus() * 2; // 8 waves per CU
-        // the tick: three launches recorded once, replayed with one graph launch
-        {
-            TickArgs T{S.cmd_dev, S.edges_dev, S.res_dev, S.max_pos_dev, S.ncmd_dev};
-            const unsigned na = (unsigned)cfg->max_active;
-            OA_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
-            hipLaunchKernelGGL(grow_prepare_kernel, dim3(na), dim3(64), 0, S.stream, T);
-            hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(S.check_blocks), dim3(256), 0, S.stream, T);
-            hipLaunchKernelGGL(grow_commit_kernel, dim3((na + 255) / 256), dim3(256), 0, S.stream, T);
-            OA_HIP(hipStreamEndCapture(S.stream, &S.tick_graph));
-            OA_HIP(hipGraphInstantiate(&S.tick_exec, S.tick_graph, nullptr, nullptr, 0));
-        }
         S.th = std::thread([&S] { S.run(); });
         *out = h;
     });
@@ -751,13 +713,12 @@ int ompl_amd_server_destroy(ompl_amd_server h)
         if (S.th.joinable()) S.th.join();
         for (auto &kv : S.reqs) delete kv.second;
         hipFree(S.pool);
-        if (S.tick_exec) hipGraphExecDestroy(S.tick_exec);
-        if (S.tick_graph) hipGraphDestroy(S.tick_graph);
+        hipFree(S.cmd_dev);
+        hipFree(S.res_dev);
         hipFree(S.edges_dev);
         hipFree(S.max_pos_dev);
         hipHostFree(S.cmd_host);
         hipHostFree(S.res_host);
-        hipHostFree(S.ncmd_host);
         hipStreamDestroy(S.stream);
         delete h;
     });
