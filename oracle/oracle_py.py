@@ -58,6 +58,7 @@ def lib():
         L.orc_longest_valid_segment.restype = C.c_double
         L.orc_distance_d.argtypes = [vp, vp, C.c_int]
         L.orc_distance_d.restype = C.c_double
+        L.orc_interpolate_d.argtypes = [vp, vp, C.c_double, C.c_int, vp]
         L.orc_valid_segment_count.argtypes = [vp, vp, C.c_int, C.c_double, C.c_uint]
         L.orc_valid_segment_count.restype = C.c_uint
         L.orc_bfs_order.argtypes = [C.c_int, vp]
@@ -168,6 +169,22 @@ def check_motion(env: OracleEnv, a, b, lvs, factor=1):
     cnt = C.c_int()
     v = lib().orc_check_motion(env.ref(), _p(a), _p(b), lvs, factor, C.byref(cnt))
     return bool(v), cnt.value
+
+
+def distance(a, b) -> float:
+    """RealVectorStateSpace::distance (FP64)."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return lib().orc_distance_d(_p(a), _p(b), a.shape[0])
+
+
+def interpolate(a, b, t: float) -> np.ndarray:
+    """RealVectorStateSpace::interpolate (FP64)."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros_like(a)
+    lib().orc_interpolate_d(_p(a), _p(b), float(t), a.shape[0], _p(out))
+    return out
 
 
 def check_states(env: OracleEnv, states):

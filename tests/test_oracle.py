@@ -371,3 +371,25 @@ def test_oracle_radius_and_nearest_match_reference_linear(oracle, name, metric):
         if not diff:
             assert np.max(np.abs(dd - ref_d), initial=0) < 2e-5
         assert oracle.nearest(metric, data, queries[j]) == f["nearest"][j]
+
+
+def test_distance_and_interpolation_invariants(oracle):
+    """The reference's own state-space invariants (tests/base/StateSpaceTest.h:71-117,
+    eps = 10 * DBL_EPSILON): d(s, s) = 0, d > 0 and symmetric for distinct states,
+    interpolation at 0 / 1 gives the endpoints, the midpoint satisfies the triangle
+    equality, and interpolate(interp(s3, s2, .5)) == interpolate(s1, s2, .75) for
+    s3 = interp(s1, s2, .5)."""
+    eps = 10 * np.finfo(np.float64).eps
+    rng = np.random.default_rng(17)
+    for _ in range(200):
+        s1, s2 = rng.uniform(-np.pi, np.pi, (2, 7))
+        assert abs(oracle.distance(s1, s1)) <= eps
+        d12 = oracle.distance(s1, s2)
+        assert d12 > 0 and abs(d12 - oracle.distance(s2, s1)) <= eps
+        assert oracle.distance(s1, oracle.interpolate(s1, s2, 0.0)) <= eps
+        assert oracle.distance(s2, oracle.interpolate(s1, s2, 1.0)) <= eps
+        s3 = oracle.interpolate(s1, s2, 0.5)
+        assert abs(oracle.distance(s1, s3) + oracle.distance(s3, s2) - d12) <= eps * max(1.0, d12)
+        s3b = oracle.interpolate(s3, s2, 0.5)
+        s2b = oracle.interpolate(s1, s2, 0.75)
+        assert oracle.distance(s2b, s3b) <= eps
