@@ -88,7 +88,9 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         return {"p50": float(np.percentile(xs, 50)), "p99": float(np.percentile(xs, 99)),
                 "mean": float(np.mean(xs))}
 
-    # (a) served
+    # (a) served (the library's HIP events time each tick's kernels on the server stream)
+    oa.profile.reset()
+    oa.profile.enable(True)
     srv = serve.Server(hbm_budget_bytes=4 << 30, max_active=1024, max_nodes=4096)
     ids = [None] * n_requests
     t0 = time.perf_counter()
@@ -103,6 +105,9 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     lat_s = [r.latency_ms for r in res_s]
     paths_s = [srv.path(rid) for rid in ids]
     srv.close()
+    n_ticks, tick_ms = oa.profile.query("serve_tick")
+    oa.profile.enable(False)
+    oa.profile.reset()
 
     # (b) plain: FIFO queue drained by plain_workers threads
     lat_p = [0.0] * n_requests
@@ -143,7 +148,8 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         "requests": n_requests, "qps": qps, "envs_used": len(envs), "envs_drawn": n_envs,
         "solved": solved, "max_iters": max_iters,
         "served": {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "ticks": stats["ticks"],
-                   "grow_steps": stats["steps"]},
+                   "grow_steps": stats["steps"], "wall_us_per_tick": 1e6 * span_s / max(stats["ticks"], 1),
+                   "gpu_us_per_tick": 1e3 * tick_ms / n_ticks if n_ticks else None},
         "plain": {"latency_ms": pct(lat_p), "throughput_rps": n_requests / span_p, "workers": plain_workers},
         "p99_plain_over_served": float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)),
         "identical_plans": bool(same),
