@@ -80,10 +80,14 @@ struct TickArgs {
     int ncmd;
 };
 
-// Tick kernel 1 (one wave per request): nearest neighbour in the request's
-// tree -- first strict minimum of distance(elem, target), lane-strided then
-// wave-reduced -- and steering by the planner range (RRTConnect.cpp:119-138).
-__global__ void __launch_bounds__(64) grow_prepare_kernel(TickArgs T)
+// Tick kernel 1 (one block of kPrepThreads per request): nearest neighbour in
+// the request's tree -- first strict minimum of distance(elem, target),
+// thread-strided, then reduced over each wave and across the block's waves
+// by (distance, index) -- and steering by the planner range
+// (RRTConnect.cpp:119-138).
+constexpr int kPrepThreads = 256;
+
+__global__ void __launch_bounds__(kPrepThreads) grow_prepare_kernel(TickArgs T)
 {
     const int r = blockIdx.x;
     if (r >= T.ncmd) return;
@@ -92,7 +96,7 @@ __global__ void __launch_bounds__(64) grow_prepare_kernel(TickArgs T)
     const int lane = lane_id();
     double best = INFINITY;
     int best_i = INT_MAX;
-    for (int i = lane; i < C.tree_n; i += 64) {
+    for (int i = threadIdx.x; i < C.tree_n; i += kPrepThreads) {
         const double *s = C.tree_s + (int64_t)i * kDof;
         double dist = 0.0;
 #pragma unroll
@@ -115,7 +119,19 @@ __global__ void __launch_bounds__(64) grow_prepare_kernel(TickArgs T)
             best_i = oi;
         }
     }
-    if (lane != 0) return;
+    __shared__ double wd[kPrepThreads / 64];
+    __shared__ int wi[kPrepThreads / 64];
+    if (lane == 0) {
+        wd[threadIdx.x >> 6] = best;
+        wi[threadIdx.x >> 6] = best_i;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < kPrepThreads / 64; ++w)
+        if (wd[w] < best || (wd[w] == best && wi[w] < best_i)) {
+            best = wd[w];
+            best_i = wi[w];
+        }
     const int nm = best_i;
     double ns_[kDof], x[kDof];
     for (int k = 0; k < kDof; ++k) ns_[k] = C.tree_s[(int64_t)nm * kDof + k];
@@ -518,7 +534,7 @@ void Server::run()
             {
                 ProfScope ps("serve_tick", stream);
                 TickArgs T{cmd_dev, edges_dev, res_dev, max_pos_dev, n};
-                hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(64), 0, stream, T);
+                hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(kPrepThreads), 0, stream, T);
                 hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
                 hipLaunchKernelGGL(grow_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, T);
                 OA_HIP(hipGetLastError());
