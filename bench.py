@@ -106,6 +106,10 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     paths_s = [srv.path(rid) for rid in ids]
     srv.close()
     n_ticks, tick_ms = oa.profile.query("serve_tick")
+    tick_parts = {}
+    for part in ("serve_prepare", "serve_check", "serve_commit"):
+        n_p, ms_p = oa.profile.query(part)
+        tick_parts[part] = 1e3 * ms_p / n_p if n_p else None
     oa.profile.enable(False)
     oa.profile.reset()
 
@@ -149,7 +153,7 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         "solved": solved, "max_iters": max_iters,
         "served": {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "ticks": stats["ticks"],
                    "grow_steps": stats["steps"], "wall_us_per_tick": 1e6 * span_s / max(stats["ticks"], 1),
-                   "gpu_us_per_tick": 1e3 * tick_ms / n_ticks if n_ticks else None},
+                   "gpu_us_per_tick": 1e3 * tick_ms / n_ticks if n_ticks else None, "tick_kernels_us": tick_parts},
         "plain": {"latency_ms": pct(lat_p), "throughput_rps": n_requests / span_p, "workers": plain_workers},
         "p99_plain_over_served": float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)),
         "identical_plans": bool(same),

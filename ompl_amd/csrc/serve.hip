@@ -534,9 +534,18 @@ void Server::run()
             {
                 ProfScope ps("serve_tick", stream);
                 TickArgs T{cmd_dev, edges_dev, res_dev, max_pos_dev, n};
-                hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(kPrepThreads), 0, stream, T);
-                hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
-                hipLaunchKernelGGL(grow_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, T);
+                {
+                    ProfScope p1("serve_prepare", stream);
+                    hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(kPrepThreads), 0, stream, T);
+                }
+                {
+                    ProfScope p2("serve_check", stream);
+                    hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
+                }
+                {
+                    ProfScope p3("serve_commit", stream);
+                    hipLaunchKernelGGL(grow_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, T);
+                }
                 OA_HIP(hipGetLastError());
             }
             OA_HIP(hipMemcpyAsync(res_host, res_dev, n * sizeof(GrowRes), hipMemcpyDeviceToHost, stream));
