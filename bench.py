@@ -49,7 +49,8 @@ def percentile(xs, p):
     return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
 
 
-def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000, log_path=None):
+def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000, log_path=None,
+                  check_parts=0, profile_ticks=False, run_plain=True):
     """Poisson stream of RRT-Connect requests over seeded environments (16-1024
     obstacles, log-uniform, half spheres / half boxes), served (a) by the
     continuous-batching server and (b) by the plain path (host RRT-Connect, one
@@ -88,10 +89,11 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         return {"p50": float(np.percentile(xs, 50)), "p99": float(np.percentile(xs, 99)),
                 "mean": float(np.mean(xs))}
 
-    # (a) served (the library's HIP events time each tick's kernels on the server stream)
-    oa.profile.reset()
-    oa.profile.enable(True)
-    srv = serve.Server(hbm_budget_bytes=4 << 30, max_active=1024, max_nodes=4096)
+    # (a) served (optionally the library's HIP events time each tick's kernels)
+    if profile_ticks:
+        oa.profile.reset()
+        oa.profile.enable(True)
+    srv = serve.Server(hbm_budget_bytes=4 << 30, max_active=1024, max_nodes=4096, check_obstacle_parts=check_parts)
     ids = [None] * n_requests
     t0 = time.perf_counter()
     for i, r in enumerate(reqs):
@@ -105,13 +107,21 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     lat_s = [r.latency_ms for r in res_s]
     paths_s = [srv.path(rid) for rid in ids]
     srv.close()
-    n_ticks, tick_ms = oa.profile.query("serve_tick")
-    tick_parts = {}
-    for part in ("serve_prepare", "serve_check", "serve_commit"):
-        n_p, ms_p = oa.profile.query(part)
-        tick_parts[part] = 1e3 * ms_p / n_p if n_p else None
-    oa.profile.enable(False)
-    oa.profile.reset()
+    n_ticks, tick_ms, tick_parts = 0, 0.0, None
+    if profile_ticks:
+        n_ticks, tick_ms = oa.profile.query("serve_tick")
+        tick_parts = {}
+        for part in ("serve_prepare", "serve_check", "serve_commit"):
+            n_p, ms_p = oa.profile.query(part)
+            tick_parts[part] = 1e3 * ms_p / n_p if n_p else None
+        oa.profile.enable(False)
+        oa.profile.reset()
+    served = {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "ticks": stats["ticks"],
+              "grow_steps": stats["steps"], "wall_us_per_tick": 1e6 * span_s / max(stats["ticks"], 1)}
+    if profile_ticks:
+        served.update(gpu_us_per_tick=1e3 * tick_ms / n_ticks if n_ticks else None, tick_kernels_us=tick_parts)
+    if not run_plain:
+        return {"requests": n_requests, "qps": qps, "check_obstacle_parts": check_parts or 8, "served": served}
 
     # (b) plain: FIFO queue drained by plain_workers threads
     lat_p = [0.0] * n_requests
@@ -151,9 +161,7 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     return {
         "requests": n_requests, "qps": qps, "envs_used": len(envs), "envs_drawn": n_envs,
         "solved": solved, "max_iters": max_iters,
-        "served": {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "ticks": stats["ticks"],
-                   "grow_steps": stats["steps"], "wall_us_per_tick": 1e6 * span_s / max(stats["ticks"], 1),
-                   "gpu_us_per_tick": 1e3 * tick_ms / n_ticks if n_ticks else None, "tick_kernels_us": tick_parts},
+        "served": served,
         "plain": {"latency_ms": pct(lat_p), "throughput_rps": n_requests / span_p, "workers": plain_workers},
         "p99_plain_over_served": float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)),
         "identical_plans": bool(same),
@@ -607,6 +615,11 @@ def main(argv=None):
     if args.stream_requests > 0:
         stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers,
                                    log_path=args.benchmark_log)
+        # tick breakdown (HIP events around each tick kernel; slower than the run above) per obstacle split
+        stream_res["tick_profile"] = {
+            parts: served_stream(oa, min(64, args.stream_requests), args.stream_qps, args.stream_envs, 0,
+                                 check_parts=parts, profile_ticks=True, run_plain=False)["served"]
+            for parts in (2, 4, 8)}
 
     prm_res = None
     if args.prm_samples > 0:

@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 303
+#define OMPL_AMD_VERSION 304
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -335,6 +335,8 @@ typedef struct {
     size_t hbm_budget_bytes; /* admission limit on the sum of active requests' tree memory */
     int max_active;          /* concurrent requests (tree slots preallocated) */
     int max_nodes;           /* nodes per tree per request */
+    int check_obstacle_parts; /* lanes splitting each environment's obstacle list in the
+                                 tick's motion check: 2, 4 or 8 (0 = 8) */
 } ompl_amd_server_config;
 
 typedef struct {

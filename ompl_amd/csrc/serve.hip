@@ -540,7 +540,12 @@ void Server::run()
                 }
                 {
                     ProfScope p2("serve_check", stream);
-                    hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
+                    if (cfg.check_obstacle_parts == 2)
+                        hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
+                    else if (cfg.check_obstacle_parts == 4)
+                        hipLaunchKernelGGL((grow_check_kernel<8, 4>), dim3(check_blocks), dim3(256), 0, stream, T);
+                    else
+                        hipLaunchKernelGGL((grow_check_kernel<8, 8>), dim3(check_blocks), dim3(256), 0, stream, T);
                 }
                 {
                     ProfScope p3("serve_commit", stream);
@@ -690,10 +695,14 @@ int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *o
         OA_REQUIRE(cfg && out, OMPL_AMD_ERR_ARG, "null argument");
         OA_REQUIRE(cfg->max_active >= 1 && cfg->max_active <= 65536, OMPL_AMD_ERR_ARG, "max_active in [1, 65536]");
         OA_REQUIRE(cfg->max_nodes >= 2, OMPL_AMD_ERR_ARG, "max_nodes >= 2");
+        OA_REQUIRE(cfg->check_obstacle_parts == 0 || cfg->check_obstacle_parts == 2 ||
+                       cfg->check_obstacle_parts == 4 || cfg->check_obstacle_parts == 8,
+                   OMPL_AMD_ERR_ARG, "check_obstacle_parts must be 0, 2, 4 or 8");
         ensure_init();
         auto *h = new ompl_amd_server_s();
         Server &S = h->srv;
         S.cfg = *cfg;
+        if (S.cfg.check_obstacle_parts == 0) S.cfg.check_obstacle_parts = 8;
         OA_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
         const size_t per_tree = (size_t)cfg->max_nodes * (kDof * sizeof(double) + sizeof(int32_t));
         const size_t total = per_tree * 2 * (size_t)cfg->max_active;
