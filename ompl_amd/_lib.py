@@ -171,6 +171,13 @@ def load(path: str | os.PathLike | None = None):
     if _lib is not None and path is None:
         return _lib
     p = Path(path) if path else LIB_PATH
+    # torch ships its own HIP runtime under the same soname as ROCm's: load it
+    # first (when present) so that this library and torch share one runtime,
+    # whichever module the program imports first
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not p.exists():
         raise OmplAmdError(-8, f"{p} is not built; run `python -m ompl_amd.build` (hipcc, gfx950)")
     lib = C.CDLL(str(p))

@@ -756,14 +756,15 @@ int ompl_amd_server_destroy(ompl_amd_server h)
         S.cv.notify_all();
         if (S.th.joinable()) S.th.join();
         for (auto &kv : S.reqs) delete kv.second;
-        hipFree(S.pool);
-        hipFree(S.cmd_dev);
-        hipFree(S.res_dev);
-        hipFree(S.edges_dev);
-        hipFree(S.max_pos_dev);
-        hipHostFree(S.cmd_host);
-        hipHostFree(S.res_host);
-        hipStreamDestroy(S.stream);
+        // teardown: release everything even if one release reports an error
+        (void)hipFree(S.pool);
+        (void)hipFree(S.cmd_dev);
+        (void)hipFree(S.res_dev);
+        (void)hipFree(S.edges_dev);
+        (void)hipFree(S.max_pos_dev);
+        (void)hipHostFree(S.cmd_host);
+        (void)hipHostFree(S.res_host);
+        (void)hipStreamDestroy(S.stream);
         delete h;
     });
 }
