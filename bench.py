@@ -111,7 +111,7 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     if profile_ticks:
         n_ticks, tick_ms = oa.profile.query("serve_tick")
         tick_parts = {}
-        for part in ("serve_prepare", "serve_check"):
+        for part in ("serve_prepare", "serve_check", "serve_commit"):
             n_p, ms_p = oa.profile.query(part)
             tick_parts[part] = 1e3 * ms_p / n_p if n_p else None
         oa.profile.enable(False)

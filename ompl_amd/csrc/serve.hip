@@ -6,12 +6,11 @@
 // states + parent indices).  A server thread runs ticks: every tick each
 // active request contributes exactly one growTree step -- nearest neighbour in
 // its tree, steering by the planner range, motion check in the direction of
-// growth, append -- executed for all requests by two launches: nearest +
-// steer (one block per request), then the motion checks of every request
+// growth, append -- executed for all requests by three launches: nearest +
+// steer (one 256-thread block per request), the motion checks of every request
 // chunk-parallel over the whole GPU (so one long valid motion in a dense
-// environment does not hold the tick for a single wave), whose last block
-// appends the new nodes and writes the outcomes.  The host then advances each
-// request's RRT-Connect state machine.
+// environment does not hold the tick for its single wave), and commit.  The
+// host then advances each request's RRT-Connect state machine.
 // New requests are admitted between ticks while their HBM footprint fits the
 // budget (continuous batching); a request's result does not depend on which
 // other requests share its ticks.
@@ -77,8 +76,7 @@ struct TickArgs {
     const GrowCmd *cmds;
     GrowEdge *edges;
     GrowRes *res;
-    int32_t *max_pos;     // largest npos of the tick (reset by the check kernel's last block)
-    int32_t *done_blocks; // check kernel blocks finished (reset by its last block)
+    int32_t *max_pos; // largest npos of the tick (reset by the commit kernel)
     int ncmd;
 };
 
@@ -233,41 +231,30 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
             if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);
         }
     }
-    // The last block to finish commits the tick: one thread per request
-    // appends the new node of a valid motion and writes the outcome; then the
-    // tick counters are cleared.
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(T.done_blocks, 1) == (int)gridDim.x - 1;
+}
+
+// Tick kernel 3 (one thread per request): append the new node of a valid
+// motion, write the outcome, clear the tick counter.
+__global__ void grow_commit_kernel(TickArgs T)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) *T.max_pos = 0;
+    if (r >= T.ncmd) return;
+    const GrowCmd &C = T.cmds[r];
+    const GrowEdge &E = T.edges[r];
+    GrowRes &R = T.res[r];
+    if (E.npos == 0 || E.first_bad != INT_MAX) {
+        R.status = kTrapped;
+        return;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-        const GrowCmd &C = T.cmds[r];
-        const GrowEdge &E = T.edges[r];
-        GrowRes &R = T.res[r];
-        const int fb = __hip_atomic_load(&E.first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (E.npos == 0 || fb != INT_MAX) {
-            R.status = kTrapped;
-            continue;
-        }
-        const int idx = C.tree_n;
-        for (int k = 0; k < kDof; ++k) {
-            C.tree_s[(int64_t)idx * kDof + k] = E.dst[k];
-            R.xstate[k] = E.dst[k];
-        }
-        C.tree_parent[idx] = E.nm;
-        R.status = E.reach ? kReached : kAdvanced;
-        R.xmotion = idx;
+    const int idx = C.tree_n;
+    for (int k = 0; k < kDof; ++k) {
+        C.tree_s[(int64_t)idx * kDof + k] = E.dst[k];
+        R.xstate[k] = E.dst[k];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *T.max_pos = 0;
-        *T.done_blocks = 0;
-    }
+    C.tree_parent[idx] = E.nm;
+    R.status = E.reach ? kReached : kAdvanced;
+    R.xmotion = idx;
 }
 
 // ---------------------------------------------------------------------------
@@ -546,7 +533,7 @@ void Server::run()
             OA_HIP(hipMemcpyAsync(cmd_dev, cmd_host, n * sizeof(GrowCmd), hipMemcpyHostToDevice, stream));
             {
                 ProfScope ps("serve_tick", stream);
-                TickArgs T{cmd_dev, edges_dev, res_dev, max_pos_dev, max_pos_dev + 1, n};
+                TickArgs T{cmd_dev, edges_dev, res_dev, max_pos_dev, n};
                 {
                     ProfScope p1("serve_prepare", stream);
                     hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(kPrepThreads), 0, stream, T);
@@ -560,7 +547,10 @@ void Server::run()
                     else
                         hipLaunchKernelGGL((grow_check_kernel<8, 8>), dim3(check_blocks), dim3(256), 0, stream, T);
                 }
-
+                {
+                    ProfScope p3("serve_commit", stream);
+                    hipLaunchKernelGGL(grow_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, T);
+                }
                 OA_HIP(hipGetLastError());
             }
             OA_HIP(hipMemcpyAsync(res_host, res_dev, n * sizeof(GrowRes), hipMemcpyDeviceToHost, stream));
@@ -736,8 +726,8 @@ int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *o
         OA_HIP(hipMalloc(&S.cmd_dev, sizeof(GrowCmd) * cfg->max_active));
         OA_HIP(hipMalloc(&S.res_dev, sizeof(GrowRes) * cfg->max_active));
         OA_HIP(hipMalloc(&S.edges_dev, sizeof(GrowEdge) * cfg->max_active));
-        OA_HIP(hipMalloc(&S.max_pos_dev, 2 * sizeof(int32_t))); // [0] max positions, [1] finished check blocks
-        OA_HIP(hipMemset(S.max_pos_dev, 0, 2 * sizeof(int32_t)));
+        OA_HIP(hipMalloc(&S.max_pos_dev, sizeof(int32_t)));
+        OA_HIP(hipMemset(S.max_pos_dev, 0, sizeof(int32_t)));
         S.check_blocks = (unsigned)num_cus() * 2; // 8 waves per CU
         S.th = std::thread([&S] { S.run(); });
         *out = h;
