@@ -670,7 +670,7 @@ def main(argv=None):
         "served_stream": stream_res,
         "prm_batch": prm_res,
         "rows": rows_res,
-        "wall_s": wall,
+        "timed_region_wall_s": wall,
     }
     print(json.dumps(res))
     return 0
