@@ -162,7 +162,11 @@ int ompl_amd_motion_graph_destroy(ompl_amd_motion_graph graph);
 /* Launch-shape knobs used by the batched kernels (set by the autotuner).
  * A state is checked by lanes_per_state x obstacle_parts lanes: the first
  * splits the arm's 28 spheres, the second the obstacle list.  Built shapes
- * (lanes, parts): (1,1) (2,1) (4,1) (8,1) (8,2) (8,4) (8,8); block in
+ * (lanes, parts): (1,1) (2,1) (4,1) (8,1) (8,2) (8,4) (8,8), and (7,9), the
+ * link layout: one state per wave, each of 7 lanes holds one link's four
+ * spheres and tests the link's bounding sphere against an obstacle before
+ * the exact sphere tests (same results; environments whose staging would
+ * exceed 64 KiB of LDS with the extra sphere radii run as (8,8)); block in
  * {64,128,256,512,1024}. */
 typedef struct {
     int validity_lanes_per_state; /* validity and s2 (endpoint) checks */

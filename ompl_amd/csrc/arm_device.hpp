@@ -61,12 +61,44 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
-// Cooperative copy of the environment into LDS by the whole block.
+// Link layout (lanes_per_state 7): bounding sphere of one link's four robot
+// spheres, centred at p + Z*0.15 (the middle of the sphere offsets), radius
+// kLinkReach + r with kLinkReach = 0.2625 - 0.15 the largest offset from that
+// centre.  A sphere obstacle (o, rr = (R + r)^2) can touch a sphere of the
+// link only if |o - centre| < sqrt(rr) + kLinkReach; the staged bound widens
+// that by kBoundAbs + kBoundRel * sqrt(rr) (and a relative 1e-6 on the
+// square), far above the FP32 rounding of the centres and distances (a few
+// 1e-7 m for a 2.1 m arm), so a pair the bound rejects is never one the exact
+// test accepts: culling changes no result.
+constexpr float kLinkCentre = 0.15f;
+constexpr float kLinkReach = 0.1125f;
+constexpr float kBoundAbs = 1e-4f;
+constexpr float kBoundRel = 1e-5f;
+// box: distance from the centre to the box below kLinkReach + r + kBoundAbs
+constexpr float kBoxBound = kLinkReach + kRobotR + kBoundAbs;
+
+__device__ __forceinline__ float sphere_bound2(float rr)
+{
+    const float b = sqrtf(rr) * (1.0f + kBoundRel) + (kLinkReach + kBoundAbs);
+    return b * b * 1.000001f;
+}
+
+// Cooperative copy of the environment into LDS by the whole block.  With
+// LINKS (link layout) the spheres are staged as (x, y, z, bound^2) and their
+// exact rr follow the boxes as a float array.
+template <bool LINKS = false>
 __device__ __forceinline__ void stage_env(const EnvView &e, float4 *lds)
 {
     const int nw = env_words(e);
-    for (int i = threadIdx.x; i < nw; i += blockDim.x)
-        lds[i] = e.obs[i];
+    float *rr = reinterpret_cast<float *>(lds + nw);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+        float4 o = e.obs[i];
+        if (LINKS && i < e.ns) {
+            rr[i] = o.w;
+            o.w = sphere_bound2(o.w);
+        }
+        lds[i] = o;
+    }
     __syncthreads();
 }
 
@@ -79,13 +111,87 @@ struct LaneSpheres {
     float x[N], y[N], z[N];
 };
 
+// Link layout: the lane holds the four spheres of one link (slot m = sphere
+// 4g + m of link g) and the centre of their bounding sphere.
+template <>
+struct LaneSpheres<7> {
+    static constexpr int N = 4;
+    float x[N], y[N], z[N];
+    float cx, cy, cz;
+};
+
+// FK of the link layout (one state per wave): lane i < 7 evaluates sincos of
+// joint i once and every lane reads the seven results; the rotation chain is
+// the one arm_fk_slots evaluates, and the lane keeps the frame (p, Z) of its
+// link g, so its sphere centres are the same bits as arm_fk_slots's.
+__device__ __forceinline__ void arm_fk_links(const float q[kDof], int g, LaneSpheres<7> &S)
+{
+    const int j = threadIdx.x & 7;
+    float qj = q[0];
+#pragma unroll
+    for (int i = 1; i < kDof; ++i) {
+        qj = (j == i) ? q[i] : qj;
+        asm volatile("" : "+v"(qj)); // keep the selects (not an indexed private-memory load)
+    }
+    float sm, cm;
+    sincos_pl(qj, sm, cm);
+    float sj[kDof], cj[kDof];
+#pragma unroll
+    for (int i = 0; i < kDof; ++i) {
+        sj[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sm), i));
+        cj[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cm), i));
+    }
+    float X0 = 1.0f, X1 = 0.0f, X2 = 0.0f;
+    float Y0 = 0.0f, Y1 = 1.0f, Y2 = 0.0f;
+    float Z0 = 0.0f, Z1 = 0.0f, Z2 = 1.0f;
+    float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+    float P0 = 0.0f, P1 = 0.0f, P2 = 0.0f, W0 = 0.0f, W1 = 0.0f, W2 = 1.0f;
+#pragma unroll
+    for (int i = 0; i < kDof; ++i) {
+        const float s = sj[i], c = cj[i];
+        if ((i & 1) == 0) {
+            float nx0 = X0 * c + Y0 * s, ny0 = Y0 * c - X0 * s;
+            float nx1 = X1 * c + Y1 * s, ny1 = Y1 * c - X1 * s;
+            float nx2 = X2 * c + Y2 * s, ny2 = Y2 * c - X2 * s;
+            X0 = nx0; X1 = nx1; X2 = nx2;
+            Y0 = ny0; Y1 = ny1; Y2 = ny2;
+        } else {
+            float nx0 = X0 * c - Z0 * s, nz0 = X0 * s + Z0 * c;
+            float nx1 = X1 * c - Z1 * s, nz1 = X1 * s + Z1 * c;
+            float nx2 = X2 * c - Z2 * s, nz2 = X2 * s + Z2 * c;
+            X0 = nx0; X1 = nx1; X2 = nx2;
+            Z0 = nz0; Z1 = nz1; Z2 = nz2;
+        }
+        const bool mine = g == i;
+        P0 = mine ? p0 : P0;
+        P1 = mine ? p1 : P1;
+        P2 = mine ? p2 : P2;
+        W0 = mine ? Z0 : W0;
+        W1 = mine ? Z1 : W1;
+        W2 = mine ? Z2 : W2;
+        p0 = p0 + Z0 * kLink;
+        p1 = p1 + Z1 * kLink;
+        p2 = p2 + Z2 * kLink;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const float off = sphere_offset(m);
+        S.x[m] = P0 + W0 * off;
+        S.y[m] = P1 + W1 * off;
+        S.z[m] = P2 + W2 * off;
+    }
+    S.cx = P0 + W0 * kLinkCentre;
+    S.cy = P1 + W1 * kLinkCentre;
+    S.cz = P2 + W2 * kLinkCentre;
+}
+
 // With LPS = 8 the eight lanes of an aligned lane octet always hold the same
 // state (every layout puts a state's sphere slots on lanes 8k .. 8k+7), so
 // lane 8k + j evaluates sincos of joint j once and the octet shares the seven
 // results by shuffles instead of each lane evaluating all seven; the values
 // are the same bits either way.
 template <int LPS>
-__device__ __forceinline__ void arm_fk(const float q[kDof], int g, LaneSpheres<LPS> &S)
+__device__ __forceinline__ void arm_fk_slots(const float q[kDof], int g, LaneSpheres<LPS> &S)
 {
     float sj[kDof], cj[kDof];
     if constexpr (LPS == 8) {
@@ -148,6 +254,15 @@ __device__ __forceinline__ void arm_fk(const float q[kDof], int g, LaneSpheres<L
         p1 = p1 + Z1 * kLink;
         p2 = p2 + Z2 * kLink;
     }
+}
+
+template <int LPS>
+__device__ __forceinline__ void arm_fk(const float q[kDof], int g, LaneSpheres<LPS> &S)
+{
+    if constexpr (LPS == 7)
+        arm_fk_links(q, g, S);
+    else
+        arm_fk_slots<LPS>(q, g, S);
 }
 
 template <int LPS>
@@ -218,7 +333,9 @@ enum class Exit { kAllGroups, kAnyGroup, kLowestGroup };
 // part, part + OPS, part + 2*OPS, ...).
 template <int LPS, int OPS>
 struct GroupShape {
-    static constexpr int GS = LPS * OPS;    // lanes per state
+    // lanes per state; the link layout (LPS 7) takes the whole wave for one
+    // state, lanes with part >= OPS idle
+    static constexpr int GS = LPS == 7 ? 64 : LPS * OPS;
     static constexpr int STATES = 64 / GS;  // states per wave
     int slot, part, group;
     __device__ __forceinline__ explicit GroupShape(int lane)
@@ -255,10 +372,74 @@ __device__ __forceinline__ void spheres_from_fk(const LaneSpheres<LPS> &F, int t
 // partition per iteration, indices clamped to the last obstacle, whose
 // repeated test is harmless) and ends each iteration with a ballot.  Returns
 // this lane's hit flag (the state's result is the OR over its group).
+// Link layout collision (one state per wave): the lane tests its link's
+// bounding sphere against K obstacles of its partition (obstacles part,
+// part + OPS, ...) per pass, then runs the exact four-sphere test on the
+// obstacles whose bound it touches, one per lane per round until no lane has
+// one left; the wave leaves after the first pass with a hit.  LDS holds the
+// link-layout staging of stage_env<true>.
+template <int OPS>
+__device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const float4 *lds, int ns, int nb, bool live,
+                                              int part)
+{
+    constexpr int K = 8;
+    const bool act = live && part < OPS;
+    bool hit = false;
+    if (!wave_ballot(act)) return false;
+    const float *rr = reinterpret_cast<const float *>(lds + ns + 2 * nb);
+    for (int ob = 0; ob < ns; ob += K * OPS) {
+        unsigned cand = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const float4 o = lds[min(ob + part + j * OPS, ns - 1)];
+            const float dx = o.x - S.cx, dy = o.y - S.cy, dz = o.z - S.cz;
+            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            cand |= (unsigned)(d2 < o.w) << j;
+        }
+        if (!act) cand = 0;
+        while (wave_ballot(cand != 0)) {
+            if (cand) {
+                const int o = min(ob + part + __builtin_ctz(cand) * OPS, ns - 1);
+                cand &= cand - 1;
+                float4 e = lds[o];
+                e.w = rr[o];
+                hit |= hits_sphere<7>(S, e);
+            }
+        }
+        if (wave_ballot(hit)) return hit;
+    }
+    const float4 *boxes = lds + ns;
+    for (int ob = 0; ob < nb; ob += K * OPS) {
+        unsigned cand = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int o = min(ob + part + j * OPS, nb - 1);
+            const float4 lo = boxes[2 * o], hi = boxes[2 * o + 1];
+            const float px = fminf(fmaxf(S.cx, lo.x), hi.x);
+            const float py = fminf(fmaxf(S.cy, lo.y), hi.y);
+            const float pz = fminf(fmaxf(S.cz, lo.z), hi.z);
+            const float dx = S.cx - px, dy = S.cy - py, dz = S.cz - pz;
+            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            cand |= (unsigned)(d2 < kBoxBound * kBoxBound) << j;
+        }
+        if (!act) cand = 0;
+        while (wave_ballot(cand != 0)) {
+            if (cand) {
+                const int o = min(ob + part + __builtin_ctz(cand) * OPS, nb - 1);
+                cand &= cand - 1;
+                hit |= hits_box<7>(S, boxes[2 * o], boxes[2 * o + 1]);
+            }
+        }
+        if (wave_ballot(hit)) return hit;
+    }
+    return hit;
+}
+
 template <int LPS, int OPS, Exit E>
 __device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live,
                                             int part)
 {
+    if constexpr (LPS == 7) return collide_links<OPS>(S, lds, ns, nb, live, part);
     constexpr int GS = LPS * OPS;
     bool hit = false;
     const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));

@@ -6,7 +6,9 @@
 // p+OPS, ...).  Obstacles are staged once per block in LDS and read as float4
 // broadcasts; every obstacle iteration ends with a ballot so a wave leaves the
 // loop as soon as its exit policy is met.  Shapes (LPS, OPS) built:
-// (1,1) (2,1) (4,1) (8,1) (8,2) (8,4) (8,8).
+// (1,1) (2,1) (4,1) (8,1) (8,2) (8,4) (8,8), and the link layout (7,9): one
+// state per wave, lane l holds link l % 7 and obstacle partition l / 7, and
+// tests its link's bounding sphere before the exact tests of its spheres.
 //
 //   validate_kernel         batched StateValidityChecker::isValid
 //   motion_endpoint_kernel  s2 of every edge (DiscreteMotionValidator.cpp:96);
@@ -36,7 +38,7 @@
 namespace ompl_amd {
 
 // (LPS, OPS) shapes instantiated for every batched kernel.
-#define OA_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(8, 2) X(8, 4) X(8, 8)
+#define OA_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(7, 9)
 // shapes with more than one obstacle partition: the shared-FK kernels
 #define OA_SHARED_SHAPES(X) X(8, 2) X(8, 4) X(8, 8)
 
@@ -54,7 +56,7 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
                                                         uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
-    stage_env(env, lds);
+    stage_env<LPS == 7>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
     const Shape sh(lane);
@@ -81,7 +83,7 @@ __global__ void __launch_bounds__(1024) validate_shared_kernel(EnvView env, cons
                                                                uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
-    stage_env(env, lds);
+    stage_env<LPS == 7>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
@@ -191,7 +193,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env(A.env, lds);
+    stage_env<LPS == 7>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int64_t wave = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t e = wave * Shape::STATES + lane_id() / Shape::GS;
@@ -207,7 +209,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_shared_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env(A.env, lds);
+    stage_env<LPS == 7>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
@@ -238,7 +240,7 @@ template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env(A.env, lds);
+    stage_env<LPS == 7>(A.env, lds);
     const int lane = lane_id();
     const int waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t count = A.list ? (int64_t)(*A.list_count) : A.n;
@@ -270,17 +272,114 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
     }
 }
 
-// Chunk-parallel walk of the survivors.  Virtual work items k = (chunk blk,
-// survivor s) in chunk-major order, one wave per item over a persistent grid
-// (wave w takes items w, w + W, w + 2W, ...).  A wave tests the static
-// liveness of its next 64 items at once (one per lane: is the chunk inside
-// the edge?), then walks the live ones in item order, each after re-reading
-// whether an earlier position of its edge is already known invalid.
+// Largest survivor count for which the chunk walk numbers the survivors'
+// deep chunks edge-major (their chunk-count offsets are scanned into LDS).
+constexpr int kFlatSurvivors = 4096;
+
+// Bytes of dynamic LDS the staged environment takes (stage_env<LINKS>).
+template <bool LINKS>
+__device__ __forceinline__ size_t env_lds_bytes(const EnvView &e)
+{
+    return (size_t)(e.ns + 2 * e.nb) * sizeof(float4) + (LINKS ? (size_t)e.ns * sizeof(float) : 0);
+}
+
+// off[0] = 0, off[i + 1] = sum over survivors t <= i of their chunks past
+// the first L0, max(0, ceil((nd_t - 1) / G) - L0), counting only survivors
+// with no invalid position found yet; computed by the whole block (each
+// thread a contiguous run of survivors, then a scan of the run totals by the
+// first wave).
+template <int G>
+__device__ __forceinline__ void scan_remaining_chunks(const int32_t *__restrict__ nd,
+                                                      const int32_t *__restrict__ first_bad, int n, int64_t L0,
+                                                      int32_t *off)
+{
+    __shared__ int32_t part[1024];
+    const int T = blockDim.x, t = threadIdx.x;
+    const int per = (n + T - 1) / T;
+    const int lo = min(n, t * per), hi = min(n, lo + per);
+    int sum = 0;
+    for (int i = lo; i < hi; ++i) {
+        const int64_t chunks = (nd[i] - 1 + G - 1) / G;
+        const int len = first_bad[i] == INT_MAX ? (int)max((int64_t)0, chunks - L0) : 0;
+        off[i + 1] = len;
+        sum += len;
+    }
+    part[t] = sum;
+    __syncthreads();
+    if (t < 64) {
+        int carry = 0;
+        for (int c0 = 0; c0 < T; c0 += 64) {
+            int v = part[c0 + t];
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int u = __shfl_up(v, d);
+                if (t >= d) v += u;
+            }
+            part[c0 + t] = v + carry;
+            carry += __shfl(v, 63);
+        }
+    }
+    __syncthreads();
+    int run = t > 0 ? part[t - 1] : 0;
+    if (t == 0) off[0] = 0;
+    for (int i = lo; i < hi; ++i) {
+        run += off[i + 1];
+        off[i + 1] = run;
+    }
+    __syncthreads();
+}
+
+// One work item of the chunk walk: positions [bj, bj + G) of survivor sj,
+// skipped when an earlier position of the edge is already known invalid.
+template <int LPS, int OPS, bool EXACT, bool SHARED>
+__device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds, int sj, int bj, int ndj)
+{
+    if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) return;
+    const double *ab = A.surv_ab + (int64_t)sj * 2 * kDof;
+    double a[kDof], b[kDof];
+#pragma unroll
+    for (int d = 0; d < kDof; ++d) {
+        a[d] = ab[d];
+        b[d] = ab[kDof + d];
+    }
+    const int bad = SHARED ? walk_chunk_shared<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb)
+                           : walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
+    if (bad != INT_MAX && lane_id() == 0) atomicMin(&A.first_bad[sj], bad);
+}
+
+// Walks the live items of a 64-item window (one per lane) in lane order.
+template <int LPS, int OPS, bool EXACT, bool SHARED>
+__device__ __forceinline__ void walk_window(const MotionArgs &A, const float4 *lds, int s, int base, int nd, bool live)
+{
+    uint64_t mask = wave_ballot(live);
+    while (mask) {
+        const int j = lowest_bit(mask);
+        mask &= mask - 1;
+        walk_item<LPS, OPS, EXACT, SHARED>(A, lds, readlane_i32(s, j), readlane_i32(base, j), readlane_i32(nd, j));
+    }
+}
+
+// Chunk-parallel walk of the survivors in two launches (A.phase 1, then 2),
+// one wave per work item (G positions of one survivor) over a persistent grid
+// of W waves, wave w taking items w, w + W, w + 2W, ... of its phase; a wave
+// tests the liveness of its next 64 items at once (one per lane) and walks
+// the live ones in item order.
+//   Phase 1: the first L0 chunks of every survivor in chunk-major order
+//     (k = blk * n_surv + s), so that the early bisection positions, where
+//     most edges that fail do fail, run first across the whole grid.
+//   Phase 2 (survivors <= kFlatSurvivors, with the LDS reserved by the
+//     launch): the remaining chunks of the survivors that phase 1 left
+//     unresolved, numbered edge-major (survivor s owns numbers off[s] ..
+//     off[s+1]-1), so the waves share them evenly; a lane finds its item's
+//     survivor by binary search in the offsets.  L0 = W / n_surv (at least
+//     1): phase 1 is one item per wave.
+//   Without phase 2, phase 1 covers every chunk (L0 = all).
+// The last block of phase 2 writes the survivors' results.
 template <int LPS, int OPS, bool EXACT, bool SHARED>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env(A.env, lds);
+    stage_env<LPS == 7>(A.env, lds);
     constexpr int G = SHARED ? 64 / LPS : GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
@@ -288,34 +387,45 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
     const int64_t n_surv = *A.list_count;
     if (n_surv == 0) return;
     const int64_t nblk = ((int64_t)*A.max_m + G - 1) / G;
-    const int64_t items = n_surv * nblk;
-    for (int64_t k0 = wave; k0 < items; k0 += 64 * waves_total) {
-        const int64_t k = k0 + lane * waves_total;
-        int s = 0, base = 0, nd = 0;
-        bool live = false;
-        if (k < items) {
-            const int64_t blk = k / n_surv;
-            s = (int)(k - blk * n_surv);
-            base = 1 + (int)(blk * G);
-            nd = A.surv_nd[s];
-            live = base < nd;
-        }
-        uint64_t mask = wave_ballot(live);
-        while (mask) {
-            const int j = lowest_bit(mask);
-            mask &= mask - 1;
-            const int sj = __shfl(s, j), bj = __shfl(base, j), ndj = __shfl(nd, j);
-            if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) continue;
-            const double *ab = A.surv_ab + (int64_t)sj * 2 * kDof;
-            double a[kDof], b[kDof];
-#pragma unroll
-            for (int d = 0; d < kDof; ++d) {
-                a[d] = ab[d];
-                b[d] = ab[kDof + d];
+    const bool flat = A.flat && n_surv <= kFlatSurvivors && n_surv * nblk < (int64_t)INT_MAX;
+    const int64_t L0 = flat ? max((int64_t)1, min(nblk, waves_total / n_surv)) : nblk;
+    if (A.phase == 1) {
+        const int64_t items1 = n_surv * L0;
+        for (int64_t k0 = wave; k0 < items1; k0 += 64 * waves_total) {
+            const int64_t k = k0 + lane * waves_total;
+            int s = 0, base = 0, nd = 0;
+            bool live = false;
+            if (k < items1) {
+                const int64_t blk = k / n_surv;
+                s = (int)(k - blk * n_surv);
+                base = 1 + (int)(blk * G);
+                nd = A.surv_nd[s];
+                live = base < nd;
             }
-            const int bad = SHARED ? walk_chunk_shared<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb)
-                                   : walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
-            if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[sj], bad);
+            walk_window<LPS, OPS, EXACT, SHARED>(A, lds, s, base, nd, live);
+        }
+        return;
+    }
+    if (flat) {
+        int32_t *off = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(lds) + env_lds_bytes<LPS == 7>(A.env));
+        scan_remaining_chunks<G>(A.surv_nd, A.first_bad, (int)n_surv, L0, off);
+        const int64_t R = off[n_surv];
+        for (int64_t r0 = wave; r0 < R; r0 += 64 * waves_total) {
+            const int64_t r = r0 + lane * waves_total;
+            int s = 0, base = 0, nd = 0;
+            const bool live = r < R;
+            if (live) {
+                int lo = 0, hi = (int)n_surv; // off[lo] <= r < off[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (off[mid] <= r) lo = mid;
+                    else hi = mid;
+                }
+                s = lo;
+                base = 1 + (int)((L0 + (r - off[lo])) * G);
+                nd = A.surv_nd[s];
+            }
+            walk_window<LPS, OPS, EXACT, SHARED>(A, lds, s, base, nd, live);
         }
     }
     if (!A.done_blocks) return;
@@ -383,7 +493,7 @@ __global__ void __launch_bounds__(64) motion_single_kernel(EnvView env, Edge1 E,
                                                            int *result)
 {
     extern __shared__ float4 lds[];
-    stage_env(env, lds);
+    stage_env<LPS == 7>(env, lds);
     int total = 0;
     const int bad = walk_edge<LPS, OPS, false>(E.a, E.b, lvs, factor, lds, env.ns, env.nb, 0, total);
     if (lane_id() == 0) *result = bad == INT_MAX ? 1 : 0; // bad < 0: too many segments, invalid
@@ -421,7 +531,7 @@ __global__ void __launch_bounds__(256) state_lists_kernel(EnvView env, const dou
                                                           int32_t *__restrict__ first_invalid)
 {
     extern __shared__ float4 lds[];
-    stage_env(env, lds);
+    stage_env<LPS == 7>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int G = Shape::STATES;
     const int lane = lane_id();
@@ -457,7 +567,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, double *__restrict__ last_t)
 {
     extern __shared__ float4 lds[];
-    stage_env(A.env, lds);
+    stage_env<LPS == 7>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int G = Shape::STATES;
     const int lane = lane_id();
@@ -539,9 +649,21 @@ __global__ void arm_spheres_kernel(const float *__restrict__ q_soa, int64_t n, f
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-static size_t lds_bytes(const EnvView &e)
+// Dynamic LDS of a launch: the staged environment, plus the spheres' exact rr
+// for the link layout (lanes_per_state 7).
+static size_t lds_bytes(const EnvView &e, int lps = 0)
 {
-    return (size_t)(e.ns + 2 * e.nb) * sizeof(float4);
+    return (size_t)(e.ns + 2 * e.nb) * sizeof(float4) + (lps == 7 ? (size_t)e.ns * sizeof(float) : 0);
+}
+
+// Where the link layout's staging exceeds one block's 64 KiB of LDS, the
+// (8, 8) shape checks the states instead (same results).
+static void fit_link_layout(const EnvView &e, int &lps, int &ops)
+{
+    if (lps == 7 && lds_bytes(e, 7) > 65536) {
+        lps = 8;
+        ops = 8;
+    }
 }
 
 static unsigned grid_for(int64_t units, int64_t units_per_block)
@@ -554,12 +676,13 @@ void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, in
                      hipStream_t s, bool shared)
 {
     if (n <= 0) return;
+    fit_link_layout(env, lps, ops);
     ProfScope ps("validate_states", s);
     if (shared && ops > 1) {
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * (64 / L);                                                     \
-        hipLaunchKernelGGL((validate_shared_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env), \
+        hipLaunchKernelGGL((validate_shared_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env, L), \
                            s, env, q, n, out);                                                                      \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
@@ -570,7 +693,7 @@ void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, in
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
-        hipLaunchKernelGGL((validate_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env), s, env, \
+        hipLaunchKernelGGL((validate_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env, L), s, env, \
                            q, n, out);                                                                              \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
@@ -593,13 +716,14 @@ void launch_zero(int *p, int n, hipStream_t s)
 void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hipStream_t s, bool shared)
 {
     if (A.n <= 0) return;
+    fit_link_layout(A.env, lps, ops);
     ProfScope ps("motion_endpoint", s);
     if (shared && ops > 1) {
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * (64 / L);                                                     \
         hipLaunchKernelGGL((motion_endpoint_shared_kernel<L, O>), dim3(grid_for(A.n, per)), dim3(block),          \
-                           lds_bytes(A.env), s, A);                                                                \
+                           lds_bytes(A.env, L), s, A);                                                                \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -610,7 +734,7 @@ void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hi
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
         hipLaunchKernelGGL((motion_endpoint_kernel<L, O>), dim3(grid_for(A.n, per)), dim3(block),                 \
-                           lds_bytes(A.env), s, A);                                                                \
+                           lds_bytes(A.env, L), s, A);                                                                \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -622,14 +746,15 @@ void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hi
 void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves, hipStream_t s)
 {
     if (waves <= 0) return;
+    fit_link_layout(A.env, lps, ops);
     ProfScope ps("motion_walk", s);
     const unsigned g = grid_for(waves, block / 64);
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         if (exact)                                                                                                  \
-            hipLaunchKernelGGL((motion_walk_kernel<L, O, true>), dim3(g), dim3(block), lds_bytes(A.env), s, A);   \
+            hipLaunchKernelGGL((motion_walk_kernel<L, O, true>), dim3(g), dim3(block), lds_bytes(A.env, L), s, A);   \
         else                                                                                                        \
-            hipLaunchKernelGGL((motion_walk_kernel<L, O, false>), dim3(g), dim3(block), lds_bytes(A.env), s, A);  \
+            hipLaunchKernelGGL((motion_walk_kernel<L, O, false>), dim3(g), dim3(block), lds_bytes(A.env, L), s, A);  \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -641,17 +766,27 @@ void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int b
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
                           hipStream_t s, bool shared)
 {
+    fit_link_layout(A.env, lps, ops);
     ProfScope ps("motion_chunk", s);
     const unsigned g = grid_for(waves, block / 64);
+    // LDS after the environment for the survivors' chunk offsets (edge-major
+    // phase), where it fits beside the kernel's 4 KiB of static LDS
+    MotionArgs B = A;
+    const size_t env_b = lds_bytes(A.env, lps), flat_b = (size_t)(kFlatSurvivors + 1) * sizeof(int32_t);
+    B.flat = env_b + flat_b + 4096 + 64 <= 65536;
+    const size_t dyn = env_b + (B.flat ? flat_b : 0);
+    MotionArgs B1 = B;
+    B1.phase = 1;
+    B.phase = 2;
     if (shared && ops > 1) {
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
-        if (exact)                                                                                                  \
-            hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, true>), dim3(g), dim3(block), lds_bytes(A.env), s, \
-                               A);                                                                                  \
-        else                                                                                                        \
-            hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, true>), dim3(g), dim3(block), lds_bytes(A.env),   \
-                               s, A);                                                                               \
+        for (const MotionArgs *P : {&B1, &B}) {                                                                   \
+            if (exact)                                                                                              \
+                hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, true>), dim3(g), dim3(block), dyn, s, *P);     \
+            else                                                                                                    \
+                hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, true>), dim3(g), dim3(block), dyn, s, *P);    \
+        }                                                                                                           \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -660,12 +795,12 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
     }
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
-        if (exact)                                                                                                  \
-            hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, false>), dim3(g), dim3(block), lds_bytes(A.env), s, \
-                               A);                                                                                  \
-        else                                                                                                        \
-            hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, false>), dim3(g), dim3(block), lds_bytes(A.env),  \
-                               s, A);                                                                               \
+        for (const MotionArgs *P : {&B1, &B}) {                                                                   \
+            if (exact)                                                                                              \
+                hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, false>), dim3(g), dim3(block), dyn, s, *P);    \
+            else                                                                                                    \
+                hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, false>), dim3(g), dim3(block), dyn, s, *P);   \
+        }                                                                                                           \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -725,10 +860,11 @@ void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int 
                               hipStream_t s)
 {
     if (A.n <= 0) return;
+    fit_link_layout(A.env, lps, ops);
     const unsigned g = grid_for(waves, block / 64);
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
-        hipLaunchKernelGGL((motion_last_valid_kernel<L, O>), dim3(g), dim3(block), lds_bytes(A.env), s, A, last_t); \
+        hipLaunchKernelGGL((motion_last_valid_kernel<L, O>), dim3(g), dim3(block), lds_bytes(A.env, L), s, A, last_t); \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }

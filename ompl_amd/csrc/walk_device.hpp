@@ -24,6 +24,20 @@ __device__ __forceinline__ int lowest_bit(uint64_t m)
     return __builtin_ctzll(m);
 }
 
+// Value of v on lane j (j wave-uniform), as a wave-uniform value.
+__device__ __forceinline__ int readlane_i32(int v, int j)
+{
+    return __builtin_amdgcn_readlane(v, j);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int j)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // State of walk position p (0 = b, else bisection index p-1) as FP32.
 __device__ __forceinline__ void walk_state(const double a[kDof], const double b[kDof], int nd, int p, float q[kDof])
 {

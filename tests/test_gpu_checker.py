@@ -38,7 +38,7 @@ def test_fk_bit_exact(gpu, oracle):
         assert np.array_equal(got[i], oracle.arm_spheres(q[i])), i
 
 
-SHAPES = [(1, 1), (2, 1), (4, 1), (8, 1), (8, 2), (8, 4), (8, 8)]
+SHAPES = [(1, 1), (2, 1), (4, 1), (8, 1), (8, 2), (8, 4), (8, 8), (7, 9)]  # (7, 9): link layout with bound culling
 # (lanes, parts, shared FK)
 VSHAPES = [(lps, ops, 0) for lps, ops in SHAPES] + [(8, 2, 1), (8, 4, 1), (8, 8, 1)]
 
@@ -66,10 +66,85 @@ def test_validate_states_bit_exact(gpu, oracle, lps, ops, shared, ns, nb):
 def test_validate_ragged_sizes(gpu, oracle, n):
     env, oenv, _ = env_pair(gpu, oracle, 7, 32, 32)
     q = gpu.workload.configs(n, n).astype(np.float32).T.copy()
-    for lps, ops, shared in ((1, 1, 0), (8, 1, 0), (8, 8, 0), (8, 8, 1), (8, 2, 1)):
+    for lps, ops, shared in ((1, 1, 0), (8, 1, 0), (8, 8, 0), (8, 8, 1), (8, 2, 1), (7, 9, 0)):
         gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops, validity_shared_fk=shared)
         assert np.array_equal(gpu.validate_states(env, to_dev(q)).cpu().numpy(), oracle.validate_states(oenv, q))
     gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2, validity_shared_fk=0)
+
+
+def box_distance(p, lo, hi):
+    return float(np.linalg.norm(p - np.clip(p, lo, hi)))
+
+
+def near_contact_envs(oracle, n_envs, seed):
+    """Environments of one sphere and one box obstacle, each within ~1e-6 m of
+    touching one robot sphere of a base state q0 and clear (by > 1 mm) of the
+    other 27 (sphere obstacle: centre at distance R + r from the robot sphere;
+    box: a face at distance r from it), with 64 states q0 + 1e-6 noise per
+    environment, so that the exact FP32 tests fall on both sides of contact."""
+    rng = np.random.default_rng(seed)
+    made = 0
+    while made < n_envs:
+        q0 = rng.uniform(-PI, PI, 7)
+        sph = oracle.arm_spheres(q0.astype(np.float32)).astype(np.float64)
+        j, k = rng.integers(0, 28, 2)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        R = rng.uniform(0.01, 0.08)
+        o = sph[j] + d * (R + 0.05) * (1 + rng.uniform(-3e-7, 3e-7))
+        others = np.delete(np.arange(28), j)
+        if np.min(np.linalg.norm(sph[others] - o, axis=1)) < R + 0.05 + 1e-3:
+            continue
+        h = rng.uniform(0.01, 0.06, 3)
+        axis = rng.integers(0, 3)
+        c = sph[k].copy()
+        c[axis] += rng.choice([-1.0, 1.0]) * (h[axis] + 0.05 * (1 + rng.uniform(-3e-6, 3e-6)))
+        lo, hi = c - h, c + h
+        if min(box_distance(sph[i], lo, hi) for i in range(28) if i != k) < 0.05 + 1e-3:
+            continue
+        made += 1
+        spheres = np.array([[*o, R]], np.float32)
+        boxes = np.array([[*c, *h]], np.float32)
+        q = (q0[None, :] + rng.normal(size=(64, 7)) * 1e-6).astype(np.float32)
+        q[0] = q0
+        yield spheres, boxes, q
+
+
+@pytest.mark.parametrize("lps,ops", [(7, 9), (8, 8), (1, 1)])
+def test_validate_near_contact(gpu, oracle, lps, ops):
+    """Obstacles within ~1e-6 m of contact: every shape, the link layout with
+    its bound culling included, reproduces the oracle's exact FP32 decisions."""
+    gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops, validity_shared_fk=0)
+    n_valid = n_total = 0
+    for spheres, boxes, q in near_contact_envs(oracle, 60, 31):
+        env = gpu.Env(spheres, boxes)
+        oenv = oracle.OracleEnv(spheres, boxes)
+        qf = np.ascontiguousarray(q.T)
+        got = gpu.validate_states(env, to_dev(qf)).cpu().numpy()
+        ref = oracle.validate_states(oenv, qf)
+        assert np.array_equal(got, ref)
+        n_valid += int(ref.sum())
+        n_total += ref.size
+        env.close()
+    assert 0 < n_valid < n_total  # contact decided both ways
+    gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2, validity_shared_fk=0)
+
+
+def test_link_layout_large_env(gpu, oracle):
+    """An environment whose link-layout staging exceeds 64 KiB of LDS (4,000
+    spheres + 48 boxes) is checked with the (8, 8) shape instead: same results."""
+    gpu.set_tuning(validity_lanes_per_state=7, validity_obstacle_parts=9, motion_lanes_per_state=7,
+                   motion_obstacle_parts=9, motion_strategy=3)
+    env, oenv, _ = env_pair(gpu, oracle, 1005, 4000, 48)
+    q = gpu.workload.configs(3, 2048).astype(np.float32).T.copy()
+    ref = oracle.validate_states(oenv, q)
+    assert np.array_equal(gpu.validate_states(env, to_dev(q)).cpu().numpy(), ref)
+    a, b = motion_case(gpu, 6, 512)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs)
+    v, c = gpu.check_motions(env, to_dev(a.T.copy()), to_dev(b.T.copy()), lvs, exact_count=True)
+    assert np.array_equal(v.cpu().numpy(), ref_v) and np.array_equal(c.cpu().numpy(), ref_c)
+    gpu.set_tuning(**DEFAULT)
 
 
 def test_is_valid_single_call(gpu, oracle):
@@ -88,7 +163,9 @@ def motion_case(gpu, seed, n, mu=-1.0, sigma=1.0, lo=0.05, hi=5.0):
 STRATEGIES = [(3, (8, 2), (8, 1)), (3, (8, 8), (8, 8)), (3, (1, 1), (1, 1)), (3, (4, 1), (8, 4)),
               (1, (8, 1), (1, 1)), (1, (4, 1), (8, 2)), (0, (8, 1), (1, 1)), (0, (8, 1), (8, 8)),
               (2, (8, 1), (1, 1)), (2, (8, 2), (8, 8)), (2, (8, 4), (8, 1)),
-              (4, (8, 8), (8, 8)), (4, (8, 2), (8, 4)), (4, (8, 4), (8, 1)), (4, (1, 1), (8, 2))]
+              (4, (8, 8), (8, 8)), (4, (8, 2), (8, 4)), (4, (8, 4), (8, 1)), (4, (1, 1), (8, 2)),
+              (3, (7, 9), (7, 9)), (3, (8, 8), (7, 9)), (1, (7, 9), (7, 9)), (0, (8, 1), (7, 9)),
+              (4, (7, 9), (8, 8))]
 DEFAULT = dict(motion_strategy=3, validity_lanes_per_state=8, validity_obstacle_parts=2,
                motion_lanes_per_state=8, motion_obstacle_parts=1)
 
@@ -159,7 +236,7 @@ def test_check_motion_plain_single_call(gpu, oracle):
         assert gpu.check_motion(env, a[i], b[i], lvs) == oracle.check_motion(oenv, a[i], b[i], lvs)[0]
 
 
-@pytest.mark.parametrize("lps,ops", [(1, 1), (8, 1), (8, 8)])
+@pytest.mark.parametrize("lps,ops", [(1, 1), (8, 1), (8, 8), (7, 9)])
 def test_check_motions_last_valid(gpu, oracle, lps, ops):
     gpu.set_tuning(motion_lanes_per_state=lps, motion_obstacle_parts=ops)
     env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
