@@ -284,7 +284,7 @@ def scope_rows(oa, run_cpu, cpu_threads):
     out = torch.empty(65536, dtype=torch.uint8, device="cuda")
     shapes = {}
     saved = oa.get_tuning()
-    for lps, ops, shared in ((8, 1, 0), (8, 2, 0), (8, 8, 0), (8, 2, 1), (8, 4, 1), (8, 8, 1), (7, 9, 0), (1, 1, 0)):
+    for lps, ops, shared in ((8, 1, 0), (8, 2, 0), (8, 8, 0), (8, 2, 1), (8, 4, 1), (8, 8, 1), (7, 1, 0), (7, 2, 0), (7, 4, 0), (7, 9, 0), (1, 1, 0)):
         oa.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops, validity_shared_fk=shared)
         shapes[f"{lps}x{ops}{'-shared' if shared else ''}"] = timed(lambda: oa.validate_states(env, qd, out=out), 50)
     best = min(shapes, key=shapes.get)
@@ -512,7 +512,11 @@ def main(argv=None):
                      ("s3_e7x9_w7x9", dict(motion_strategy=3, validity_lanes_per_state=7, validity_obstacle_parts=9,
                                            motion_lanes_per_state=7, motion_obstacle_parts=9)),
                      ("s3_e8x8_w7x9", dict(motion_strategy=3, validity_obstacle_parts=8, motion_lanes_per_state=7,
-                                           motion_obstacle_parts=9))):
+                                           motion_obstacle_parts=9)),
+                     ("s3_e7x4_w7x9", dict(motion_strategy=3, validity_lanes_per_state=7, validity_obstacle_parts=4,
+                                           motion_lanes_per_state=7, motion_obstacle_parts=9)),
+                     ("s3_e7x2_w7x9", dict(motion_strategy=3, validity_lanes_per_state=7, validity_obstacle_parts=2,
+                                           motion_lanes_per_state=7, motion_obstacle_parts=9))):
         oa.set_tuning(**{**t, "validity_lanes_per_state": 8, "motion_lanes_per_state": 8, "validity_shared_fk": 0,
                          **kw})
         oa.profile.reset()

@@ -120,10 +120,13 @@ struct LaneSpheres<7> {
     float cx, cy, cz;
 };
 
-// FK of the link layout (one state per wave): lane i < 7 evaluates sincos of
-// joint i once and every lane reads the seven results; the rotation chain is
-// the one arm_fk_slots evaluates, and the lane keeps the frame (p, Z) of its
-// link g, so its sphere centres are the same bits as arm_fk_slots's.
+// FK of the link layout (GS lanes per state, GS >= 8 a power of two): lane
+// i < 7 of the state's group evaluates sincos of joint i once and the group
+// reads the seven results (readlane when the group is the whole wave); the
+// rotation chain is the one arm_fk_slots evaluates, and the lane keeps the
+// frame (p, Z) of its link g, so its sphere centres are the same bits as
+// arm_fk_slots's.
+template <int GS>
 __device__ __forceinline__ void arm_fk_links(const float q[kDof], int g, LaneSpheres<7> &S)
 {
     const int j = threadIdx.x & 7;
@@ -138,8 +141,14 @@ __device__ __forceinline__ void arm_fk_links(const float q[kDof], int g, LaneSph
     float sj[kDof], cj[kDof];
 #pragma unroll
     for (int i = 0; i < kDof; ++i) {
-        sj[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sm), i));
-        cj[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cm), i));
+        if constexpr (GS == 64) {
+            sj[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sm), i));
+            cj[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cm), i));
+        } else {
+            const int src = (threadIdx.x & 63 & ~(GS - 1)) + i;
+            sj[i] = __shfl(sm, src);
+            cj[i] = __shfl(cm, src);
+        }
     }
     float X0 = 1.0f, X1 = 0.0f, X2 = 0.0f;
     float Y0 = 0.0f, Y1 = 1.0f, Y2 = 0.0f;
@@ -256,11 +265,13 @@ __device__ __forceinline__ void arm_fk_slots(const float q[kDof], int g, LaneSph
     }
 }
 
-template <int LPS>
+// FK of the lane's spheres (slot or link g); GS: lanes per state of the link
+// layout.
+template <int LPS, int GS = 64>
 __device__ __forceinline__ void arm_fk(const float q[kDof], int g, LaneSpheres<LPS> &S)
 {
     if constexpr (LPS == 7)
-        arm_fk_links(q, g, S);
+        arm_fk_links<GS>(q, g, S);
     else
         arm_fk_slots<LPS>(q, g, S);
 }
@@ -333,9 +344,9 @@ enum class Exit { kAllGroups, kAnyGroup, kLowestGroup };
 // part, part + OPS, part + 2*OPS, ...).
 template <int LPS, int OPS>
 struct GroupShape {
-    // lanes per state; the link layout (LPS 7) takes the whole wave for one
-    // state, lanes with part >= OPS idle
-    static constexpr int GS = LPS == 7 ? 64 : LPS * OPS;
+    // lanes per state; the link layout (LPS 7) rounds 7*OPS up to a power of
+    // two, lanes with part >= OPS idle
+    static constexpr int GS = LPS != 7 ? LPS * OPS : (OPS == 1 ? 8 : OPS == 2 ? 16 : OPS <= 4 ? 32 : 64);
     static constexpr int STATES = 64 / GS;  // states per wave
     int slot, part, group;
     __device__ __forceinline__ explicit GroupShape(int lane)
@@ -372,20 +383,30 @@ __device__ __forceinline__ void spheres_from_fk(const LaneSpheres<LPS> &F, int t
 // partition per iteration, indices clamped to the last obstacle, whose
 // repeated test is harmless) and ends each iteration with a ballot.  Returns
 // this lane's hit flag (the state's result is the OR over its group).
-// Link layout collision (one state per wave): the lane tests its link's
-// bounding sphere against K obstacles of its partition (obstacles part,
-// part + OPS, ...) per pass, then runs the exact four-sphere test on the
-// obstacles whose bound it touches, one per lane per round until no lane has
-// one left; the wave leaves after the first pass with a hit.  LDS holds the
+// Link layout collision: the lane tests its link's bounding sphere against K
+// obstacles of its partition (obstacles part, part + OPS, ...) per pass,
+// then runs the exact four-sphere test on the obstacles whose bound it
+// touches, one per lane per round until no lane has one left; after each
+// pass the wave leaves once its exit policy is met.  LDS holds the
 // link-layout staging of stage_env<true>.
-template <int OPS>
+template <int OPS, Exit E>
 __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const float4 *lds, int ns, int nb, bool live,
                                               int part)
 {
+    constexpr int GS = GroupShape<7, OPS>::GS;
     constexpr int K = 8;
     const bool act = live && part < OPS;
     bool hit = false;
-    if (!wave_ballot(act)) return false;
+    const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));
+    if (live_leaders == 0) return false;
+    auto done = [&]() -> bool {
+        const uint64_t H = fold_groups<GS>(wave_ballot(hit && live));
+        if (E == Exit::kAllGroups) return (H & live_leaders) == live_leaders;
+        if (E == Exit::kAnyGroup) return H != 0;
+        if (H == 0) return false;
+        const uint64_t low = H & (~H + 1);
+        return ((low - 1) & live_leaders) == 0;
+    };
     const float *rr = reinterpret_cast<const float *>(lds + ns + 2 * nb);
     for (int ob = 0; ob < ns; ob += K * OPS) {
         unsigned cand = 0;
@@ -406,7 +427,7 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
                 hit |= hits_sphere<7>(S, e);
             }
         }
-        if (wave_ballot(hit)) return hit;
+        if (done()) return hit;
     }
     const float4 *boxes = lds + ns;
     for (int ob = 0; ob < nb; ob += K * OPS) {
@@ -430,7 +451,7 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
                 hit |= hits_box<7>(S, boxes[2 * o], boxes[2 * o + 1]);
             }
         }
-        if (wave_ballot(hit)) return hit;
+        if (done()) return hit;
     }
     return hit;
 }
@@ -439,7 +460,7 @@ template <int LPS, int OPS, Exit E>
 __device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live,
                                             int part)
 {
-    if constexpr (LPS == 7) return collide_links<OPS>(S, lds, ns, nb, live, part);
+    if constexpr (LPS == 7) return collide_links<OPS, E>(S, lds, ns, nb, live, part);
     constexpr int GS = LPS * OPS;
     bool hit = false;
     const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));

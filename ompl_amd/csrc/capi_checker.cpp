@@ -83,7 +83,7 @@ void check_motions(const Env &env, const double *a, const double *b, int64_t n, 
 std::vector<Tuning> motion_candidates()
 {
     std::vector<Tuning> c;
-    const int shapes[][2] = {{8, 1}, {8, 2}, {8, 4}, {8, 8}, {7, 9}, {4, 1}, {1, 1}};
+    const int shapes[][2] = {{8, 1}, {8, 2}, {8, 4}, {8, 8}, {7, 1}, {7, 2}, {7, 4}, {7, 9}, {4, 1}, {1, 1}};
     for (const auto &e : shapes)
         for (const auto &w : shapes) {
             const int LE = e[0], OE = e[1], LW = w[0], OW = w[1];

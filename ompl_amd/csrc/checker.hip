@@ -38,7 +38,7 @@
 namespace ompl_amd {
 
 // (LPS, OPS) shapes instantiated for every batched kernel.
-#define OA_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(7, 9)
+#define OA_SHAPES(X) X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(7, 1) X(7, 2) X(7, 4) X(7, 9)
 // shapes with more than one obstacle partition: the shared-FK kernels
 #define OA_SHARED_SHAPES(X) X(8, 2) X(8, 4) X(8, 8)
 
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
 #pragma unroll
     for (int j = 0; j < kDof; ++j) q[j] = live ? (float)q_soa[(int64_t)j * n + st] : 0.0f;
     LaneSpheres<LPS> S;
-    arm_fk<LPS>(q, sh.slot, S);
+    arm_fk<LPS, Shape::GS>(q, sh.slot, S);
     const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, env.ns, env.nb, live, sh.part);
     const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
     if (live && lane % Shape::GS == 0) out[st] = ((H >> lane) & 1) ? 0 : 1;
@@ -155,7 +155,7 @@ __device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4
 #pragma unroll
     for (int j = 0; j < kDof; ++j) q[j] = live ? (float)A.b[(int64_t)j * A.n + e] : 0.0f;
     LaneSpheres<LPS> S;
-    arm_fk<LPS>(q, sh.slot, S);
+    arm_fk<LPS, Shape::GS>(q, sh.slot, S);
     const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
     const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
     return endpoint_finish<Shape::GS>(A, e, live, H, m, a, b);
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(256) state_lists_kernel(EnvView env, const dou
 #pragma unroll
         for (int j = 0; j < kDof; ++j) q[j] = live ? (float)q_soa[(int64_t)j * n_states + st] : 0.0f;
         LaneSpheres<LPS> S;
-        arm_fk<LPS>(q, sh.slot, S);
+        arm_fk<LPS, Shape::GS>(q, sh.slot, S);
         const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, env.ns, env.nb, live, sh.part);
         const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
         if (H) {
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, d
                 interpolate_f32<kDof>(a, b, (double)p / (double)nd, q);
             }
             LaneSpheres<LPS> S;
-            arm_fk<LPS>(q, sh.slot, S);
+            arm_fk<LPS, Shape::GS>(q, sh.slot, S);
             const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, A.env.ns, A.env.nb, live, sh.part);
             const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
             if (H) {

@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
                 walk_state(a, b, E.nd, plive ? p : 0, q);
             }
             LaneSpheres<LPS> S;
-            arm_fk<LPS>(q, sh.slot, S);
+            arm_fk<LPS, Shape::GS>(q, sh.slot, S);
             const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, C.obs, C.ns, C.nb, plive, sh.part);
             const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
             if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);

@@ -64,7 +64,7 @@ __device__ __forceinline__ int walk_chunk(const double a[kDof], const double b[k
     float q[kDof];
     walk_state(a, b, nd, live ? p : 0, q);
     LaneSpheres<LPS> S;
-    arm_fk<LPS>(q, sh.slot, S);
+    arm_fk<LPS, Shape::GS>(q, sh.slot, S);
     const bool hit =
         collide_env<LPS, OPS, EXACT ? Exit::kLowestGroup : Exit::kAnyGroup>(S, lds, ns, nb, live, sh.part);
     const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));

@@ -38,7 +38,8 @@ def test_fk_bit_exact(gpu, oracle):
         assert np.array_equal(got[i], oracle.arm_spheres(q[i])), i
 
 
-SHAPES = [(1, 1), (2, 1), (4, 1), (8, 1), (8, 2), (8, 4), (8, 8), (7, 9)]  # (7, 9): link layout with bound culling
+# (7, x): link layout with bounding-sphere culling, 8 / 4 / 2 / 1 states per wave
+SHAPES = [(1, 1), (2, 1), (4, 1), (8, 1), (8, 2), (8, 4), (8, 8), (7, 1), (7, 2), (7, 4), (7, 9)]
 # (lanes, parts, shared FK)
 VSHAPES = [(lps, ops, 0) for lps, ops in SHAPES] + [(8, 2, 1), (8, 4, 1), (8, 8, 1)]
 
@@ -66,7 +67,7 @@ def test_validate_states_bit_exact(gpu, oracle, lps, ops, shared, ns, nb):
 def test_validate_ragged_sizes(gpu, oracle, n):
     env, oenv, _ = env_pair(gpu, oracle, 7, 32, 32)
     q = gpu.workload.configs(n, n).astype(np.float32).T.copy()
-    for lps, ops, shared in ((1, 1, 0), (8, 1, 0), (8, 8, 0), (8, 8, 1), (8, 2, 1), (7, 9, 0)):
+    for lps, ops, shared in ((1, 1, 0), (8, 1, 0), (8, 8, 0), (8, 8, 1), (8, 2, 1), (7, 9, 0), (7, 1, 0), (7, 4, 0)):
         gpu.set_tuning(validity_lanes_per_state=lps, validity_obstacle_parts=ops, validity_shared_fk=shared)
         assert np.array_equal(gpu.validate_states(env, to_dev(q)).cpu().numpy(), oracle.validate_states(oenv, q))
     gpu.set_tuning(validity_lanes_per_state=8, validity_obstacle_parts=2, validity_shared_fk=0)
@@ -110,7 +111,7 @@ def near_contact_envs(oracle, n_envs, seed):
         yield spheres, boxes, q
 
 
-@pytest.mark.parametrize("lps,ops", [(7, 9), (8, 8), (1, 1)])
+@pytest.mark.parametrize("lps,ops", [(7, 9), (7, 4), (7, 2), (7, 1), (8, 8), (1, 1)])
 def test_validate_near_contact(gpu, oracle, lps, ops):
     """Obstacles within ~1e-6 m of contact: every shape, the link layout with
     its bound culling included, reproduces the oracle's exact FP32 decisions."""
@@ -165,7 +166,8 @@ STRATEGIES = [(3, (8, 2), (8, 1)), (3, (8, 8), (8, 8)), (3, (1, 1), (1, 1)), (3,
               (2, (8, 1), (1, 1)), (2, (8, 2), (8, 8)), (2, (8, 4), (8, 1)),
               (4, (8, 8), (8, 8)), (4, (8, 2), (8, 4)), (4, (8, 4), (8, 1)), (4, (1, 1), (8, 2)),
               (3, (7, 9), (7, 9)), (3, (8, 8), (7, 9)), (1, (7, 9), (7, 9)), (0, (8, 1), (7, 9)),
-              (4, (7, 9), (8, 8))]
+              (4, (7, 9), (8, 8)), (3, (7, 4), (7, 2)), (3, (7, 1), (7, 4)), (1, (7, 2), (7, 1)),
+              (0, (8, 1), (7, 2))]
 DEFAULT = dict(motion_strategy=3, validity_lanes_per_state=8, validity_obstacle_parts=2,
                motion_lanes_per_state=8, motion_obstacle_parts=1)
 
@@ -236,7 +238,7 @@ def test_check_motion_plain_single_call(gpu, oracle):
         assert gpu.check_motion(env, a[i], b[i], lvs) == oracle.check_motion(oenv, a[i], b[i], lvs)[0]
 
 
-@pytest.mark.parametrize("lps,ops", [(1, 1), (8, 1), (8, 8), (7, 9)])
+@pytest.mark.parametrize("lps,ops", [(1, 1), (8, 1), (8, 8), (7, 9), (7, 2)])
 def test_check_motions_last_valid(gpu, oracle, lps, ops):
     gpu.set_tuning(motion_lanes_per_state=lps, motion_obstacle_parts=ops)
     env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
