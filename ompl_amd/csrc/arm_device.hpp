@@ -289,15 +289,25 @@ __device__ __forceinline__ bool hits_sphere(const LaneSpheres<LPS> &S, float4 o)
     return hit;
 }
 
+// min(max(x, lo), hi) for finite lo <= hi (every stored box: coordinates are
+// checked finite and extents non-negative when an environment is created) in
+// one instruction: v_med3_f32 returns the same bits as the min/max pair for
+// every x (a NaN x gives a NaN distance either way), without the
+// canonicalising moves IEEE-mode min/max need.
+__device__ __forceinline__ float clamp_box(float x, float lo, float hi)
+{
+    return __builtin_amdgcn_fmed3f(x, lo, hi);
+}
+
 template <int LPS>
 __device__ __forceinline__ bool hits_box(const LaneSpheres<LPS> &S, float4 lo, float4 hi)
 {
     bool hit = false;
 #pragma unroll
     for (int m = 0; m < LaneSpheres<LPS>::N; ++m) {
-        float px = fminf(fmaxf(S.x[m], lo.x), hi.x);
-        float py = fminf(fmaxf(S.y[m], lo.y), hi.y);
-        float pz = fminf(fmaxf(S.z[m], lo.z), hi.z);
+        float px = clamp_box(S.x[m], lo.x, hi.x);
+        float py = clamp_box(S.y[m], lo.y, hi.y);
+        float pz = clamp_box(S.z[m], lo.z, hi.z);
         float dx = S.x[m] - px, dy = S.y[m] - py, dz = S.z[m] - pz;
         float d2 = (dx * dx + dy * dy) + dz * dz;
         hit |= d2 < kRobotR * kRobotR;
@@ -412,9 +422,10 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
         unsigned cand = 0;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
+            // bound tests need only be conservative: fused multiply-adds
             const float4 o = lds[min(ob + part + j * OPS, ns - 1)];
             const float dx = o.x - S.cx, dy = o.y - S.cy, dz = o.z - S.cz;
-            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
             cand |= (unsigned)(d2 < o.w) << j;
         }
         if (!act) cand = 0;
@@ -436,11 +447,10 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
         for (int j = 0; j < K; ++j) {
             const int o = min(ob + part + j * OPS, nb - 1);
             const float4 lo = boxes[2 * o], hi = boxes[2 * o + 1];
-            const float px = fminf(fmaxf(S.cx, lo.x), hi.x);
-            const float py = fminf(fmaxf(S.cy, lo.y), hi.y);
-            const float pz = fminf(fmaxf(S.cz, lo.z), hi.z);
-            const float dx = S.cx - px, dy = S.cy - py, dz = S.cz - pz;
-            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            const float dx = S.cx - clamp_box(S.cx, lo.x, hi.x);
+            const float dy = S.cy - clamp_box(S.cy, lo.y, hi.y);
+            const float dz = S.cz - clamp_box(S.cz, lo.z, hi.z);
+            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
             cand |= (unsigned)(d2 < kBoxBound * kBoxBound) << j;
         }
         if (!act) cand = 0;

@@ -141,12 +141,17 @@ int ompl_amd_env_create(const float *spheres, int n_spheres, const float *boxes,
         std::vector<float4> h((size_t)n_spheres + 2 * (size_t)n_boxes + 1);
         for (int i = 0; i < n_spheres; ++i) {
             const float *o = spheres + 4 * i;
+            OA_REQUIRE(std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]) && std::isfinite(o[3]),
+                       OMPL_AMD_ERR_ARG, "sphere obstacle %d: non-finite value", i);
             OA_REQUIRE(o[3] >= 0.0f, OMPL_AMD_ERR_ARG, "negative sphere radius");
             const float rs = o[3] + kRobotR;
             h[i] = make_float4(o[0], o[1], o[2], rs * rs);
         }
         for (int i = 0; i < n_boxes; ++i) {
             const float *o = boxes + 6 * i;
+            OA_REQUIRE(std::isfinite(o[0]) && std::isfinite(o[1]) && std::isfinite(o[2]) && std::isfinite(o[3]) &&
+                           std::isfinite(o[4]) && std::isfinite(o[5]),
+                       OMPL_AMD_ERR_ARG, "box obstacle %d: non-finite value", i);
             OA_REQUIRE(o[3] >= 0.0f && o[4] >= 0.0f && o[5] >= 0.0f, OMPL_AMD_ERR_ARG, "negative box extent");
             h[n_spheres + 2 * i] = make_float4(o[0] - o[3], o[1] - o[4], o[2] - o[5], 0.0f);
             h[n_spheres + 2 * i + 1] = make_float4(o[0] + o[3], o[1] + o[4], o[2] + o[5], 0.0f);

@@ -148,6 +148,24 @@ def test_link_layout_large_env(gpu, oracle):
     gpu.set_tuning(**DEFAULT)
 
 
+def test_env_rejects_non_finite_obstacles(gpu):
+    from ompl_amd._lib import OmplAmdError
+
+    good_s, good_b = np.array([[0.5, 0.5, 0.5, 0.1]], np.float32), np.array([[0.5, 0, 0, 0.1, 0.1, 0.1]], np.float32)
+    for bad in (np.nan, np.inf, -np.inf):
+        for k in range(4):
+            s = good_s.copy()
+            s[0, k] = bad
+            with pytest.raises(OmplAmdError):
+                gpu.Env(s, good_b)
+        for k in range(6):
+            b = good_b.copy()
+            b[0, k] = bad
+            with pytest.raises(OmplAmdError):
+                gpu.Env(good_s, b)
+    gpu.Env(good_s, good_b).close()
+
+
 def test_is_valid_single_call(gpu, oracle):
     env, oenv, _ = env_pair(gpu, oracle, 1000, 64, 0)
     q = gpu.workload.configs(5, 200)
