@@ -89,16 +89,18 @@ std::vector<Tuning> motion_candidates()
             const int LE = e[0], OE = e[1], LW = w[0], OW = w[1];
             // chunk-parallel walk of the survivors; shared-FK forms of the s2
             // pass (validity_shared_fk) and of the walk (strategy 4)
+            // (shared FK exists for the sphere-slot shapes (8, parts > 1) only)
+            const int es_max = (LE == 8 && OE > 1) ? 1 : 0;
             for (int wpc : {8, 16, 32})
-                for (int es = 0; es <= (OE > 1 ? 1 : 0); ++es) {
+                for (int es = 0; es <= es_max; ++es) {
                     c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 3, es});
-                    if (OW > 1) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 4, es});
+                    if (LW == 8 && OW > 1) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, wpc, 4, es});
                 }
             // fused one-launch form
             if (fused_supported(LE, OE, LW, OW))
                 for (int block : {64, 256}) c.push_back(Tuning{LE, OE, block, LW, OW, block, 8, 2});
             // persistent queue, one wave per survivor
-            for (int es = 0; es <= (OE > 1 ? 1 : 0); ++es) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, 16, 1, es});
+            for (int es = 0; es <= es_max; ++es) c.push_back(Tuning{LE, OE, 256, LW, OW, 256, 16, 1, es});
         }
     for (const auto &w : shapes) c.push_back(Tuning{8, 1, 256, w[0], w[1], 256, 8, 0});
     return c;

@@ -359,7 +359,7 @@ __device__ __forceinline__ void walk_window(const MotionArgs &A, const float4 *l
     }
 }
 
-// Chunk-parallel walk of the survivors in two launches (A.phase 1, then 2),
+// Chunk-parallel walk of the survivors in two launches (PHASE 1, then 2),
 // one wave per work item (G positions of one survivor) over a persistent grid
 // of W waves, wave w taking items w, w + W, w + 2W, ... of its phase; a wave
 // tests the liveness of its next 64 items at once (one per lane) and walks
@@ -375,7 +375,7 @@ __device__ __forceinline__ void walk_window(const MotionArgs &A, const float4 *l
 //     1): phase 1 is one item per wave.
 //   Without phase 2, phase 1 covers every chunk (L0 = all).
 // The last block of phase 2 writes the survivors' results.
-template <int LPS, int OPS, bool EXACT, bool SHARED>
+template <int LPS, int OPS, bool EXACT, bool SHARED, int PHASE>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
     const int64_t nblk = ((int64_t)*A.max_m + G - 1) / G;
     const bool flat = A.flat && n_surv <= kFlatSurvivors && n_surv * nblk < (int64_t)INT_MAX;
     const int64_t L0 = flat ? max((int64_t)1, min(nblk, waves_total / n_surv)) : nblk;
-    if (A.phase == 1) {
+    if (PHASE == 1) {
         const int64_t items1 = n_surv * L0;
         for (int64_t k0 = wave; k0 < items1; k0 += 64 * waves_total) {
             const int64_t k = k0 + lane * waves_total;
@@ -775,18 +775,23 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
     const size_t env_b = lds_bytes(A.env, lps), flat_b = (size_t)(kFlatSurvivors + 1) * sizeof(int32_t);
     B.flat = env_b + flat_b + 4096 + 64 <= 65536;
     const size_t dyn = env_b + (B.flat ? flat_b : 0);
-    MotionArgs B1 = B;
-    B1.phase = 1;
-    B.phase = 2;
+    // phase 1 (early chunks), then phase 2 (the rest and the results)
+    auto launch = [&](auto early, auto rest) {
+        {
+            ProfScope p1("motion_chunk_early", s);
+            hipLaunchKernelGGL(early, dim3(g), dim3(block), dyn, s, B);
+        }
+        ProfScope p2("motion_chunk_rest", s);
+        hipLaunchKernelGGL(rest, dim3(g), dim3(block), dyn, s, B);
+    };
+
     if (shared && ops > 1) {
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
-        for (const MotionArgs *P : {&B1, &B}) {                                                                   \
-            if (exact)                                                                                              \
-                hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, true>), dim3(g), dim3(block), dyn, s, *P);     \
-            else                                                                                                    \
-                hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, true>), dim3(g), dim3(block), dyn, s, *P);    \
-        }                                                                                                           \
+        if (exact)                                                                                                  \
+            launch(motion_chunk_kernel<L, O, true, true, 1>, motion_chunk_kernel<L, O, true, true, 2>);            \
+        else                                                                                                        \
+            launch(motion_chunk_kernel<L, O, false, true, 1>, motion_chunk_kernel<L, O, false, true, 2>);          \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -795,12 +800,10 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
     }
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
-        for (const MotionArgs *P : {&B1, &B}) {                                                                   \
-            if (exact)                                                                                              \
-                hipLaunchKernelGGL((motion_chunk_kernel<L, O, true, false>), dim3(g), dim3(block), dyn, s, *P);    \
-            else                                                                                                    \
-                hipLaunchKernelGGL((motion_chunk_kernel<L, O, false, false>), dim3(g), dim3(block), dyn, s, *P);   \
-        }                                                                                                           \
+        if (exact)                                                                                                  \
+            launch(motion_chunk_kernel<L, O, true, false, 1>, motion_chunk_kernel<L, O, true, false, 2>);          \
+        else                                                                                                        \
+            launch(motion_chunk_kernel<L, O, false, false, 1>, motion_chunk_kernel<L, O, false, false, 2>);        \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }

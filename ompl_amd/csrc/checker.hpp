@@ -28,7 +28,6 @@ struct MotionArgs {
     double *surv_ab;     // per survivor: endpoints a then b (14 doubles, contiguous), optional
     int32_t *done_blocks; // chunk walk: finished blocks; the last one finalizes (optional)
     int flat;             // chunk walk: the launch reserved LDS for the survivors' chunk offsets
-    int phase;            // chunk walk: 1 (early chunks, chunk-major) or 2 (the rest, then results)
     int start;           // first walk position (0 = s2, 1 = skip s2)
 };
 
