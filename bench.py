@@ -50,7 +50,7 @@ def percentile(xs, p):
 
 
 def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000, log_path=None,
-                  check_parts=0, profile_ticks=False, run_plain=True):
+                  check_parts=0, profile_ticks=False, run_plain=True, check_lanes=0):
     """Poisson stream of RRT-Connect requests over seeded environments (16-1024
     obstacles, log-uniform, half spheres / half boxes), served (a) by the
     continuous-batching server and (b) by the plain path (host RRT-Connect, one
@@ -93,7 +93,8 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     if profile_ticks:
         oa.profile.reset()
         oa.profile.enable(True)
-    srv = serve.Server(hbm_budget_bytes=4 << 30, max_active=1024, max_nodes=4096, check_obstacle_parts=check_parts)
+    srv = serve.Server(hbm_budget_bytes=4 << 30, max_active=1024, max_nodes=4096, check_obstacle_parts=check_parts,
+                       check_lanes_per_state=check_lanes)
     ids = [None] * n_requests
     t0 = time.perf_counter()
     for i, r in enumerate(reqs):
@@ -121,7 +122,7 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     if profile_ticks:
         served.update(gpu_us_per_tick=1e3 * tick_ms / n_ticks if n_ticks else None, tick_kernels_us=tick_parts)
     if not run_plain:
-        return {"requests": n_requests, "qps": qps, "check_obstacle_parts": check_parts or 8, "served": served}
+        return {"requests": n_requests, "qps": qps, "check_shape": [check_lanes, check_parts], "served": served}
 
     # (b) plain: FIFO queue drained by plain_workers threads
     lat_p = [0.0] * n_requests
@@ -637,11 +638,13 @@ def main(argv=None):
     if args.stream_requests > 0:
         stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers,
                                    log_path=args.benchmark_log)
-        # tick breakdown (HIP events around each tick kernel; slower than the run above) per obstacle split
+        # tick breakdown (HIP events around each tick kernel; slower than the run above) per check shape
+        # (lanes per state, obstacle parts): sphere slots (8, x), link layout (7, x)
         stream_res["tick_profile"] = {
-            parts: served_stream(oa, min(64, args.stream_requests), args.stream_qps, args.stream_envs, 0,
-                                 check_parts=parts, profile_ticks=True, run_plain=False)["served"]
-            for parts in (2, 4, 8)}
+            f"{lanes}x{parts}": served_stream(oa, min(64, args.stream_requests), args.stream_qps, args.stream_envs, 0,
+                                              check_parts=parts, profile_ticks=True, run_plain=False,
+                                              check_lanes=lanes)["served"]
+            for lanes, parts in ((8, 4), (8, 8), (7, 2), (7, 4), (7, 9))}
 
     prm_res = None
     if args.prm_samples > 0:

@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 304
+#define OMPL_AMD_VERSION 305
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -340,7 +340,11 @@ typedef struct {
     int max_active;          /* concurrent requests (tree slots preallocated) */
     int max_nodes;           /* nodes per tree per request */
     int check_obstacle_parts; /* lanes splitting each environment's obstacle list in the
-                                 tick's motion check: 2, 4 or 8 (0 = 8) */
+                                 tick's motion check: 2, 4 or 8 with sphere slots, 1, 2, 4
+                                 or 9 with the link layout (0 = 4 for the link layout, 8
+                                 for sphere slots) */
+    int check_lanes_per_state; /* 8: sphere slots; 7: link layout with bounding-sphere
+                                  culling (see ompl_amd_tuning); 0 = 7 */
 } ompl_amd_server_config;
 
 typedef struct {

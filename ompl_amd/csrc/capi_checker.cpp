@@ -168,6 +168,17 @@ int ompl_amd_env_create(const float *spheres, int n_spheres, const float *boxes,
             fail(OMPL_AMD_ERR_NOMEM, "environment allocation failed: %s", hipGetErrorString(err));
         }
         OA_HIP(hipMemcpy(e->env.obs, h.data(), e->env.bytes, hipMemcpyHostToDevice));
+        const size_t lb = links_bytes(e->env.view()) + sizeof(float4);
+        err = hipMalloc(&e->env.obs_links, lb);
+        if (err != hipSuccess) {
+            (void)hipFree(e->env.obs);
+            delete e;
+            fail(OMPL_AMD_ERR_NOMEM, "environment allocation failed: %s", hipGetErrorString(err));
+        }
+        e->env.bytes += lb;
+        ThreadResources &r = thread_resources();
+        launch_stage_links(e->env.view(), e->env.obs_links, r.stream);
+        OA_HIP(hipStreamSynchronize(r.stream));
         *out = e;
     });
 }
@@ -177,6 +188,7 @@ int ompl_amd_env_destroy(ompl_amd_env e)
     return guarded([&] {
         if (!e) return;
         if (e->env.obs) OA_HIP(hipFree(e->env.obs));
+        if (e->env.obs_links) OA_HIP(hipFree(e->env.obs_links));
         delete e;
     });
 }

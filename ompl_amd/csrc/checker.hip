@@ -623,6 +623,22 @@ __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, d
     }
 }
 
+// The link layout's staging of an environment (stage_env<true>) written to
+// HBM, for kernels that read obstacles from HBM (the planning server).
+__global__ void stage_links_kernel(EnvView e, float4 *__restrict__ out)
+{
+    const int nw = env_words(e);
+    float *rr = reinterpret_cast<float *>(out + nw);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
+        float4 o = e.obs[i];
+        if (i < e.ns) {
+            rr[i] = o.w;
+            o.w = sphere_bound2(o.w);
+        }
+        out[i] = o;
+    }
+}
+
 __global__ void zero_ints_kernel(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -706,6 +722,19 @@ template void launch_validate<float>(const EnvView &, const float *, int64_t, ui
                                      hipStream_t, bool);
 template void launch_validate<double>(const EnvView &, const double *, int64_t, uint8_t *, int, int, int,
                                       hipStream_t, bool);
+
+size_t links_bytes(const EnvView &env)
+{
+    return lds_bytes(env, 7);
+}
+
+void launch_stage_links(const EnvView &env, float4 *out, hipStream_t s)
+{
+    const int nw = env.ns + 2 * env.nb;
+    if (nw == 0) return;
+    hipLaunchKernelGGL(stage_links_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, env, out);
+    OA_HIP(hipGetLastError());
+}
 
 void launch_zero(int *p, int n, hipStream_t s)
 {

@@ -41,6 +41,9 @@ struct State1 {
 };
 
 bool shape_supported(int lps, int ops);
+// Bytes of an environment's link-layout copy, and the kernel that writes it.
+size_t links_bytes(const EnvView &env);
+void launch_stage_links(const EnvView &env, float4 *out, hipStream_t s);
 bool fused_supported(int le, int oe, int lw, int ow);
 
 template <typename T>

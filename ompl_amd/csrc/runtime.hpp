@@ -62,6 +62,9 @@ Tuning tuning_from_c(const ompl_amd_tuning &t);
 // Device-resident obstacle set.
 struct Env {
     float4 *obs = nullptr; // ns spheres (x,y,z,(R+r)^2), then nb boxes as (lo),(hi)
+    // the same obstacles in the link layout's staging (spheres (x,y,z,bound^2),
+    // boxes, then the spheres' rr as floats), for kernels that read them from HBM
+    float4 *obs_links = nullptr;
     int ns = 0;
     int nb = 0;
     size_t bytes = 0;

@@ -42,6 +42,7 @@ enum GrowStatus { kTrapped = 0, kAdvanced = 1, kReached = 2 };
 // One growTree step (device).
 struct GrowCmd {
     const float4 *obs;
+    const float4 *obs_links; // the same obstacles in the link layout's staging
     int32_t ns, nb;
     double *tree_s;        // [cap][7]
     int32_t *tree_parent;  // [cap]
@@ -226,7 +227,8 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
             }
             LaneSpheres<LPS> S;
             arm_fk<LPS, Shape::GS>(q, sh.slot, S);
-            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, C.obs, C.ns, C.nb, plive, sh.part);
+            const float4 *obs = LPS == 7 ? C.obs_links : C.obs;
+            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, obs, C.ns, C.nb, plive, sh.part);
             const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
             if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);
         }
@@ -312,6 +314,7 @@ static void next_command(Planner &p, GrowCmd &c)
     }
     // phases 1/2 grow the other tree toward rstate with the current tgi_start
     c.obs = p.env->obs;
+    c.obs_links = p.env->obs_links;
     c.ns = p.env->ns;
     c.nb = p.env->nb;
     c.tree_s = p.tree_s[p.grow_tree];
@@ -540,12 +543,15 @@ void Server::run()
                 }
                 {
                     ProfScope p2("serve_check", stream);
-                    if (cfg.check_obstacle_parts == 2)
-                        hipLaunchKernelGGL((grow_check_kernel<8, 2>), dim3(check_blocks), dim3(256), 0, stream, T);
-                    else if (cfg.check_obstacle_parts == 4)
-                        hipLaunchKernelGGL((grow_check_kernel<8, 4>), dim3(check_blocks), dim3(256), 0, stream, T);
-                    else
-                        hipLaunchKernelGGL((grow_check_kernel<8, 8>), dim3(check_blocks), dim3(256), 0, stream, T);
+                    const int L = cfg.check_lanes_per_state, O = cfg.check_obstacle_parts;
+#define OA_CHECK(LL, OO)                                                                                            \
+    else if (L == LL && O == OO) hipLaunchKernelGGL((grow_check_kernel<LL, OO>), dim3(check_blocks), dim3(256), 0, \
+                                                    stream, T);
+                    if (false) {
+                    }
+                    OA_CHECK(8, 2) OA_CHECK(8, 4) OA_CHECK(8, 8) OA_CHECK(7, 1) OA_CHECK(7, 2) OA_CHECK(7, 4)
+                    OA_CHECK(7, 9)
+#undef OA_CHECK
                 }
                 {
                     ProfScope p3("serve_commit", stream);
@@ -695,14 +701,19 @@ int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *o
         OA_REQUIRE(cfg && out, OMPL_AMD_ERR_ARG, "null argument");
         OA_REQUIRE(cfg->max_active >= 1 && cfg->max_active <= 65536, OMPL_AMD_ERR_ARG, "max_active in [1, 65536]");
         OA_REQUIRE(cfg->max_nodes >= 2, OMPL_AMD_ERR_ARG, "max_nodes >= 2");
-        OA_REQUIRE(cfg->check_obstacle_parts == 0 || cfg->check_obstacle_parts == 2 ||
-                       cfg->check_obstacle_parts == 4 || cfg->check_obstacle_parts == 8,
-                   OMPL_AMD_ERR_ARG, "check_obstacle_parts must be 0, 2, 4 or 8");
+        const int cl = cfg->check_lanes_per_state, co = cfg->check_obstacle_parts;
+        OA_REQUIRE(cl == 0 || cl == 7 || cl == 8, OMPL_AMD_ERR_ARG, "check_lanes_per_state must be 0, 7 or 8");
+        OA_REQUIRE(cl == 7 ? (co == 0 || co == 1 || co == 2 || co == 4 || co == 9)
+                           : (co == 0 || co == 2 || co == 4 || co == 8),
+                   OMPL_AMD_ERR_ARG,
+                   "check_obstacle_parts must be 0, 2, 4 or 8 (sphere slots) or 0, 1, 2, 4 or 9 (link layout)");
         ensure_init();
         auto *h = new ompl_amd_server_s();
         Server &S = h->srv;
         S.cfg = *cfg;
-        if (S.cfg.check_obstacle_parts == 0) S.cfg.check_obstacle_parts = 8;
+        // defaults: the link layout with two states per wave, the fastest tick check measured
+        if (S.cfg.check_lanes_per_state == 0) S.cfg.check_lanes_per_state = 7;
+        if (S.cfg.check_obstacle_parts == 0) S.cfg.check_obstacle_parts = S.cfg.check_lanes_per_state == 7 ? 4 : 8;
         OA_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
         const size_t per_tree = (size_t)cfg->max_nodes * (kDof * sizeof(double) + sizeof(int32_t));
         const size_t total = per_tree * 2 * (size_t)cfg->max_active;

@@ -16,7 +16,8 @@ STATUS = {0: "queued", 1: "running", 2: "solved", 3: "failed", 4: "deadline", 5:
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("max_nodes", C.c_int),
-                ("check_obstacle_parts", C.c_int)]
+                ("check_obstacle_parts", C.c_int),
+                ("check_lanes_per_state", C.c_int)]
 
 
 class PlanRequest(C.Structure):
@@ -68,8 +69,9 @@ def make_request(env, start, goal, seed, lvs, max_iters=5000, rng=0.0, factor=1,
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes=1 << 30, max_active=256, max_nodes=8192, check_obstacle_parts=0):
-        cfg = ServerConfig(hbm_budget_bytes, max_active, max_nodes, check_obstacle_parts)
+    def __init__(self, hbm_budget_bytes=1 << 30, max_active=256, max_nodes=8192, check_obstacle_parts=0,
+                 check_lanes_per_state=0):
+        cfg = ServerConfig(hbm_budget_bytes, max_active, max_nodes, check_obstacle_parts, check_lanes_per_state)
         h = C.c_void_p()
         call("ompl_amd_server_create", C.byref(cfg), C.byref(h))
         self.handle = h

@@ -56,12 +56,16 @@ def test_plain_planner_matches_oracle(gpu, oracle, scene):
         assert np.array_equal(path, opath)
 
 
-def test_served_equals_oracle_alone_and_in_mix(gpu, oracle, scene):
+# tick check shapes (lanes per state, obstacle parts): defaults, link layout, sphere slots
+@pytest.mark.parametrize("lanes,parts", [(0, 0), (7, 9), (7, 2), (8, 2)])
+def test_served_equals_oracle_alone_and_in_mix(gpu, oracle, scene, lanes, parts):
     from ompl_amd.serve import SOLVED, Server
 
     env, oenv, pairs = scene
     reqs = reqs_for(gpu, env, pairs)
-    srv = Server(hbm_budget_bytes=1 << 30, max_active=64, max_nodes=8192)
+    print(f"served check shape ({lanes}, {parts})", flush=True)
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=64, max_nodes=8192, check_obstacle_parts=parts,
+                 check_lanes_per_state=lanes)
     alone = []
     for r in reqs:  # one at a time
         rid = srv.submit(r)
@@ -125,3 +129,13 @@ def test_failure_paths_forced_from_host(gpu, scene):
     assert srv.wait(rid, 5000).status == CANCELLED
     assert srv.stats()["used_bytes"] == 0
     srv.close()
+
+
+def test_server_rejects_unbuilt_check_shapes(gpu):
+    from ompl_amd._lib import OmplAmdError
+    from ompl_amd.serve import Server
+
+    for lanes, parts in ((8, 9), (8, 1), (7, 8), (6, 0), (7, 3)):
+        with pytest.raises(OmplAmdError):
+            Server(hbm_budget_bytes=1 << 30, max_active=2, max_nodes=64, check_obstacle_parts=parts,
+                   check_lanes_per_state=lanes)
