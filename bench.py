@@ -532,7 +532,7 @@ def main(argv=None):
         ms = timed(lambda: oa.check_motions(env, A, B, lvs, out=out), 30)
         oa.profile.enable(False)
         per = {}
-        for kname in ("motion_endpoint", "motion_chunk"):
+        for kname in ("motion_endpoint", "motion_chunk_early", "motion_chunk_rest"):
             n, kms = oa.profile.query(kname)
             if n:
                 per[kname] = 1e3 * kms / n
@@ -568,11 +568,14 @@ def main(argv=None):
         oa.check_motions(env, A, B, lvs, out=out)
     torch.cuda.synchronize()
     kernels = {}
-    for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk", "motion_chunk_early",
-                 "motion_chunk_rest"):
+    for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk_early", "motion_chunk_rest"):
         n, ms = oa.profile.query(name)
         if n:
             kernels[name] = {"launches": n, "avg_us": 1e3 * ms / n}
+    if "motion_chunk_early" in kernels:  # the chunk walk: its two launches per step
+        kernels["motion_chunk"] = {"launches": kernels["motion_chunk_early"]["launches"],
+                                   "avg_us": kernels["motion_chunk_early"]["avg_us"]
+                                   + kernels["motion_chunk_rest"]["avg_us"]}
     oa.profile.enable(False)
     oa.profile.reset()
 
@@ -585,17 +588,21 @@ def main(argv=None):
              "motion_chunk": flops_step - flops_end}
     if t["motion_strategy"] == 0:
         share["motion_walk"] = flops_step
-    # the dominant kernel is the longest single dispatch; when that is a phase
-    # of the chunk walk (two dispatches per step, no per-phase work split),
-    # the walk is taken as a whole: both dispatches and the walk's work
-    single = [k for k in kernels if k != "motion_chunk"]
-    dom = max(single, key=lambda k: kernels[k]["avg_us"])
-    if dom in ("motion_chunk_early", "motion_chunk_rest"):
-        dom = "motion_chunk"
+    # the dominant kernel is the one with the most time per step; the chunk
+    # walk's two launches count as one kernel (its work is not split by phase)
+    units = [k for k in kernels if k not in ("motion_chunk_early", "motion_chunk_rest")]
+    dom = max(units, key=lambda k: kernels[k]["avg_us"])
     dom_flops = share[dom]
     dom_us = kernels[dom]["avg_us"]
     achieved = dom_flops / (dom_us * 1e-6) / 1e12
     traffic, traffic_src = hbm_traffic(dom, t)
+    # the s2 pass's figure beside it
+    roof_s2 = None
+    if dom != "motion_endpoint" and "motion_endpoint" in kernels:
+        a2 = flops_end / (kernels["motion_endpoint"]["avg_us"] * 1e-6) / 1e12
+        roof_s2 = {"kernel": "motion_endpoint", "achieved": a2, "frac": a2 / PEAK_FP32_TFLOPS,
+                   "kernel_avg_us": kernels["motion_endpoint"]["avg_us"], "algorithmic_flops_per_launch": flops_end,
+                   "traffic": hbm_traffic("motion_endpoint", t)[0]}
 
     # ---- plain one-launch-per-call path on a prefix of the same edges
     P = min(args.plain_edges, N_EDGES)
@@ -689,6 +696,7 @@ def main(argv=None):
                      "kernel": dom, "kernel_avg_us": dom_us, "algorithmic_flops_per_launch": dom_flops,
                      "dispatches": 2 if dom == "motion_chunk" and "motion_chunk_early" in kernels else 1,
                      "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
+        "roofline_s2_pass": roof_s2,
         "cpu_baseline": cpu,
         "cpu_all_cores": cpu_all,
         "autotuned_us_per_batch": tuned_us,

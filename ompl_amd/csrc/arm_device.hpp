@@ -83,21 +83,45 @@ __device__ __forceinline__ float sphere_bound2(float rr)
     return b * b * 1.000001f;
 }
 
+// Obstacles per lane per pass of the link layout's bound tests.
+constexpr int kLinkPass = 8;
+
+__host__ __device__ constexpr int pad_to(int n, int m)
+{
+    return (n + m - 1) / m * m;
+}
+
 // Cooperative copy of the environment into LDS by the whole block.  With
-// LINKS (link layout) the spheres are staged as (x, y, z, bound^2) and their
-// exact rr follow the boxes as a float array.
-template <bool LINKS = false>
+// LINKS (link layout) the spheres are staged as (x, y, z, bound^2), the sphere
+// and box lists are each padded to a multiple of PAD entries (spheres (0, 0,
+// 0, -1), boxes at 1e30: their bound tests always fail), and the spheres'
+// exact rr follow the boxes as a float array; layout [pad(ns) spheres]
+// [2 pad(nb) box words][ns rr].
+template <bool LINKS = false, int PAD = 1>
 __device__ __forceinline__ void stage_env(const EnvView &e, float4 *lds)
 {
-    const int nw = env_words(e);
-    float *rr = reinterpret_cast<float *>(lds + nw);
-    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
-        float4 o = e.obs[i];
-        if (LINKS && i < e.ns) {
-            rr[i] = o.w;
-            o.w = sphere_bound2(o.w);
+    if constexpr (!LINKS) {
+        const int nw = env_words(e);
+        for (int i = threadIdx.x; i < nw; i += blockDim.x) lds[i] = e.obs[i];
+    } else {
+        const int nsp = pad_to(e.ns, PAD), nbp = pad_to(e.nb, PAD);
+        float *rr = reinterpret_cast<float *>(lds + nsp + 2 * nbp);
+        for (int i = threadIdx.x; i < nsp + 2 * nbp; i += blockDim.x) {
+            float4 o;
+            if (i < nsp) {
+                if (i < e.ns) {
+                    o = e.obs[i];
+                    rr[i] = o.w;
+                    o.w = sphere_bound2(o.w);
+                } else {
+                    o = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+                }
+            } else {
+                const int b = i - nsp; // box word
+                o = b < 2 * e.nb ? e.obs[e.ns + b] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+            }
+            lds[i] = o;
         }
-        lds[i] = o;
     }
     __syncthreads();
 }
@@ -399,13 +423,19 @@ __device__ __forceinline__ void spheres_from_fk(const LaneSpheres<LPS> &F, int t
 // touches, one per lane per round until no lane has one left; after each
 // pass the wave leaves once its exit policy is met.  LDS holds the
 // link-layout staging of stage_env<true>.
-template <int OPS, Exit E>
+template <int OPS, Exit E, bool PADDED>
 __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const float4 *lds, int ns, int nb, bool live,
                                               int part)
 {
     constexpr int GS = GroupShape<7, OPS>::GS;
-    constexpr int K = 8;
+    constexpr int K = kLinkPass;
+    // PADDED: lists padded to whole passes (stage_env<true, K * OPS>), no
+    // index clamps; else the unpadded layout, indices clamped to the last entry
+    const int nsp = PADDED ? pad_to(ns, K * OPS) : ns, nbp = PADDED ? pad_to(nb, K * OPS) : nb;
+    auto sidx = [&](int o) { return PADDED ? o : min(o, ns - 1); };
+    auto bidx = [&](int o) { return PADDED ? o : min(o, nb - 1); };
     const bool act = live && part < OPS;
+    part = part < OPS ? part : 0; // idle lanes read partition 0 (their candidates are dropped)
     bool hit = false;
     const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));
     if (live_leaders == 0) return false;
@@ -417,13 +447,13 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
         const uint64_t low = H & (~H + 1);
         return ((low - 1) & live_leaders) == 0;
     };
-    const float *rr = reinterpret_cast<const float *>(lds + ns + 2 * nb);
+    const float *rr = reinterpret_cast<const float *>(lds + nsp + 2 * nbp);
     for (int ob = 0; ob < ns; ob += K * OPS) {
         unsigned cand = 0;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             // bound tests need only be conservative: fused multiply-adds
-            const float4 o = lds[min(ob + part + j * OPS, ns - 1)];
+            const float4 o = lds[sidx(ob + part + j * OPS)];
             const float dx = o.x - S.cx, dy = o.y - S.cy, dz = o.z - S.cz;
             const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
             cand |= (unsigned)(d2 < o.w) << j;
@@ -431,7 +461,7 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
         if (!act) cand = 0;
         while (wave_ballot(cand != 0)) {
             if (cand) {
-                const int o = min(ob + part + __builtin_ctz(cand) * OPS, ns - 1);
+                const int o = sidx(ob + part + __builtin_ctz(cand) * OPS);
                 cand &= cand - 1;
                 float4 e = lds[o];
                 e.w = rr[o];
@@ -440,12 +470,12 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
         }
         if (done()) return hit;
     }
-    const float4 *boxes = lds + ns;
+    const float4 *boxes = lds + nsp;
     for (int ob = 0; ob < nb; ob += K * OPS) {
         unsigned cand = 0;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const int o = min(ob + part + j * OPS, nb - 1);
+            const int o = bidx(ob + part + j * OPS);
             const float4 lo = boxes[2 * o], hi = boxes[2 * o + 1];
             const float dx = S.cx - clamp_box(S.cx, lo.x, hi.x);
             const float dy = S.cy - clamp_box(S.cy, lo.y, hi.y);
@@ -456,7 +486,7 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
         if (!act) cand = 0;
         while (wave_ballot(cand != 0)) {
             if (cand) {
-                const int o = min(ob + part + __builtin_ctz(cand) * OPS, nb - 1);
+                const int o = bidx(ob + part + __builtin_ctz(cand) * OPS);
                 cand &= cand - 1;
                 hit |= hits_box<7>(S, boxes[2 * o], boxes[2 * o + 1]);
             }
@@ -466,11 +496,13 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
     return hit;
 }
 
-template <int LPS, int OPS, Exit E>
+// PADDED (link layout only): the obstacles are in the padded LDS staging of
+// stage_env<true, kLinkPass * OPS>; false for the unpadded copy in HBM.
+template <int LPS, int OPS, Exit E, bool PADDED = true>
 __device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live,
                                             int part)
 {
-    if constexpr (LPS == 7) return collide_links<OPS, E>(S, lds, ns, nb, live, part);
+    if constexpr (LPS == 7) return collide_links<OPS, E, PADDED>(S, lds, ns, nb, live, part);
     constexpr int GS = LPS * OPS;
     bool hit = false;
     const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));

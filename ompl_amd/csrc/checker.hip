@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
                                                         uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
     const Shape sh(lane);
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(1024) validate_shared_kernel(EnvView env, cons
                                                                uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
@@ -193,7 +193,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(A.env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int64_t wave = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t e = wave * Shape::STATES + lane_id() / Shape::GS;
@@ -209,7 +209,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_shared_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(A.env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
@@ -240,7 +240,7 @@ template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(A.env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
     const int lane = lane_id();
     const int waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t count = A.list ? (int64_t)(*A.list_count) : A.n;
@@ -277,10 +277,11 @@ __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 constexpr int kFlatSurvivors = 4096;
 
 // Bytes of dynamic LDS the staged environment takes (stage_env<LINKS>).
-template <bool LINKS>
+template <bool LINKS, int PAD>
 __device__ __forceinline__ size_t env_lds_bytes(const EnvView &e)
 {
-    return (size_t)(e.ns + 2 * e.nb) * sizeof(float4) + (LINKS ? (size_t)e.ns * sizeof(float) : 0);
+    if (!LINKS) return (size_t)(e.ns + 2 * e.nb) * sizeof(float4);
+    return (size_t)(pad_to(e.ns, PAD) + 2 * pad_to(e.nb, PAD)) * sizeof(float4) + (size_t)e.ns * sizeof(float);
 }
 
 // off[0] = 0, off[i + 1] = sum over survivors t <= i of their chunks past
@@ -379,7 +380,7 @@ template <int LPS, int OPS, bool EXACT, bool SHARED, int PHASE>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(A.env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
     constexpr int G = SHARED ? 64 / LPS : GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
@@ -407,7 +408,7 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
         return;
     }
     if (flat) {
-        int32_t *off = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(lds) + env_lds_bytes<LPS == 7>(A.env));
+        int32_t *off = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(lds) + env_lds_bytes<LPS == 7, kLinkPass * OPS>(A.env));
         scan_remaining_chunks<G>(A.surv_nd, A.first_bad, (int)n_surv, L0, off);
         const int64_t R = off[n_surv];
         for (int64_t r0 = wave; r0 < R; r0 += 64 * waves_total) {
@@ -493,7 +494,7 @@ __global__ void __launch_bounds__(64) motion_single_kernel(EnvView env, Edge1 E,
                                                            int *result)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
     int total = 0;
     const int bad = walk_edge<LPS, OPS, false>(E.a, E.b, lvs, factor, lds, env.ns, env.nb, 0, total);
     if (lane_id() == 0) *result = bad == INT_MAX ? 1 : 0; // bad < 0: too many segments, invalid
@@ -531,7 +532,7 @@ __global__ void __launch_bounds__(256) state_lists_kernel(EnvView env, const dou
                                                           int32_t *__restrict__ first_invalid)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int G = Shape::STATES;
     const int lane = lane_id();
@@ -567,7 +568,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, double *__restrict__ last_t)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7>(A.env, lds);
+    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int G = Shape::STATES;
     const int lane = lane_id();
@@ -665,18 +666,21 @@ __global__ void arm_spheres_kernel(const float *__restrict__ q_soa, int64_t n, f
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-// Dynamic LDS of a launch: the staged environment, plus the spheres' exact rr
-// for the link layout (lanes_per_state 7).
-static size_t lds_bytes(const EnvView &e, int lps = 0)
+// Dynamic LDS of a launch: the staged environment; for the link layout
+// (lanes_per_state 7) the sphere and box lists padded to whole passes of
+// kLinkPass * ops entries plus the spheres' exact rr (stage_env<true, ...>).
+static size_t lds_bytes(const EnvView &e, int lps = 0, int ops = 1)
 {
-    return (size_t)(e.ns + 2 * e.nb) * sizeof(float4) + (lps == 7 ? (size_t)e.ns * sizeof(float) : 0);
+    if (lps != 7) return (size_t)(e.ns + 2 * e.nb) * sizeof(float4);
+    const int pad = kLinkPass * ops;
+    return (size_t)(pad_to(e.ns, pad) + 2 * pad_to(e.nb, pad)) * sizeof(float4) + (size_t)e.ns * sizeof(float);
 }
 
 // Where the link layout's staging exceeds one block's 64 KiB of LDS, the
 // (8, 8) shape checks the states instead (same results).
 static void fit_link_layout(const EnvView &e, int &lps, int &ops)
 {
-    if (lps == 7 && lds_bytes(e, 7) > 65536) {
+    if (lps == 7 && lds_bytes(e, 7, ops) > 65536) {
         lps = 8;
         ops = 8;
     }
@@ -698,7 +702,7 @@ void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, in
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * (64 / L);                                                     \
-        hipLaunchKernelGGL((validate_shared_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env, L), \
+        hipLaunchKernelGGL((validate_shared_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env, L, O), \
                            s, env, q, n, out);                                                                      \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
@@ -709,7 +713,7 @@ void launch_validate(const EnvView &env, const T *q, int64_t n, uint8_t *out, in
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
-        hipLaunchKernelGGL((validate_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env, L), s, env, \
+        hipLaunchKernelGGL((validate_kernel<L, O, T>), dim3(grid_for(n, per)), dim3(block), lds_bytes(env, L, O), s, env, \
                            q, n, out);                                                                              \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
@@ -725,7 +729,7 @@ template void launch_validate<double>(const EnvView &, const double *, int64_t, 
 
 size_t links_bytes(const EnvView &env)
 {
-    return lds_bytes(env, 7);
+    return (size_t)(env.ns + 2 * env.nb) * sizeof(float4) + (size_t)env.ns * sizeof(float);
 }
 
 void launch_stage_links(const EnvView &env, float4 *out, hipStream_t s)
@@ -752,7 +756,7 @@ void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hi
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * (64 / L);                                                     \
         hipLaunchKernelGGL((motion_endpoint_shared_kernel<L, O>), dim3(grid_for(A.n, per)), dim3(block),          \
-                           lds_bytes(A.env, L), s, A);                                                                \
+                           lds_bytes(A.env, L, O), s, A);                                                                \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -763,7 +767,7 @@ void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hi
     if (lps == L && ops == O) {                                                                                     \
         const int64_t per = (int64_t)(block / 64) * GroupShape<L, O>::STATES;                                     \
         hipLaunchKernelGGL((motion_endpoint_kernel<L, O>), dim3(grid_for(A.n, per)), dim3(block),                 \
-                           lds_bytes(A.env, L), s, A);                                                                \
+                           lds_bytes(A.env, L, O), s, A);                                                                \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -781,9 +785,9 @@ void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int b
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
         if (exact)                                                                                                  \
-            hipLaunchKernelGGL((motion_walk_kernel<L, O, true>), dim3(g), dim3(block), lds_bytes(A.env, L), s, A);   \
+            hipLaunchKernelGGL((motion_walk_kernel<L, O, true>), dim3(g), dim3(block), lds_bytes(A.env, L, O), s, A);   \
         else                                                                                                        \
-            hipLaunchKernelGGL((motion_walk_kernel<L, O, false>), dim3(g), dim3(block), lds_bytes(A.env, L), s, A);  \
+            hipLaunchKernelGGL((motion_walk_kernel<L, O, false>), dim3(g), dim3(block), lds_bytes(A.env, L, O), s, A);  \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }
@@ -796,12 +800,11 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
                           hipStream_t s, bool shared)
 {
     fit_link_layout(A.env, lps, ops);
-    ProfScope ps("motion_chunk", s);
     const unsigned g = grid_for(waves, block / 64);
     // LDS after the environment for the survivors' chunk offsets (edge-major
     // phase), where it fits beside the kernel's 4 KiB of static LDS
     MotionArgs B = A;
-    const size_t env_b = lds_bytes(A.env, lps), flat_b = (size_t)(kFlatSurvivors + 1) * sizeof(int32_t);
+    const size_t env_b = lds_bytes(A.env, lps, ops), flat_b = (size_t)(kFlatSurvivors + 1) * sizeof(int32_t);
     B.flat = env_b + flat_b + 4096 + 64 <= 65536;
     const size_t dyn = env_b + (B.flat ? flat_b : 0);
     // phase 1 (early chunks), then phase 2 (the rest and the results)
@@ -896,7 +899,7 @@ void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int 
     const unsigned g = grid_for(waves, block / 64);
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
-        hipLaunchKernelGGL((motion_last_valid_kernel<L, O>), dim3(g), dim3(block), lds_bytes(A.env, L), s, A, last_t); \
+        hipLaunchKernelGGL((motion_last_valid_kernel<L, O>), dim3(g), dim3(block), lds_bytes(A.env, L, O), s, A, last_t); \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }

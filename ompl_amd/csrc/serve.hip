@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
             LaneSpheres<LPS> S;
             arm_fk<LPS, Shape::GS>(q, sh.slot, S);
             const float4 *obs = LPS == 7 ? C.obs_links : C.obs;
-            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup>(S, obs, C.ns, C.nb, plive, sh.part);
+            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup, false>(S, obs, C.ns, C.nb, plive, sh.part);
             const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
             if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);
         }
