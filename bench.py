@@ -213,6 +213,8 @@ def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
 
 
 TRAFFIC_RECORD = ROOT / "profiles" / "motion_batch_16384_edges_256_obstacles.hbm_traffic.json"
+# the chunk walk's launches (strategies 3 and 4), timed one by one
+walk_parts = ("motion_chunk_early", "motion_chunk_rest", "motion_chunk_results")
 
 
 def hbm_traffic(kernel: str, tuning: dict):
@@ -229,8 +231,9 @@ def hbm_traffic(kernel: str, tuning: dict):
     lw, ow, le, oe = (tuning["motion_lanes_per_state"], tuning["motion_obstacle_parts"],
                       tuning["validity_lanes_per_state"], tuning["validity_obstacle_parts"])
     shared = "true" if tuning["motion_strategy"] == 4 and lw == 8 and ow > 1 else "false"
-    if kernel == "motion_chunk":  # both phases
+    if kernel == "motion_chunk":  # both phases and the results kernel
         names = [f"motion_chunk_kernel<{lw}, {ow}, false, {shared}, {ph}>" for ph in (1, 2)]
+        names.append("motion_chunk_results_kernel")
     elif kernel == "motion_endpoint":
         es = tuning.get("validity_shared_fk") and le == 8 and oe > 1
         names = [f"{'motion_endpoint_shared_kernel' if es else 'motion_endpoint_kernel'}<{le}, {oe}>"]
@@ -532,7 +535,7 @@ def main(argv=None):
         ms = timed(lambda: oa.check_motions(env, A, B, lvs, out=out), 30)
         oa.profile.enable(False)
         per = {}
-        for kname in ("motion_endpoint", "motion_chunk_early", "motion_chunk_rest"):
+        for kname in ("motion_endpoint",) + walk_parts:
             n, kms = oa.profile.query(kname)
             if n:
                 per[kname] = 1e3 * kms / n
@@ -568,14 +571,13 @@ def main(argv=None):
         oa.check_motions(env, A, B, lvs, out=out)
     torch.cuda.synchronize()
     kernels = {}
-    for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk_early", "motion_chunk_rest"):
+    for name in ("motion_fused", "motion_endpoint", "motion_walk") + walk_parts:
         n, ms = oa.profile.query(name)
         if n:
             kernels[name] = {"launches": n, "avg_us": 1e3 * ms / n}
-    if "motion_chunk_early" in kernels:  # the chunk walk: its two launches per step
+    if "motion_chunk_early" in kernels:  # the chunk walk: its three launches per step
         kernels["motion_chunk"] = {"launches": kernels["motion_chunk_early"]["launches"],
-                                   "avg_us": kernels["motion_chunk_early"]["avg_us"]
-                                   + kernels["motion_chunk_rest"]["avg_us"]}
+                                   "avg_us": sum(kernels[k]["avg_us"] for k in walk_parts)}
     oa.profile.enable(False)
     oa.profile.reset()
 
@@ -589,8 +591,8 @@ def main(argv=None):
     if t["motion_strategy"] == 0:
         share["motion_walk"] = flops_step
     # the dominant kernel is the one with the most time per step; the chunk
-    # walk's two launches count as one kernel (its work is not split by phase)
-    units = [k for k in kernels if k not in ("motion_chunk_early", "motion_chunk_rest")]
+    # walk's launches count as one kernel (its work is not split by phase)
+    units = [k for k in kernels if k not in walk_parts]
     dom = max(units, key=lambda k: kernels[k]["avg_us"])
     dom_flops = share[dom]
     dom_us = kernels[dom]["avg_us"]
@@ -694,7 +696,7 @@ def main(argv=None):
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "kernel": dom, "kernel_avg_us": dom_us, "algorithmic_flops_per_launch": dom_flops,
-                     "dispatches": 2 if dom == "motion_chunk" and "motion_chunk_early" in kernels else 1,
+                     "dispatches": 3 if dom == "motion_chunk" and "motion_chunk_early" in kernels else 1,
                      "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
         "roofline_s2_pass": roof_s2,
         "cpu_baseline": cpu,

@@ -37,8 +37,8 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
     const int LW = t.motion_lanes_per_state, OW = t.motion_obstacle_parts;
     const int64_t persistent_waves = (int64_t)num_cus() * t.motion_waves_per_cu;
     if (t.motion_strategy == 3 || t.motion_strategy == 4) {
-        // counters [8..10] belong to these strategies only: zero when the
-        // workspace is created, cleared by the last block of every chunk walk.
+        // counters [8..9] belong to these strategies only: zero when the
+        // workspace is created, cleared by the results kernel of every walk.
         // Strategy 4 walks chunks with each position's FK computed once per
         // wave and shared across the obstacle partitions; the s2 pass shares
         // its FK when validity_shared_fk is set.
@@ -51,7 +51,6 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         A.first_bad = w.first_bad;
         A.surv_nd = w.surv_nd;
         A.surv_ab = w.surv_ab;
-        A.done_blocks = w.counters + 10;
         A.start = 1;
         launch_motion_endpoint(A, LE, OE, t.validity_block, s, eshared);
         launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s, shared);

@@ -375,7 +375,7 @@ __device__ __forceinline__ void walk_window(const MotionArgs &A, const float4 *l
 //     survivor by binary search in the offsets.  L0 = W / n_surv (at least
 //     1): phase 1 is one item per wave.
 //   Without phase 2, phase 1 covers every chunk (L0 = all).
-// The last block of phase 2 writes the survivors' results.
+// motion_chunk_results_kernel then writes the survivors' results.
 template <int LPS, int OPS, bool EXACT, bool SHARED, int PHASE>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
@@ -429,21 +429,18 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
             walk_window<LPS, OPS, EXACT, SHARED>(A, lds, s, base, nd, live);
         }
     }
-    if (!A.done_blocks) return;
-    // the last block to finish writes the survivors' results and clears the
-    // counters for the next batch (one counter update per block)
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(A.done_blocks, 1) == (int)gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+
+// Results of the chunk walk's survivors (one block, after both phases): the
+// kernel boundary makes every block's first-invalid updates visible, which
+// costs less than an agent-scope fence in each block of the walk.  Clears the
+// counters for the next batch.
+__global__ void __launch_bounds__(1024) motion_chunk_results_kernel(MotionArgs A)
+{
+    const int64_t n_surv = *A.list_count;
     for (int64_t sv = threadIdx.x; sv < n_surv; sv += blockDim.x) {
         const int64_t e = A.list[sv];
-        const int fb = __hip_atomic_load(&A.first_bad[sv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int fb = A.first_bad[sv];
         A.out[e] = fb == INT_MAX ? 1 : 0;
         if (A.checked) A.checked[e] = fb == INT_MAX ? A.surv_nd[sv] : fb + 1;
     }
@@ -451,7 +448,6 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
     if (threadIdx.x == 0) {
         *A.list_count = 0;
         *A.max_m = 0;
-        *A.done_blocks = 0;
     }
 }
 
@@ -807,14 +803,18 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
     const size_t env_b = lds_bytes(A.env, lps, ops), flat_b = (size_t)(kFlatSurvivors + 1) * sizeof(int32_t);
     B.flat = env_b + flat_b + 4096 + 64 <= 65536;
     const size_t dyn = env_b + (B.flat ? flat_b : 0);
-    // phase 1 (early chunks), then phase 2 (the rest and the results)
+    // phase 1 (early chunks), phase 2 (the rest), then the results
     auto launch = [&](auto early, auto rest) {
         {
             ProfScope p1("motion_chunk_early", s);
             hipLaunchKernelGGL(early, dim3(g), dim3(block), dyn, s, B);
         }
-        ProfScope p2("motion_chunk_rest", s);
-        hipLaunchKernelGGL(rest, dim3(g), dim3(block), dyn, s, B);
+        {
+            ProfScope p2("motion_chunk_rest", s);
+            hipLaunchKernelGGL(rest, dim3(g), dim3(block), dyn, s, B);
+        }
+        ProfScope p3("motion_chunk_results", s);
+        hipLaunchKernelGGL(motion_chunk_results_kernel, dim3(1), dim3(1024), 0, s, B);
     };
 
     if (shared && ops > 1) {

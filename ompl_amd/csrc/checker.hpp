@@ -26,7 +26,6 @@ struct MotionArgs {
     int32_t *max_m;      // largest bisection position count among survivors
     int32_t *surv_nd;    // per survivor: segment count nd
     double *surv_ab;     // per survivor: endpoints a then b (14 doubles, contiguous), optional
-    int32_t *done_blocks; // chunk walk: finished blocks; the last one finalizes (optional)
     int flat;             // chunk walk: the launch reserved LDS for the survivors' chunk offsets
     int start;           // first walk position (0 = s2, 1 = skip s2)
 };

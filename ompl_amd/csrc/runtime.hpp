@@ -21,8 +21,8 @@ struct Workspace {
     double *surv_ab = nullptr;    // per survivor endpoints a, b (14 doubles)
     size_t list_cap = 0;
     // [0] survivor count, [1] walk queue (strategy 1, cleared per call);
-    // [8] survivor count, [9] max positions, [10] finished chunk-walk blocks
-    // (strategies 3 and 4: zero at creation, cleared by the walk's last block)
+    // [8] survivor count, [9] max positions (strategies 3 and 4: zero at
+    // creation, cleared by the walk's results kernel)
     int32_t *counters = nullptr;
 };
 Workspace &workspace(hipStream_t s, size_t list_entries);
