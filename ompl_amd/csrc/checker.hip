@@ -16,11 +16,12 @@
 //   motion_walk_kernel      one wave per edge: positions 0 (s2) and 1..nd-1 (the
 //                           bisection order of DiscreteMotionValidator.cpp:106-134)
 //                           in chunks, stop at the first chunk with a collision
-//   motion_chunk_kernel     chunk-parallel walk of the survivors: work item
-//                           (chunk b, survivor s) in chunk-major order over a
-//                           persistent grid; a chunk is skipped once an earlier
-//                           position of its edge is known invalid (atomicMin);
-//                           its last block writes the survivors' results
+//   motion_chunk_kernel     chunk-parallel walk of the survivors in two
+//                           launches (early chunks chunk-major, then the
+//                           unresolved survivors' remaining chunks); a chunk is
+//                           skipped once an earlier position of its edge is
+//                           known invalid (atomicMin); motion_chunk_results_kernel
+//                           then writes the survivors' results
 //   motion_fused_kernel     s2 of the wave's edges, then the wave walks its own
 //                           survivors (one launch)
 //   motion_single_kernel    one edge passed by value (the plain per-call path)
