@@ -9,7 +9,10 @@ uniform on S^6, lengths lognormal(mu=-1, sigma=1) clipped to [0.05, 5] rad,
 exit.  Inputs are resident in HBM before the timed region.
 
 Reported (one JSON line):
-  value            batched edge-checks/s (edges validated per second)
+  value            batched edge-checks/s (edges validated per second), the K
+                   steps issued round-robin on --streams HIP streams (2):
+                   independent batches, each fully checked; the one-stream
+                   figure is reported beside it
   plain            the same edges through the one-launch-per-call path
   roofline         dominant kernel: algorithmic FP32 work in the reference order
                    / its average duration (HIP events on its launch stream)
@@ -428,6 +431,8 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--streams", type=int, default=2,
+                    help="HIP streams the timed steps are issued on, round-robin (independent batches)")
     ap.add_argument("--plain-edges", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -489,21 +494,43 @@ def main(argv=None):
         oa.check_motions(env, A, B, lvs, out=out)
     torch.cuda.synchronize()
 
-    # ---- timed region: K batched steps
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # ---- timed region: K batched steps, issued round-robin on S HIP streams
+    #      (independent batches: each stream has its own output and survivor
+    #      workspace), between events on the launch stream that waits for all
+    def run_steps(n_streams, k):
+        side = [torch.cuda.Stream() for _ in range(n_streams - 1)]  # default priority
+        outs = [out] + [torch.empty_like(out) for _ in side]
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for j, sd in enumerate(side):  # warm the side streams' workspaces
+            with torch.cuda.stream(sd):
+                oa.check_motions(env, A, B, lvs, out=outs[1 + j])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for sd in side:
+            sd.wait_event(ev0)
+        for i in range(k):
+            j = i % n_streams
+            if j == 0:
+                oa.check_motions(env, A, B, lvs, out=outs[0])
+            else:
+                with torch.cuda.stream(side[j - 1]):
+                    oa.check_motions(env, A, B, lvs, out=outs[j])
+        for sd in side:
+            stream.wait_stream(sd)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall_s = time.perf_counter() - t0
+        same = all(np.array_equal(o.cpu().numpy(), outs[0].cpu().numpy()) for o in outs[1:])
+        return ev0.elapsed_time(ev1), wall_s, same
+
     step_ms = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        oa.check_motions(env, A, B, lvs, out=out)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    gpu_ms, wall, streams_same = run_steps(args.streams, args.steps)
     ms_per_step = gpu_ms / args.steps
     value = N_EDGES * args.steps / (gpu_ms / 1e3)
     valid = out.cpu().numpy().copy()
+    single_ms, _, _ = run_steps(1, args.steps) if args.streams > 1 else (gpu_ms, wall, True)
 
     # ---- per-step latency (synchronous steps) for p50/p99
     for _ in range(min(args.steps, 200)):
@@ -680,7 +707,10 @@ def main(argv=None):
         "data": "synthetic (seeded std::mt19937 workload)",
         "config": {"workload": "uneven motion validation: 7-DOF arm, 128 spheres + 128 boxes, 16384 edges "
                                "lognormal(-1,1) length, 0.01 rad, bisection + first-collision exit",
-                   "edges": N_EDGES, "obstacles": N_SPHERES + N_BOXES, "tuning": t},
+                   "edges": N_EDGES, "obstacles": N_SPHERES + N_BOXES, "tuning": t, "streams": args.streams},
+        "one_stream": {"edge_checks_per_s": N_EDGES * args.steps / (single_ms / 1e3),
+                       "ms_per_step": single_ms / args.steps},
+        "streams_same_result": streams_same,
         "kernels": kernels,
         "valid_fraction": float(valid.mean()),
         "step_latency_ms": {"p50": percentile(step_ms, 50), "p99": percentile(step_ms, 99)},
