@@ -216,8 +216,6 @@ def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
 
 
 TRAFFIC_RECORD = ROOT / "profiles" / "motion_batch_16384_edges_256_obstacles.hbm_traffic.json"
-# the chunk walk's launches (strategies 3 and 4), timed one by one
-walk_parts = ("motion_chunk_early", "motion_chunk_rest", "motion_chunk_results")
 
 
 def hbm_traffic(kernel: str, tuning: dict):
@@ -562,7 +560,7 @@ def main(argv=None):
         ms = timed(lambda: oa.check_motions(env, A, B, lvs, out=out), 30)
         oa.profile.enable(False)
         per = {}
-        for kname in ("motion_endpoint",) + walk_parts:
+        for kname in ("motion_endpoint", "motion_chunk"):
             n, kms = oa.profile.query(kname)
             if n:
                 per[kname] = 1e3 * kms / n
@@ -598,13 +596,10 @@ def main(argv=None):
         oa.check_motions(env, A, B, lvs, out=out)
     torch.cuda.synchronize()
     kernels = {}
-    for name in ("motion_fused", "motion_endpoint", "motion_walk") + walk_parts:
+    for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk"):
         n, ms = oa.profile.query(name)
         if n:
             kernels[name] = {"launches": n, "avg_us": 1e3 * ms / n}
-    if "motion_chunk_early" in kernels:  # the chunk walk: its three launches per step
-        kernels["motion_chunk"] = {"launches": kernels["motion_chunk_early"]["launches"],
-                                   "avg_us": sum(kernels[k]["avg_us"] for k in walk_parts)}
     oa.profile.enable(False)
     oa.profile.reset()
 
@@ -618,9 +613,9 @@ def main(argv=None):
     if t["motion_strategy"] == 0:
         share["motion_walk"] = flops_step
     # the dominant kernel is the one with the most time per step; the chunk
-    # walk's launches count as one kernel (its work is not split by phase)
-    units = [k for k in kernels if k not in walk_parts]
-    dom = max(units, key=lambda k: kernels[k]["avg_us"])
+    # walk's three launches (one event scope) count as one kernel (its work is
+    # not split by phase)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"])
     dom_flops = share[dom]
     dom_us = kernels[dom]["avg_us"]
     achieved = dom_flops / (dom_us * 1e-6) / 1e12
@@ -726,7 +721,7 @@ def main(argv=None):
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "kernel": dom, "kernel_avg_us": dom_us, "algorithmic_flops_per_launch": dom_flops,
-                     "dispatches": 3 if dom == "motion_chunk" and "motion_chunk_early" in kernels else 1,
+                     "dispatches": 3 if dom == "motion_chunk" else 1,
                      "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
         "roofline_s2_pass": roof_s2,
         "cpu_baseline": cpu,

@@ -804,17 +804,12 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
     const size_t env_b = lds_bytes(A.env, lps, ops), flat_b = (size_t)(kFlatSurvivors + 1) * sizeof(int32_t);
     B.flat = env_b + flat_b + 4096 + 64 <= 65536;
     const size_t dyn = env_b + (B.flat ? flat_b : 0);
-    // phase 1 (early chunks), phase 2 (the rest), then the results
+    // phase 1 (early chunks), phase 2 (the rest), then the results; one
+    // profiling scope around the three (a scope adds its own dispatch latency)
     auto launch = [&](auto early, auto rest) {
-        {
-            ProfScope p1("motion_chunk_early", s);
-            hipLaunchKernelGGL(early, dim3(g), dim3(block), dyn, s, B);
-        }
-        {
-            ProfScope p2("motion_chunk_rest", s);
-            hipLaunchKernelGGL(rest, dim3(g), dim3(block), dyn, s, B);
-        }
-        ProfScope p3("motion_chunk_results", s);
+        ProfScope ps("motion_chunk", s);
+        hipLaunchKernelGGL(early, dim3(g), dim3(block), dyn, s, B);
+        hipLaunchKernelGGL(rest, dim3(g), dim3(block), dyn, s, B);
         hipLaunchKernelGGL(motion_chunk_results_kernel, dim3(1), dim3(1024), 0, s, B);
     };
 
