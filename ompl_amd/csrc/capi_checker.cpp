@@ -37,23 +37,30 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
     const int LW = t.motion_lanes_per_state, OW = t.motion_obstacle_parts;
     const int64_t persistent_waves = (int64_t)num_cus() * t.motion_waves_per_cu;
     if (t.motion_strategy == 3 || t.motion_strategy == 4) {
-        // counters [8..9] belong to these strategies only: zero when the
-        // workspace is created, cleared by the results kernel of every walk.
         // Strategy 4 walks chunks with each position's FK computed once per
         // wave and shared across the obstacle partitions; the s2 pass shares
-        // its FK when validity_shared_fk is set.
+        // its FK when validity_shared_fk is set.  The survivors' booleans are
+        // written by the s2 pass (1) and the walk (0 at an invalid position);
+        // a results kernel runs after the walk when checked counts are asked
+        // for, and for a captured graph (which replays one counter set), and
+        // clears the batch's counter set; otherwise phase 1 of the walk clears
+        // the other set, which the next batch on this workspace uses.
         const bool shared = t.motion_strategy == 4;
         const bool eshared = t.validity_shared_fk != 0;
         Workspace &w = own ? *own : workspace(s, (size_t)n);
+        const bool results = exact || own != nullptr;
+        int32_t *cur = w.counters + 8 + 2 * w.parity, *other = w.counters + 8 + 2 * (1 - w.parity);
         A.list = w.list;
-        A.list_count = w.counters + 8;
-        A.max_m = w.counters + 9;
+        A.list_count = cur;
+        A.max_m = cur + 1;
+        A.reset_next = results ? nullptr : other;
         A.first_bad = w.first_bad;
         A.surv_nd = w.surv_nd;
         A.surv_ab = w.surv_ab;
         A.start = 1;
         launch_motion_endpoint(A, LE, OE, t.validity_block, s, eshared);
-        launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s, shared);
+        launch_motion_chunks(A, exact, LW, OW, t.motion_block, persistent_waves, s, shared, results);
+        if (!results) w.parity ^= 1;
     } else if (t.motion_strategy == 2) {
         A.start = 1;
         launch_motion_fused(A, exact, LE, OE, LW, OW, t.motion_block, s);

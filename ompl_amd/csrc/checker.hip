@@ -175,6 +175,7 @@ __device__ __forceinline__ void append_survivor(const MotionArgs &A, int64_t e, 
     base = __shfl(base, lowest_bit(sm));
     if (survivor) {
         const int slot = base + __popcll(sm & (((uint64_t)1 << lane) - 1));
+        A.out[e] = 1; // until the walk finds an invalid position
         A.list[slot] = (int32_t)e;
         if (A.first_bad) A.first_bad[slot] = INT_MAX;
         if (A.surv_nd) A.surv_nd[slot] = m + 1;
@@ -346,7 +347,10 @@ __device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds
     }
     const int bad = SHARED ? walk_chunk_shared<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb)
                            : walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
-    if (bad != INT_MAX && lane_id() == 0) atomicMin(&A.first_bad[sj], bad);
+    if (bad != INT_MAX && lane_id() == 0) {
+        atomicMin(&A.first_bad[sj], bad);
+        A.out[A.list[sj]] = 0; // the same byte from every wave that finds one
+    }
 }
 
 // Walks the live items of a 64-item window (one per lane) in lane order.
@@ -381,6 +385,11 @@ template <int LPS, int OPS, bool EXACT, bool SHARED, int PHASE>
 __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
+    // without a results kernel, phase 1 clears the counter set the next batch uses
+    if (PHASE == 1 && A.reset_next && blockIdx.x == 0 && threadIdx.x == 0) {
+        A.reset_next[0] = 0;
+        A.reset_next[1] = 0;
+    }
     stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
     constexpr int G = SHARED ? 64 / LPS : GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
@@ -794,7 +803,7 @@ void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int b
 }
 
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
-                          hipStream_t s, bool shared)
+                          hipStream_t s, bool shared, bool results)
 {
     fit_link_layout(A.env, lps, ops);
     const unsigned g = grid_for(waves, block / 64);
@@ -804,13 +813,14 @@ void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int
     const size_t env_b = lds_bytes(A.env, lps, ops), flat_b = (size_t)(kFlatSurvivors + 1) * sizeof(int32_t);
     B.flat = env_b + flat_b + 4096 + 64 <= 65536;
     const size_t dyn = env_b + (B.flat ? flat_b : 0);
-    // phase 1 (early chunks), phase 2 (the rest), then the results; one
-    // profiling scope around the three (a scope adds its own dispatch latency)
+    // phase 1 (early chunks), phase 2 (the rest), then (checked counts, or a
+    // captured graph) the results; one profiling scope around them (a scope
+    // adds its own dispatch latency)
     auto launch = [&](auto early, auto rest) {
         ProfScope ps("motion_chunk", s);
         hipLaunchKernelGGL(early, dim3(g), dim3(block), dyn, s, B);
         hipLaunchKernelGGL(rest, dim3(g), dim3(block), dyn, s, B);
-        hipLaunchKernelGGL(motion_chunk_results_kernel, dim3(1), dim3(1024), 0, s, B);
+        if (results) hipLaunchKernelGGL(motion_chunk_results_kernel, dim3(1), dim3(1024), 0, s, B);
     };
 
     if (shared && ops > 1) {

@@ -27,6 +27,7 @@ struct MotionArgs {
     int32_t *surv_nd;    // per survivor: segment count nd
     double *surv_ab;     // per survivor: endpoints a then b (14 doubles, contiguous), optional
     int flat;             // chunk walk: the launch reserved LDS for the survivors' chunk offsets
+    int32_t *reset_next;  // chunk walk without a results kernel: the counter set phase 1 clears
     int start;           // first walk position (0 = s2, 1 = skip s2)
 };
 
@@ -52,7 +53,7 @@ void launch_zero(int *p, int n, hipStream_t s);
 void launch_motion_endpoint(const MotionArgs &A, int lps, int ops, int block, hipStream_t s, bool shared = false);
 void launch_motion_walk(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves, hipStream_t s);
 void launch_motion_chunks(const MotionArgs &A, bool exact, int lps, int ops, int block, int64_t waves,
-                          hipStream_t s, bool shared = false);
+                          hipStream_t s, bool shared, bool results);
 void launch_motion_fused(const MotionArgs &A, bool exact, int le, int oe, int lw, int ow, int block, hipStream_t s);
 void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result,
                           hipStream_t s);

@@ -21,9 +21,12 @@ struct Workspace {
     double *surv_ab = nullptr;    // per survivor endpoints a, b (14 doubles)
     size_t list_cap = 0;
     // [0] survivor count, [1] walk queue (strategy 1, cleared per call);
-    // [8] survivor count, [9] max positions (strategies 3 and 4: zero at
-    // creation, cleared by the walk's results kernel)
+    // strategies 3 and 4: two sets of (survivor count, max positions) at
+    // [8..9] and [10..11], zero at creation.  A batch uses set `parity`; either
+    // its results kernel clears that set, or (no results kernel) its phase-1
+    // walk clears the other set and the next batch uses that one.
     int32_t *counters = nullptr;
+    int parity = 0;
 };
 Workspace &workspace(hipStream_t s, size_t list_entries);
 
