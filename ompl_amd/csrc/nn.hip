@@ -220,6 +220,25 @@ __device__ __forceinline__ f32x16 tile_mma(const float *__restrict__ xt, int64_t
     return acc;
 }
 
+// The lane's A operands of tile t (one float per K step), and the MFMA chain
+// on operands loaded beforehand.
+template <int KSTEPS>
+__device__ __forceinline__ void tile_load(const float *__restrict__ xt, int64_t t, int lane, float v[KSTEPS])
+{
+    const float *base = xt + t * (KSTEPS * 64) + lane;
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) v[s] = base[s * 64];
+}
+
+template <int KSTEPS>
+__device__ __forceinline__ f32x16 tile_mma_regs(const float v[KSTEPS], const float *b)
+{
+    f32x16 acc = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v[s], b[s], acc, 0, 0, 0);
+    return acc;
+}
+
 // Orders the wave's LDS accesses: its lanes exchange data through the wave's
 // own LDS slice, so only the wave itself needs to be ordered.
 __device__ __forceinline__ void wave_lds_sync()
@@ -234,9 +253,11 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
     // the current tile's 32 points, per wave: candidates that pass the MFMA
     // filter read their exact coordinates from here, not from HBM
     __shared__ float tile_pts[4][32 * DIM];
+    __shared__ uint8_t tile_rm[4][32]; // the tile's removal flags (1 past the end)
     const int lane = threadIdx.x & 63;
     const int h = lane >> 5;
     float *tp = tile_pts[(threadIdx.x >> 6) & 3];
+    uint8_t *trm = tile_rm[(threadIdx.x >> 6) & 3];
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t qtiles = (A.nq + 31) / 32;
     const int64_t qtile = wave % qtiles;
@@ -252,8 +273,15 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
     const int64_t t0 = (int64_t)split * A.tiles_per_split;
     const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
     const int64_t n_words = A.n * DIM;
+    float cur[KSTEPS];
+    if (t0 < t1) tile_load<KSTEPS>(A.xt, t0, lane, cur);
     for (int64_t t = t0; t < t1; ++t) {
-        const f32x16 acc = tile_mma<KSTEPS>(A.xt, t, lane, Q.b);
+        // the next tile's operands are in flight while this one is filtered
+        float nxt[KSTEPS];
+        tile_load<KSTEPS>(A.xt, t + 1 < t1 ? t + 1 : t, lane, nxt);
+        const f32x16 acc = tile_mma_regs<KSTEPS>(cur, Q.b);
+#pragma unroll
+        for (int s2 = 0; s2 < KSTEPS; ++s2) cur[s2] = nxt[s2];
         uint32_t mask = 0;
 #pragma unroll
         for (int j = 0; j < 16; ++j) mask |= (acc[j] <= thr ? 1u : 0u) << j;
@@ -262,13 +290,14 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
         wave_lds_sync(); // every lane is done with the previous tile's points
         const int64_t w0 = t * 32 * DIM;
         for (int i = lane; i < 32 * DIM; i += 64) tp[i] = w0 + i < n_words ? A.pts[w0 + i] : 0.0f;
+        if (lane < 32) trm[lane] = t * 32 + lane < A.n ? A.removed[t * 32 + lane] : (uint8_t)1;
         wave_lds_sync();
         while (mask) {
             const int j = __builtin_ctz(mask);
             mask &= mask - 1;
             const int row = mfma_row(j, h);
             const int64_t p = t * 32 + row;
-            if (p >= A.n || A.removed[p]) continue;
+            if (trm[row]) continue;
             const float d = exact_dist<METRIC, DIM>(tp + row * DIM, Q.q);
             if (key_less(d, (int)p, kd, ki)) {
                 top.insert(d, (int)p);
@@ -295,28 +324,53 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
     }
 }
 
-// Merges the per-split partial lists of each query (one lane per query).
+// Merges the per-split partial lists of each query: one wave per query.
+// Each lane keeps the k best keys (distance, then index) of its share of the
+// splits' entries (lanes read consecutive entries: coalesced), then k rounds
+// of a wave-wide minimum take the k best overall in key order, the lane that
+// holds each winner dropping it from its list.  Point ids are distinct across
+// splits, so every key is held by one lane.
 template <int KMAX>
-__global__ void knn_merge_kernel(const float *__restrict__ pd, const int32_t *__restrict__ pi, int64_t nq, int k,
-                                 int splits, float *__restrict__ out_d, int32_t *__restrict__ out_i)
+__global__ void __launch_bounds__(256) knn_merge_kernel(const float *__restrict__ pd, const int32_t *__restrict__ pi,
+                                                        int64_t nq, int k, int splits, float *__restrict__ out_d,
+                                                        int32_t *__restrict__ out_i)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (q >= nq) return; // wave-uniform
     TopK<KMAX> top;
     top.init();
-    for (int s = 0; s < splits; ++s) {
-        const int64_t base = ((int64_t)s * nq + q) * k;
-        for (int j = 0; j < k; ++j) {
-            const int id = pi[base + j];
-            if (id < 0) break;
-            top.insert(pd[base + j], id);
-        }
+    const int total = splits * k;
+    for (int c = lane; c < total; c += 64) {
+        const int sp = c / k, j = c - sp * k;
+        const int64_t at = ((int64_t)sp * nq + q) * k + j;
+        const int id = pi[at];
+        if (id >= 0) top.insert(pd[at], id);
     }
+    for (int r = 0; r < k; ++r) {
+        float bd = top.d[0];
+        int bi = top.i[0];
 #pragma unroll
-    for (int s = 0; s < KMAX; ++s) {
-        if (s < k) {
-            out_d[q * k + s] = top.d[s];
-            out_i[q * k + s] = top.i[s] == INT_MAX ? -1 : top.i[s];
+        for (int off = 32; off > 0; off >>= 1) {
+            const float od = __shfl_xor(bd, off);
+            const int oi = __shfl_xor(bi, off);
+            if (key_less(od, oi, bd, bi)) {
+                bd = od;
+                bi = oi;
+            }
+        }
+        if (bi != INT_MAX && top.i[0] == bi && top.d[0] == bd) {
+#pragma unroll
+            for (int s = 0; s + 1 < KMAX; ++s) {
+                top.d[s] = top.d[s + 1];
+                top.i[s] = top.i[s + 1];
+            }
+            top.d[KMAX - 1] = INFINITY;
+            top.i[KMAX - 1] = INT_MAX;
+        }
+        if (lane == 0) {
+            out_d[q * k + r] = bd;
+            out_i[q * k + r] = bi == INT_MAX ? -1 : bi;
         }
     }
 }
@@ -580,7 +634,7 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
         A.out_d = (float *)buf;
         A.out_i = (int32_t *)(buf + per * sizeof(float));
         dispatch_search<0>(x, A, kmax, nullptr, qtiles * splits, s, false);
-        const unsigned blocks = (unsigned)((nq + 255) / 256);
+        const unsigned blocks = (unsigned)((nq + 3) / 4); // one wave per query
         switch (kmax) {
         case 16:
             hipLaunchKernelGGL(knn_merge_kernel<16>, dim3(blocks), dim3(256), 0, s, A.out_d, A.out_i, nq, k,
