@@ -345,3 +345,70 @@ def test_capabilities_match_enforced_limits(gpu):
     gpu.Env(np.zeros((w - 2, 4), np.float32), np.zeros((1, 6), np.float32)).close()  # exactly at the limit
     with pytest.raises(OmplAmdError):
         gpu.Env(np.zeros((w - 1, 4), np.float32), np.zeros((1, 6), np.float32))
+
+
+def test_counter_sets_across_mixed_calls(gpu, oracle):
+    """Batches with and without checked counts, captured-graph replays, a batch
+    with no survivors and batches of other sizes, in one sequence on one stream
+    (the walk's counter sets alternate only for batches without a results
+    kernel): every result equals the oracle's."""
+    import torch
+
+    gpu.set_tuning(motion_strategy=3, validity_lanes_per_state=7, validity_obstacle_parts=4,
+                   motion_lanes_per_state=7, motion_obstacle_parts=9)
+    env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
+    lvs = lvs_config()
+    cases = []
+    for seed, n in ((4, 4096), (5, 1000), (6, 77)):
+        a, b = motion_case(gpu, seed, n)
+        cases.append((to_dev(a.T.copy()), to_dev(b.T.copy()), *oracle.check_motions(oenv, a, b, lvs)))
+    # a batch whose every s2 collides: no survivors (s2 at the all-zero configuration inside an obstacle is
+    # not guaranteed, so take edges ending in states the oracle marks invalid)
+    a, b = motion_case(gpu, 7, 3000)
+    s2_ok = oracle.validate_states(oenv, np.ascontiguousarray(b.T.astype(np.float32)))
+    bad = np.nonzero(s2_ok == 0)[0][:500]
+    a0, b0 = a[bad], b[bad]
+    cases.append((to_dev(a0.T.copy()), to_dev(b0.T.copy()), *oracle.check_motions(oenv, a0, b0, lvs)))
+    A0, B0, ref_v0, ref_c0 = cases[0]
+    g = gpu.MotionGraph(env, A0, B0, lvs)
+    for rep in range(3):
+        for i, (A, B, ref_v, ref_c) in enumerate(cases):
+            print(f"mixed calls rep {rep} case {i}", flush=True)
+            v = gpu.check_motions(env, A, B, lvs)
+            assert np.array_equal(v.cpu().numpy(), ref_v), (rep, i)
+            if i % 2 == rep % 2:
+                v, c = gpu.check_motions(env, A, B, lvs, exact_count=True)
+                assert np.array_equal(v.cpu().numpy(), ref_v) and np.array_equal(c.cpu().numpy(), ref_c), (rep, i)
+            g.launch()
+            torch.cuda.synchronize()
+            assert np.array_equal(g.out.cpu().numpy(), ref_v0), (rep, i)
+    g.close()
+    gpu.set_tuning(**DEFAULT)
+
+
+def test_two_streams_concurrently(gpu, oracle):
+    """Batches issued round-robin on two streams (each with its own survivor
+    workspace) give the oracle's results."""
+    import torch
+
+    gpu.set_tuning(motion_strategy=3, validity_lanes_per_state=7, validity_obstacle_parts=4,
+                   motion_lanes_per_state=7, motion_obstacle_parts=9)
+    env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
+    lvs = lvs_config()
+    a1, b1 = motion_case(gpu, 4, 8192)
+    a2, b2 = motion_case(gpu, 8, 5000)
+    r1, _ = oracle.check_motions(oenv, a1, b1, lvs)
+    r2, _ = oracle.check_motions(oenv, a2, b2, lvs)
+    A1, B1, A2, B2 = to_dev(a1.T.copy()), to_dev(b1.T.copy()), to_dev(a2.T.copy()), to_dev(b2.T.copy())
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    o1 = [torch.empty(8192, dtype=torch.uint8, device="cuda") for _ in range(4)]
+    o2 = [torch.empty(5000, dtype=torch.uint8, device="cuda") for _ in range(4)]
+    for i in range(4):
+        with torch.cuda.stream(s1):
+            gpu.check_motions(env, A1, B1, lvs, out=o1[i])
+        with torch.cuda.stream(s2):
+            gpu.check_motions(env, A2, B2, lvs, out=o2[i])
+    torch.cuda.synchronize()
+    for i in range(4):
+        assert np.array_equal(o1[i].cpu().numpy(), r1) and np.array_equal(o2[i].cpu().numpy(), r2), i
+    gpu.set_tuning(**DEFAULT)
