@@ -83,8 +83,16 @@ __device__ __forceinline__ float sphere_bound2(float rr)
     return b * b * 1.000001f;
 }
 
-// Obstacles per lane per pass of the link layout's bound tests.
-constexpr int kLinkPass = 8;
+// Obstacles per lane per pass of the link layout's bound tests: 16 with 9
+// obstacle partitions (one pass covers 144 obstacles of a type: a state of
+// the 128 + 128 benchmark environment takes one sphere and one box pass),
+// else 8 (shorter passes let colliding states leave sooner; measured: the
+// (7, 4) s2 pass 18.4 us with 8, 18.9 with 16; the (7, 9) walk 19.1 us with
+// 8, 18.0 with 16).
+__host__ __device__ constexpr int link_pass(int ops)
+{
+    return ops >= 9 ? 16 : 8;
+}
 
 __host__ __device__ constexpr int pad_to(int n, int m)
 {
@@ -428,7 +436,7 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
                                               int part)
 {
     constexpr int GS = GroupShape<7, OPS>::GS;
-    constexpr int K = kLinkPass;
+    constexpr int K = link_pass(OPS);
     // PADDED: lists padded to whole passes (stage_env<true, K * OPS>), no
     // index clamps; else the unpadded layout, indices clamped to the last entry
     const int nsp = PADDED ? pad_to(ns, K * OPS) : ns, nbp = PADDED ? pad_to(nb, K * OPS) : nb;
@@ -497,7 +505,7 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
 }
 
 // PADDED (link layout only): the obstacles are in the padded LDS staging of
-// stage_env<true, kLinkPass * OPS>; false for the unpadded copy in HBM.
+// stage_env<true, link_pass(OPS) * OPS>; false for the unpadded copy in HBM.
 template <int LPS, int OPS, Exit E, bool PADDED = true>
 __device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live,
                                             int part)

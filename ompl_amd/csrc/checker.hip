@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
                                                         uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
     const Shape sh(lane);
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(1024) validate_shared_kernel(EnvView env, cons
                                                                uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
@@ -195,7 +195,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int64_t wave = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t e = wave * Shape::STATES + lane_id() / Shape::GS;
@@ -211,7 +211,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_shared_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int FK = 64 / LPS;
     const int lane = lane_id();
@@ -242,7 +242,7 @@ template <int LPS, int OPS, bool EXACT>
 __global__ void __launch_bounds__(1024) motion_walk_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
     const int lane = lane_id();
     const int waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t count = A.list ? (int64_t)(*A.list_count) : A.n;
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
         A.reset_next[0] = 0;
         A.reset_next[1] = 0;
     }
-    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
     constexpr int G = SHARED ? 64 / LPS : GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
         return;
     }
     if (flat) {
-        int32_t *off = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(lds) + env_lds_bytes<LPS == 7, kLinkPass * OPS>(A.env));
+        int32_t *off = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(lds) + env_lds_bytes<LPS == 7, link_pass(OPS) * OPS>(A.env));
         scan_remaining_chunks<G>(A.surv_nd, A.first_bad, (int)n_surv, L0, off);
         const int64_t R = off[n_surv];
         for (int64_t r0 = wave; r0 < R; r0 += 64 * waves_total) {
@@ -500,7 +500,7 @@ __global__ void __launch_bounds__(64) motion_single_kernel(EnvView env, Edge1 E,
                                                            int *result)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(env, lds);
     int total = 0;
     const int bad = walk_edge<LPS, OPS, false>(E.a, E.b, lvs, factor, lds, env.ns, env.nb, 0, total);
     if (lane_id() == 0) *result = bad == INT_MAX ? 1 : 0; // bad < 0: too many segments, invalid
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(256) state_lists_kernel(EnvView env, const dou
                                                           int32_t *__restrict__ first_invalid)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int G = Shape::STATES;
     const int lane = lane_id();
@@ -574,7 +574,7 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, double *__restrict__ last_t)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, kLinkPass * OPS>(A.env, lds);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     constexpr int G = Shape::STATES;
     const int lane = lane_id();
@@ -674,11 +674,11 @@ __global__ void arm_spheres_kernel(const float *__restrict__ q_soa, int64_t n, f
 // ---------------------------------------------------------------------------
 // Dynamic LDS of a launch: the staged environment; for the link layout
 // (lanes_per_state 7) the sphere and box lists padded to whole passes of
-// kLinkPass * ops entries plus the spheres' exact rr (stage_env<true, ...>).
+// link_pass(ops) * ops entries plus the spheres' exact rr (stage_env<true, ...>).
 static size_t lds_bytes(const EnvView &e, int lps = 0, int ops = 1)
 {
     if (lps != 7) return (size_t)(e.ns + 2 * e.nb) * sizeof(float4);
-    const int pad = kLinkPass * ops;
+    const int pad = link_pass(ops) * ops;
     return (size_t)(pad_to(e.ns, pad) + 2 * pad_to(e.nb, pad)) * sizeof(float4) + (size_t)e.ns * sizeof(float);
 }
 
