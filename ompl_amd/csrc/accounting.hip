@@ -117,7 +117,7 @@ int ompl_amd_motion_work(ompl_amd_env e, const double *a, const double *b, int64
         OA_REQUIRE(e && out && n >= 0, OMPL_AMD_ERR_ARG, "bad arguments");
         OA_REQUIRE(n == 0 || (a && b), OMPL_AMD_ERR_ARG, "null edges");
         OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "bad segment settings");
-        const EnvView env = e->env.view();
+        const EnvView env = e->env.input_view();
         run_work("motion", [&](Work *acc, hipStream_t s) {
             if (n > 0)
                 hipLaunchKernelGGL(motion_work_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, env, a, b, n,
@@ -131,7 +131,7 @@ int ompl_amd_validity_work(ompl_amd_env e, const float *q, int64_t n, int64_t ou
     return guarded([&] {
         OA_REQUIRE(e && out && n >= 0, OMPL_AMD_ERR_ARG, "bad arguments");
         OA_REQUIRE(n == 0 || q, OMPL_AMD_ERR_ARG, "null states");
-        const EnvView env = e->env.view();
+        const EnvView env = e->env.input_view();
         run_work("validity", [&](Work *acc, hipStream_t s) {
             if (n > 0)
                 hipLaunchKernelGGL(validity_work_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, env, q, n,

@@ -456,15 +456,29 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
         return ((low - 1) & live_leaders) == 0;
     };
     const float *rr = reinterpret_cast<const float *>(lds + nsp + 2 * nbp);
-    for (int ob = 0; ob < ns; ob += K * OPS) {
+    const float4 *boxes = lds + nsp;
+    // One pass: K obstacles of the lane's partition, bound tests first, then
+    // exact tests of the candidates, lowest first, until no lane has any.
+    // The bound tests read their obstacles KL at a time, all reads issued
+    // before the first test (the scheduler barrier keeps them together): a
+    // wave left alone on its SIMD, which the last states of a launch are,
+    // then waits for one LDS round trip per KL obstacles instead of one each.
+    constexpr int KL = 4, KB = 2;
+    auto sphere_pass = [&](int ob) {
         unsigned cand = 0;
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            // bound tests need only be conservative: fused multiply-adds
-            const float4 o = lds[sidx(ob + part + j * OPS)];
-            const float dx = o.x - S.cx, dy = o.y - S.cy, dz = o.z - S.cz;
-            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-            cand |= (unsigned)(d2 < o.w) << j;
+        for (int j0 = 0; j0 < K; j0 += KL) {
+            float4 o[KL];
+#pragma unroll
+            for (int j = 0; j < KL; ++j) o[j] = lds[sidx(ob + part + (j0 + j) * OPS)];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < KL; ++j) {
+                // bound tests need only be conservative: fused multiply-adds
+                const float dx = o[j].x - S.cx, dy = o[j].y - S.cy, dz = o[j].z - S.cz;
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                cand |= (unsigned)(d2 < o[j].w) << (j0 + j);
+            }
         }
         if (!act) cand = 0;
         while (wave_ballot(cand != 0)) {
@@ -476,20 +490,28 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
                 hit |= hits_sphere<7>(S, e);
             }
         }
-        if (done()) return hit;
-    }
-    const float4 *boxes = lds + nsp;
-    for (int ob = 0; ob < nb; ob += K * OPS) {
+    };
+    auto box_pass = [&](int ob) {
         unsigned cand = 0;
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const int o = bidx(ob + part + j * OPS);
-            const float4 lo = boxes[2 * o], hi = boxes[2 * o + 1];
-            const float dx = S.cx - clamp_box(S.cx, lo.x, hi.x);
-            const float dy = S.cy - clamp_box(S.cy, lo.y, hi.y);
-            const float dz = S.cz - clamp_box(S.cz, lo.z, hi.z);
-            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-            cand |= (unsigned)(d2 < kBoxBound * kBoxBound) << j;
+        for (int j0 = 0; j0 < K; j0 += KB) {
+            float3 lo[KB], hi[KB];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                const int o = bidx(ob + part + (j0 + j) * OPS);
+                const float4 l = boxes[2 * o], h = boxes[2 * o + 1];
+                lo[j] = make_float3(l.x, l.y, l.z);
+                hi[j] = make_float3(h.x, h.y, h.z);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                const float dx = S.cx - clamp_box(S.cx, lo[j].x, hi[j].x);
+                const float dy = S.cy - clamp_box(S.cy, lo[j].y, hi[j].y);
+                const float dz = S.cz - clamp_box(S.cz, lo[j].z, hi[j].z);
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                cand |= (unsigned)(d2 < kBoxBound * kBoxBound) << (j0 + j);
+            }
         }
         if (!act) cand = 0;
         while (wave_ballot(cand != 0)) {
@@ -499,7 +521,20 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
                 hit |= hits_box<7>(S, boxes[2 * o], boxes[2 * o + 1]);
             }
         }
-        if (done()) return hit;
+    };
+    // Box and sphere passes alternate, boxes first (a box of half-extent h
+    // holds the sphere of radius h), each list nearest to the base first (the
+    // environment's staging order), so that a colliding configuration usually
+    // meets its first hit in an early pass.
+    for (int ob = 0; ob < nb || ob < ns; ob += K * OPS) {
+        if (ob < nb) {
+            box_pass(ob);
+            if (done()) return hit;
+        }
+        if (ob < ns) {
+            sphere_pass(ob);
+            if (done()) return hit;
+        }
     }
     return hit;
 }

@@ -1,5 +1,7 @@
 // C-ABI of environments, state validity and motion validation.
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "runtime.hpp"
@@ -12,6 +14,25 @@ Env &env_of(ompl_amd_env e)
 {
     OA_REQUIRE(e, OMPL_AMD_ERR_ARG, "null environment");
     return e->env;
+}
+
+// Staging order of an obstacle list: increasing distance from the arm's base
+// (the world origin) to the obstacle's nearest point, ties in the caller's
+// order.  Every check result is "any obstacle hit", so the order leaves results
+// unchanged; links near the base sweep the least volume, so the obstacles
+// nearest to it are hit by the most configurations, and the checkers, which
+// leave a state's obstacle loop at the first pass with a hit, resolve most
+// invalid states in their first passes.
+std::vector<int> nearest_first(int n, const std::function<double(int)> &gap)
+{
+    std::vector<double> key(n);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) {
+        key[i] = gap(i);
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+    return order;
 }
 
 // Batched DiscreteMotionValidator::checkMotion(s1, s2) over device arrays.
@@ -164,16 +185,37 @@ int ompl_amd_env_create(const float *spheres, int n_spheres, const float *boxes,
             h[n_spheres + 2 * i] = make_float4(o[0] - o[3], o[1] - o[4], o[2] - o[5], 0.0f);
             h[n_spheres + 2 * i + 1] = make_float4(o[0] + o[3], o[1] + o[4], o[2] + o[5], 0.0f);
         }
+        // h (caller's order) goes after the staging-order copy in one allocation
+        const std::vector<int> so = nearest_first(n_spheres, [&](int i) {
+            const float *o = spheres + 4 * i;
+            return std::sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]) - (double)o[3];
+        });
+        const std::vector<int> bo = nearest_first(n_boxes, [&](int i) {
+            const float4 lo = h[n_spheres + 2 * i], hi = h[n_spheres + 2 * i + 1];
+            const double dx = std::max({(double)lo.x, -(double)hi.x, 0.0});
+            const double dy = std::max({(double)lo.y, -(double)hi.y, 0.0});
+            const double dz = std::max({(double)lo.z, -(double)hi.z, 0.0});
+            return std::sqrt(dx * dx + dy * dy + dz * dz);
+        });
+        std::vector<float4> both(2 * h.size());
+        for (int i = 0; i < n_spheres; ++i) both[i] = h[so[i]];
+        for (int i = 0; i < n_boxes; ++i) {
+            both[n_spheres + 2 * i] = h[n_spheres + 2 * bo[i]];
+            both[n_spheres + 2 * i + 1] = h[n_spheres + 2 * bo[i] + 1];
+        }
+        both[h.size() - 1] = h[h.size() - 1];
+        std::copy(h.begin(), h.end(), both.begin() + h.size());
         auto *e = new ompl_amd_env_s();
         e->env.ns = n_spheres;
         e->env.nb = n_boxes;
-        e->env.bytes = h.size() * sizeof(float4);
+        e->env.bytes = both.size() * sizeof(float4);
         hipError_t err = hipMalloc(&e->env.obs, e->env.bytes);
         if (err != hipSuccess) {
             delete e;
             fail(OMPL_AMD_ERR_NOMEM, "environment allocation failed: %s", hipGetErrorString(err));
         }
-        OA_HIP(hipMemcpy(e->env.obs, h.data(), e->env.bytes, hipMemcpyHostToDevice));
+        e->env.obs_input = e->env.obs + h.size();
+        OA_HIP(hipMemcpy(e->env.obs, both.data(), e->env.bytes, hipMemcpyHostToDevice));
         const size_t lb = links_bytes(e->env.view()) + sizeof(float4);
         err = hipMalloc(&e->env.obs_links, lb);
         if (err != hipSuccess) {

@@ -64,7 +64,12 @@ Tuning tuning_from_c(const ompl_amd_tuning &t);
 
 // Device-resident obstacle set.
 struct Env {
-    float4 *obs = nullptr; // ns spheres (x,y,z,(R+r)^2), then nb boxes as (lo),(hi)
+    // ns spheres (x,y,z,(R+r)^2), then nb boxes as (lo),(hi); each list in
+    // staging order (nearest to the arm's base first), for the checkers
+    float4 *obs = nullptr;
+    // the same words in the caller's order (same allocation), for the
+    // reference-order work accounting only
+    float4 *obs_input = nullptr;
     // the same obstacles in the link layout's staging (spheres (x,y,z,bound^2),
     // boxes, then the spheres' rr as floats), for kernels that read them from HBM
     float4 *obs_links = nullptr;
@@ -72,6 +77,7 @@ struct Env {
     int nb = 0;
     size_t bytes = 0;
     EnvView view() const { return EnvView{obs, ns, nb}; }
+    EnvView input_view() const { return EnvView{obs_input, ns, nb}; }
 };
 
 // Largest obstacle set staged in LDS by one block (float4 words).
