@@ -166,6 +166,70 @@ def test_env_rejects_non_finite_obstacles(gpu):
     gpu.Env(good_s, good_b).close()
 
 
+@pytest.mark.parametrize("eshape,wshape", [((7, 4), (7, 9)), ((8, 2), (8, 1)), ((7, 1), (7, 2))])
+def test_results_independent_of_obstacle_order(gpu, oracle, eshape, wshape):
+    """The environment keeps its own staging order of the obstacles; validity and
+    motion results must not depend on the order the caller lists them in."""
+    s, b = gpu.workload.obstacles(1001, 128, 128)
+    rng = np.random.default_rng(5)
+    envs = [gpu.Env(s, b), gpu.Env(s[::-1].copy(), b[::-1].copy()),
+            gpu.Env(s[rng.permutation(len(s))], b[rng.permutation(len(b))])]
+    q = gpu.workload.configs(12, 4096).astype(np.float32)
+    a, bb = motion_case(gpu, 13, 2048)
+    A, B = to_dev(a.T.copy()), to_dev(bb.T.copy())
+    set_strategy(gpu, 3, eshape, wshape)
+    try:
+        got = [(gpu.validate_states(e, to_dev(q.T.copy())).cpu().numpy(),
+                gpu.check_motions(e, A, B, lvs_config()).cpu().numpy()) for e in envs]
+    finally:
+        gpu.set_tuning(**DEFAULT)
+        for e in envs:
+            e.close()
+    oenv = oracle.OracleEnv(s, b)
+    assert np.array_equal(got[0][0], oracle.validate_states(oenv, q.T.copy()))
+    for v, m in got[1:]:
+        assert np.array_equal(v, got[0][0])
+        assert np.array_equal(m, got[0][1])
+
+
+def reference_order_pairs(oracle, s, b, q):
+    """Pairs a scalar checker visits for each state in the caller's obstacle
+    order: robot spheres in order, each against the sphere obstacles then the
+    box obstacles, stopping at the first hit (float32, the checker's formulas)."""
+    f = np.float32
+    rr = ((s[:, 3] + f(0.05)) * (s[:, 3] + f(0.05))).astype(f)
+    lo, hi = (b[:, :3] - b[:, 3:]).astype(f), (b[:, :3] + b[:, 3:]).astype(f)
+    sp = bp = 0
+    for i in range(q.shape[0]):
+        c = oracle.arm_spheres(q[i]).astype(f)  # 28 x 3
+        d = s[None, :, :3] - c[:, None, :]
+        hs = ((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]) < rr[None]
+        e = c[:, None, :] - np.minimum(np.maximum(c[:, None, :], lo[None]), hi[None])
+        hb = ((e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1]) + e[..., 2] * e[..., 2]) < f(0.05) * f(0.05)
+        seq = np.concatenate([hs, hb], 1).reshape(-1)  # robot-sphere major
+        kind = np.tile(np.r_[np.zeros(len(s), bool), np.ones(len(b), bool)], c.shape[0])
+        n = int(np.argmax(seq)) + 1 if seq.any() else seq.size
+        bp += int(kind[:n].sum())
+        sp += n - int(kind[:n].sum())
+    return sp, bp
+
+
+def test_work_accounting_in_caller_order(gpu, oracle):
+    """The roofline's work count follows the caller's obstacle order, not the
+    environment's staging order: listing the obstacles differently changes the
+    count exactly as a scalar checker's would change."""
+    s, b = gpu.workload.obstacles(1001, 128, 128)
+    q = gpu.workload.configs(14, 300).astype(np.float32)
+    for ss, bb in ((s, b), (s[::-1].copy(), b[::-1].copy())):
+        env = gpu.Env(ss, bb)
+        try:
+            w = gpu.validity_work(env, to_dev(q.T.copy()))
+        finally:
+            env.close()
+        sp, bp = reference_order_pairs(oracle, ss, bb, q)
+        assert (w["states"], w["sphere_pairs"], w["box_pairs"]) == (q.shape[0], sp, bp)
+
+
 def test_is_valid_single_call(gpu, oracle):
     env, oenv, _ = env_pair(gpu, oracle, 1000, 64, 0)
     q = gpu.workload.configs(5, 200)
