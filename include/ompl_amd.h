@@ -77,7 +77,10 @@ int ompl_amd_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
 typedef struct ompl_amd_env_s *ompl_amd_env;
 
 /* spheres: n_spheres x (cx, cy, cz, radius); boxes: n_boxes x (cx, cy, cz,
- * hx, hy, hz) axis-aligned, half-extents.  Host pointers, copied to HBM. */
+ * hx, hy, hz) axis-aligned, half-extents.  Host pointers, copied to HBM.
+ * Values must be finite, radii and half-extents >= 0.  The checkers visit
+ * each list nearest to the arm's base first; results do not depend on the
+ * order given here. */
 int ompl_amd_env_create(const float *spheres, int n_spheres, const float *boxes, int n_boxes,
                         ompl_amd_env *out);
 int ompl_amd_env_destroy(ompl_amd_env env);
