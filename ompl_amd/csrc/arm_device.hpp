@@ -539,13 +539,151 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
     return hit;
 }
 
+// Link layout (7,4) under the all-groups exit: two states per wave, lanes
+// 0-31 and 32-63.  Once one state is resolved (a hit found, or not live) and
+// the other is not, the resolved state's lanes take the other state's link
+// spheres (from the lane 32 apart, which holds the same link and partition)
+// and check every other pass of it from then on (its own lanes at passes
+// ob, ob + 2P, ..., the helpers at ob + P, ob + 3P, ...), so a state that
+// scans the whole list -- a valid one does -- is not left with half of its
+// wave idle.  Every obstacle is still tested for each state until that state
+// is resolved; returns the lane's state's result.
+__device__ __forceinline__ bool collide_links_pair(const LaneSpheres<7> &S0, const float4 *lds, int ns, int nb,
+                                                   bool live, int part)
+{
+    constexpr int OPS = 4, K = link_pass(OPS), P = K * OPS;
+    constexpr int KL = 4, KB = 2; // LDS reads in flight per bound-test group (spheres, boxes)
+    const int nsp = pad_to(ns, P), nbp = pad_to(nb, P);
+    const int g = (threadIdx.x & 63) >> 5;
+    const bool has_link = part < OPS;
+    part = has_link ? part : 0;
+    const uint64_t lm = wave_ballot(live);
+    bool res0 = (lm & 0xffffffffull) == 0, res1 = (lm >> 32) == 0; // resolved: hit, or not live
+    bool hit0 = false, hit1 = false;
+    if (res0 && res1) return false;
+    LaneSpheres<7> S = S0;
+    bool helper = false, coop = false, act = live && has_link, hit = false;
+    uint64_t w1 = 0xffffffff00000000ull; // lanes whose working state is state 1
+    const float *rr = reinterpret_cast<const float *>(lds + nsp + 2 * nbp);
+    const float4 *boxes = lds + nsp;
+    auto join = [&]() {
+        const int u = res0 ? 1 : 0; // the unresolved state
+        helper = g != u;
+#pragma unroll
+        for (int m = 0; m < LaneSpheres<7>::N; ++m) {
+            const float x = __shfl_xor(S.x[m], 32), y = __shfl_xor(S.y[m], 32), z = __shfl_xor(S.z[m], 32);
+            if (helper) {
+                S.x[m] = x;
+                S.y[m] = y;
+                S.z[m] = z;
+            }
+        }
+        const float cx = __shfl_xor(S.cx, 32), cy = __shfl_xor(S.cy, 32), cz = __shfl_xor(S.cz, 32);
+        if (helper) {
+            S.cx = cx;
+            S.cy = cy;
+            S.cz = cz;
+        }
+        act = has_link;
+        hit = false;
+        w1 = u == 1 ? ~0ull : 0ull;
+        coop = true;
+    };
+    auto update = [&]() -> bool {
+        const uint64_t m = wave_ballot(hit);
+        if (m & ~w1) hit0 = res0 = true;
+        if (m & w1) hit1 = res1 = true;
+        return res0 && res1;
+    };
+    // one pass of the lane's partition starting at list index `base`
+    auto sphere_pass = [&](int base, bool valid) {
+        unsigned cand = 0;
+#pragma unroll
+        for (int j0 = 0; j0 < K; j0 += KL) {
+            float4 o[KL];
+#pragma unroll
+            for (int j = 0; j < KL; ++j) o[j] = lds[base + part + (j0 + j) * OPS];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < KL; ++j) {
+                const float dx = o[j].x - S.cx, dy = o[j].y - S.cy, dz = o[j].z - S.cz;
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                cand |= (unsigned)(d2 < o[j].w) << (j0 + j);
+            }
+        }
+        if (!(act && valid)) cand = 0;
+        while (wave_ballot(cand != 0)) {
+            if (cand) {
+                const int o = base + part + __builtin_ctz(cand) * OPS;
+                cand &= cand - 1;
+                float4 e = lds[o];
+                e.w = rr[o];
+                hit |= hits_sphere<7>(S, e);
+            }
+        }
+    };
+    auto box_pass = [&](int base, bool valid) {
+        unsigned cand = 0;
+#pragma unroll
+        for (int j0 = 0; j0 < K; j0 += KB) {
+            float3 lo[KB], hi[KB];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                const int o = base + part + (j0 + j) * OPS;
+                const float4 l = boxes[2 * o], h = boxes[2 * o + 1];
+                lo[j] = make_float3(l.x, l.y, l.z);
+                hi[j] = make_float3(h.x, h.y, h.z);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                const float dx = S.cx - clamp_box(S.cx, lo[j].x, hi[j].x);
+                const float dy = S.cy - clamp_box(S.cy, lo[j].y, hi[j].y);
+                const float dz = S.cz - clamp_box(S.cz, lo[j].z, hi[j].z);
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                cand |= (unsigned)(d2 < kBoxBound * kBoxBound) << (j0 + j);
+            }
+        }
+        if (!(act && valid)) cand = 0;
+        while (wave_ballot(cand != 0)) {
+            if (cand) {
+                const int o = base + part + __builtin_ctz(cand) * OPS;
+                cand &= cand - 1;
+                hit |= hits_box<7>(S, boxes[2 * o], boxes[2 * o + 1]);
+            }
+        }
+    };
+    if (res0 != res1) join();
+    // box and sphere passes alternate, boxes first, as in collide_links
+    for (int ob = 0; ob < nb || ob < ns;) {
+        const int step = coop ? 2 * P : P;
+        const int mine = ob + (helper ? P : 0); // helpers take the next pass
+        if (ob < nb) {
+            const bool v = mine < nbp; // else no pass left for this lane: read in range, test nothing
+            box_pass(v ? mine : ob, v);
+            if (update()) break;
+        }
+        if (ob < ns) {
+            const bool v = mine < nsp;
+            sphere_pass(v ? mine : ob, v);
+            if (update()) break;
+        }
+        ob += step;
+        if (!coop && res0 != res1) join();
+    }
+    return g == 0 ? hit0 : hit1;
+}
+
 // PADDED (link layout only): the obstacles are in the padded LDS staging of
 // stage_env<true, link_pass(OPS) * OPS>; false for the unpadded copy in HBM.
 template <int LPS, int OPS, Exit E, bool PADDED = true>
 __device__ __forceinline__ bool collide_env(const LaneSpheres<LPS> &S, const float4 *lds, int ns, int nb, bool live,
                                             int part)
 {
-    if constexpr (LPS == 7) return collide_links<OPS, E, PADDED>(S, lds, ns, nb, live, part);
+    if constexpr (LPS == 7 && OPS == 4 && E == Exit::kAllGroups && PADDED)
+        return collide_links_pair(S, lds, ns, nb, live, part);
+    else if constexpr (LPS == 7)
+        return collide_links<OPS, E, PADDED>(S, lds, ns, nb, live, part);
     constexpr int GS = LPS * OPS;
     bool hit = false;
     const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));
