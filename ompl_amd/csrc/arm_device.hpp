@@ -112,23 +112,35 @@ __device__ __forceinline__ void stage_env(const EnvView &e, float4 *lds)
         const int nw = env_words(e);
         for (int i = threadIdx.x; i < nw; i += blockDim.x) lds[i] = e.obs[i];
     } else {
-        const int nsp = pad_to(e.ns, PAD), nbp = pad_to(e.nb, PAD);
+        const int nsp = pad_to(e.ns, PAD), nbp = pad_to(e.nb, PAD), total = nsp + 2 * nbp;
         float *rr = reinterpret_cast<float *>(lds + nsp + 2 * nbp);
-        for (int i = threadIdx.x; i < nsp + 2 * nbp; i += blockDim.x) {
-            float4 o;
+        // source word of staging word i in e.obs, or -1 for padding
+        auto src = [&](int i) -> int {
+            if (i < nsp) return i < e.ns ? i : -1;
+            return i - nsp < 2 * e.nb ? e.ns + (i - nsp) : -1;
+        };
+        auto put = [&](int i, int w, float4 o) {
             if (i < nsp) {
-                if (i < e.ns) {
-                    o = e.obs[i];
+                if (w >= 0) {
                     rr[i] = o.w;
                     o.w = sphere_bound2(o.w);
                 } else {
                     o = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
                 }
-            } else {
-                const int b = i - nsp; // box word
-                o = b < 2 * e.nb ? e.obs[e.ns + b] : make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+            } else if (w < 0) {
+                o = make_float4(1e30f, 1e30f, 1e30f, 0.0f);
             }
             lds[i] = o;
+        };
+        // two words per thread and step, both read before either is written
+        // (the reads are unconditional -- word 0 stands in for padding; the
+        // array always holds at least one word -- so they share one round trip)
+        for (int i0 = threadIdx.x; i0 < total; i0 += 2 * blockDim.x) {
+            const int i1 = i0 + blockDim.x;
+            const int w0 = src(i0), w1 = i1 < total ? src(i1) : -1;
+            const float4 o0 = e.obs[w0 < 0 ? 0 : w0], o1 = e.obs[w1 < 0 ? 0 : w1];
+            put(i0, w0, o0);
+            if (i1 < total) put(i1, w1, o1);
         }
     }
     __syncthreads();

@@ -52,12 +52,27 @@ bool shape_supported(int lps, int ops)
     return false;
 }
 
+// Configuration i of a dof-major array (n entries per joint) as FP32, zeros
+// for a dead lane, which reads entry `fallback` (a valid index) and drops it:
+// unconditional reads are issued together and wait for one memory round trip,
+// where reads under the lane's condition were issued and awaited a few at a time.
+template <typename T>
+__device__ __forceinline__ void load_config(const T *soa, int64_t n, int64_t i, bool live, float q[kDof],
+                                            int64_t fallback = 0)
+{
+    const int64_t ic = live ? i : fallback;
+    T v[kDof];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) v[j] = soa[(int64_t)j * n + ic];
+#pragma unroll
+    for (int j = 0; j < kDof; ++j) q[j] = live ? (float)v[j] : 0.0f;
+}
+
 template <int LPS, int OPS, typename T>
 __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__restrict__ q_soa, int64_t n,
                                                         uint8_t *__restrict__ out)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, link_pass(OPS) * OPS>(env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int lane = lane_id();
     const Shape sh(lane);
@@ -65,8 +80,8 @@ __global__ void __launch_bounds__(1024) validate_kernel(EnvView env, const T *__
     const int64_t st = wave * Shape::STATES + sh.group;
     const bool live = st < n;
     float q[kDof];
-#pragma unroll
-    for (int j = 0; j < kDof; ++j) q[j] = live ? (float)q_soa[(int64_t)j * n + st] : 0.0f;
+    load_config(q_soa, n, st, live, q); // before the staging: the two reads' latencies overlap
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(env, lds);
     LaneSpheres<LPS> S;
     arm_fk<LPS, Shape::GS>(q, sh.slot, S);
     const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, env.ns, env.nb, live, sh.part);
@@ -92,8 +107,7 @@ __global__ void __launch_bounds__(1024) validate_shared_kernel(EnvView env, cons
     const int64_t s0 = ((xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6) * FK;
     const int64_t sf = s0 + lane / LPS;
     float q[kDof];
-#pragma unroll
-    for (int j = 0; j < kDof; ++j) q[j] = sf < n ? (float)q_soa[(int64_t)j * n + sf] : 0.0f;
+    load_config(q_soa, n, sf, sf < n, q);
     LaneSpheres<LPS> F;
     arm_fk<LPS>(q, lane % LPS, F);
 #pragma unroll 1
@@ -144,17 +158,21 @@ __device__ __forceinline__ bool endpoint_finish(const MotionArgs &A, int64_t e, 
     return true;
 }
 
-// s2 check of one edge per lane group; returns true for the group leader of a
-// survivor and writes the final result of every other edge.
+// s2 (FP32) of edge e, or zeros for a dead lane.
+__device__ __forceinline__ void load_s2(const MotionArgs &A, int64_t e, bool live, float q[kDof])
+{
+    load_config(A.b, A.n, e, live, q);
+}
+
+// s2 check of one edge per lane group (q: its s2 from load_s2); returns true
+// for the group leader of a survivor and writes the final result of every
+// other edge.
 template <int LPS, int OPS>
-__device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4 *lds, int64_t e, bool live, int &m,
-                                               double a[kDof], double b[kDof])
+__device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4 *lds, int64_t e, bool live,
+                                               const float q[kDof], int &m, double a[kDof], double b[kDof])
 {
     using Shape = GroupShape<LPS, OPS>;
     const Shape sh(lane_id());
-    float q[kDof];
-#pragma unroll
-    for (int j = 0; j < kDof; ++j) q[j] = live ? (float)A.b[(int64_t)j * A.n + e] : 0.0f;
     LaneSpheres<LPS> S;
     arm_fk<LPS, Shape::GS>(q, sh.slot, S);
     const bool hit = collide_env<LPS, OPS, Exit::kAllGroups>(S, lds, A.env.ns, A.env.nb, live, sh.part);
@@ -195,13 +213,16 @@ template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_endpoint_kernel(MotionArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
     using Shape = GroupShape<LPS, OPS>;
     const int64_t wave = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t e = wave * Shape::STATES + lane_id() / Shape::GS;
+    // s2 is read before the obstacles are staged, so the two reads' latencies overlap
+    float q[kDof];
+    load_s2(A, e, e < A.n, q);
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
     int m = 0;
     double a[kDof], b[kDof];
-    const bool survivor = endpoint_check<LPS, OPS>(A, lds, e, e < A.n, m, a, b);
+    const bool survivor = endpoint_check<LPS, OPS>(A, lds, e, e < A.n, q, m, a, b);
     append_survivor(A, e, survivor, m, a, b);
 }
 
@@ -219,8 +240,7 @@ __global__ void __launch_bounds__(1024) motion_endpoint_shared_kernel(MotionArgs
     const int64_t e0 = ((xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6) * FK;
     const int64_t ef = e0 + lane / LPS;
     float q[kDof];
-#pragma unroll
-    for (int j = 0; j < kDof; ++j) q[j] = ef < A.n ? (float)A.b[(int64_t)j * A.n + ef] : 0.0f;
+    load_s2(A, ef, ef < A.n, q);
     LaneSpheres<LPS> F;
     arm_fk<LPS>(q, lane % LPS, F);
 #pragma unroll 1
@@ -337,7 +357,7 @@ __device__ __forceinline__ void scan_remaining_chunks(const int32_t *__restrict_
 template <int LPS, int OPS, bool EXACT, bool SHARED>
 __device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds, int sj, int bj, int ndj)
 {
-    if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) return;
+    // the endpoints are read together with first_bad (one round trip, not two)
     const double *ab = A.surv_ab + (int64_t)sj * 2 * kDof;
     double a[kDof], b[kDof];
 #pragma unroll
@@ -345,6 +365,7 @@ __device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds
         a[d] = ab[d];
         b[d] = ab[kDof + d];
     }
+    if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) return;
     const int bad = SHARED ? walk_chunk_shared<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb)
                            : walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
     if (bad != INT_MAX && lane_id() == 0) {
@@ -390,14 +411,19 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
         A.reset_next[0] = 0;
         A.reset_next[1] = 0;
     }
+    // both counters are read before the staging, so the reads' latencies
+    // overlap; the empty asm keeps the values in vector registers until then
+    // (their move to scalar registers would wait for the reads there)
+    int n_surv_v = *A.list_count, max_m = *A.max_m;
     stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
+    asm volatile("" : "+v"(n_surv_v), "+v"(max_m));
+    const int64_t n_surv = n_surv_v;
     constexpr int G = SHARED ? 64 / LPS : GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t n_surv = *A.list_count;
     if (n_surv == 0) return;
-    const int64_t nblk = ((int64_t)*A.max_m + G - 1) / G;
+    const int64_t nblk = ((int64_t)max_m + G - 1) / G;
     const bool flat = A.flat && n_surv <= kFlatSurvivors && n_surv * nblk < (int64_t)INT_MAX;
     const int64_t L0 = flat ? max((int64_t)1, min(nblk, waves_total / n_surv)) : nblk;
     if (PHASE == 1) {
@@ -474,7 +500,9 @@ __global__ void __launch_bounds__(1024) motion_fused_kernel(MotionArgs A)
     const int64_t e = wave * Shape::STATES + lane / Shape::GS;
     int m = 0;
     double a_unused[kDof], b_unused[kDof];
-    const bool survivor = endpoint_check<LE, OE>(A, lds, e, e < A.n, m, a_unused, b_unused);
+    float q[kDof];
+    load_s2(A, e, e < A.n, q);
+    const bool survivor = endpoint_check<LE, OE>(A, lds, e, e < A.n, q, m, a_unused, b_unused);
     uint64_t sm = wave_ballot(survivor);
     while (sm) {
         const int src = lowest_bit(sm);
@@ -552,8 +580,7 @@ __global__ void __launch_bounds__(256) state_lists_kernel(EnvView env, const dou
         const int64_t st = base + sh.group;
         const bool live = st < hi;
         float q[kDof];
-#pragma unroll
-        for (int j = 0; j < kDof; ++j) q[j] = live ? (float)q_soa[(int64_t)j * n_states + st] : 0.0f;
+        load_config(q_soa, n_states, st, live, q, base);
         LaneSpheres<LPS> S;
         arm_fk<LPS, Shape::GS>(q, sh.slot, S);
         const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, env.ns, env.nb, live, sh.part);
