@@ -353,11 +353,15 @@ __device__ __forceinline__ void scan_remaining_chunks(const int32_t *__restrict_
 }
 
 // One work item of the chunk walk: positions [bj, bj + G) of survivor sj,
-// skipped when an earlier position of the edge is already known invalid.
+// skipped when bj is past the edge's last position or an earlier position of
+// the edge is already known invalid.
 template <int LPS, int OPS, bool EXACT, bool SHARED>
-__device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds, int sj, int bj, int ndj)
+__device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds, int sj, int bj)
 {
-    // the endpoints are read together with first_bad (one round trip, not two)
+    int ndj;
+    // the endpoints, the position count and first_bad are read together (one
+    // round trip); the item is dropped past the edge's last position or behind
+    // a known invalid one
     const double *ab = A.surv_ab + (int64_t)sj * 2 * kDof;
     double a[kDof], b[kDof];
 #pragma unroll
@@ -365,7 +369,9 @@ __device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds
         a[d] = ab[d];
         b[d] = ab[kDof + d];
     }
-    if (__hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) return;
+    ndj = A.surv_nd[sj];
+    const int fb = __hip_atomic_load(&A.first_bad[sj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bj >= ndj || fb < bj) return;
     const int bad = SHARED ? walk_chunk_shared<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb)
                            : walk_chunk<LPS, OPS, EXACT>(a, b, ndj, ndj, bj, lds, A.env.ns, A.env.nb);
     if (bad != INT_MAX && lane_id() == 0) {
@@ -376,13 +382,13 @@ __device__ __forceinline__ void walk_item(const MotionArgs &A, const float4 *lds
 
 // Walks the live items of a 64-item window (one per lane) in lane order.
 template <int LPS, int OPS, bool EXACT, bool SHARED>
-__device__ __forceinline__ void walk_window(const MotionArgs &A, const float4 *lds, int s, int base, int nd, bool live)
+__device__ __forceinline__ void walk_window(const MotionArgs &A, const float4 *lds, int s, int base, bool live)
 {
     uint64_t mask = wave_ballot(live);
     while (mask) {
         const int j = lowest_bit(mask);
         mask &= mask - 1;
-        walk_item<LPS, OPS, EXACT, SHARED>(A, lds, readlane_i32(s, j), readlane_i32(base, j), readlane_i32(nd, j));
+        walk_item<LPS, OPS, EXACT, SHARED>(A, lds, readlane_i32(s, j), readlane_i32(base, j));
     }
 }
 
@@ -430,16 +436,18 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
         const int64_t items1 = n_surv * L0;
         for (int64_t k0 = wave; k0 < items1; k0 += 64 * waves_total) {
             const int64_t k = k0 + lane * waves_total;
-            int s = 0, base = 0, nd = 0;
+            int s = 0, base = 0;
             bool live = false;
             if (k < items1) {
                 const int64_t blk = k / n_surv;
                 s = (int)(k - blk * n_surv);
                 base = 1 + (int)(blk * G);
-                nd = A.surv_nd[s];
-                live = base < nd;
+                // flat: a wave holds at most one item (L0 <= waves / n_surv), or
+                // L0 = 1 and none is dead (nd >= 2): walk_item reads the count
+                // with its other reads; else drop the many dead items here
+                live = flat || base < A.surv_nd[s];
             }
-            walk_window<LPS, OPS, EXACT, SHARED>(A, lds, s, base, nd, live);
+            walk_window<LPS, OPS, EXACT, SHARED>(A, lds, s, base, live);
         }
         return;
     }
@@ -449,7 +457,7 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
         const int64_t R = off[n_surv];
         for (int64_t r0 = wave; r0 < R; r0 += 64 * waves_total) {
             const int64_t r = r0 + lane * waves_total;
-            int s = 0, base = 0, nd = 0;
+            int s = 0, base = 0;
             const bool live = r < R;
             if (live) {
                 int lo = 0, hi = (int)n_surv; // off[lo] <= r < off[hi]
@@ -460,9 +468,8 @@ __global__ void __launch_bounds__(1024) motion_chunk_kernel(MotionArgs A)
                 }
                 s = lo;
                 base = 1 + (int)((L0 + (r - off[lo])) * G);
-                nd = A.surv_nd[s];
             }
-            walk_window<LPS, OPS, EXACT, SHARED>(A, lds, s, base, nd, live);
+            walk_window<LPS, OPS, EXACT, SHARED>(A, lds, s, base, live);
         }
     }
 }
