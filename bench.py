@@ -53,7 +53,7 @@ def percentile(xs, p):
 
 
 def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000, log_path=None,
-                  check_parts=0, profile_ticks=False, run_plain=True, check_lanes=0):
+                  check_parts=0, profile_slices=False, run_plain=True, block_threads=0, slice_us=0):
     """Poisson stream of RRT-Connect requests over seeded environments (16-1024
     obstacles, log-uniform, half spheres / half boxes), served (a) by the
     continuous-batching server and (b) by the plain path (host RRT-Connect, one
@@ -92,12 +92,12 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         return {"p50": float(np.percentile(xs, 50)), "p99": float(np.percentile(xs, 99)),
                 "mean": float(np.mean(xs))}
 
-    # (a) served (optionally the library's HIP events time each tick's kernels)
-    if profile_ticks:
+    # (a) served (optionally the library's HIP events time each slice launch)
+    if profile_slices:
         oa.profile.reset()
         oa.profile.enable(True)
     srv = serve.Server(hbm_budget_bytes=4 << 30, max_active=1024, max_nodes=4096, check_obstacle_parts=check_parts,
-                       check_lanes_per_state=check_lanes)
+                       block_threads=block_threads, slice_us=slice_us)
     ids = [None] * n_requests
     t0 = time.perf_counter()
     for i, r in enumerate(reqs):
@@ -110,22 +110,26 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
     stats = srv.stats()
     lat_s = [r.latency_ms for r in res_s]
     paths_s = [srv.path(rid) for rid in ids]
+    checks_s = sum(r.motion_checks for r in res_s)
+    phases = srv.phase_us()
     srv.close()
-    n_ticks, tick_ms, tick_parts = 0, 0.0, None
-    if profile_ticks:
-        n_ticks, tick_ms = oa.profile.query("serve_tick")
-        tick_parts = {}
-        for part in ("serve_prepare", "serve_check", "serve_commit"):
-            n_p, ms_p = oa.profile.query(part)
-            tick_parts[part] = 1e3 * ms_p / n_p if n_p else None
+    slice_us_gpu = None
+    if profile_slices:
+        n_sl, ms_sl = oa.profile.query("serve_slice")
+        slice_us_gpu = 1e3 * ms_sl / n_sl if n_sl else None
         oa.profile.enable(False)
         oa.profile.reset()
-    served = {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "ticks": stats["ticks"],
-              "grow_steps": stats["steps"], "wall_us_per_tick": 1e6 * span_s / max(stats["ticks"], 1)}
-    if profile_ticks:
-        served.update(gpu_us_per_tick=1e3 * tick_ms / n_ticks if n_ticks else None, tick_kernels_us=tick_parts)
+    served = {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "slices": stats["slices"],
+              "grow_steps": stats["steps"], "motion_checks": checks_s,
+              "edge_checks_per_s": checks_s / span_s,
+              "run_ms": pct([r.run_ms for r in res_s]), "queued_ms": pct([r.queued_ms for r in res_s]),
+              "device_us_per_step": {k: v / max(stats["steps"], 1) for k, v in phases.items()},
+              "config": {"check_obstacle_parts": check_parts, "block_threads": block_threads or 512,
+                         "slice_us": slice_us or 200}}
+    if profile_slices:
+        served["gpu_us_per_slice"] = slice_us_gpu
     if not run_plain:
-        return {"requests": n_requests, "qps": qps, "check_shape": [check_lanes, check_parts], "served": served}
+        return {"requests": n_requests, "qps": qps, "served": served}
 
     # (b) plain: FIFO queue drained by plain_workers threads
     lat_p = [0.0] * n_requests
@@ -150,7 +154,8 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
             f.result()
     span_p = time.perf_counter() - t0
     same = all(res_p[i][0].status == res_s[i].status and np.array_equal(res_p[i][1], paths_s[i])
-               for i in range(n_requests))
+               and res_p[i][0].motion_checks == res_s[i].motion_checks for i in range(n_requests))
+    checks_p = sum(r.motion_checks for r, _ in res_p)
     solved = sum(r.status == serve.SOLVED for r in res_s)
     if log_path:
         from ompl_amd import benchlog
@@ -166,8 +171,11 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         "requests": n_requests, "qps": qps, "envs_used": len(envs), "envs_drawn": n_envs,
         "solved": solved, "max_iters": max_iters,
         "served": served,
-        "plain": {"latency_ms": pct(lat_p), "throughput_rps": n_requests / span_p, "workers": plain_workers},
+        "plain": {"latency_ms": pct(lat_p), "throughput_rps": n_requests / span_p, "workers": plain_workers,
+                  "motion_checks": checks_p, "edge_checks_per_s": checks_p / span_p},
         "p99_plain_over_served": float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)),
+        "p50_plain_over_served": float(np.percentile(lat_p, 50) / np.percentile(lat_s, 50)),
+        "edge_checks_served_over_plain": float((checks_s / span_s) / (checks_p / span_p)),
         "identical_plans": bool(same),
     }
 
@@ -667,14 +675,18 @@ def main(argv=None):
     stream_res = None
     if args.stream_requests > 0:
         stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers,
-                                   log_path=args.benchmark_log)
-        # tick breakdown (HIP events around each tick kernel; slower than the run above) per check shape
-        # (lanes per state, obstacle parts): sphere slots (8, x), link layout (7, x)
-        stream_res["tick_profile"] = {
-            f"{lanes}x{parts}": served_stream(oa, min(64, args.stream_requests), args.stream_qps, args.stream_envs, 0,
-                                              check_parts=parts, profile_ticks=True, run_plain=False,
-                                              check_lanes=lanes)["served"]
-            for lanes, parts in ((8, 4), (8, 8), (7, 2), (7, 4), (7, 9))}
+                                   log_path=args.benchmark_log, run_plain=args.plain_workers > 0)
+        # the same stream served alone at higher offered rates (saturation sweep)
+        stream_res["served_qps_sweep"] = {
+            str(int(q)): served_stream(oa, args.stream_requests, q, args.stream_envs, 0, run_plain=False)["served"]
+            for q in (1000.0, 5000.0, 20000.0)}
+        # server settings: obstacle partitions of the motion check (0 = by motion length) and workgroup size;
+        # HIP events around each slice launch
+        stream_res["served_config_sweep"] = {
+            f"parts{parts}_t{threads}": served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, 0,
+                                                      check_parts=parts, block_threads=threads, profile_slices=True,
+                                                      run_plain=False)["served"]
+            for parts, threads in ((0, 512), (1, 512), (2, 512), (4, 512), (9, 512), (0, 1024))}
 
     prm_res = None
     if args.prm_samples > 0:

@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 305
+#define OMPL_AMD_VERSION 306
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -320,12 +320,14 @@ double ompl_amd_rrtstar_radius(int dim, double measure, double rewire_factor, do
 /* ------------------------------------------------------------------ */
 /* Planning server (continuous batching) -- RRT-Connect requests        */
 /* (geometric/planners/rrt/src/RRTConnect.cpp:116-196, 245-330) on the  */
-/* 7-DOF arm with device-resident trees.  Every tick advances each      */
-/* active request by one growTree step in a single launch; requests are */
-/* admitted between ticks while their tree memory fits the HBM budget.  */
-/* A request's result does not depend on the other requests it shares   */
-/* ticks with.  No counterpart in the reference (it plans one query per */
-/* thread).                                                             */
+/* 7-DOF arm.  Each request's whole solve loop (sampler, trees, connect */
+/* loop, path) is resident in a region of a device pool of              */
+/* hbm_budget_bytes; the server runs every active request for a time    */
+/* slice per launch (one workgroup per request) and admits queued       */
+/* requests between slices while their region fits the pool.  A        */
+/* request's result does not depend on the slices it ran in or the      */
+/* requests it shared them with.  No counterpart in the reference (it   */
+/* plans one query per thread).                                         */
 /* ------------------------------------------------------------------ */
 #define OMPL_AMD_PLAN_QUEUED 0
 #define OMPL_AMD_PLAN_RUNNING 1
@@ -339,15 +341,17 @@ double ompl_amd_rrtstar_radius(int dim, double measure, double rewire_factor, do
 typedef struct ompl_amd_server_s *ompl_amd_server;
 
 typedef struct {
-    size_t hbm_budget_bytes; /* admission limit on the sum of active requests' tree memory */
-    int max_active;          /* concurrent requests (tree slots preallocated) */
-    int max_nodes;           /* nodes per tree per request */
-    int check_obstacle_parts; /* lanes splitting each environment's obstacle list in the
-                                 tick's motion check: 2, 4 or 8 with sphere slots, 1, 2, 4
-                                 or 9 with the link layout (0 = 4 for the link layout, 8
-                                 for sphere slots) */
-    int check_lanes_per_state; /* 8: sphere slots; 7: link layout with bounding-sphere
-                                  culling (see ompl_amd_tuning); 0 = 7 */
+    size_t hbm_budget_bytes; /* device pool holding the active requests' regions (allocated at
+                                creation); a request is admitted when its region fits */
+    int max_active;          /* concurrent requests */
+    int max_nodes;           /* largest nodes per tree of a request */
+    int check_obstacle_parts; /* lanes splitting the environment's obstacle list in a motion
+                                 check (link layout): 1, 2, 4 or 9 (8, 4, 2 or 1 positions
+                                 per wave); 0 = chosen per motion from its length */
+    int check_lanes_per_state; /* 0 or 7 (the link layout with bounding-sphere culling) */
+    int slice_us;            /* device time per slice before the host admits and retires
+                                (at least one growTree step per request); 0 = 200 */
+    int block_threads;       /* threads of each request's workgroup: 512 or 1024 (0 = 512) */
 } ompl_amd_server_config;
 
 typedef struct {
@@ -360,7 +364,8 @@ typedef struct {
     double longest_valid_segment; /* DiscreteMotionValidator resolution */
     unsigned factor;
     double deadline_ms;           /* <= 0: none */
-    int max_nodes;                /* <= 0: the server's max_nodes; footprint = 2*max_nodes*60 B */
+    int max_nodes;                /* <= 0: the server's max_nodes; region size from
+                                     ompl_amd_server_footprint */
 } ompl_amd_plan_request;
 
 typedef struct {
@@ -371,6 +376,8 @@ typedef struct {
     double queued_ms;   /* submit -> admission */
     double run_ms;      /* admission -> finish */
     double latency_ms;  /* submit -> finish */
+    int64_t motion_checks; /* growTree steps that checked a motion (isValid + checkMotion
+                              of a goal-tree step count once) */
 } ompl_amd_plan_result;
 
 int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *out);
@@ -384,8 +391,15 @@ int ompl_amd_server_path(ompl_amd_server server, int64_t request_id, double *out
 int ompl_amd_server_cancel(ompl_amd_server server, int64_t request_id);
 /* Frees a finished request's record. */
 int ompl_amd_server_release(ompl_amd_server server, int64_t request_id);
-/* out: ticks, growTree steps, active, queued, used bytes, budget bytes */
+/* out: slices, growTree steps, active, queued, used bytes, budget bytes */
 int ompl_amd_server_stats(ompl_amd_server server, int64_t out[6]);
+/* Device time of all growTree steps run so far, summed over requests, in
+ * microseconds: out[0] sampling, [1] nearest-neighbour scans, [2] motion
+ * checks (steering included), [3] whole steps. */
+int ompl_amd_server_phase_us(ompl_amd_server server, double out[4]);
+/* Bytes of one request's region (trees, path, solve-loop state) for
+ * max_nodes nodes per tree (0 if max_nodes < 2). */
+size_t ompl_amd_server_footprint(int max_nodes);
 /* The same RRT-Connect on the host: host nearest neighbours, one GPU launch
  * and synchronisation per isValid / checkMotion call (the plain path). */
 int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *res, double *path, int cap,

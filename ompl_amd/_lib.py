@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 305
+ABI_VERSION = 306
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -89,6 +89,8 @@ SIGNATURES = {
     "ompl_amd_server_cancel": (C.c_int, [vp, C.c_int64]),
     "ompl_amd_server_release": (C.c_int, [vp, C.c_int64]),
     "ompl_amd_server_stats": (C.c_int, [vp, vp]),
+    "ompl_amd_server_footprint": (C.c_size_t, [C.c_int]),
+    "ompl_amd_server_phase_us": (C.c_int, [vp, vp]),
     "ompl_amd_plan_plain": (C.c_int, [vp, vp, vp, C.c_int, c_int_p]),
     "ompl_amd_prm_build": (C.c_int, [vp, C.c_uint32, C.c_int, C.c_int64, C.c_int, C.c_double, C.c_uint, C.c_uint,
                                      C.POINTER(vp), vp]),

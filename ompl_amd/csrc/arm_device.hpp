@@ -443,16 +443,23 @@ __device__ __forceinline__ void spheres_from_fk(const LaneSpheres<LPS> &F, int t
 // then runs the exact four-sphere test on the obstacles whose bound it
 // touches, one per lane per round until no lane has one left; after each
 // pass the wave leaves once its exit policy is met.  LDS holds the
-// link-layout staging of stage_env<true>.
-template <int OPS, Exit E, bool PADDED>
+// link-layout staging of stage_env<true>.  PADTO (PADDED only): the staging's
+// list padding when it is a multiple of a whole pass rather than one pass
+// (several obstacle partitionings reading one staging); 0 = one pass.
+// stop (kAnyGroup only, may be null): an LDS flag shared by the waves of a
+// block checking one motion; a wave that finds a hit sets it, and a wave that
+// sees it set leaves after its current pass (the motion is invalid anyway).
+template <int OPS, Exit E, bool PADDED, int PADTO = 0>
 __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const float4 *lds, int ns, int nb, bool live,
-                                              int part)
+                                              int part, int *stop = nullptr)
 {
     constexpr int GS = GroupShape<7, OPS>::GS;
     constexpr int K = link_pass(OPS);
-    // PADDED: lists padded to whole passes (stage_env<true, K * OPS>), no
-    // index clamps; else the unpadded layout, indices clamped to the last entry
-    const int nsp = PADDED ? pad_to(ns, K * OPS) : ns, nbp = PADDED ? pad_to(nb, K * OPS) : nb;
+    constexpr int PADM = PADTO ? PADTO : K * OPS;
+    static_assert(PADM % (K * OPS) == 0, "staging padding must hold whole passes");
+    // PADDED: lists padded to whole passes (stage_env<true, PADM>), no index
+    // clamps; else the unpadded layout, indices clamped to the last entry
+    const int nsp = PADDED ? pad_to(ns, PADM) : ns, nbp = PADDED ? pad_to(nb, PADM) : nb;
     auto sidx = [&](int o) { return PADDED ? o : min(o, ns - 1); };
     auto bidx = [&](int o) { return PADDED ? o : min(o, nb - 1); };
     const bool act = live && part < OPS;
@@ -460,13 +467,22 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
     bool hit = false;
     const uint64_t live_leaders = fold_groups<GS>(wave_ballot(live));
     if (live_leaders == 0) return false;
+    int stopped = 0; // the shared flag as read at the start of the current pass
     auto done = [&]() -> bool {
         const uint64_t H = fold_groups<GS>(wave_ballot(hit && live));
         if (E == Exit::kAllGroups) return (H & live_leaders) == live_leaders;
-        if (E == Exit::kAnyGroup) return H != 0;
+        if (E == Exit::kAnyGroup) {
+            if (stop && H != 0 && (threadIdx.x & 63) == 0)
+                __hip_atomic_store(stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return H != 0 || stopped != 0;
+        }
         if (H == 0) return false;
         const uint64_t low = H & (~H + 1);
         return ((low - 1) & live_leaders) == 0;
+    };
+    // read before a pass's bound tests, tested after them
+    auto peek = [&]() {
+        if (E == Exit::kAnyGroup && stop) stopped = __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     const float *rr = reinterpret_cast<const float *>(lds + nsp + 2 * nbp);
     const float4 *boxes = lds + nsp;
@@ -541,10 +557,12 @@ __device__ __forceinline__ bool collide_links(const LaneSpheres<7> &S, const flo
     // meets its first hit in an early pass.
     for (int ob = 0; ob < nb || ob < ns; ob += K * OPS) {
         if (ob < nb) {
+            peek();
             box_pass(ob);
             if (done()) return hit;
         }
         if (ob < ns) {
+            peek();
             sphere_pass(ob);
             if (done()) return hit;
         }

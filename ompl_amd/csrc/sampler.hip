@@ -6,88 +6,19 @@
 // (w1 + w2*2^32) / 2^64, values rounding to 1 replaced by the largest double
 // below 1), then (hi - lo) * u + lo per dimension in order.
 //
-// One wave per generator.  The 624-word state lives in LDS; a twist is done in
-// the four dependency phases of the in-place recurrence
-//   new[i] = src[i+397] ^ twist(old[i], old[i+1])   (indices mod 624)
-// phase A: i in [0,227)   src = old,
-// phase B: i in [227,454) src = new (phase A),
-// phase C: i in [454,623) src = new (phase B),
-// phase D: i = 623        old[624 mod 624] = new[0], src = new[396],
-// each phase reading its inputs before any lane writes.
+// One wave per generator; the 624-word state lives in LDS and is twisted by
+// the block in the recurrence's four dependency phases (mt_device.hpp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <vector>
 
+#include "mt_device.hpp"
 #include "runtime.hpp"
 
 namespace ompl_amd {
 
-constexpr int kMtN = 624;
 constexpr int kDim7 = OMPL_AMD_ARM_DOF;
-constexpr int kMtM = 397;
-constexpr uint32_t kMtMatrix = 0x9908b0dfu;
-constexpr uint32_t kUpper = 0x80000000u, kLower = 0x7fffffffu;
-
-__device__ __forceinline__ uint32_t mt_twist_word(uint32_t cur, uint32_t next, uint32_t src)
-{
-    const uint32_t y = (cur & kUpper) | (next & kLower);
-    return src ^ (y >> 1) ^ ((y & 1u) ? kMtMatrix : 0u);
-}
-
-__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
-{
-    y ^= y >> 11;
-    y ^= (y << 7) & 0x9d2c5680u;
-    y ^= (y << 15) & 0xefc60000u;
-    y ^= y >> 18;
-    return y;
-}
-
-__device__ void mt_twist(uint32_t *mt)
-{
-    const int t = threadIdx.x;
-    // phase A
-    for (int i0 = 0; i0 < 227; i0 += 64) {
-        const int i = i0 + t;
-        uint32_t v = 0;
-        if (i < 227) v = mt_twist_word(mt[i], mt[i + 1], mt[i + kMtM]);
-        __syncthreads();
-        if (i < 227) mt[i] = v;
-        __syncthreads();
-    }
-    // phase B
-    for (int i0 = 227; i0 < 454; i0 += 64) {
-        const int i = i0 + t;
-        uint32_t v = 0;
-        if (i < 454) v = mt_twist_word(mt[i], mt[i + 1], mt[i - 227]);
-        __syncthreads();
-        if (i < 454) mt[i] = v;
-        __syncthreads();
-    }
-    // phase C
-    for (int i0 = 454; i0 < 623; i0 += 64) {
-        const int i = i0 + t;
-        uint32_t v = 0;
-        if (i < 623) v = mt_twist_word(mt[i], mt[i + 1], mt[i - 227]);
-        __syncthreads();
-        if (i < 623) mt[i] = v;
-        __syncthreads();
-    }
-    // phase D
-    if (t == 0) mt[623] = mt_twist_word(mt[623], mt[0], mt[396]);
-    __syncthreads();
-}
-
-__device__ __forceinline__ double canonical53(uint32_t w1, uint32_t w2)
-{
-    double sum = 0.0;
-    sum += (double)w1 * 1.0;
-    sum += (double)w2 * 4294967296.0;
-    double r = sum / 18446744073709551616.0;
-    if (r >= 1.0) r = 0x1.fffffffffffffp-1; // nextafter(1, 0)
-    return r;
-}
 
 struct SampleArgs {
     const uint32_t *seeds;
@@ -123,7 +54,7 @@ __global__ void __launch_bounds__(64) sample_uniform_kernel(SampleArgs A)
     const int64_t total_states = (int64_t)A.n_streams * A.n_per_stream;
     double *out = A.out + (int64_t)s * total_pairs;
     for (int64_t base = 0; base < total_pairs; base += kMtN / 2) {
-        mt_twist(mt);
+        mt_twist_block(mt);
         for (int p = t; p < kMtN / 2; p += 64) {
             const int64_t z = base + p;
             if (z >= total_pairs) break;
@@ -204,7 +135,7 @@ __global__ void __launch_bounds__(64) resume_uniform_kernel(MtState *__restrict_
     double *o = out + (int64_t)a * total;
     for (int64_t z = 0; z < total;) {
         if (idx >= kMtN) {
-            mt_twist(mt);
+            mt_twist_block(mt);
             idx = 0;
         }
         const int64_t cnt = min((int64_t)((kMtN - idx) / 2), total - z);
@@ -240,7 +171,7 @@ __global__ void __launch_bounds__(64) mt_advance_kernel(MtState *__restrict__ st
     int64_t skip = cur[s].used * 2 * dim;
     while (skip > 0) {
         if (idx >= kMtN) {
-            mt_twist(mt);
+            mt_twist_block(mt);
             idx = 0;
         }
         const int take = (int)min((int64_t)(kMtN - idx), skip);

@@ -1,19 +1,28 @@
-// Planning server: continuous batching of RRT-Connect requests on one GPU.
+// Planning server: continuous batching of RRT-Connect requests on one GPU,
+// each request's planner loop resident on the device.
 //
-// Each request is a single-query RRT-Connect plan (geometric/planners/rrt/src/
+// A request is a single-query RRT-Connect plan (geometric/planners/rrt/src/
 // RRTConnect.cpp:116-196 growTree, :245-330 solve) on the 7-DOF arm in one of
-// the registered environments.  Its two trees live in HBM (append-only FP64
-// states + parent indices).  A server thread runs ticks: every tick each
-// active request contributes exactly one growTree step -- nearest neighbour in
-// its tree, steering by the planner range, motion check in the direction of
-// growth, append -- executed for all requests by three launches: nearest +
-// steer (one 256-thread block per request), the motion checks of every request
-// chunk-parallel over the whole GPU (so one long valid motion in a dense
-// environment does not hold the tick for its single wave), and commit.  The
-// host then advances each request's RRT-Connect state machine.
-// New requests are admitted between ticks while their HBM footprint fits the
-// budget (continuous batching); a request's result does not depend on which
-// other requests share its ticks.
+// the registered environments.  Everything the loop touches lives in one
+// region of a device pool sized to the HBM budget: both trees (append-only
+// FP64 states + parent indices), the state sampler's mt19937 and the solve
+// loop's variables, and the solution path.  The server thread launches
+// slices: one block (1,024 or 512 threads) per active request runs its loop
+// -- sample, nearest neighbour, steer, motion check, append, connect loop,
+// path splice -- for a bounded time, then stores its state; the host then
+// retires finished requests and admits queued ones while their region fits
+// the pool (continuous batching), and launches the next slice.  A slice
+// resumes each loop exactly where the last one stopped, so a request's result
+// does not depend on the slices it ran in or the requests it shared them
+// with.
+//
+// Inside a block: the nearest neighbour is a thread-strided FP64 scan of the
+// tree (first strict minimum), reduced by (distance, index); a motion check
+// takes the block's waves over its positions in rounds (the link layout
+// with bounding-sphere culling, the environment staged in LDS once per
+// slice), with a block-wide OR after each round as the early exit; the
+// sampler's words are tempered from the LDS state and the state is twisted
+// by the whole block.
 //
 // Plain counterpart: ompl_amd_plan_plain runs the same RRT-Connect on the host
 // with host nearest neighbours and one GPU launch + synchronisation per
@@ -26,12 +35,14 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <random>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
+#include "mt_device.hpp"
 #include "runtime.hpp"
 #include "walk_device.hpp"
 
@@ -39,228 +50,487 @@ namespace ompl_amd {
 
 enum GrowStatus { kTrapped = 0, kAdvanced = 1, kReached = 2 };
 
-// One growTree step (device).
-struct GrowCmd {
-    const float4 *obs;
-    const float4 *obs_links; // the same obstacles in the link layout's staging
-    int32_t ns, nb;
-    double *tree_s;        // [cap][7]
-    int32_t *tree_parent;  // [cap]
-    int32_t tree_n;        // nodes before this step
-    int32_t start_tree;    // tgi.start: 1 = motion nearest -> new, 0 = new -> nearest (+ isValid(new))
-    double lvs;
-    double maxd;
-    uint32_t factor;
-    int32_t pad;
-    double target[kDof];
-};
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kEqualEps = 2.220446049250313080847e-16 * 2.0; // StateSpace::equalStates (RealVector)
+// Obstacle staging padding shared by the motion check's four partitionings
+// (whole passes of 8, 16, 32 and 144 obstacles): their least common multiple.
+constexpr int kPlanPad = 288;
+// Dynamic LDS of one planner block (the staged environment).
+constexpr size_t kPlanMaxEnvBytes = 60 * 1024;
 
-struct GrowRes {
+// Persistent state of one request's RRTConnect::solve loop (device).
+struct PlanState {
+    uint32_t mt[kMtN]; // the state sampler's RNG (std::mt19937)
+    int32_t mti;       // next word (kMtN: twist first)
     int32_t status;
-    int32_t xmotion; // index of the appended node
-    double xstate[kDof];
+    int32_t iteration;
+    int32_t n0, n1;     // nodes of the start / goal tree
+    int32_t start_tree; // RRTConnect::startTree_
+    int32_t tgi_start;  // tgi.start
+    int32_t grow_tree;  // tree of the next growTree (0 start, 1 goal)
+    int32_t phase;      // 0: sample + grow, 1: first connect step, 2: connect loop
+    int32_t added, xmotion;
+    int32_t path_len;
+    int64_t steps, checks;
+    double rstate[kDof];
 };
 
-// Per-request scratch of one tick (device).
-struct GrowEdge {
-    double a[kDof], b[kDof]; // motion checked in the direction of growth
-    double dst[kDof];        // state to append
-    int32_t nd;              // segments of the motion
-    int32_t npos;            // positions to check (0 = none: trapped before the check)
-    int32_t extra;           // 1: the last position is isValid(dst) (goal tree)
-    int32_t nm;              // nearest node
-    int32_t reach;
-    int32_t first_bad;       // lowest invalid position found (INT_MAX: none)
-};
-
-struct TickArgs {
-    const GrowCmd *cmds;
-    GrowEdge *edges;
-    GrowRes *res;
-    int32_t *max_pos; // largest npos of the tick (reset by the commit kernel)
-    int ncmd;
-};
-
-// Tick kernel 1 (one block of kPrepThreads per request): nearest neighbour in
-// the request's tree -- first strict minimum of distance(elem, target),
-// thread-strided, then reduced over each wave and across the block's waves
-// by (distance, index) -- and steering by the planner range
-// (RRTConnect.cpp:119-138).
-constexpr int kPrepThreads = 256;
-
-__global__ void __launch_bounds__(kPrepThreads) grow_prepare_kernel(TickArgs T)
+__host__ __device__ inline size_t align256(size_t b)
 {
-    const int r = blockIdx.x;
-    if (r >= T.ncmd) return;
-    const GrowCmd &C = T.cmds[r];
-    GrowEdge &E = T.edges[r];
-    const int lane = lane_id();
+    return (b + 255) & ~(size_t)255;
+}
+
+// One request's region: [start tree states][goal tree states][start parents]
+// [goal parents][path states][path node indices][PlanState].  Tree states
+// are dof-major (coordinate k of node i at k * cap + i), so a wave's reads of
+// consecutive nodes coalesce; the path is row-major.
+struct RegionPtrs {
+    double *tree_s[2];
+    int32_t *parent[2];
+    double *path;
+    int32_t *path_idx;
+    PlanState *state;
+    size_t bytes;
+};
+
+__host__ __device__ inline RegionPtrs region_ptrs(char *base, int cap)
+{
+    RegionPtrs R;
+    size_t off = 0;
+    for (int t = 0; t < 2; ++t) {
+        R.tree_s[t] = reinterpret_cast<double *>(base + off);
+        off += align256((size_t)cap * kDof * sizeof(double));
+    }
+    for (int t = 0; t < 2; ++t) {
+        R.parent[t] = reinterpret_cast<int32_t *>(base + off);
+        off += align256((size_t)cap * sizeof(int32_t));
+    }
+    R.path = reinterpret_cast<double *>(base + off);
+    off += align256((size_t)2 * cap * kDof * sizeof(double));
+    R.path_idx = reinterpret_cast<int32_t *>(base + off);
+    off += align256((size_t)2 * cap * sizeof(int32_t));
+    R.state = reinterpret_cast<PlanState *>(base + off);
+    off += align256(sizeof(PlanState));
+    R.bytes = off;
+    return R;
+}
+
+static size_t footprint(int max_nodes)
+{
+    return region_ptrs(nullptr, max_nodes).bytes;
+}
+
+// Per active request and slice (device copy of a host array).
+struct PlanCmd {
+    const float4 *obs; // environment, staging order (spheres, then boxes as lo/hi pairs)
+    char *region;
+    int32_t ns, nb;
+    int32_t cap;       // nodes per tree
+    int32_t max_iters;
+    double maxd;       // maxDistance_
+    double lvs;        // longest valid segment
+    uint32_t factor;
+    uint32_t seed;
+    int32_t init;      // 1: first slice (seed the sampler, plant the roots)
+    int32_t ops;       // obstacle partitions of the motion check (0: by motion length)
+    double start[kDof];
+    double goal[kDof];
+};
+
+// Per active request, written at the end of each slice.  t_*: device
+// wall-clock ticks this slice spent sampling, in nearest-neighbour scans, in
+// motion checks (steering included) and in whole growTree steps.
+struct PlanOut {
+    int32_t status, iteration, n0, n1, path_len, pad;
+    int64_t steps, checks;
+    int64_t t_sample, t_nearest, t_check, t_step;
+};
+
+struct SliceArgs {
+    const PlanCmd *cmds;
+    PlanOut *out;
+    int64_t budget; // wall-clock ticks per slice (at least one step is always made)
+};
+
+static size_t env_stage_bytes(int ns, int nb)
+{
+    return (size_t)(pad_to(ns, kPlanPad) + 2 * pad_to(nb, kPlanPad)) * sizeof(float4) + (size_t)ns * sizeof(float);
+}
+
+// Obstacle partitions for a motion of npos positions: the widest split whose
+// round (NW waves x states per wave) still covers the motion, else one.
+template <int NW>
+__device__ __forceinline__ int auto_ops(int npos)
+{
+    return npos <= NW * 1 ? 9 : npos <= NW * 2 ? 4 : npos <= NW * 4 ? 2 : 1;
+}
+
+// Nearest node of tree T (n nodes, dof-major with stride cap) to tgt: first
+// strict minimum of distance(node, tgt) (NearestNeighborsLinear::nearest
+// order).  Every thread returns the same index.
+template <int NT>
+__device__ __forceinline__ int block_nearest(const double *__restrict__ T, int cap, int n, const double tgt[kDof],
+                                             double *s_wd, int *s_wi)
+{
     double best = INFINITY;
-    int best_i = INT_MAX;
-    for (int i = threadIdx.x; i < C.tree_n; i += kPrepThreads) {
-        const double *s = C.tree_s + (int64_t)i * kDof;
+    int bi = INT_MAX;
+    for (int i = threadIdx.x; i < n; i += NT) {
         double dist = 0.0;
 #pragma unroll
         for (int k = 0; k < kDof; ++k) {
-            const double diff = s[k] - C.target[k];
+            const double diff = T[(int64_t)k * cap + i] - tgt[k];
             dist += diff * diff;
         }
         const double d = __builtin_sqrt(dist);
-        if (d < best) { // increasing i per lane: strict < keeps the lowest index
+        if (d < best) { // a thread's indices increase: strict < keeps its lowest
             best = d;
-            best_i = i;
+            bi = i;
         }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         const double od = __shfl_xor(best, off);
-        const int oi = __shfl_xor(best_i, off);
-        if (od < best || (od == best && oi < best_i)) {
+        const int oi = __shfl_xor(bi, off);
+        if (od < best || (od == best && oi < bi)) {
             best = od;
-            best_i = oi;
+            bi = oi;
         }
     }
-    __shared__ double wd[kPrepThreads / 64];
-    __shared__ int wi[kPrepThreads / 64];
-    if (lane == 0) {
-        wd[threadIdx.x >> 6] = best;
-        wi[threadIdx.x >> 6] = best_i;
+    if (lane_id() == 0) {
+        s_wd[threadIdx.x >> 6] = best;
+        s_wi[threadIdx.x >> 6] = bi;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < kPrepThreads / 64; ++w)
-        if (wd[w] < best || (wd[w] == best && wi[w] < best_i)) {
-            best = wd[w];
-            best_i = wi[w];
+    best = s_wd[0];
+    bi = s_wi[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w)
+        if (s_wd[w] < best || (s_wd[w] == best && s_wi[w] < bi)) {
+            best = s_wd[w];
+            bi = s_wi[w];
         }
-    const int nm = best_i;
-    double ns_[kDof], x[kDof];
-    for (int k = 0; k < kDof; ++k) ns_[k] = C.tree_s[(int64_t)nm * kDof + k];
-    E.nm = nm;
-    E.first_bad = INT_MAX;
-    const double d = best;
-    bool reach = true;
-    for (int k = 0; k < kDof; ++k) x[k] = C.target[k];
-    if (d > C.maxd) {
-        const double t = C.maxd / d;
-        bool equal = true;
-        for (int k = 0; k < kDof; ++k) {
-            x[k] = ns_[k] + (C.target[k] - ns_[k]) * t;
-            if (fabs(ns_[k] - x[k]) > 2.220446049250313080847e-16 * 2.0) equal = false;
-        }
-        if (equal) { // no progress: TRAPPED without a check
-            E.npos = 0;
-            return;
-        }
-        reach = false;
-    }
-    E.reach = reach;
-    for (int k = 0; k < kDof; ++k) {
-        E.dst[k] = x[k];
-        E.a[k] = C.start_tree ? ns_[k] : x[k]; // start tree: checkMotion(nearest, dst)
-        E.b[k] = C.start_tree ? x[k] : ns_[k]; // goal tree: isValid(dst) && checkMotion(dst, nearest)
-    }
-    const int nd = valid_segment_count<kDof>(E.a, E.b, C.lvs, C.factor);
-    if (nd > kMaxSegments) { // reported invalid
-        E.npos = 1;
-        E.first_bad = 0;
-        return;
-    }
-    E.nd = nd;
-    E.extra = C.start_tree ? 0 : 1;
-    E.npos = 1 + (nd >= 2 ? nd - 1 : 0) + E.extra;
-    atomicMax(T.max_pos, E.npos);
+    return bi;
 }
 
-// Tick kernel 2: every request's positions, chunk-parallel over a persistent
-// grid: items (chunk, request) chunk-major, one wave per item (wave w takes
-// items w, w + W, ...); a wave tests whether its next 64 items lie inside
-// their motions at once, then checks the live ones in item order, skipping a
-// request whose motion is already known invalid.  Obstacles are read from
-// each request's environment in HBM (wave-uniform loads), so requests in
-// different environments share the launch.
-template <int LPS, int OPS>
-__global__ void __launch_bounds__(256) grow_check_kernel(TickArgs T)
+// The motion's positions by the whole block, in rounds of NT/64 waves x G
+// states: with `extra`, position 0 is isValid(dst) and position p >= 1 the
+// DiscreteMotionValidator position p-1 of (a, b) (0 = b, then the bisection
+// order); without, position p is the validator's position p.  Returns false
+// at the first round holding an invalid position.  Validity of the whole
+// motion does not depend on the order positions are visited in.  a, b and
+// dst are in LDS (read per round; no registers held across the rounds);
+// `stop` is an LDS flag, zero on entry, that lets the other waves of a round
+// leave once one has found an invalid position.
+template <int NT, int OPS>
+__device__ __forceinline__ bool block_check(const double *a, const double *b, const double *dst, int nd, int npos,
+                                            bool extra, const float4 *env, int ns, int nb, int *stop)
 {
-    using Shape = GroupShape<LPS, OPS>;
+    using Shape = GroupShape<7, OPS>;
     constexpr int G = Shape::STATES;
-    const int lane = lane_id();
-    const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int n = T.ncmd;
-    const int64_t nblk = (*T.max_pos + G - 1) / G;
-    const int64_t items = (int64_t)n * nblk;
-    const Shape sh(lane);
-    for (int64_t k0 = wave; k0 < items; k0 += 64 * waves_total) {
-        const int64_t k = k0 + lane * waves_total;
-        int r = 0, base = 0;
-        bool live = false;
-        if (k < items) {
-            const int64_t blk = k / n;
-            r = (int)(k - blk * n);
-            base = (int)(blk * G);
-            live = base < T.edges[r].npos;
-        }
-        uint64_t mask = wave_ballot(live);
-        while (mask) {
-            const int j = lowest_bit(mask);
-            mask &= mask - 1;
-            const int rj = __shfl(r, j), bj = __shfl(base, j);
-            GrowEdge &E = T.edges[rj];
-            if (__hip_atomic_load(&E.first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != INT_MAX) continue;
-            const GrowCmd &C = T.cmds[rj];
-            const int p = bj + sh.group;
-            const int motion_pos = E.npos - E.extra;
-            const bool plive = p < E.npos;
-            float q[kDof];
-            if (plive && p >= motion_pos) {
+    const int wave = threadIdx.x >> 6;
+    const Shape sh(lane_id());
+    for (int base = 0; base < npos; base += NT / 64 * G) {
+        const int p = base + wave * G + sh.group;
+        const bool live = p < npos;
+        float q[kDof];
+        if (extra && p == 0) {
 #pragma unroll
-                for (int d = 0; d < kDof; ++d) q[d] = (float)E.dst[d];
-            } else {
-                double a[kDof], b[kDof];
-#pragma unroll
-                for (int d = 0; d < kDof; ++d) {
-                    a[d] = E.a[d];
-                    b[d] = E.b[d];
-                }
-                walk_state(a, b, E.nd, plive ? p : 0, q);
-            }
-            LaneSpheres<LPS> S;
-            arm_fk<LPS, Shape::GS>(q, sh.slot, S);
-            const float4 *obs = LPS == 7 ? C.obs_links : C.obs;
-            const bool hit = collide_env<LPS, OPS, Exit::kAnyGroup, false>(S, obs, C.ns, C.nb, plive, sh.part);
-            const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && plive));
-            if (H && lane == 0) atomicMin(&E.first_bad, bj + lowest_bit(H) / Shape::GS);
+            for (int k = 0; k < kDof; ++k) q[k] = (float)dst[k];
+        } else {
+            const int mp = extra ? p - 1 : p;
+            walk_state(a, b, nd, live ? mp : 0, q);
         }
+        LaneSpheres<7> S;
+        arm_fk<7, Shape::GS>(q, sh.slot, S);
+        const bool hit = collide_links<OPS, Exit::kAnyGroup, true, kPlanPad>(S, env, ns, nb, live, sh.part, stop);
+        if (__syncthreads_or(hit && live)) return false;
     }
+    return true;
 }
 
-// Tick kernel 3 (one thread per request): append the new node of a valid
-// motion, write the outcome, clear the tick counter.
-__global__ void grow_commit_kernel(TickArgs T)
+// Runs request blockIdx.x's RRTConnect::solve loop for one slice.  Control
+// flow is block-uniform: every thread holds the loop's variables and updates
+// them identically; values one thread produces (the sample, the nearest
+// node) reach the others through LDS.
+template <int NT>
+__global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r == 0) *T.max_pos = 0;
-    if (r >= T.ncmd) return;
-    const GrowCmd &C = T.cmds[r];
-    const GrowEdge &E = T.edges[r];
-    GrowRes &R = T.res[r];
-    if (E.npos == 0 || E.first_bad != INT_MAX) {
-        R.status = kTrapped;
-        return;
+    extern __shared__ float4 s_env[];
+    __shared__ uint32_t s_mt[kMtN];
+    __shared__ double s_rstate[kDof];
+    __shared__ double s_wd[NT / 64];
+    __shared__ int s_wi[NT / 64];
+    __shared__ double s_a[kDof], s_b[kDof], s_x[kDof]; // the step's motion and new state
+    __shared__ double s_u[kMtN / 2]; // the 0-1 draws of the generator's current 624-word block
+    __shared__ int s_stop, s_hit, s_n1, s_len;
+    const int tid = threadIdx.x;
+    const int64_t t0 = wall_clock64();
+    const PlanCmd &C = A.cmds[blockIdx.x];
+    const RegionPtrs R = region_ptrs(C.region, C.cap);
+    PlanState *st = R.state;
+
+    int mti, status, iteration, n0, n1, start_tree, tgi_start, grow_tree, phase, added, xmotion, path_len;
+    int64_t steps, checks;
+    if (C.init) {
+        if (tid == 0) {
+            mt_seed_serial(s_mt, C.seed);
+            for (int k = 0; k < kDof; ++k) {
+                R.tree_s[0][(int64_t)k * C.cap] = C.start[k];
+                R.tree_s[1][(int64_t)k * C.cap] = C.goal[k];
+            }
+            R.parent[0][0] = -1;
+            R.parent[1][0] = -1;
+        }
+        if (tid < kDof) s_rstate[tid] = 0.0;
+        mti = kMtN;
+        status = OMPL_AMD_PLAN_RUNNING;
+        iteration = 0;
+        n0 = n1 = 1;
+        start_tree = 1;
+        tgi_start = 1;
+        grow_tree = 0;
+        phase = 0;
+        added = xmotion = -1;
+        path_len = 0;
+        steps = checks = 0;
+    } else {
+        for (int i = tid; i < kMtN; i += NT) s_mt[i] = st->mt[i];
+        if (tid < kDof) s_rstate[tid] = st->rstate[tid];
+        mti = st->mti;
+        status = st->status;
+        iteration = st->iteration;
+        n0 = st->n0;
+        n1 = st->n1;
+        start_tree = st->start_tree;
+        tgi_start = st->tgi_start;
+        grow_tree = st->grow_tree;
+        phase = st->phase;
+        added = st->added;
+        xmotion = st->xmotion;
+        path_len = st->path_len;
+        steps = st->steps;
+        checks = st->checks;
     }
-    const int idx = C.tree_n;
-    for (int k = 0; k < kDof; ++k) {
-        C.tree_s[(int64_t)idx * kDof + k] = E.dst[k];
-        R.xstate[k] = E.dst[k];
+    __syncthreads();
+    // the draws of the current block (consecutive word pairs; a sample's seven
+    // draws are seven consecutive pairs, so a pair never spans two twists)
+    auto refill_draws = [&]() {
+        for (int i = tid; i < kMtN / 2; i += NT) s_u[i] = canonical53(mt_temper(s_mt[2 * i]), mt_temper(s_mt[2 * i + 1]));
+    };
+    if (mti < kMtN) refill_draws();
+    int pidx = mti / 2; // next draw (words are consumed in pairs: mti is even)
+    stage_env<true, kPlanPad>(EnvView{C.obs, C.ns, C.nb}, s_env); // ends with a barrier
+
+    int done_here = 0;
+    int64_t t_sample = 0, t_nearest = 0, t_check = 0, t_step = 0, c0 = 0, c1 = 0; // thread 0's timers
+    while (status == OMPL_AMD_PLAN_RUNNING) {
+        if (tid == 0) s_stop = done_here > 0 && wall_clock64() - t0 >= A.budget;
+        __syncthreads();
+        if (s_stop) break;
+        // a step appends at most one node to one tree
+        if (n0 >= C.cap || n1 >= C.cap) {
+            status = OMPL_AMD_PLAN_TREE_FULL;
+            break;
+        }
+        ++done_here;
+        if (tid == 0) c0 = c1 = wall_clock64();
+        if (phase == 0) {
+            // RRTConnect.cpp:247-250: pick the tree, flip startTree_
+            grow_tree = start_tree ? 0 : 1;
+            tgi_start = start_tree;
+            start_tree = !start_tree;
+            // sampler_->sampleUniform(rstate): RNG::uniformReal(-pi, pi) per joint,
+            // from the next seven draws (twisting when the block runs out)
+            for (int d0 = 0; d0 < kDof;) {
+                if (pidx >= kMtN / 2) {
+                    mt_twist_block(s_mt);
+                    refill_draws();
+                    __syncthreads();
+                    pidx = 0;
+                }
+                const int take = min(kDof - d0, kMtN / 2 - pidx);
+                if (tid < take) s_rstate[d0 + tid] = (kPi - -kPi) * s_u[pidx + tid] + -kPi;
+                pidx += take;
+                d0 += take;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                c1 = wall_clock64();
+                t_sample += c1 - c0;
+            }
+        }
+
+        // ---- growTree(tree, tgi, rmotion) (RRTConnect.cpp:116-196)
+        double tgt[kDof];
+#pragma unroll
+        for (int k = 0; k < kDof; ++k) tgt[k] = s_rstate[k];
+        const int nt = grow_tree ? n1 : n0;
+        double *T = R.tree_s[grow_tree];
+        const int nm = block_nearest<NT>(T, C.cap, nt, tgt, s_wd, s_wi);
+        if (tid == 0) {
+            const int64_t c2 = wall_clock64();
+            t_nearest += c2 - c1;
+            c1 = c2;
+        }
+        double nst[kDof], x[kDof];
+        double dist = 0.0;
+#pragma unroll
+        for (int k = 0; k < kDof; ++k) {
+            nst[k] = T[(int64_t)k * C.cap + nm];
+            const double diff = nst[k] - tgt[k];
+            dist += diff * diff;
+        }
+        const double dd = __builtin_sqrt(dist);
+        bool reach = true, moved = true;
+#pragma unroll
+        for (int k = 0; k < kDof; ++k) x[k] = tgt[k];
+        if (dd > C.maxd) {
+            const double t = C.maxd / dd;
+            bool equal = true;
+#pragma unroll
+            for (int k = 0; k < kDof; ++k) {
+                x[k] = nst[k] + (tgt[k] - nst[k]) * t;
+                if (fabs(nst[k] - x[k]) > kEqualEps) equal = false;
+            }
+            moved = !equal; // no progress: TRAPPED without a check
+            reach = false;
+        }
+        int gs = kTrapped;
+        if (moved) {
+            // tgi.start ? checkMotion(nearest, dst) : isValid(dst) && checkMotion(dst, nearest)
+            const bool from_start = tgi_start != 0;
+            double a[kDof], b[kDof];
+#pragma unroll
+            for (int k = 0; k < kDof; ++k) {
+                a[k] = from_start ? nst[k] : x[k];
+                b[k] = from_start ? x[k] : nst[k];
+            }
+            const int nd = valid_segment_count<kDof>(a, b, C.lvs, C.factor);
+            if (tid == 0) {
+                for (int k = 0; k < kDof; ++k) {
+                    s_a[k] = a[k];
+                    s_b[k] = b[k];
+                    s_x[k] = x[k];
+                }
+                s_hit = 0;
+            }
+            __syncthreads();
+            ++checks;
+            bool valid = false;
+            if (nd <= kMaxSegments) { // longer motions are reported invalid
+                const int npos = 1 + (nd >= 2 ? nd - 1 : 0) + (from_start ? 0 : 1);
+                const int ops = C.ops ? C.ops : auto_ops<NT / 64>(npos);
+                if (ops == 9)
+                    valid = block_check<NT, 9>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
+                else if (ops == 4)
+                    valid = block_check<NT, 4>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
+                else if (ops == 2)
+                    valid = block_check<NT, 2>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
+                else
+                    valid = block_check<NT, 1>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
+            }
+            if (valid) {
+                if (tid < kDof) T[(int64_t)tid * C.cap + nt] = s_x[tid];
+                if (tid == 0) R.parent[grow_tree][nt] = nm;
+                if (grow_tree) ++n1;
+                else ++n0;
+                xmotion = nt;
+                gs = reach ? kReached : kAdvanced;
+            }
+        }
+        ++steps;
+        if (tid == 0) t_check += wall_clock64() - c1;
+
+        // ---- solve loop (RRTConnect.cpp:245-330)
+        bool end_iteration = false;
+        if (phase == 0) {
+            if (gs == kTrapped) {
+                end_iteration = true;
+            } else {
+                added = xmotion;
+                // if not reached, grow the other tree towards the added state
+                if (gs != kReached && tid == 0)
+                    for (int k = 0; k < kDof; ++k) s_rstate[k] = s_x[k];
+                tgi_start = start_tree;
+                grow_tree = start_tree ? 0 : 1; // otherTree
+                phase = 1;
+            }
+        } else {
+            if (phase == 1 && gs == kTrapped) tgi_start = !tgi_start;
+            if (gs == kAdvanced) {
+                phase = 2;
+            } else if (gs == kReached) {
+                // RRTConnect.cpp:301-330: step back once on the start side (or
+                // the goal side at its root), then start chain reversed + goal chain
+                int sm = tgi_start ? xmotion : added, gm = tgi_start ? added : xmotion;
+                if (tid == 0) {
+                    const int32_t *ps = R.parent[0], *pg = R.parent[1];
+                    if (ps[sm] >= 0) sm = ps[sm];
+                    else gm = pg[gm];
+                    int c = 0;
+                    for (int m = sm; m >= 0; m = ps[m]) R.path_idx[c++] = m;
+                    s_n1 = c;
+                    for (int m = gm; m >= 0; m = pg[m]) R.path_idx[c++] = m;
+                    s_len = c;
+                }
+                __syncthreads();
+                const int c1 = s_n1, len = s_len;
+                for (int r = tid; r < len * kDof; r += NT) {
+                    const int row = r / kDof, k = r - row * kDof;
+                    R.path[r] = row < c1 ? R.tree_s[0][(int64_t)k * C.cap + R.path_idx[c1 - 1 - row]]
+                                         : R.tree_s[1][(int64_t)k * C.cap + R.path_idx[row]];
+                }
+                path_len = len;
+                status = OMPL_AMD_PLAN_SOLVED;
+                ++iteration;
+            } else {
+                end_iteration = true;
+            }
+        }
+        if (end_iteration) {
+            phase = 0;
+            ++iteration;
+            if (iteration >= C.max_iters) status = OMPL_AMD_PLAN_FAILED;
+        }
+        if (tid == 0) t_step += wall_clock64() - c0;
     }
-    C.tree_parent[idx] = E.nm;
-    R.status = E.reach ? kReached : kAdvanced;
-    R.xmotion = idx;
+
+    __syncthreads();
+    for (int i = tid; i < kMtN; i += NT) st->mt[i] = s_mt[i];
+    if (tid < kDof) st->rstate[tid] = s_rstate[tid];
+    if (tid == 0) {
+        st->mti = 2 * pidx;
+        st->status = status;
+        st->iteration = iteration;
+        st->n0 = n0;
+        st->n1 = n1;
+        st->start_tree = start_tree;
+        st->tgi_start = tgi_start;
+        st->grow_tree = grow_tree;
+        st->phase = phase;
+        st->added = added;
+        st->xmotion = xmotion;
+        st->path_len = path_len;
+        st->steps = steps;
+        st->checks = checks;
+        PlanOut &o = A.out[blockIdx.x];
+        o.status = status;
+        o.iteration = iteration;
+        o.n0 = n0;
+        o.n1 = n1;
+        o.path_len = path_len;
+        o.pad = 0;
+        o.steps = steps;
+        o.checks = checks;
+        o.t_sample = t_sample;
+        o.t_nearest = t_nearest;
+        o.t_check = t_check;
+        o.t_step = t_step;
+    }
 }
 
 // ---------------------------------------------------------------------------
-// RRT-Connect request state machine (host)
+// host side
 // ---------------------------------------------------------------------------
 using Clock = std::chrono::steady_clock;
 
@@ -270,217 +540,118 @@ static double ms_since(Clock::time_point t0)
 }
 
 struct Planner {
-    // configuration
     ompl_amd_plan_request req;
-    const Env *env = nullptr;
     double maxd = 0.0;
-    // RNG of the state sampler (RNG::uniformReal over std::mt19937)
-    std::mt19937 gen;
-    std::uniform_real_distribution<> uni{0, 1};
-    // trees: host mirrors of the node counts; parents/states live on device
-    int n[2] = {0, 0};
-    int cap = 0;
-    double *tree_s[2] = {nullptr, nullptr};
-    int32_t *tree_parent[2] = {nullptr, nullptr};
-    // solve-loop state
-    int iteration = 0;
-    bool start_tree = true; // RRTConnect::startTree_
-    int tgi_start = 1;
-    int grow_tree = 0;      // tree index of the pending growTree (0 = start, 1 = goal)
-    int phase = 0;          // 0 sample+grow, 1 first connect, 2 connect loop
-    int added = -1, xmotion = -1;
-    double rstate[kDof];
-    // result
-    int status = OMPL_AMD_PLAN_QUEUED;
-    std::vector<double> path;
-    int64_t nodes = 0;
     Clock::time_point t_submit, t_start, t_finish;
+    // published: written and read under Server::mu only
+    int status = OMPL_AMD_PLAN_QUEUED;
+    int iterations = 0;
+    int64_t nodes = 0;
+    int64_t checks = 0;
+    std::vector<double> path;
+    bool started = false;
     bool cancel = false;
-
-    double uniformReal(double lo, double hi) { return (hi - lo) * uni(gen) + lo; }
+    // server thread only
+    size_t off = 0, bytes = 0; // region in the pool
+    bool init = true;
+    int64_t steps_seen = 0;
 };
 
-constexpr double kPi = 3.141592653589793238462643383279502884;
-
-// Prepares the next growTree command of a planner (phase 0 samples first).
-static void next_command(Planner &p, GrowCmd &c)
-{
-    if (p.phase == 0) {
-        const int tree = p.start_tree ? 0 : 1;
-        p.tgi_start = p.start_tree ? 1 : 0;
-        p.start_tree = !p.start_tree;
-        for (int k = 0; k < kDof; ++k) p.rstate[k] = p.uniformReal(-kPi, kPi);
-        p.grow_tree = tree;
+// First-fit allocator over the device pool (offsets; neighbours coalesce).
+struct PoolAlloc {
+    std::map<size_t, size_t> free; // offset -> size
+    void reset(size_t size)
+    {
+        free.clear();
+        if (size) free[0] = size;
     }
-    // phases 1/2 grow the other tree toward rstate with the current tgi_start
-    c.obs = p.env->obs;
-    c.obs_links = p.env->obs_links;
-    c.ns = p.env->ns;
-    c.nb = p.env->nb;
-    c.tree_s = p.tree_s[p.grow_tree];
-    c.tree_parent = p.tree_parent[p.grow_tree];
-    c.tree_n = p.n[p.grow_tree];
-    c.start_tree = p.tgi_start;
-    c.lvs = p.req.longest_valid_segment;
-    c.maxd = p.maxd;
-    c.factor = p.req.factor;
-    c.pad = 0;
-    for (int k = 0; k < kDof; ++k) c.target[k] = p.rstate[k];
-}
-
-// Reconstructs the solution path (RRTConnect.cpp:303-340) from device trees.
-static void build_path(Planner &p, int sm, int gm, hipStream_t s)
-{
-    std::vector<double> ss((size_t)p.n[0] * kDof), gs((size_t)p.n[1] * kDof);
-    std::vector<int32_t> sp((size_t)p.n[0]), gp((size_t)p.n[1]);
-    OA_HIP(hipMemcpyAsync(ss.data(), p.tree_s[0], ss.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    OA_HIP(hipMemcpyAsync(gs.data(), p.tree_s[1], gs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    OA_HIP(hipMemcpyAsync(sp.data(), p.tree_parent[0], sp.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    OA_HIP(hipMemcpyAsync(gp.data(), p.tree_parent[1], gp.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    OA_HIP(hipStreamSynchronize(s));
-    if (sp[sm] >= 0) sm = sp[sm];
-    else gm = gp[gm];
-    std::vector<int> chain1, chain2;
-    for (int m = sm; m >= 0; m = sp[m]) chain1.push_back(m);
-    for (int m = gm; m >= 0; m = gp[m]) chain2.push_back(m);
-    p.path.clear();
-    for (int i = (int)chain1.size() - 1; i >= 0; --i)
-        p.path.insert(p.path.end(), ss.begin() + (size_t)chain1[i] * kDof, ss.begin() + (size_t)(chain1[i] + 1) * kDof);
-    for (int m : chain2) p.path.insert(p.path.end(), gs.begin() + (size_t)m * kDof, gs.begin() + (size_t)(m + 1) * kDof);
-}
-
-// Applies a growTree outcome; returns true when the request finished.
-static bool apply_result(Planner &p, const GrowRes &r, hipStream_t s)
-{
-    if (r.status != kTrapped) {
-        p.n[p.grow_tree] = r.xmotion + 1;
-        p.xmotion = r.xmotion;
+    bool alloc(size_t bytes, size_t &off)
+    {
+        for (auto it = free.begin(); it != free.end(); ++it)
+            if (it->second >= bytes) {
+                off = it->first;
+                const size_t rest = it->second - bytes;
+                free.erase(it);
+                if (rest) free[off + bytes] = rest;
+                return true;
+            }
+        return false;
     }
-    auto end_iteration = [&]() {
-        p.phase = 0;
-        ++p.iteration;
-        if (p.iteration >= p.req.max_iters) {
-            p.status = OMPL_AMD_PLAN_FAILED;
-            return true;
+    void release(size_t off, size_t bytes)
+    {
+        auto it = free.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free.erase(nx);
         }
-        return false;
-    };
-    if (p.phase == 0) {
-        if (r.status == kTrapped) return end_iteration();
-        p.added = r.xmotion;
-        if (r.status != kReached)
-            for (int k = 0; k < kDof; ++k) p.rstate[k] = r.xstate[k];
-        p.tgi_start = p.start_tree ? 1 : 0;
-        p.grow_tree = p.start_tree ? 0 : 1; // otherTree
-        p.phase = 1;
-        return false;
+        if (it != free.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free.erase(it);
+            }
+        }
     }
-    // connect steps
-    const int gsc = r.status;
-    if (p.phase == 1 && gsc == kTrapped) p.tgi_start = !p.tgi_start;
-    if (gsc == kAdvanced) {
-        p.phase = 2;
-        return false;
-    }
-    if (gsc == kReached) {
-        const int sm = p.tgi_start ? p.xmotion : p.added;
-        const int gm = p.tgi_start ? p.added : p.xmotion;
-        build_path(p, sm, gm, s);
-        p.status = OMPL_AMD_PLAN_SOLVED;
-        ++p.iteration;
-        return true;
-    }
-    return end_iteration();
-}
-
-static void init_planner(Planner &p, const ompl_amd_plan_request &req, double *s0, int32_t *p0, double *s1,
-                         int32_t *p1, int cap, hipStream_t s)
-{
-    p.req = req;
-    p.env = &req.env->env;
-    p.maxd = req.range > 0.0 ? req.range : 0.2 * 2.0 * kPi * std::sqrt(7.0);
-    p.gen.seed(req.seed);
-    p.uni.reset();
-    p.cap = cap;
-    p.tree_s[0] = s0;
-    p.tree_parent[0] = p0;
-    p.tree_s[1] = s1;
-    p.tree_parent[1] = p1;
-    p.n[0] = p.n[1] = 1;
-    const int32_t root = -1;
-    OA_HIP(hipMemcpyAsync(s0, req.start, kDof * sizeof(double), hipMemcpyHostToDevice, s));
-    OA_HIP(hipMemcpyAsync(s1, req.goal, kDof * sizeof(double), hipMemcpyHostToDevice, s));
-    OA_HIP(hipMemcpyAsync(p0, &root, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    OA_HIP(hipMemcpyAsync(p1, &root, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    OA_HIP(hipStreamSynchronize(s));
-    p.iteration = 0;
-    p.start_tree = true;
-    p.phase = 0;
-    p.status = OMPL_AMD_PLAN_RUNNING;
-}
-
-static size_t footprint(const ompl_amd_plan_request &r)
-{
-    return (size_t)2 * (size_t)r.max_nodes * (kDof * sizeof(double) + sizeof(int32_t));
-}
-
-// ---------------------------------------------------------------------------
-// server
-// ---------------------------------------------------------------------------
-struct Slot {
-    double *s[2];
-    int32_t *p[2];
 };
 
 struct Server {
     ompl_amd_server_config cfg;
     hipStream_t stream = nullptr;
-    std::vector<Slot> slots;
-    std::vector<int> free_slots;
     char *pool = nullptr;
-    GrowCmd *cmd_host = nullptr, *cmd_dev = nullptr;
-    GrowRes *res_host = nullptr, *res_dev = nullptr;
-    GrowEdge *edges_dev = nullptr;
-    int32_t *max_pos_dev = nullptr;
-    unsigned check_blocks = 0;
+    PoolAlloc alloc;
+    PlanCmd *cmd_host = nullptr, *cmd_dev = nullptr;
+    PlanOut *out_host = nullptr, *out_dev = nullptr;
+    int64_t budget_ticks = 0;
 
     std::mutex mu;
     std::condition_variable cv, done_cv;
     std::deque<int64_t> queue;
     std::unordered_map<int64_t, Planner *> reqs;
-    std::vector<std::pair<int64_t, int>> active; // (id, slot)
+    std::vector<int64_t> active;
     int64_t next_id = 1;
     size_t used_bytes = 0;
     bool stop = false;
     std::thread th;
-    std::atomic<int64_t> ticks{0}, steps{0};
+    std::atomic<int64_t> slices{0}, steps{0};
+    int64_t phase_ticks[4] = {0, 0, 0, 0}; // sample, nearest, check, step (under mu)
+    int64_t clock_khz = 0;
     std::string fault;
 
     void run();
     void admit_locked();
+    void finish_locked(Planner *p, int status);
 };
+
+// Publishes a final status (caller holds mu) and returns the region.
+void Server::finish_locked(Planner *p, int status)
+{
+    p->status = status;
+    p->t_finish = Clock::now();
+    if (p->bytes) {
+        alloc.release(p->off, p->bytes);
+        used_bytes -= p->bytes;
+        p->bytes = 0;
+    }
+}
 
 void Server::admit_locked()
 {
-    while (!queue.empty() && !free_slots.empty()) {
+    while (!queue.empty() && (int)active.size() < cfg.max_active) {
         const int64_t id = queue.front();
         Planner *p = reqs[id];
-        if (p->cancel) {
-            queue.pop_front();
-            p->status = OMPL_AMD_PLAN_CANCELLED;
-            p->t_finish = Clock::now();
-            continue;
-        }
-        const size_t fp = footprint(p->req);
-        if (used_bytes + fp > cfg.hbm_budget_bytes) break; // FIFO: wait for memory
+        const size_t fp = footprint(p->req.max_nodes);
+        size_t off = 0;
+        if (!alloc.alloc(fp, off)) break; // FIFO: the head waits for memory
         queue.pop_front();
-        const int slot = free_slots.back();
-        free_slots.pop_back();
+        p->off = off;
+        p->bytes = fp;
         used_bytes += fp;
-        const Slot &S = slots[slot];
-        init_planner(*p, p->req, S.s[0], S.p[0], S.s[1], S.p[1], p->req.max_nodes, stream);
+        p->init = true;
+        p->status = OMPL_AMD_PLAN_RUNNING;
+        p->started = true;
         p->t_start = Clock::now();
-        active.emplace_back(id, slot);
+        active.push_back(id);
     }
 }
 
@@ -488,110 +659,130 @@ void Server::run()
 {
     try {
         OA_HIP(hipSetDevice(device()));
+        std::vector<Planner *> batch;
+        std::vector<int64_t> batch_ids;
+        std::vector<std::vector<double>> paths;
         for (;;) {
-            std::vector<std::pair<int64_t, int>> batch;
-            std::vector<Planner *> planners;
+            batch.clear();
+            batch_ids.clear();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
                 if (stop) return;
-                // retire cancelled / expired / full requests, then admit
+                bool notify = false;
                 for (auto q = queue.begin(); q != queue.end();) {
                     Planner *p = reqs[*q];
                     if (p->req.deadline_ms > 0.0 && ms_since(p->t_submit) > p->req.deadline_ms) {
-                        p->status = OMPL_AMD_PLAN_DEADLINE;
-                        p->t_finish = Clock::now();
+                        finish_locked(p, OMPL_AMD_PLAN_DEADLINE);
                         q = queue.erase(q);
-                        done_cv.notify_all();
+                        notify = true;
                     } else {
                         ++q;
                     }
                 }
                 for (size_t i = 0; i < active.size();) {
-                    Planner *p = reqs[active[i].first];
+                    Planner *p = reqs[active[i]];
                     const bool expired = p->req.deadline_ms > 0.0 && ms_since(p->t_submit) > p->req.deadline_ms;
-                    // a step appends at most one node to one tree
-                    const bool full = p->n[0] >= p->cap || p->n[1] >= p->cap;
-                    if (p->cancel || expired || full) {
-                        p->status = p->cancel ? OMPL_AMD_PLAN_CANCELLED
-                                              : (expired ? OMPL_AMD_PLAN_DEADLINE : OMPL_AMD_PLAN_TREE_FULL);
-                        p->t_finish = Clock::now();
-                        p->nodes = p->n[0] + p->n[1];
-                        used_bytes -= footprint(p->req);
-                        free_slots.push_back(active[i].second);
-                        active[i] = active.back();
-                        active.pop_back();
-                        done_cv.notify_all();
+                    if (p->cancel || expired) {
+                        finish_locked(p, p->cancel ? OMPL_AMD_PLAN_CANCELLED : OMPL_AMD_PLAN_DEADLINE);
+                        active.erase(active.begin() + (std::ptrdiff_t)i);
+                        notify = true;
                     } else {
                         ++i;
                     }
                 }
                 admit_locked();
-                batch = active;
-                for (auto &a : batch) planners.push_back(reqs[a.first]);
+                for (int64_t id : active) {
+                    batch.push_back(reqs[id]);
+                    batch_ids.push_back(id);
+                }
+                if (notify) done_cv.notify_all();
             }
             if (batch.empty()) continue;
             const int n = (int)batch.size();
-            for (int i = 0; i < n; ++i) next_command(*planners[i], cmd_host[i]);
-            OA_HIP(hipMemcpyAsync(cmd_dev, cmd_host, n * sizeof(GrowCmd), hipMemcpyHostToDevice, stream));
+            size_t lds = 0;
+            for (int i = 0; i < n; ++i) {
+                Planner &p = *batch[i];
+                const Env &env = p.req.env->env;
+                PlanCmd &c = cmd_host[i];
+                c.obs = env.obs;
+                c.region = pool + p.off;
+                c.ns = env.ns;
+                c.nb = env.nb;
+                c.cap = p.req.max_nodes;
+                c.max_iters = p.req.max_iters;
+                c.maxd = p.maxd;
+                c.lvs = p.req.longest_valid_segment;
+                c.factor = p.req.factor;
+                c.seed = p.req.seed;
+                c.init = p.init ? 1 : 0;
+                c.ops = cfg.check_obstacle_parts;
+                for (int k = 0; k < kDof; ++k) {
+                    c.start[k] = p.req.start[k];
+                    c.goal[k] = p.req.goal[k];
+                }
+                lds = std::max(lds, env_stage_bytes(env.ns, env.nb));
+                p.init = false;
+            }
+            OA_HIP(hipMemcpyAsync(cmd_dev, cmd_host, n * sizeof(PlanCmd), hipMemcpyHostToDevice, stream));
             {
-                ProfScope ps("serve_tick", stream);
-                TickArgs T{cmd_dev, edges_dev, res_dev, max_pos_dev, n};
-                {
-                    ProfScope p1("serve_prepare", stream);
-                    hipLaunchKernelGGL(grow_prepare_kernel, dim3(n), dim3(kPrepThreads), 0, stream, T);
-                }
-                {
-                    ProfScope p2("serve_check", stream);
-                    const int L = cfg.check_lanes_per_state, O = cfg.check_obstacle_parts;
-#define OA_CHECK(LL, OO)                                                                                            \
-    else if (L == LL && O == OO) hipLaunchKernelGGL((grow_check_kernel<LL, OO>), dim3(check_blocks), dim3(256), 0, \
-                                                    stream, T);
-                    if (false) {
-                    }
-                    OA_CHECK(8, 2) OA_CHECK(8, 4) OA_CHECK(8, 8) OA_CHECK(7, 1) OA_CHECK(7, 2) OA_CHECK(7, 4)
-                    OA_CHECK(7, 9)
-#undef OA_CHECK
-                }
-                {
-                    ProfScope p3("serve_commit", stream);
-                    hipLaunchKernelGGL(grow_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, T);
-                }
+                ProfScope ps("serve_slice", stream);
+                const SliceArgs args{cmd_dev, out_dev, budget_ticks};
+                if (cfg.block_threads == 1024)
+                    hipLaunchKernelGGL(plan_slice_kernel<1024>, dim3(n), dim3(1024), lds, stream, args);
+                else
+                    hipLaunchKernelGGL(plan_slice_kernel<512>, dim3(n), dim3(512), lds, stream, args);
                 OA_HIP(hipGetLastError());
             }
-            OA_HIP(hipMemcpyAsync(res_host, res_dev, n * sizeof(GrowRes), hipMemcpyDeviceToHost, stream));
+            OA_HIP(hipMemcpyAsync(out_host, out_dev, n * sizeof(PlanOut), hipMemcpyDeviceToHost, stream));
             OA_HIP(hipStreamSynchronize(stream));
-            ++ticks;
-            steps += n;
-            std::vector<int> finished;
-            for (int i = 0; i < n; ++i)
-                if (apply_result(*planners[i], res_host[i], stream)) finished.push_back(i);
-            if (!finished.empty()) {
-                std::lock_guard<std::mutex> lk(mu);
-                for (int i : finished) {
-                    Planner *p = planners[i];
-                    p->t_finish = Clock::now();
-                    p->nodes = p->n[0] + p->n[1];
-                    used_bytes -= footprint(p->req);
-                    for (size_t j = 0; j < active.size(); ++j)
-                        if (active[j].first == batch[i].first) {
-                            free_slots.push_back(active[j].second);
-                            active[j] = active.back();
-                            active.pop_back();
-                            break;
-                        }
-                }
-                done_cv.notify_all();
+            ++slices;
+            // solution paths of the requests that solved in this slice
+            paths.assign(n, {});
+            bool copied = false;
+            for (int i = 0; i < n; ++i) {
+                const PlanOut &o = out_host[i];
+                if (o.status != OMPL_AMD_PLAN_SOLVED || o.path_len <= 0) continue;
+                paths[i].resize((size_t)o.path_len * kDof);
+                const RegionPtrs Rp = region_ptrs(pool + batch[i]->off, batch[i]->req.max_nodes);
+                OA_HIP(hipMemcpyAsync(paths[i].data(), Rp.path, paths[i].size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, stream));
+                copied = true;
             }
+            if (copied) OA_HIP(hipStreamSynchronize(stream));
+            int64_t new_steps = 0;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                bool notify = false;
+                for (int i = 0; i < n; ++i) {
+                    Planner *p = batch[i];
+                    const PlanOut &o = out_host[i];
+                    new_steps += o.steps - p->steps_seen;
+                    phase_ticks[0] += o.t_sample;
+                    phase_ticks[1] += o.t_nearest;
+                    phase_ticks[2] += o.t_check;
+                    phase_ticks[3] += o.t_step;
+                    p->steps_seen = o.steps;
+                    p->iterations = o.iteration;
+                    p->nodes = (int64_t)o.n0 + o.n1;
+                    p->checks = o.checks;
+                    if (o.status == OMPL_AMD_PLAN_RUNNING) continue;
+                    p->path.swap(paths[i]);
+                    finish_locked(p, o.status);
+                    active.erase(std::find(active.begin(), active.end(), batch_ids[i]));
+                    notify = true;
+                }
+                if (notify) done_cv.notify_all();
+            }
+            steps += new_steps;
         }
     } catch (const std::exception &e) {
         std::lock_guard<std::mutex> lk(mu);
         fault = e.what();
-        for (auto &a : active) {
-            reqs[a.first]->status = OMPL_AMD_PLAN_FAILED;
-            reqs[a.first]->t_finish = Clock::now();
-        }
+        for (int64_t id : active) finish_locked(reqs[id], OMPL_AMD_PLAN_FAILED);
+        for (int64_t id : queue) finish_locked(reqs[id], OMPL_AMD_PLAN_FAILED);
         active.clear();
+        queue.clear();
         done_cv.notify_all();
     }
 }
@@ -601,20 +792,22 @@ static bool finished_status(int s)
     return s >= OMPL_AMD_PLAN_SOLVED;
 }
 
+// Caller holds Server::mu.
 static void fill_result(const Planner &p, ompl_amd_plan_result *r)
 {
     r->status = p.status;
-    r->iterations = p.iteration;
+    r->iterations = p.iterations;
     r->path_len = (int)(p.path.size() / kDof);
-    r->tree_nodes = p.nodes ? p.nodes : p.n[0] + p.n[1];
+    r->tree_nodes = p.nodes;
+    r->motion_checks = p.checks;
     auto ms = [](Clock::time_point a, Clock::time_point b) {
         return std::chrono::duration<double, std::milli>(b - a).count();
     };
-    const bool started = p.t_start.time_since_epoch().count() != 0;
     const bool done = finished_status(p.status);
-    r->queued_ms = started ? ms(p.t_submit, p.t_start) : ms(p.t_submit, Clock::now());
-    r->run_ms = started ? ms(p.t_start, done ? p.t_finish : Clock::now()) : 0.0;
-    r->latency_ms = ms(p.t_submit, done ? p.t_finish : Clock::now());
+    const Clock::time_point end = done ? p.t_finish : Clock::now();
+    r->queued_ms = p.started ? ms(p.t_submit, p.t_start) : ms(p.t_submit, end);
+    r->run_ms = p.started ? ms(p.t_start, end) : 0.0;
+    r->latency_ms = ms(p.t_submit, end);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,7 +820,7 @@ struct HostTree {
 };
 
 static int plain_grow(const Env &env, HostTree &t, bool start_tree, const double *target, double maxd, double lvs,
-                      unsigned factor, int &xmotion, double *xstate, ThreadResources &tr)
+                      unsigned factor, int &xmotion, double *xstate, int64_t &checks, ThreadResources &tr)
 {
     int nm = 0;
     double dmin = 0.0;
@@ -668,6 +861,7 @@ static int plain_grow(const Env &env, HostTree &t, bool start_tree, const double
         return tr.pinned[0] != 0;
     };
     bool valid;
+    ++checks;
     if (start_tree) {
         valid = check(ns, dst);
     } else {
@@ -701,45 +895,41 @@ int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *o
         OA_REQUIRE(cfg && out, OMPL_AMD_ERR_ARG, "null argument");
         OA_REQUIRE(cfg->max_active >= 1 && cfg->max_active <= 65536, OMPL_AMD_ERR_ARG, "max_active in [1, 65536]");
         OA_REQUIRE(cfg->max_nodes >= 2, OMPL_AMD_ERR_ARG, "max_nodes >= 2");
-        const int cl = cfg->check_lanes_per_state, co = cfg->check_obstacle_parts;
-        OA_REQUIRE(cl == 0 || cl == 7 || cl == 8, OMPL_AMD_ERR_ARG, "check_lanes_per_state must be 0, 7 or 8");
-        OA_REQUIRE(cl == 7 ? (co == 0 || co == 1 || co == 2 || co == 4 || co == 9)
-                           : (co == 0 || co == 2 || co == 4 || co == 8),
-                   OMPL_AMD_ERR_ARG,
-                   "check_obstacle_parts must be 0, 2, 4 or 8 (sphere slots) or 0, 1, 2, 4 or 9 (link layout)");
+        OA_REQUIRE(cfg->check_lanes_per_state == 0 || cfg->check_lanes_per_state == 7, OMPL_AMD_ERR_ARG,
+                   "check_lanes_per_state must be 0 or 7 (the link layout)");
+        const int co = cfg->check_obstacle_parts;
+        OA_REQUIRE(co == 0 || co == 1 || co == 2 || co == 4 || co == 9, OMPL_AMD_ERR_ARG,
+                   "check_obstacle_parts must be 0 (by motion length), 1, 2, 4 or 9");
+        OA_REQUIRE(cfg->slice_us >= 0, OMPL_AMD_ERR_ARG, "slice_us must be >= 0");
+        OA_REQUIRE(cfg->block_threads == 0 || cfg->block_threads == 512 || cfg->block_threads == 1024,
+                   OMPL_AMD_ERR_ARG, "block_threads must be 0, 512 or 1024");
         ensure_init();
         auto *h = new ompl_amd_server_s();
         Server &S = h->srv;
         S.cfg = *cfg;
-        // defaults: the link layout with two states per wave, the fastest tick check measured
-        if (S.cfg.check_lanes_per_state == 0) S.cfg.check_lanes_per_state = 7;
-        if (S.cfg.check_obstacle_parts == 0) S.cfg.check_obstacle_parts = S.cfg.check_lanes_per_state == 7 ? 4 : 8;
+        S.cfg.check_lanes_per_state = 7;
+        if (S.cfg.slice_us == 0) S.cfg.slice_us = 200;
+        if (S.cfg.block_threads == 0) S.cfg.block_threads = 512; // 1,024 spills registers (measured slower)
+        int rate_khz = 0;
+        OA_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device()));
+        OA_REQUIRE(rate_khz > 0, OMPL_AMD_ERR_HIP, "device wall clock rate unavailable");
+        S.clock_khz = rate_khz;
+        S.budget_ticks = (int64_t)S.cfg.slice_us * rate_khz / 1000;
+        if (S.budget_ticks < 1) S.budget_ticks = 1;
         OA_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
-        const size_t per_tree = (size_t)cfg->max_nodes * (kDof * sizeof(double) + sizeof(int32_t));
-        const size_t total = per_tree * 2 * (size_t)cfg->max_active;
-        hipError_t e = hipMalloc(&S.pool, total);
-        if (e != hipSuccess) {
-            delete h;
-            fail(OMPL_AMD_ERR_NOMEM, "server tree pool of %zu bytes: %s", total, hipGetErrorString(e));
-        }
-        for (int i = 0; i < cfg->max_active; ++i) {
-            Slot sl;
-            for (int t = 0; t < 2; ++t) {
-                char *base = S.pool + per_tree * (2 * (size_t)i + t);
-                sl.s[t] = (double *)base;
-                sl.p[t] = (int32_t *)(base + (size_t)cfg->max_nodes * kDof * sizeof(double));
+        if (cfg->hbm_budget_bytes > 0) {
+            hipError_t e = hipMalloc(&S.pool, cfg->hbm_budget_bytes);
+            if (e != hipSuccess) {
+                (void)hipStreamDestroy(S.stream);
+                delete h;
+                fail(OMPL_AMD_ERR_NOMEM, "server pool of %zu bytes: %s", cfg->hbm_budget_bytes, hipGetErrorString(e));
             }
-            S.slots.push_back(sl);
-            S.free_slots.push_back(cfg->max_active - 1 - i);
         }
-        OA_HIP(hipHostMalloc((void **)&S.cmd_host, sizeof(GrowCmd) * cfg->max_active, hipHostMallocDefault));
-        OA_HIP(hipHostMalloc((void **)&S.res_host, sizeof(GrowRes) * cfg->max_active, hipHostMallocDefault));
-        OA_HIP(hipMalloc(&S.cmd_dev, sizeof(GrowCmd) * cfg->max_active));
-        OA_HIP(hipMalloc(&S.res_dev, sizeof(GrowRes) * cfg->max_active));
-        OA_HIP(hipMalloc(&S.edges_dev, sizeof(GrowEdge) * cfg->max_active));
-        OA_HIP(hipMalloc(&S.max_pos_dev, sizeof(int32_t)));
-        OA_HIP(hipMemset(S.max_pos_dev, 0, sizeof(int32_t)));
-        S.check_blocks = (unsigned)num_cus() * 2; // 8 waves per CU
+        S.alloc.reset(cfg->hbm_budget_bytes);
+        OA_HIP(hipHostMalloc((void **)&S.cmd_host, sizeof(PlanCmd) * cfg->max_active, hipHostMallocDefault));
+        OA_HIP(hipHostMalloc((void **)&S.out_host, sizeof(PlanOut) * cfg->max_active, hipHostMallocDefault));
+        OA_HIP(hipMalloc(&S.cmd_dev, sizeof(PlanCmd) * cfg->max_active));
+        OA_HIP(hipMalloc(&S.out_dev, sizeof(PlanOut) * cfg->max_active));
         S.th = std::thread([&S] { S.run(); });
         *out = h;
     });
@@ -758,13 +948,12 @@ int ompl_amd_server_destroy(ompl_amd_server h)
         if (S.th.joinable()) S.th.join();
         for (auto &kv : S.reqs) delete kv.second;
         // teardown: release everything even if one release reports an error
-        (void)hipFree(S.pool);
+        (void)hipStreamSynchronize(S.stream);
+        if (S.pool) (void)hipFree(S.pool);
         (void)hipFree(S.cmd_dev);
-        (void)hipFree(S.res_dev);
-        (void)hipFree(S.edges_dev);
-        (void)hipFree(S.max_pos_dev);
+        (void)hipFree(S.out_dev);
         (void)hipHostFree(S.cmd_host);
-        (void)hipHostFree(S.res_host);
+        (void)hipHostFree(S.out_host);
         (void)hipStreamDestroy(S.stream);
         delete h;
     });
@@ -777,16 +966,26 @@ int ompl_amd_server_submit(ompl_amd_server h, const ompl_amd_plan_request *req, 
         OA_REQUIRE(req->env, OMPL_AMD_ERR_ARG, "request without environment");
         OA_REQUIRE(req->max_iters >= 1 && req->longest_valid_segment > 0.0 && req->factor >= 1, OMPL_AMD_ERR_ARG,
                    "bad request parameters");
+        const Env &env = req->env->env;
+        OA_REQUIRE(env_stage_bytes(env.ns, env.nb) <= kPlanMaxEnvBytes, OMPL_AMD_ERR_ARG,
+                   "environment too large for the server: %d spheres and %d boxes stage %zu bytes (limit %zu)",
+                   env.ns, env.nb, env_stage_bytes(env.ns, env.nb), kPlanMaxEnvBytes);
         Server &S = h->srv;
         auto *p = new Planner();
         p->req = *req;
         if (p->req.max_nodes <= 0 || p->req.max_nodes > S.cfg.max_nodes) p->req.max_nodes = S.cfg.max_nodes;
+        if (p->req.max_nodes < 2) p->req.max_nodes = 2;
+        p->maxd = req->range > 0.0 ? req->range : 0.2 * 2.0 * kPi * std::sqrt(7.0);
+        p->nodes = 2;
         p->t_submit = Clock::now();
         std::lock_guard<std::mutex> lk(S.mu);
-        OA_REQUIRE(S.fault.empty(), OMPL_AMD_ERR_HIP, "server stopped after a fault: %s", S.fault.c_str());
+        if (!S.fault.empty()) {
+            delete p;
+            fail(OMPL_AMD_ERR_HIP, "server stopped after a fault: %s", S.fault.c_str());
+        }
         const int64_t rid = S.next_id++;
         S.reqs[rid] = p;
-        if (footprint(p->req) > S.cfg.hbm_budget_bytes) {
+        if (footprint(p->req.max_nodes) > S.cfg.hbm_budget_bytes) {
             p->status = OMPL_AMD_PLAN_REJECTED;
             p->t_finish = p->t_submit;
         } else {
@@ -835,7 +1034,8 @@ int ompl_amd_server_path(ompl_amd_server h, int64_t id, double *out, int cap, in
         OA_REQUIRE(it != S.reqs.end(), OMPL_AMD_ERR_ARG, "unknown request %lld", (long long)id);
         const auto &path = it->second->path;
         *n = (int)(path.size() / kDof);
-        if (out) std::copy(path.begin(), path.begin() + std::min<size_t>(path.size(), (size_t)cap * kDof), out);
+        if (out && cap > 0)
+            std::copy(path.begin(), path.begin() + std::min<size_t>(path.size(), (size_t)cap * kDof), out);
     });
 }
 
@@ -852,8 +1052,7 @@ int ompl_amd_server_cancel(ompl_amd_server h, int64_t id)
         if (p->status == OMPL_AMD_PLAN_QUEUED) {
             auto q = std::find(S.queue.begin(), S.queue.end(), id);
             if (q != S.queue.end()) S.queue.erase(q);
-            p->status = OMPL_AMD_PLAN_CANCELLED;
-            p->t_finish = Clock::now();
+            S.finish_locked(p, OMPL_AMD_PLAN_CANCELLED);
             S.done_cv.notify_all();
         }
         S.cv.notify_all();
@@ -881,13 +1080,28 @@ int ompl_amd_server_stats(ompl_amd_server h, int64_t out[6])
         OA_REQUIRE(h && out, OMPL_AMD_ERR_ARG, "null argument");
         Server &S = h->srv;
         std::lock_guard<std::mutex> lk(S.mu);
-        out[0] = S.ticks.load();
+        out[0] = S.slices.load();
         out[1] = S.steps.load();
         out[2] = (int64_t)S.active.size();
         out[3] = (int64_t)S.queue.size();
         out[4] = (int64_t)S.used_bytes;
         out[5] = (int64_t)S.cfg.hbm_budget_bytes;
     });
+}
+
+int ompl_amd_server_phase_us(ompl_amd_server h, double out[4])
+{
+    return guarded([&] {
+        OA_REQUIRE(h && out, OMPL_AMD_ERR_ARG, "null argument");
+        Server &S = h->srv;
+        std::lock_guard<std::mutex> lk(S.mu);
+        for (int i = 0; i < 4; ++i) out[i] = 1e3 * (double)S.phase_ticks[i] / (double)S.clock_khz;
+    });
+}
+
+size_t ompl_amd_server_footprint(int max_nodes)
+{
+    return max_nodes >= 2 ? footprint(max_nodes) : 0;
 }
 
 int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *res, double *path, int cap,
@@ -908,6 +1122,7 @@ int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *
         tg.parent.push_back(-1);
         bool start_tree = true, solved = false;
         int it = 0;
+        int64_t checks = 0;
         double rstate[kDof], xstate[kDof];
         std::vector<double> out;
         for (it = 0; it < req->max_iters && !solved; ++it) {
@@ -919,18 +1134,18 @@ int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *
             for (int k = 0; k < kDof; ++k) rstate[k] = (kPi - -kPi) * uni(gen) + -kPi;
             int xm = -1;
             int gs = plain_grow(env, tree, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, xm,
-                                xstate, tr);
+                                xstate, checks, tr);
             if (gs == kTrapped) continue;
             const int added = xm;
             if (gs != kReached)
                 for (int k = 0; k < kDof; ++k) rstate[k] = xstate[k];
             tgi_start = start_tree;
             int gsc = plain_grow(env, other, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, xm,
-                                 xstate, tr);
+                                 xstate, checks, tr);
             if (gsc == kTrapped) tgi_start = !tgi_start;
             while (gsc == kAdvanced)
                 gsc = plain_grow(env, other, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, xm,
-                                 xstate, tr);
+                                 xstate, checks, tr);
             if (gsc != kReached) continue;
             int sm = tgi_start ? xm : added, gm = tgi_start ? added : xm;
             if (ts.parent[sm] >= 0) sm = ts.parent[sm];
@@ -950,6 +1165,7 @@ int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *
         res->iterations = it;
         res->path_len = (int)(out.size() / kDof);
         res->tree_nodes = ts.size() + tg.size();
+        res->motion_checks = checks;
         res->queued_ms = 0.0;
         res->run_ms = ms;
         res->latency_ms = ms;

@@ -1,5 +1,6 @@
 """Python client of the planning server (continuous batching of RRT-Connect
-requests, include/ompl_amd.h) and of the plain one-launch-per-call planner."""
+requests with device-resident solve loops, include/ompl_amd.h) and of the
+plain one-launch-per-call planner."""
 from __future__ import annotations
 
 import ctypes as C
@@ -16,8 +17,8 @@ STATUS = {0: "queued", 1: "running", 2: "solved", 3: "failed", 4: "deadline", 5:
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("max_nodes", C.c_int),
-                ("check_obstacle_parts", C.c_int),
-                ("check_lanes_per_state", C.c_int)]
+                ("check_obstacle_parts", C.c_int), ("check_lanes_per_state", C.c_int), ("slice_us", C.c_int),
+                ("block_threads", C.c_int)]
 
 
 class PlanRequest(C.Structure):
@@ -44,6 +45,7 @@ class PlanResult(C.Structure):
         ("queued_ms", C.c_double),
         ("run_ms", C.c_double),
         ("latency_ms", C.c_double),
+        ("motion_checks", C.c_int64),
     ]
 
     def as_dict(self):
@@ -68,10 +70,18 @@ def make_request(env, start, goal, seed, lvs, max_iters=5000, rng=0.0, factor=1,
     return r
 
 
+def footprint(max_nodes: int) -> int:
+    """Bytes of one request's device region for max_nodes nodes per tree."""
+    from ._lib import load
+
+    return int(load().ompl_amd_server_footprint(int(max_nodes)))
+
+
 class Server:
     def __init__(self, hbm_budget_bytes=1 << 30, max_active=256, max_nodes=8192, check_obstacle_parts=0,
-                 check_lanes_per_state=0):
-        cfg = ServerConfig(hbm_budget_bytes, max_active, max_nodes, check_obstacle_parts, check_lanes_per_state)
+                 check_lanes_per_state=0, slice_us=0, block_threads=0):
+        cfg = ServerConfig(hbm_budget_bytes, max_active, max_nodes, check_obstacle_parts, check_lanes_per_state,
+                           slice_us, block_threads)
         h = C.c_void_p()
         call("ompl_amd_server_create", C.byref(cfg), C.byref(h))
         self.handle = h
@@ -107,7 +117,13 @@ class Server:
     def stats(self) -> dict:
         out = (C.c_int64 * 6)()
         call("ompl_amd_server_stats", self.handle, out)
-        return dict(zip(("ticks", "steps", "active", "queued", "used_bytes", "budget_bytes"), list(out)))
+        return dict(zip(("slices", "steps", "active", "queued", "used_bytes", "budget_bytes"), list(out)))
+
+    def phase_us(self) -> dict:
+        """Device time of the growTree steps so far (summed over requests), us."""
+        out = (C.c_double * 4)()
+        call("ompl_amd_server_phase_us", self.handle, out)
+        return dict(zip(("sample", "nearest", "check", "step"), list(out)))
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:

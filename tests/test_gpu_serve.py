@@ -3,6 +3,7 @@ per-call planner: plans equal the CPU RRT-Connect restatement, each request's
 result is the same alone and in the mix, and the failure paths (deadline,
 admission, cancellation, tree capacity) are forced from the host."""
 import math
+import time
 
 import numpy as np
 import pytest
@@ -56,29 +57,30 @@ def test_plain_planner_matches_oracle(gpu, oracle, scene):
         assert np.array_equal(path, opath)
 
 
-# tick check shapes (lanes per state, obstacle parts): defaults, link layout, sphere slots
-@pytest.mark.parametrize("lanes,parts", [(0, 0), (7, 9), (7, 2), (8, 2)])
-def test_served_equals_oracle_alone_and_in_mix(gpu, oracle, scene, lanes, parts):
+# motion-check obstacle partitions (0 = by motion length) and workgroup sizes
+@pytest.mark.parametrize("parts,threads", [(0, 1024), (9, 1024), (2, 1024), (1, 512), (4, 512)])
+def test_served_equals_oracle_alone_and_in_mix(gpu, oracle, scene, parts, threads):
     from ompl_amd.serve import SOLVED, Server
 
     env, oenv, pairs = scene
     reqs = reqs_for(gpu, env, pairs)
-    print(f"served check shape ({lanes}, {parts})", flush=True)
+    print(f"served check partitions {parts}, {threads} threads", flush=True)
     srv = Server(hbm_budget_bytes=1 << 30, max_active=64, max_nodes=8192, check_obstacle_parts=parts,
-                 check_lanes_per_state=lanes)
+                 block_threads=threads)
     alone = []
     for r in reqs:  # one at a time
         rid = srv.submit(r)
         res = srv.wait(rid, 60000)
-        alone.append((res.status, res.iterations, res.tree_nodes, srv.path(rid)))
+        alone.append((res.status, res.iterations, res.tree_nodes, srv.path(rid), res.motion_checks))
         srv.release(rid)
     ids = [srv.submit(r) for r in reqs]  # all together
     mix = []
     for rid in ids:
         res = srv.wait(rid, 60000)
-        mix.append((res.status, res.iterations, res.tree_nodes, srv.path(rid)))
+        mix.append((res.status, res.iterations, res.tree_nodes, srv.path(rid), res.motion_checks))
     for i, (a, m) in enumerate(zip(alone, mix)):
         assert a[0] == m[0] and a[1] == m[1] and a[2] == m[2] and np.array_equal(a[3], m[3]), i
+        assert a[4] == m[4], i
         ok, it, opath, nodes = oracle.rrt_connect(oenv, pairs[i][0], pairs[i][1], 100 + i, MAXD, lvs_config(),
                                                   max_iters=3000)
         assert (a[0] == SOLVED) == ok and a[1] == it and a[2] == nodes
@@ -86,6 +88,23 @@ def test_served_equals_oracle_alone_and_in_mix(gpu, oracle, scene, lanes, parts)
     st = srv.stats()
     assert st["active"] == 0 and st["queued"] == 0 and st["used_bytes"] == 0
     assert sum(m[0] == SOLVED for m in mix) >= 5
+    srv.close()
+
+
+def test_served_motion_checks_equal_plain(gpu, scene):
+    """The served and plain planners make the same growTree steps, so they
+    check the same number of motions."""
+    from ompl_amd.serve import Server, plan_plain
+
+    env, _, pairs = scene
+    reqs = reqs_for(gpu, env, pairs[:4])
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=8, max_nodes=8192)
+    ids = [srv.submit(r) for r in reqs]
+    for r, rid in zip(reqs, ids):
+        res = srv.wait(rid, 60000)
+        pres, ppath = plan_plain(r)
+        assert res.motion_checks == pres.motion_checks > 0
+        assert res.iterations == pres.iterations and np.array_equal(srv.path(rid), ppath)
     srv.close()
 
 
@@ -135,7 +154,70 @@ def test_server_rejects_unbuilt_check_shapes(gpu):
     from ompl_amd._lib import OmplAmdError
     from ompl_amd.serve import Server
 
-    for lanes, parts in ((8, 9), (8, 1), (7, 8), (6, 0), (7, 3)):
+    for lanes, parts in ((8, 2), (8, 8), (7, 8), (6, 0), (7, 3)):
         with pytest.raises(OmplAmdError):
             Server(hbm_budget_bytes=1 << 30, max_active=2, max_nodes=64, check_obstacle_parts=parts,
                    check_lanes_per_state=lanes)
+    with pytest.raises(OmplAmdError):
+        Server(hbm_budget_bytes=1 << 30, max_active=2, max_nodes=64, block_threads=256)
+
+
+def test_admission_bound_by_hbm_budget(gpu, scene):
+    """With a pool of 2.5 request regions at most two requests run at once: the
+    third waits in the queue for memory, not for a slot, and its plan is the
+    one it makes alone."""
+    from ompl_amd.serve import SOLVED, Server, footprint, make_request
+
+    env, _, pairs = scene
+    lvs = lvs_config()
+    fp = footprint(2048)
+    reqs = [make_request(env, pairs[i][0], pairs[i][1], 500 + i, lvs, max_iters=3000) for i in range(6)]
+    alone = Server(hbm_budget_bytes=1 << 28, max_active=8, max_nodes=2048)
+    ref = []
+    for r in reqs:
+        rid = alone.submit(r)
+        res = alone.wait(rid, 60000)
+        ref.append((res.status, res.iterations, alone.path(rid)))
+    alone.close()
+    srv = Server(hbm_budget_bytes=int(2.5 * fp), max_active=8, max_nodes=2048)
+    ids = [srv.submit(r) for r in reqs]
+    peak = 0
+    while True:
+        st = srv.stats()
+        peak = max(peak, st["active"])
+        assert st["used_bytes"] <= st["budget_bytes"]
+        if st["active"] == 0 and st["queued"] == 0:
+            break
+        time.sleep(2e-4)
+    res = [srv.wait(rid, 60000) for rid in ids]
+    assert peak <= 2
+    assert any(r.queued_ms > 0.0 for r in res[2:])
+    for r, rid, (st0, it0, p0) in zip(res, ids, ref):
+        assert r.status == st0 and r.iterations == it0 and np.array_equal(srv.path(rid), p0)
+    assert sum(r.status == SOLVED for r in res) >= 3
+    srv.close()
+
+
+def test_release_right_after_finish_is_safe(gpu, scene):
+    """Requests released as soon as they report a final status: the server
+    publishes status, path and timing together, so nothing of a released
+    request is touched afterwards (repeated, with more requests than slots)."""
+    from ompl_amd.serve import SOLVED, Server, make_request
+
+    env, _, pairs = scene
+    lvs = lvs_config()
+    srv = Server(hbm_budget_bytes=1 << 28, max_active=3, max_nodes=2048)
+    for rep in range(3):
+        ids = [srv.submit(make_request(env, a, b, 900 + i, lvs, max_iters=3000)) for i, (a, b) in enumerate(pairs)]
+        left = set(ids)
+        while left:
+            for rid in list(left):
+                r = srv.poll(rid)
+                if r.status >= SOLVED:
+                    assert r.latency_ms >= r.run_ms >= 0.0 and r.latency_ms >= r.queued_ms
+                    srv.release(rid)
+                    left.discard(rid)
+            time.sleep(1e-4)
+    st = srv.stats()
+    assert st["active"] == 0 and st["used_bytes"] == 0
+    srv.close()
