@@ -35,9 +35,12 @@ std::vector<int> nearest_first(int n, const std::function<double(int)> &gap)
     return order;
 }
 
+} // namespace
+
 // Batched DiscreteMotionValidator::checkMotion(s1, s2) over device arrays.
-void check_motions_t(const Tuning &t, const Env &env, const double *a, const double *b, int64_t n, double lvs,
-                     unsigned factor, uint8_t *out, int32_t *checked, hipStream_t s, Workspace *own = nullptr)
+void ompl_amd::check_motions_tuned(const Tuning &t, const Env &env, const double *a, const double *b, int64_t n,
+                                   double lvs, unsigned factor, uint8_t *out, int32_t *checked, hipStream_t s,
+                                   Workspace *own)
 {
     OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative edge count");
     if (n == 0) return;
@@ -68,7 +71,8 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         // the other set, which the next batch on this workspace uses.
         const bool shared = t.motion_strategy == 4;
         const bool eshared = t.validity_shared_fk != 0;
-        Workspace &w = own ? *own : workspace(s, (size_t)n);
+        std::unique_lock<std::mutex> lk; // held until this batch's launches are enqueued
+        Workspace &w = own ? *own : workspace(s, (size_t)n, lk);
         const bool results = exact || own != nullptr;
         int32_t *cur = w.counters + 8 + 2 * w.parity, *other = w.counters + 8 + 2 * (1 - w.parity);
         A.list = w.list;
@@ -86,7 +90,8 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
         A.start = 1;
         launch_motion_fused(A, exact, LE, OE, LW, OW, t.motion_block, s);
     } else if (t.motion_strategy == 1) {
-        Workspace &w = own ? *own : workspace(s, (size_t)n);
+        std::unique_lock<std::mutex> lk; // held until this batch's launches are enqueued
+        Workspace &w = own ? *own : workspace(s, (size_t)n, lk);
         A.list = w.list;
         A.list_count = w.counters;
         A.queue = w.counters + 1;
@@ -100,10 +105,12 @@ void check_motions_t(const Tuning &t, const Env &env, const double *a, const dou
     }
 }
 
+namespace {
+
 void check_motions(const Env &env, const double *a, const double *b, int64_t n, double lvs, unsigned factor,
                    uint8_t *out, int32_t *checked, hipStream_t s)
 {
-    check_motions_t(get_tuning(), env, a, b, n, lvs, factor, out, checked, s);
+    check_motions_tuned(get_tuning(), env, a, b, n, lvs, factor, out, checked, s);
 }
 
 // Candidate launch shapes of the motion autotuner.
@@ -435,9 +442,9 @@ extern "C" int ompl_amd_autotune_motions(ompl_amd_env e, const double *a, const 
         Tuning win = base;
         float win_ms = INFINITY;
         for (const Tuning &t : motion_candidates()) {
-            for (int w = 0; w < 2; ++w) check_motions_t(t, env, a, b, n, lvs, factor, out, nullptr, s);
+            for (int w = 0; w < 2; ++w) check_motions_tuned(t, env, a, b, n, lvs, factor, out, nullptr, s);
             OA_HIP(hipEventRecord(e0, s));
-            for (int i = 0; i < reps; ++i) check_motions_t(t, env, a, b, n, lvs, factor, out, nullptr, s);
+            for (int i = 0; i < reps; ++i) check_motions_tuned(t, env, a, b, n, lvs, factor, out, nullptr, s);
             OA_HIP(hipEventRecord(e1, s));
             OA_HIP(hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -483,7 +490,7 @@ int ompl_amd_motion_graph_create(ompl_amd_env e, const double *a, const double *
             const Tuning t = get_tuning();
             OA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
             try {
-                check_motions_t(t, env, a, b, n, lvs, factor, out, checked, cs, &g->ws);
+                check_motions_tuned(t, env, a, b, n, lvs, factor, out, checked, cs, &g->ws);
             } catch (...) {
                 hipGraph_t dropped = nullptr;
                 hipStreamEndCapture(cs, &dropped);

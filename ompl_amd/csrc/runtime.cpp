@@ -100,16 +100,22 @@ hipStream_t default_stream()
     return g_stream;
 }
 
-Workspace &workspace(hipStream_t s, size_t list_entries)
+Workspace &workspace(hipStream_t s, size_t list_entries, std::unique_lock<std::mutex> &lock)
 {
-    std::lock_guard<std::mutex> lk(g_ws_mutex);
-    Workspace *&w = g_ws[s];
-    if (!w) {
-        w = new Workspace();
-        OA_HIP(hipMalloc(&w->counters, 64 * sizeof(int32_t)));
-        OA_HIP(hipMemset(w->counters, 0, 64 * sizeof(int32_t)));
-        OA_HIP(hipDeviceSynchronize());
+    Workspace *w = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mutex);
+        Workspace *&slot = g_ws[s];
+        if (!slot) {
+            slot = new Workspace();
+            OA_HIP(hipMalloc(&slot->counters, 64 * sizeof(int32_t)));
+            OA_HIP(hipMemset(slot->counters, 0, 64 * sizeof(int32_t)));
+            OA_HIP(hipDeviceSynchronize());
+        }
+        w = slot;
     }
+    // workspaces are never freed: the pointer stays valid without g_ws_mutex
+    lock = std::unique_lock<std::mutex>(w->mu);
     if (w->list_cap < list_entries) {
         // the stream may still use the old arrays
         OA_HIP(hipStreamSynchronize(s));

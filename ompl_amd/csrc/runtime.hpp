@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 
 #include "checker.hpp"
 #include "common.hpp"
@@ -27,8 +28,15 @@ struct Workspace {
     // walk clears the other set and the next batch uses that one.
     int32_t *counters = nullptr;
     int parity = 0;
+    // Held by a batch from choosing its counter set to enqueuing its launches
+    // (and around resizing), so host threads that share a stream cannot pick
+    // the same set or see arrays being replaced.
+    std::mutex mu;
 };
-Workspace &workspace(hipStream_t s, size_t list_entries);
+// The stream's workspace with room for list_entries survivors.  The caller
+// must hold the returned workspace's mu while it uses the workspace (lock
+// passes that lock back, already taken, before any resize).
+Workspace &workspace(hipStream_t s, size_t list_entries, std::unique_lock<std::mutex> &lock);
 
 // Per-thread stream and pinned result slot of the blocking single-call entry
 // points (OMPL checkers must be thread-safe: base/StateValidityChecker.h:87-89).
@@ -58,6 +66,12 @@ struct Tuning {
 };
 
 Tuning get_tuning();
+
+struct Env;
+// Batched DiscreteMotionValidator::checkMotion(s1, s2) over device arrays with
+// launch shapes t; `own` supplies the survivor workspace (null: the stream's).
+void check_motions_tuned(const Tuning &t, const Env &env, const double *a, const double *b, int64_t n, double lvs,
+                         unsigned factor, uint8_t *out, int32_t *checked, hipStream_t s, Workspace *own = nullptr);
 void set_tuning(const Tuning &t);
 ompl_amd_tuning tuning_to_c(const Tuning &x);
 Tuning tuning_from_c(const ompl_amd_tuning &t);

@@ -212,7 +212,7 @@ __device__ __forceinline__ int block_nearest(const double *__restrict__ T, int c
     return bi;
 }
 
-// The motion's positions by the whole block, in rounds of NT/64 waves x G
+// Positions [first, npos) of the motion by the whole block, in rounds of NT/64 waves x G
 // states: with `extra`, position 0 is isValid(dst) and position p >= 1 the
 // DiscreteMotionValidator position p-1 of (a, b) (0 = b, then the bisection
 // order); without, position p is the validator's position p.  Returns false
@@ -222,14 +222,14 @@ __device__ __forceinline__ int block_nearest(const double *__restrict__ T, int c
 // `stop` is an LDS flag, zero on entry, that lets the other waves of a round
 // leave once one has found an invalid position.
 template <int NT, int OPS>
-__device__ __forceinline__ bool block_check(const double *a, const double *b, const double *dst, int nd, int npos,
-                                            bool extra, const float4 *env, int ns, int nb, int *stop)
+__device__ __forceinline__ bool block_check(const double *a, const double *b, const double *dst, int nd, int first,
+                                            int npos, bool extra, const float4 *env, int ns, int nb, int *stop)
 {
     using Shape = GroupShape<7, OPS>;
     constexpr int G = Shape::STATES;
     const int wave = threadIdx.x >> 6;
     const Shape sh(lane_id());
-    for (int base = 0; base < npos; base += NT / 64 * G) {
+    for (int base = first; base < npos; base += NT / 64 * G) {
         const int p = base + wave * G + sh.group;
         const bool live = p < npos;
         float q[kDof];
@@ -419,15 +419,27 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
             bool valid = false;
             if (nd <= kMaxSegments) { // longer motions are reported invalid
                 const int npos = 1 + (nd >= 2 ? nd - 1 : 0) + (from_start ? 0 : 1);
-                const int ops = C.ops ? C.ops : auto_ops<NT / 64>(npos);
-                if (ops == 9)
-                    valid = block_check<NT, 9>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
-                else if (ops == 4)
-                    valid = block_check<NT, 4>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
-                else if (ops == 2)
-                    valid = block_check<NT, 2>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
-                else
-                    valid = block_check<NT, 1>(s_a, s_b, s_x, nd, npos, !from_start, s_env, C.ns, C.nb, &s_hit);
+                // First round: one position per wave, the obstacle list split nine
+                // ways (a failing motion -- most are -- usually fails at the new
+                // state or an early bisection point, found within a pass or two);
+                // then the rest in rounds shaped by the configured or auto split.
+                const int nfirst = min(npos, NT / 64);
+                valid = block_check<NT, 9>(s_a, s_b, s_x, nd, 0, nfirst, !from_start, s_env, C.ns, C.nb, &s_hit);
+                if (valid && nfirst < npos) {
+                    const int ops = C.ops ? C.ops : auto_ops<NT / 64>(npos - nfirst);
+                    if (ops == 9)
+                        valid = block_check<NT, 9>(s_a, s_b, s_x, nd, nfirst, npos, !from_start, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                    else if (ops == 4)
+                        valid = block_check<NT, 4>(s_a, s_b, s_x, nd, nfirst, npos, !from_start, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                    else if (ops == 2)
+                        valid = block_check<NT, 2>(s_a, s_b, s_x, nd, nfirst, npos, !from_start, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                    else
+                        valid = block_check<NT, 1>(s_a, s_b, s_x, nd, nfirst, npos, !from_start, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                }
             }
             if (valid) {
                 if (tid < kDof) T[(int64_t)tid * C.cap + nt] = s_x[tid];

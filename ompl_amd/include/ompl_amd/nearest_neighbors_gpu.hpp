@@ -139,7 +139,14 @@ public:
     {
         out.assign(queries.size(), {});
         if (queries.empty() || k == 0) return;
-        const std::size_t kk = k > 64 ? 64 : k;
+        // nearestK returns min(k, size()) neighbours; the device search keeps at
+        // most kMaxK per query, so a larger request is an error, not a silent cap
+        const std::size_t n = size();
+        const std::size_t kk = k < n ? k : n;
+        if (kk == 0) return;
+        if (kk > kMaxK)
+            throw ompl::Exception("NearestNeighborsGPU::nearestK: k = " + std::to_string(k) + " exceeds the limit of " +
+                                  std::to_string(kMaxK) + " neighbours per query");
         std::vector<float> q(queries.size() * dim_);
         for (std::size_t i = 0; i < queries.size(); ++i) coords_(queries[i], q.data() + i * dim_);
         std::vector<int32_t> idx(queries.size() * kk);
@@ -174,6 +181,9 @@ public:
     {
         return nn_;
     }
+
+    // Largest k of one device query (ompl_amd_nn_knn).
+    static constexpr std::size_t kMaxK = 64;
 
 private:
     int metric_;

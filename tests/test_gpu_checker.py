@@ -476,3 +476,45 @@ def test_two_streams_concurrently(gpu, oracle):
     for i in range(4):
         assert np.array_equal(o1[i].cpu().numpy(), r1) and np.array_equal(o2[i].cpu().numpy(), r2), i
     gpu.set_tuning(**DEFAULT)
+
+
+def test_two_threads_share_one_stream(gpu, oracle):
+    """Boolean-only batches issued by two host threads on one stream share that
+    stream's survivor workspace: each batch takes its counter set and enqueues
+    its launches under the workspace's lock, so every batch gives the oracle's
+    results (without the lock two batches could pick the same set)."""
+    import threading
+
+    import torch
+
+    gpu.set_tuning(motion_strategy=3, validity_lanes_per_state=7, validity_obstacle_parts=4,
+                   motion_lanes_per_state=7, motion_obstacle_parts=9)
+    env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
+    lvs = lvs_config()
+    cases = [motion_case(gpu, 4, 8192), motion_case(gpu, 8, 3000)]
+    refs = [oracle.check_motions(oenv, a, b, lvs)[0] for a, b in cases]
+    dev = [(to_dev(a.T.copy()), to_dev(b.T.copy())) for a, b in cases]
+    shared = torch.cuda.Stream()
+    outs = {0: [], 1: []}
+    errors = []
+
+    def work(t):
+        try:
+            with torch.cuda.stream(shared):
+                for i in range(12):
+                    A, B = dev[(i + t) % 2]
+                    outs[t].append(((i + t) % 2, gpu.check_motions(env, A, B, lvs)))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in (0, 1)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for t in (0, 1):
+        for i, (c, o) in enumerate(outs[t]):
+            assert np.array_equal(o.cpu().numpy(), refs[c]), (t, i)
+    gpu.set_tuning(**DEFAULT)

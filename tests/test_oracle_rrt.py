@@ -1,0 +1,267 @@
+"""Pins the oracle's RRT-Connect (orc_rrt_connect, restating
+geometric/planners/rrt/src/RRTConnect.cpp:116-196 growTree and :245-330 solve)
+and RNG::quaternion (util/src/RandomNumbers.cpp:266-279) on the CPU.
+
+RRT-Connect is pinned three ways, each catching a different slip:
+  * free space, closed form: the plan solves in iteration 1 and its states
+    follow from the first sample (the libstdc++ golden draws of mt19937(7))
+    and straight-line steps of `range` from the goal -- pins the tree swap,
+    steering, the connect loop's direction and the path splice;
+  * obstacle scenes, invariants: a solved path starts and ends exactly at
+    start and goal, each step is at most `range` long and passes the pinned
+    motion check in the direction the planner checked it (start to goal);
+  * an independent replay written from the reference's structure (Motion
+    objects with parent links, TreeGrowingInfo), on the pinned sampler,
+    motion check and isValid: identical iteration counts, tree sizes and
+    paths, on scenes where both TRAPPED outcomes (sampling step and first
+    connect step) occur.
+The quaternion is pinned by known answers: Shoemake's map (Graphics Gems III)
+applied to the golden libstdc++ draws, component order x, y, z, w."""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+PI = math.pi
+MAXD = 0.2 * 2 * PI * math.sqrt(7)  # SelfConfig default range: 0.2 * maximum extent of [-pi, pi]^7
+
+
+def lvs_config():
+    ext = 2 * PI * math.sqrt(7)
+    return ext * (0.01 / ext)
+
+
+def golden():
+    return json.loads((GOLDEN / "stdlib_random.json").read_text())
+
+
+# ---------------------------------------------------------------- quaternion
+def shoemake(x0, u1, u2):
+    r1, r2 = math.sqrt(1.0 - x0), math.sqrt(x0)
+    t1, t2 = 2.0 * PI * u1, 2.0 * PI * u2
+    return np.array([math.sin(t1) * r1, math.cos(t1) * r1, math.sin(t2) * r2, math.cos(t2) * r2])
+
+
+@pytest.mark.parametrize("seed", ["1", "42", "1000000000"])
+def test_quaternion_known_answers(oracle, seed):
+    draws = golden()["uniform01"][seed]
+    g = oracle.Mt19937(int(seed))
+    for j in range(len(draws) // 3):
+        want = shoemake(*draws[3 * j:3 * j + 3])
+        got = g.quaternion()
+        # libm sin/cos on both sides; the map itself is exact arithmetic
+        assert np.allclose(got, want, rtol=0, atol=4e-16), (j, got, want)
+
+
+def test_quaternion_swaps_are_detected(oracle):
+    """The known answer separates the plausible slips: s1/c1 swapped, r1/r2
+    swapped, or the (x, y) and (z, w) pairs exchanged all keep unit norm but
+    move some component by far more than the tolerance."""
+    draws = golden()["uniform01"]["1"]
+    q = shoemake(*draws[:3])
+    x0, u1, u2 = draws[:3]
+    r1, r2 = math.sqrt(1 - x0), math.sqrt(x0)
+    t1, t2 = 2 * PI * u1, 2 * PI * u2
+    slips = [np.array([math.cos(t1) * r1, math.sin(t1) * r1, math.sin(t2) * r2, math.cos(t2) * r2]),
+             np.array([math.sin(t1) * r2, math.cos(t1) * r2, math.sin(t2) * r1, math.cos(t2) * r1]),
+             np.array([q[2], q[3], q[0], q[1]])]
+    for s in slips:
+        assert abs(np.linalg.norm(s) - 1) < 1e-12 and np.abs(s - q).max() > 1e-3
+
+
+# ---------------------------------------------------------------- RRT-Connect, free space
+def free_space_plan(start, goal, sample, maxd):
+    """RRT-Connect's first iteration in free space, from the reference's
+    growTree / solve steps written out: grow the start tree towards the sample
+    (steer to maxd), then grow the goal tree towards the added state until it
+    is reached; path = start, then the goal tree's chain from the added state."""
+    d = math.dist(start, sample)
+    margin = abs(d - maxd)  # closest call of a reach decision
+    a = sample if d <= maxd else start + (sample - start) * (maxd / d)
+    chain = [goal]
+    x = goal
+    while True:
+        d = math.dist(x, a)
+        margin = min(margin, abs(d - maxd))
+        if d <= maxd:
+            chain.append(a)
+            break
+        x = x + (a - x) * (maxd / d)
+        chain.append(x)
+    path = [start] + chain[::-1]
+    nodes = 2 + len(chain)
+    return np.array(path), nodes, margin
+
+
+def test_rrt_connect_free_space_closed_form(oracle):
+    g = golden()["sample_uniform_seed7_pi"]
+    sample = np.array(g[:7])
+    empty = oracle.OracleEnv(np.zeros((0, 4), np.float32), np.zeros((0, 6), np.float32))
+    rng = np.random.default_rng(3)
+    cases = 0
+    for _ in range(12):
+        start, goal = rng.uniform(-PI, PI, 7), rng.uniform(-PI, PI, 7)
+        for maxd in (MAXD, 1.0, 0.37):
+            want, nodes, margin = free_space_plan(start, goal, sample, maxd)
+            if margin < 1e-9:  # a reach decision within rounding of maxd
+                continue
+            ok, it, path, tree_nodes = oracle.rrt_connect(empty, start, goal, 7, maxd, lvs_config(), max_iters=5)
+            assert ok and it == 1
+            assert tree_nodes == nodes
+            assert path.shape == want.shape
+            assert np.array_equal(path[0], start) and np.array_equal(path[-1], goal)
+            assert np.allclose(path, want, rtol=0, atol=1e-12)
+            cases += 1
+    assert cases >= 24
+
+
+# ---------------------------------------------------------------- RRT-Connect, obstacle scenes
+def scene(gpu_free_workload_seed, ns, nb, oracle):
+    from ompl_amd import workload
+
+    s, b = workload.obstacles(gpu_free_workload_seed, ns, nb)
+    env = oracle.OracleEnv(s, b)
+    q = workload.configs(gpu_free_workload_seed + 77, 3000)
+    ok = [row for row in q if oracle.state_valid(env, row.astype(np.float32))]
+    return env, ok
+
+
+@pytest.mark.parametrize("env_seed,ns,nb", [(1020, 12, 12), (1003, 24, 24)])
+def test_rrt_connect_paths_are_valid_chains(oracle, env_seed, ns, nb):
+    env, ok = scene(env_seed, ns, nb, oracle)
+    lvs = lvs_config()
+    solved = 0
+    for i in range(8):
+        start, goal = ok[2 * i], ok[2 * i + 1]
+        res, it, path, nodes = oracle.rrt_connect(env, start, goal, 300 + i, MAXD, lvs, max_iters=1500)
+        if not res:
+            assert it == 1500 and len(path) == 0
+            continue
+        solved += 1
+        assert 1 <= it <= 1500 and nodes >= len(path) >= 2
+        assert np.array_equal(path[0], start) and np.array_equal(path[-1], goal)
+        for a, b in zip(path[:-1], path[1:]):
+            assert math.dist(a, b) <= MAXD * (1 + 1e-12)
+            valid, _ = oracle.check_motion(env, a, b, lvs)
+            assert valid
+    assert solved >= 4
+
+
+class Motion:
+    __slots__ = ("state", "parent")
+
+    def __init__(self, state, parent):
+        self.state = state
+        self.parent = parent
+
+
+class Replay:
+    """RRTConnect::solve written from the reference's structure: two lists of
+    Motion (nearest = first strict minimum, as NearestNeighborsLinear), a
+    TreeGrowingInfo (xstate, xmotion, start), and the sampler's
+    RNG::uniformReal over mt19937(seed)."""
+
+    def __init__(self, oracle, env, start, goal, seed, maxd, lvs):
+        self.o, self.env, self.maxd, self.lvs = oracle, env, maxd, lvs
+        self.t_start = [Motion(np.array(start, np.float64), None)]
+        self.t_goal = [Motion(np.array(goal, np.float64), None)]
+        self.rng = oracle.Mt19937(seed)
+        self.outcomes = {"trapped_sample": 0, "trapped_first_connect": 0, "advanced": 0, "reached": 0}
+
+    def nearest(self, tree, x):
+        best, bd = None, None
+        for m in tree:
+            d = math.sqrt(sum((m.state[k] - x[k]) ** 2 for k in range(7)))
+            if bd is None or bd > d:
+                best, bd = m, d
+        return best
+
+    def grow(self, tree, tgi, rstate):
+        nm = self.nearest(tree, rstate)
+        reach = True
+        d = math.sqrt(sum((nm.state[k] - rstate[k]) ** 2 for k in range(7)))
+        dstate = rstate
+        if d > self.maxd:
+            t = self.maxd / d
+            x = np.array([nm.state[k] + (rstate[k] - nm.state[k]) * t for k in range(7)])
+            if all(abs(nm.state[k] - x[k]) <= 2.220446049250313080847e-16 * 2 for k in range(7)):
+                return "TRAPPED"
+            dstate = x
+            reach = False
+        if tgi["start"]:
+            valid = self.o.check_motion(self.env, nm.state, dstate, self.lvs)[0]
+        else:
+            valid = (self.o.state_valid(self.env, dstate.astype(np.float32))
+                     and self.o.check_motion(self.env, dstate, nm.state, self.lvs)[0])
+        if not valid:
+            return "TRAPPED"
+        m = Motion(np.array(dstate), nm)
+        tree.append(m)
+        tgi["xmotion"] = m
+        tgi["xstate"] = m.state
+        return "REACHED" if reach else "ADVANCED"
+
+    def solve(self, max_iters):
+        start_tree = True
+        tgi = {"start": True, "xmotion": None, "xstate": None}
+        for it in range(max_iters):
+            tree = self.t_start if start_tree else self.t_goal
+            tgi["start"] = start_tree
+            start_tree = not start_tree
+            other = self.t_start if start_tree else self.t_goal
+            rstate = np.array([self.rng.uniform_real(-PI, PI) for _ in range(7)])
+            gs = self.grow(tree, tgi, rstate)
+            if gs == "TRAPPED":
+                self.outcomes["trapped_sample"] += 1
+                continue
+            added = tgi["xmotion"]
+            if gs != "REACHED":
+                rstate = tgi["xstate"].copy()
+            tgi["start"] = start_tree
+            gsc = self.grow(other, tgi, rstate)
+            if gsc == "TRAPPED":
+                self.outcomes["trapped_first_connect"] += 1
+                tgi["start"] = not tgi["start"]
+            while gsc == "ADVANCED":
+                self.outcomes["advanced"] += 1
+                gsc = self.grow(other, tgi, rstate)
+            if gsc != "REACHED":
+                continue
+            self.outcomes["reached"] += 1
+            sm = tgi["xmotion"] if tgi["start"] else added
+            gm = added if tgi["start"] else tgi["xmotion"]
+            if sm.parent is not None:
+                sm = sm.parent
+            else:
+                gm = gm.parent
+            p1, p2 = [], []
+            while sm is not None:
+                p1.append(sm.state)
+                sm = sm.parent
+            while gm is not None:
+                p2.append(gm.state)
+                gm = gm.parent
+            return True, it + 1, np.array(p1[::-1] + p2)
+        return False, max_iters, np.zeros((0, 7))
+
+
+@pytest.mark.parametrize("env_seed,ns,nb,maxd", [(1020, 12, 12, MAXD), (1003, 24, 24, 1.5)])
+def test_rrt_connect_matches_independent_replay(oracle, env_seed, ns, nb, maxd):
+    env, ok = scene(env_seed, ns, nb, oracle)
+    lvs = lvs_config()
+    seen = {"trapped_sample": 0, "trapped_first_connect": 0, "advanced": 0, "reached": 0}
+    for i in range(5):
+        start, goal = ok[2 * i], ok[2 * i + 1]
+        rp = Replay(oracle, env, start, goal, 40 + i, maxd, lvs)
+        want_ok, want_it, want_path = rp.solve(250)
+        for k in seen:
+            seen[k] += rp.outcomes[k]
+        got_ok, got_it, got_path, nodes = oracle.rrt_connect(env, start, goal, 40 + i, maxd, lvs, max_iters=250)
+        assert got_ok == want_ok and got_it == want_it
+        assert nodes == len(rp.t_start) + len(rp.t_goal)
+        assert np.array_equal(got_path, want_path)
+    assert seen["trapped_sample"] > 0 and seen["trapped_first_connect"] > 0 and seen["advanced"] > 0
+    assert seen["reached"] > 0
