@@ -120,6 +120,17 @@ int ompl_amd_arm_spheres_dev(const float *q_soa_dev, int64_t n, float *out_dev, 
 int ompl_amd_check_motions_dev(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
                                double longest_valid_segment, unsigned factor, uint8_t *out_dev,
                                int32_t *checked_dev, void *stream);
+/* ompl_amd_check_motions_dev without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * survivor workspace is a fixed device array of the library (8.1 MiB per
+ * device, held until the process ends), so n is at most 65,536 edges
+ * (OMPL_AMD_ERR_ARG above that).  Every call on a device shares that array:
+ * calls that overlap on one device race; calls on one stream are ordered.
+ * Same results as ompl_amd_check_motions_dev. */
+int ompl_amd_check_motions_noalloc_dev(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev,
+                                       int64_t n, double longest_valid_segment, unsigned factor, uint8_t *out_dev,
+                                       int32_t *checked_dev, void *stream);
 /* lastValid variant: last_t_dev[i] = (j-1)/nd for the first invalid j of the
  * sequential scan, (nd-1)/nd when only s2 is invalid; untouched when valid. */
 int ompl_amd_check_motions_last_valid_dev(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev,

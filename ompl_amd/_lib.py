@@ -40,6 +40,7 @@ SIGNATURES = {
     "ompl_amd_is_valid": (C.c_int, [vp, vp, c_int_p]),
     "ompl_amd_arm_spheres_dev": (C.c_int, [vp, C.c_int64, vp, vp]),
     "ompl_amd_check_motions_dev": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, vp, vp]),
+    "ompl_amd_check_motions_noalloc_dev": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, vp, vp]),
     "ompl_amd_check_motions_last_valid_dev": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, vp,
                                                         vp]),
     "ompl_amd_check_motion": (C.c_int, [vp, vp, vp, C.c_double, C.c_uint, c_int_p]),

@@ -61,6 +61,7 @@ static void do_init(int dev)
     g_device = dev;
     g_cus = prop.multiProcessorCount;
     OA_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    init_noalloc_scratch(dev);
 }
 
 void init_device(int dev)

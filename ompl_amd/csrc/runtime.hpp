@@ -13,6 +13,9 @@
 namespace ompl_amd {
 
 void init_device(int dev);
+// Takes the device address of the no-allocation entry point's scratch array
+// (once per device, with that device current).
+void init_noalloc_scratch(int dev);
 
 // Per-stream scratch of the batched motion path: survivor list + counters.
 struct Workspace {
