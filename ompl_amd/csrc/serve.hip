@@ -248,6 +248,73 @@ __device__ __forceinline__ bool block_check(const double *a, const double *b, co
     return true;
 }
 
+// Nearest node of tree T to tgt by one wave (the speculative sampling steps):
+// the same first strict minimum as block_nearest; every lane returns it.
+__device__ __forceinline__ int wave_nearest(const double *__restrict__ T, int cap, int n, const double tgt[kDof])
+{
+    double best = INFINITY;
+    int bi = INT_MAX;
+    for (int i = lane_id(); i < n; i += 64) {
+        double dist = 0.0;
+#pragma unroll
+        for (int k = 0; k < kDof; ++k) {
+            const double diff = T[(int64_t)k * cap + i] - tgt[k];
+            dist += diff * diff;
+        }
+        const double d = __builtin_sqrt(dist);
+        if (d < best) {
+            best = d;
+            bi = i;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double od = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (od < best || (od == best && oi < bi)) {
+            best = od;
+            bi = oi;
+        }
+    }
+    return bi;
+}
+
+// Positions [first, last) of a motion (numbered as in block_check) by one
+// wave; false at the first round holding an invalid position.
+template <int OPS>
+__device__ __forceinline__ bool wave_check(const double *a, const double *b, const double *dst, int nd, int first,
+                                           int last, bool extra, const float4 *env, int ns, int nb)
+{
+    using Shape = GroupShape<7, OPS>;
+    constexpr int G = Shape::STATES;
+    const Shape sh(lane_id());
+    for (int base = first; base < last; base += G) {
+        const int p = base + sh.group;
+        const bool live = p < last;
+        float q[kDof];
+        if (extra && p == 0) {
+#pragma unroll
+            for (int k = 0; k < kDof; ++k) q[k] = (float)dst[k];
+        } else {
+            const int mp = extra ? p - 1 : p;
+            walk_state(a, b, nd, live ? mp : 0, q);
+        }
+        LaneSpheres<7> S;
+        arm_fk<7, Shape::GS>(q, sh.slot, S);
+        const bool hit = collide_links<OPS, Exit::kAnyGroup, true, kPlanPad>(S, env, ns, nb, live, sh.part);
+        if (wave_ballot(hit && live)) return false;
+    }
+    return true;
+}
+
+// Outcome of a speculative sampling step's own check.
+enum SpecResult { kSpecTrapped = 0, kSpecValid = 1, kSpecUndecided = 2 };
+// Positions a speculative step checks with its own wave: the new state (or
+// s2) with the obstacle list split nine ways, then the next eight with one
+// state per link group; a motion still valid after them is finished by the
+// whole block if its step is the next one to commit.
+constexpr int kSpecPositions = 9;
+
 // Runs request blockIdx.x's RRTConnect::solve loop for one slice.  Control
 // flow is block-uniform: every thread holds the loop's variables and updates
 // them identically; values one thread produces (the sample, the nearest
@@ -263,6 +330,10 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
     __shared__ double s_a[kDof], s_b[kDof], s_x[kDof]; // the step's motion and new state
     __shared__ double s_u[kMtN / 2]; // the 0-1 draws of the generator's current 624-word block
     __shared__ int s_stop, s_hit, s_n1, s_len;
+    // speculative sampling steps, one per wave: samples, motions, outcomes
+    __shared__ double s_samp[NT / 64 * kDof], s_ga[NT / 64 * kDof], s_gb[NT / 64 * kDof], s_gx[NT / 64 * kDof];
+    __shared__ int s_res[NT / 64], s_nm[NT / 64], s_reach[NT / 64], s_chk[NT / 64], s_nd[NT / 64],
+        s_npos[NT / 64];
     const int tid = threadIdx.x;
     const int64_t t0 = wall_clock64();
     const PlanCmd &C = A.cmds[blockIdx.x];
@@ -334,6 +405,163 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
         }
         ++done_here;
         if (tid == 0) c0 = c1 = wall_clock64();
+        // ---- speculative sampling steps.  A sampling step (phase 0) that is
+        // TRAPPED changes neither tree, and the samples of the next steps are
+        // the generator's next draws whatever happens, so the next m steps can
+        // run at once, one per wave, against the current trees (step j grows
+        // the start tree iff start_tree ^ (j & 1)); they are then taken in order
+        // up to the first that is not TRAPPED, which is committed, and the rest
+        // are dropped (their draws are not consumed).  Same steps, same results
+        // as one at a time.  m stays within the generator's current block of
+        // draws and the iterations left.
+        const int m_spec = phase == 0 ? min(NT / 64, min((kMtN / 2 - pidx) / kDof, C.max_iters - iteration)) : 0;
+        if (m_spec >= 2) {
+            const int m = m_spec;
+            if (tid < kDof * m) s_samp[tid] = (kPi - -kPi) * s_u[pidx + tid] + -kPi;
+            __syncthreads();
+            if (tid == 0) {
+                c1 = wall_clock64();
+                t_sample += c1 - c0;
+            }
+            const int wave = tid >> 6, lane = tid & 63;
+            if (wave < m) {
+                const int st_j = start_tree ^ (wave & 1);
+                const int tree_j = st_j ? 0 : 1;
+                const double *Tj = R.tree_s[tree_j];
+                double tgt[kDof];
+#pragma unroll
+                for (int k = 0; k < kDof; ++k) tgt[k] = s_samp[kDof * wave + k];
+                const int nm = wave_nearest(Tj, C.cap, tree_j ? n1 : n0, tgt);
+                if (tid == 0) {
+                    const int64_t c2 = wall_clock64();
+                    t_nearest += c2 - c1;
+                }
+                double nst[kDof], x[kDof];
+                double dist = 0.0;
+#pragma unroll
+                for (int k = 0; k < kDof; ++k) {
+                    nst[k] = Tj[(int64_t)k * C.cap + nm];
+                    const double diff = nst[k] - tgt[k];
+                    dist += diff * diff;
+                }
+                const double dd = __builtin_sqrt(dist);
+                bool reach = true, moved = true;
+#pragma unroll
+                for (int k = 0; k < kDof; ++k) x[k] = tgt[k];
+                if (dd > C.maxd) {
+                    const double t = C.maxd / dd;
+                    bool equal = true;
+#pragma unroll
+                    for (int k = 0; k < kDof; ++k) {
+                        x[k] = nst[k] + (tgt[k] - nst[k]) * t;
+                        if (fabs(nst[k] - x[k]) > kEqualEps) equal = false;
+                    }
+                    moved = !equal;
+                    reach = false;
+                }
+                int res = kSpecTrapped, nd = 0, npos = 0;
+                if (moved) {
+                    double *ga = s_ga + kDof * wave, *gb = s_gb + kDof * wave, *gx = s_gx + kDof * wave;
+                    double a[kDof], b[kDof];
+#pragma unroll
+                    for (int k = 0; k < kDof; ++k) {
+                        a[k] = st_j ? nst[k] : x[k];
+                        b[k] = st_j ? x[k] : nst[k];
+                    }
+                    nd = valid_segment_count<kDof>(a, b, C.lvs, C.factor);
+                    if (lane == 0)
+                        for (int k = 0; k < kDof; ++k) {
+                            ga[k] = a[k];
+                            gb[k] = b[k];
+                            gx[k] = x[k];
+                        }
+                    // lane 0's LDS writes before the wave's reads
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (nd <= kMaxSegments) {
+                        const bool extra = !st_j;
+                        npos = 1 + (nd >= 2 ? nd - 1 : 0) + (extra ? 1 : 0);
+                        bool valid = wave_check<9>(ga, gb, gx, nd, 0, 1, extra, s_env, C.ns, C.nb);
+                        if (valid && npos > 1)
+                            valid = wave_check<1>(ga, gb, gx, nd, 1, min(npos, kSpecPositions), extra, s_env, C.ns,
+                                                  C.nb);
+                        res = !valid ? kSpecTrapped : (npos <= kSpecPositions ? kSpecValid : kSpecUndecided);
+                    }
+                }
+                if (lane == 0) {
+                    s_res[wave] = res;
+                    s_nm[wave] = nm;
+                    s_reach[wave] = reach ? 1 : 0;
+                    s_chk[wave] = moved ? 1 : 0;
+                    s_nd[wave] = nd;
+                    s_npos[wave] = npos;
+                }
+            }
+            __syncthreads();
+            // take the steps in order
+            for (int j = 0; j < m; ++j) {
+                const int st_j = start_tree;
+                int res = s_res[j];
+                ++steps;
+                checks += s_chk[j];
+                if (res == kSpecUndecided) { // the rest of the motion by the whole block
+                    if (tid == 0) s_hit = 0;
+                    __syncthreads();
+                    const int nd = s_nd[j], npos = s_npos[j];
+                    const double *ga = s_ga + kDof * j, *gb = s_gb + kDof * j, *gx = s_gx + kDof * j;
+                    const bool extra = !st_j;
+                    const int ops = C.ops ? C.ops : auto_ops<NT / 64>(npos - kSpecPositions);
+                    bool valid;
+                    if (ops == 9)
+                        valid = block_check<NT, 9>(ga, gb, gx, nd, kSpecPositions, npos, extra, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                    else if (ops == 4)
+                        valid = block_check<NT, 4>(ga, gb, gx, nd, kSpecPositions, npos, extra, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                    else if (ops == 2)
+                        valid = block_check<NT, 2>(ga, gb, gx, nd, kSpecPositions, npos, extra, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                    else
+                        valid = block_check<NT, 1>(ga, gb, gx, nd, kSpecPositions, npos, extra, s_env, C.ns, C.nb,
+                                                   &s_hit);
+                    res = valid ? kSpecValid : kSpecTrapped;
+                }
+                start_tree = !start_tree; // RRTConnect.cpp:249
+                pidx += kDof;             // this step's sample is consumed
+                if (res == kSpecTrapped) {
+                    ++iteration;
+                    if (iteration >= C.max_iters) {
+                        status = OMPL_AMD_PLAN_FAILED;
+                        break;
+                    }
+                    continue;
+                }
+                // commit step j: append its new state, then connect (phase 1)
+                const int tree_j = st_j ? 0 : 1;
+                const int ntj = tree_j ? n1 : n0;
+                if (tid < kDof) {
+                    R.tree_s[tree_j][(int64_t)tid * C.cap + ntj] = s_gx[kDof * j + tid];
+                    // rstate: the sample when reached, else the new state
+                    s_rstate[tid] = s_reach[j] ? s_samp[kDof * j + tid] : s_gx[kDof * j + tid];
+                }
+                if (tid == 0) R.parent[tree_j][ntj] = s_nm[j];
+                if (tree_j) ++n1;
+                else ++n0;
+                xmotion = ntj;
+                added = ntj;
+                tgi_start = start_tree;
+                grow_tree = start_tree ? 0 : 1; // otherTree
+                phase = 1;
+                break;
+            }
+            if (tid == 0) {
+                const int64_t c3 = wall_clock64();
+                t_check += c3 - c1;
+                t_step += c3 - c0;
+            }
+            continue;
+        }
         if (phase == 0) {
             // RRTConnect.cpp:247-250: pick the tree, flip startTree_
             grow_tree = start_tree ? 0 : 1;
