@@ -182,14 +182,20 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
 
 def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
     """Batched PRM roadmap (samples -> validity -> kNN k=10 -> candidate edges ->
-    motion checks -> components): the 200k-sample, 512-sphere + 512-box scene,
-    and the same pipeline over the 64-sphere scene of the validity batch, where
-    most samples are valid.  The CPU restatement runs on a bounded sample."""
+    motion checks -> components) on 200k samples: the 512-sphere + 512-box
+    scene (empty roadmap: no sample is collision free), the same count of
+    small obstacles, and the 64-sphere scene of the validity batch, where most
+    samples are valid.  The CPU restatement runs on a bounded sample."""
     from ompl_amd.prm import build_roadmap
 
     out = {}
-    for name, env_seed, ns, nb in (("1024_obstacles", 1003, 512, 512), ("64_spheres", 1000, 64, 0)):
-        s, b = oa.workload.obstacles(env_seed, ns, nb)
+    scenes = (("1024_obstacles", 1003, 512, 512, (0.05, 0.15), (0.05, 0.2)),
+              # the same count of small obstacles (about 9 % of samples collision free): exercises kNN, motion
+              # checks and graph assembly at 1,024 obstacles, which the scene above leaves empty
+              ("1024_small_obstacles", 1003, 512, 512, (0.01, 0.03), (0.01, 0.03)),
+              ("64_spheres", 1000, 64, 0, (0.05, 0.15), (0.05, 0.2)))
+    for name, env_seed, ns, nb, rr, hh in scenes:
+        s, b = oa.workload.obstacles(env_seed, ns, nb, r=rr, h=hh)
         env = oa.Env(s, b)
         # warm the pipeline and tune the motion launch shape on this roadmap's candidate edges
         build_roadmap(env, PRM_SEED, n_samples, n_streams=256, fetch=False, autotune=True)
@@ -446,7 +452,10 @@ def main(argv=None):
     ap.add_argument("--stream-requests", type=int, default=256, help="served-stream requests (0 = skip)")
     ap.add_argument("--stream-qps", type=float, default=200.0)
     ap.add_argument("--stream-envs", type=int, default=32)
-    ap.add_argument("--plain-workers", type=int, default=8)
+    ap.add_argument("--plain-workers", type=int, default=8,
+                    help="threads of the plain per-call planner on the served stream (0 = skip the plain leg)")
+    ap.add_argument("--no-stream-sweeps", action="store_true",
+                    help="skip the served stream's offered-rate and server-setting sweeps")
     ap.add_argument("--prm-samples", type=int, default=PRM_SAMPLES, help="PRM roadmap batch size (0 = skip)")
     ap.add_argument("--prm-cpu-samples", type=int, default=20000)
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements")
@@ -676,6 +685,7 @@ def main(argv=None):
     if args.stream_requests > 0:
         stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers,
                                    log_path=args.benchmark_log, run_plain=args.plain_workers > 0)
+    if args.stream_requests > 0 and not args.no_stream_sweeps:
         # the same stream served alone at higher offered rates (saturation sweep)
         stream_res["served_qps_sweep"] = {
             str(int(q)): served_stream(oa, args.stream_requests, q, args.stream_envs, 0, run_plain=False)["served"]

@@ -83,6 +83,8 @@ typedef struct ompl_amd_env_s *ompl_amd_env;
  * order given here. */
 int ompl_amd_env_create(const float *spheres, int n_spheres, const float *boxes, int n_boxes,
                         ompl_amd_env *out);
+/* Fails with OMPL_AMD_ERR_ARG (and frees nothing) while queued or running
+ * server requests still use the environment. */
 int ompl_amd_env_destroy(ompl_amd_env env);
 /* device bytes held by the environment */
 int ompl_amd_env_bytes(ompl_amd_env env, size_t *bytes);

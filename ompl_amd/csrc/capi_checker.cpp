@@ -242,6 +242,10 @@ int ompl_amd_env_destroy(ompl_amd_env e)
 {
     return guarded([&] {
         if (!e) return;
+        const int users = e->server_users.load();
+        OA_REQUIRE(users == 0, OMPL_AMD_ERR_ARG,
+                   "environment still in use by %d queued or running server request(s); destroy it after they finish",
+                   users);
         if (e->env.obs) OA_HIP(hipFree(e->env.obs));
         if (e->env.obs_links) OA_HIP(hipFree(e->env.obs_links));
         delete e;

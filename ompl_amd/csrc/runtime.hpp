@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <atomic>
 #include <mutex>
 
 #include "checker.hpp"
@@ -104,4 +105,7 @@ constexpr int kMaxEnvWords = 4096; // 64 KiB of dynamic LDS
 
 struct ompl_amd_env_s {
     ompl_amd::Env env;
+    // server requests (queued or running) that read this environment;
+    // ompl_amd_env_destroy refuses while any remain
+    std::atomic<int> server_users{0};
 };

@@ -791,6 +791,7 @@ struct Planner {
     std::vector<double> path;
     bool started = false;
     bool cancel = false;
+    bool env_held = false; // counted in the environment's server_users
     // server thread only
     size_t off = 0, bytes = 0; // region in the pool
     bool init = true;
@@ -863,11 +864,16 @@ struct Server {
     void finish_locked(Planner *p, int status);
 };
 
-// Publishes a final status (caller holds mu) and returns the region.
+// Publishes a final status (caller holds mu), returns the region and
+// releases the request's hold on its environment.
 void Server::finish_locked(Planner *p, int status)
 {
     p->status = status;
     p->t_finish = Clock::now();
+    if (p->env_held) {
+        p->req.env->server_users.fetch_sub(1);
+        p->env_held = false;
+    }
     if (p->bytes) {
         alloc.release(p->off, p->bytes);
         used_bytes -= p->bytes;
@@ -1186,7 +1192,10 @@ int ompl_amd_server_destroy(ompl_amd_server h)
         }
         S.cv.notify_all();
         if (S.th.joinable()) S.th.join();
-        for (auto &kv : S.reqs) delete kv.second;
+        for (auto &kv : S.reqs) {
+            if (kv.second->env_held) kv.second->req.env->server_users.fetch_sub(1);
+            delete kv.second;
+        }
         // teardown: release everything even if one release reports an error
         (void)hipStreamSynchronize(S.stream);
         if (S.pool) (void)hipFree(S.pool);
@@ -1229,6 +1238,8 @@ int ompl_amd_server_submit(ompl_amd_server h, const ompl_amd_plan_request *req, 
             p->status = OMPL_AMD_PLAN_REJECTED;
             p->t_finish = p->t_submit;
         } else {
+            p->req.env->server_users.fetch_add(1); // the environment must outlive the request
+            p->env_held = true;
             S.queue.push_back(rid);
         }
         *id = rid;

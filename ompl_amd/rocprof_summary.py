@@ -1,7 +1,8 @@
 """Text summaries of rocprofv3 csv output, for the profile records.
 
     python -m ompl_amd.rocprof_summary stats DIR
-        per-kernel calls and average / min / max duration (kernel_stats.csv)
+        per-kernel calls and average / min / max duration (kernel_stats.csv, or
+        the SQLite results database)
     python -m ompl_amd.rocprof_summary traffic BENCH_FETCH BENCH_WRITE COPY_FETCH COPY_WRITE COPY_BYTES
         per-kernel HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE runs, each
         scaled by what the counter reads for a plain device copy of COPY_BYTES
@@ -22,7 +23,22 @@ def _find(d: str, suffix: str) -> str:
     return f[0]
 
 
+def kernel_stats_db(d: str) -> list[dict]:
+    """The same statistics from a rocprofv3 SQLite results database (the
+    default output format of ROCm 7.x: its `kernels` view, durations in ns)."""
+    import sqlite3
+
+    db = sqlite3.connect(_find(d, "_results.db"))
+    rows = db.execute("SELECT name, COUNT(*), SUM(duration), AVG(duration), MIN(duration), MAX(duration) "
+                      "FROM kernels GROUP BY name ORDER BY SUM(duration) DESC").fetchall()
+    total = sum(r[2] for r in rows) or 1.0
+    return [{"kernel": r[0], "calls": int(r[1]), "avg_us": r[3] / 1e3, "min_us": r[4] / 1e3, "max_us": r[5] / 1e3,
+             "pct": 100.0 * r[2] / total} for r in rows]
+
+
 def kernel_stats(d: str) -> list[dict]:
+    if not glob.glob(f"{d}/**/*kernel_stats.csv", recursive=True):
+        return kernel_stats_db(d)
     rows = list(csv.DictReader(open(_find(d, "kernel_stats.csv"))))
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
     return [{"kernel": r["Name"], "calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3,

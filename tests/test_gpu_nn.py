@@ -79,13 +79,17 @@ def test_knn_config3_se3_sample(gpu, oracle):
 
 
 def test_radius_config3_se3_sample(gpu, oracle):
-    data = se3_points(gpu, 5, 200_000)
-    queries = se3_points(gpu, 6, 2048)
+    """Config 3 at its stated shape: 1M SE3 states, 65,536 queries, radius from
+    the RRT* schedule at n = 1e6; 48 queries spread over the batch checked
+    exactly (index lists and FP32 distances) against the oracle's full scan."""
+    data = se3_points(gpu, 5, 1_000_000)
+    queries = se3_points(gpu, 6, 65_536)
     r = gpu.rrtstar_radius(6, 8000.0 * np.pi ** 2, 1_000_000)
     assert abs(r - 0.97753) < 1e-4  # RRT* schedule at n = 1e6
     nn = make_nn(gpu, 1, 7, data)
     off, gi, gd = nn.radius_batch(queries, r)
-    sel = list(range(0, 2048, 64))
+    assert off[-1] > 65_536 and np.all(np.diff(off) >= 0)
+    sel = np.linspace(0, 65_535, 48).astype(int).tolist()
     ores = oracle.radius(1, data, queries[sel], r)
     for j, (oi, od) in zip(sel, ores):
         assert np.array_equal(gi[off[j]:off[j + 1]], oi)

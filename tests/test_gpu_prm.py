@@ -127,3 +127,65 @@ def test_prm_kstar_matches_oracle(gpu, oracle):
     o = oracle.prm_batch(oracle.OracleEnv(s, b), 11, 2500, 10, 0.01, 1, 16, threads=8, kstar=True)
     assert g.stats["k"] == o["k"] == oracle.kstar_k(g.milestones.shape[0])
     assert_same(g, o)
+
+
+def test_prm_config4_literal_scene_full_size(gpu, oracle):
+    """Config 4 as stated: 200,192 samples (256 generators) with 512 spheres +
+    512 boxes of the motion benchmark's sizes.  That scene is degenerate: the
+    obstacles fill the arm's reach, so no sample is collision free; the full
+    sample set's validity is checked against the oracle and the roadmap is
+    empty."""
+    g, o = roadmap_pair(gpu, oracle, 1003, 512, 512, 7, 200_192, 10, 256)
+    assert o["samples"].shape[0] == 200_192 and not o["valid"].any()
+    assert g.milestones.shape[0] == 0
+    assert_same(g, o)
+
+
+def test_prm_config4_small_obstacles_full_size(gpu, oracle):
+    """200,192 samples among 1,024 small obstacles (radii and half-extents
+    U[0.01, 0.03]; about 9 % of samples are collision free): milestones equal
+    the oracle's over the whole sample set; candidate edges are the undirected
+    kNN union (checked for a spread of milestones and of edges with the
+    oracle's kNN); edge validity equals the oracle's on a spread of 4,096
+    edges; components equal scipy's on the roadmap's own edges."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    from ompl_amd.prm import build_roadmap
+
+    s, b = gpu.workload.obstacles(1003, 512, 512, r=(0.01, 0.03), h=(0.01, 0.03))
+    env = gpu.Env(s, b)
+    oenv = oracle.OracleEnv(s, b)
+    print("prm config 4, small obstacles, 200192 samples", flush=True)
+    g = build_roadmap(env, 7, 200_192, k=10, n_streams=256)
+    per = 200_192 // 256
+    lo, hi = np.full(7, -np.pi), np.full(7, np.pi)
+    samples = np.concatenate([oracle.Mt19937(7 + s_).sample_uniform(lo, hi, per) for s_ in range(256)])
+    valid = oracle.validate_states_mt(oenv, samples.T.astype(np.float32), 16).astype(bool)
+    sidx = np.nonzero(valid)[0].astype(np.int32)
+    assert np.array_equal(g.sample_index, sidx) and np.array_equal(g.milestones, samples[sidx])
+    nv = len(sidx)
+    assert nv > 10_000
+    edges = g.edges
+    keyset = set(map(tuple, edges.tolist()))
+    f32 = g.milestones.astype(np.float32)
+    rows = np.linspace(0, nv - 1, 512).astype(int)
+    oi, _ = oracle.knn_mt(0, f32, f32[rows], 11, 16)
+    for i, nbr in zip(rows, oi):
+        for j in [j for j in nbr.tolist() if j != i][:10]:
+            assert (min(i, j), max(i, j)) in keyset
+    pick = np.linspace(0, len(edges) - 1, 2048).astype(int)
+    ends = np.unique(edges[pick].ravel())
+    ei, _ = oracle.knn_mt(0, f32, f32[ends], 11, 16)
+    nbrs = {int(u): set(j for j in row.tolist() if j != u) for u, row in zip(ends, ei)}
+    for u, v in edges[pick]:
+        assert v in nbrs[int(u)] or u in nbrs[int(v)]
+    sel = np.linspace(0, len(edges) - 1, 4096).astype(int)
+    ov, _ = oracle.check_motions(oenv, g.milestones[edges[sel, 0]], g.milestones[edges[sel, 1]], 0.01, 1, threads=16)
+    assert np.array_equal(g.edge_valid[sel], ov.astype(bool))
+    assert 0 < g.edge_valid.sum() < len(edges)
+    e = edges[g.edge_valid]
+    nc, lab = connected_components(coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(nv, nv)), directed=False)
+    first = np.full(nc, nv, np.int64)
+    np.minimum.at(first, lab, np.arange(nv))
+    assert nc == g.stats["components"] and np.array_equal(g.component, first[lab])

@@ -221,3 +221,106 @@ def test_release_right_after_finish_is_safe(gpu, scene):
     st = srv.stats()
     assert st["active"] == 0 and st["used_bytes"] == 0
     srv.close()
+
+
+def stream_requests(gpu, oracle, n_requests=64, seed=8, n_envs=32):
+    """The served stream's workload (config 5): environments with log-uniform
+    16-1,024 obstacles (half spheres, half boxes of the motion benchmark's
+    sizes; the densest leave no collision-free configuration and drop out),
+    plus two environments of 1,024 small obstacles (radii and half-extents
+    U[0.01, 0.03]) so the largest obstacle count is served; start and goal
+    are valid seeded configurations; max_iters 2,000 as in the benchmark."""
+    import torch
+
+    from ompl_amd.serve import make_request
+
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for e in range(n_envs):
+        n_obs = int(round(math.exp(rng.uniform(math.log(16), math.log(1024)))))
+        scenes.append((1000 + e, n_obs - n_obs // 2, n_obs // 2, (0.05, 0.15), (0.05, 0.2)))
+    scenes += [(1100, 512, 512, (0.01, 0.03), (0.01, 0.03)), (1101, 512, 512, (0.01, 0.03), (0.01, 0.03))]
+    envs = []
+    for env_seed, ns, nb, r, h in scenes:
+        s, b = gpu.workload.obstacles(env_seed, ns, nb, r=r, h=h)
+        env = gpu.Env(s, b)
+        q = gpu.workload.configs(env_seed + 1000, 4096)
+        v = gpu.validate_states(env, torch.from_numpy(q.T.copy()).cuda()).cpu().numpy()
+        ok = q[v == 1]
+        if len(ok) >= 2:
+            envs.append((env, oracle.OracleEnv(s, b), ok, ns + nb))
+    reqs = []
+    for i in range(n_requests + 4):
+        k = i % (len(envs) - 2) if i < n_requests else len(envs) - 2 + (i % 2)
+        env, oenv, ok, n_obs = envs[k]
+        a, b = ok[rng.integers(len(ok))], ok[rng.integers(len(ok))]
+        reqs.append((make_request(env, a, b, seed=10_000 + i, lvs=lvs_config(), max_iters=2000), oenv, a, b,
+                     10_000 + i, n_obs))
+    return reqs, envs  # the caller keeps the environments alive while requests use them
+
+
+def test_served_stream_32_environments(gpu, oracle):
+    """Config 5's stream at its shape: 68 requests over 27 environments of
+    16-1,024 obstacles; each request's plan is the same alone and in the mix
+    (all submitted at once, more than the server runs at a time) and equal to
+    the oracle's RRT-Connect."""
+    import concurrent.futures as cf
+
+    from ompl_amd.serve import Server
+
+    reqs, envs = stream_requests(gpu, oracle)
+    assert max(r[5] for r in reqs) == 1024 and min(r[5] for r in reqs) <= 32
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=24, max_nodes=4096)
+    alone = []
+    for r, *_ in reqs:
+        rid = srv.submit(r)
+        res = srv.wait(rid, 60000)
+        alone.append((res.status, res.iterations, res.tree_nodes, res.motion_checks, srv.path(rid)))
+        srv.release(rid)
+    print(f"stream: {len(reqs)} requests served alone", flush=True)
+    ids = [srv.submit(r) for r, *_ in reqs]
+    mix = []
+    for rid in ids:
+        res = srv.wait(rid, 120000)
+        mix.append((res.status, res.iterations, res.tree_nodes, res.motion_checks, srv.path(rid)))
+    st = srv.stats()
+    srv.close()
+    assert st["active"] == 0 and st["used_bytes"] == 0
+    del envs
+
+    def plan(item):
+        _, oenv, a, b, seed, _ = item
+        return oracle.rrt_connect(oenv, a, b, seed, MAXD, lvs_config(), max_iters=2000)
+
+    with cf.ThreadPoolExecutor(16) as ex:
+        ref = list(ex.map(plan, reqs))
+    from ompl_amd.serve import SOLVED
+
+    for i, (a_, m_, (ok, it, opath, nodes)) in enumerate(zip(alone, mix, ref)):
+        assert a_[:4] == m_[:4] and np.array_equal(a_[4], m_[4]), i
+        assert (a_[0] == SOLVED) == ok and a_[1] == it and a_[2] == nodes, i
+        assert np.array_equal(a_[4], opath), i
+    assert sum(a_[0] == SOLVED for a_ in alone) >= 20
+
+
+def test_environment_in_use_cannot_be_destroyed(gpu, scene):
+    """An environment that a queued or running request reads cannot be
+    destroyed (the call fails and frees nothing); once the request finishes,
+    it can."""
+    from ompl_amd._lib import OmplAmdError
+    from ompl_amd.serve import Server, make_request
+
+    _, _, pairs = scene
+    s, b = gpu.workload.obstacles(1020, 12, 12)
+    env = gpu.Env(s, b)
+    srv = Server(hbm_budget_bytes=1 << 28, max_active=2, max_nodes=2048)
+    # a long request (range 1e-4 rad between configurations radians apart), cancelled afterwards
+    rid = srv.submit(make_request(env, pairs[0][0], pairs[0][1], 5, lvs_config(), max_iters=10 ** 7, rng=1e-4))
+    with pytest.raises(OmplAmdError, match="in use"):
+        env.close()
+    assert env.handle is not None and env.handle.value
+    srv.cancel(rid)
+    assert srv.wait(rid, 60000).status >= 2
+    env.close()
+    assert env.handle is None
+    srv.close()
