@@ -130,6 +130,20 @@ int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const d
                     uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters, double *path,
                     int path_cap, int *path_len, int *iterations, long *tree_nodes);
 
+/* NearestNeighborsGNAT restatement (gnat.cpp): built in bulk over n points
+ * of `dim` floats (metric 0: L2, metric 1: SE3 with dim 7), pivots from
+ * GreedyKCenters with an mt19937(seed) draw for each first centre.  The
+ * points are not copied and must outlive the structure.  kNN results are
+ * ascending (index -1 / +inf past the set's size); radius search fills
+ * counts[nq] and, up to cap entries, the ascending index lists, and returns
+ * the total.  threads > 1 splits the queries over POSIX threads. */
+typedef struct orc_gnat orc_gnat;
+orc_gnat *orc_gnat_build(int metric, const float *pts, int64_t n, int dim, uint32_t seed);
+void orc_gnat_free(orc_gnat *g);
+void orc_gnat_knn(const orc_gnat *g, const float *q, int64_t nq, int k, int32_t *idx, double *dist, int threads);
+int64_t orc_gnat_radius(const orc_gnat *g, const float *q, int64_t nq, double r, int64_t *counts, int32_t *idx,
+                        int64_t cap, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,12 @@ def lib():
         L.orc_rrt_connect.argtypes = [C.POINTER(Env), vp, vp, C.c_uint32, C.c_double, C.c_double, C.c_uint,
                                       C.c_int, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_long)]
+        L.orc_gnat_build.argtypes = [C.c_int, vp, C.c_int64, C.c_int, C.c_uint32]
+        L.orc_gnat_build.restype = vp
+        L.orc_gnat_free.argtypes = [vp]
+        L.orc_gnat_knn.argtypes = [vp, vp, C.c_int64, C.c_int, vp, vp, C.c_int]
+        L.orc_gnat_radius.argtypes = [vp, vp, C.c_int64, C.c_double, vp, vp, C.c_int64, C.c_int]
+        L.orc_gnat_radius.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -431,3 +437,42 @@ def prm_batch(env: OracleEnv, seed: int, n_samples: int, k: int, lvs: float, fac
     ok, _ = check_motions(env, nodes[edges[:, 0]], nodes[edges[:, 1]], lvs, factor, threads)
     out.update(edges=edges, edge_valid=ok.astype(bool), component=disjoint_set_labels(nv, edges, ok.astype(bool)))
     return out
+
+
+class Gnat:
+    """NearestNeighborsGNAT restatement (oracle/gnat.cpp) over FP32 points with
+    FP64 distances (metric 0: L2, 1: SE3)."""
+
+    def __init__(self, metric: int, data: np.ndarray, seed: int = 1):
+        self.data = np.ascontiguousarray(data, np.float32)
+        self.metric, self.dim = metric, self.data.shape[1]
+        self.h = lib().orc_gnat_build(metric, _p(self.data), self.data.shape[0], self.dim, seed)
+        if not self.h:
+            raise ValueError("GNAT build: empty set or bad dimension")
+
+    def knn(self, queries: np.ndarray, k: int, threads: int = 1):
+        q = np.ascontiguousarray(queries, np.float32)
+        idx = np.zeros((q.shape[0], k), np.int32)
+        dist = np.zeros((q.shape[0], k), np.float64)
+        lib().orc_gnat_knn(self.h, _p(q), q.shape[0], k, _p(idx), _p(dist), threads)
+        return idx, dist
+
+    def radius(self, queries: np.ndarray, r: float, threads: int = 1):
+        """-> (offsets [nq+1], ascending index lists)"""
+        q = np.ascontiguousarray(queries, np.float32)
+        counts = np.zeros(q.shape[0], np.int64)
+        total = lib().orc_gnat_radius(self.h, _p(q), q.shape[0], r, _p(counts), None, 0, threads)
+        idx = np.zeros(max(total, 1), np.int32)
+        lib().orc_gnat_radius(self.h, _p(q), q.shape[0], r, _p(counts), _p(idx), total, threads)
+        return np.concatenate([[0], np.cumsum(counts)]), idx[:total]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orc_gnat_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

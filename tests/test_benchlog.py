@@ -69,3 +69,40 @@ def test_log_round_trip():
     assert abs(float(r[0]["solution_length"]) - (0.3 + 0.2) * 7 ** 0.5) < 1e-12
     assert float(r[1]["time"]) == 80.0 / 1e3 and float(r[2]["latency"]) == 2.1 / 1e3
     assert len(planners["geometric_RRTConnect_plain"][1]) == 1
+
+
+def test_reference_reader_fixture():
+    """benchlog.py still writes the exact log that the reference's own reader
+    (readBenchmarkLog, py-bindings/ompl/_benchmark_statistics.py:110) was run
+    on to make tests/golden/benchlog_reference_db.json, and the reader's
+    database holds the values written: experiment header and properties,
+    planner settings, the status enum, and every run property."""
+    import importlib.util
+    import json
+    from pathlib import Path
+
+    golden = Path(__file__).resolve().parent / "golden"
+    spec = importlib.util.spec_from_file_location("make_benchlog_golden", golden / "make_benchlog_golden.py")
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    fix = json.loads((golden / "benchlog_reference_db.json").read_text())
+    text = mk.fixed_log()
+    assert text == fix["log"]
+    t = fix["tables"]
+    exp = dict(zip(t["experiments"]["columns"], t["experiments"]["rows"][0]))
+    assert exp["name"] == "served_vs_plain" and exp["version"] == "OMPL 2.0.2" and exp["seed"] == "8"
+    assert exp["timelimit"] == 10.0 and exp["memorylimit"] == 4096.0 and exp["runcount"] == 3
+    assert exp["totaltime"] == 1.25 and exp["qps"] == 200.0 and exp["requests"] == 3
+    assert [r[1] for r in t["plannerConfigs"]["rows"]] == ["geometric_RRTConnect_served", "geometric_RRTConnect_plain"]
+    assert [r[2] for r in t["enums"]["rows"]] == benchlog.STATUS_NAMES
+    cols = t["runs"]["columns"]
+    runs = [dict(zip(cols, r)) for r in t["runs"]["rows"]]
+    _, _, _, _, planners = parse(text)
+    mine = planners["geometric_RRTConnect_served"][1] + planners["geometric_RRTConnect_plain"][1]
+    assert len(runs) == len(mine) == 5
+    for ref, got in zip(runs, mine):
+        for k, v in got.items():
+            if v == "":
+                assert ref[k] is None, k
+            else:
+                assert float(ref[k]) == float(v), (k, ref[k], v)
