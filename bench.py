@@ -87,12 +87,15 @@ def percentile(xs, p):
 
 
 def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000, log_path=None,
-                  check_parts=0, profile_slices=False, run_plain=True, block_threads=0, slice_us=0):
-    """Poisson stream of RRT-Connect requests over seeded environments (16-1024
-    obstacles, log-uniform, half spheres / half boxes), served (a) by the
-    continuous-batching server and (b) by the plain path (host RRT-Connect, one
-    GPU launch + sync per check) on `plain_workers` threads.  Latency =
-    completion - arrival.  Returns a dict of p50/p99 latency and throughput."""
+                  check_parts=0, profile_slices=False, run_plain=True, block_threads=0, slice_us=0, prm_every=4,
+                  prm_samples=2048):
+    """Poisson stream of planning requests over seeded environments (16-1024
+    obstacles, log-uniform, half spheres / half boxes): RRT-Connect, and every
+    prm_every-th request a PRM query (batch roadmap of prm_samples samples),
+    served (a) by the continuous-batching server and (b) by the plain path
+    (host planner, one GPU launch + sync per check) on `plain_workers`
+    threads.  Latency = completion - arrival.  Returns a dict of p50/p99
+    latency and throughput, overall and per planner."""
     import concurrent.futures as cf
     import threading
 
@@ -118,7 +121,9 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         k = i % len(envs)
         cand = starts[k]
         a, b = cand[rng.integers(len(cand))], cand[rng.integers(len(cand))]
-        reqs.append(serve.make_request(envs[k], a, b, seed=10_000 + i, lvs=lvs, max_iters=max_iters))
+        prm = prm_every > 0 and i % prm_every == prm_every - 1  # every prm_every-th request is a PRM query
+        reqs.append(serve.make_request(envs[k], a, b, seed=10_000 + i, lvs=lvs, max_iters=max_iters,
+                                       planner=serve.PRM if prm else serve.RRT_CONNECT, prm_samples=prm_samples))
     gaps = rng.exponential(1.0 / qps, n_requests)
     arrivals = np.cumsum(gaps)
 
@@ -153,7 +158,10 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         slice_us_gpu = 1e3 * ms_sl / n_sl if n_sl else None
         oa.profile.enable(False)
         oa.profile.reset()
+    is_prm = [r.planner == serve.PRM for r in reqs]
     served = {"latency_ms": pct(lat_s), "throughput_rps": n_requests / span_s, "slices": stats["slices"],
+              "latency_ms_rrt_connect": pct([x for x, q in zip(lat_s, is_prm) if not q]),
+              "latency_ms_prm": pct([x for x, q in zip(lat_s, is_prm) if q]) if any(is_prm) else None,
               "grow_steps": stats["steps"], "motion_checks": checks_s,
               "edge_checks_per_s": checks_s / span_s,
               "run_ms": pct([r.run_ms for r in res_s]), "queued_ms": pct([r.queued_ms for r in res_s]),
@@ -205,7 +213,10 @@ def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=
         "requests": n_requests, "qps": qps, "envs_used": len(envs), "envs_drawn": n_envs,
         "solved": solved, "max_iters": max_iters,
         "served": served,
+        "prm_requests": int(sum(is_prm)),
         "plain": {"latency_ms": pct(lat_p), "throughput_rps": n_requests / span_p, "workers": plain_workers,
+                  "latency_ms_rrt_connect": pct([x for x, q in zip(lat_p, is_prm) if not q]),
+                  "latency_ms_prm": pct([x for x, q in zip(lat_p, is_prm) if q]) if any(is_prm) else None,
                   "motion_checks": checks_p, "edge_checks_per_s": checks_p / span_p},
         "p99_plain_over_served": float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)),
         "p50_plain_over_served": float(np.percentile(lat_p, 50) / np.percentile(lat_s, 50)),
@@ -756,6 +767,11 @@ def main(argv=None):
     if args.stream_requests > 0:
         stream_res = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, args.plain_workers,
                                    log_path=args.benchmark_log, run_plain=args.plain_workers > 0)
+    if args.stream_requests > 0:
+        # the same stream with RRT-Connect requests only (the plain path keeps up with the offered rate there)
+        stream_res["rrt_connect_only"] = served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs,
+                                                       args.plain_workers, run_plain=args.plain_workers > 0,
+                                                       prm_every=0)
     if args.stream_requests > 0 and not args.no_stream_sweeps:
         # the same stream served alone at higher offered rates (saturation sweep)
         stream_res["served_qps_sweep"] = {
@@ -766,7 +782,7 @@ def main(argv=None):
         stream_res["served_config_sweep"] = {
             f"parts{parts}_t{threads}": served_stream(oa, args.stream_requests, args.stream_qps, args.stream_envs, 0,
                                                       check_parts=parts, block_threads=threads, profile_slices=True,
-                                                      run_plain=False)["served"]
+                                                      run_plain=False, prm_every=0)["served"]
             for parts, threads in ((0, 512), (1, 512), (2, 512), (4, 512), (9, 512), (0, 1024))}
 
     prm_res = None

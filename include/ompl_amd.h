@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 306
+#define OMPL_AMD_VERSION 307
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -379,7 +379,23 @@ typedef struct {
     double deadline_ms;           /* <= 0: none */
     int max_nodes;                /* <= 0: the server's max_nodes; region size from
                                      ompl_amd_server_footprint */
+    int planner;                  /* OMPL_AMD_PLANNER_RRT_CONNECT or OMPL_AMD_PLANNER_PRM */
+    int prm_samples;              /* PRM: uniform samples of the roadmap (<= 0: 4,096) */
+    int prm_k;                    /* PRM: neighbours per milestone (<= 0: 10, PRM's default) */
 } ompl_amd_plan_request;
+
+/* Planner of a request.  PRM (geometric/planners/prm/src/PRM.cpp:454-536,
+ * 570-604, 696-727) is served as one batch roadmap per request: start and
+ * goal become milestones 0 and 1 of a roadmap over prm_samples uniform
+ * samples (64 generators seeded seed, seed + 1, ...; KStrategy with prm_k
+ * neighbours, every milestone connected to its k nearest among all), and the
+ * path is the shortest one between them over the valid edges (FP64 state
+ * distances).  FAILED when start or goal is invalid or they are in different
+ * components.  iterations = 1; tree_nodes = milestones; motion_checks =
+ * candidate edges.  PRM requests run on the server's second stream beside
+ * the RRT-Connect slices and reserve their roadmap's footprint in the pool. */
+#define OMPL_AMD_PLANNER_RRT_CONNECT 0
+#define OMPL_AMD_PLANNER_PRM 1
 
 typedef struct {
     int status;

@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <queue>
 #include <vector>
 
 #include "runtime.hpp"
@@ -61,7 +62,7 @@ struct PrmRoadmap {
     }
     ~PrmRoadmap()
     {
-        for (void *p : allocs) hipFree(p);
+        for (void *p : allocs) (void)hipFree(p);
     }
 };
 
@@ -196,14 +197,16 @@ static unsigned blocks_for(int64_t n, int threads = 256)
     return (unsigned)((n + threads - 1) / threads);
 }
 
-static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k,
-                  double lvs, unsigned factor, unsigned flags)
+// Builds the roadmap on stream s.  extra: n_extra states (host, row-major)
+// placed before the samples, so that a valid one becomes a milestone with the
+// lowest indices (a planning query's start and goal).
+void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams, int64_t n_samples, int k, double lvs,
+           unsigned factor, unsigned flags, hipStream_t s, const double *extra, int n_extra)
 {
     using Clock = std::chrono::steady_clock;
     auto ms = [](Clock::time_point a) {
         return std::chrono::duration<double, std::milli>(Clock::now() - a).count();
     };
-    hipStream_t s = default_stream();
     const auto t_all = Clock::now();
     auto check = [](int rc) {
         if (rc != OMPL_AMD_OK) fail(rc, "%s", last_error());
@@ -211,7 +214,8 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
     // 1. samples
     auto t0 = Clock::now();
     const int64_t per = (n_samples + n_streams - 1) / n_streams;
-    const int64_t n = per * n_streams;
+    const int64_t ns = per * n_streams;
+    const int64_t n = ns + n_extra;
     R.samples = n;
     R.samples_f64 = R.alloc<double>((size_t)n * kDim);
     R.samples_f32 = R.alloc<float>((size_t)n * kDim);
@@ -219,7 +223,28 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
     for (int i = 0; i < n_streams; ++i) seeds[(size_t)i] = seed + (uint32_t)i;
     const double pi = 3.141592653589793238462643383279502884;
     const double lo[kDim] = {-pi, -pi, -pi, -pi, -pi, -pi, -pi}, hi[kDim] = {pi, pi, pi, pi, pi, pi, pi};
-    check(ompl_amd_sample_uniform_dev(seeds.data(), n_streams, per, kDim, lo, hi, R.samples_f64, R.samples_f32, s));
+    if (n_extra == 0) {
+        check(ompl_amd_sample_uniform_dev(seeds.data(), n_streams, per, kDim, lo, hi, R.samples_f64, R.samples_f32,
+                                          s));
+    } else {
+        double *t64 = R.alloc<double>((size_t)ns * kDim);
+        float *t32 = R.alloc<float>((size_t)ns * kDim);
+        check(ompl_amd_sample_uniform_dev(seeds.data(), n_streams, per, kDim, lo, hi, t64, t32, s));
+        std::vector<float> e32((size_t)n_extra * kDim);
+        for (int i = 0; i < n_extra; ++i)
+            for (int d = 0; d < kDim; ++d) e32[(size_t)d * n_extra + i] = (float)extra[(size_t)i * kDim + d];
+        OA_HIP(hipMemcpyAsync(R.samples_f64, extra, (size_t)n_extra * kDim * sizeof(double), hipMemcpyHostToDevice,
+                              s));
+        OA_HIP(hipMemcpyAsync(R.samples_f64 + (size_t)n_extra * kDim, t64, (size_t)ns * kDim * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+        for (int d = 0; d < kDim; ++d) {
+            OA_HIP(hipMemcpyAsync(R.samples_f32 + (size_t)d * n, e32.data() + (size_t)d * n_extra,
+                                  (size_t)n_extra * sizeof(float), hipMemcpyHostToDevice, s));
+            OA_HIP(hipMemcpyAsync(R.samples_f32 + (size_t)d * n + n_extra, t32 + (size_t)d * ns,
+                                  (size_t)ns * sizeof(float), hipMemcpyDeviceToDevice, s));
+        }
+        OA_HIP(hipStreamSynchronize(s)); // e32 is a host temporary
+    }
     R.stats.ms_sample = ms(t0);
     // 2. validity + milestones
     t0 = Clock::now();
@@ -292,7 +317,8 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
     OA_HIP(hipStreamSynchronize(s));
     if (nu > 0) {
         uint64_t last = 0;
-        OA_HIP(hipMemcpy(&last, R.keys + nu - 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
+        OA_HIP(hipMemcpyAsync(&last, R.keys + nu - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        OA_HIP(hipStreamSynchronize(s));
         if (last == ~(uint64_t)0) --nu; // padding key of milestones with fewer than k neighbours
     }
     R.ne = nu;
@@ -387,6 +413,98 @@ static void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams,
     R.stats.ms_total = ms(t_all);
 }
 
+// Shortest path between milestones 0 and 1 over a symmetric CSR roadmap
+// (host): Dijkstra with FP64 RealVectorStateSpace distances as edge weights
+// (base/spaces/src/RealVectorStateSpace.cpp:230-242), nodes popped in
+// (distance, index) order and relaxed only on a strictly shorter distance.
+// The reference searches its roadmap with A* under the same weights
+// (PRM::constructSolution, geometric/planners/prm/src/PRM.cpp:696-727), which
+// finds a shortest path too.  Returns the node sequence 0 -> 1 (empty when 1 is unreachable).
+std::vector<int32_t> roadmap_shortest_path(const std::vector<int64_t> &off, const std::vector<int32_t> &adj,
+                                           const std::vector<double> &nodes)
+{
+    const int64_t nv = (int64_t)off.size() - 1;
+    std::vector<double> dist((size_t)nv, INFINITY);
+    std::vector<int32_t> prev((size_t)nv, -1);
+    using Item = std::pair<double, int32_t>;
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+    dist[0] = 0.0;
+    pq.push({0.0, 0});
+    while (!pq.empty()) {
+        const Item it = pq.top();
+        pq.pop();
+        const int32_t u = it.second;
+        if (it.first > dist[(size_t)u]) continue;
+        if (u == 1) break;
+        for (int64_t e = off[(size_t)u]; e < off[(size_t)u + 1]; ++e) {
+            const int32_t v = adj[(size_t)e];
+            double d2 = 0.0;
+            for (int k = 0; k < kDim; ++k) {
+                const double diff = nodes[(size_t)u * kDim + k] - nodes[(size_t)v * kDim + k];
+                d2 += diff * diff;
+            }
+            const double nd = it.first + std::sqrt(d2);
+            if (nd < dist[(size_t)v]) {
+                dist[(size_t)v] = nd;
+                prev[(size_t)v] = u;
+                pq.push({nd, v});
+            }
+        }
+    }
+    std::vector<int32_t> path;
+    if (nv < 2 || !std::isfinite(dist[1])) return path;
+    for (int32_t v = 1; v >= 0; v = prev[(size_t)v]) path.push_back(v);
+    std::reverse(path.begin(), path.end());
+    return path;
+}
+
+PrmQuery prm_query(ompl_amd_env env, const double start[kDim], const double goal[kDim], uint32_t seed, int n_streams,
+                   int64_t n_samples, int k, double lvs, unsigned factor, hipStream_t s)
+{
+    PrmQuery q;
+    PrmRoadmap R;
+    double extra[2 * kDim];
+    for (int d = 0; d < kDim; ++d) {
+        extra[d] = start[d];
+        extra[kDim + d] = goal[d];
+    }
+    build(R, env, seed, n_streams, n_samples, k, lvs, factor, 0, s, extra, 2);
+    q.milestones = R.nv;
+    q.candidate_edges = R.ne;
+    q.valid_edges = R.valid_edges;
+    if (R.nv < 2) return q;
+    int32_t first[2], lab[2];
+    OA_HIP(hipMemcpyAsync(first, R.milestone, sizeof(first), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipMemcpyAsync(lab, R.label, sizeof(lab), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipStreamSynchronize(s));
+    // start and goal are milestones 0 and 1 only when both are valid
+    // (PRM::solve takes valid start and goal states only)
+    if (first[0] != 0 || first[1] != 1 || lab[0] != lab[1]) return q;
+    std::vector<int64_t> off((size_t)R.nv + 1);
+    std::vector<int32_t> adj((size_t)(2 * R.valid_edges));
+    std::vector<double> nodes((size_t)R.nv * kDim);
+    OA_HIP(hipMemcpyAsync(off.data(), R.csr_off, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (!adj.empty())
+        OA_HIP(hipMemcpyAsync(adj.data(), R.csr_adj, adj.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipMemcpyAsync(nodes.data(), R.nodes, nodes.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipStreamSynchronize(s));
+    const std::vector<int32_t> p = roadmap_shortest_path(off, adj, nodes);
+    if (p.empty()) return q;
+    q.solved = true;
+    for (int32_t v : p) q.path.insert(q.path.end(), nodes.begin() + (size_t)v * kDim, nodes.begin() + (size_t)(v + 1) * kDim);
+    return q;
+}
+
+size_t prm_footprint(int64_t n_samples, int k)
+{
+    // samples (FP64 rows, FP32 SoA, validity, indices) and at most as many
+    // milestones (nodes FP64 + FP32, labels, neighbour lists) and k edges per
+    // milestone (keys x3, endpoints, FP64 motion endpoints, flags, CSR)
+    const size_t n = (size_t)n_samples + 2;
+    return n * (56 + 28 + 1 + 4 + 4 + 8) + n * (56 + 28 + 4 + 8 * (size_t)(k + 1)) +
+           n * (size_t)k * (24 + 8 + 112 + 1 + 4 + 16 + 8);
+}
+
 } // namespace ompl_amd
 
 struct ompl_amd_prm_s {
@@ -413,7 +531,7 @@ int ompl_amd_prm_build(ompl_amd_env env, uint32_t seed, int n_streams, int64_t n
         OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "bad segment settings");
         auto *h = new ompl_amd_prm_s();
         try {
-            build(h->R, env, seed, n_streams, n_samples, k, lvs, factor, flags);
+            build(h->R, env, seed, n_streams, n_samples, k, lvs, factor, flags, default_stream(), nullptr, 0);
         } catch (...) {
             delete h;
             throw;

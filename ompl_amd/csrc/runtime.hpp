@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <atomic>
 #include <mutex>
+#include <vector>
 
 #include "checker.hpp"
 #include "common.hpp"
@@ -97,6 +98,23 @@ struct Env {
     EnvView view() const { return EnvView{obs, ns, nb}; }
     EnvView input_view() const { return EnvView{obs_input, ns, nb}; }
 };
+
+// A planning query on a batch PRM roadmap (prm.hip): start and goal become
+// milestones 0 and 1 of a roadmap over n_samples uniform samples (n_streams
+// generators seeded seed, seed + 1, ...) with k neighbours per milestone; the
+// path is the shortest one between them over the valid edges.
+struct PrmQuery {
+    bool solved = false;
+    int64_t milestones = 0, candidate_edges = 0, valid_edges = 0;
+    std::vector<double> path; // row-major states
+};
+PrmQuery prm_query(ompl_amd_env env, const double start[7], const double goal[7], uint32_t seed, int n_streams,
+                   int64_t n_samples, int k, double lvs, unsigned factor, hipStream_t s);
+// Device bytes a prm_query of n_samples with k neighbours may hold at once (an upper estimate).
+size_t prm_footprint(int64_t n_samples, int k);
+// Shortest path between milestones 0 and 1 of a symmetric CSR roadmap.
+std::vector<int32_t> roadmap_shortest_path(const std::vector<int64_t> &off, const std::vector<int32_t> &adj,
+                                           const std::vector<double> &nodes);
 
 // Largest obstacle set staged in LDS by one block (float4 words).
 constexpr int kMaxEnvWords = 4096; // 64 KiB of dynamic LDS

@@ -858,8 +858,15 @@ struct Server {
     int64_t phase_ticks[4] = {0, 0, 0, 0}; // sample, nearest, check, step (under mu)
     int64_t clock_khz = 0;
     std::string fault;
+    // PRM requests: their own queue, worker thread and stream
+    std::deque<int64_t> prm_queue;
+    std::condition_variable prm_cv; // PRM arrivals and freed pool memory
+    std::thread prm_th;
+    hipStream_t prm_stream = nullptr;
+    int prm_running = 0;
 
     void run();
+    void run_prm();
     void admit_locked();
     void finish_locked(Planner *p, int status);
 };
@@ -878,6 +885,76 @@ void Server::finish_locked(Planner *p, int status)
         alloc.release(p->off, p->bytes);
         used_bytes -= p->bytes;
         p->bytes = 0;
+        prm_cv.notify_all(); // a PRM request may be waiting for memory
+    }
+}
+
+static size_t request_footprint(const ompl_amd_plan_request &r)
+{
+    return r.planner == OMPL_AMD_PLANNER_PRM ? prm_footprint(r.prm_samples, r.prm_k) : footprint(r.max_nodes);
+}
+
+// PRM requests, in arrival order, each once its roadmap's footprint is
+// reserved in the pool (the roadmap itself is built in memory of its own).
+void Server::run_prm()
+{
+    try {
+        OA_HIP(hipSetDevice(device()));
+        for (;;) {
+            Planner *p = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                for (;;) {
+                    if (stop) return;
+                    if (!fault.empty()) return;
+                    if (!prm_queue.empty()) {
+                        Planner *q = reqs[prm_queue.front()];
+                        const bool expired = q->req.deadline_ms > 0.0 && ms_since(q->t_submit) > q->req.deadline_ms;
+                        if (q->cancel || expired) {
+                            prm_queue.pop_front();
+                            finish_locked(q, q->cancel ? OMPL_AMD_PLAN_CANCELLED : OMPL_AMD_PLAN_DEADLINE);
+                            done_cv.notify_all();
+                            continue;
+                        }
+                        const size_t fp = request_footprint(q->req);
+                        size_t off = 0;
+                        if (alloc.alloc(fp, off)) {
+                            prm_queue.pop_front();
+                            q->off = off;
+                            q->bytes = fp;
+                            used_bytes += fp;
+                            q->status = OMPL_AMD_PLAN_RUNNING;
+                            q->started = true;
+                            q->t_start = Clock::now();
+                            ++prm_running;
+                            p = q;
+                            break;
+                        }
+                    }
+                    prm_cv.wait_for(lk, std::chrono::milliseconds(50));
+                }
+            }
+            const ompl_amd_plan_request &r = p->req;
+            const PrmQuery q = prm_query(r.env, r.start, r.goal, r.seed, 64, r.prm_samples, r.prm_k,
+                                         r.longest_valid_segment, r.factor, prm_stream);
+            std::lock_guard<std::mutex> lk(mu);
+            p->iterations = 1;
+            p->nodes = q.milestones;
+            p->checks = q.candidate_edges;
+            p->path = q.path;
+            --prm_running;
+            finish_locked(p, q.solved ? OMPL_AMD_PLAN_SOLVED : OMPL_AMD_PLAN_FAILED);
+            done_cv.notify_all();
+        }
+    } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (fault.empty()) fault = e.what();
+        for (auto &kv : reqs)
+            if (kv.second->req.planner == OMPL_AMD_PLANNER_PRM && kv.second->status < OMPL_AMD_PLAN_SOLVED)
+                finish_locked(kv.second, OMPL_AMD_PLAN_FAILED);
+        prm_queue.clear();
+        prm_running = 0;
+        done_cv.notify_all();
     }
 }
 
@@ -1027,9 +1104,12 @@ void Server::run()
         fault = e.what();
         for (int64_t id : active) finish_locked(reqs[id], OMPL_AMD_PLAN_FAILED);
         for (int64_t id : queue) finish_locked(reqs[id], OMPL_AMD_PLAN_FAILED);
+        for (int64_t id : prm_queue) finish_locked(reqs[id], OMPL_AMD_PLAN_FAILED);
         active.clear();
         queue.clear();
+        prm_queue.clear();
         done_cv.notify_all();
+        prm_cv.notify_all();
     }
 }
 
@@ -1125,6 +1205,104 @@ static int plain_grow(const Env &env, HostTree &t, bool start_tree, const double
     return reach ? kReached : kAdvanced;
 }
 
+// Plain PRM: the served PRM request's batch roadmap built on the host with one
+// GPU launch and synchronisation per isValid / checkMotion call: samples from
+// host std::mt19937 generators (the device sampler's streams), k-NN over the
+// milestones in FP32 with the device search's distance and (distance, index)
+// order, candidate edges checked from the lower milestone to the higher, host
+// union-find, and the same shortest path.
+static bool plain_prm(const ompl_amd_plan_request &req, const Env &env, ThreadResources &tr, std::vector<double> &out,
+                      int64_t &milestones, int64_t &checks)
+{
+    const int streams = 64;
+    const int samples = req.prm_samples > 0 ? req.prm_samples : 4096;
+    const int k = req.prm_k > 0 ? req.prm_k : 10;
+    const int64_t per = (samples + streams - 1) / streams;
+    const int64_t n = 2 + per * streams;
+    std::vector<double> smp((size_t)n * kDof);
+    for (int d = 0; d < kDof; ++d) {
+        smp[(size_t)d] = req.start[d];
+        smp[(size_t)kDof + d] = req.goal[d];
+    }
+    for (int st = 0; st < streams; ++st) {
+        std::mt19937 gen(req.seed + (uint32_t)st);
+        std::uniform_real_distribution<> uni{0, 1};
+        for (int64_t i = 0; i < per; ++i)
+            for (int d = 0; d < kDof; ++d)
+                smp[(size_t)(2 + st * per + i) * kDof + d] = (kPi - -kPi) * uni(gen) + -kPi;
+    }
+    std::vector<int32_t> ms;
+    for (int64_t r = 0; r < n; ++r) {
+        State1 q;
+        for (int d = 0; d < kDof; ++d) q.q[d] = smp[(size_t)r * kDof + d];
+        launch_validate_single(env.view(), q, tr.pinned_dev, tr.stream);
+        OA_HIP(hipStreamSynchronize(tr.stream));
+        if (tr.pinned[0] != 0) ms.push_back((int32_t)r);
+    }
+    const int64_t nv = (int64_t)ms.size();
+    milestones = nv;
+    checks = 0;
+    if (nv < 2 || ms[0] != 0 || ms[1] != 1) return false;
+    std::vector<double> nodes((size_t)nv * kDof);
+    std::vector<float> f32((size_t)nv * kDof);
+    for (int64_t i = 0; i < nv; ++i)
+        for (int d = 0; d < kDof; ++d) {
+            nodes[(size_t)i * kDof + d] = smp[(size_t)ms[(size_t)i] * kDof + d];
+            f32[(size_t)i * kDof + d] = (float)nodes[(size_t)i * kDof + d];
+        }
+    // k + 1 nearest (the milestone itself among them), first k others kept
+    const int64_t kk = std::min<int64_t>(k + 1, nv);
+    std::vector<uint64_t> keys;
+    std::vector<std::pair<float, int32_t>> cand((size_t)nv);
+    for (int64_t i = 0; i < nv; ++i) {
+        for (int64_t j = 0; j < nv; ++j) {
+            float dist = 0.0f;
+            for (int d = 0; d < kDof; ++d) {
+                const float diff = f32[(size_t)j * kDof + d] - f32[(size_t)i * kDof + d];
+                dist += diff * diff;
+            }
+            cand[(size_t)j] = {std::sqrt(dist), (int32_t)j};
+        }
+        std::partial_sort(cand.begin(), cand.begin() + kk, cand.end());
+        int taken = 0;
+        for (int64_t t = 0; t < kk && taken < k; ++t) {
+            const int64_t j = cand[(size_t)t].second;
+            if (j == i) continue;
+            keys.push_back(((uint64_t)std::min(i, j) << 32) | (uint64_t)std::max(i, j));
+            ++taken;
+        }
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    checks = (int64_t)keys.size();
+    std::vector<std::vector<int32_t>> nbr((size_t)nv);
+    for (uint64_t key : keys) {
+        const int32_t u = (int32_t)(key >> 32), v = (int32_t)(key & 0xffffffffu);
+        Edge1 E;
+        for (int d = 0; d < kDof; ++d) {
+            E.a[d] = nodes[(size_t)u * kDof + d];
+            E.b[d] = nodes[(size_t)v * kDof + d];
+        }
+        launch_motion_single(env.view(), E, req.longest_valid_segment, req.factor, tr.pinned_dev, tr.stream);
+        OA_HIP(hipStreamSynchronize(tr.stream));
+        if (tr.pinned[0] != 0) {
+            nbr[(size_t)u].push_back(v);
+            nbr[(size_t)v].push_back(u);
+        }
+    }
+    std::vector<int64_t> off((size_t)nv + 1, 0);
+    std::vector<int32_t> adj;
+    for (int64_t i = 0; i < nv; ++i) {
+        std::sort(nbr[(size_t)i].begin(), nbr[(size_t)i].end());
+        adj.insert(adj.end(), nbr[(size_t)i].begin(), nbr[(size_t)i].end());
+        off[(size_t)i + 1] = (int64_t)adj.size();
+    }
+    const std::vector<int32_t> path = roadmap_shortest_path(off, adj, nodes);
+    for (int32_t v : path)
+        out.insert(out.end(), nodes.begin() + (size_t)v * kDof, nodes.begin() + (size_t)(v + 1) * kDof);
+    return !path.empty();
+}
+
 } // namespace ompl_amd
 
 struct ompl_amd_server_s {
@@ -1176,7 +1354,9 @@ int ompl_amd_server_create(const ompl_amd_server_config *cfg, ompl_amd_server *o
         OA_HIP(hipHostMalloc((void **)&S.out_host, sizeof(PlanOut) * cfg->max_active, hipHostMallocDefault));
         OA_HIP(hipMalloc(&S.cmd_dev, sizeof(PlanCmd) * cfg->max_active));
         OA_HIP(hipMalloc(&S.out_dev, sizeof(PlanOut) * cfg->max_active));
+        OA_HIP(hipStreamCreateWithFlags(&S.prm_stream, hipStreamNonBlocking));
         S.th = std::thread([&S] { S.run(); });
+        S.prm_th = std::thread([&S] { S.run_prm(); });
         *out = h;
     });
 }
@@ -1191,7 +1371,9 @@ int ompl_amd_server_destroy(ompl_amd_server h)
             S.stop = true;
         }
         S.cv.notify_all();
+        S.prm_cv.notify_all();
         if (S.th.joinable()) S.th.join();
+        if (S.prm_th.joinable()) S.prm_th.join();
         for (auto &kv : S.reqs) {
             if (kv.second->env_held) kv.second->req.env->server_users.fetch_sub(1);
             delete kv.second;
@@ -1204,6 +1386,8 @@ int ompl_amd_server_destroy(ompl_amd_server h)
         (void)hipHostFree(S.cmd_host);
         (void)hipHostFree(S.out_host);
         (void)hipStreamDestroy(S.stream);
+        (void)hipStreamSynchronize(S.prm_stream);
+        (void)hipStreamDestroy(S.prm_stream);
         delete h;
     });
 }
@@ -1215,13 +1399,22 @@ int ompl_amd_server_submit(ompl_amd_server h, const ompl_amd_plan_request *req, 
         OA_REQUIRE(req->env, OMPL_AMD_ERR_ARG, "request without environment");
         OA_REQUIRE(req->max_iters >= 1 && req->longest_valid_segment > 0.0 && req->factor >= 1, OMPL_AMD_ERR_ARG,
                    "bad request parameters");
+        OA_REQUIRE(req->planner == OMPL_AMD_PLANNER_RRT_CONNECT || req->planner == OMPL_AMD_PLANNER_PRM,
+                   OMPL_AMD_ERR_ARG, "unknown planner %d", req->planner);
         const Env &env = req->env->env;
-        OA_REQUIRE(env_stage_bytes(env.ns, env.nb) <= kPlanMaxEnvBytes, OMPL_AMD_ERR_ARG,
+        const bool prm = req->planner == OMPL_AMD_PLANNER_PRM;
+        OA_REQUIRE(prm || env_stage_bytes(env.ns, env.nb) <= kPlanMaxEnvBytes, OMPL_AMD_ERR_ARG,
                    "environment too large for the server: %d spheres and %d boxes stage %zu bytes (limit %zu)",
                    env.ns, env.nb, env_stage_bytes(env.ns, env.nb), kPlanMaxEnvBytes);
         Server &S = h->srv;
         auto *p = new Planner();
         p->req = *req;
+        if (prm) {
+            if (p->req.prm_samples <= 0) p->req.prm_samples = 4096;
+            if (p->req.prm_k <= 0) p->req.prm_k = 10;
+            OA_REQUIRE(p->req.prm_k <= 63 && p->req.prm_samples <= (1 << 24), OMPL_AMD_ERR_ARG,
+                       "PRM request: prm_k <= 63 and prm_samples <= 2^24");
+        }
         if (p->req.max_nodes <= 0 || p->req.max_nodes > S.cfg.max_nodes) p->req.max_nodes = S.cfg.max_nodes;
         if (p->req.max_nodes < 2) p->req.max_nodes = 2;
         p->maxd = req->range > 0.0 ? req->range : 0.2 * 2.0 * kPi * std::sqrt(7.0);
@@ -1234,13 +1427,18 @@ int ompl_amd_server_submit(ompl_amd_server h, const ompl_amd_plan_request *req, 
         }
         const int64_t rid = S.next_id++;
         S.reqs[rid] = p;
-        if (footprint(p->req.max_nodes) > S.cfg.hbm_budget_bytes) {
+        if (request_footprint(p->req) > S.cfg.hbm_budget_bytes) {
             p->status = OMPL_AMD_PLAN_REJECTED;
             p->t_finish = p->t_submit;
         } else {
             p->req.env->server_users.fetch_add(1); // the environment must outlive the request
             p->env_held = true;
-            S.queue.push_back(rid);
+            if (prm) {
+                S.prm_queue.push_back(rid);
+                S.prm_cv.notify_all();
+            } else {
+                S.queue.push_back(rid);
+            }
         }
         *id = rid;
         S.cv.notify_all();
@@ -1303,6 +1501,8 @@ int ompl_amd_server_cancel(ompl_amd_server h, int64_t id)
         if (p->status == OMPL_AMD_PLAN_QUEUED) {
             auto q = std::find(S.queue.begin(), S.queue.end(), id);
             if (q != S.queue.end()) S.queue.erase(q);
+            auto qp = std::find(S.prm_queue.begin(), S.prm_queue.end(), id);
+            if (qp != S.prm_queue.end()) S.prm_queue.erase(qp);
             S.finish_locked(p, OMPL_AMD_PLAN_CANCELLED);
             S.done_cv.notify_all();
         }
@@ -1333,8 +1533,8 @@ int ompl_amd_server_stats(ompl_amd_server h, int64_t out[6])
         std::lock_guard<std::mutex> lk(S.mu);
         out[0] = S.slices.load();
         out[1] = S.steps.load();
-        out[2] = (int64_t)S.active.size();
-        out[3] = (int64_t)S.queue.size();
+        out[2] = (int64_t)S.active.size() + S.prm_running;
+        out[3] = (int64_t)(S.queue.size() + S.prm_queue.size());
         out[4] = (int64_t)S.used_bytes;
         out[5] = (int64_t)S.cfg.hbm_budget_bytes;
     });
@@ -1363,6 +1563,25 @@ int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *
         const Clock::time_point t0 = Clock::now();
         const Env &env = req->env->env;
         ThreadResources &tr = thread_resources();
+        if (req->planner == OMPL_AMD_PLANNER_PRM) {
+            std::vector<double> out;
+            int64_t nv = 0, checks = 0;
+            const bool ok = plain_prm(*req, env, tr, out, nv, checks);
+            const double ms = ms_since(t0);
+            res->status = ok ? OMPL_AMD_PLAN_SOLVED : OMPL_AMD_PLAN_FAILED;
+            res->iterations = 1;
+            res->path_len = (int)(out.size() / kDof);
+            res->tree_nodes = nv;
+            res->motion_checks = checks;
+            res->queued_ms = 0.0;
+            res->run_ms = ms;
+            res->latency_ms = ms;
+            if (path_len) *path_len = res->path_len;
+            if (path) std::copy(out.begin(), out.begin() + std::min<size_t>(out.size(), (size_t)cap * kDof), path);
+            return;
+        }
+        OA_REQUIRE(req->planner == OMPL_AMD_PLANNER_RRT_CONNECT, OMPL_AMD_ERR_ARG, "unknown planner %d",
+                   req->planner);
         std::mt19937 gen(req->seed);
         std::uniform_real_distribution<> uni{0, 1};
         const double maxd = req->range > 0.0 ? req->range : 0.2 * 2.0 * kPi * std::sqrt(7.0);

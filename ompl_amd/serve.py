@@ -33,6 +33,9 @@ class PlanRequest(C.Structure):
         ("factor", C.c_uint),
         ("deadline_ms", C.c_double),
         ("max_nodes", C.c_int),
+        ("planner", C.c_int),
+        ("prm_samples", C.c_int),
+        ("prm_k", C.c_int),
     ]
 
 
@@ -54,8 +57,15 @@ class PlanResult(C.Structure):
         return d
 
 
-def make_request(env, start, goal, seed, lvs, max_iters=5000, rng=0.0, factor=1, deadline_ms=0.0, max_nodes=0):
+RRT_CONNECT, PRM = 0, 1
+
+
+def make_request(env, start, goal, seed, lvs, max_iters=5000, rng=0.0, factor=1, deadline_ms=0.0, max_nodes=0,
+                 planner=RRT_CONNECT, prm_samples=0, prm_k=0):
     r = PlanRequest()
+    r.planner = int(planner)
+    r.prm_samples = int(prm_samples)
+    r.prm_k = int(prm_k)
     r.env = env.handle.value if hasattr(env, "handle") else env
     for k in range(DOF):
         r.start[k] = float(start[k])

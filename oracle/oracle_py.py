@@ -401,7 +401,7 @@ def kstar_k(n_milestones: int, dim: int = 7) -> int:
 
 
 def prm_batch(env: OracleEnv, seed: int, n_samples: int, k: int, lvs: float, factor: int = 1, n_streams: int = 1,
-              threads: int = 1, kstar: bool = False):
+              threads: int = 1, kstar: bool = False, extra=None):
     """Batch PRM roadmap: the steps of PRM::growRoadmap / addMilestone (geometric/planners/prm/src/
     PRM.cpp:364-385, 570-604) with KStrategy (geometric/planners/prm/ConnectionStrategy.h:56-81) over a
     fixed sample set.  Samples: stream s = mt19937(seed + s), n_samples rounded up to a multiple of
@@ -412,6 +412,8 @@ def prm_batch(env: OracleEnv, seed: int, n_samples: int, k: int, lvs: float, fac
     per = -(-n_samples // n_streams)
     lo, hi = np.full(7, -np.pi), np.full(7, np.pi)
     samples = np.concatenate([Mt19937(seed + s).sample_uniform(lo, hi, per) for s in range(n_streams)])
+    if extra is not None:  # query states placed before the samples (milestones 0, 1, ... when valid)
+        samples = np.concatenate([np.asarray(extra, np.float64).reshape(-1, 7), samples])
     valid = validate_states_mt(env, samples.T.astype(np.float32), threads).astype(bool)
     sidx = np.nonzero(valid)[0].astype(np.int32)
     nodes = samples[sidx]
@@ -476,3 +478,61 @@ class Gnat:
             self.close()
         except Exception:
             pass
+
+
+def roadmap_shortest_path(nv: int, edges: np.ndarray, valid: np.ndarray, nodes: np.ndarray):
+    """Shortest path from milestone 0 to milestone 1 over the valid edges, FP64
+    state distances as weights (the weight PRM::constructSolution's A* uses,
+    geometric/planners/prm/src/PRM.cpp:696-727); nodes popped in (distance,
+    index) order, relaxed only on a strictly shorter distance.  Node list, or
+    [] when 1 is unreachable."""
+    import heapq
+    import math
+
+    adj = [[] for _ in range(nv)]
+    for (u, v), ok in zip(edges.tolist(), valid.tolist()):
+        if ok:
+            adj[u].append(v)
+            adj[v].append(u)
+    for a in adj:
+        a.sort()
+    dist = [math.inf] * nv
+    prev = [-1] * nv
+    dist[0] = 0.0
+    pq = [(0.0, 0)]
+    while pq:
+        d, u = heapq.heappop(pq)
+        if d > dist[u]:
+            continue
+        if u == 1:
+            break
+        for v in adj[u]:
+            s = 0.0
+            for kk in range(7):
+                diff = float(nodes[u, kk]) - float(nodes[v, kk])
+                s += diff * diff
+            nd = d + math.sqrt(s)
+            if nd < dist[v]:
+                dist[v] = nd
+                prev[v] = u
+                heapq.heappush(pq, (nd, v))
+    if nv < 2 or not math.isfinite(dist[1]):
+        return []
+    path, v = [], 1
+    while v >= 0:
+        path.append(v)
+        v = prev[v]
+    return path[::-1]
+
+
+def prm_plan(env: OracleEnv, start, goal, seed: int, n_samples: int, k: int, lvs: float, factor: int = 1,
+             n_streams: int = 64, threads: int = 1):
+    """A served PRM request: start and goal as milestones 0 and 1 of the batch
+    roadmap, then the shortest path.  -> (solved, path [n][7], milestones, candidate edges)."""
+    r = prm_batch(env, seed, n_samples, k, lvs, factor, n_streams, threads, extra=np.stack([start, goal]))
+    idx = r["sample_index"]
+    nv = len(idx)
+    if nv < 2 or idx[0] != 0 or idx[1] != 1:
+        return False, np.zeros((0, 7)), nv, len(r["edges"])
+    p = roadmap_shortest_path(nv, r["edges"], r["edge_valid"], r["milestones"])
+    return bool(p), r["milestones"][p] if p else np.zeros((0, 7)), nv, len(r["edges"])

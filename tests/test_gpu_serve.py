@@ -324,3 +324,66 @@ def test_environment_in_use_cannot_be_destroyed(gpu, scene):
     env.close()
     assert env.handle is None
     srv.close()
+
+
+def test_served_prm_requests_equal_oracle(gpu, oracle, scene):
+    """PRM requests: start and goal as milestones 0 and 1 of a batch roadmap,
+    shortest path over its valid edges.  Served path, milestone count and
+    checked candidate edges equal the oracle's; the plain host PRM (one launch
+    per check) gives the same; an invalid goal fails."""
+    from ompl_amd.serve import FAILED, PRM, SOLVED, Server, make_request, plan_plain
+
+    env, oenv, pairs = scene
+    lvs = lvs_config()
+    reqs = [make_request(env, a, b, 700 + i, lvs, planner=PRM, prm_samples=1500, prm_k=10)
+            for i, (a, b) in enumerate(pairs[:6])]
+    bad = gpu.workload.configs(5, 4000)
+    bad = next(q for q in bad if not oracle.state_valid(oenv, q.astype(np.float32)))
+    reqs.append(make_request(env, pairs[0][0], bad, 800, lvs, planner=PRM, prm_samples=500, prm_k=10))
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=8, max_nodes=2048)
+    ids = [srv.submit(r) for r in reqs]
+    res = [srv.wait(rid, 60000) for rid in ids]
+    paths = [srv.path(rid) for rid in ids]
+    srv.close()
+    solved = 0
+    for i, (r, p) in enumerate(zip(res, paths)):
+        goal = pairs[i][1] if i < 6 else bad
+        ok, opath, nv, ne = oracle.prm_plan(oenv, pairs[min(i, 5) if i < 6 else 0][0], goal, 700 + i if i < 6 else 800,
+                                            1500 if i < 6 else 500, 10, lvs, threads=8)
+        print(f"prm request {i}: status {r.status} milestones {r.tree_nodes} oracle {ok} {nv}", flush=True)
+        assert (r.status == SOLVED) == ok and r.iterations == 1
+        assert r.tree_nodes == nv and r.motion_checks == ne
+        assert np.array_equal(p, opath)
+        solved += ok
+    assert res[-1].status == FAILED and solved >= 3
+    for i in (0, 1):
+        pres, ppath = plan_plain(reqs[i])
+        assert pres.status == res[i].status and pres.tree_nodes == res[i].tree_nodes
+        assert pres.motion_checks == res[i].motion_checks and np.array_equal(ppath, paths[i])
+
+
+def test_rrt_connect_and_prm_requests_alone_and_in_mix(gpu, scene):
+    """RRT-Connect and PRM requests served together (PRM on the server's second
+    stream beside the RRT-Connect slices) give the results each gives alone."""
+    from ompl_amd.serve import PRM, Server, make_request
+
+    env, _, pairs = scene
+    lvs = lvs_config()
+    reqs = []
+    for i, (a, b) in enumerate(pairs):
+        reqs.append(make_request(env, a, b, 900 + i, lvs, max_iters=3000,
+                                 planner=PRM if i % 3 == 0 else 0, prm_samples=1200))
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=8, max_nodes=4096)
+    alone = []
+    for r in reqs:
+        rid = srv.submit(r)
+        x = srv.wait(rid, 60000)
+        alone.append((x.status, x.iterations, x.tree_nodes, x.motion_checks, srv.path(rid)))
+    ids = [srv.submit(r) for r in reqs]
+    for i, rid in enumerate(ids):
+        x = srv.wait(rid, 60000)
+        m = (x.status, x.iterations, x.tree_nodes, x.motion_checks, srv.path(rid))
+        assert alone[i][:4] == m[:4] and np.array_equal(alone[i][4], m[4]), i
+    st = srv.stats()
+    assert st["active"] == 0 and st["queued"] == 0 and st["used_bytes"] == 0
+    srv.close()

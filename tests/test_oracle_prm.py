@@ -65,3 +65,38 @@ def test_kstar_k_known_values(oracle):
     assert oracle.kstar_k(2) == 3
     assert oracle.kstar_k(1) == 1
     assert oracle.kstar_k(10 ** 12) == 63
+
+
+def test_roadmap_shortest_path_against_scipy(oracle):
+    """The oracle's shortest path on a roadmap (PRM request answers) is a
+    chain of valid edges from milestone 0 to 1 whose FP64 length equals
+    scipy's Dijkstra distance; unreachable goal -> empty."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import dijkstra
+
+    rng = np.random.default_rng(4)
+    for trial in range(6):
+        nv = 300
+        nodes = rng.uniform(-3, 3, (nv, 7))
+        e = set()
+        for i in range(nv):
+            for j in rng.choice(nv, 4, replace=False):
+                if i != j:
+                    e.add((min(i, int(j)), max(i, int(j))))
+        edges = np.array(sorted(e), np.int32)
+        valid = rng.uniform(size=len(edges)) < (0.8 if trial < 5 else 0.05)
+        w = np.sqrt(((nodes[edges[:, 0]] - nodes[edges[:, 1]]) ** 2).sum(1))
+        ev = edges[valid]
+        g = coo_matrix((w[valid], (ev[:, 0], ev[:, 1])), shape=(nv, nv))
+        dist = dijkstra(g, directed=False, indices=0)
+        path = oracle.roadmap_shortest_path(nv, edges, valid, nodes)
+        if not np.isfinite(dist[1]):
+            assert path == []
+            continue
+        assert path[0] == 0 and path[-1] == 1
+        keys = set(map(tuple, ev.tolist()))
+        length = 0.0
+        for a, b in zip(path[:-1], path[1:]):
+            assert (min(a, b), max(a, b)) in keys
+            length += float(np.sqrt(((nodes[a] - nodes[b]) ** 2).sum()))
+        assert abs(length - dist[1]) < 1e-9 * max(1.0, dist[1])
