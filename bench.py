@@ -361,9 +361,10 @@ def gnat_baseline(oracle_py, metric, db, qs, k, radius, threads, gpu_idx, second
     return out
 
 
-def scope_rows(oa, run_cpu, cpu_threads):
+def scope_rows(oa, run_cpu, cpu_threads, only=None):
     """Throughput of the other rows of the hot path, each on its configuration,
-    with its algorithmic work and a bounded CPU-restatement baseline."""
+    with its algorithmic work and a bounded CPU-restatement baseline.  only:
+    names of the rows to run (None: all)."""
     import torch
 
     from ompl_amd.nn import NearestNeighborsGPU
@@ -378,7 +379,26 @@ def scope_rows(oa, run_cpu, cpu_threads):
         a = flops / (ms * 1e-3) / 1e12
         return {"achieved_tflops": a, "frac": a / peak, "algorithmic_gflop": flops / 1e9}
 
+    def want(name):
+        return only is None or name in only
+
     # validity: 65,536 configurations U[-pi,pi]^7 against 64 spheres
+    if want("validity_64_spheres"):
+        validity_row(oa, rows, oracle_py, cpu_threads, roof, timed)
+    if want("knn_l2_100k"):
+        knn_row(oa, rows, oracle_py, cpu_threads, roof, timed, NearestNeighborsGPU)
+    if want("se3_1m"):
+        se3_row(oa, rows, oracle_py, cpu_threads, roof, timed, NearestNeighborsGPU)
+    if want("sampling"):
+        sampling_rows(oa, rows, oracle_py, timed, sample_uniform, sample_valid)
+    if want("state_lists"):
+        state_list_row(oa, rows, oracle_py, timed)
+    return rows
+
+
+def validity_row(oa, rows, oracle_py, cpu_threads, roof, timed):
+    import torch
+
     s, b = oa.workload.obstacles(1000, 64, 0)
     env = oa.Env(s, b)
     q = oa.workload.configs(1, 65536)
@@ -406,6 +426,10 @@ def scope_rows(oa, run_cpu, cpu_threads):
         row["parity_bit_exact"] = bool(np.array_equal(ref, out.cpu().numpy()))
     rows["validity_64_spheres"] = row
 
+
+def knn_row(oa, rows, oracle_py, cpu_threads, roof, timed, NearestNeighborsGPU):
+    import torch
+
     # k-NN, R^7 L2: 100,000 points, 4,096 queries, k = 10
     db = oa.workload.configs(2, 100_000).astype(np.float32)
     qs = oa.workload.configs(3, 4096).astype(np.float32)
@@ -427,6 +451,10 @@ def scope_rows(oa, run_cpu, cpu_threads):
         row["cpu_baseline"] = gnat_baseline(oracle_py, 0, db, qs, 10, None, cpu_threads, oi.cpu().numpy())
     nn.close()
     rows["knn_l2_100k"] = row
+
+
+def se3_row(oa, rows, oracle_py, cpu_threads, roof, timed, NearestNeighborsGPU):
+    import torch
 
     # SE3 at scale: 1,000,000 points, 65,536 queries, k = 16 and the RRT* radius
     db = oa.workload.se3(5, 1_000_000)
@@ -457,6 +485,8 @@ def scope_rows(oa, run_cpu, cpu_threads):
     nn.close()
     rows["se3_1m"] = row
 
+
+def sampling_rows(oa, rows, oracle_py, timed, sample_uniform, sample_valid):
     # uniform sampling: 256 generators x 782 states (200,192 states)
     lo, hi = np.full(7, -np.pi), np.full(7, np.pi)
     seeds = np.arange(256, dtype=np.uint32) + PRM_SEED
@@ -491,7 +521,13 @@ def scope_rows(oa, run_cpu, cpu_threads):
         rows["sample_valid"]["cpu_baseline"] = {"calls_per_s": 8 * 64 / dt, "cores": 1, "kind": "port",
                                                 "sample": f"8 of the 256 samplers, {dt:.2f} s"}
 
+
+def state_list_row(oa, rows, oracle_py, timed):
+    import torch
+
     # precomputed state lists: the motion states of the 16,384 benchmark edges, first 64 each
+    s2, b2 = oa.workload.obstacles(ENV_SEED, N_SPHERES, N_BOXES)
+    env2 = oa.Env(s2, b2)
     a_rows, b_rows = oa.workload.edges(EDGE_SEED, 4096)
     t = np.linspace(0.0, 1.0, 64)
     st = (a_rows[:, None, :] + (b_rows - a_rows)[:, None, :] * t[None, :, None]).reshape(-1, 7)
@@ -539,6 +575,9 @@ def main(argv=None):
     ap.add_argument("--prm-samples", type=int, default=PRM_SAMPLES, help="PRM roadmap batch size (0 = skip)")
     ap.add_argument("--prm-cpu-samples", type=int, default=20000)
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements")
+    ap.add_argument("--rows", type=str, default=None,
+                    help="comma-separated rows to measure (validity_64_spheres, knn_l2_100k, se3_1m, sampling, "
+                         "state_lists; default all)")
     ap.add_argument("--benchmark-log", type=str, default=None,
                     help="also write the served stream's runs as an OMPL benchmark log file")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -793,7 +832,8 @@ def main(argv=None):
     rows_res = None
     if not args.no_rows:
         oa.set_tuning(**t)  # the batch's tuned shapes (the PRM leg tunes its own)
-        rows_res = scope_rows(oa, not args.no_cpu, args.cpu_threads or cpu_threads_default())
+        rows_res = scope_rows(oa, not args.no_cpu, args.cpu_threads or cpu_threads_default(),
+                              only=set(args.rows.split(",")) if args.rows else None)
 
     res = {
         "metric": "edge-checks/s + p50/p99 plan latency, served vs plain GPU path, bit-exact",

@@ -273,12 +273,33 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
     const int64_t t0 = (int64_t)split * A.tiles_per_split;
     const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
     const int64_t n_words = A.n * DIM;
-    float cur[KSTEPS];
-    if (t0 < t1) tile_load<KSTEPS>(A.xt, t0, lane, cur);
+    // A tile's exact coordinates (32 x DIM words, PW per lane) and removal
+    // flags are read one tile ahead with its MFMA operands, so a tile with
+    // candidates -- nearly every tile while the lists fill -- stages them in
+    // LDS without waiting on a global read.
+    constexpr int PW = (32 * DIM + 63) / 64;
+    auto pts_load = [&](int64_t t, float v[PW], uint8_t &rm) {
+        const int64_t w0 = t * 32 * DIM;
+#pragma unroll
+        for (int w = 0; w < PW; ++w) {
+            const int i = lane + 64 * w;
+            v[w] = (i < 32 * DIM && w0 + i < n_words) ? A.pts[w0 + i] : 0.0f;
+        }
+        rm = (lane < 32 && t * 32 + lane < A.n) ? A.removed[t * 32 + lane] : (uint8_t)1;
+    };
+    float cur[KSTEPS], cpts[PW];
+    uint8_t crm = 1;
+    if (t0 < t1) {
+        tile_load<KSTEPS>(A.xt, t0, lane, cur);
+        pts_load(t0, cpts, crm);
+    }
     for (int64_t t = t0; t < t1; ++t) {
         // the next tile's operands are in flight while this one is filtered
-        float nxt[KSTEPS];
-        tile_load<KSTEPS>(A.xt, t + 1 < t1 ? t + 1 : t, lane, nxt);
+        float nxt[KSTEPS], npts[PW];
+        uint8_t nrm;
+        const int64_t tn = t + 1 < t1 ? t + 1 : t;
+        tile_load<KSTEPS>(A.xt, tn, lane, nxt);
+        pts_load(tn, npts, nrm);
         const f32x16 acc = tile_mma_regs<KSTEPS>(cur, Q.b);
 #pragma unroll
         for (int s2 = 0; s2 < KSTEPS; ++s2) cur[s2] = nxt[s2];
@@ -286,11 +307,21 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
 #pragma unroll
         for (int j = 0; j < 16; ++j) mask |= (acc[j] <= thr ? 1u : 0u) << j;
         if (!Q.live) mask = 0;
-        if (__ballot(mask != 0) == 0) continue; // wave-uniform
+        if (__ballot(mask != 0) == 0) { // wave-uniform
+#pragma unroll
+            for (int w = 0; w < PW; ++w) cpts[w] = npts[w];
+            crm = nrm;
+            continue;
+        }
         wave_lds_sync(); // every lane is done with the previous tile's points
-        const int64_t w0 = t * 32 * DIM;
-        for (int i = lane; i < 32 * DIM; i += 64) tp[i] = w0 + i < n_words ? A.pts[w0 + i] : 0.0f;
-        if (lane < 32) trm[lane] = t * 32 + lane < A.n ? A.removed[t * 32 + lane] : (uint8_t)1;
+#pragma unroll
+        for (int w = 0; w < PW; ++w) {
+            const int i = lane + 64 * w;
+            if (i < 32 * DIM) tp[i] = cpts[w];
+            cpts[w] = npts[w];
+        }
+        if (lane < 32) trm[lane] = crm;
+        crm = nrm;
         wave_lds_sync();
         while (mask) {
             const int j = __builtin_ctz(mask);
@@ -523,6 +554,7 @@ static void launch_knn_t(const KnnArgs &A, int kmax, int64_t waves, hipStream_t 
 {
     const unsigned blocks = (unsigned)((waves + 3) / 4);
     switch (kmax) {
+    case 10: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 10>), dim3(blocks), dim3(256), 0, s, A); break;
     case 16: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 16>), dim3(blocks), dim3(256), 0, s, A); break;
     case 32: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 32>), dim3(blocks), dim3(256), 0, s, A); break;
     default: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 64>), dim3(blocks), dim3(256), 0, s, A); break;
@@ -605,6 +637,9 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
     OA_REQUIRE(q && out_i && out_d, OMPL_AMD_ERR_ARG, "null pointer");
     OA_REQUIRE(x.n < INT32_MAX, OMPL_AMD_ERR_ARG, "too many points for int32 ids");
     const int kmax = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
+    // the search keeps a 10-slot list for k <= 10 (PRM's default k): fewer
+    // slots per insertion; the split merge keeps its 16
+    const int kmax_search = k <= 10 ? 10 : kmax;
     KnnArgs A = base_args(x, q, nq);
     A.k = k;
     if (x.n == 0) {
@@ -615,7 +650,7 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
         A.out_d = out_d;
         A.out_i = out_i;
         const int64_t qtiles = (nq + 31) / 32;
-        dispatch_search<0>(x, A, kmax, nullptr, qtiles, s, false);
+        dispatch_search<0>(x, A, kmax_search, nullptr, qtiles, s, false);
         OA_HIP(hipGetLastError());
         return;
     }
@@ -627,13 +662,13 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
     if (splits == 1) {
         A.out_d = out_d;
         A.out_i = out_i;
-        dispatch_search<0>(x, A, kmax, nullptr, qtiles, s, false);
+        dispatch_search<0>(x, A, kmax_search, nullptr, qtiles, s, false);
     } else {
         const size_t per = (size_t)splits * nq * k;
         char *buf = (char *)scratch(x, per * (sizeof(float) + sizeof(int32_t)));
         A.out_d = (float *)buf;
         A.out_i = (int32_t *)(buf + per * sizeof(float));
-        dispatch_search<0>(x, A, kmax, nullptr, qtiles * splits, s, false);
+        dispatch_search<0>(x, A, kmax_search, nullptr, qtiles * splits, s, false);
         const unsigned blocks = (unsigned)((nq + 3) / 4); // one wave per query
         switch (kmax) {
         case 16:
