@@ -313,9 +313,12 @@ int ompl_amd_nn_build(ompl_amd_nn nn, void *stream);
  * (-1 past the end), out_dist_dev [nq][k] (+inf past the end). */
 int ompl_amd_nn_knn_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, int k, int32_t *out_idx_dev,
                         float *out_dist_dev, void *stream);
-/* Radius search.  Two-pass CSR: out_offsets_dev [nq+1] is always written; if
- * out_idx_dev is non-null it must hold out_offsets[nq] entries (call once
- * with nulls to size it, or pass capacity >= total). */
+/* Radius search, CSR results sorted by (distance, id) per query:
+ * out_offsets_dev [nq+1] and *total are always written.  With null outputs the
+ * call only counts.  With outputs of `capacity` entries the call counts and
+ * writes in one pass over the points (a second pass when some query has more
+ * than 64 results); if capacity < total it fails with OMPL_AMD_ERR_ARG after
+ * writing the offsets and *total, and the outputs are untouched. */
 int ompl_amd_nn_radius_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, float radius,
                            int64_t *out_offsets_dev, int32_t *out_idx_dev, float *out_dist_dev, int64_t capacity,
                            int64_t *total, void *stream);

@@ -167,8 +167,9 @@ struct KnnArgs {
     float *out_d;    // final [nq][k] or partial [split][nq][k]
     int32_t *out_i;
     float radius;    // radius search
-    int64_t *counts; // radius pass 1: per-query counts; pass 2: per-query cursors
+    int64_t *counts; // radius pass 0 / 2: per-query counts; pass 1: per-query cursors
     const int64_t *offsets;
+    int slot_cap;    // radius pass 2: keys kept per query in its fixed slot
 };
 
 // Row of the 32x32 MFMA output held in element j of a lane in half h.
@@ -406,7 +407,9 @@ __global__ void __launch_bounds__(256) knn_merge_kernel(const float *__restrict_
     }
 }
 
-// Radius search: pass 0 counts, pass 1 writes (idx, dist) into CSR slots.
+// Radius search: pass 0 counts; pass 1 writes (idx, dist) keys into CSR
+// slots; pass 2 counts and writes the first slot_cap keys of each query into
+// a fixed slot per query (one pass when no query has more).
 template <int METRIC, int DIM, int KSTEPS, int PASS>
 __global__ void __launch_bounds__(256) radius_kernel(KnnArgs A, uint64_t *__restrict__ keys)
 {
@@ -437,11 +440,15 @@ __global__ void __launch_bounds__(256) radius_kernel(KnnArgs A, uint64_t *__rest
             if (p >= A.n || A.removed[p]) continue;
             const float d = exact_dist<METRIC, DIM>(A.pts + p * DIM, Q.q);
             if (d <= r) {
+                const uint64_t key = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)p;
                 if (PASS == 0) {
                     ++found;
-                } else {
+                } else if (PASS == 1) {
                     const int64_t slot = A.offsets[Q.qi] + atomicAdd((unsigned long long *)&A.counts[Q.qi], 1ull);
-                    keys[slot] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)p;
+                    keys[slot] = key;
+                } else {
+                    const unsigned long long c = atomicAdd((unsigned long long *)&A.counts[Q.qi], 1ull);
+                    if (c < (unsigned long long)A.slot_cap) keys[Q.qi * A.slot_cap + (int64_t)c] = key;
                 }
             }
         }
@@ -483,6 +490,56 @@ __global__ void build_xt_kernel(const float *__restrict__ pts, int dim, int tdim
     if (real) atomicMax(xn_max, __float_as_uint(xn));
 }
 
+// Each query's radius keys (distance bits << 32 | index; unique per query)
+// sorted ascending, one wave per query: segments of up to kRadiusSortMax keys
+// in the wave's LDS slice by a bitonic network (padding keys ~0 sort last).
+// Most queries hold a handful of keys, which a device-wide segmented radix
+// sort handles poorly.
+constexpr int kRadiusSortMax = 512;
+
+// in_stride: 0 = query keys at [off[q], off[q+1]) of `in`; else at
+// [q * in_stride, q * in_stride + count) (the fixed slots of radius pass 2).
+__global__ void __launch_bounds__(256) radius_sort_kernel(const uint64_t *__restrict__ in,
+                                                          const int64_t *__restrict__ off, int64_t nq,
+                                                          int64_t in_stride, uint64_t *__restrict__ out)
+{
+    __shared__ uint64_t buf[4][kRadiusSortMax];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t qi = (int64_t)blockIdx.x * 4 + w;
+    if (qi >= nq) return; // wave-uniform; the kernel uses no block barrier
+    const int64_t b = off[qi];
+    const int m = (int)(off[qi + 1] - b);
+    if (m == 0) return;
+    int np2 = 1;
+    while (np2 < m) np2 <<= 1;
+    uint64_t *sb = buf[w];
+    const int64_t src = in_stride ? qi * in_stride : b;
+    for (int i = lane; i < np2; i += 64) sb[i] = i < m ? in[src + i] : ~0ull;
+    wave_lds_sync();
+    for (int k = 2; k <= np2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < np2; i += 64) {
+                const int ixj = i ^ j;
+                if (ixj > i) { // each pair is handled by one lane
+                    const uint64_t a = sb[i], c = sb[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > c) == up) {
+                        sb[i] = c;
+                        sb[ixj] = a;
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+    for (int i = lane; i < m; i += 64) out[b + i] = sb[i];
+}
+
+__global__ void max_i64_kernel(const int64_t *__restrict__ v, int64_t n, unsigned long long *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicMax(out, (unsigned long long)v[i]);
+}
+
 __global__ void fill_i64_kernel(int64_t *p, int64_t n, int64_t v)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,6 +558,24 @@ static void *scratch(NNIndex &x, size_t bytes)
         if (e != hipSuccess) fail(OMPL_AMD_ERR_NOMEM, "nn scratch allocation of %zu bytes failed", bytes);
         x.scratch_bytes = bytes;
     }
+    return x.scratch;
+}
+
+// Grows the scratch buffer to `bytes`, keeping its first `keep` bytes (the
+// copy is finished before the old buffer is freed).
+static void *scratch_keep(NNIndex &x, size_t bytes, size_t keep, hipStream_t s)
+{
+    if (x.scratch_bytes >= bytes) return x.scratch;
+    void *fresh = nullptr;
+    hipError_t e = hipMalloc(&fresh, bytes);
+    if (e != hipSuccess) fail(OMPL_AMD_ERR_NOMEM, "nn scratch allocation of %zu bytes failed", bytes);
+    if (x.scratch) {
+        if (keep) OA_HIP(hipMemcpyAsync(fresh, x.scratch, keep, hipMemcpyDeviceToDevice, s));
+        OA_HIP(hipStreamSynchronize(s));
+        OA_HIP(hipFree(x.scratch));
+    }
+    x.scratch = fresh;
+    x.scratch_bytes = bytes;
     return x.scratch;
 }
 
@@ -688,6 +763,12 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
     OA_HIP(hipGetLastError());
 }
 
+// Keys kept per query by the one-pass radius search (pass 2), and the most
+// slot memory it may take; a query with more results sends the call to the
+// two-pass path (count, then write into CSR slots).
+constexpr int kRadiusSlotCap = 64;
+constexpr size_t kRadiusSlotBytesMax = (size_t)256 << 20;
+
 static void radius(NNIndex &x, const float *q, int64_t nq, float r, int64_t *offsets, int32_t *out_i, float *out_d,
                    int64_t capacity, int64_t *total, hipStream_t s)
 {
@@ -699,6 +780,7 @@ static void radius(NNIndex &x, const float *q, int64_t nq, float r, int64_t *off
         return;
     }
     OA_REQUIRE(q, OMPL_AMD_ERR_ARG, "null queries");
+    const bool want = out_i || out_d;
     KnnArgs A = base_args(x, q, nq);
     A.radius = r;
     const int64_t n_tiles = A.n_tiles;
@@ -706,49 +788,79 @@ static void radius(NNIndex &x, const float *q, int64_t nq, float r, int64_t *off
     A.splits = (int)splits;
     A.tiles_per_split = n_tiles ? (n_tiles + splits - 1) / splits : 0;
     const int64_t qtiles = (nq + 31) / 32;
-    // scratch: counts [nq], cub temp, keys (pass 2)
+    // one pass (count and write the first kRadiusSlotCap keys of each query
+    // into its own slot) when results are wanted and the slots fit
+    const bool one_pass = want && n_tiles && (size_t)nq * kRadiusSlotCap * sizeof(uint64_t) <= kRadiusSlotBytesMax;
+    // scratch: counts [nq + 1] and the largest count (256-byte aligned), cub
+    // temp, slots (one pass)
     size_t cub_bytes = 0;
     OA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, (int64_t *)nullptr, (int64_t *)nullptr,
                                             (int)(nq + 1), s));
-    const size_t counts_bytes = (size_t)(nq + 1) * sizeof(int64_t);
-    char *buf = (char *)scratch(x, counts_bytes + cub_bytes + 256);
+    const size_t counts_bytes = ((size_t)(nq + 2) * sizeof(int64_t) + 255) / 256 * 256;
+    const size_t cub_span = (cub_bytes + 255) / 256 * 256;
+    const size_t slot_bytes = one_pass ? (size_t)nq * kRadiusSlotCap * sizeof(uint64_t) : 0;
+    char *buf = (char *)scratch(x, counts_bytes + cub_span + slot_bytes);
     int64_t *counts = (int64_t *)buf;
-    void *cub_tmp = buf + ((counts_bytes + 255) / 256) * 256;
+    unsigned long long *max_count = (unsigned long long *)(counts + nq + 1);
+    void *cub_tmp = buf + counts_bytes;
+    uint64_t *slots = one_pass ? (uint64_t *)(buf + counts_bytes + cub_span) : nullptr;
     ProfScope ps("nn_radius", s);
-    hipLaunchKernelGGL(fill_i64_kernel, dim3((unsigned)((nq + 1 + 255) / 256)), dim3(256), 0, s, counts, nq + 1,
+    hipLaunchKernelGGL(fill_i64_kernel, dim3((unsigned)((nq + 2 + 255) / 256)), dim3(256), 0, s, counts, nq + 2,
                        (int64_t)0);
     A.counts = counts;
-    if (n_tiles) dispatch_search<0>(x, A, 16, nullptr, qtiles * splits, s, true);
+    if (one_pass) {
+        A.slot_cap = kRadiusSlotCap;
+        dispatch_search<2>(x, A, 16, slots, qtiles * splits, s, true);
+    } else if (n_tiles) {
+        dispatch_search<0>(x, A, 16, nullptr, qtiles * splits, s, true);
+    }
+    hipLaunchKernelGGL(max_i64_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, counts, nq, max_count);
     OA_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp, cub_bytes, counts, offsets, (int)(nq + 1), s));
     int64_t tot = 0;
+    unsigned long long longest = 0;
     OA_HIP(hipMemcpyAsync(&tot, offsets + nq, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    OA_HIP(hipMemcpyAsync(&longest, max_count, sizeof(longest), hipMemcpyDeviceToHost, s));
     OA_HIP(hipStreamSynchronize(s));
     *total = tot;
-    if ((!out_i && !out_d) || tot == 0) return;
+    if (!want || tot == 0) return;
     OA_REQUIRE(capacity >= tot, OMPL_AMD_ERR_ARG, "radius output capacity %lld < %lld results", (long long)capacity,
                (long long)tot);
     OA_REQUIRE(tot < INT32_MAX, OMPL_AMD_ERR_ARG, "too many radius results for one call");
-    // pass 2: keys (dist bits << 32 | idx), per-query segmented sort
+    // keys (dist bits << 32 | idx) sorted per query: one wave per query when
+    // every query holds at most kRadiusSortMax keys, else a segmented radix sort
+    const bool from_slots = one_pass && longest <= (unsigned long long)kRadiusSlotCap;
+    const bool wave_sort = longest <= (unsigned long long)kRadiusSortMax;
     size_t sort_bytes = 0;
-    OA_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, sort_bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                                      (int)tot, (int)nq, (int64_t *)nullptr, (int64_t *)nullptr, 0,
-                                                      64, s));
-    const size_t keys_bytes = (size_t)tot * sizeof(uint64_t);
-    const size_t need = counts_bytes + 256 + keys_bytes * 2 + sort_bytes + 512;
-    buf = (char *)scratch(x, need > counts_bytes + cub_bytes + 256 ? need : counts_bytes + cub_bytes + 256);
+    if (!wave_sort)
+        OA_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, sort_bytes, (uint64_t *)nullptr,
+                                                          (uint64_t *)nullptr, (int)tot, (int)nq, (int64_t *)nullptr,
+                                                          (int64_t *)nullptr, 0, 64, s));
+    const size_t keys_span = ((size_t)tot * sizeof(uint64_t) + 255) / 256 * 256;
+    const size_t head = counts_bytes + cub_span + slot_bytes;
+    buf = (char *)scratch_keep(x, head + 2 * keys_span + sort_bytes, head, s);
     counts = (int64_t *)buf;
-    char *p = buf + ((counts_bytes + 255) / 256) * 256;
-    uint64_t *keys = (uint64_t *)p;
-    p += ((keys_bytes + 255) / 256) * 256;
-    uint64_t *keys_sorted = (uint64_t *)p;
-    p += ((keys_bytes + 255) / 256) * 256;
-    void *sort_tmp = p;
-    hipLaunchKernelGGL(fill_i64_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, counts, nq, (int64_t)0);
-    A.counts = counts;
-    A.offsets = offsets;
-    dispatch_search<1>(x, A, 16, keys, qtiles * splits, s, true);
-    OA_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(sort_tmp, sort_bytes, keys, keys_sorted, (int)tot, (int)nq,
-                                                      offsets, offsets + 1, 0, 64, s));
+    slots = one_pass ? (uint64_t *)(buf + counts_bytes + cub_span) : nullptr;
+    uint64_t *keys = (uint64_t *)(buf + head);
+    uint64_t *keys_sorted = (uint64_t *)(buf + head + keys_span);
+    void *sort_tmp = buf + head + 2 * keys_span;
+    if (from_slots) {
+        hipLaunchKernelGGL(radius_sort_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, slots, offsets, nq,
+                           (int64_t)kRadiusSlotCap, keys_sorted);
+    } else {
+        // a query overflowed its slot (or no slots): write every key into
+        // its CSR span, then sort
+        hipLaunchKernelGGL(fill_i64_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, counts, nq,
+                           (int64_t)0);
+        A.counts = counts;
+        A.offsets = offsets;
+        dispatch_search<1>(x, A, 16, keys, qtiles * splits, s, true);
+        if (wave_sort)
+            hipLaunchKernelGGL(radius_sort_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, s, keys, offsets,
+                               nq, (int64_t)0, keys_sorted);
+        else
+            OA_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(sort_tmp, sort_bytes, keys, keys_sorted, (int)tot,
+                                                              (int)nq, offsets, offsets + 1, 0, 64, s));
+    }
     hipLaunchKernelGGL(radius_unpack_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, keys_sorted, tot,
                        out_i, out_d);
     OA_HIP(hipGetLastError());

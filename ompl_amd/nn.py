@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import METRIC_L2, METRIC_SE3, _ptr, _stream, _torch
-from ._lib import OmplAmdError, call
+from ._lib import OmplAmdError, call, check, load
 
 __all__ = ["NearestNeighborsGPU", "METRIC_L2", "METRIC_SE3"]
 
@@ -90,16 +90,31 @@ class NearestNeighborsGPU:
         return i.cpu().numpy(), d.cpu().numpy()
 
     def radius_dev(self, q_dev, r: float, stream=None):
+        """CSR radius results (offsets [nq+1], ids, distances) on the device.
+
+        One library call when the outputs, sized from the last call's result
+        count per query (at least 16 per query), hold every result; otherwise
+        the call reports the total and a second call writes into outputs of
+        exactly that size."""
         torch = _torch()
         nq = q_dev.shape[0]
         off = torch.empty(nq + 1, dtype=torch.int64, device=q_dev.device)
         total = C.c_int64()
-        call("ompl_amd_nn_radius_dev", self.handle, _ptr(q_dev), nq, float(r), _ptr(off), None, None, 0,
-             C.byref(total), _stream(stream))
-        idx = torch.empty(max(total.value, 1), dtype=torch.int32, device=q_dev.device)
-        dist = torch.empty(max(total.value, 1), dtype=torch.float32, device=q_dev.device)
-        call("ompl_amd_nn_radius_dev", self.handle, _ptr(q_dev), nq, float(r), _ptr(off), _ptr(idx), _ptr(dist),
-             total.value, C.byref(total), _stream(stream))
+        per_q = max(16, getattr(self, "_radius_per_query", 16))
+        cap = max(1, min(nq * per_q, 1 << 26))
+        for _ in range(2):
+            idx = torch.empty(cap, dtype=torch.int32, device=q_dev.device)
+            dist = torch.empty(cap, dtype=torch.float32, device=q_dev.device)
+            code = load().ompl_amd_nn_radius_dev(self.handle, _ptr(q_dev), nq, float(r), _ptr(off), _ptr(idx),
+                                                 _ptr(dist), cap, C.byref(total), _stream(stream))
+            if code == 0 or total.value <= cap:
+                check(code)
+                break
+            cap = total.value  # outputs too small: offsets and total are written
+        else:
+            check(code)
+        if nq:
+            self._radius_per_query = -(-int(total.value) // nq)
         return off, idx[:total.value], dist[:total.value]
 
     def radius_batch(self, q, r: float):

@@ -157,3 +157,55 @@ def test_other_dimensions(gpu, oracle, dim):
     gi, gd = nn.knn_batch(q, 12)
     oi, od = oracle.knn(0, data, q, 12)
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+@pytest.mark.parametrize("r,most", [(0.9, 64), (1.2, 512), (3.0, 3000)])
+def test_radius_slot_and_segment_paths(gpu, oracle, r, most):
+    """Radius results of every path equal the oracle's lists: one pass into
+    64-key slots sorted by one wave per query (every query at most 64 hits),
+    the second CSR pass sorted by one wave (at most 512), and the second pass
+    sorted by the segmented sort (more)."""
+    rng = np.random.default_rng(31)
+    data = rng.uniform(-1, 1, (3000, 7)).astype(np.float32)
+    queries = rng.uniform(-1, 1, (100, 7)).astype(np.float32)
+    nn = make_nn(gpu, 0, 7, data, batches=3)
+    off, gi, gd = nn.radius_batch(queries, r)
+    hits = np.diff(off)
+    assert most // 8 < hits.max() <= most
+    ores = oracle.radius(0, data, queries, r)
+    for j, (oi, od) in enumerate(ores):
+        assert np.array_equal(gi[off[j]:off[j + 1]], oi), j
+        assert np.array_equal(gd[off[j]:off[j + 1]], od), j
+
+
+def test_radius_capacity_too_small(gpu, oracle):
+    """Outputs smaller than the result count: the call fails with
+    OMPL_AMD_ERR_ARG, writes the offsets and the total, and leaves the
+    outputs untouched; the Python wrapper then retries at the exact size."""
+    import ctypes as C
+    import torch
+    from ompl_amd import _ptr
+    from ompl_amd._lib import load
+    rng = np.random.default_rng(32)
+    data = rng.uniform(-1, 1, (3000, 7)).astype(np.float32)
+    queries = rng.uniform(-1, 1, (50, 7)).astype(np.float32)
+    nn = make_nn(gpu, 0, 7, data)
+    ores = oracle.radius(0, data, queries, 1.2)
+    want = sum(len(oi) for oi, _ in ores)
+    qd = torch.from_numpy(queries).cuda()
+    off = torch.full((51,), -5, dtype=torch.int64, device="cuda")
+    idx = torch.full((8,), -7, dtype=torch.int32, device="cuda")
+    dist = torch.full((8,), -7.0, dtype=torch.float32, device="cuda")
+    total = C.c_int64()
+    code = load().ompl_amd_nn_radius_dev(nn.handle, _ptr(qd), 50, 1.2, _ptr(off), _ptr(idx), _ptr(dist), 8,
+                                         C.byref(total), None)
+    torch.cuda.synchronize()
+    assert code == -1 and total.value == want
+    assert off[-1].item() == want and np.array_equal(np.diff(off.cpu().numpy()), [len(oi) for oi, _ in ores])
+    assert (idx.cpu() == -7).all() and (dist.cpu() == -7.0).all()
+    nn._radius_per_query = 1  # the wrapper's first guess is too small
+    off2, gi, gd = nn.radius_dev(qd, 1.2)
+    gi, gd, off2 = gi.cpu().numpy(), gd.cpu().numpy(), off2.cpu().numpy()
+    for j, (oi, od) in enumerate(ores):
+        assert np.array_equal(gi[off2[j]:off2[j + 1]], oi), j
+        assert np.array_equal(gd[off2[j]:off2[j + 1]], od), j
