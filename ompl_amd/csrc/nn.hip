@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -170,7 +171,39 @@ struct KnnArgs {
     int64_t *counts; // radius pass 0 / 2: per-query counts; pass 1: per-query cursors
     const int64_t *offsets;
     int slot_cap;    // radius pass 2: keys kept per query in its fixed slot
+    // k-NN: splits searched by one workgroup (one wave each, same query
+    // tile), and the bound slots its lanes share per query
+    int group_waves; // W
+    int bound_slots; // M = min(k, 2 W)
+    int bound_rank;  // c = ceil(k / M): a lane publishes its c-th smallest key
 };
+
+// k-NN lanes exchange their bounds after each of a split's first 4 tiles,
+// then every 8th tile up to tile 64, then every 64th: bounds tighten fastest
+// early, and an exchange costs about as much as filtering a tile.
+__device__ __forceinline__ bool share_tile(int64_t tj)
+{
+    return tj != 0 && (tj < 4 || (tj < 64 ? (tj & 7) == 0 : (tj & 63) == 0));
+}
+
+// Dynamic LDS of a k-NN workgroup of W waves: each wave's current tile, then
+// the bound slots (M <= 2 W per query, 32 queries).
+__host__ __device__ constexpr size_t knn_lds_bytes(int dim, int w)
+{
+    return (size_t)w * 32 * dim * sizeof(float) + (size_t)32 * 2 * w * sizeof(unsigned long long);
+}
+
+// Waves of one k-NN workgroup at most: 16 (1,024 threads, 4 waves per SIMD)
+// for lists of up to 16 keys, 4 for the longer lists (more registers).
+template <int KMAX>
+struct KnnGroup {
+    static constexpr int kMaxWaves = KMAX <= 16 ? 16 : 4;
+};
+
+__device__ __forceinline__ unsigned long long pack_key(float d, int i)
+{
+    return ((unsigned long long)__float_as_uint(d) << 32) | (uint32_t)i;
+}
 
 // Row of the 32x32 MFMA output held in element j of a lane in half h.
 __device__ __forceinline__ int mfma_row(int j, int h)
@@ -197,15 +230,16 @@ struct QueryLane {
         float qn = 0.0f;
 #pragma unroll
         for (int i = 0; i < TD; ++i) qn += q[i] * q[i];
+        // operand row kk of the augmented query (-2 q, |q|^2, 1, 0 ...); the
+        // lane's half picks the odd or even row of each K step
+        auto row_of = [&](int kk) {
+            if (kk < TD) return -2.0f * q[kk];
+            if (kk == TD) return qn;
+            if (kk == TD + 1) return 1.0f;
+            return 0.0f;
+        };
 #pragma unroll
-        for (int s = 0; s < KSTEPS; ++s) {
-            const int kk = 2 * s + h;
-            float v = 0.0f;
-            if (kk < TD) v = -2.0f * q[kk];
-            else if (kk == TD) v = qn;
-            else if (kk == TD + 1) v = 1.0f;
-            b[s] = v;
-        }
+        for (int s = 0; s < KSTEPS; ++s) b[s] = h ? row_of(2 * s + 1) : row_of(2 * s);
         const float xm = __uint_as_float(*A.xn_max);
         margin = 64.0f * kUnit * (qn + xm);
     }
@@ -248,72 +282,173 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int METRIC, int DIM, int KSTEPS, int KMAX>
-__global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
+// A lane's k best keys ((dist bits << 32) | id), ascending, in KMAX slots:
+// the first KMAX - k slots hold 0, which no key is below, so the k-th key is
+// always the last slot; an empty slot holds ~0 (above every key).
+template <int KMAX>
+struct TopKey {
+    unsigned long long key[KMAX];
+    __device__ __forceinline__ void init(int k)
+    {
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) key[s] = s < KMAX - k ? 0ull : ~0ull;
+    }
+    __device__ __forceinline__ void insert(unsigned long long nk)
+    {
+        // each slot keeps the smaller of its key and the carried one and
+        // passes the larger on; the largest leaves the list
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) {
+            const unsigned long long lo = nk < key[s] ? nk : key[s];
+            nk = nk < key[s] ? key[s] : nk;
+            key[s] = lo;
+        }
+    }
+    __device__ __forceinline__ unsigned long long kth() const { return key[KMAX - 1]; }
+};
+
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m)
 {
-    // the current tile's 32 points, per wave: candidates that pass the MFMA
-    // filter read their exact coordinates from here, not from HBM
-    __shared__ float tile_pts[4][32 * DIM];
-    __shared__ uint8_t tile_rm[4][32]; // the tile's removal flags (1 past the end)
+    const uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// The filter threshold for k-th key `kth`: its squared distance, widened by
+// the relative rounding of the squared distance and the MFMA error bound.
+__device__ __forceinline__ float filter_threshold(unsigned long long kth, float margin)
+{
+    const float kd = __uint_as_float((uint32_t)(kth >> 32));
+    return kd * kd * (1.0f + 8e-6f) + margin;
+}
+
+// One workgroup = one query tile (32 queries) x W consecutive splits of the
+// point tiles, one wave per split.
+template <int METRIC, int DIM, int KSTEPS, int KMAX>
+__global__ void __launch_bounds__(64 * KnnGroup<KMAX>::kMaxWaves, (KMAX <= 16 ? 4 : (KMAX <= 32 ? 3 : 2)))
+knn_kernel(KnnArgs A)
+{
+    constexpr int MW = KnnGroup<KMAX>::kMaxWaves;
+    // the current tile's 32 points, per wave (candidates that pass the MFMA
+    // filter read their exact coordinates from here, not from HBM), then the
+    // bound slots of the workgroup's 32 queries, M per query
+    extern __shared__ __align__(16) unsigned char knn_smem[];
     const int lane = threadIdx.x & 63;
     const int h = lane >> 5;
-    float *tp = tile_pts[(threadIdx.x >> 6) & 3];
-    uint8_t *trm = tile_rm[(threadIdx.x >> 6) & 3];
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // the wave's query tile and split are wave-uniform (scalar registers), so
+    // is every tile address below
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float *tp = reinterpret_cast<float *>(knn_smem) + wid * (32 * DIM);
+    unsigned long long *slots =
+        reinterpret_cast<unsigned long long *>(knn_smem + (size_t)A.group_waves * 32 * DIM * sizeof(float));
     const int64_t qtiles = (A.nq + 31) / 32;
-    const int64_t qtile = wave % qtiles;
-    const int split = (int)(wave / qtiles);
-    if (split >= A.splits) return;
+    const int64_t qtile = blockIdx.x % qtiles;
+    const int split = (int)(blockIdx.x / qtiles) * A.group_waves + wid;
+    const int M = A.bound_slots;
+    for (int i = threadIdx.x; i < 32 * M; i += blockDim.x) slots[i] = ~0ull;
+    __syncthreads();
     QueryLane<METRIC, DIM, KSTEPS> Q;
     Q.load(A, qtile, lane);
-    TopK<KMAX> top;
-    top.init();
+    TopKey<KMAX> top;
+    top.init(A.k);
+    // bkey: a key that at least k distinct points of the query are at or
+    // below, so no point of its k nearest is above it: a point is kept only
+    // below it, and the filter threshold follows it.  Sources: the lane's own
+    // k-th key, its partner lane's bound (the query's other half of each
+    // tile), and the query's slots in the workgroup: its 2 W lanes are dealt
+    // round-robin over M slots, each lane publishes its c-th smallest key
+    // (c M >= k) into its slot by atomic minimum, and once every slot holds
+    // a key their largest is a bound (c keys at or below it in each slot's
+    // lane, lanes disjoint).  A key equal to a bound is one that lane holds
+    // itself, and no other lane sees that point.
     float thr = INFINITY;
-    float kd = INFINITY; // current k-th key of this lane's list
-    int ki = INT_MAX;
+    unsigned long long bkey = ~0ull, published = ~0ull;
+    unsigned long long *qslots = slots + (lane & 31) * M;
+    const int my_slot = (2 * wid + h) % M;
     const int64_t t0 = (int64_t)split * A.tiles_per_split;
     const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
-    const int64_t n_words = A.n * DIM;
     // A tile's exact coordinates (32 x DIM words, PW per lane) and removal
     // flags are read one tile ahead with its MFMA operands, so a tile with
-    // candidates -- nearly every tile while the lists fill -- stages them in
-    // LDS without waiting on a global read.
+    // candidates stages them in LDS without waiting on a global read.  The
+    // point and flag arrays are allocated in whole tiles, so every read of a
+    // tile is in bounds; rows past the last point count as removed.
     constexpr int PW = (32 * DIM + 63) / 64;
-    auto pts_load = [&](int64_t t, float v[PW], uint8_t &rm) {
-        const int64_t w0 = t * 32 * DIM;
+    auto pts_load = [&](int64_t t, float v[PW], uint32_t &rm) {
+        const float *tile = A.pts + t * (32 * DIM);
 #pragma unroll
         for (int w = 0; w < PW; ++w) {
             const int i = lane + 64 * w;
-            v[w] = (i < 32 * DIM && w0 + i < n_words) ? A.pts[w0 + i] : 0.0f;
+            v[w] = (w + 1 < PW || i < 32 * DIM) ? tile[i < 32 * DIM ? i : 0] : 0.0f;
         }
-        rm = (lane < 32 && t * 32 + lane < A.n) ? A.removed[t * 32 + lane] : (uint8_t)1;
+        rm = A.removed[t * 32 + (lane & 31)];
     };
     float cur[KSTEPS], cpts[PW];
-    uint8_t crm = 1;
+    uint32_t crm = 1;
     if (t0 < t1) {
         tile_load<KSTEPS>(A.xt, t0, lane, cur);
         pts_load(t0, cpts, crm);
     }
     for (int64_t t = t0; t < t1; ++t) {
+        if (share_tile(t - t0)) { // wave-uniform
+            unsigned long long best = bkey;
+            const unsigned long long partner = shfl_xor_u64(best, 32);
+            if (partner < best) best = partner;
+            // publish this lane's c-th smallest key (~0 while it has fewer)
+            const int at = KMAX - A.k + A.bound_rank - 1;
+            unsigned long long mine = ~0ull;
+#pragma unroll
+            for (int s = 0; s < KMAX; ++s)
+                if (s == at) mine = top.key[s];
+            if (Q.live && mine < published) {
+                atomicMin(qslots + my_slot, mine);
+                published = mine;
+            }
+            // the slots' largest key, once every slot holds one (slots are
+            // only lowered, so any value read is one some lane published)
+            unsigned long long b = 0;
+#pragma unroll
+            for (int s = 0; s < 2 * MW; ++s) {
+                if (s < M) {
+                    const unsigned long long v =
+                        __hip_atomic_load(qslots + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    b = v > b ? v : b;
+                }
+            }
+            if (b < best) best = b;
+            if (best < bkey) {
+                bkey = best;
+                thr = filter_threshold(best, Q.margin);
+            }
+        }
         // the next tile's operands are in flight while this one is filtered
         float nxt[KSTEPS], npts[PW];
-        uint8_t nrm;
+        uint32_t nrm;
         const int64_t tn = t + 1 < t1 ? t + 1 : t;
         tile_load<KSTEPS>(A.xt, tn, lane, nxt);
         pts_load(tn, npts, nrm);
         const f32x16 acc = tile_mma_regs<KSTEPS>(cur, Q.b);
 #pragma unroll
         for (int s2 = 0; s2 < KSTEPS; ++s2) cur[s2] = nxt[s2];
-        uint32_t mask = 0;
+        // any value of the lane at or below the threshold?
+        float lo = fminf(fminf(acc[0], acc[1]), acc[2]);
 #pragma unroll
-        for (int j = 0; j < 16; ++j) mask |= (acc[j] <= thr ? 1u : 0u) << j;
-        if (!Q.live) mask = 0;
-        if (__ballot(mask != 0) == 0) { // wave-uniform
+        for (int j = 3; j < 15; j += 2) lo = fminf(fminf(lo, acc[j]), acc[j + 1]);
+        lo = fminf(lo, acc[15]);
+        if (__ballot(Q.live && lo <= thr) == 0) { // wave-uniform
 #pragma unroll
             for (int w = 0; w < PW; ++w) cpts[w] = npts[w];
             crm = nrm;
             continue;
         }
+        uint32_t mask = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) mask |= (acc[j] <= thr ? 1u : 0u) << j;
+        if (!Q.live) mask = 0;
+        // the tile's removed rows (and rows past the last point) as a 32-bit
+        // row mask, then as this lane's 16 elements: element j of half h is
+        // row 8 (j >> 2) + 4 h + (j & 3)
+        const uint32_t gone = (uint32_t)__ballot(lane < 32 && (crm != 0 || t * 32 + lane >= A.n));
+        const uint32_t g = gone >> (4 * h);
+        mask &= ~((g & 0xFu) | ((g >> 4) & 0xF0u) | ((g >> 8) & 0xF00u) | ((g >> 12) & 0xF000u));
         wave_lds_sync(); // every lane is done with the previous tile's points
 #pragma unroll
         for (int w = 0; w < PW; ++w) {
@@ -321,37 +456,53 @@ __global__ void __launch_bounds__(256) knn_kernel(KnnArgs A)
             if (i < 32 * DIM) tp[i] = cpts[w];
             cpts[w] = npts[w];
         }
-        if (lane < 32) trm[lane] = crm;
         crm = nrm;
         wave_lds_sync();
         while (mask) {
             const int j = __builtin_ctz(mask);
             mask &= mask - 1;
             const int row = mfma_row(j, h);
-            const int64_t p = t * 32 + row;
-            if (trm[row]) continue;
             const float d = exact_dist<METRIC, DIM>(tp + row * DIM, Q.q);
-            if (key_less(d, (int)p, kd, ki)) {
-                top.insert(d, (int)p);
-                top.kth(A.k, kd, ki);
-                if (kd < INFINITY) thr = kd * kd * (1.0f + 8e-6f) + Q.margin;
+            const unsigned long long nk = pack_key(d, (int)(t * 32 + row));
+            if (KMAX <= 16) {
+                if (nk < bkey) top.insert(nk);
+            } else {
+                // a key at or above the bound is inserted as ~0, which
+                // changes nothing: no branch, whose two list versions would
+                // both be live in registers
+                top.insert(nk < bkey ? nk : ~0ull);
+            }
+            const unsigned long long kk = top.kth();
+            if (kk < bkey) {
+                bkey = kk;
+                thr = filter_threshold(kk, Q.margin);
             }
         }
     }
-    // wave-level merge: lane l (< 32) absorbs the list of lane l + 32
+    // wave-level merge: lane l (< 32) absorbs the k keys of lane l + 32, which
+    // hands over its first slot and shifts its list down, one key a round
+    // (one exchange in flight at a time keeps the register count of the
+    // search loop)
+    const int skip = KMAX - A.k;
+#pragma unroll 1
+    for (int r = 0; r < KMAX; ++r) {
+        const unsigned long long ok = shfl_xor_u64(top.key[0], 32);
+        if (h == 0) {
+            if (r >= skip) top.insert(ok);
+        } else {
 #pragma unroll
-    for (int s = 0; s < KMAX; ++s) {
-        const float od = __shfl_xor(top.d[s], 32);
-        const int oi = __shfl_xor(top.i[s], 32);
-        if (h == 0 && s < A.k) top.insert(od, oi);
+            for (int s = 0; s + 1 < KMAX; ++s) top.key[s] = top.key[s + 1];
+            top.key[KMAX - 1] = ~0ull;
+        }
     }
     if (h != 0 || !Q.live) return;
-    const int64_t base = ((int64_t)split * A.nq + Q.qi) * A.k;
+    const int64_t base = ((int64_t)split * A.nq + Q.qi) * A.k - skip;
 #pragma unroll
     for (int s = 0; s < KMAX; ++s) {
-        if (s < A.k) {
-            A.out_d[base + s] = top.d[s];
-            A.out_i[base + s] = top.i[s] == INT_MAX ? -1 : top.i[s];
+        if (s >= skip) {
+            const unsigned long long kk = top.key[s];
+            A.out_d[base + s] = kk == ~0ull ? INFINITY : __uint_as_float((uint32_t)(kk >> 32));
+            A.out_i[base + s] = kk == ~0ull ? -1 : (int32_t)(uint32_t)kk;
         }
     }
 }
@@ -625,14 +776,18 @@ static void append(NNIndex &x, const float *src, int64_t n, bool src_dev, hipStr
 }
 
 template <int METRIC, int DIM, int KSTEPS>
-static void launch_knn_t(const KnnArgs &A, int kmax, int64_t waves, hipStream_t s)
+static void launch_knn_t(const KnnArgs &A, int kmax, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    // one workgroup per (query tile, group of W splits)
+    const int64_t qtiles = (A.nq + 31) / 32;
+    const unsigned blocks = (unsigned)(qtiles * (A.splits / A.group_waves));
+    const dim3 block((unsigned)(64 * A.group_waves));
+    const size_t lds = knn_lds_bytes(DIM, A.group_waves);
     switch (kmax) {
-    case 10: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 10>), dim3(blocks), dim3(256), 0, s, A); break;
-    case 16: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 16>), dim3(blocks), dim3(256), 0, s, A); break;
-    case 32: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 32>), dim3(blocks), dim3(256), 0, s, A); break;
-    default: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 64>), dim3(blocks), dim3(256), 0, s, A); break;
+    case 10: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 10>), dim3(blocks), block, lds, s, A); break;
+    case 16: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 16>), dim3(blocks), block, lds, s, A); break;
+    case 32: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 32>), dim3(blocks), block, lds, s, A); break;
+    default: hipLaunchKernelGGL((knn_kernel<METRIC, DIM, KSTEPS, 64>), dim3(blocks), block, lds, s, A); break;
     }
 }
 
@@ -651,11 +806,11 @@ static void dispatch_search(const NNIndex &x, const KnnArgs &A, int kmax, uint64
 #define OA_CASE(D, KS)                                                                                              \
     case D:                                                                                                         \
         if (radius) launch_radius_t<OMPL_AMD_METRIC_L2, D, KS, PASS>(A, keys, waves, s);                          \
-        else launch_knn_t<OMPL_AMD_METRIC_L2, D, KS>(A, kmax, waves, s);                                          \
+        else launch_knn_t<OMPL_AMD_METRIC_L2, D, KS>(A, kmax, s);                                          \
         break;
     if (x.metric == OMPL_AMD_METRIC_SE3) {
         if (radius) launch_radius_t<OMPL_AMD_METRIC_SE3, 7, 3, PASS>(A, keys, waves, s);
-        else launch_knn_t<OMPL_AMD_METRIC_SE3, 7, 3>(A, kmax, waves, s);
+        else launch_knn_t<OMPL_AMD_METRIC_SE3, 7, 3>(A, kmax, s);
         return;
     }
     switch (x.dim) {
@@ -717,24 +872,22 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
     const int kmax_search = k <= 10 ? 10 : kmax;
     KnnArgs A = base_args(x, q, nq);
     A.k = k;
-    if (x.n == 0) {
-        // empty structure: every slot is (-1, +inf)
-        A.n_tiles = 0;
-        A.splits = 1;
-        A.tiles_per_split = 0;
-        A.out_d = out_d;
-        A.out_i = out_i;
-        const int64_t qtiles = (nq + 31) / 32;
-        dispatch_search<0>(x, A, kmax_search, nullptr, qtiles, s, false);
-        OA_HIP(hipGetLastError());
-        return;
-    }
-    const int64_t splits = choose_splits(nq, A.n_tiles);
+    // splits of the point tiles, searched W to a workgroup (W a power of two
+    // up to the kernel's maximum; the split count rounded up to a multiple)
+    int64_t splits = x.n == 0 ? 1 : choose_splits(nq, A.n_tiles);
+    const int max_w = kmax_search <= 16 ? 16 : 4;
+    int W = 1;
+    while (W * 2 <= max_w && W * 2 <= splits) W *= 2;
+    splits = (splits + W - 1) / W * W;
     A.splits = (int)splits;
+    A.group_waves = W;
+    A.bound_slots = std::min(k, 2 * W);
+    A.bound_rank = (k + A.bound_slots - 1) / A.bound_slots;
     A.tiles_per_split = (A.n_tiles + splits - 1) / splits;
     const int64_t qtiles = (nq + 31) / 32;
     ProfScope ps("nn_knn", s);
     if (splits == 1) {
+        // one split (or an empty structure: every slot is (-1, +inf))
         A.out_d = out_d;
         A.out_i = out_i;
         dispatch_search<0>(x, A, kmax_search, nullptr, qtiles, s, false);

@@ -6,6 +6,9 @@
     python -m ompl_amd.rocprof_summary traffic BENCH_FETCH BENCH_WRITE COPY_FETCH COPY_WRITE COPY_BYTES
         per-kernel HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE runs, each
         scaled by what the counter reads for a plain device copy of COPY_BYTES
+    python -m ompl_amd.rocprof_summary counters DIR [DIR ...] [--match TEXT ...]
+        per-kernel average of every counter per dispatch, over one or more
+        --pmc runs of the same program (kernels whose name holds a TEXT)
 """
 from __future__ import annotations
 
@@ -54,6 +57,22 @@ def counter_per_dispatch(d: str, counter: str) -> dict[str, float]:
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
+def counters_per_dispatch(dirs: list[str]) -> dict[str, dict[str, float]]:
+    """{kernel: {counter: average value per dispatch}} over the --pmc runs in
+    `dirs` (each counter is taken from the run that collected it)."""
+    acc: dict[str, dict[str, list[float]]] = defaultdict(lambda: defaultdict(list))
+    for d in dirs:
+        # a dispatch's value of a counter is the sum of its rows (one row per
+        # dimension instance when the counter is not reduced)
+        per: dict[tuple, float] = defaultdict(float)
+        for r in csv.DictReader(open(_find(d, "counter_collection.csv"))):
+            key = (r["Kernel_Name"], r["Counter_Name"], r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+            per[key] += float(r["Counter_Value"])
+        for (k, c, _), v in per.items():
+            acc[k][c].append(v)
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
+
+
 def traffic(bench_fetch: str, bench_write: str, copy_fetch: str, copy_write: str, copy_bytes: int) -> dict:
     cf = counter_per_dispatch(copy_fetch, "FETCH_SIZE")
     cw = counter_per_dispatch(copy_write, "WRITE_SIZE")
@@ -87,6 +106,23 @@ def main(argv: list[str]) -> int:
         for k, v in t["kernels"].items():
             print(f"{k[:78]:78s} {v['read_bytes']:12.0f} {v['write_bytes']:12.0f} {v['bytes']:12.0f}")
         print("json: " + json.dumps(t))
+    elif argv[0] == "counters":
+        dirs = [a for a in argv[1:] if not a.startswith("--")]
+        match = []
+        if "--match" in argv:
+            i = argv.index("--match")
+            dirs = [a for a in argv[1:i]]
+            match = argv[i + 1:]
+        t = counters_per_dispatch(dirs)
+        names = sorted({c for cs in t.values() for c in cs})
+        for k in sorted(t):
+            if match and not any(m in k for m in match):
+                continue
+            print(k[:110])
+            for c in names:
+                if c in t[k]:
+                    print(f"    {c:34s} {t[k][c]:16.1f}")
+        print("json: " + json.dumps({k: v for k, v in t.items() if not match or any(m in k for m in match)}))
     else:
         print(__doc__)
         return 2

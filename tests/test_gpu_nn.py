@@ -113,6 +113,26 @@ def test_ties_duplicates_and_k_edge_cases(gpu, oracle):
     assert set(gi[0, :5].tolist()) == set(range(5))
 
 
+def test_knn_split_bounds_with_duplicates_and_removal(gpu, oracle):
+    """64 queries over 20,480 points searched in many splits, which exchange
+    bounds on each query's k-th key: with every point present ten times (ties
+    broken by the lower id) and 500 ids removed, the lists equal the
+    oracle's for k from 1 (one slot) to 64 (one slot per 2.5 lanes)."""
+    rng = np.random.default_rng(77)
+    base = rng.uniform(-1, 1, (2048, 7)).astype(np.float32)
+    data = np.tile(base, (10, 1))
+    nn = make_nn(gpu, 0, 7, data, batches=2)
+    removed = rng.choice(len(data), 500, replace=False)
+    for i in removed:
+        nn.remove(int(i))
+    keep = np.setdiff1d(np.arange(len(data)), removed)
+    q = np.concatenate([base[:16], rng.uniform(-1, 1, (48, 7)).astype(np.float32)])
+    for k in (1, 4, 10, 16, 40, 64):
+        gi, gd = nn.knn_batch(q, k)
+        oi, od = oracle.knn(0, data[keep], q, k)
+        assert np.array_equal(gi, keep[oi]) and np.array_equal(gd, od), k
+
+
 def test_empty_nearest_raises_and_removal(gpu, oracle):
     from ompl_amd._lib import OmplAmdError
 
