@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 307
+#define OMPL_AMD_VERSION 308
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -172,6 +172,16 @@ typedef struct ompl_amd_motion_graph_s *ompl_amd_motion_graph;
 int ompl_amd_motion_graph_create(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
                                  double longest_valid_segment, unsigned factor, uint8_t *out_dev,
                                  int32_t *checked_dev, ompl_amd_motion_graph *graph);
+/* Captured group of independent batches: batch i (a_soa_dev[i],
+ * b_soa_dev[i], n[i] edges, out_dev[i], and checked_dev[i] unless checked_dev
+ * is null) is recorded on its own stream, forked from the capturing stream and
+ * joined back inside the same graph, with its own survivor workspace; a
+ * launch runs the n_batches (1..16) batches concurrently and ends when all
+ * have ended.  Same replay rules as a single captured batch. */
+int ompl_amd_motion_graph_create_group(ompl_amd_env env, int n_batches, const double *const *a_soa_dev,
+                                       const double *const *b_soa_dev, const int64_t *n,
+                                       double longest_valid_segment, unsigned factor, uint8_t *const *out_dev,
+                                       int32_t *const *checked_dev, ompl_amd_motion_graph *graph);
 int ompl_amd_motion_graph_launch(ompl_amd_motion_graph graph, void *stream);
 int ompl_amd_motion_graph_destroy(ompl_amd_motion_graph graph);
 

@@ -272,6 +272,31 @@ class MotionGraph:
         call("ompl_amd_motion_graph_launch", self.handle, _stream(stream))
         return self.out
 
+    @classmethod
+    def group(cls, env: Env, batches, lvs: float, factor: int = 1, outs=None, checked=None):
+        """Independent batches [(a_soa, b_soa), ...] captured in one graph,
+        each on its own forked stream: launch() runs them concurrently.
+        `outs` / `checked` are per-batch lists (created when None)."""
+        torch = _torch()
+        self = cls.__new__(cls)
+        nb = len(batches)
+        if outs is None:
+            outs = [torch.empty(a.shape[1], dtype=torch.uint8, device=a.device) for a, _ in batches]
+        self.out = outs
+        self.checked = checked
+        self._keep = (batches, outs, checked)
+        vpn = C.c_void_p * nb
+        a_arr = vpn(*[a.data_ptr() for a, _ in batches])
+        b_arr = vpn(*[b.data_ptr() for _, b in batches])
+        n_arr = (C.c_int64 * nb)(*[a.shape[1] for a, _ in batches])
+        o_arr = vpn(*[o.data_ptr() for o in outs])
+        c_arr = vpn(*[c.data_ptr() for c in checked]) if checked is not None else None
+        h = C.c_void_p()
+        call("ompl_amd_motion_graph_create_group", env.handle, nb, a_arr, b_arr, n_arr, float(lvs), int(factor),
+             o_arr, c_arr, C.byref(h))
+        self.handle = h
+        return self
+
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             call("ompl_amd_motion_graph_destroy", self.handle)

@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 307
+ABI_VERSION = 308
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -49,6 +49,8 @@ SIGNATURES = {
     "ompl_amd_check_state_lists_dev": (C.c_int, [vp, vp, C.c_int64, vp, C.c_int64, vp, vp, vp]),
     "ompl_amd_check_states": (C.c_int, [vp, vp, C.c_int, c_int_p, c_int_p]),
     "ompl_amd_motion_graph_create": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, vp, C.POINTER(vp)]),
+    "ompl_amd_motion_graph_create_group": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_double, C.c_uint, vp, vp,
+                                                     C.POINTER(vp)]),
     "ompl_amd_motion_graph_launch": (C.c_int, [vp, vp]),
     "ompl_amd_motion_graph_destroy": (C.c_int, [vp]),
     "ompl_amd_get_tuning": (C.c_int, [vp]),

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <memory>
 #include <vector>
 
 #include "runtime.hpp"
@@ -66,14 +67,15 @@ void ompl_amd::check_motions_tuned(const Tuning &t, const Env &env, const double
         // its FK when validity_shared_fk is set.  The survivors' booleans are
         // written by the s2 pass (1) and the walk (0 at an invalid position);
         // a results kernel runs after the walk when checked counts are asked
-        // for, and for a captured graph (which replays one counter set), and
-        // clears the batch's counter set; otherwise phase 1 of the walk clears
-        // the other set, which the next batch on this workspace uses.
+        // for, and clears the batch's counter set; otherwise phase 1 of the
+        // walk clears the other set, which the next batch on this workspace
+        // uses (a captured batch is recorded once per set, and its replays
+        // alternate between the two).
         const bool shared = t.motion_strategy == 4;
         const bool eshared = t.validity_shared_fk != 0;
         std::unique_lock<std::mutex> lk; // held until this batch's launches are enqueued
         Workspace &w = own ? *own : workspace(s, (size_t)n, lk);
-        const bool results = exact || own != nullptr;
+        const bool results = exact;
         int32_t *cur = w.counters + 8 + 2 * w.parity, *other = w.counters + 8 + 2 * (1 - w.parity);
         A.list = w.list;
         A.list_count = cur;
@@ -143,24 +145,106 @@ std::vector<Tuning> motion_candidates()
 } // namespace
 
 // A captured motion batch: the launches of one check_motions call recorded in
-// a HIP graph (pointers, sizes and launch shape fixed at capture) with its own
+// HIP graphs (pointers, sizes and launch shape fixed at capture) with their own
 // survivor workspace, so replays on any stream do not share state with eager
 // calls.  No memset nodes: the strategies clear their counters with kernels.
+// The batch is recorded twice, once per survivor counter set (each recording
+// clears the other set for the next replay), and replays alternate between
+// the two, as eager batches on one workspace do.
+// A group of independent batches is recorded in one graph, batch 0 on the
+// capturing stream and batch i > 0 on a side stream forked from it and joined
+// back, so a replay runs the batches concurrently.
 struct ompl_amd_motion_graph_s {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    Workspace ws;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int next = 0; // the recording the next replay launches
+    std::vector<std::unique_ptr<Workspace>> ws; // one per batch
+    std::vector<hipStream_t> side;     // batches 1.. are recorded on these
+    std::vector<hipEvent_t> fork_join; // fork, then one join per side stream
     ~ompl_amd_motion_graph_s()
     {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
-        if (ws.list) hipFree(ws.list);
-        if (ws.first_bad) hipFree(ws.first_bad);
-        if (ws.surv_nd) hipFree(ws.surv_nd);
-        if (ws.surv_ab) hipFree(ws.surv_ab);
-        if (ws.counters) hipFree(ws.counters);
+        for (int i = 0; i < 2; ++i) {
+            if (exec[i]) hipGraphExecDestroy(exec[i]);
+            if (graph[i]) hipGraphDestroy(graph[i]);
+        }
+        for (auto &w : ws) {
+            if (w->list) hipFree(w->list);
+            if (w->first_bad) hipFree(w->first_bad);
+            if (w->surv_nd) hipFree(w->surv_nd);
+            if (w->surv_ab) hipFree(w->surv_ab);
+            if (w->counters) hipFree(w->counters);
+        }
+        for (hipStream_t st : side) hipStreamDestroy(st);
+        for (hipEvent_t ev : fork_join) hipEventDestroy(ev);
     }
 };
+
+namespace {
+
+void graph_workspace(Workspace &w, int64_t n)
+{
+    const size_t cap = (size_t)n < 1024 ? 1024 : (size_t)n;
+    OA_HIP(hipMalloc(&w.counters, 64 * sizeof(int32_t)));
+    OA_HIP(hipMemset(w.counters, 0, 64 * sizeof(int32_t)));
+    OA_HIP(hipMalloc(&w.list, cap * sizeof(int32_t)));
+    OA_HIP(hipMalloc(&w.first_bad, cap * sizeof(int32_t)));
+    OA_HIP(hipMalloc(&w.surv_nd, cap * sizeof(int32_t)));
+    OA_HIP(hipMalloc(&w.surv_ab, cap * 14 * sizeof(double)));
+    w.list_cap = cap;
+    w.parity = 0;
+}
+
+// Records the group twice (once per survivor counter set of every batch's
+// workspace) and instantiates both recordings.
+void record_group(ompl_amd_motion_graph_s *g, const Env &env, int nb, const double *const *a,
+                  const double *const *b, const int64_t *n, double lvs, unsigned factor, uint8_t *const *out,
+                  int32_t *const *checked)
+{
+    for (int i = 0; i < nb; ++i) graph_workspace(*g->ws[(size_t)i], n[i]);
+    for (int i = 1; i < nb; ++i) {
+        hipStream_t st = nullptr;
+        OA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); // default priority
+        g->side.push_back(st);
+    }
+    for (int i = 0; i < nb; ++i) {
+        hipEvent_t ev = nullptr;
+        OA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        g->fork_join.push_back(ev);
+    }
+    OA_HIP(hipDeviceSynchronize());
+    // capture on the calling thread's own stream (the null stream cannot capture)
+    hipStream_t cs = thread_resources().stream;
+    const Tuning t = get_tuning();
+    for (int r = 0; r < 2; ++r) {
+        // recording r uses counter set r of each workspace when its batch
+        // alternates sets (check_motions_tuned flips ws.parity after recording)
+        OA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        try {
+            if (nb > 1) {
+                OA_HIP(hipEventRecord(g->fork_join[0], cs));
+                for (int i = 1; i < nb; ++i) OA_HIP(hipStreamWaitEvent(g->side[(size_t)i - 1], g->fork_join[0], 0));
+            }
+            for (int i = 0; i < nb; ++i) {
+                hipStream_t st = i == 0 ? cs : g->side[(size_t)i - 1];
+                check_motions_tuned(t, env, a[i], b[i], n[i], lvs, factor, out[i], checked ? checked[i] : nullptr, st,
+                                    g->ws[(size_t)i].get());
+            }
+            for (int i = 1; i < nb; ++i) {
+                OA_HIP(hipEventRecord(g->fork_join[(size_t)i], g->side[(size_t)i - 1]));
+                OA_HIP(hipStreamWaitEvent(cs, g->fork_join[(size_t)i], 0));
+            }
+        } catch (...) {
+            hipGraph_t dropped = nullptr;
+            hipStreamEndCapture(cs, &dropped);
+            if (dropped) hipGraphDestroy(dropped);
+            throw;
+        }
+        OA_HIP(hipStreamEndCapture(cs, &g->graph[r]));
+        OA_HIP(hipGraphInstantiate(&g->exec[r], g->graph[r], nullptr, nullptr, 0));
+    }
+}
+
+} // namespace
 
 extern "C" {
 
@@ -474,35 +558,27 @@ extern "C" {
 int ompl_amd_motion_graph_create(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
                                  unsigned factor, uint8_t *out, int32_t *checked, ompl_amd_motion_graph *graph)
 {
+    return ompl_amd_motion_graph_create_group(e, 1, &a, &b, &n, lvs, factor, &out, checked ? &checked : nullptr,
+                                              graph);
+}
+
+int ompl_amd_motion_graph_create_group(ompl_amd_env e, int n_batches, const double *const *a, const double *const *b,
+                                       const int64_t *n, double lvs, unsigned factor, uint8_t *const *out,
+                                       int32_t *const *checked, ompl_amd_motion_graph *graph)
+{
     return guarded([&] {
         const Env &env = env_of(e);
         OA_REQUIRE(graph, OMPL_AMD_ERR_ARG, "null graph handle");
-        OA_REQUIRE(n >= 1, OMPL_AMD_ERR_ARG, "a captured batch needs at least one edge");
+        OA_REQUIRE(n_batches >= 1 && n_batches <= 16, OMPL_AMD_ERR_ARG, "a captured group holds 1 to 16 batches");
+        OA_REQUIRE(a && b && n && out, OMPL_AMD_ERR_ARG, "null batch arrays");
+        for (int i = 0; i < n_batches; ++i) {
+            OA_REQUIRE(n[i] >= 1, OMPL_AMD_ERR_ARG, "a captured batch needs at least one edge");
+            OA_REQUIRE(a[i] && b[i] && out[i], OMPL_AMD_ERR_ARG, "null pointer in batch %d", i);
+        }
         auto *g = new ompl_amd_motion_graph_s();
+        for (int i = 0; i < n_batches; ++i) g->ws.push_back(std::make_unique<Workspace>());
         try {
-            const size_t cap = (size_t)n < 1024 ? 1024 : (size_t)n;
-            OA_HIP(hipMalloc(&g->ws.counters, 64 * sizeof(int32_t)));
-            OA_HIP(hipMemset(g->ws.counters, 0, 64 * sizeof(int32_t)));
-            OA_HIP(hipMalloc(&g->ws.list, cap * sizeof(int32_t)));
-            OA_HIP(hipMalloc(&g->ws.first_bad, cap * sizeof(int32_t)));
-            OA_HIP(hipMalloc(&g->ws.surv_nd, cap * sizeof(int32_t)));
-            OA_HIP(hipMalloc(&g->ws.surv_ab, cap * 14 * sizeof(double)));
-            g->ws.list_cap = cap;
-            OA_HIP(hipDeviceSynchronize());
-            // capture on the calling thread's own stream (the null stream cannot capture)
-            hipStream_t cs = thread_resources().stream;
-            const Tuning t = get_tuning();
-            OA_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            try {
-                check_motions_tuned(t, env, a, b, n, lvs, factor, out, checked, cs, &g->ws);
-            } catch (...) {
-                hipGraph_t dropped = nullptr;
-                hipStreamEndCapture(cs, &dropped);
-                if (dropped) hipGraphDestroy(dropped);
-                throw;
-            }
-            OA_HIP(hipStreamEndCapture(cs, &g->graph));
-            OA_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+            record_group(g, env, n_batches, a, b, n, lvs, factor, out, checked);
         } catch (...) {
             delete g;
             throw;
@@ -514,8 +590,9 @@ int ompl_amd_motion_graph_create(ompl_amd_env e, const double *a, const double *
 int ompl_amd_motion_graph_launch(ompl_amd_motion_graph graph, void *stream)
 {
     return guarded([&] {
-        OA_REQUIRE(graph && graph->exec, OMPL_AMD_ERR_ARG, "null graph");
-        OA_HIP(hipGraphLaunch(graph->exec, resolve_stream(stream)));
+        OA_REQUIRE(graph && graph->exec[0] && graph->exec[1], OMPL_AMD_ERR_ARG, "null graph");
+        OA_HIP(hipGraphLaunch(graph->exec[graph->next], resolve_stream(stream)));
+        graph->next ^= 1;
     });
 }
 

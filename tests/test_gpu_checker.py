@@ -351,11 +351,52 @@ def test_empty_batches(gpu):
     assert gpu.validate_states(env, z.float()).numel() == 0
 
 
-@pytest.mark.parametrize("strategy,eshape,wshape", [(4, (8, 8), (8, 8)), (3, (8, 2), (8, 1)), (1, (8, 1), (1, 1)),
-                                                    (0, (8, 1), (8, 8)), (2, (8, 2), (8, 8))])
-def test_motion_graph_replays_match_eager(gpu, oracle, strategy, eshape, wshape):
+@pytest.mark.parametrize("counts", [True, False])
+def test_motion_graph_group_replays(gpu, oracle, counts):
+    """Three independent batches of different sizes captured in one graph on
+    forked streams: every replay gives each batch's eager result, with eager
+    batches in between."""
+    import torch
+
+    set_strategy(gpu, 3, (7, 4), (7, 9))
+    env, oenv, _ = env_pair(gpu, oracle, 1001, 128, 128)
+    lvs = lvs_config()
+    cases = [motion_case(gpu, 11 + i, n) for i, n in enumerate((4096, 1000, 2500))]
+    refs = [oracle.check_motions(oenv, a, b, lvs) for a, b in cases]
+    batches = [(to_dev(a.T.copy()), to_dev(b.T.copy())) for a, b in cases]
+    checked = [torch.empty(a.shape[0], dtype=torch.int32, device="cuda") for a, _ in cases] if counts else None
+    g = gpu.MotionGraph.group(env, batches, lvs, checked=checked)
+    a2, b2 = motion_case(gpu, 5, 2048)
+    A2, B2 = to_dev(a2.T.copy()), to_dev(b2.T.copy())
+    ref2, _ = oracle.check_motions(oenv, a2, b2, lvs)
+    for rep in range(5):
+        for o in g.out:
+            o.fill_(7)
+        if counts:
+            for c in checked:
+                c.fill_(-1)
+        g.launch()
+        torch.cuda.synchronize()
+        for i, (rv, rc) in enumerate(refs):
+            assert np.array_equal(g.out[i].cpu().numpy(), rv), (rep, i)
+            if counts:
+                assert np.array_equal(checked[i].cpu().numpy(), rc), (rep, i)
+        v2 = gpu.check_motions(env, A2, B2, lvs)
+        torch.cuda.synchronize()
+        assert np.array_equal(v2.cpu().numpy(), ref2), rep
+    g.close()
+    gpu.set_tuning(**DEFAULT)
+
+
+@pytest.mark.parametrize("strategy,eshape,wshape", [(4, (8, 8), (8, 8)), (3, (8, 2), (8, 1)), (3, (7, 4), (7, 9)),
+                                                    (1, (8, 1), (1, 1)), (0, (8, 1), (8, 8)), (2, (8, 2), (8, 8))])
+@pytest.mark.parametrize("counts", [True, False])
+def test_motion_graph_replays_match_eager(gpu, oracle, strategy, eshape, wshape, counts):
     """A captured batch replayed several times, with eager batches and host
-    synchronisations in between, gives the eager result on every replay."""
+    synchronisations in between, gives the eager result on every replay --
+    with checked counts (a results kernel in the graph) and without (the
+    replays alternate between the batch's two recordings, one per survivor
+    counter set)."""
     import torch
 
     set_strategy(gpu, strategy, eshape, wshape)
@@ -364,18 +405,20 @@ def test_motion_graph_replays_match_eager(gpu, oracle, strategy, eshape, wshape)
     lvs = lvs_config()
     ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs)
     A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
-    checked = torch.empty(4096, dtype=torch.int32, device="cuda")
+    checked = torch.empty(4096, dtype=torch.int32, device="cuda") if counts else None
     g = gpu.MotionGraph(env, A, B, lvs, checked=checked)
     a2, b2 = motion_case(gpu, 5, 2048)
     A2, B2 = to_dev(a2.T.copy()), to_dev(b2.T.copy())
     ref2, _ = oracle.check_motions(oenv, a2, b2, lvs)
-    for rep in range(4):
+    for rep in range(5):
         g.out.fill_(7)
-        checked.fill_(-1)
+        if counts:
+            checked.fill_(-1)
         g.launch()
         torch.cuda.synchronize()
         assert np.array_equal(g.out.cpu().numpy(), ref_v), rep
-        assert np.array_equal(checked.cpu().numpy(), ref_c), rep
+        if counts:
+            assert np.array_equal(checked.cpu().numpy(), ref_c), rep
         v2 = gpu.check_motions(env, A2, B2, lvs)  # eager work between replays
         torch.cuda.synchronize()
         assert np.array_equal(v2.cpu().numpy(), ref2), rep
