@@ -441,6 +441,20 @@ def knn_row(oa, rows, oracle_py, cpu_threads, roof, timed, NearestNeighborsGPU):
     ms = timed(lambda: nn.knn_dev(qsd, 10, oi, od), 20)
     row = {"db": 100_000, "queries": 4096, "k": 10, "ms": ms, "queries_per_s": 4096 / (ms * 1e-3),
            "roofline": roof(20.0 * 100_000 * 4096, ms)}
+    # packed batch: 16 concurrent requests of 64..512 queries with k in
+    # {1, 4, 10, 16} in one launch, against one launch per request
+    prng = np.random.default_rng(17)
+    sizes = prng.integers(64, 513, 16)
+    kreq = prng.choice([1, 4, 10, 16], 16)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    pq = torch.from_numpy(qs[np.arange(offs[-1]) % 4096]).cuda()
+    pk = torch.from_numpy(np.repeat(kreq, sizes).astype(np.int32)).cuda()
+    ms_packed = timed(lambda: nn.knn_packed_dev(pq, pk, 16, trim=False), 20)
+    parts = [(pq[offs[i]:offs[i + 1]], int(kreq[i])) for i in range(16)]
+    ms_each = timed(lambda: [nn.knn_dev(qp, kp) for qp, kp in parts], 20)
+    row["packed_requests"] = {"requests": 16, "queries": int(offs[-1]), "k_per_request": [int(v) for v in kreq],
+                              "ms_one_launch": ms_packed, "ms_one_launch_per_request": ms_each,
+                              "speedup": ms_each / ms_packed}
     if oracle_py is not None:
         t0 = time.perf_counter()
         ri, _ = oracle_py.knn_mt(0, db, qs[:512], 10, cpu_threads)

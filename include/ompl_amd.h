@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 308
+#define OMPL_AMD_VERSION 309
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -332,6 +332,19 @@ int ompl_amd_nn_knn_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, int k, i
 int ompl_amd_nn_radius_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, float radius,
                            int64_t *out_offsets_dev, int32_t *out_idx_dev, float *out_dist_dev, int64_t capacity,
                            int64_t *total, void *stream);
+/* Packed batches: the queries of several requests in one launch, each query
+ * with its own k or radius -- NearestNeighbors::nearestK / nearestR
+ * (datastructures/NearestNeighbors.h:97, 104) for many callers at once.
+ * k-NN: k_dev [nq] (clamped to [0, kmax], kmax in [1, 64]); results are CSR:
+ * out_offsets_dev [nq+1] is written (exclusive scan of the clamped k) and
+ * out_idx_dev / out_dist_dev must hold nq * kmax entries (or the sum of k).
+ * Radius: r_dev [nq] (a negative or NaN radius gives no results); otherwise
+ * as ompl_amd_nn_radius_dev. */
+int ompl_amd_nn_knn_packed_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, const int32_t *k_dev, int kmax,
+                               int64_t *out_offsets_dev, int32_t *out_idx_dev, float *out_dist_dev, void *stream);
+int ompl_amd_nn_radius_packed_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq, const float *r_dev,
+                                  int64_t *out_offsets_dev, int32_t *out_idx_dev, float *out_dist_dev,
+                                  int64_t capacity, int64_t *total, void *stream);
 /* Host convenience (blocking). */
 int ompl_amd_nn_knn(ompl_amd_nn nn, const float *q, int64_t nq, int k, int32_t *out_idx, float *out_dist);
 int ompl_amd_nn_radius(ompl_amd_nn nn, const float *q, int64_t nq, float radius, int64_t *out_offsets,

@@ -174,8 +174,11 @@ struct KnnArgs {
     // k-NN: splits searched by one workgroup (one wave each, same query
     // tile), and the bound slots its lanes share per query
     int group_waves; // W
-    int bound_slots; // M = min(k, 2 W)
-    int bound_rank;  // c = ceil(k / M): a lane publishes its c-th smallest key
+    // packed batches: per-query k (clamped to [0, k]; k is then the largest)
+    // with results at out_off[q] (one split), per-query radius
+    const int32_t *kq;
+    const int64_t *out_off;
+    const float *rq;
 };
 
 // k-NN lanes exchange their bounds after each of a split's first 4 tiles,
@@ -343,13 +346,25 @@ knn_kernel(KnnArgs A)
     const int64_t qtiles = (A.nq + 31) / 32;
     const int64_t qtile = blockIdx.x % qtiles;
     const int split = (int)(blockIdx.x / qtiles) * A.group_waves + wid;
-    const int M = A.bound_slots;
-    for (int i = threadIdx.x; i < 32 * M; i += blockDim.x) slots[i] = ~0ull;
+    const int SW = 2 * A.group_waves; // slots per query in LDS
+    for (int i = threadIdx.x; i < 32 * SW; i += blockDim.x) slots[i] = ~0ull;
     __syncthreads();
     QueryLane<METRIC, DIM, KSTEPS> Q;
     Q.load(A, qtile, lane);
+    // the query's k: the batch's, or its own in a packed batch (a query with
+    // k = 0 is not searched)
+    int kq = A.k;
+    if (A.kq && Q.live) {
+        const int v = A.kq[Q.qi];
+        kq = v < 0 ? 0 : (v > A.k ? A.k : v);
+    }
+    if (kq == 0) Q.live = false;
+    // M = min(k, 2 W) slots per query, each lane publishing its c-th smallest
+    // key, c = ceil(k / M)
+    const int M = kq < 1 ? 1 : (kq < SW ? kq : SW);
+    const int crank = (kq + M - 1) / M;
     TopKey<KMAX> top;
-    top.init(A.k);
+    top.init(kq);
     // bkey: a key that at least k distinct points of the query are at or
     // below, so no point of its k nearest is above it: a point is kept only
     // below it, and the filter threshold follows it.  Sources: the lane's own
@@ -362,7 +377,7 @@ knn_kernel(KnnArgs A)
     // itself, and no other lane sees that point.
     float thr = INFINITY;
     unsigned long long bkey = ~0ull, published = ~0ull;
-    unsigned long long *qslots = slots + (lane & 31) * M;
+    unsigned long long *qslots = slots + (lane & 31) * SW;
     const int my_slot = (2 * wid + h) % M;
     const int64_t t0 = (int64_t)split * A.tiles_per_split;
     const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
@@ -393,7 +408,7 @@ knn_kernel(KnnArgs A)
             const unsigned long long partner = shfl_xor_u64(best, 32);
             if (partner < best) best = partner;
             // publish this lane's c-th smallest key (~0 while it has fewer)
-            const int at = KMAX - A.k + A.bound_rank - 1;
+            const int at = KMAX - kq + crank - 1;
             unsigned long long mine = ~0ull;
 #pragma unroll
             for (int s = 0; s < KMAX; ++s)
@@ -407,7 +422,7 @@ knn_kernel(KnnArgs A)
             unsigned long long b = 0;
 #pragma unroll
             for (int s = 0; s < 2 * MW; ++s) {
-                if (s < M) {
+                if (s < M) { // M of this lane's query
                     const unsigned long long v =
                         __hip_atomic_load(qslots + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     b = v > b ? v : b;
@@ -483,7 +498,7 @@ knn_kernel(KnnArgs A)
     // hands over its first slot and shifts its list down, one key a round
     // (one exchange in flight at a time keeps the register count of the
     // search loop)
-    const int skip = KMAX - A.k;
+    const int skip = KMAX - kq;
 #pragma unroll 1
     for (int r = 0; r < KMAX; ++r) {
         const unsigned long long ok = shfl_xor_u64(top.key[0], 32);
@@ -496,15 +511,24 @@ knn_kernel(KnnArgs A)
         }
     }
     if (h != 0 || !Q.live) return;
-    const int64_t base = ((int64_t)split * A.nq + Q.qi) * A.k - skip;
+    // one split of a packed batch writes its kq results at the query's
+    // offset; otherwise each list has k entries (a split's list of a packed
+    // query is padded with empty entries for the merge)
+    const bool packed_out = A.out_off && A.splits == 1;
+    const int64_t base = packed_out ? A.out_off[Q.qi] : ((int64_t)split * A.nq + Q.qi) * A.k;
 #pragma unroll
     for (int s = 0; s < KMAX; ++s) {
         if (s >= skip) {
             const unsigned long long kk = top.key[s];
-            A.out_d[base + s] = kk == ~0ull ? INFINITY : __uint_as_float((uint32_t)(kk >> 32));
-            A.out_i[base + s] = kk == ~0ull ? -1 : (int32_t)(uint32_t)kk;
+            A.out_d[base + s - skip] = kk == ~0ull ? INFINITY : __uint_as_float((uint32_t)(kk >> 32));
+            A.out_i[base + s - skip] = kk == ~0ull ? -1 : (int32_t)(uint32_t)kk;
         }
     }
+    if (!packed_out)
+        for (int j = kq; j < A.k; ++j) {
+            A.out_d[base + j] = INFINITY;
+            A.out_i[base + j] = -1;
+        }
 }
 
 // Merges the per-split partial lists of each query: one wave per query.
@@ -516,11 +540,19 @@ knn_kernel(KnnArgs A)
 template <int KMAX>
 __global__ void __launch_bounds__(256) knn_merge_kernel(const float *__restrict__ pd, const int32_t *__restrict__ pi,
                                                         int64_t nq, int k, int splits, float *__restrict__ out_d,
-                                                        int32_t *__restrict__ out_i)
+                                                        int32_t *__restrict__ out_i, const int32_t *__restrict__ kq,
+                                                        const int64_t *__restrict__ out_off)
 {
     const int lane = threadIdx.x & 63;
     const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (q >= nq) return; // wave-uniform
+    // packed batch: the query's own k (clamped to [0, k]) and output offset
+    int kout = k;
+    if (kq) {
+        const int v = kq[q];
+        kout = v < 0 ? 0 : (v > k ? k : v);
+    }
+    const int64_t obase = out_off ? out_off[q] : q * k;
     TopK<KMAX> top;
     top.init();
     const int total = splits * k;
@@ -530,7 +562,7 @@ __global__ void __launch_bounds__(256) knn_merge_kernel(const float *__restrict_
         const int id = pi[at];
         if (id >= 0) top.insert(pd[at], id);
     }
-    for (int r = 0; r < k; ++r) {
+    for (int r = 0; r < kout; ++r) {
         float bd = top.d[0];
         int bi = top.i[0];
 #pragma unroll
@@ -552,8 +584,8 @@ __global__ void __launch_bounds__(256) knn_merge_kernel(const float *__restrict_
             top.i[KMAX - 1] = INT_MAX;
         }
         if (lane == 0) {
-            out_d[q * k + r] = bd;
-            out_i[q * k + r] = bi == INT_MAX ? -1 : bi;
+            out_d[obase + r] = bd;
+            out_i[obase + r] = bi == INT_MAX ? -1 : bi;
         }
     }
 }
@@ -573,7 +605,11 @@ __global__ void __launch_bounds__(256) radius_kernel(KnnArgs A, uint64_t *__rest
     if (split >= A.splits) return;
     QueryLane<METRIC, DIM, KSTEPS> Q;
     Q.load(A, qtile, lane);
-    const float r = A.radius;
+    // the batch's radius, or the query's own in a packed batch (a query whose
+    // radius is negative or not a number has no results)
+    float r = A.radius;
+    if (A.rq && Q.live) r = A.rq[Q.qi];
+    if (!(r >= 0.0f)) Q.live = false;
     const float thr = r * r * (1.0f + 8e-6f) + Q.margin;
     const int64_t t0 = (int64_t)split * A.tiles_per_split;
     const int64_t t1 = t0 + A.tiles_per_split < A.n_tiles ? t0 + A.tiles_per_split : A.n_tiles;
@@ -859,11 +895,31 @@ static KnnArgs base_args(const NNIndex &x, const float *q, int64_t nq)
     return A;
 }
 
-static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, float *out_d, hipStream_t s)
+__global__ void packed_counts_kernel(const int32_t *__restrict__ kq, int64_t nq, int kmax, int64_t *__restrict__ cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nq) return;
+    if (i == nq) {
+        cnt[i] = 0;
+        return;
+    }
+    const int v = kq[i];
+    cnt[i] = v < 0 ? 0 : (v > kmax ? kmax : v);
+}
+
+// k nearest of nq queries.  Uniform k (kq null): results [nq][k].  Packed
+// batch (kq non-null, per-query k clamped to [0, k]): results CSR at
+// out_off, which this call writes (exclusive scan of the clamped k).
+static void knn(NNIndex &x, const float *q, int64_t nq, int k, const int32_t *kq, int64_t *out_off, int32_t *out_i,
+                float *out_d, hipStream_t s)
 {
     OA_REQUIRE(k >= 1 && k <= 64, OMPL_AMD_ERR_ARG, "k must be in [1, 64] (got %d)", k);
     OA_REQUIRE(nq >= 0, OMPL_AMD_ERR_ARG, "negative query count");
-    if (nq == 0) return;
+    OA_REQUIRE(!kq || out_off, OMPL_AMD_ERR_ARG, "a packed batch needs an offsets array");
+    if (nq == 0) {
+        if (out_off) OA_HIP(hipMemsetAsync(out_off, 0, sizeof(int64_t), s));
+        return;
+    }
     OA_REQUIRE(q && out_i && out_d, OMPL_AMD_ERR_ARG, "null pointer");
     OA_REQUIRE(x.n < INT32_MAX, OMPL_AMD_ERR_ARG, "too many points for int32 ids");
     const int kmax = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
@@ -872,6 +928,8 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
     const int kmax_search = k <= 10 ? 10 : kmax;
     KnnArgs A = base_args(x, q, nq);
     A.k = k;
+    A.kq = kq;
+    A.out_off = kq ? out_off : nullptr;
     // splits of the point tiles, searched W to a workgroup (W a power of two
     // up to the kernel's maximum; the split count rounded up to a multiple)
     int64_t splits = x.n == 0 ? 1 : choose_splits(nq, A.n_tiles);
@@ -881,35 +939,49 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
     splits = (splits + W - 1) / W * W;
     A.splits = (int)splits;
     A.group_waves = W;
-    A.bound_slots = std::min(k, 2 * W);
-    A.bound_rank = (k + A.bound_slots - 1) / A.bound_slots;
     A.tiles_per_split = (A.n_tiles + splits - 1) / splits;
     const int64_t qtiles = (nq + 31) / 32;
+    // one scratch allocation: packed counts and scan space, then the splits'
+    // partial lists
+    size_t cub_bytes = 0;
+    if (kq)
+        OA_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, cub_bytes, (int64_t *)nullptr, (int64_t *)nullptr,
+                                                (int)(nq + 1), s));
+    const size_t counts_bytes = kq ? ((size_t)(nq + 1) * sizeof(int64_t) + 255) / 256 * 256 : 0;
+    const size_t cub_span = (cub_bytes + 255) / 256 * 256;
+    const size_t per = splits > 1 ? (size_t)splits * nq * k : 0;
+    char *buf = (char *)scratch(x, counts_bytes + cub_span + per * (sizeof(float) + sizeof(int32_t)) + 256);
     ProfScope ps("nn_knn", s);
+    if (kq) {
+        int64_t *cnt = (int64_t *)buf;
+        hipLaunchKernelGGL(packed_counts_kernel, dim3((unsigned)((nq + 1 + 255) / 256)), dim3(256), 0, s, kq, nq, k,
+                           cnt);
+        OA_HIP(hipcub::DeviceScan::ExclusiveSum(buf + counts_bytes, cub_bytes, cnt, out_off, (int)(nq + 1), s));
+    }
     if (splits == 1) {
         // one split (or an empty structure: every slot is (-1, +inf))
         A.out_d = out_d;
         A.out_i = out_i;
         dispatch_search<0>(x, A, kmax_search, nullptr, qtiles, s, false);
     } else {
-        const size_t per = (size_t)splits * nq * k;
-        char *buf = (char *)scratch(x, per * (sizeof(float) + sizeof(int32_t)));
-        A.out_d = (float *)buf;
-        A.out_i = (int32_t *)(buf + per * sizeof(float));
+        char *lists = buf + counts_bytes + cub_span;
+        A.out_d = (float *)lists;
+        A.out_i = (int32_t *)(lists + per * sizeof(float));
         dispatch_search<0>(x, A, kmax_search, nullptr, qtiles * splits, s, false);
         const unsigned blocks = (unsigned)((nq + 3) / 4); // one wave per query
+        const int64_t *moff = kq ? out_off : nullptr;
         switch (kmax) {
         case 16:
             hipLaunchKernelGGL(knn_merge_kernel<16>, dim3(blocks), dim3(256), 0, s, A.out_d, A.out_i, nq, k,
-                               (int)splits, out_d, out_i);
+                               (int)splits, out_d, out_i, kq, moff);
             break;
         case 32:
             hipLaunchKernelGGL(knn_merge_kernel<32>, dim3(blocks), dim3(256), 0, s, A.out_d, A.out_i, nq, k,
-                               (int)splits, out_d, out_i);
+                               (int)splits, out_d, out_i, kq, moff);
             break;
         default:
             hipLaunchKernelGGL(knn_merge_kernel<64>, dim3(blocks), dim3(256), 0, s, A.out_d, A.out_i, nq, k,
-                               (int)splits, out_d, out_i);
+                               (int)splits, out_d, out_i, kq, moff);
             break;
         }
     }
@@ -922,11 +994,13 @@ static void knn(NNIndex &x, const float *q, int64_t nq, int k, int32_t *out_i, f
 constexpr int kRadiusSlotCap = 64;
 constexpr size_t kRadiusSlotBytesMax = (size_t)256 << 20;
 
-static void radius(NNIndex &x, const float *q, int64_t nq, float r, int64_t *offsets, int32_t *out_i, float *out_d,
-                   int64_t capacity, int64_t *total, hipStream_t s)
+// Radius search of nq queries, at radius r or, in a packed batch (rq
+// non-null), at each query's own radius.
+static void radius(NNIndex &x, const float *q, int64_t nq, float r, const float *rq, int64_t *offsets, int32_t *out_i,
+                   float *out_d, int64_t capacity, int64_t *total, hipStream_t s)
 {
     OA_REQUIRE(nq >= 0 && offsets && total, OMPL_AMD_ERR_ARG, "bad radius arguments");
-    OA_REQUIRE(r >= 0.0f, OMPL_AMD_ERR_ARG, "negative radius");
+    OA_REQUIRE(rq || r >= 0.0f, OMPL_AMD_ERR_ARG, "negative radius");
     *total = 0;
     if (nq == 0) {
         OA_HIP(hipMemsetAsync(offsets, 0, sizeof(int64_t), s));
@@ -936,6 +1010,7 @@ static void radius(NNIndex &x, const float *q, int64_t nq, float r, int64_t *off
     const bool want = out_i || out_d;
     KnnArgs A = base_args(x, q, nq);
     A.radius = r;
+    A.rq = rq;
     const int64_t n_tiles = A.n_tiles;
     const int64_t splits = n_tiles ? choose_splits(nq, n_tiles) : 1;
     A.splits = (int)splits;
@@ -1138,14 +1213,32 @@ int ompl_amd_nn_build(ompl_amd_nn h, void *stream)
 
 int ompl_amd_nn_knn_dev(ompl_amd_nn h, const float *q, int64_t nq, int k, int32_t *out_i, float *out_d, void *stream)
 {
-    return guarded([&] { knn(nn_of(h), q, nq, k, out_i, out_d, resolve_stream(stream)); });
+    return guarded([&] { knn(nn_of(h), q, nq, k, nullptr, nullptr, out_i, out_d, resolve_stream(stream)); });
 }
 
 int ompl_amd_nn_radius_dev(ompl_amd_nn h, const float *q, int64_t nq, float r, int64_t *offsets, int32_t *out_i,
                            float *out_d, int64_t capacity, int64_t *total, void *stream)
 {
     return guarded(
-        [&] { radius(nn_of(h), q, nq, r, offsets, out_i, out_d, capacity, total, resolve_stream(stream)); });
+        [&] { radius(nn_of(h), q, nq, r, nullptr, offsets, out_i, out_d, capacity, total, resolve_stream(stream)); });
+}
+
+int ompl_amd_nn_knn_packed_dev(ompl_amd_nn h, const float *q, int64_t nq, const int32_t *k, int kmax,
+                               int64_t *out_offsets, int32_t *out_i, float *out_d, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(k && out_offsets, OMPL_AMD_ERR_ARG, "null per-query k or offsets");
+        knn(nn_of(h), q, nq, kmax, k, out_offsets, out_i, out_d, resolve_stream(stream));
+    });
+}
+
+int ompl_amd_nn_radius_packed_dev(ompl_amd_nn h, const float *q, int64_t nq, const float *r, int64_t *offsets,
+                                  int32_t *out_i, float *out_d, int64_t capacity, int64_t *total, void *stream)
+{
+    return guarded([&] {
+        OA_REQUIRE(r, OMPL_AMD_ERR_ARG, "null per-query radius");
+        radius(nn_of(h), q, nq, 0.0f, r, offsets, out_i, out_d, capacity, total, resolve_stream(stream));
+    });
 }
 
 int ompl_amd_nn_knn(ompl_amd_nn h, const float *q, int64_t nq, int k, int32_t *out_i, float *out_d)
@@ -1161,7 +1254,7 @@ int ompl_amd_nn_knn(ompl_amd_nn h, const float *q, int64_t nq, int k, int32_t *o
         OA_HIP(hipMallocAsync((void **)&dd, (size_t)nq * k * sizeof(float), s));
         OA_HIP(hipMallocAsync((void **)&di, (size_t)nq * k * sizeof(int32_t), s));
         OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
-        knn(x, dq, nq, k, di, dd, s);
+        knn(x, dq, nq, k, nullptr, nullptr, di, dd, s);
         OA_HIP(hipMemcpyAsync(out_d, dd, (size_t)nq * k * sizeof(float), hipMemcpyDeviceToHost, s));
         OA_HIP(hipMemcpyAsync(out_i, di, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         OA_HIP(hipFreeAsync(dq, s));
@@ -1191,7 +1284,7 @@ int ompl_amd_nn_radius(ompl_amd_nn h, const float *q, int64_t nq, float r, int64
         }
         if (nq) OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
         int64_t tot = 0;
-        radius(x, dq, nq, r, doff, want ? di : nullptr, want ? dd : nullptr, capacity, &tot, s);
+        radius(x, dq, nq, r, nullptr, doff, want ? di : nullptr, want ? dd : nullptr, capacity, &tot, s);
         OA_HIP(hipMemcpyAsync(offsets, doff, (size_t)(nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         if (want && tot > 0) {
             if (out_i) OA_HIP(hipMemcpyAsync(out_i, di, (size_t)tot * sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -117,6 +117,44 @@ class NearestNeighborsGPU:
             self._radius_per_query = -(-int(total.value) // nq)
         return off, idx[:total.value], dist[:total.value]
 
+    def knn_packed_dev(self, q_dev, k_dev, kmax: int, stream=None, trim: bool = True):
+        """Packed batch: each query with its own k (int32 device array,
+        clamped to [0, kmax]).  Returns CSR (offsets [nq+1], ids, distances);
+        with trim=False the id and distance arrays keep their nq * kmax
+        capacity and the call does not wait for the device."""
+        torch = _torch()
+        nq = q_dev.shape[0]
+        off = torch.empty(nq + 1, dtype=torch.int64, device=q_dev.device)
+        idx = torch.empty(max(nq * kmax, 1), dtype=torch.int32, device=q_dev.device)
+        dist = torch.empty(max(nq * kmax, 1), dtype=torch.float32, device=q_dev.device)
+        call("ompl_amd_nn_knn_packed_dev", self.handle, _ptr(q_dev), nq, _ptr(k_dev), int(kmax), _ptr(off),
+             _ptr(idx), _ptr(dist), _stream(stream))
+        if not trim:
+            return off, idx, dist
+        total = int(off[-1].item()) if nq else 0
+        return off, idx[:total], dist[:total]
+
+    def radius_packed_dev(self, q_dev, r_dev, stream=None):
+        """Packed batch: each query with its own radius (float32 device
+        array).  Returns CSR (offsets [nq+1], ids, distances)."""
+        torch = _torch()
+        nq = q_dev.shape[0]
+        off = torch.empty(nq + 1, dtype=torch.int64, device=q_dev.device)
+        total = C.c_int64()
+        cap = max(1, min(nq * 16, 1 << 26))
+        for _ in range(2):
+            idx = torch.empty(cap, dtype=torch.int32, device=q_dev.device)
+            dist = torch.empty(cap, dtype=torch.float32, device=q_dev.device)
+            code = load().ompl_amd_nn_radius_packed_dev(self.handle, _ptr(q_dev), nq, _ptr(r_dev), _ptr(off),
+                                                        _ptr(idx), _ptr(dist), cap, C.byref(total), _stream(stream))
+            if code == 0 or total.value <= cap:
+                check(code)
+                break
+            cap = total.value  # outputs too small: offsets and total are written
+        else:
+            check(code)
+        return off, idx[:total.value], dist[:total.value]
+
     def radius_batch(self, q, r: float):
         torch = _torch()
         qd = torch.from_numpy(np.ascontiguousarray(q, np.float32)).cuda()

@@ -229,3 +229,60 @@ def test_radius_capacity_too_small(gpu, oracle):
     for j, (oi, od) in enumerate(ores):
         assert np.array_equal(gi[off2[j]:off2[j + 1]], oi), j
         assert np.array_equal(gd[off2[j]:off2[j + 1]], od), j
+
+
+@pytest.mark.parametrize("n", [300, 20000])
+def test_knn_packed_mixed_requests(gpu, oracle, n):
+    """Queries of four requests packed in one launch, each request with its
+    own k (1, 10, 33, 64) and some queries with k = 0 or above the batch's
+    largest: every query's list equals the oracle's at its k (clamped), in
+    the CSR layout; one split (300 points) and many (20,000)."""
+    import torch
+
+    rng = np.random.default_rng(40 + n)
+    data = rng.uniform(-1, 1, (n, 7)).astype(np.float32)
+    nn = make_nn(gpu, 0, 7, data)
+    ks = [1, 10, 33, 64]
+    sizes = [37, 100, 5, 50]
+    q = rng.uniform(-1, 1, (sum(sizes), 7)).astype(np.float32)
+    kq = np.concatenate([np.full(m, k, np.int32) for m, k in zip(sizes, ks)])
+    kq[3] = 0
+    kq[60] = 200  # clamped to the batch's largest
+    kmax = 64
+    off, gi, gd = nn.knn_packed_dev(torch.from_numpy(q).cuda(), torch.from_numpy(kq).cuda(), kmax)
+    off, gi, gd = off.cpu().numpy(), gi.cpu().numpy(), gd.cpu().numpy()
+    want_k = np.clip(kq, 0, kmax)
+    assert np.array_equal(np.diff(off), want_k)
+    for k in sorted(set(want_k.tolist()) - {0}):
+        sel = np.nonzero(want_k == k)[0]
+        oi, od = oracle.knn(0, data, q[sel], k)
+        for j, qi in enumerate(sel):
+            assert np.array_equal(gi[off[qi]:off[qi + 1]], oi[j]), (k, qi)
+            assert np.array_equal(gd[off[qi]:off[qi + 1]], od[j]), (k, qi)
+
+
+def test_radius_packed_mixed_requests(gpu, oracle):
+    """Queries of three requests packed in one launch, each with its own
+    radius, plus a negative and a NaN radius (no results): every query's list
+    equals the oracle's at its radius."""
+    import torch
+
+    rng = np.random.default_rng(44)
+    data = rng.uniform(-1, 1, (5000, 7)).astype(np.float32)
+    nn = make_nn(gpu, 0, 7, data)
+    radii = [0.4, 0.9, 1.2]
+    sizes = [40, 30, 20]
+    q = rng.uniform(-1, 1, (sum(sizes), 7)).astype(np.float32)
+    rq = np.concatenate([np.full(m, r, np.float32) for m, r in zip(sizes, radii)])
+    rq[5] = -1.0
+    rq[50] = np.nan
+    off, gi, gd = nn.radius_packed_dev(torch.from_numpy(q).cuda(), torch.from_numpy(rq).cuda())
+    off, gi, gd = off.cpu().numpy(), gi.cpu().numpy(), gd.cpu().numpy()
+    for qi in range(len(q)):
+        lo, hi = off[qi], off[qi + 1]
+        if not rq[qi] >= 0:
+            assert hi == lo, qi
+            continue
+        (oi, od), = oracle.radius(0, data, q[qi:qi + 1], float(rq[qi]))
+        assert np.array_equal(gi[lo:hi], oi), qi
+        assert np.array_equal(gd[lo:hi], od), qi
