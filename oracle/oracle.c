@@ -717,8 +717,15 @@ enum { ORC_TRAPPED = 0, ORC_ADVANCED = 1, ORC_REACHED = 2 };
 
 /* growTree: nearest (first strict minimum of distance(elem, target)), steer
  * by max_distance, check the motion in the direction of tree growth. */
+/* growTree (RRTConnect.cpp:116-196).  With addIntermediateStates
+ * (:148-183) the motion's getMotionStates (SpaceInformation.cpp:215-286) with
+ * count = validSegmentCount(a, b) intermediate states -- count + 1 segments,
+ * states a + (b - a) j / (count + 1) -- are added as a chain: from the start
+ * tree (a = nearest, b = new state) states 1 .. count + 1, from the goal tree
+ * (a = new state, b = nearest) states count .. 0, so the last added is always
+ * the new state. */
 static int grow_tree(const orc_env *env, orc_tree *t, int start_tree, const double *target, double maxd, double lvs,
-                     unsigned factor, int *xmotion)
+                     unsigned factor, int intermediate, int *xmotion)
 {
     int nm = 0;
     double dmin = 0.0;
@@ -729,7 +736,8 @@ static int grow_tree(const orc_env *env, orc_tree *t, int start_tree, const doub
             dmin = d;
         }
     }
-    const double *ns = t->s + (size_t)nm * ORC_ARM_DOF;
+    double ns[ORC_ARM_DOF];
+    memcpy(ns, t->s + (size_t)nm * ORC_ARM_DOF, sizeof(ns)); /* the tree may grow below */
     int reach = 1;
     double x[ORC_ARM_DOF];
     const double *dst = target;
@@ -749,41 +757,97 @@ static int grow_tree(const orc_env *env, orc_tree *t, int start_tree, const doub
     else
         valid = orc_state_valid_d(env, dst, NULL) && orc_check_motion(env, dst, ns, lvs, factor, NULL);
     if (!valid) return ORC_TRAPPED;
-    tree_push(t, dst, nm);
+    if (intermediate) {
+        const double *a = start_tree ? ns : dst, *b = start_tree ? dst : ns;
+        const unsigned count = orc_valid_segment_count(a, b, ORC_ARM_DOF, lvs, factor);
+        const unsigned seg = count + 1;
+        double st[ORC_ARM_DOF];
+        int parent = nm;
+        for (unsigned i = 1; i <= seg; ++i) {
+            /* start tree: state i; goal tree: state seg - i (0 = the new state) */
+            const unsigned j = start_tree ? i : seg - i;
+            if (j == 0) memcpy(st, a, sizeof(st));
+            else if (j == seg) memcpy(st, b, sizeof(st));
+            else orc_interpolate_d(a, b, (double)j / (double)seg, ORC_ARM_DOF, st);
+            tree_push(t, st, parent);
+            parent = t->n - 1;
+        }
+    } else {
+        tree_push(t, dst, nm);
+    }
     *xmotion = t->n - 1;
     return reach ? ORC_REACHED : ORC_ADVANCED;
 }
 
-int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const double goal[ORC_ARM_DOF],
-                    uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters, double *path,
-                    int path_cap, int *path_len, int *iterations, long *tree_nodes)
+/* SpaceInformation::satisfiesBounds on [-pi, pi]^7
+ * (RealVectorStateSpace.cpp:200-208: within machine epsilon of the bounds). */
+static int in_bounds(const double *q)
+{
+    const double pi = 3.141592653589793238462643383279502884, eps = 2.220446049250313080847e-16;
+    for (int k = 0; k < ORC_ARM_DOF; ++k)
+        if (q[k] - eps > pi || q[k] + eps < -pi) return 0;
+    return 1;
+}
+
+int orc_rrt_connect_ex(const orc_env *env, const double *starts, int n_starts, const double *goals, int n_goals,
+                       uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters,
+                       int intermediate, double *path, int path_cap, int *path_len, int *iterations,
+                       long *tree_nodes)
 {
     const double pi = 3.141592653589793238462643383279502884;
     orc_tree ts = {0}, tg = {0};
-    tree_push(&ts, start, -1);
-    tree_push(&tg, goal, -1);
     orc_mt19937 rng;
     orc_mt_seed(&rng, seed);
-    int start_tree = 1, solved = 0, it = 0;
+    int start_tree = 1, status = ORC_PLAN_FAILED, it = 0, sampled_goals = 0;
     double rstate[ORC_ARM_DOF];
     *path_len = 0;
-    for (it = 0; it < max_iters && !solved; ++it) {
+    /* PlannerInputStates::nextStart (Planner.cpp:228-256): every start state
+     * in order, skipping those out of bounds or invalid */
+    for (int i = 0; i < n_starts; ++i) {
+        const double *q = starts + (size_t)i * ORC_ARM_DOF;
+        if (in_bounds(q) && orc_state_valid_d(env, q, NULL)) tree_push(&ts, q, -1);
+    }
+    if (ts.n == 0) status = ORC_PLAN_INVALID_START;
+    for (it = 0; it < max_iters && ts.n > 0; ++it) {
         orc_tree *tree = start_tree ? &ts : &tg;
         int tgi_start = start_tree;
         start_tree = !start_tree;
         orc_tree *other = start_tree ? &ts : &tg;
+        /* RRTConnect.cpp:251-269: while the goal tree is empty, the next valid
+         * goal state (nextGoal(ptc) tries them all); later, one goal state per
+         * iteration (nextGoal() tries one) while fewer than half the goal
+         * tree's size have been sampled (GoalStates: in order, Planner.cpp:
+         * 265-336) */
+        if (tg.n == 0 || sampled_goals < tg.n / 2) {
+            const int empty = tg.n == 0;
+            while (sampled_goals < n_goals) {
+                const double *q = goals + (size_t)sampled_goals * ORC_ARM_DOF;
+                ++sampled_goals;
+                if (in_bounds(q) && orc_state_valid_d(env, q, NULL)) {
+                    tree_push(&tg, q, -1);
+                    break;
+                }
+                if (!empty) break;
+            }
+            if (tg.n == 0) {
+                status = ORC_PLAN_INVALID_GOAL;
+                break;
+            }
+        }
         for (int k = 0; k < ORC_ARM_DOF; ++k) rstate[k] = orc_uniform_real(&rng, -pi, pi);
         int xmotion = -1;
-        int gs = grow_tree(env, tree, tgi_start, rstate, max_distance, lvs, factor, &xmotion);
+        int gs = grow_tree(env, tree, tgi_start, rstate, max_distance, lvs, factor, intermediate, &xmotion);
         if (gs == ORC_TRAPPED) continue;
         int added = xmotion;
         if (gs != ORC_REACHED) memcpy(rstate, tree->s + (size_t)added * ORC_ARM_DOF, sizeof(rstate));
         tgi_start = start_tree;
-        int gsc = grow_tree(env, other, tgi_start, rstate, max_distance, lvs, factor, &xmotion);
+        int gsc = grow_tree(env, other, tgi_start, rstate, max_distance, lvs, factor, intermediate, &xmotion);
         if (gsc == ORC_TRAPPED) tgi_start = !tgi_start;
-        while (gsc == ORC_ADVANCED) gsc = grow_tree(env, other, tgi_start, rstate, max_distance, lvs, factor, &xmotion);
+        while (gsc == ORC_ADVANCED)
+            gsc = grow_tree(env, other, tgi_start, rstate, max_distance, lvs, factor, intermediate, &xmotion);
         if (gsc != ORC_REACHED) continue;
-        /* tgi.start tells which tree xmotion belongs to */
+        /* tgi.start tells which tree xmotion belongs to; GoalStates accepts
+         * every start/goal pair (Goal::isStartGoalPairValid) */
         int sm = tgi_start ? xmotion : added;
         int gm = tgi_start ? added : xmotion;
         if (ts.parent[sm] >= 0) sm = ts.parent[sm];
@@ -798,7 +862,9 @@ int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const d
         pos = n1;
         for (int m = gm; m >= 0; m = tg.parent[m], ++pos)
             if (pos < path_cap) memcpy(path + (size_t)pos * ORC_ARM_DOF, tg.s + (size_t)m * ORC_ARM_DOF, sizeof(rstate));
-        solved = 1;
+        status = ORC_PLAN_SOLVED;
+        ++it;
+        break;
     }
     *iterations = it;
     if (tree_nodes) *tree_nodes = (long)ts.n + tg.n;
@@ -806,5 +872,13 @@ int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const d
     free(ts.parent);
     free(tg.s);
     free(tg.parent);
-    return solved;
+    return status;
+}
+
+int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const double goal[ORC_ARM_DOF],
+                    uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters, double *path,
+                    int path_cap, int *path_len, int *iterations, long *tree_nodes)
+{
+    return orc_rrt_connect_ex(env, start, 1, goal, 1, seed, max_distance, lvs, factor, max_iters, 0, path, path_cap,
+                              path_len, iterations, tree_nodes) == ORC_PLAN_SOLVED;
 }

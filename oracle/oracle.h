@@ -124,11 +124,29 @@ void orc_knn_batch(int metric, const float *data, long n, int dim, const float *
  * :245-330 solve) on the 7-DOF arm: one start, one goal state, samples from
  * RNG(seed)::uniformReal(-pi, pi) per joint, Linear nearest neighbours in FP64
  * (first strict minimum), DiscreteMotionValidator checks, addIntermediateStates
- * off.  Returns 1 when solved within max_iters iterations; the path (states,
+ * off, start and goal assumed valid.  Returns 1 when solved within max_iters
+ * iterations; the path (states,
  * row-major) goes to path (up to path_cap states, *path_len = full length). */
 int orc_rrt_connect(const orc_env *env, const double start[ORC_ARM_DOF], const double goal[ORC_ARM_DOF],
                     uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters, double *path,
                     int path_cap, int *path_len, int *iterations, long *tree_nodes);
+
+/* The same with several start and goal states (rows of 7) and
+ * addIntermediateStates: start states out of bounds or invalid are skipped
+ * (PlannerInputStates::nextStart, base/src/Planner.cpp:228-256); goal states
+ * are taken in order as GoalStates hands them out (Planner.cpp:265-336,
+ * GoalStates.cpp:81-97), each checked the same way, one at the start of an
+ * iteration while the goal tree is empty (trying every remaining one) or has
+ * more than twice as many states as goal states sampled (trying one),
+ * RRTConnect.cpp:251-269.  Returns one of ORC_PLAN_*. */
+#define ORC_PLAN_FAILED 0
+#define ORC_PLAN_SOLVED 1
+#define ORC_PLAN_INVALID_START 2
+#define ORC_PLAN_INVALID_GOAL 3
+int orc_rrt_connect_ex(const orc_env *env, const double *starts, int n_starts, const double *goals, int n_goals,
+                       uint32_t seed, double max_distance, double lvs, unsigned factor, int max_iters,
+                       int intermediate, double *path, int path_cap, int *path_len, int *iterations,
+                       long *tree_nodes);
 
 /* NearestNeighborsGNAT restatement (gnat.cpp): built in bulk over n points
  * of `dim` floats (metric 0: L2, metric 1: SE3 with dim 7), pivots from

@@ -93,6 +93,9 @@ def lib():
         L.orc_rrt_connect.argtypes = [C.POINTER(Env), vp, vp, C.c_uint32, C.c_double, C.c_double, C.c_uint,
                                       C.c_int, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_long)]
+        L.orc_rrt_connect_ex.argtypes = [C.POINTER(Env), vp, C.c_int, vp, C.c_int, C.c_uint32, C.c_double,
+                                         C.c_double, C.c_uint, C.c_int, C.c_int, vp, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_long)]
         L.orc_gnat_build.argtypes = [C.c_int, vp, C.c_int64, C.c_int, C.c_uint32]
         L.orc_gnat_build.restype = vp
         L.orc_gnat_free.argtypes = [vp]
@@ -239,6 +242,23 @@ def rrt_connect(env: OracleEnv, start, goal, seed: int, max_distance: float, lvs
     ok = lib().orc_rrt_connect(env.ref(), _p(s), _p(g), seed, max_distance, lvs, factor, max_iters, _p(path), cap,
                                C.byref(n), C.byref(it), C.byref(nodes))
     return bool(ok), it.value, path[:min(n.value, cap)].copy(), nodes.value
+
+
+PLAN_FAILED, PLAN_SOLVED, PLAN_INVALID_START, PLAN_INVALID_GOAL = 0, 1, 2, 3
+
+
+def rrt_connect_ex(env: OracleEnv, starts, goals, seed: int, max_distance: float, lvs: float, factor: int = 1,
+                   max_iters: int = 5000, intermediate: bool = False, cap: int = 1 << 16):
+    """RRT-Connect restatement with start and goal lists ([n][7]) and
+    addIntermediateStates -> (status PLAN_*, iterations, path[n][7], tree_nodes)."""
+    s = np.ascontiguousarray(np.reshape(starts, (-1, 7)), np.float64)
+    g = np.ascontiguousarray(np.reshape(goals, (-1, 7)), np.float64)
+    path = np.zeros((cap, 7), np.float64)
+    n, it, nodes = C.c_int(), C.c_int(), C.c_long()
+    st = lib().orc_rrt_connect_ex(env.ref(), _p(s), s.shape[0], _p(g), g.shape[0], seed, max_distance, lvs, factor,
+                                  max_iters, int(bool(intermediate)), _p(path), cap, C.byref(n), C.byref(it),
+                                  C.byref(nodes))
+    return st, it.value, path[:min(n.value, cap)].copy(), nodes.value
 
 
 def validate_states_mt(env: OracleEnv, q_soa: np.ndarray, threads: int) -> np.ndarray:

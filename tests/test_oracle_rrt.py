@@ -164,12 +164,35 @@ class Replay:
     TreeGrowingInfo (xstate, xmotion, start), and the sampler's
     RNG::uniformReal over mt19937(seed)."""
 
-    def __init__(self, oracle, env, start, goal, seed, maxd, lvs):
+    def __init__(self, oracle, env, start, goal, seed, maxd, lvs, starts=None, goals=None, intermediate=False):
         self.o, self.env, self.maxd, self.lvs = oracle, env, maxd, lvs
-        self.t_start = [Motion(np.array(start, np.float64), None)]
-        self.t_goal = [Motion(np.array(goal, np.float64), None)]
+        self.intermediate = intermediate
+        self.starts = [np.array(start, np.float64)] if starts is None else [np.array(q, np.float64) for q in starts]
+        self.goals = [np.array(goal, np.float64)] if goals is None else [np.array(q, np.float64) for q in goals]
+        self.t_start, self.t_goal = [], []
+        self.sampled_goals = 0
         self.rng = oracle.Mt19937(seed)
         self.outcomes = {"trapped_sample": 0, "trapped_first_connect": 0, "advanced": 0, "reached": 0}
+
+    def usable(self, q):
+        """satisfiesBounds (within machine epsilon of [-pi, pi]) and isValid."""
+        eps = 2.220446049250313080847e-16
+        if any(q[k] - eps > PI or q[k] + eps < -PI for k in range(7)):
+            return False
+        return bool(self.o.state_valid(self.env, q.astype(np.float32)))
+
+    def next_goal(self, keep_trying):
+        """PlannerInputStates::nextGoal over GoalStates: the goal states in
+        order, each counted as sampled; nextGoal(ptc) keeps trying, nextGoal()
+        tries one."""
+        while self.sampled_goals < len(self.goals):
+            q = self.goals[self.sampled_goals]
+            self.sampled_goals += 1
+            if self.usable(q):
+                return q
+            if not keep_trying:
+                return None
+        return None
 
     def nearest(self, tree, x):
         best, bd = None, None
@@ -198,6 +221,24 @@ class Replay:
                      and self.o.check_motion(self.env, dstate, nm.state, self.lvs)[0])
         if not valid:
             return "TRAPPED"
+        if self.intermediate:
+            # getMotionStates(a, b, states, validSegmentCount(a, b), true, true):
+            # count + 1 segments, both endpoints; forwards from the start tree
+            # without the first, backwards from the goal tree without the last
+            a = nm.state if tgi["start"] else dstate
+            b = dstate if tgi["start"] else nm.state
+            count = int(math.ceil(math.sqrt(sum((a[k] - b[k]) ** 2 for k in range(7))) / self.lvs))
+            seg = count + 1
+            states = [np.array(a)] + [np.array([a[k] + (b[k] - a[k]) * (j / seg) for k in range(7)])
+                                      for j in range(1, seg)] + [np.array(b)]
+            chain = states[1:] if tgi["start"] else states[-2::-1]
+            m = nm
+            for q in chain:
+                m = Motion(q, m)
+                tree.append(m)
+            tgi["xmotion"] = m
+            tgi["xstate"] = m.state
+            return "REACHED" if reach else "ADVANCED"
         m = Motion(np.array(dstate), nm)
         tree.append(m)
         tgi["xmotion"] = m
@@ -205,6 +246,13 @@ class Replay:
         return "REACHED" if reach else "ADVANCED"
 
     def solve(self, max_iters):
+        """-> (status, iterations, path) with status SOLVED, FAILED,
+        INVALID_START or INVALID_GOAL."""
+        for q in self.starts:  # nextStart: skip unusable start states
+            if self.usable(q):
+                self.t_start.append(Motion(q, None))
+        if not self.t_start:
+            return "INVALID_START", 0, np.zeros((0, 7))
         start_tree = True
         tgi = {"start": True, "xmotion": None, "xstate": None}
         for it in range(max_iters):
@@ -212,6 +260,12 @@ class Replay:
             tgi["start"] = start_tree
             start_tree = not start_tree
             other = self.t_start if start_tree else self.t_goal
+            if len(self.t_goal) == 0 or self.sampled_goals < len(self.t_goal) // 2:
+                q = self.next_goal(keep_trying=len(self.t_goal) == 0)
+                if q is not None:
+                    self.t_goal.append(Motion(q, None))
+                if len(self.t_goal) == 0:
+                    return "INVALID_GOAL", it, np.zeros((0, 7))
             rstate = np.array([self.rng.uniform_real(-PI, PI) for _ in range(7)])
             gs = self.grow(tree, tgi, rstate)
             if gs == "TRAPPED":
@@ -244,8 +298,8 @@ class Replay:
             while gm is not None:
                 p2.append(gm.state)
                 gm = gm.parent
-            return True, it + 1, np.array(p1[::-1] + p2)
-        return False, max_iters, np.zeros((0, 7))
+            return "SOLVED", it + 1, np.array(p1[::-1] + p2)
+        return "FAILED", max_iters, np.zeros((0, 7))
 
 
 @pytest.mark.parametrize("env_seed,ns,nb,maxd", [(1020, 12, 12, MAXD), (1003, 24, 24, 1.5)])
@@ -256,12 +310,68 @@ def test_rrt_connect_matches_independent_replay(oracle, env_seed, ns, nb, maxd):
     for i in range(5):
         start, goal = ok[2 * i], ok[2 * i + 1]
         rp = Replay(oracle, env, start, goal, 40 + i, maxd, lvs)
-        want_ok, want_it, want_path = rp.solve(250)
+        want_status, want_it, want_path = rp.solve(250)
         for k in seen:
             seen[k] += rp.outcomes[k]
         got_ok, got_it, got_path, nodes = oracle.rrt_connect(env, start, goal, 40 + i, maxd, lvs, max_iters=250)
-        assert got_ok == want_ok and got_it == want_it
+        assert got_ok == (want_status == "SOLVED") and got_it == want_it
         assert nodes == len(rp.t_start) + len(rp.t_goal)
         assert np.array_equal(got_path, want_path)
     assert seen["trapped_sample"] > 0 and seen["trapped_first_connect"] > 0 and seen["advanced"] > 0
     assert seen["reached"] > 0
+
+
+STATUS = {"FAILED": 0, "SOLVED": 1, "INVALID_START": 2, "INVALID_GOAL": 3}
+
+
+@pytest.mark.parametrize("intermediate", [False, True])
+@pytest.mark.parametrize("env_seed,ns,nb", [(1020, 12, 12), (1003, 24, 24)])
+def test_rrt_connect_start_goal_lists_match_independent_replay(oracle, env_seed, ns, nb, intermediate):
+    """Several start and goal states, some out of bounds or in collision, and
+    addIntermediateStates: the oracle's status, iteration count, tree sizes and
+    path equal the replay written from RRTConnect.cpp and PlannerInputStates
+    (start states filtered up front; goal states in order, all tried while
+    the goal tree is empty, then one per iteration while fewer than half the
+    goal tree's size have been sampled)."""
+    env, ok = scene(env_seed, ns, nb, oracle)
+    bad = [q for q in __import__("ompl_amd").workload.configs(env_seed + 91, 400)
+           if not oracle.state_valid(env, q.astype(np.float32))]
+    outside = np.array([3.2, 0, 0, 0, 0, 0, 0], np.float64)
+    lvs = lvs_config()
+    multi_goal_roots = 0
+    for i in range(4):
+        starts = [bad[i], outside, ok[10 * i], ok[10 * i + 1]]
+        goals = [bad[i + 1], ok[10 * i + 2], bad[i + 2], ok[10 * i + 3], ok[10 * i + 4]]
+        rp = Replay(oracle, env, None, None, 60 + i, 1.0, lvs, starts=starts, goals=goals,
+                    intermediate=intermediate)
+        want_status, want_it, want_path = rp.solve(300)
+        multi_goal_roots += sum(1 for m in rp.t_goal if m.parent is None) > 1
+        st, it, path, nodes = oracle.rrt_connect_ex(env, starts, goals, 60 + i, 1.0, lvs, max_iters=300,
+                                                    intermediate=intermediate)
+        assert st == STATUS[want_status] and it == want_it, (i, st, want_status, it, want_it)
+        assert nodes == len(rp.t_start) + len(rp.t_goal)
+        assert np.array_equal(path, want_path)
+        if st == 1:
+            # a path starts at a usable start state and ends at a usable goal state
+            assert any(np.array_equal(path[0], q) for q in starts[2:])
+            assert any(np.array_equal(path[-1], q) for q in (goals[1], goals[3], goals[4]))
+    assert multi_goal_roots > 0  # later goal states did enter the goal tree
+
+
+def test_rrt_connect_invalid_start_and_goal(oracle):
+    """No usable start state: INVALID_START before any iteration; no usable
+    goal state: INVALID_GOAL in the first iteration (the sampler untouched)."""
+    env, ok = scene(1020, 12, 12, oracle)
+    bad = [q for q in __import__("ompl_amd").workload.configs(1111, 400)
+           if not oracle.state_valid(env, q.astype(np.float32))]
+    outside = np.array([0, 0, -3.2, 0, 0, 0, 0], np.float64)
+    st, it, path, nodes = oracle.rrt_connect_ex(env, [bad[0], outside], [ok[0]], 1, MAXD, lvs_config())
+    assert (st, it, len(path), nodes) == (2, 0, 0, 0)
+    st, it, path, nodes = oracle.rrt_connect_ex(env, [ok[0]], [bad[1], bad[2], outside], 1, MAXD, lvs_config())
+    assert (st, it, len(path), nodes) == (3, 0, 0, 1)
+    # a start exactly on a bound (and within machine epsilon past it) is usable
+    edge = np.array(ok[3], np.float64)
+    edge[0] = math.pi + 2.0e-16
+    if oracle.state_valid(env, edge.astype(np.float32)):
+        st, *_ = oracle.rrt_connect_ex(env, [edge], [ok[1]], 1, MAXD, lvs_config(), max_iters=5)
+        assert st != 2
