@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 309
+#define OMPL_AMD_VERSION 310
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -376,6 +376,8 @@ double ompl_amd_rrtstar_radius(int dim, double measure, double rewire_factor, do
 #define OMPL_AMD_PLAN_CANCELLED 5
 #define OMPL_AMD_PLAN_REJECTED 6  /* footprint larger than the whole budget */
 #define OMPL_AMD_PLAN_TREE_FULL 7 /* a tree reached max_nodes */
+#define OMPL_AMD_PLAN_INVALID_START 8 /* no start state in bounds and valid */
+#define OMPL_AMD_PLAN_INVALID_GOAL 9  /* no goal state in bounds and valid */
 
 typedef struct ompl_amd_server_s *ompl_amd_server;
 
@@ -408,7 +410,21 @@ typedef struct {
     int planner;                  /* OMPL_AMD_PLANNER_RRT_CONNECT or OMPL_AMD_PLANNER_PRM */
     int prm_samples;              /* PRM: uniform samples of the roadmap (<= 0: 4,096) */
     int prm_k;                    /* PRM: neighbours per milestone (<= 0: 10, PRM's default) */
+    /* RRT-Connect: several start / goal states (host rows of 7, copied at
+     * submit, at most OMPL_AMD_MAX_ENDPOINTS each); n_starts > 0 replaces
+     * `start`, n_goals > 0 replaces `goal`.  Start states out of bounds
+     * ([-pi, pi] within machine epsilon) or invalid are skipped
+     * (PlannerInputStates::nextStart, base/src/Planner.cpp:228-256); goal
+     * states enter the goal tree in order as GoalStates hands them out
+     * (RRTConnect.cpp:251-269, Planner.cpp:265-336), checked the same way. */
+    const double *starts;
+    int n_starts;
+    const double *goals;
+    int n_goals;
+    int intermediate_states;      /* addIntermediateStates_ (RRTConnect.cpp:148-183) */
 } ompl_amd_plan_request;
+
+#define OMPL_AMD_MAX_ENDPOINTS 8
 
 /* Planner of a request.  PRM (geometric/planners/prm/src/PRM.cpp:454-536,
  * 570-604, 696-727) is served as one batch roadmap per request: start and

@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 309
+ABI_VERSION = 310
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)

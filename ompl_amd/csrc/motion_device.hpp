@@ -68,8 +68,8 @@ __host__ __device__ __forceinline__ int bisection_index(int p, int nd)
 // StateSpace::validSegmentCount (base/src/StateSpace.cpp:851-854) over
 // RealVectorStateSpace::distance (base/spaces/src/RealVectorStateSpace.cpp:230-242).
 template <int DIM>
-__device__ __forceinline__ int valid_segment_count(const double a[DIM], const double b[DIM], double lvs,
-                                                   unsigned factor)
+__host__ __device__ __forceinline__ int valid_segment_count(const double a[DIM], const double b[DIM], double lvs,
+                                                            unsigned factor)
 {
     double dist = 0.0;
 #pragma unroll

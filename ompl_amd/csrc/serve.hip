@@ -57,6 +57,8 @@ constexpr double kEqualEps = 2.220446049250313080847e-16 * 2.0; // StateSpace::e
 constexpr int kPlanPad = 288;
 // Dynamic LDS of one planner block (the staged environment).
 constexpr size_t kPlanMaxEnvBytes = 60 * 1024;
+// Start / goal states a request may list (each).
+constexpr int kMaxEnds = OMPL_AMD_MAX_ENDPOINTS;
 
 // Persistent state of one request's RRTConnect::solve loop (device).
 struct PlanState {
@@ -71,6 +73,7 @@ struct PlanState {
     int32_t phase;      // 0: sample + grow, 1: first connect step, 2: connect loop
     int32_t added, xmotion;
     int32_t path_len;
+    int32_t sampled_goals; // goal states handed out so far (PlannerInputStates)
     int64_t steps, checks;
     double rstate[kDof];
 };
@@ -81,7 +84,8 @@ __host__ __device__ inline size_t align256(size_t b)
 }
 
 // One request's region: [start tree states][goal tree states][start parents]
-// [goal parents][path states][path node indices][PlanState].  Tree states
+// [goal parents][path states][path node indices][start and goal states]
+// [PlanState].  Tree states
 // are dof-major (coordinate k of node i at k * cap + i), so a wave's reads of
 // consecutive nodes coalesce; the path is row-major.
 struct RegionPtrs {
@@ -89,6 +93,7 @@ struct RegionPtrs {
     int32_t *parent[2];
     double *path;
     int32_t *path_idx;
+    double *ends; // kMaxEnds start rows, then kMaxEnds goal rows
     PlanState *state;
     size_t bytes;
 };
@@ -109,6 +114,8 @@ __host__ __device__ inline RegionPtrs region_ptrs(char *base, int cap)
     off += align256((size_t)2 * cap * kDof * sizeof(double));
     R.path_idx = reinterpret_cast<int32_t *>(base + off);
     off += align256((size_t)2 * cap * sizeof(int32_t));
+    R.ends = reinterpret_cast<double *>(base + off);
+    off += align256((size_t)2 * kMaxEnds * kDof * sizeof(double));
     R.state = reinterpret_cast<PlanState *>(base + off);
     off += align256(sizeof(PlanState));
     R.bytes = off;
@@ -133,8 +140,8 @@ struct PlanCmd {
     uint32_t seed;
     int32_t init;      // 1: first slice (seed the sampler, plant the roots)
     int32_t ops;       // obstacle partitions of the motion check (0: by motion length)
-    double start[kDof];
-    double goal[kDof];
+    int32_t n_starts, n_goals; // rows of the region's start and goal states
+    int32_t inter;     // addIntermediateStates
 };
 
 // Per active request, written at the end of each slice.  t_*: device
@@ -307,6 +314,60 @@ __device__ __forceinline__ bool wave_check(const double *a, const double *b, con
     return true;
 }
 
+// SpaceInformation::satisfiesBounds && isValid of state q (global memory) by
+// the whole block (PlannerInputStates, base/src/Planner.cpp:243-245): within
+// machine epsilon of [-pi, pi] per joint (RealVectorStateSpace.cpp:200-208),
+// then the checker.  q is staged in sx; sa / sb are any LDS states.
+template <int NT>
+__device__ bool endpoint_usable(const double *q, double *sx, const double *sa, const double *sb, const float4 *env,
+                                int ns, int nb, int *stop)
+{
+    if (threadIdx.x < kDof) sx[threadIdx.x] = q[threadIdx.x];
+    if (threadIdx.x == 0) *stop = 0;
+    __syncthreads();
+    const double eps = 2.220446049250313080847e-16;
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < kDof; ++k)
+        if (sx[k] - eps > kPi || sx[k] + eps < -kPi) in = false;
+    if (!in) return false; // block-uniform: every thread read the same state
+    return block_check<NT, 9>(sa, sb, sx, 0, 0, 1, true, env, ns, nb, stop);
+}
+
+// Appends a valid growTree step to tree (T, P) after node nm: the new state x,
+// or with addIntermediateStates (RRTConnect.cpp:148-183) the states of
+// getMotionStates(a, b, validSegmentCount(a, b) = nd, endpoints) --
+// a + (b - a) j / (nd + 1) -- without the tree's own end: from the start tree
+// (a = nearest, b = x) states 1 .. nd + 1, from the goal tree (a = x, b =
+// nearest) states nd .. 0, each the parent of the next.  Returns the index of
+// the last (the new state), or -1 when the tree cannot hold them.
+template <int NT>
+__device__ int append_step(double *T, int32_t *P, int cap, int &n, int nm, const double *a, const double *b,
+                           const double *x, int nd, bool from_start, bool inter)
+{
+    const int tid = threadIdx.x;
+    if (!inter) {
+        if (n + 1 > cap) return -1;
+        if (tid < kDof) T[(int64_t)tid * cap + n] = x[tid];
+        if (tid == 0) P[n] = nm;
+        return n++;
+    }
+    const int seg = nd + 1;
+    if (n + seg > cap) return -1;
+    for (int i = tid; i < seg * kDof; i += NT) {
+        const int r = i / kDof, k = i - r * kDof; // chain element r (0-based)
+        const int j = from_start ? r + 1 : seg - 1 - r;
+        double v;
+        if (j == 0) v = a[k];
+        else if (j == seg) v = b[k];
+        else v = a[k] + (b[k] - a[k]) * ((double)j / (double)seg);
+        T[(int64_t)k * cap + n + r] = v;
+    }
+    for (int r = tid; r < seg; r += NT) P[n + r] = r == 0 ? nm : n + r - 1;
+    n += seg;
+    return n - 1;
+}
+
 // Outcome of a speculative sampling step's own check.
 enum SpecResult { kSpecTrapped = 0, kSpecValid = 1, kSpecUndecided = 2 };
 // Positions a speculative step checks with its own wave: the new state (or
@@ -341,22 +402,16 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
     PlanState *st = R.state;
 
     int mti, status, iteration, n0, n1, start_tree, tgi_start, grow_tree, phase, added, xmotion, path_len;
+    int sampled_goals;
     int64_t steps, checks;
     if (C.init) {
-        if (tid == 0) {
-            mt_seed_serial(s_mt, C.seed);
-            for (int k = 0; k < kDof; ++k) {
-                R.tree_s[0][(int64_t)k * C.cap] = C.start[k];
-                R.tree_s[1][(int64_t)k * C.cap] = C.goal[k];
-            }
-            R.parent[0][0] = -1;
-            R.parent[1][0] = -1;
-        }
+        if (tid == 0) mt_seed_serial(s_mt, C.seed);
         if (tid < kDof) s_rstate[tid] = 0.0;
         mti = kMtN;
         status = OMPL_AMD_PLAN_RUNNING;
         iteration = 0;
-        n0 = n1 = 1;
+        n0 = n1 = 0; // roots: the usable start states below, goal states in the loop
+        sampled_goals = 0;
         start_tree = 1;
         tgi_start = 1;
         grow_tree = 0;
@@ -379,6 +434,7 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
         added = st->added;
         xmotion = st->xmotion;
         path_len = st->path_len;
+        sampled_goals = st->sampled_goals;
         steps = st->steps;
         checks = st->checks;
     }
@@ -391,6 +447,21 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
     if (mti < kMtN) refill_draws();
     int pidx = mti / 2; // next draw (words are consumed in pairs: mti is even)
     stage_env<true, kPlanPad>(EnvView{C.obs, C.ns, C.nb}, s_env); // ends with a barrier
+    if (C.init) {
+        // PlannerInputStates::nextStart (Planner.cpp:228-256): the start
+        // states in order; those in bounds and valid become the start tree's
+        // roots
+        for (int i = 0; i < C.n_starts; ++i) {
+            if (endpoint_usable<NT>(R.ends + (size_t)i * kDof, s_x, s_a, s_b, s_env, C.ns, C.nb, &s_hit) &&
+                n0 < C.cap) {
+                if (tid < kDof) R.tree_s[0][(int64_t)tid * C.cap + n0] = s_x[tid];
+                if (tid == 0) R.parent[0][n0] = -1;
+                ++n0;
+            }
+            __syncthreads(); // s_x is rewritten by the next state
+        }
+        if (n0 == 0) status = OMPL_AMD_PLAN_INVALID_START;
+    }
 
     int done_here = 0;
     int64_t t_sample = 0, t_nearest = 0, t_check = 0, t_step = 0, c0 = 0, c1 = 0; // thread 0's timers
@@ -414,7 +485,11 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
         // are dropped (their draws are not consumed).  Same steps, same results
         // as one at a time.  m stays within the generator's current block of
         // draws and the iterations left.
-        const int m_spec = phase == 0 ? min(NT / 64, min((kMtN / 2 - pidx) / kDof, C.max_iters - iteration)) : 0;
+        // (not while a goal state is due: the goal tree would change between
+        // the steps)
+        const bool goal_due = n1 == 0 || (sampled_goals < C.n_goals && sampled_goals < n1 / 2);
+        const int m_spec =
+            phase == 0 && !goal_due ? min(NT / 64, min((kMtN / 2 - pidx) / kDof, C.max_iters - iteration)) : 0;
         if (m_spec >= 2) {
             const int m = m_spec;
             if (tid < kDof * m) s_samp[tid] = (kPi - -kPi) * s_u[pidx + tid] + -kPi;
@@ -539,17 +614,17 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
                 }
                 // commit step j: append its new state, then connect (phase 1)
                 const int tree_j = st_j ? 0 : 1;
-                const int ntj = tree_j ? n1 : n0;
-                if (tid < kDof) {
-                    R.tree_s[tree_j][(int64_t)tid * C.cap + ntj] = s_gx[kDof * j + tid];
-                    // rstate: the sample when reached, else the new state
-                    s_rstate[tid] = s_reach[j] ? s_samp[kDof * j + tid] : s_gx[kDof * j + tid];
+                const int last = append_step<NT>(R.tree_s[tree_j], R.parent[tree_j], C.cap, tree_j ? n1 : n0, s_nm[j],
+                                                 s_ga + kDof * j, s_gb + kDof * j, s_gx + kDof * j, s_nd[j], st_j != 0,
+                                                 C.inter != 0);
+                if (last < 0) {
+                    status = OMPL_AMD_PLAN_TREE_FULL;
+                    break;
                 }
-                if (tid == 0) R.parent[tree_j][ntj] = s_nm[j];
-                if (tree_j) ++n1;
-                else ++n0;
-                xmotion = ntj;
-                added = ntj;
+                // rstate: the sample when reached, else the new state
+                if (tid < kDof) s_rstate[tid] = s_reach[j] ? s_samp[kDof * j + tid] : s_gx[kDof * j + tid];
+                xmotion = last;
+                added = last;
                 tgi_start = start_tree;
                 grow_tree = start_tree ? 0 : 1; // otherTree
                 phase = 1;
@@ -567,6 +642,32 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
             grow_tree = start_tree ? 0 : 1;
             tgi_start = start_tree;
             start_tree = !start_tree;
+            // RRTConnect.cpp:251-269: goal states (GoalStates, in order) --
+            // every remaining one tried while the goal tree is empty
+            // (nextGoal(ptc)), else one while fewer than half the goal tree's
+            // size have been handed out (nextGoal())
+            if (n1 == 0 || sampled_goals < n1 / 2) {
+                const bool empty = n1 == 0;
+                while (sampled_goals < C.n_goals) {
+                    const int g = sampled_goals++;
+                    if (endpoint_usable<NT>(R.ends + (size_t)(kMaxEnds + g) * kDof, s_x, s_a, s_b, s_env, C.ns, C.nb,
+                                            &s_hit)) {
+                        if (n1 >= C.cap) {
+                            status = OMPL_AMD_PLAN_TREE_FULL;
+                        } else {
+                            if (tid < kDof) R.tree_s[1][(int64_t)tid * C.cap + n1] = s_x[tid];
+                            if (tid == 0) R.parent[1][n1] = -1;
+                            ++n1;
+                        }
+                        __syncthreads();
+                        break;
+                    }
+                    __syncthreads();
+                    if (!empty) break;
+                }
+                if (status == OMPL_AMD_PLAN_RUNNING && n1 == 0) status = OMPL_AMD_PLAN_INVALID_GOAL;
+                if (status != OMPL_AMD_PLAN_RUNNING) break;
+            }
             // sampler_->sampleUniform(rstate): RNG::uniformReal(-pi, pi) per joint,
             // from the next seven draws (twisting when the block runs out)
             for (int d0 = 0; d0 < kDof;) {
@@ -670,11 +771,13 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
                 }
             }
             if (valid) {
-                if (tid < kDof) T[(int64_t)tid * C.cap + nt] = s_x[tid];
-                if (tid == 0) R.parent[grow_tree][nt] = nm;
-                if (grow_tree) ++n1;
-                else ++n0;
-                xmotion = nt;
+                const int last = append_step<NT>(T, R.parent[grow_tree], C.cap, grow_tree ? n1 : n0, nm, s_a, s_b, s_x,
+                                                 nd, from_start, C.inter != 0);
+                if (last < 0) {
+                    status = OMPL_AMD_PLAN_TREE_FULL;
+                    break;
+                }
+                xmotion = last;
                 gs = reach ? kReached : kAdvanced;
             }
         }
@@ -751,6 +854,7 @@ __global__ void __launch_bounds__(NT) plan_slice_kernel(SliceArgs A)
         st->added = added;
         st->xmotion = xmotion;
         st->path_len = path_len;
+        st->sampled_goals = sampled_goals;
         st->steps = steps;
         st->checks = checks;
         PlanOut &o = A.out[blockIdx.x];
@@ -780,7 +884,9 @@ static double ms_since(Clock::time_point t0)
 }
 
 struct Planner {
-    ompl_amd_plan_request req;
+    ompl_amd_plan_request req; // starts / goals point into `ends`
+    // kMaxEnds start rows, then kMaxEnds goal rows (the region's layout)
+    std::vector<double> ends = std::vector<double>((size_t)2 * kMaxEnds * kDof, 0.0);
     double maxd = 0.0;
     Clock::time_point t_submit, t_start, t_finish;
     // published: written and read under Server::mu only
@@ -797,6 +903,24 @@ struct Planner {
     bool init = true;
     int64_t steps_seen = 0;
 };
+
+// Copies a request's start and goal states into the planner (the caller's
+// lists need not outlive the submit): the lists, or start / goal when a list
+// is empty.
+static void take_endpoints(Planner &p)
+{
+    ompl_amd_plan_request &r = p.req;
+    if (r.n_starts > 0) std::copy(r.starts, r.starts + (size_t)r.n_starts * kDof, p.ends.begin());
+    else std::copy(r.start, r.start + kDof, p.ends.begin());
+    if (r.n_goals > 0) std::copy(r.goals, r.goals + (size_t)r.n_goals * kDof, p.ends.begin() + (size_t)kMaxEnds * kDof);
+    else std::copy(r.goal, r.goal + kDof, p.ends.begin() + (size_t)kMaxEnds * kDof);
+    r.n_starts = r.n_starts > 0 ? r.n_starts : 1;
+    r.n_goals = r.n_goals > 0 ? r.n_goals : 1;
+    std::copy(p.ends.begin(), p.ends.begin() + kDof, r.start);
+    std::copy(p.ends.begin() + (size_t)kMaxEnds * kDof, p.ends.begin() + (size_t)(kMaxEnds + 1) * kDof, r.goal);
+    r.starts = p.ends.data();
+    r.goals = p.ends.data() + (size_t)kMaxEnds * kDof;
+}
 
 // First-fit allocator over the device pool (offsets; neighbours coalesce).
 struct PoolAlloc {
@@ -1040,9 +1164,14 @@ void Server::run()
                 c.seed = p.req.seed;
                 c.init = p.init ? 1 : 0;
                 c.ops = cfg.check_obstacle_parts;
-                for (int k = 0; k < kDof; ++k) {
-                    c.start[k] = p.req.start[k];
-                    c.goal[k] = p.req.goal[k];
+                c.n_starts = p.req.n_starts;
+                c.n_goals = p.req.n_goals;
+                c.inter = p.req.intermediate_states ? 1 : 0;
+                if (p.init) {
+                    // the request's start and goal states into its region
+                    const RegionPtrs Rp = region_ptrs(pool + p.off, p.req.max_nodes);
+                    OA_HIP(hipMemcpyAsync(Rp.ends, p.ends.data(), p.ends.size() * sizeof(double),
+                                          hipMemcpyHostToDevice, stream));
                 }
                 lds = std::max(lds, env_stage_bytes(env.ns, env.nb));
                 p.init = false;
@@ -1146,7 +1275,8 @@ struct HostTree {
 };
 
 static int plain_grow(const Env &env, HostTree &t, bool start_tree, const double *target, double maxd, double lvs,
-                      unsigned factor, int &xmotion, double *xstate, int64_t &checks, ThreadResources &tr)
+                      unsigned factor, bool inter, int &xmotion, double *xstate, int64_t &checks,
+                      ThreadResources &tr)
 {
     int nm = 0;
     double dmin = 0.0;
@@ -1162,7 +1292,8 @@ static int plain_grow(const Env &env, HostTree &t, bool start_tree, const double
             dmin = d;
         }
     }
-    const double *ns = &t.s[(size_t)nm * kDof];
+    double ns[kDof]; // the tree may grow below
+    std::copy(t.s.begin() + (size_t)nm * kDof, t.s.begin() + (size_t)(nm + 1) * kDof, ns);
     bool reach = true;
     double x[kDof];
     const double *dst = target;
@@ -1198,11 +1329,39 @@ static int plain_grow(const Env &env, HostTree &t, bool start_tree, const double
         valid = tr.pinned[0] != 0 && check(dst, ns);
     }
     if (!valid) return kTrapped;
-    t.s.insert(t.s.end(), dst, dst + kDof);
-    t.parent.push_back(nm);
+    if (inter) {
+        // getMotionStates(a, b, validSegmentCount(a, b), endpoints): states
+        // 1 .. nd + 1 from the start tree, nd .. 0 from the goal tree
+        const double *a = start_tree ? ns : dst, *b = start_tree ? dst : ns;
+        const int seg = (int)valid_segment_count<kDof>(a, b, lvs, factor) + 1;
+        int parent = nm;
+        for (int r = 0; r < seg; ++r) {
+            const int j = start_tree ? r + 1 : seg - 1 - r;
+            for (int k = 0; k < kDof; ++k)
+                t.s.push_back(j == 0 ? a[k] : (j == seg ? b[k] : a[k] + (b[k] - a[k]) * ((double)j / (double)seg)));
+            t.parent.push_back(parent);
+            parent = t.size() - 1;
+        }
+    } else {
+        t.s.insert(t.s.end(), dst, dst + kDof);
+        t.parent.push_back(nm);
+    }
     xmotion = t.size() - 1;
     for (int k = 0; k < kDof; ++k) xstate[k] = dst[k];
     return reach ? kReached : kAdvanced;
+}
+
+// satisfiesBounds && isValid of one state (PlannerInputStates): one launch.
+static bool plain_usable(const Env &env, const double *q, ThreadResources &tr)
+{
+    const double eps = 2.220446049250313080847e-16;
+    for (int k = 0; k < kDof; ++k)
+        if (q[k] - eps > kPi || q[k] + eps < -kPi) return false;
+    State1 st;
+    for (int k = 0; k < kDof; ++k) st.q[k] = q[k];
+    launch_validate_single(env.view(), st, tr.pinned_dev, tr.stream);
+    OA_HIP(hipStreamSynchronize(tr.stream));
+    return tr.pinned[0] != 0;
 }
 
 // Plain PRM: the served PRM request's batch roadmap built on the host with one
@@ -1406,9 +1565,16 @@ int ompl_amd_server_submit(ompl_amd_server h, const ompl_amd_plan_request *req, 
         OA_REQUIRE(prm || env_stage_bytes(env.ns, env.nb) <= kPlanMaxEnvBytes, OMPL_AMD_ERR_ARG,
                    "environment too large for the server: %d spheres and %d boxes stage %zu bytes (limit %zu)",
                    env.ns, env.nb, env_stage_bytes(env.ns, env.nb), kPlanMaxEnvBytes);
+        OA_REQUIRE(req->n_starts >= 0 && req->n_starts <= kMaxEnds && req->n_goals >= 0 && req->n_goals <= kMaxEnds,
+                   OMPL_AMD_ERR_ARG, "a request lists at most %d start and %d goal states", kMaxEnds, kMaxEnds);
+        OA_REQUIRE((req->n_starts == 0 || req->starts) && (req->n_goals == 0 || req->goals), OMPL_AMD_ERR_ARG,
+                   "null start or goal list");
+        OA_REQUIRE(!prm || (req->n_starts <= 1 && req->n_goals <= 1 && !req->intermediate_states), OMPL_AMD_ERR_ARG,
+                   "PRM requests take one start and one goal state");
         Server &S = h->srv;
         auto *p = new Planner();
         p->req = *req;
+        take_endpoints(*p);
         if (prm) {
             if (p->req.prm_samples <= 0) p->req.prm_samples = 4096;
             if (p->req.prm_k <= 0) p->req.prm_k = 10;
@@ -1417,6 +1583,11 @@ int ompl_amd_server_submit(ompl_amd_server h, const ompl_amd_plan_request *req, 
         }
         if (p->req.max_nodes <= 0 || p->req.max_nodes > S.cfg.max_nodes) p->req.max_nodes = S.cfg.max_nodes;
         if (p->req.max_nodes < 2) p->req.max_nodes = 2;
+        if (p->req.max_nodes <= p->req.n_starts) {
+            const int mn = p->req.max_nodes, ns = p->req.n_starts;
+            delete p;
+            fail(OMPL_AMD_ERR_ARG, "max_nodes %d cannot hold the %d start states and a step", mn, ns);
+        }
         p->maxd = req->range > 0.0 ? req->range : 0.2 * 2.0 * kPi * std::sqrt(7.0);
         p->nodes = 2;
         p->t_submit = Clock::now();
@@ -1585,37 +1756,62 @@ int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *
         std::mt19937 gen(req->seed);
         std::uniform_real_distribution<> uni{0, 1};
         const double maxd = req->range > 0.0 ? req->range : 0.2 * 2.0 * kPi * std::sqrt(7.0);
+        OA_REQUIRE(req->n_starts >= 0 && req->n_starts <= kMaxEnds && req->n_goals >= 0 && req->n_goals <= kMaxEnds &&
+                       (req->n_starts == 0 || req->starts) && (req->n_goals == 0 || req->goals),
+                   OMPL_AMD_ERR_ARG, "bad start or goal list");
+        const double *starts = req->n_starts > 0 ? req->starts : req->start;
+        const double *goals = req->n_goals > 0 ? req->goals : req->goal;
+        const int n_starts = req->n_starts > 0 ? req->n_starts : 1, n_goals = req->n_goals > 0 ? req->n_goals : 1;
+        const bool inter = req->intermediate_states != 0;
         HostTree ts, tg;
-        ts.s.assign(req->start, req->start + kDof);
-        ts.parent.push_back(-1);
-        tg.s.assign(req->goal, req->goal + kDof);
-        tg.parent.push_back(-1);
+        for (int i = 0; i < n_starts; ++i) // nextStart: usable start states become roots
+            if (plain_usable(env, starts + (size_t)i * kDof, tr)) {
+                ts.s.insert(ts.s.end(), starts + (size_t)i * kDof, starts + (size_t)(i + 1) * kDof);
+                ts.parent.push_back(-1);
+            }
         bool start_tree = true, solved = false;
-        int it = 0;
+        int it = 0, sampled_goals = 0;
+        int status = ts.size() == 0 ? OMPL_AMD_PLAN_INVALID_START : OMPL_AMD_PLAN_RUNNING;
         int64_t checks = 0;
         double rstate[kDof], xstate[kDof];
         std::vector<double> out;
-        for (it = 0; it < req->max_iters && !solved; ++it) {
+        for (it = 0; it < req->max_iters && !solved && status == OMPL_AMD_PLAN_RUNNING; ++it) {
             if (req->deadline_ms > 0.0 && ms_since(t0) > req->deadline_ms) break;
             HostTree &tree = start_tree ? ts : tg;
             bool tgi_start = start_tree;
             start_tree = !start_tree;
             HostTree &other = start_tree ? ts : tg;
+            if (tg.size() == 0 || sampled_goals < tg.size() / 2) { // goal states, as the served planner
+                const bool empty = tg.size() == 0;
+                while (sampled_goals < n_goals) {
+                    const double *q = goals + (size_t)(sampled_goals++) * kDof;
+                    if (plain_usable(env, q, tr)) {
+                        tg.s.insert(tg.s.end(), q, q + kDof);
+                        tg.parent.push_back(-1);
+                        break;
+                    }
+                    if (!empty) break;
+                }
+                if (tg.size() == 0) {
+                    status = OMPL_AMD_PLAN_INVALID_GOAL;
+                    break;
+                }
+            }
             for (int k = 0; k < kDof; ++k) rstate[k] = (kPi - -kPi) * uni(gen) + -kPi;
             int xm = -1;
-            int gs = plain_grow(env, tree, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, xm,
+            int gs = plain_grow(env, tree, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, inter, xm,
                                 xstate, checks, tr);
             if (gs == kTrapped) continue;
             const int added = xm;
             if (gs != kReached)
                 for (int k = 0; k < kDof; ++k) rstate[k] = xstate[k];
             tgi_start = start_tree;
-            int gsc = plain_grow(env, other, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, xm,
-                                 xstate, checks, tr);
+            int gsc = plain_grow(env, other, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, inter,
+                                 xm, xstate, checks, tr);
             if (gsc == kTrapped) tgi_start = !tgi_start;
             while (gsc == kAdvanced)
-                gsc = plain_grow(env, other, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, xm,
-                                 xstate, checks, tr);
+                gsc = plain_grow(env, other, tgi_start, rstate, maxd, req->longest_valid_segment, req->factor, inter,
+                                 xm, xstate, checks, tr);
             if (gsc != kReached) continue;
             int sm = tgi_start ? xm : added, gm = tgi_start ? added : xm;
             if (ts.parent[sm] >= 0) sm = ts.parent[sm];
@@ -1630,7 +1826,9 @@ int ompl_amd_plan_plain(const ompl_amd_plan_request *req, ompl_amd_plan_result *
         }
         const double ms = ms_since(t0);
         res->status = solved ? OMPL_AMD_PLAN_SOLVED
-                             : ((req->deadline_ms > 0.0 && ms > req->deadline_ms) ? OMPL_AMD_PLAN_DEADLINE
+                      : status != OMPL_AMD_PLAN_RUNNING
+                          ? status
+                          : ((req->deadline_ms > 0.0 && ms > req->deadline_ms) ? OMPL_AMD_PLAN_DEADLINE
                                                                                   : OMPL_AMD_PLAN_FAILED);
         res->iterations = it;
         res->path_len = (int)(out.size() / kDof);

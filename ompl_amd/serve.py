@@ -12,7 +12,7 @@ from ._lib import call
 DOF = 7
 QUEUED, RUNNING, SOLVED, FAILED, DEADLINE, CANCELLED, REJECTED, TREE_FULL = range(8)
 STATUS = {0: "queued", 1: "running", 2: "solved", 3: "failed", 4: "deadline", 5: "cancelled", 6: "rejected",
-          7: "tree_full"}
+          7: "tree_full", 8: "invalid_start", 9: "invalid_goal"}
 
 
 class ServerConfig(C.Structure):
@@ -36,6 +36,11 @@ class PlanRequest(C.Structure):
         ("planner", C.c_int),
         ("prm_samples", C.c_int),
         ("prm_k", C.c_int),
+        ("starts", C.c_void_p),
+        ("n_starts", C.c_int),
+        ("goals", C.c_void_p),
+        ("n_goals", C.c_int),
+        ("intermediate_states", C.c_int),
     ]
 
 
@@ -61,8 +66,23 @@ RRT_CONNECT, PRM = 0, 1
 
 
 def make_request(env, start, goal, seed, lvs, max_iters=5000, rng=0.0, factor=1, deadline_ms=0.0, max_nodes=0,
-                 planner=RRT_CONNECT, prm_samples=0, prm_k=0):
+                 planner=RRT_CONNECT, prm_samples=0, prm_k=0, starts=None, goals=None, intermediate_states=False):
+    """A plan request.  starts / goals ([n][7]) replace start / goal when
+    given (the arrays are kept on the request until it is submitted)."""
     r = PlanRequest()
+    r.intermediate_states = int(bool(intermediate_states))
+    if starts is not None:
+        a = np.ascontiguousarray(np.reshape(starts, (-1, DOF)), np.float64)
+        r._starts = a
+        r.starts = a.ctypes.data
+        r.n_starts = a.shape[0]
+        start = a[0] if start is None else start
+    if goals is not None:
+        a = np.ascontiguousarray(np.reshape(goals, (-1, DOF)), np.float64)
+        r._goals = a
+        r.goals = a.ctypes.data
+        r.n_goals = a.shape[0]
+        goal = a[0] if goal is None else goal
     r.planner = int(planner)
     r.prm_samples = int(prm_samples)
     r.prm_k = int(prm_k)

@@ -142,8 +142,9 @@ def test_failure_paths_forced_from_host(gpu, scene):
     # tree capacity: 3 nodes per tree cannot hold a plan
     rid = srv.submit(make_request(env, a, b, 2, lvs, max_iters=10 ** 6, max_nodes=3))
     assert srv.wait(rid, 5000).status == TREE_FULL
-    # cancellation of a long request
-    rid = srv.submit(make_request(env, a, np.array(a) + 1e-3, 3, lvs, max_iters=10 ** 7, rng=1e-3))
+    # cancellation of a long request (a far goal in steps of 1e-4: thousands of
+    # slices, so the cancellation is applied between two of them)
+    rid = srv.submit(make_request(env, a, b, 3, lvs, max_iters=10 ** 7, rng=1e-4, max_nodes=1024))
     srv.cancel(rid)
     assert srv.wait(rid, 5000).status == CANCELLED
     assert srv.stats()["used_bytes"] == 0
@@ -386,4 +387,55 @@ def test_rrt_connect_and_prm_requests_alone_and_in_mix(gpu, scene):
         assert alone[i][:4] == m[:4] and np.array_equal(alone[i][4], m[4]), i
     st = srv.stats()
     assert st["active"] == 0 and st["queued"] == 0 and st["used_bytes"] == 0
+    srv.close()
+
+
+@pytest.mark.parametrize("intermediate", [False, True])
+def test_start_goal_lists_and_intermediate_states(gpu, oracle, intermediate):
+    """Requests listing several start and goal states (some out of bounds or in
+    collision) with and without addIntermediateStates: served (alone and in
+    the mix) and plain plans equal the oracle's status, iterations, tree sizes
+    and paths; the goal schedule puts later goal states into the goal tree."""
+    from ompl_amd.serve import Server, make_request, plan_plain
+
+    s, b = gpu.workload.obstacles(1003, 24, 24)
+    env = gpu.Env(s, b)
+    oenv = oracle.OracleEnv(s, b)
+    q = gpu.workload.configs(1003 + 77, 3000)
+    ok = [row for row in q if oracle.state_valid(oenv, row.astype(np.float32))]
+    bad = [row for row in gpu.workload.configs(1003 + 91, 400) if not oracle.state_valid(oenv, row.astype(np.float32))]
+    outside = np.array([3.2, 0, 0, 0, 0, 0, 0], np.float64)
+    cases = []
+    for i in range(6):
+        starts = [bad[i], outside, ok[10 * i], ok[10 * i + 1]]
+        goals = [bad[i + 1], ok[10 * i + 2], bad[i + 2], ok[10 * i + 3], ok[10 * i + 4]]
+        cases.append((starts, goals))
+    cases.append(([bad[0], outside], [ok[0]]))           # no usable start
+    cases.append(([ok[1]], [bad[1], bad[2], outside]))   # no usable goal
+    reqs = [make_request(env, None, None, seed=60 + i, lvs=lvs_config(), max_iters=300, rng=1.0, starts=st,
+                         goals=gl, intermediate_states=intermediate) for i, (st, gl) in enumerate(cases)]
+    srv = Server(hbm_budget_bytes=1 << 30, max_active=16, max_nodes=1 << 15)
+    alone = []
+    for r in reqs:
+        rid = srv.submit(r)
+        res = srv.wait(rid, 60000)
+        alone.append((res.status, res.iterations, res.tree_nodes, srv.path(rid)))
+        srv.release(rid)
+    ids = [srv.submit(r) for r in reqs]
+    mix = [(lambda res, rid: (res.status, res.iterations, res.tree_nodes, srv.path(rid)))(srv.wait(rid, 60000), rid)
+           for rid in ids]
+    names = {0: 3, 1: 2, 2: 8, 3: 9}  # oracle status -> server status
+    solved = 0
+    for i, (st, gl) in enumerate(cases):
+        ost, oit, opath, onodes = oracle.rrt_connect_ex(oenv, st, gl, 60 + i, 1.0, lvs_config(), max_iters=300,
+                                                       intermediate=intermediate)
+        print(f"case {i}: served {alone[i][:3]} oracle {(ost, oit, onodes)}", flush=True)
+        for got in (alone[i], mix[i]):
+            assert got[0] == names[ost] and got[1] == oit and got[2] == onodes, (i, got[:3], ost, oit, onodes)
+            assert np.array_equal(got[3], opath), i
+        pres, ppath = plan_plain(reqs[i], cap=1 << 15)
+        assert (pres.status, pres.iterations, pres.tree_nodes) == (names[ost], oit, onodes), i
+        assert np.array_equal(ppath, opath), i
+        solved += ost == 1
+    assert solved >= 3
     srv.close()
