@@ -3,11 +3,13 @@
 //
 // A request is a single-query RRT-Connect plan (geometric/planners/rrt/src/
 // RRTConnect.cpp:116-196 growTree, :245-330 solve) on the 7-DOF arm in one of
-// the registered environments.  Everything the loop touches lives in one
-// region of a device pool sized to the HBM budget: both trees (append-only
-// FP64 states + parent indices), the state sampler's mt19937 and the solve
-// loop's variables, and the solution path.  The server thread launches
-// slices: one block (1,024 or 512 threads) per active request runs its loop
+// the registered environments, from one or more start states to one or more
+// goal states (PlannerInputStates order, base/src/Planner.cpp:228-336).
+// Everything the loop touches lives in one region of a device pool sized to
+// the HBM budget: both trees (append-only FP64 states + parent indices), the
+// start and goal states, the state sampler's mt19937 and the solve loop's
+// variables, and the solution path.  The server thread launches slices: one
+// block (512 threads, or 1,024) per active request runs its loop
 // -- sample, nearest neighbour, steer, motion check, append, connect loop,
 // path splice -- for a bounded time, then stores its state; the host then
 // retires finished requests and admits queued ones while their region fits
