@@ -106,6 +106,27 @@ __device__ __forceinline__ float exact_dist(const float *__restrict__ e, const f
     }
 }
 
+// SE3: true when the point's distance is certainly above `bound` (a finite
+// distance), from a lower bound that skips the acos: acos(x) >= sqrt(2 (1 - x))
+// on [0, 1] (2 sin(t / 2) <= t), so d >= d_t + sqrt(2 (1 - |q.x|)).  The margin
+// (relative 1e-5 plus 1e-5) covers the FP32 rounding of both sides and
+// acos_as's error, so a point this skips is one whose exact distance is above
+// the bound.  Other metrics: never.
+template <int METRIC, int DIM>
+__device__ __forceinline__ bool certainly_above(const float *__restrict__ e, const float q[DIM], float bound)
+{
+    if (METRIC != OMPL_AMD_METRIC_SE3) return false;
+    float t = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float diff = e[i] - q[i];
+        t += diff * diff;
+    }
+    const float dq = fabsf(((e[3] * q[3] + e[4] * q[4]) + e[5] * q[5]) + e[6] * q[6]);
+    const float lb = __builtin_amdgcn_sqrtf(t) + __builtin_amdgcn_sqrtf(2.0f * (1.0f - dq));
+    return lb > bound * (1.0f + 1e-5f) + 1e-5f; // false for NaN (dq > 1) or an infinite bound
+}
+
 struct Key {
     float d;
     int i;
@@ -477,6 +498,10 @@ knn_kernel(KnnArgs A)
             const int j = __builtin_ctz(mask);
             mask &= mask - 1;
             const int row = mfma_row(j, h);
+            // SE3: most candidates of the translation filter are far in
+            // rotation; skip their acos when they are certainly above bkey
+            if (certainly_above<METRIC, DIM>(tp + row * DIM, Q.q, __uint_as_float((uint32_t)(bkey >> 32))))
+                continue;
             const float d = exact_dist<METRIC, DIM>(tp + row * DIM, Q.q);
             const unsigned long long nk = pack_key(d, (int)(t * 32 + row));
             if (KMAX <= 16) {
@@ -625,6 +650,7 @@ __global__ void __launch_bounds__(256) radius_kernel(KnnArgs A, uint64_t *__rest
             mask &= mask - 1;
             const int64_t p = t * 32 + mfma_row(j, h);
             if (p >= A.n || A.removed[p]) continue;
+            if (certainly_above<METRIC, DIM>(A.pts + p * DIM, Q.q, r)) continue;
             const float d = exact_dist<METRIC, DIM>(A.pts + p * DIM, Q.q);
             if (d <= r) {
                 const uint64_t key = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)p;
