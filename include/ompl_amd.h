@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 310
+#define OMPL_AMD_VERSION 311
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -347,6 +347,10 @@ int ompl_amd_nn_radius_packed_dev(ompl_amd_nn nn, const float *q_dev, int64_t nq
                                   int64_t capacity, int64_t *total, void *stream);
 /* Host convenience (blocking). */
 int ompl_amd_nn_knn(ompl_amd_nn nn, const float *q, int64_t nq, int k, int32_t *out_idx, float *out_dist);
+int ompl_amd_nn_knn_packed(ompl_amd_nn nn, const float *q, int64_t nq, const int32_t *k, int kmax,
+                           int64_t *out_offsets, int32_t *out_idx, float *out_dist);
+int ompl_amd_nn_radius_packed(ompl_amd_nn nn, const float *q, int64_t nq, const float *r, int64_t *out_offsets,
+                              int32_t *out_idx, float *out_dist, int64_t capacity, int64_t *total);
 int ompl_amd_nn_radius(ompl_amd_nn nn, const float *q, int64_t nq, float radius, int64_t *out_offsets,
                        int32_t *out_idx, float *out_dist, int64_t capacity, int64_t *total);
 int ompl_amd_nn_nearest(ompl_amd_nn nn, const float *q, int64_t *out_id);

@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 310
+ABI_VERSION = 311
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -80,6 +80,8 @@ SIGNATURES = {
     "ompl_amd_nn_knn_dev": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp, vp]),
     "ompl_amd_nn_radius_dev": (C.c_int, [vp, vp, C.c_int64, C.c_float, vp, vp, vp, C.c_int64, c_i64_p, vp]),
     "ompl_amd_nn_knn_packed_dev": (C.c_int, [vp, vp, C.c_int64, vp, C.c_int, vp, vp, vp, vp]),
+    "ompl_amd_nn_knn_packed": (C.c_int, [vp, vp, C.c_int64, vp, C.c_int, vp, vp, vp]),
+    "ompl_amd_nn_radius_packed": (C.c_int, [vp, vp, C.c_int64, vp, vp, vp, vp, C.c_int64, c_i64_p]),
     "ompl_amd_nn_radius_packed_dev": (C.c_int, [vp, vp, C.c_int64, vp, vp, vp, vp, C.c_int64, c_i64_p, vp]),
     "ompl_amd_nn_knn": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp]),
     "ompl_amd_nn_radius": (C.c_int, [vp, vp, C.c_int64, C.c_float, vp, vp, vp, C.c_int64, c_i64_p]),

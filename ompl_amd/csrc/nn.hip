@@ -1131,6 +1131,25 @@ static NNIndex &nn_of(ompl_amd_nn h)
     return h->idx;
 }
 
+// Device buffers of one host-convenience call, freed on every exit.
+struct DevBufs {
+    hipStream_t s;
+    std::vector<void *> p;
+    explicit DevBufs(hipStream_t st) : s(st) {}
+    template <class T> T *alloc(size_t n)
+    {
+        void *q = nullptr;
+        OA_HIP(hipMallocAsync(&q, (n ? n : 1) * sizeof(T), s));
+        p.push_back(q);
+        return static_cast<T *>(q);
+    }
+    ~DevBufs()
+    {
+        for (void *q : p) (void)hipFreeAsync(q, s);
+        (void)hipStreamSynchronize(s);
+    }
+};
+
 } // namespace ompl_amd
 
 using namespace ompl_amd;
@@ -1325,6 +1344,69 @@ int ompl_amd_nn_radius(ompl_amd_nn h, const float *q, int64_t nq, float r, int64
         OA_HIP(hipFreeAsync(doff, s));
         if (di) OA_HIP(hipFreeAsync(di, s));
         if (dd) OA_HIP(hipFreeAsync(dd, s));
+        OA_HIP(hipStreamSynchronize(s));
+        *total = tot;
+    });
+}
+
+
+int ompl_amd_nn_knn_packed(ompl_amd_nn h, const float *q, int64_t nq, const int32_t *k, int kmax, int64_t *offsets,
+                           int32_t *out_i, float *out_d)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(nq >= 0 && offsets && (nq == 0 || (q && k && out_i && out_d)), OMPL_AMD_ERR_ARG, "bad arguments");
+        OA_REQUIRE(kmax >= 1 && kmax <= 64, OMPL_AMD_ERR_ARG, "kmax must be in [1, 64] (got %d)", kmax);
+        if (nq == 0) {
+            offsets[0] = 0;
+            return;
+        }
+        hipStream_t s = default_stream();
+        DevBufs B(s);
+        float *dq = B.alloc<float>((size_t)nq * x.dim);
+        int32_t *dk = B.alloc<int32_t>((size_t)nq);
+        int64_t *doff = B.alloc<int64_t>((size_t)nq + 1);
+        int32_t *di = B.alloc<int32_t>((size_t)nq * kmax);
+        float *dd = B.alloc<float>((size_t)nq * kmax);
+        OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
+        OA_HIP(hipMemcpyAsync(dk, k, (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        knn(x, dq, nq, kmax, dk, doff, di, dd, s);
+        OA_HIP(hipMemcpyAsync(offsets, doff, (size_t)(nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        OA_HIP(hipStreamSynchronize(s));
+        const int64_t tot = offsets[nq];
+        if (tot > 0) {
+            OA_HIP(hipMemcpyAsync(out_i, di, (size_t)tot * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            OA_HIP(hipMemcpyAsync(out_d, dd, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+    });
+}
+
+int ompl_amd_nn_radius_packed(ompl_amd_nn h, const float *q, int64_t nq, const float *r, int64_t *offsets,
+                              int32_t *out_i, float *out_d, int64_t capacity, int64_t *total)
+{
+    return guarded([&] {
+        NNIndex &x = nn_of(h);
+        OA_REQUIRE(nq >= 0 && offsets && total && (nq == 0 || (q && r)), OMPL_AMD_ERR_ARG, "bad arguments");
+        hipStream_t s = default_stream();
+        DevBufs B(s);
+        const bool want = out_i || out_d;
+        float *dq = B.alloc<float>((size_t)nq * x.dim);
+        float *dr = B.alloc<float>((size_t)nq);
+        int64_t *doff = B.alloc<int64_t>((size_t)nq + 1);
+        int32_t *di = want && capacity > 0 ? B.alloc<int32_t>((size_t)capacity) : nullptr;
+        float *dd = want && capacity > 0 ? B.alloc<float>((size_t)capacity) : nullptr;
+        if (nq) {
+            OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
+            OA_HIP(hipMemcpyAsync(dr, r, (size_t)nq * sizeof(float), hipMemcpyHostToDevice, s));
+        }
+        int64_t tot = 0;
+        *total = 0;
+        radius(x, dq, nq, 0.0f, dr, doff, want ? di : nullptr, want ? dd : nullptr, capacity, &tot, s);
+        OA_HIP(hipMemcpyAsync(offsets, doff, (size_t)(nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        if (want && tot > 0) {
+            if (out_i) OA_HIP(hipMemcpyAsync(out_i, di, (size_t)tot * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            if (out_d) OA_HIP(hipMemcpyAsync(out_d, dd, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
         OA_HIP(hipStreamSynchronize(s));
         *total = tot;
     });

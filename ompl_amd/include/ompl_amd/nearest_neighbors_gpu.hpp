@@ -177,6 +177,62 @@ public:
             for (int64_t j = off[i]; j < off[i + 1]; ++j) out[i].push_back(data_[(std::size_t)idx[(std::size_t)j]]);
     }
 
+    // ---- packed forms: the queries of many callers, each with its own k or
+    // radius (NearestNeighbors::nearestK / nearestR for each), in one launch
+    void nearestKPacked(const std::vector<_T> &queries, const std::vector<std::size_t> &k,
+                        std::vector<std::vector<_T>> &out) const
+    {
+        if (k.size() != queries.size())
+            throw ompl::Exception("NearestNeighborsGPU::nearestKPacked: one k per query");
+        out.assign(queries.size(), {});
+        if (queries.empty()) return;
+        const std::size_t n = size();
+        std::vector<int32_t> kq(queries.size());
+        std::size_t kmax = 1;
+        for (std::size_t i = 0; i < queries.size(); ++i) {
+            const std::size_t kk = k[i] < n ? k[i] : n; // min(k, size()) neighbours
+            if (kk > kMaxK)
+                throw ompl::Exception("NearestNeighborsGPU::nearestKPacked: k = " + std::to_string(k[i]) +
+                                      " exceeds the limit of " + std::to_string(kMaxK) + " neighbours per query");
+            kq[i] = (int32_t)kk;
+            kmax = kk > kmax ? kk : kmax;
+        }
+        std::vector<float> q(queries.size() * dim_);
+        for (std::size_t i = 0; i < queries.size(); ++i) coords_(queries[i], q.data() + i * dim_);
+        std::vector<int64_t> off(queries.size() + 1);
+        std::vector<int32_t> idx(queries.size() * kmax);
+        std::vector<float> dist(queries.size() * kmax);
+        throw_on_error(ompl_amd_nn_knn_packed(nn_, q.data(), (int64_t)queries.size(), kq.data(), (int)kmax,
+                                              off.data(), idx.data(), dist.data()));
+        for (std::size_t i = 0; i < queries.size(); ++i)
+            for (int64_t j = off[i]; j < off[i + 1]; ++j)
+                if (idx[(std::size_t)j] >= 0) out[i].push_back(data_[(std::size_t)idx[(std::size_t)j]]);
+    }
+
+    void nearestRPacked(const std::vector<_T> &queries, const std::vector<double> &radius,
+                        std::vector<std::vector<_T>> &out) const
+    {
+        if (radius.size() != queries.size())
+            throw ompl::Exception("NearestNeighborsGPU::nearestRPacked: one radius per query");
+        out.assign(queries.size(), {});
+        if (queries.empty()) return;
+        std::vector<float> q(queries.size() * dim_), r(queries.size());
+        for (std::size_t i = 0; i < queries.size(); ++i) {
+            coords_(queries[i], q.data() + i * dim_);
+            r[i] = (float)radius[i];
+        }
+        std::vector<int64_t> off(queries.size() + 1);
+        int64_t total = 0;
+        throw_on_error(ompl_amd_nn_radius_packed(nn_, q.data(), (int64_t)queries.size(), r.data(), off.data(),
+                                                 nullptr, nullptr, 0, &total));
+        std::vector<int32_t> idx((std::size_t)total);
+        std::vector<float> dist((std::size_t)total);
+        throw_on_error(ompl_amd_nn_radius_packed(nn_, q.data(), (int64_t)queries.size(), r.data(), off.data(),
+                                                 idx.data(), dist.data(), total, &total));
+        for (std::size_t i = 0; i < queries.size(); ++i)
+            for (int64_t j = off[i]; j < off[i + 1]; ++j) out[i].push_back(data_[(std::size_t)idx[(std::size_t)j]]);
+    }
+
     ompl_amd_nn handle() const
     {
         return nn_;

@@ -114,6 +114,32 @@ int main(int argc, char **argv)
         small.nearestK(pts[0], 100, a);
         if (a.size() != 5 || a[0].id != 0) ++bad;
     }
+    // packed forms: the same lists as one query at a time, id for id
+    {
+        auto *gpu = static_cast<ompl_amd::NearestNeighborsGPU<Pt> *>(base.get());
+        std::vector<Pt> qs;
+        std::vector<std::size_t> ks;
+        std::vector<double> rs;
+        for (int q = 0; q < 40; ++q) {
+            qs.push_back(pts[(size_t)q * 11 + 3]);
+            ks.push_back((std::size_t)(q % 13));  // 0 .. 12
+            rs.push_back(0.4 + 0.05 * (q % 9));
+        }
+        std::vector<std::vector<Pt>> pk, pr;
+        gpu->nearestKPacked(qs, ks, pk);
+        gpu->nearestRPacked(qs, rs, pr);
+        for (size_t i = 0; i < qs.size(); ++i) {
+            std::vector<Pt> a, b;
+            base->nearestK(qs[i], ks[i], a);
+            base->nearestR(qs[i], rs[i], b);
+            if (ids(a) != ids(pk[i]) || a.size() != pk[i].size()) ++bad;
+            for (size_t j = 0; j < a.size() && j < pk[i].size(); ++j)
+                if (a[j].id != pk[i][j].id) ++bad;
+            if (b.size() != pr[i].size()) ++bad;
+            for (size_t j = 0; j < b.size() && j < pr[i].size(); ++j)
+                if (b[j].id != pr[i][j].id) ++bad;
+        }
+    }
     base->remove(pts[3]);
     if (base->size() != 499) ++bad;
     std::printf("adapter check: %d mismatches, %d boundary ties\n", bad, boundary);
