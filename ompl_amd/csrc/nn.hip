@@ -1297,20 +1297,16 @@ int ompl_amd_nn_knn(ompl_amd_nn h, const float *q, int64_t nq, int k, int32_t *o
         NNIndex &x = nn_of(h);
         OA_REQUIRE(nq >= 0 && (nq == 0 || (q && out_i && out_d)), OMPL_AMD_ERR_ARG, "bad arguments");
         if (nq == 0) return;
+        OA_REQUIRE(k >= 1 && k <= 64, OMPL_AMD_ERR_ARG, "k must be in [1, 64] (got %d)", k);
         hipStream_t s = default_stream();
-        float *dq = nullptr, *dd = nullptr;
-        int32_t *di = nullptr;
-        OA_HIP(hipMallocAsync((void **)&dq, (size_t)nq * x.dim * sizeof(float), s));
-        OA_HIP(hipMallocAsync((void **)&dd, (size_t)nq * k * sizeof(float), s));
-        OA_HIP(hipMallocAsync((void **)&di, (size_t)nq * k * sizeof(int32_t), s));
+        DevBufs B(s); // freed (after the copies) on every exit
+        float *dq = B.alloc<float>((size_t)nq * x.dim);
+        float *dd = B.alloc<float>((size_t)nq * k);
+        int32_t *di = B.alloc<int32_t>((size_t)nq * k);
         OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
         knn(x, dq, nq, k, nullptr, nullptr, di, dd, s);
         OA_HIP(hipMemcpyAsync(out_d, dd, (size_t)nq * k * sizeof(float), hipMemcpyDeviceToHost, s));
         OA_HIP(hipMemcpyAsync(out_i, di, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        OA_HIP(hipFreeAsync(dq, s));
-        OA_HIP(hipFreeAsync(dd, s));
-        OA_HIP(hipFreeAsync(di, s));
-        OA_HIP(hipStreamSynchronize(s));
     });
 }
 
@@ -1321,29 +1317,21 @@ int ompl_amd_nn_radius(ompl_amd_nn h, const float *q, int64_t nq, float r, int64
         NNIndex &x = nn_of(h);
         OA_REQUIRE(nq >= 0 && offsets && total && (nq == 0 || q), OMPL_AMD_ERR_ARG, "bad arguments");
         hipStream_t s = default_stream();
-        float *dq = nullptr;
-        int64_t *doff = nullptr;
-        int32_t *di = nullptr;
-        float *dd = nullptr;
+        DevBufs B(s); // freed (after the copies) on every exit
         const bool want = out_i || out_d;
-        OA_HIP(hipMallocAsync((void **)&dq, (size_t)(nq ? nq : 1) * x.dim * sizeof(float), s));
-        OA_HIP(hipMallocAsync((void **)&doff, (size_t)(nq + 1) * sizeof(int64_t), s));
-        if (want && capacity > 0) {
-            OA_HIP(hipMallocAsync((void **)&di, (size_t)capacity * sizeof(int32_t), s));
-            OA_HIP(hipMallocAsync((void **)&dd, (size_t)capacity * sizeof(float), s));
-        }
+        float *dq = B.alloc<float>((size_t)nq * x.dim);
+        int64_t *doff = B.alloc<int64_t>((size_t)nq + 1);
+        int32_t *di = want && capacity > 0 ? B.alloc<int32_t>((size_t)capacity) : nullptr;
+        float *dd = want && capacity > 0 ? B.alloc<float>((size_t)capacity) : nullptr;
         if (nq) OA_HIP(hipMemcpyAsync(dq, q, (size_t)nq * x.dim * sizeof(float), hipMemcpyHostToDevice, s));
         int64_t tot = 0;
+        *total = 0;
         radius(x, dq, nq, r, nullptr, doff, want ? di : nullptr, want ? dd : nullptr, capacity, &tot, s);
         OA_HIP(hipMemcpyAsync(offsets, doff, (size_t)(nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         if (want && tot > 0) {
             if (out_i) OA_HIP(hipMemcpyAsync(out_i, di, (size_t)tot * sizeof(int32_t), hipMemcpyDeviceToHost, s));
             if (out_d) OA_HIP(hipMemcpyAsync(out_d, dd, (size_t)tot * sizeof(float), hipMemcpyDeviceToHost, s));
         }
-        OA_HIP(hipFreeAsync(dq, s));
-        OA_HIP(hipFreeAsync(doff, s));
-        if (di) OA_HIP(hipFreeAsync(di, s));
-        if (dd) OA_HIP(hipFreeAsync(dd, s));
         OA_HIP(hipStreamSynchronize(s));
         *total = tot;
     });
