@@ -399,7 +399,71 @@ def scope_rows(oa, run_cpu, cpu_threads, only=None):
         sampling_rows(oa, rows, oracle_py, timed, sample_uniform, sample_valid)
     if want("state_lists"):
         state_list_row(oa, rows, oracle_py, timed)
+    if want("motion_sparse"):
+        motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed)
     return rows
+
+
+def motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed):
+    """The motion check where most edges are collision-free: 16 spheres + 16
+    boxes, 16,384 edges of lognormal(-0.5, 1) length, so that most edges pass
+    their end state and are walked to full resolution (the walk, not the end
+    check, carries the time).  Timed with the launch shape in force (the
+    dense batch's) and with the shape the autotuner picks for this batch."""
+    import torch
+
+    s, b = oa.workload.obstacles(1002, 16, 16)
+    env = oa.Env(s, b)
+    a_rows, b_rows = oa.workload.edges(9, N_EDGES, -0.5, 1.0, 0.05, 5.0)
+    lvs = lvs_0p01()
+    A = torch.from_numpy(a_rows.T.copy()).cuda()
+    B = torch.from_numpy(b_rows.T.copy()).cuda()
+    out = torch.empty(N_EDGES, dtype=torch.uint8, device="cuda")
+    work = oa.motion_work(env, A, B, lvs)
+    flops = oa.work_flops(work)
+    flops_end = oa.work_flops(oa.validity_work(env, B.float().contiguous()))
+
+    def measure():
+        ms = timed(lambda: oa.check_motions(env, A, B, lvs, out=out), 50, warm=5)
+        valid = out.cpu().numpy().copy()
+        oa.profile.reset()
+        oa.profile.enable(True)
+        for _ in range(20):
+            oa.check_motions(env, A, B, lvs, out=out)
+        torch.cuda.synchronize()
+        kernels = {}
+        for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk"):
+            n, kms = oa.profile.query(name)
+            if n:
+                kernels[name] = {"launches": n, "avg_us": 1e3 * kms / n}
+        oa.profile.enable(False)
+        oa.profile.reset()
+        m = {"ms": ms, "edge_checks_per_s": N_EDGES / (ms * 1e-3), "states_checked_per_s": work["states"] / (ms * 1e-3),
+             "kernels": kernels, "roofline_step": roof(flops, ms)}
+        walk = next((k for k in ("motion_walk", "motion_chunk") if k in kernels), None)
+        if walk is not None:
+            r = roof(flops - flops_end, kernels[walk]["avg_us"] * 1e-3)
+            m["roofline_walk"] = dict(r, kernel=walk, kernel_avg_us=kernels[walk]["avg_us"])
+        return m, valid
+
+    saved = oa.get_tuning()
+    dense, valid = measure()
+    tuned, _ = oa.autotune_motions(env, A, B, lvs, reps=5)
+    own, valid_own = measure()
+    oa.set_tuning(**saved)
+    row = {"edges": N_EDGES, "obstacles": 32, "states_per_step": work["states"], "valid_fraction": float(valid.mean()),
+           "algorithmic_gflop_per_step": flops / 1e9, "dense_batch_shape": dict(dense, tuning=saved),
+           "own_shape": dict(own, tuning=tuned)}
+    if oracle_py is not None:
+        oenv = oracle_py.OracleEnv(s, b)
+        t0 = time.perf_counter()
+        ref, _ = oracle_py.check_motions(oenv, a_rows, b_rows, lvs, threads=cpu_threads)
+        dt = time.perf_counter() - t0
+        row["cpu_baseline"] = {"edge_checks_per_s": N_EDGES / dt, "cores": cpu_threads, "kind": "port",
+                               "sample": f"the full 16,384-edge batch, {dt:.2f} s; {CPU_FLAGS}"}
+        row["parity_bit_exact"] = bool(np.array_equal(ref, valid)) and bool(np.array_equal(ref, valid_own))
+    env.close()
+    rows["motion_sparse"] = row
 
 
 def validity_row(oa, rows, oracle_py, cpu_threads, roof, timed):
@@ -597,7 +661,7 @@ def main(argv=None):
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements")
     ap.add_argument("--rows", type=str, default=None,
                     help="comma-separated rows to measure (validity_64_spheres, knn_l2_100k, se3_1m, sampling, "
-                         "state_lists; default all)")
+                         "state_lists, motion_sparse; default all)")
     ap.add_argument("--benchmark-log", type=str, default=None,
                     help="also write the served stream's runs as an OMPL benchmark log file")
     ap.add_argument("--calibrate-copy", type=int, default=0,

@@ -180,19 +180,43 @@ __device__ __forceinline__ bool endpoint_check(const MotionArgs &A, const float4
     return endpoint_finish<Shape::GS>(A, e, live, H, m, a, b);
 }
 
-// Wave-aggregated append of the survivors to the list (with their segment
+// Block-aggregated append of the survivors to the list (with their segment
 // count and, when kept, their endpoints contiguous for the chunk walk).
+// Every thread of the block calls it (it holds three barriers).  The waves'
+// counts and largest segment counts meet in LDS, and one thread per block
+// makes the two device-wide atomics: device-scope atomics on one address
+// from every CU serialise, and with many survivors (a sparse scene) one
+// atomic per survivor cost more than the s2 checks themselves.
 __device__ __forceinline__ void append_survivor(const MotionArgs &A, int64_t e, bool survivor, int m,
                                                 const double a[kDof], const double b[kDof])
 {
+    __shared__ int blk_count, blk_max, blk_base;
     const int lane = lane_id();
+    if (threadIdx.x == 0) {
+        blk_count = 0;
+        blk_max = 0;
+    }
+    __syncthreads();
     const uint64_t sm = wave_ballot(survivor);
-    if (!sm) return;
-    int base = 0;
-    if (lane == lowest_bit(sm)) base = atomicAdd(A.list_count, __popcll(sm));
-    base = __shfl(base, lowest_bit(sm));
+    int woff = 0;
+    if (sm) {
+        int mm = survivor ? m : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) mm = max(mm, __shfl_xor(mm, d));
+        if (lane == lowest_bit(sm)) {
+            woff = atomicAdd(&blk_count, __popcll(sm));
+            atomicMax(&blk_max, mm);
+        }
+        woff = __shfl(woff, lowest_bit(sm));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blk_count > 0) {
+        blk_base = atomicAdd(A.list_count, blk_count);
+        if (A.max_m) atomicMax(A.max_m, blk_max);
+    }
+    __syncthreads();
     if (survivor) {
-        const int slot = base + __popcll(sm & (((uint64_t)1 << lane) - 1));
+        const int slot = blk_base + woff + __popcll(sm & (((uint64_t)1 << lane) - 1));
         A.out[e] = 1; // until the walk finds an invalid position
         A.list[slot] = (int32_t)e;
         if (A.first_bad) A.first_bad[slot] = INT_MAX;
@@ -205,7 +229,6 @@ __device__ __forceinline__ void append_survivor(const MotionArgs &A, int64_t e, 
                 ab[kDof + j] = b[j];
             }
         }
-        if (A.max_m) atomicMax(A.max_m, m);
     }
 }
 
