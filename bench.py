@@ -409,7 +409,8 @@ def motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed):
     boxes, 16,384 edges of lognormal(-0.5, 1) length, so that most edges pass
     their end state and are walked to full resolution (the walk, not the end
     check, carries the time).  Timed with the launch shape in force (the
-    dense batch's) and with the shape the autotuner picks for this batch."""
+    dense batch's) and with the environment's own shape, which the
+    autotuner picks for this batch."""
     import torch
 
     s, b = oa.workload.obstacles(1002, 16, 16)
@@ -448,9 +449,8 @@ def motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed):
 
     saved = oa.get_tuning()
     dense, valid = measure()
-    tuned, _ = oa.autotune_motions(env, A, B, lvs, reps=5)
+    tuned, _ = env.autotune(A, B, lvs, reps=5)  # the environment's own shapes; the library's stay
     own, valid_own = measure()
-    oa.set_tuning(**saved)
     row = {"edges": N_EDGES, "obstacles": 32, "states_per_step": work["states"], "valid_fraction": float(valid.mean()),
            "algorithmic_gflop_per_step": flops / 1e9, "dense_batch_shape": dict(dense, tuning=saved),
            "own_shape": dict(own, tuning=tuned)}

@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 311
+#define OMPL_AMD_VERSION 312
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -229,6 +229,17 @@ int ompl_amd_set_tuning(const ompl_amd_tuning *t);
 int ompl_amd_autotune_motions(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
                               double longest_valid_segment, unsigned factor, int reps, ompl_amd_tuning *best,
                               double *best_us);
+/* Launch shapes of one environment: the batched checks on `env` (validity,
+ * motions, last-valid motions, captured motion graphs at capture, the
+ * no-allocation motion check) use `t` instead of the library's current tuning;
+ * t == NULL makes the environment follow the library's tuning again.  The
+ * fastest shape depends on the obstacle count (a dense scene wants many
+ * obstacle partitions per state, a sparse one a single partition and more
+ * waves per CU).  ompl_amd_env_get_tuning returns the shapes in force for
+ * `env` and sets *own to 1 when they are the environment's own.  No
+ * counterpart in the reference. */
+int ompl_amd_env_set_tuning(ompl_amd_env env, const ompl_amd_tuning *t);
+int ompl_amd_env_get_tuning(ompl_amd_env env, ompl_amd_tuning *t, int *own);
 
 /* Work a scalar checker performs on the same inputs in the reference order
  * (s2, then bisection positions up to the first invalid state; per state the

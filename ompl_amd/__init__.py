@@ -163,6 +163,42 @@ class Env:
         call("ompl_amd_env_bytes", self.handle, C.byref(b))
         return b.value
 
+    def set_tuning(self, tuning: dict | None = None, **kw) -> dict:
+        """Launch shapes of this environment's batched checks: the fields of
+        `tuning` and `kw` over the shapes now in force for it; set_tuning(None)
+        with no keywords makes it follow the library's tuning again."""
+        if tuning is None and not kw:
+            call("ompl_amd_env_set_tuning", self.handle, None)
+            return self.tuning()
+        t = Tuning()
+        call("ompl_amd_env_get_tuning", self.handle, C.byref(t), None)
+        for k, v in dict(tuning or {}, **kw).items():
+            if not hasattr(t, k):
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        call("ompl_amd_env_set_tuning", self.handle, C.byref(t))
+        return t.as_dict()
+
+    def tuning(self) -> dict:
+        """The launch shapes in force for this environment, with "own": True
+        when they are its own rather than the library's."""
+        t = Tuning()
+        own = C.c_int()
+        call("ompl_amd_env_get_tuning", self.handle, C.byref(t), C.byref(own))
+        return dict(t.as_dict(), own=bool(own.value))
+
+    def autotune(self, a_soa, b_soa, lvs: float, factor: int = 1, reps: int = 20):
+        """Runs the motion autotuner on a device-resident batch of this
+        environment and keeps the winner as the environment's own shapes (the
+        library's tuning is left as it was); returns (tuning, us per batch)."""
+        saved = get_tuning()
+        try:
+            t, us = autotune_motions(self, a_soa, b_soa, lvs, factor, reps)
+        finally:
+            set_tuning(**saved)
+        self.set_tuning(t)
+        return t, us
+
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             call("ompl_amd_env_destroy", self.handle)

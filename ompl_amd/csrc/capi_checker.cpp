@@ -109,10 +109,10 @@ void ompl_amd::check_motions_tuned(const Tuning &t, const Env &env, const double
 
 namespace {
 
-void check_motions(const Env &env, const double *a, const double *b, int64_t n, double lvs, unsigned factor,
+void check_motions(const ompl_amd_env_s &e, const double *a, const double *b, int64_t n, double lvs, unsigned factor,
                    uint8_t *out, int32_t *checked, hipStream_t s)
 {
-    check_motions_tuned(get_tuning(), env, a, b, n, lvs, factor, out, checked, s);
+    check_motions_tuned(tuning_for(e), e.env, a, b, n, lvs, factor, out, checked, s);
 }
 
 // Candidate launch shapes of the motion autotuner.
@@ -196,7 +196,7 @@ void graph_workspace(Workspace &w, int64_t n)
 
 // Records the group twice (once per survivor counter set of every batch's
 // workspace) and instantiates both recordings.
-void record_group(ompl_amd_motion_graph_s *g, const Env &env, int nb, const double *const *a,
+void record_group(ompl_amd_motion_graph_s *g, const Env &env, const Tuning &t, int nb, const double *const *a,
                   const double *const *b, const int64_t *n, double lvs, unsigned factor, uint8_t *const *out,
                   int32_t *const *checked)
 {
@@ -214,7 +214,6 @@ void record_group(ompl_amd_motion_graph_s *g, const Env &env, int nb, const doub
     OA_HIP(hipDeviceSynchronize());
     // capture on the calling thread's own stream (the null stream cannot capture)
     hipStream_t cs = thread_resources().stream;
-    const Tuning t = get_tuning();
     for (int r = 0; r < 2; ++r) {
         // recording r uses counter set r of each workspace when its batch
         // alternates sets (check_motions_tuned flips ws.parity after recording)
@@ -351,7 +350,7 @@ int ompl_amd_validate_states_dev(ompl_amd_env e, const float *q, int64_t n, uint
         OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative state count");
         if (n == 0) return;
         OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
-        const Tuning t = get_tuning();
+        const Tuning t = tuning_for(*e);
         launch_validate<float>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_obstacle_parts,
                                t.validity_block, resolve_stream(stream), t.validity_shared_fk != 0);
     });
@@ -364,7 +363,7 @@ int ompl_amd_validate_states_f64_dev(ompl_amd_env e, const double *q, int64_t n,
         OA_REQUIRE(n >= 0, OMPL_AMD_ERR_ARG, "negative state count");
         if (n == 0) return;
         OA_REQUIRE(q && out, OMPL_AMD_ERR_ARG, "null state or output pointer");
-        const Tuning t = get_tuning();
+        const Tuning t = tuning_for(*e);
         launch_validate<double>(env.view(), q, n, out, t.validity_lanes_per_state, t.validity_obstacle_parts,
                                 t.validity_block, resolve_stream(stream), t.validity_shared_fk != 0);
     });
@@ -397,7 +396,10 @@ int ompl_amd_arm_spheres_dev(const float *q, int64_t n, float *out, void *stream
 int ompl_amd_check_motions_dev(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
                                unsigned factor, uint8_t *out, int32_t *checked, void *stream)
 {
-    return guarded([&] { check_motions(env_of(e), a, b, n, lvs, factor, out, checked, resolve_stream(stream)); });
+    return guarded([&] {
+        env_of(e);
+        check_motions(*e, a, b, n, lvs, factor, out, checked, resolve_stream(stream));
+    });
 }
 
 int ompl_amd_check_motions_last_valid_dev(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
@@ -409,7 +411,7 @@ int ompl_amd_check_motions_last_valid_dev(ompl_amd_env e, const double *a, const
         if (n == 0) return;
         OA_REQUIRE(a && b && out && last_t, OMPL_AMD_ERR_ARG, "null pointer");
         OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "longest valid segment must be > 0 and factor >= 1");
-        const Tuning t = get_tuning();
+        const Tuning t = tuning_for(*e);
         MotionArgs A{};
         A.env = env.view();
         A.a = a;
@@ -513,6 +515,32 @@ int ompl_amd_check_motion_last_valid(ompl_amd_env e, const double *a, const doub
 
 } // extern "C"
 
+extern "C" int ompl_amd_env_set_tuning(ompl_amd_env e, const ompl_amd_tuning *t)
+{
+    return guarded([&] {
+        env_of(e);
+        if (t) check_tuning(tuning_from_c(*t));
+        std::lock_guard<std::mutex> lk(e->tuning_mutex);
+        e->has_tuning = t != nullptr;
+        if (t) e->tuning = tuning_from_c(*t);
+    });
+}
+
+extern "C" int ompl_amd_env_get_tuning(ompl_amd_env e, ompl_amd_tuning *t, int *own)
+{
+    return guarded([&] {
+        env_of(e);
+        OA_REQUIRE(t, OMPL_AMD_ERR_ARG, "null tuning");
+        bool has = false;
+        {
+            std::lock_guard<std::mutex> lk(e->tuning_mutex);
+            has = e->has_tuning;
+        }
+        *t = tuning_to_c(tuning_for(*e));
+        if (own) *own = has ? 1 : 0;
+    });
+}
+
 extern "C" int ompl_amd_autotune_motions(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,
                                          unsigned factor, int reps, ompl_amd_tuning *best, double *best_us)
 {
@@ -578,7 +606,7 @@ int ompl_amd_motion_graph_create_group(ompl_amd_env e, int n_batches, const doub
         auto *g = new ompl_amd_motion_graph_s();
         for (int i = 0; i < n_batches; ++i) g->ws.push_back(std::make_unique<Workspace>());
         try {
-            record_group(g, env, n_batches, a, b, n, lvs, factor, out, checked);
+            record_group(g, env, tuning_for(*e), n_batches, a, b, n, lvs, factor, out, checked);
         } catch (...) {
             delete g;
             throw;

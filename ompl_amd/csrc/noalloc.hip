@@ -78,6 +78,6 @@ extern "C" int ompl_amd_check_motions_noalloc_dev(ompl_amd_env e, const double *
         w.list_cap = (size_t)kNoAllocMaxEdges;
         w.parity = 0;
         launch_zero(w.counters, (int)(kNoAllocCounterBytes / sizeof(int32_t)), s);
-        check_motions_tuned(get_tuning(), e->env, a, b, n, lvs, factor, out, checked, s, &w);
+        check_motions_tuned(tuning_for(*e), e->env, a, b, n, lvs, factor, out, checked, s, &w);
     });
 }

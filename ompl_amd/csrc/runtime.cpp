@@ -207,7 +207,7 @@ Tuning get_tuning()
     return g_tuning;
 }
 
-void set_tuning(const Tuning &t)
+void check_tuning(const Tuning &t)
 {
     auto pow2 = [](int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; };
     OA_REQUIRE(shape_supported(t.validity_lanes_per_state, t.validity_obstacle_parts), OMPL_AMD_ERR_ARG,
@@ -228,8 +228,22 @@ void set_tuning(const Tuning &t)
         OA_REQUIRE(fused_supported(t.validity_lanes_per_state, t.validity_obstacle_parts, t.motion_lanes_per_state,
                                    t.motion_obstacle_parts),
                    OMPL_AMD_ERR_ARG, "fused strategy: shape pair not built");
+}
+
+void set_tuning(const Tuning &t)
+{
+    check_tuning(t);
     std::lock_guard<std::mutex> lk(g_tuning_mutex);
     g_tuning = t;
+}
+
+Tuning tuning_for(const ompl_amd_env_s &e)
+{
+    {
+        std::lock_guard<std::mutex> lk(e.tuning_mutex);
+        if (e.has_tuning) return e.tuning;
+    }
+    return get_tuning();
 }
 
 ompl_amd_tuning tuning_to_c(const Tuning &x)

@@ -71,6 +71,8 @@ struct Tuning {
 };
 
 Tuning get_tuning();
+// Rejects (OA_REQUIRE) a tuning whose shapes or limits are not built.
+void check_tuning(const Tuning &t);
 
 struct Env;
 // Batched DiscreteMotionValidator::checkMotion(s1, s2) over device arrays with
@@ -126,4 +128,15 @@ struct ompl_amd_env_s {
     // server requests (queued or running) that read this environment;
     // ompl_amd_env_destroy refuses while any remain
     std::atomic<int> server_users{0};
+    // launch shapes of this environment's batched checks, when set; else the
+    // library's current tuning (the fastest shape depends on the obstacle
+    // count: many obstacle partitions suit a dense scene, one a sparse one)
+    mutable std::mutex tuning_mutex;
+    bool has_tuning = false;
+    ompl_amd::Tuning tuning{};
 };
+
+namespace ompl_amd {
+// The tuning the batched checks on environment e use.
+Tuning tuning_for(const ompl_amd_env_s &e);
+} // namespace ompl_amd

@@ -292,6 +292,74 @@ def test_check_motions_sparse_env_deep_walks(gpu, oracle, strategy, eshape, wsha
     gpu.set_tuning(**DEFAULT)
 
 
+@pytest.mark.parametrize("strategy,eshape,wshape,wpc", [(3, (8, 2), (8, 1), 8), (3, (4, 1), (7, 1), 32),
+                                                         (3, (7, 4), (7, 9), 8), (4, (8, 2), (8, 4), 16),
+                                                         (1, (8, 1), (1, 1), 16), (2, (8, 1), (8, 1), 8)])
+def test_check_motions_many_survivors(gpu, oracle, strategy, eshape, wshape, wpc):
+    """16,384 edges in a 16 + 16 obstacle scene: about 10,000 edges pass s2,
+    more than the chunk walk numbers edge-major, so its chunk-major form walks
+    every position; the s2 pass appends that many survivors."""
+    print(f"many survivors strategy={strategy} shapes={eshape},{wshape} wpc={wpc}", flush=True)
+    set_strategy(gpu, strategy, eshape, wshape)
+    gpu.set_tuning(motion_waves_per_cu=wpc)
+    env, oenv, _ = env_pair(gpu, oracle, 1002, 16, 16)
+    a, b = motion_case(gpu, 9, 16384, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs, threads=8)
+    assert ref_v.sum() > 4096
+    A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
+    v, c = gpu.check_motions(env, A, B, lvs, exact_count=True)
+    assert np.array_equal(v.cpu().numpy(), ref_v)
+    assert np.array_equal(c.cpu().numpy(), ref_c)
+    for _ in range(2):  # repeated batches reuse the counter sets
+        assert np.array_equal(gpu.check_motions(env, A, B, lvs).cpu().numpy(), ref_v)
+    gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8)
+
+
+def test_env_tuning_overrides_library_tuning(gpu, oracle):
+    """An environment's own launch shapes take the place of the library's for
+    its batches only, and clearing them restores the library's."""
+    from ompl_amd._lib import OmplAmdError
+
+    set_strategy(gpu, 3, (8, 2), (8, 1))
+    env, oenv, _ = env_pair(gpu, oracle, 1002, 6, 4)
+    other, _, _ = env_pair(gpu, oracle, 1003, 6, 4)
+    a, b = motion_case(gpu, 9, 3000, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs)
+    A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
+    assert env.tuning()["own"] is False
+    t = env.set_tuning(motion_strategy=2, validity_lanes_per_state=8, validity_obstacle_parts=1,
+                       motion_lanes_per_state=8, motion_obstacle_parts=1)
+    assert t["motion_strategy"] == 2 and env.tuning()["own"] is True
+    assert gpu.get_tuning()["motion_strategy"] == 3 and other.tuning()["own"] is False
+
+    def launches(e):
+        gpu.profile.reset()
+        gpu.profile.enable(True)
+        try:
+            v, c = gpu.check_motions(e, A, B, lvs, exact_count=True)
+            return v, c, gpu.profile.query("motion_fused")[0], gpu.profile.query("motion_chunk")[0]
+        finally:
+            gpu.profile.enable(False)
+            gpu.profile.reset()
+
+    v, c, fused, chunk = launches(env)
+    assert (fused, chunk) == (1, 0)
+    assert np.array_equal(v.cpu().numpy(), ref_v) and np.array_equal(c.cpu().numpy(), ref_c)
+    _, _, fused, chunk = launches(other)
+    assert (fused, chunk) == (0, 1)
+    with pytest.raises(OmplAmdError):
+        env.set_tuning(motion_waves_per_cu=99)
+    assert env.tuning()["motion_strategy"] == 2  # a rejected tuning leaves the shapes in force
+    env.set_tuning(None)
+    assert env.tuning()["own"] is False
+    v, c, fused, chunk = launches(env)
+    assert (fused, chunk) == (0, 1)
+    assert np.array_equal(v.cpu().numpy(), ref_v) and np.array_equal(c.cpu().numpy(), ref_c)
+    gpu.set_tuning(**DEFAULT)
+
+
 def test_check_motions_edge_cases(gpu, oracle):
     env, oenv, _ = env_pair(gpu, oracle, 1003, 8, 8)
     rng = np.random.default_rng(4)
