@@ -367,7 +367,7 @@ def gnat_baseline(oracle_py, metric, db, qs, k, radius, threads, gpu_idx, second
     return out
 
 
-def scope_rows(oa, run_cpu, cpu_threads, only=None):
+def scope_rows(oa, run_cpu, cpu_threads, only=None, sparse_tuning=None):
     """Throughput of the other rows of the hot path, each on its configuration,
     with its algorithmic work and a bounded CPU-restatement baseline.  only:
     names of the rows to run (None: all)."""
@@ -400,17 +400,17 @@ def scope_rows(oa, run_cpu, cpu_threads, only=None):
     if want("state_lists"):
         state_list_row(oa, rows, oracle_py, timed)
     if want("motion_sparse"):
-        motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed)
+        motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed, sparse_tuning)
     return rows
 
 
-def motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed):
+def motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed, own_tuning=None):
     """The motion check where most edges are collision-free: 16 spheres + 16
     boxes, 16,384 edges of lognormal(-0.5, 1) length, so that most edges pass
     their end state and are walked to full resolution (the walk, not the end
     check, carries the time).  Timed with the launch shape in force (the
     dense batch's) and with the environment's own shape, which the
-    autotuner picks for this batch."""
+    autotuner picks for this batch (or `own_tuning` when given)."""
     import torch
 
     s, b = oa.workload.obstacles(1002, 16, 16)
@@ -449,7 +449,10 @@ def motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed):
 
     saved = oa.get_tuning()
     dense, valid = measure()
-    tuned, _ = env.autotune(A, B, lvs, reps=5)  # the environment's own shapes; the library's stay
+    if own_tuning:
+        tuned = env.set_tuning(own_tuning)
+    else:
+        tuned, _ = env.autotune(A, B, lvs, reps=5)  # the environment's own shapes; the library's stay
     own, valid_own = measure()
     row = {"edges": N_EDGES, "obstacles": 32, "states_per_step": work["states"], "valid_fraction": float(valid.mean()),
            "algorithmic_gflop_per_step": flops / 1e9, "dense_batch_shape": dict(dense, tuning=saved),
@@ -662,6 +665,11 @@ def main(argv=None):
     ap.add_argument("--rows", type=str, default=None,
                     help="comma-separated rows to measure (validity_64_spheres, knn_l2_100k, se3_1m, sampling, "
                          "state_lists, motion_sparse; default all)")
+    ap.add_argument("--sparse-tuning", type=str, default=None,
+                    help="JSON dict: the sparse-scene row's own launch shapes (skips its autotuner run)")
+    ap.add_argument("--rows-only", action="store_true",
+                    help="measure only the per-row workloads (with --tuning's shapes, else the library's) and print "
+                         "them as one JSON line; no headline value")
     ap.add_argument("--benchmark-log", type=str, default=None,
                     help="also write the served stream's runs as an OMPL benchmark log file")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -688,6 +696,14 @@ def main(argv=None):
     import ompl_amd as oa
 
     oa.init(0)
+    if args.rows_only:
+        if args.tuning:
+            oa.set_tuning(**json.loads(args.tuning))
+        rows_res = scope_rows(oa, not args.no_cpu, args.cpu_threads or cpu_threads_default(),
+                              only=set(args.rows.split(",")) if args.rows else None,
+                              sparse_tuning=json.loads(args.sparse_tuning) if args.sparse_tuning else None)
+        print(json.dumps({"rows": rows_res, "tuning": oa.get_tuning()}), flush=True)
+        return 0
     spheres, boxes = oa.workload.obstacles(ENV_SEED, N_SPHERES, N_BOXES)
     env = oa.Env(spheres, boxes)
     a_rows, b_rows = oa.workload.edges(EDGE_SEED, N_EDGES)
@@ -932,7 +948,8 @@ def main(argv=None):
     if not args.no_rows:
         oa.set_tuning(**t)  # the batch's tuned shapes (the PRM leg tunes its own)
         rows_res = scope_rows(oa, not args.no_cpu, args.cpu_threads or cpu_threads_default(),
-                              only=set(args.rows.split(",")) if args.rows else None)
+                              only=set(args.rows.split(",")) if args.rows else None,
+                              sparse_tuning=json.loads(args.sparse_tuning) if args.sparse_tuning else None)
 
     res = {
         "metric": "edge-checks/s + p50/p99 plan latency, served vs plain GPU path, bit-exact",
