@@ -210,11 +210,18 @@ __device__ __forceinline__ bool share_tile(int64_t tj)
     return tj != 0 && (tj < 4 || (tj < 64 ? (tj & 7) == 0 : (tj & 63) == 0));
 }
 
+// Words per point row of a wave's tile in LDS: the coordinates padded to a
+// multiple of 4, so a candidate's row is read with 16-byte LDS reads.
+__host__ __device__ constexpr int knn_row_words(int dim)
+{
+    return (dim + 3) & ~3;
+}
+
 // Dynamic LDS of a k-NN workgroup of W waves: each wave's current tile, then
 // the bound slots (M <= 2 W per query, 32 queries).
 __host__ __device__ constexpr size_t knn_lds_bytes(int dim, int w)
 {
-    return (size_t)w * 32 * dim * sizeof(float) + (size_t)32 * 2 * w * sizeof(unsigned long long);
+    return (size_t)w * 32 * knn_row_words(dim) * sizeof(float) + (size_t)32 * 2 * w * sizeof(unsigned long long);
 }
 
 // Waves of one k-NN workgroup at most: 16 (1,024 threads, 4 waves per SIMD)
@@ -319,14 +326,17 @@ struct TopKey {
     }
     __device__ __forceinline__ void insert(unsigned long long nk)
     {
-        // each slot keeps the smaller of its key and the carried one and
-        // passes the larger on; the largest leaves the list
+        // the list is ascending, so slot s of the new list is
+        // max(old[s - 1], min(old[s], nk)): old[s] below nk stays, the first
+        // slot at or above nk takes nk, later slots take their predecessor,
+        // and the largest key leaves.  The slots depend only on the old list,
+        // not on each other (no carried chain).
 #pragma unroll
-        for (int s = 0; s < KMAX; ++s) {
-            const unsigned long long lo = nk < key[s] ? nk : key[s];
-            nk = nk < key[s] ? key[s] : nk;
-            key[s] = lo;
+        for (int s = KMAX - 1; s > 0; --s) {
+            const unsigned long long lo = key[s] < nk ? key[s] : nk;
+            key[s] = key[s - 1] > lo ? key[s - 1] : lo;
         }
+        key[0] = key[0] < nk ? key[0] : nk;
     }
     __device__ __forceinline__ unsigned long long kth() const { return key[KMAX - 1]; }
 };
@@ -361,9 +371,10 @@ knn_kernel(KnnArgs A)
     // the wave's query tile and split are wave-uniform (scalar registers), so
     // is every tile address below
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float *tp = reinterpret_cast<float *>(knn_smem) + wid * (32 * DIM);
+    constexpr int ROW = knn_row_words(DIM);
+    float *tp = reinterpret_cast<float *>(knn_smem) + wid * (32 * ROW);
     unsigned long long *slots =
-        reinterpret_cast<unsigned long long *>(knn_smem + (size_t)A.group_waves * 32 * DIM * sizeof(float));
+        reinterpret_cast<unsigned long long *>(knn_smem + (size_t)A.group_waves * 32 * ROW * sizeof(float));
     const int64_t qtiles = (A.nq + 31) / 32;
     const int64_t qtile = blockIdx.x % qtiles;
     const int split = (int)(blockIdx.x / qtiles) * A.group_waves + wid;
@@ -417,6 +428,13 @@ knn_kernel(KnnArgs A)
         }
         rm = A.removed[t * 32 + (lane & 31)];
     };
+    // LDS word of each coordinate a lane stages (rows padded to ROW words)
+    int soff[PW];
+#pragma unroll
+    for (int w = 0; w < PW; ++w) {
+        const int i = lane + 64 * w;
+        soff[w] = (i / DIM) * ROW + i % DIM;
+    }
     float cur[KSTEPS], cpts[PW];
     uint32_t crm = 1;
     if (t0 < t1) {
@@ -489,7 +507,7 @@ knn_kernel(KnnArgs A)
 #pragma unroll
         for (int w = 0; w < PW; ++w) {
             const int i = lane + 64 * w;
-            if (i < 32 * DIM) tp[i] = cpts[w];
+            if (i < 32 * DIM) tp[soff[w]] = cpts[w];
             cpts[w] = npts[w];
         }
         crm = nrm;
@@ -498,11 +516,21 @@ knn_kernel(KnnArgs A)
             const int j = __builtin_ctz(mask);
             mask &= mask - 1;
             const int row = mfma_row(j, h);
+            // the row's coordinates in 16-byte reads
+            float e[ROW];
+#pragma unroll
+            for (int c = 0; c < ROW / 4; ++c) {
+                const float4 v = reinterpret_cast<const float4 *>(tp + row * ROW)[c];
+                e[4 * c] = v.x;
+                e[4 * c + 1] = v.y;
+                e[4 * c + 2] = v.z;
+                e[4 * c + 3] = v.w;
+            }
             // SE3: most candidates of the translation filter are far in
             // rotation; skip their acos when they are certainly above bkey
-            if (certainly_above<METRIC, DIM>(tp + row * DIM, Q.q, __uint_as_float((uint32_t)(bkey >> 32))))
+            if (certainly_above<METRIC, DIM>(e, Q.q, __uint_as_float((uint32_t)(bkey >> 32))))
                 continue;
-            const float d = exact_dist<METRIC, DIM>(tp + row * DIM, Q.q);
+            const float d = exact_dist<METRIC, DIM>(e, Q.q);
             const unsigned long long nk = pack_key(d, (int)(t * 32 + row));
             if (KMAX <= 16) {
                 if (nk < bkey) top.insert(nk);
