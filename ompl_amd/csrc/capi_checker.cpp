@@ -451,7 +451,15 @@ int ompl_amd_check_state_lists_dev(ompl_amd_env e, const double *q_soa, int64_t 
         OA_REQUIRE(n_states >= 0 && n_lists >= 0, OMPL_AMD_ERR_ARG, "negative sizes");
         if (n_lists == 0) return;
         OA_REQUIRE(offsets && out && (n_states == 0 || q_soa), OMPL_AMD_ERR_ARG, "null pointer");
-        launch_state_lists(env.view(), q_soa, n_states, offsets, n_lists, out, first_invalid, resolve_stream(stream));
+        OA_REQUIRE(n_lists < (int64_t)1 << 31 && n_states < (int64_t)1 << 31, OMPL_AMD_ERR_ARG,
+                   "at most 2^31-1 lists and states per call");
+        const Tuning t = tuning_for(*e);
+        const hipStream_t st = resolve_stream(stream);
+        std::unique_lock<std::mutex> lk; // held until the launches are enqueued
+        Workspace &w = workspace(st, (size_t)n_lists, lk);
+        launch_state_lists(env.view(), q_soa, n_states, offsets, n_lists, out, first_invalid,
+                           t.motion_lanes_per_state, t.motion_obstacle_parts,
+                           (int64_t)num_cus() * t.motion_waves_per_cu, w.list, w.first_bad, w.counters + 2, st);
     });
 }
 
@@ -470,10 +478,17 @@ int ompl_amd_check_states(ompl_amd_env e, const double *states, int count, int *
         double *q = reinterpret_cast<double *>(r.stage + head);
         for (int i = 0; i < count; ++i)
             for (int j = 0; j < kDof; ++j) q[(size_t)j * count + i] = states[(size_t)i * kDof + j];
-        launch_state_lists(env.view(), reinterpret_cast<const double *>(r.stage_dev + head), count,
-                           reinterpret_cast<const int64_t *>(r.stage_dev), 1,
-                           reinterpret_cast<uint8_t *>(r.stage_dev + 16),
-                           reinterpret_cast<int32_t *>(r.stage_dev + 32), r.stream);
+        const Tuning t = tuning_for(*e);
+        {
+            std::unique_lock<std::mutex> lk; // held until the launches are enqueued
+            Workspace &w = workspace(r.stream, 1, lk);
+            launch_state_lists(env.view(), reinterpret_cast<const double *>(r.stage_dev + head), count,
+                               reinterpret_cast<const int64_t *>(r.stage_dev), 1,
+                               reinterpret_cast<uint8_t *>(r.stage_dev + 16),
+                               reinterpret_cast<int32_t *>(r.stage_dev + 32), t.motion_lanes_per_state,
+                               t.motion_obstacle_parts, (int64_t)num_cus() * t.motion_waves_per_cu, w.list,
+                               w.first_bad, w.counters + 2, r.stream);
+        }
         OA_HIP(hipStreamSynchronize(r.stream));
         *valid = reinterpret_cast<uint8_t *>(r.stage)[16];
         if (first_invalid && !*valid) *first_invalid = *reinterpret_cast<int32_t *>(r.stage + 32);

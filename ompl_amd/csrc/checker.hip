@@ -583,46 +583,141 @@ __global__ void __launch_bounds__(64) validate_single_kernel(EnvView env, State1
 }
 
 // Precomputed state lists (SpaceInformation::checkMotion(states, count[,
-// firstInvalidStateIndex]), base/src/SpaceInformation.cpp:288-341): one wave
-// per list walks its states in index order, G at a time, and stops at the
-// first chunk holding an invalid state; the exact lowest invalid index in that
-// chunk is the list's first invalid state.  Whether a list is valid does not
-// depend on the order the reference visits the states in (all must be valid).
-// Offsets are clamped to [0, n_states], so any offsets stay in bounds.
+// firstInvalidStateIndex]), base/src/SpaceInformation.cpp:288-341), walked in
+// chunks of G states in index order with the lowest-invalid exit, so the
+// lowest invalid index of the first chunk holding one is the list's first
+// invalid state.  Whether a list is valid does not depend on the order the
+// reference visits the states in (all must be valid).  Offsets are clamped to
+// [0, n_states], so any offsets stay in bounds.
+//   state_lists_kernel: one wave per list checks the list's first chunk; a
+//     list resolved there (an invalid state, or no states past the chunk)
+//     gets its result, the others are appended (one device atomic pair per
+//     block) to the unresolved lists, with first_bad = INT_MAX, and the
+//     largest chunk count is kept.
+//   state_list_chunks_kernel: the unresolved lists' remaining chunks, over a
+//     persistent grid, chunk-major (chunk 1 of every list, then chunk 2, ...);
+//     a chunk is skipped once an earlier state of its list is known invalid,
+//     and an invalid one lowers first_bad (atomicMin).  A long valid list is
+//     spread over many waves instead of walked by one.
+//   state_list_results_kernel: the unresolved lists' results.
+struct StateListArgs {
+    EnvView env;
+    const double *q; // dof-major [7][n_states]
+    int64_t n_states;
+    const int64_t *offsets;
+    int64_t n_lists;
+    uint8_t *out;
+    int32_t *first_invalid;
+    int32_t *unres;    // unresolved list ids
+    int32_t *unres_fb; // their lowest invalid index found so far (INT_MAX: none)
+    int32_t *counters; // [0] unresolved lists, [1] largest chunk count (zero at launch)
+};
+
+__device__ __forceinline__ void list_bounds(const StateListArgs &A, int64_t list, int64_t &lo, int64_t &hi)
+{
+    lo = min(max(A.offsets[list], (int64_t)0), A.n_states);
+    hi = max(min(A.offsets[list + 1], A.n_states), lo);
+}
+
+// Checks states [from, to) (at most G of them, from < to) of one list;
+// returns the index of the lowest invalid one, or -1.
 template <int LPS, int OPS>
-__global__ void __launch_bounds__(256) state_lists_kernel(EnvView env, const double *__restrict__ q_soa,
-                                                          int64_t n_states, const int64_t *__restrict__ offsets,
-                                                          int64_t n_lists, uint8_t *__restrict__ out,
-                                                          int32_t *__restrict__ first_invalid)
+__device__ __forceinline__ int64_t check_list_chunk(const StateListArgs &A, const float4 *lds, int64_t from,
+                                                    int64_t to)
+{
+    using Shape = GroupShape<LPS, OPS>;
+    const Shape sh(lane_id());
+    const int64_t st = from + sh.group;
+    const bool live = st < to;
+    float q[kDof];
+    load_config(A.q, A.n_states, st, live, q, from);
+    LaneSpheres<LPS> S;
+    arm_fk<LPS, Shape::GS>(q, sh.slot, S);
+    const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, A.env.ns, A.env.nb, live, sh.part);
+    const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+    return H ? from + lowest_bit(H) / Shape::GS : -1;
+}
+
+template <int LPS, int OPS>
+__global__ void __launch_bounds__(1024) state_lists_kernel(StateListArgs A)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, link_pass(OPS) * OPS>(env, lds);
-    using Shape = GroupShape<LPS, OPS>;
-    constexpr int G = Shape::STATES;
+    __shared__ int blk_count, blk_max, blk_base;
+    if (threadIdx.x == 0) {
+        blk_count = 0;
+        blk_max = 0;
+    }
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds); // ends with a barrier
+    constexpr int G = GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
-    const Shape sh(lane);
     const int64_t list = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (list >= n_lists) return;
-    const int64_t lo = min(max(offsets[list], (int64_t)0), n_states);
-    const int64_t hi = max(min(offsets[list + 1], n_states), lo);
-    int64_t bad = -1;
-    for (int64_t base = lo; base < hi; base += G) {
-        const int64_t st = base + sh.group;
-        const bool live = st < hi;
-        float q[kDof];
-        load_config(q_soa, n_states, st, live, q, base);
-        LaneSpheres<LPS> S;
-        arm_fk<LPS, Shape::GS>(q, sh.slot, S);
-        const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, env.ns, env.nb, live, sh.part);
-        const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
-        if (H) {
-            bad = base + lowest_bit(H) / Shape::GS - lo;
-            break;
+    int64_t lo = 0, hi = 0, bad = -1;
+    bool pending = false;
+    if (list < A.n_lists) { // wave-uniform
+        list_bounds(A, list, lo, hi);
+        const int64_t end = min(hi, lo + G);
+        if (lo < end) bad = check_list_chunk<LPS, OPS>(A, lds, lo, end);
+        if (bad >= 0) bad -= lo;
+        pending = bad < 0 && end < hi;
+    }
+    const int chunks = (int)((hi - lo + G - 1) / G);
+    int woff = 0;
+    if (pending && lane == 0) {
+        woff = atomicAdd(&blk_count, 1);
+        atomicMax(&blk_max, chunks);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blk_count > 0) {
+        blk_base = atomicAdd(&A.counters[0], blk_count);
+        atomicMax(&A.counters[1], blk_max);
+    }
+    __syncthreads();
+    if (lane == 0 && list < A.n_lists) {
+        if (pending) {
+            A.unres[blk_base + woff] = (int32_t)list;
+            A.unres_fb[blk_base + woff] = INT_MAX;
+        } else {
+            A.out[list] = bad < 0 ? 1 : 0;
+            if (A.first_invalid) A.first_invalid[list] = (int32_t)bad;
         }
     }
-    if (lane == 0) {
-        out[list] = bad < 0 ? 1 : 0;
-        if (first_invalid) first_invalid[list] = (int32_t)bad;
+}
+
+template <int LPS, int OPS>
+__global__ void __launch_bounds__(1024) state_list_chunks_kernel(StateListArgs A)
+{
+    extern __shared__ float4 lds[];
+    // the counters are read before the staging, so the reads overlap it
+    int n_un_v = A.counters[0], maxc_v = A.counters[1];
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
+    asm volatile("" : "+v"(n_un_v), "+v"(maxc_v));
+    const int64_t n_un = n_un_v, maxc = maxc_v;
+    if (n_un == 0 || maxc < 2) return;
+    constexpr int G = GroupShape<LPS, OPS>::STATES;
+    const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t items = n_un * (maxc - 1);
+    for (int64_t k = wave; k < items; k += waves_total) { // wave-uniform
+        const int64_t c = 1 + k / n_un, u = k - (c - 1) * n_un;
+        int64_t lo, hi;
+        list_bounds(A, A.unres[u], lo, hi);
+        const int64_t base = c * G; // relative to the list's first state
+        if (lo + base >= hi) continue;
+        const int fb = __hip_atomic_load(&A.unres_fb[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (fb < base) continue;
+        const int64_t bad = check_list_chunk<LPS, OPS>(A, lds, lo + base, min(hi, lo + base + G));
+        if (bad >= 0 && lane_id() == 0) atomicMin(&A.unres_fb[u], (int)(bad - lo));
+    }
+}
+
+__global__ void __launch_bounds__(1024) state_list_results_kernel(StateListArgs A)
+{
+    const int n_un = A.counters[0];
+    for (int u = threadIdx.x; u < n_un; u += blockDim.x) {
+        const int64_t list = A.unres[u];
+        const int fb = A.unres_fb[u];
+        A.out[list] = fb == INT_MAX ? 1 : 0;
+        if (A.first_invalid) A.first_invalid[list] = fb == INT_MAX ? -1 : fb;
     }
 }
 
@@ -972,13 +1067,27 @@ void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int 
 }
 
 void launch_state_lists(const EnvView &env, const double *q_soa, int64_t n_states, const int64_t *offsets,
-                        int64_t n_lists, uint8_t *out, int32_t *first_invalid, hipStream_t s)
+                        int64_t n_lists, uint8_t *out, int32_t *first_invalid, int lps, int ops, int64_t waves,
+                        int32_t *unres, int32_t *unres_fb, int32_t *counters, hipStream_t s)
 {
     if (n_lists <= 0) return;
+    fit_link_layout(env, lps, ops);
+    StateListArgs A{env, q_soa, n_states, offsets, n_lists, out, first_invalid, unres, unres_fb, counters};
+    const unsigned g2 = grid_for(waves, 4);
     ProfScope ps("state_lists", s);
-    hipLaunchKernelGGL((state_lists_kernel<8, 1>), dim3(grid_for(n_lists, 4)), dim3(256), lds_bytes(env), s, env,
-                       q_soa, n_states, offsets, n_lists, out, first_invalid);
-    OA_HIP(hipGetLastError());
+    launch_zero(counters, 2, s);
+#define OA_LAUNCH(L, O)                                                                                             \
+    if (lps == L && ops == O) {                                                                                     \
+        const size_t lds = lds_bytes(env, L, O);                                                                    \
+        hipLaunchKernelGGL((state_lists_kernel<L, O>), dim3(grid_for(n_lists, 16)), dim3(1024), lds, s, A);        \
+        hipLaunchKernelGGL((state_list_chunks_kernel<L, O>), dim3(g2), dim3(256), lds, s, A);                      \
+        hipLaunchKernelGGL(state_list_results_kernel, dim3(1), dim3(1024), 0, s, A);                               \
+        OA_HIP(hipGetLastError());                                                                                  \
+        return;                                                                                                     \
+    }
+    OA_SHAPES(OA_LAUNCH)
+#undef OA_LAUNCH
+    fail(OMPL_AMD_ERR_ARG, "state-list shape (%d, %d) not built", lps, ops);
 }
 
 void launch_arm_spheres(const float *q, int64_t n, float *out, hipStream_t s)

@@ -60,8 +60,12 @@ void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsign
 void launch_validate_single(const EnvView &env, const State1 &st, int *result, hipStream_t s);
 void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int ops, int block, int64_t waves,
                               hipStream_t s);
+// lps, ops: the launch shape (the walk's: lowest-invalid exit over a list's
+// states); waves: the persistent grid of the chunk phase; unres / unres_fb:
+// room for n_lists entries each; counters: two ints (cleared by the launch).
 void launch_state_lists(const EnvView &env, const double *q_soa, int64_t n_states, const int64_t *offsets,
-                        int64_t n_lists, uint8_t *out, int32_t *first_invalid, hipStream_t s);
+                        int64_t n_lists, uint8_t *out, int32_t *first_invalid, int lps, int ops, int64_t waves,
+                        int32_t *unres, int32_t *unres_fb, int32_t *counters, hipStream_t s);
 void launch_arm_spheres(const float *q, int64_t n, float *out, hipStream_t s);
 
 } // namespace ompl_amd

@@ -32,13 +32,27 @@ def make_lists(good, bad, seed, n_lists, max_count):
     return np.concatenate(rows) if rows else np.zeros((0, 7)), np.array(offsets, np.int64)
 
 
+# launch shapes of the list walk (the motion walk's tuning fields)
+SHAPES = [(8, 1), (7, 1), (7, 4), (7, 9), (1, 1), (8, 8)]
+
+
+@pytest.fixture
+def walk_shape(gpu, request):
+    saved = gpu.get_tuning()
+    lps, ops = request.param
+    gpu.set_tuning(motion_lanes_per_state=lps, motion_obstacle_parts=ops)
+    yield request.param
+    gpu.set_tuning(**saved)
+
+
+@pytest.mark.parametrize("walk_shape", SHAPES, indirect=True)
 @pytest.mark.parametrize("seed,n_lists,max_count", [(1, 50, 10), (2, 400, 600), (3, 3000, 40)])
-def test_state_lists_match_oracle(gpu, oracle, lists_scene, seed, n_lists, max_count):
+def test_state_lists_match_oracle(gpu, oracle, lists_scene, seed, n_lists, max_count, walk_shape):
     import torch
 
     env, oenv, good, bad = lists_scene
     states, off = make_lists(good, bad, seed, n_lists, max_count)
-    print(f"state lists seed {seed}: {n_lists} lists, {states.shape[0]} states", flush=True)
+    print(f"state lists seed {seed}: {n_lists} lists, {states.shape[0]} states, shape {walk_shape}", flush=True)
     q = torch.from_numpy(states.T.copy()).cuda()
     o = torch.from_numpy(off).cuda()
     valid, first = gpu.check_state_lists(env, q, o)
@@ -65,7 +79,8 @@ def test_state_lists_clamped_offsets(gpu, oracle, lists_scene):
     assert first.cpu().tolist() == [-1, 2, -1]
 
 
-def test_check_states_single_call(gpu, oracle, lists_scene):
+@pytest.mark.parametrize("walk_shape", [(8, 1), (7, 9)], indirect=True)
+def test_check_states_single_call(gpu, oracle, lists_scene, walk_shape):
     env, oenv, good, bad = lists_scene
     rng = np.random.default_rng(5)
     for count in (0, 1, 2, 3, 9, 64, 65, 300):
