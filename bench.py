@@ -401,7 +401,54 @@ def scope_rows(oa, run_cpu, cpu_threads, only=None, sparse_tuning=None):
         state_list_row(oa, rows, oracle_py, timed)
     if want("motion_sparse"):
         motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed, sparse_tuning)
+    if want("last_valid"):
+        last_valid_row(oa, rows, oracle_py, timed)
     return rows
+
+
+def last_valid_row(oa, rows, oracle_py, timed):
+    """DiscreteMotionValidator::checkMotion(s1, s2, lastValid) batched over the
+    16,384 edges of the dense batch (128 + 128 obstacles) and of the sparse
+    batch (16 + 16 obstacles, lognormal(-0.5, 1) lengths), with the launch
+    shape in force."""
+    import torch
+
+    out = {}
+    for name, seed, ns, nb, eseed, mu in (("dense_256_obstacles", ENV_SEED, N_SPHERES, N_BOXES, EDGE_SEED, -1.0),
+                                          ("sparse_32_obstacles", 1002, 16, 16, 9, -0.5)):
+        s, b = oa.workload.obstacles(seed, ns, nb)
+        env = oa.Env(s, b)
+        a_rows, b_rows = oa.workload.edges(eseed, N_EDGES, mu, 1.0, 0.05, 5.0)
+        lvs = lvs_0p01()
+        A = torch.from_numpy(a_rows.T.copy()).cuda()
+        B = torch.from_numpy(b_rows.T.copy()).cuda()
+        ms = timed(lambda: oa.check_motions_last_valid(env, A, B, lvs), 20)
+        v, t = oa.check_motions_last_valid(env, A, B, lvs)
+        v, t = v.cpu().numpy(), t.cpu().numpy()
+        r = {"edges": N_EDGES, "obstacles": ns + nb, "ms": ms, "edge_checks_per_s": N_EDGES / (ms * 1e-3),
+             "valid_fraction": float(v.mean())}
+        if ns + nb < 64:
+            # the scene's own launch shapes (the motion autotuner on the same edges)
+            tuned, _ = env.autotune(A, B, lvs, reps=3)
+            ms_own = timed(lambda: oa.check_motions_last_valid(env, A, B, lvs), 20)
+            v2, t2 = oa.check_motions_last_valid(env, A, B, lvs)
+            r["own_shape"] = {"ms": ms_own, "edge_checks_per_s": N_EDGES / (ms_own * 1e-3), "tuning": tuned,
+                              "same_result": bool(np.array_equal(v2.cpu().numpy(), v)) and
+                              bool(np.array_equal(t2.cpu().numpy(), t, equal_nan=True))}
+        if oracle_py is not None:
+            oenv = oracle_py.OracleEnv(s, b)
+            t0 = time.perf_counter()
+            same = True
+            for i in range(2048):
+                ov, ot, _ = oracle_py.check_motion_last_valid(oenv, a_rows[i], b_rows[i], lvs)
+                same &= bool(v[i]) == bool(ov) and (bool(ov) or t[i] == ot)
+            dt = time.perf_counter() - t0
+            r["cpu_baseline"] = {"edge_checks_per_s": 2048 / dt, "cores": 1, "kind": "port",
+                                 "sample": f"2,048 of the 16,384 edges, {dt:.2f} s"}
+            r["parity_bit_exact_on_sample"] = bool(same)
+        env.close()
+        out[name] = r
+    rows["last_valid"] = out
 
 
 def motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed, own_tuning=None):
@@ -664,7 +711,7 @@ def main(argv=None):
     ap.add_argument("--no-rows", action="store_true", help="skip the per-row measurements")
     ap.add_argument("--rows", type=str, default=None,
                     help="comma-separated rows to measure (validity_64_spheres, knn_l2_100k, se3_1m, sampling, "
-                         "state_lists, motion_sparse; default all)")
+                         "state_lists, motion_sparse, last_valid; default all)")
     ap.add_argument("--sparse-tuning", type=str, default=None,
                     help="JSON dict: the sparse-scene row's own launch shapes (skips its autotuner run)")
     ap.add_argument("--rows-only", action="store_true",

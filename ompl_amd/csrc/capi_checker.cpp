@@ -411,6 +411,7 @@ int ompl_amd_check_motions_last_valid_dev(ompl_amd_env e, const double *a, const
         if (n == 0) return;
         OA_REQUIRE(a && b && out && last_t, OMPL_AMD_ERR_ARG, "null pointer");
         OA_REQUIRE(lvs > 0.0 && factor >= 1, OMPL_AMD_ERR_ARG, "longest valid segment must be > 0 and factor >= 1");
+        OA_REQUIRE(n < (int64_t)1 << 31, OMPL_AMD_ERR_ARG, "at most 2^31-1 edges per call");
         const Tuning t = tuning_for(*e);
         MotionArgs A{};
         A.env = env.view();
@@ -420,8 +421,17 @@ int ompl_amd_check_motions_last_valid_dev(ompl_amd_env e, const double *a, const
         A.lvs = lvs;
         A.factor = factor;
         A.out = out;
-        launch_motion_last_valid(A, last_t, t.motion_lanes_per_state, t.motion_obstacle_parts, t.motion_block, n,
-                                 resolve_stream(stream));
+        const hipStream_t st = resolve_stream(stream);
+        std::unique_lock<std::mutex> lk; // held until the launches are enqueued
+        Workspace &w = workspace(st, (size_t)n, lk);
+        A.list = w.list;
+        A.first_bad = w.first_bad;
+        A.surv_nd = w.surv_nd;
+        A.surv_ab = w.surv_ab;
+        A.list_count = w.counters + 2;
+        A.max_m = w.counters + 3;
+        launch_motion_last_valid(A, last_t, t.motion_lanes_per_state, t.motion_obstacle_parts,
+                                 (int64_t)num_cus() * t.motion_waves_per_cu, st);
     });
 }
 
@@ -521,7 +531,19 @@ int ompl_amd_check_motion_last_valid(ompl_amd_env e, const double *a, const doub
         A.lvs = lvs;
         A.factor = factor;
         A.out = dev_out;
-        launch_motion_last_valid(A, dev_pd + 2 * kDof, 1, 1, 64, 1, r.stream);
+        {
+            std::unique_lock<std::mutex> lk; // held until the launches are enqueued
+            Workspace &w = workspace(r.stream, 1, lk);
+            A.list = w.list;
+            A.first_bad = w.first_bad;
+            A.surv_nd = w.surv_nd;
+            A.surv_ab = w.surv_ab;
+            A.list_count = w.counters + 2;
+            A.max_m = w.counters + 3;
+            const Tuning t = tuning_for(*e);
+            launch_motion_last_valid(A, dev_pd + 2 * kDof, t.motion_lanes_per_state, t.motion_obstacle_parts,
+                                     (int64_t)num_cus() * t.motion_waves_per_cu, r.stream);
+        }
         OA_HIP(hipStreamSynchronize(r.stream));
         *valid = reinterpret_cast<uint8_t *>(r.pinned)[0];
         *last_t = pd[2 * kDof];

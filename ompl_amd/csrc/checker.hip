@@ -697,16 +697,33 @@ __global__ void __launch_bounds__(1024) state_list_chunks_kernel(StateListArgs A
     const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t items = n_un * (maxc - 1);
-    for (int64_t k = wave; k < items; k += waves_total) { // wave-uniform
-        const int64_t c = 1 + k / n_un, u = k - (c - 1) * n_un;
-        int64_t lo, hi;
-        list_bounds(A, A.unres[u], lo, hi);
-        const int64_t base = c * G; // relative to the list's first state
-        if (lo + base >= hi) continue;
-        const int fb = __hip_atomic_load(&A.unres_fb[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (fb < base) continue;
-        const int64_t bad = check_list_chunk<LPS, OPS>(A, lds, lo + base, min(hi, lo + base + G));
-        if (bad >= 0 && lane_id() == 0) atomicMin(&A.unres_fb[u], (int)(bad - lo));
+    const int lane = lane_id();
+    // a wave tests 64 items at once (one per lane: items k0 + lane * W) and
+    // walks the live ones; most items of a chunk-major numbering are past
+    // their list's last state
+    for (int64_t k0 = wave; k0 < items; k0 += 64 * waves_total) {
+        const int64_t k = k0 + lane * waves_total;
+        int u = 0;
+        int64_t lo = 0, hi = 0, base = 0;
+        bool live = false;
+        if (k < items) {
+            const int64_t c = 1 + k / n_un;
+            u = (int)(k - (c - 1) * n_un);
+            list_bounds(A, A.unres[u], lo, hi);
+            base = c * G; // relative to the list's first state
+            live = lo + base < hi &&
+                   __hip_atomic_load(&A.unres_fb[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= base;
+        }
+        uint64_t mask = wave_ballot(live);
+        while (mask) {
+            const int j = lowest_bit(mask);
+            mask &= mask - 1;
+            const int uj = readlane_i32(u, j);
+            const int64_t loj = readlane_i64(lo, j), hij = readlane_i64(hi, j), bj = readlane_i64(base, j);
+            if (__hip_atomic_load(&A.unres_fb[uj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) continue;
+            const int64_t bad = check_list_chunk<LPS, OPS>(A, lds, loj + bj, min(hij, loj + bj + G));
+            if (bad >= 0 && lane == 0) atomicMin(&A.unres_fb[uj], (int)(bad - loj));
+        }
     }
 }
 
@@ -721,64 +738,174 @@ __global__ void __launch_bounds__(1024) state_list_results_kernel(StateListArgs 
     }
 }
 
-// Sequential lastValid overload: positions j = 1..nd-1 in order, then s2.
+// Sequential lastValid overload (DiscreteMotionValidator.cpp:37-93):
+// positions j = 1..nd-1 in order, then s2 as position nd; the lowest invalid
+// position j gives lastValid (j-1)/nd (s2: (nd-1)/nd).  Walked in chunks of G
+// positions like the state lists:
+//   motion_last_valid_kernel: one wave per edge checks positions 1..G; an
+//     edge resolved there (an invalid position, no positions past the chunk,
+//     or too many segments) gets its result, the others are appended (one
+//     device atomic pair per workgroup) with their segment count, endpoints
+//     and first_bad = INT_MAX, and the largest chunk count is kept;
+//   motion_last_valid_chunks_kernel: the unresolved edges' remaining chunks,
+//     chunk-major over a persistent grid, skipping a chunk once an earlier
+//     position of its edge is known invalid (atomicMin);
+//   motion_last_valid_results_kernel: their results.
+// Uses list, first_bad, surv_nd, surv_ab, list_count and max_m (the two
+// counters zero at launch).
+
+// Positions base .. base+G-1 (those <= last; `last` is s2) of one edge;
+// returns the lowest invalid one, or INT_MAX.
+template <int LPS, int OPS>
+__device__ __forceinline__ int check_positions(const MotionArgs &A, const float4 *lds, const double a[kDof],
+                                              const double b[kDof], int nd, int last, int base)
+{
+    using Shape = GroupShape<LPS, OPS>;
+    const Shape sh(lane_id());
+    const int p = base + sh.group;
+    const bool live = p <= last;
+    float q[kDof];
+    if (p >= last) {
+#pragma unroll
+        for (int j = 0; j < kDof; ++j) q[j] = (float)b[j];
+    } else {
+        interpolate_f32<kDof>(a, b, (double)p / (double)nd, q);
+    }
+    LaneSpheres<LPS> S;
+    arm_fk<LPS, Shape::GS>(q, sh.slot, S);
+    const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, A.env.ns, A.env.nb, live, sh.part);
+    const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
+    return H ? base + lowest_bit(H) / Shape::GS : INT_MAX;
+}
+
+__device__ __forceinline__ void last_valid_result(const MotionArgs &A, double *last_t, int64_t e, int bad, int nd,
+                                                  int last)
+{
+    if (bad == INT_MAX) {
+        A.out[e] = 1;
+    } else {
+        A.out[e] = 0;
+        // interior j fails: (j-1)/nd; s2 fails: (nd-1)/nd
+        last_t[e] = bad < last ? (double)(bad - 1) / (double)nd : (double)(nd - 1) / (double)nd;
+    }
+    if (A.checked) A.checked[e] = bad == INT_MAX ? last : bad;
+}
+
 template <int LPS, int OPS>
 __global__ void __launch_bounds__(1024) motion_last_valid_kernel(MotionArgs A, double *__restrict__ last_t)
 {
     extern __shared__ float4 lds[];
-    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
-    using Shape = GroupShape<LPS, OPS>;
-    constexpr int G = Shape::STATES;
+    __shared__ int blk_count, blk_max, blk_base;
+    if (threadIdx.x == 0) {
+        blk_count = 0;
+        blk_max = 0;
+    }
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds); // ends with a barrier
+    constexpr int G = GroupShape<LPS, OPS>::STATES;
     const int lane = lane_id();
-    const Shape sh(lane);
-    const int waves_total = (gridDim.x * blockDim.x) >> 6;
-    for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; e < A.n; e += waves_total) {
-        double a[kDof], b[kDof];
+    const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    double a[kDof], b[kDof];
+    int nd = 0, last = 0;
+    bool pending = false;
+    if (e < A.n) { // wave-uniform
 #pragma unroll
         for (int j = 0; j < kDof; ++j) {
             a[j] = A.a[(int64_t)j * A.n + e];
             b[j] = A.b[(int64_t)j * A.n + e];
         }
-        const int nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
+        nd = valid_segment_count<kDof>(a, b, A.lvs, A.factor);
         if (nd > kMaxSegments) { // reported invalid without a walk
             if (lane == 0) {
                 A.out[e] = 0;
                 if (A.checked) A.checked[e] = 0;
             }
-            continue;
+        } else {
+            last = nd > 1 ? nd : 1;
+            const int bad = check_positions<LPS, OPS>(A, lds, a, b, nd, last, 1);
+            pending = bad == INT_MAX && last > G;
+            if (!pending && lane == 0) last_valid_result(A, last_t, e, bad, nd, last);
         }
-        // positions 1..nd-1 are interpolation indices j, position `last` is s2
-        const int last = nd > 1 ? nd : 1;
-        int bad = INT_MAX;
-        for (int base = 1; base <= last; base += G) {
-            const int p = base + sh.group;
-            const bool live = p <= last;
-            float q[kDof];
-            if (p >= last) {
+    }
+    int woff = 0;
+    if (pending && lane == 0) {
+        woff = atomicAdd(&blk_count, 1);
+        atomicMax(&blk_max, (last - 1) / G + 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && blk_count > 0) {
+        blk_base = atomicAdd(A.list_count, blk_count);
+        atomicMax(A.max_m, blk_max);
+    }
+    __syncthreads();
+    if (pending && lane == 0) {
+        const int slot = blk_base + woff;
+        A.list[slot] = (int32_t)e;
+        A.first_bad[slot] = INT_MAX;
+        A.surv_nd[slot] = nd;
+        double *ab = A.surv_ab + (int64_t)slot * 2 * kDof;
 #pragma unroll
-                for (int j = 0; j < kDof; ++j) q[j] = (float)b[j];
-            } else {
-                interpolate_f32<kDof>(a, b, (double)p / (double)nd, q);
-            }
-            LaneSpheres<LPS> S;
-            arm_fk<LPS, Shape::GS>(q, sh.slot, S);
-            const bool hit = collide_env<LPS, OPS, Exit::kLowestGroup>(S, lds, A.env.ns, A.env.nb, live, sh.part);
-            const uint64_t H = fold_groups<Shape::GS>(wave_ballot(hit && live));
-            if (H) {
-                bad = base + lowest_bit(H) / Shape::GS;
-                break;
-            }
+        for (int j = 0; j < kDof; ++j) {
+            ab[j] = a[j];
+            ab[kDof + j] = b[j];
         }
-        if (lane == 0) {
-            if (bad == INT_MAX) {
-                A.out[e] = 1;
-            } else {
-                A.out[e] = 0;
-                // interior j fails: (j-1)/nd; s2 fails: (nd-1)/nd
-                last_t[e] = bad < last ? (double)(bad - 1) / (double)nd : (double)(nd - 1) / (double)nd;
-            }
-            if (A.checked) A.checked[e] = bad == INT_MAX ? last : bad;
+    }
+}
+
+template <int LPS, int OPS>
+__global__ void __launch_bounds__(1024) motion_last_valid_chunks_kernel(MotionArgs A)
+{
+    extern __shared__ float4 lds[];
+    // the counters are read before the staging, so the reads overlap it
+    int n_un_v = *A.list_count, maxc_v = *A.max_m;
+    stage_env<LPS == 7, link_pass(OPS) * OPS>(A.env, lds);
+    asm volatile("" : "+v"(n_un_v), "+v"(maxc_v));
+    const int64_t n_un = n_un_v, maxc = maxc_v;
+    if (n_un == 0 || maxc < 2) return;
+    constexpr int G = GroupShape<LPS, OPS>::STATES;
+    const int64_t waves_total = (gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t items = n_un * (maxc - 1);
+    const int lane = lane_id();
+    // a wave tests 64 items at once (one per lane: items k0 + lane * W) and
+    // walks the live ones; most items of a chunk-major numbering are past
+    // their edge's last position
+    for (int64_t k0 = wave; k0 < items; k0 += 64 * waves_total) {
+        const int64_t k = k0 + lane * waves_total;
+        int u = 0, nd = 0, base = 0;
+        bool live = false;
+        if (k < items) {
+            const int64_t c = 1 + k / n_un;
+            u = (int)(k - (c - 1) * n_un);
+            nd = A.surv_nd[u]; // >= 2: an unresolved edge has positions past its first chunk
+            base = 1 + (int)(c * G);
+            live = base <= nd &&
+                   __hip_atomic_load(&A.first_bad[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= base;
         }
+        uint64_t mask = wave_ballot(live);
+        while (mask) {
+            const int j = lowest_bit(mask);
+            mask &= mask - 1;
+            const int uj = readlane_i32(u, j), bj = readlane_i32(base, j), nj = readlane_i32(nd, j);
+            if (__hip_atomic_load(&A.first_bad[uj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < bj) continue;
+            const double *ab = A.surv_ab + (int64_t)uj * 2 * kDof;
+            double a[kDof], b[kDof];
+#pragma unroll
+            for (int d = 0; d < kDof; ++d) {
+                a[d] = ab[d];
+                b[d] = ab[kDof + d];
+            }
+            const int bad = check_positions<LPS, OPS>(A, lds, a, b, nj, nj, bj);
+            if (bad != INT_MAX && lane == 0) atomicMin(&A.first_bad[uj], bad);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) motion_last_valid_results_kernel(MotionArgs A, double *__restrict__ last_t)
+{
+    const int n_un = *A.list_count;
+    for (int u = threadIdx.x; u < n_un; u += blockDim.x) {
+        const int nd = A.surv_nd[u];
+        last_valid_result(A, last_t, A.list[u], A.first_bad[u], nd, nd);
     }
 }
 
@@ -1049,15 +1176,19 @@ void launch_validate_single(const EnvView &env, const State1 &st, int *result, h
     OA_HIP(hipGetLastError());
 }
 
-void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int ops, int block, int64_t waves,
-                              hipStream_t s)
+void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int ops, int64_t waves, hipStream_t s)
 {
     if (A.n <= 0) return;
     fit_link_layout(A.env, lps, ops);
-    const unsigned g = grid_for(waves, block / 64);
+    const unsigned g2 = grid_for(waves, 4);
+    launch_zero(A.list_count, 2, s); // list_count, then max_m
 #define OA_LAUNCH(L, O)                                                                                             \
     if (lps == L && ops == O) {                                                                                     \
-        hipLaunchKernelGGL((motion_last_valid_kernel<L, O>), dim3(g), dim3(block), lds_bytes(A.env, L, O), s, A, last_t); \
+        const size_t lds = lds_bytes(A.env, L, O);                                                                  \
+        hipLaunchKernelGGL((motion_last_valid_kernel<L, O>), dim3(grid_for(A.n, 16)), dim3(1024), lds, s, A,      \
+                           last_t);                                                                                 \
+        hipLaunchKernelGGL((motion_last_valid_chunks_kernel<L, O>), dim3(g2), dim3(256), lds, s, A);               \
+        hipLaunchKernelGGL(motion_last_valid_results_kernel, dim3(1), dim3(1024), 0, s, A, last_t);                \
         OA_HIP(hipGetLastError());                                                                                  \
         return;                                                                                                     \
     }

@@ -58,8 +58,10 @@ void launch_motion_fused(const MotionArgs &A, bool exact, int le, int oe, int lw
 void launch_motion_single(const EnvView &env, const Edge1 &E, double lvs, unsigned factor, int *result,
                           hipStream_t s);
 void launch_validate_single(const EnvView &env, const State1 &st, int *result, hipStream_t s);
-void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int ops, int block, int64_t waves,
-                              hipStream_t s);
+// Uses A.list, first_bad, surv_nd, surv_ab (room for A.n entries) and the
+// adjacent counters list_count, max_m (cleared by the launch); waves: the
+// persistent grid of the chunk phase.
+void launch_motion_last_valid(const MotionArgs &A, double *last_t, int lps, int ops, int64_t waves, hipStream_t s);
 // lps, ops: the launch shape (the walk's: lowest-invalid exit over a list's
 // states); waves: the persistent grid of the chunk phase; unres / unres_fb:
 // room for n_lists entries each; counters: two ints (cleared by the launch).

@@ -30,6 +30,14 @@ __device__ __forceinline__ int readlane_i32(int v, int j)
     return __builtin_amdgcn_readlane(v, j);
 }
 
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int j)
+{
+    const unsigned long long u = (unsigned long long)v;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), j);
+    return (int64_t)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int j)
 {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
