@@ -50,6 +50,9 @@ def test_error_path_without_gpu():
     h = C.c_void_p()
     rc = lib.ompl_amd_nn_create(7, 3, C.byref(h))
     assert rc == -1 and b"metric" in lib.ompl_amd_last_error()
+    # an environment's own launch shapes need an environment
+    rc = lib.ompl_amd_env_set_tuning(None, None)
+    assert rc == -1 and b"environment" in lib.ompl_amd_last_error()
 
 
 def test_product_does_not_use_the_oracle():
