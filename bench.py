@@ -1039,6 +1039,11 @@ def main(argv=None):
                      "dispatches": 2 if dom == "motion_chunk" else 1,
                      "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
         "roofline_s2_pass": roof_s2,
+        # the whole step: the batch's reference-order work over the timed step
+        # (both kernels, two streams overlapping)
+        "roofline_step": {"achieved": flops_step / (ms_per_step * 1e-3) / 1e12,
+                          "frac": flops_step / (ms_per_step * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                          "algorithmic_flops_per_step": flops_step, "ms_per_step": ms_per_step},
         "cpu_baseline": cpu,
         "cpu_one_thread": cpu_all,
         "autotuned_us_per_batch": tuned_us,
