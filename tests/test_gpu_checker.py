@@ -522,6 +522,49 @@ def test_capabilities_match_enforced_limits(gpu):
         gpu.Env(np.zeros((w - 1, 4), np.float32), np.zeros((1, 6), np.float32))
 
 
+def test_largest_environment_every_batched_path(gpu, oracle):
+    """An environment at the obstacle limit (64 KiB of obstacles in LDS, beside
+    the kernels' own LDS): validity, motions, lastValid and state lists match
+    the oracle.  Most obstacles are far from the arm, a few near ones decide."""
+    import torch
+
+    w = gpu.capabilities()["max_env_words"]
+    rng = np.random.default_rng(21)
+    s = np.zeros((w - 2, 4), np.float32)
+    s[:, :3] = rng.uniform(20.0, 30.0, (w - 2, 3))
+    s[:, 3] = 0.1
+    near, _ = gpu.workload.obstacles(1004, 12, 0)
+    s[:12] = near
+    b = np.array([[25.0, 25.0, 25.0, 0.5, 0.5, 0.5]], np.float32)
+    env, oenv = gpu.Env(s, b), oracle.OracleEnv(s, b)
+    q = gpu.workload.configs(31, 256)
+    ref = np.array([oracle.state_valid(oenv, r.astype(np.float32)) for r in q])
+    assert 0 < ref.sum() < len(ref)
+    got = gpu.validate_states(env, to_dev(q.T.astype(np.float32).copy())).cpu().numpy()
+    assert np.array_equal(got.astype(bool), ref)
+    a, bb = motion_case(gpu, 32, 128, mu=-1.5)
+    lvs = lvs_config()
+    ref_v, ref_c = oracle.check_motions(oenv, a, bb, lvs)
+    A, B = to_dev(a.T.copy()), to_dev(bb.T.copy())
+    for strategy, e, wsh in ((3, (8, 2), (8, 1)), (1, (8, 1), (1, 1)), (3, (7, 4), (7, 9))):
+        set_strategy(gpu, strategy, e, wsh)
+        v, c = gpu.check_motions(env, A, B, lvs, exact_count=True)
+        assert np.array_equal(v.cpu().numpy(), ref_v) and np.array_equal(c.cpu().numpy(), ref_c), strategy
+        lv, lt = gpu.check_motions_last_valid(env, A, B, lvs)
+        lv, lt = lv.cpu().numpy(), lt.cpu().numpy()
+        for i in range(0, 128, 9):
+            ov, ot, _ = oracle.check_motion_last_valid(oenv, a[i], bb[i], lvs)
+            assert lv[i] == ov and (ov or lt[i] == ot), (strategy, i)
+        states = np.concatenate([q[:40], q[40:80]])
+        off = np.array([0, 40, 80], np.int64)
+        sv, sf = gpu.check_state_lists(env, to_dev(states.T.copy()), torch.from_numpy(off).cuda())
+        for i in range(2):
+            ok, f = oracle.check_states_first_invalid(oenv, states[off[i]:off[i + 1]])
+            assert bool(sv[i].item()) == ok and int(sf[i].item()) == (-1 if ok else f)
+    gpu.set_tuning(**DEFAULT)
+    env.close()
+
+
 def test_counter_sets_across_mixed_calls(gpu, oracle):
     """Batches with and without checked counts, captured-graph replays, a batch
     with no survivors and batches of other sizes, in one sequence on one stream
