@@ -398,15 +398,37 @@ def scope_rows(oa, run_cpu, cpu_threads, only=None, sparse_tuning=None):
     if want("sampling"):
         sampling_rows(oa, rows, oracle_py, timed, sample_uniform, sample_valid)
     if want("state_lists"):
-        state_list_row(oa, rows, oracle_py, timed)
+        state_list_row(oa, rows, oracle_py, timed, roof)
     if want("motion_sparse"):
         motion_sparse_row(oa, rows, oracle_py, cpu_threads, roof, timed, sparse_tuning)
     if want("last_valid"):
-        last_valid_row(oa, rows, oracle_py, timed)
+        last_valid_row(oa, rows, oracle_py, timed, roof)
     return rows
 
 
-def last_valid_row(oa, rows, oracle_py, timed):
+def sequential_motion_states(a_rows, b_rows, lvs, valid, last_t):
+    """The states DiscreteMotionValidator::checkMotion(s1, s2, lastValid)
+    checks, in its order (positions 1..nd-1, then s2 as position nd), up to the
+    first invalid one (every position of a valid edge): FP32 [7, n], for the
+    roofline's reference-order work.  nd and the interpolation are the
+    library's (FP64, then FP32)."""
+    diff = a_rows - b_rows
+    d = np.sqrt(np.sum(diff * diff, axis=1))
+    nd = np.ceil(d / lvs).astype(np.int64)
+    last = np.maximum(nd, 1)
+    # lastValid (j-1)/nd for the first invalid position j (s2: (nd-1)/nd)
+    j = np.where(valid == 1, last, np.rint(np.nan_to_num(last_t) * nd).astype(np.int64) + 1)
+    j = np.minimum(j, last)
+    edge = np.repeat(np.arange(len(a_rows)), j)
+    pos = np.arange(int(j.sum())) - np.repeat(np.cumsum(j) - j, j) + 1
+    t = pos / np.maximum(nd[edge], 1)
+    st = a_rows[edge] + (b_rows[edge] - a_rows[edge]) * t[:, None]
+    s2 = pos >= last[edge]
+    st[s2] = b_rows[edge[s2]]
+    return st.astype(np.float32).T.copy()
+
+
+def last_valid_row(oa, rows, oracle_py, timed, roof):
     """DiscreteMotionValidator::checkMotion(s1, s2, lastValid) batched over the
     16,384 edges of the dense batch (128 + 128 obstacles) and of the sparse
     batch (16 + 16 obstacles, lognormal(-0.5, 1) lengths), with the launch
@@ -425,14 +447,17 @@ def last_valid_row(oa, rows, oracle_py, timed):
         ms = timed(lambda: oa.check_motions_last_valid(env, A, B, lvs), 20)
         v, t = oa.check_motions_last_valid(env, A, B, lvs)
         v, t = v.cpu().numpy(), t.cpu().numpy()
+        w = oa.validity_work(env, torch.from_numpy(sequential_motion_states(a_rows, b_rows, lvs, v, t)).cuda())
         r = {"edges": N_EDGES, "obstacles": ns + nb, "ms": ms, "edge_checks_per_s": N_EDGES / (ms * 1e-3),
-             "valid_fraction": float(v.mean())}
+             "valid_fraction": float(v.mean()), "states_checked": w["states"],
+             "roofline": roof(oa.work_flops(w), ms)}
         if ns + nb < 64:
             # the scene's own launch shapes (the motion autotuner on the same edges)
             tuned, _ = env.autotune(A, B, lvs, reps=3)
             ms_own = timed(lambda: oa.check_motions_last_valid(env, A, B, lvs), 20)
             v2, t2 = oa.check_motions_last_valid(env, A, B, lvs)
             r["own_shape"] = {"ms": ms_own, "edge_checks_per_s": N_EDGES / (ms_own * 1e-3), "tuning": tuned,
+                              "roofline": roof(oa.work_flops(w), ms_own),
                               "same_result": bool(np.array_equal(v2.cpu().numpy(), v)) and
                               bool(np.array_equal(t2.cpu().numpy(), t, equal_nan=True))}
         if oracle_py is not None:
@@ -656,7 +681,7 @@ def sampling_rows(oa, rows, oracle_py, timed, sample_uniform, sample_valid):
                                                 "sample": f"8 of the 256 samplers, {dt:.2f} s"}
 
 
-def state_list_row(oa, rows, oracle_py, timed):
+def state_list_row(oa, rows, oracle_py, timed, roof):
     import torch
 
     # precomputed state lists: the motion states of the 16,384 benchmark edges, first 64 each
@@ -669,8 +694,15 @@ def state_list_row(oa, rows, oracle_py, timed):
     offs = torch.arange(0, 4097 * 64, 64, dtype=torch.int64, device="cuda")
     ms = timed(lambda: oa.check_state_lists(env2, std, offs), 20)
     v, f = oa.check_state_lists(env2, std, offs)
+    # reference-order work of the sequential overload: each list's states up
+    # to its first invalid one
+    fh = f.cpu().numpy()
+    cnt = np.where(fh >= 0, fh + 1, 64)
+    idx = np.repeat(np.arange(4096) * 64, cnt) + (np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    w = oa.validity_work(env2, torch.from_numpy(st[idx].astype(np.float32).T.copy()).cuda())
     rows["state_lists"] = {"lists": 4096, "states_per_list": 64, "ms": ms, "lists_per_s": 4096 / (ms * 1e-3),
-                           "valid_fraction": float(v.float().mean().item())}
+                           "valid_fraction": float(v.float().mean().item()), "states_checked": w["states"],
+                           "roofline": roof(oa.work_flops(w), ms)}
     if oracle_py is not None:
         oenv2 = oracle_py.OracleEnv(s2, b2)
         t0 = time.perf_counter()
