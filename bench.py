@@ -916,6 +916,26 @@ def main(argv=None):
     graph2_ms = ev0.elapsed_time(ev1) / (2 * reps)
     graph2_same = all(bool(np.array_equal(o.cpu().numpy(), valid)) for o in g2.out)
     g2.close()
+    # two graphs (each its own workspace and output) replayed alternately on
+    # two streams, as the eager two-stream pipeline issues its batches
+    gs = [oa.MotionGraph(env, A, B, lvs) for _ in range(2)]
+    side = torch.cuda.Stream()  # default priority
+    handles = (stream.cuda_stream, side.cuda_stream)
+    for _ in range(3):
+        gs[0].launch(handles[0])
+        gs[1].launch(handles[1])
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    side.wait_event(ev0)
+    for i in range(args.steps):
+        gs[i % 2].launch(handles[i % 2])
+    stream.wait_stream(side)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    graph_2s_ms = ev0.elapsed_time(ev1) / args.steps
+    graph_2s_same = all(bool(np.array_equal(g_.out.cpu().numpy(), valid)) for g_ in gs)
+    for g_ in gs:
+        g_.close()
 
     # ---- per-kernel durations: HIP events recorded by the library on the
     #      launch stream around every launch of the same K steps
@@ -1056,6 +1076,9 @@ def main(argv=None):
         "graph": {"ms_per_step": graph_ms, "edge_checks_per_s": N_EDGES / (graph_ms * 1e-3),
                   "step_latency_ms": {"p50": percentile(graph_lat, 50), "p99": percentile(graph_lat, 99)},
                   "same_result": graph_same,
+                  "two_streams": {"ms_per_step": graph_2s_ms, "edge_checks_per_s": N_EDGES / (graph_2s_ms * 1e-3),
+                                  "same_result": graph_2s_same,
+                                  "note": "two graphs replayed alternately on two streams; time per batch"},
                   "two_batch_group": {"ms_per_step": graph2_ms, "edge_checks_per_s": N_EDGES / (graph2_ms * 1e-3),
                                       "same_result": graph2_same,
                                       "note": "two independent batches on forked streams in one graph; "
