@@ -63,6 +63,32 @@ def test_noalloc_matches_allocating_entry_point(gpu, strategy):
                        motion_lanes_per_state=8, motion_obstacle_parts=1)
 
 
+def test_noalloc_with_environment_shapes_many_survivors(gpu):
+    """A sparse scene with its own launch shapes (about 10,000 survivors, the
+    walk's chunk-major-only form): the fixed-array entry point uses the
+    environment's shapes and matches the allocating one, twice in a row."""
+    import torch
+
+    s, b = gpu.workload.obstacles(1002, 16, 16)
+    env = gpu.Env(s, b)
+    env.set_tuning(motion_strategy=3, validity_lanes_per_state=4, validity_obstacle_parts=1,
+                   motion_lanes_per_state=7, motion_obstacle_parts=1, motion_waves_per_cu=32)
+    lvs = lvs_config()
+    for n, seed in ((16384, 9), (5000, 10)):
+        a, bb = gpu.workload.edges(seed, n, -0.5, 1.0, 0.05, 5.0)
+        A = torch.from_numpy(a.T.copy()).cuda()
+        B = torch.from_numpy(bb.T.copy()).cuda()
+        print(f"noalloc environment shapes n {n}", flush=True)
+        ref_v, ref_c = gpu.check_motions(env, A, B, lvs, exact_count=True)
+        ref_v, ref_c = ref_v.cpu().numpy(), ref_c.cpu().numpy()
+        v, c = noalloc(gpu, env, A, B, lvs, exact=True)
+        assert np.array_equal(v, ref_v) and np.array_equal(c, ref_c), n
+        v, _ = noalloc(gpu, env, A, B, lvs, exact=False)
+        assert np.array_equal(v, ref_v), n
+    assert int(ref_v.sum()) > 0
+    env.close()
+
+
 def test_noalloc_rejects_batches_above_its_array(gpu):
     import torch
 
