@@ -1030,6 +1030,13 @@ def main(argv=None):
         stream_res["served_qps_sweep"] = {
             str(int(q)): served_stream(oa, args.stream_requests, q, args.stream_envs, 0, run_plain=False)["served"]
             for q in (1000.0, 5000.0, 20000.0)}
+        # saturated rates: the served path at its highest offered rate against
+        # the plain path, which the 200 QPS stream already saturates (its
+        # request rate falls below the offered one)
+        plain = stream_res.get("plain")
+        if plain is not None and plain["throughput_rps"] < 0.9 * args.stream_qps:
+            top = max(v["edge_checks_per_s"] for v in stream_res["served_qps_sweep"].values())
+            stream_res["saturated_edge_checks_served_over_plain"] = top / plain["edge_checks_per_s"]
         # server settings: obstacle partitions of the motion check (0 = by motion length) and workgroup size;
         # HIP events around each slice launch
         stream_res["served_config_sweep"] = {
