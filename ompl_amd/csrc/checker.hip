@@ -331,9 +331,14 @@ __device__ __forceinline__ size_t env_lds_bytes(const EnvView &e)
 
 // off[0] = 0, off[i + 1] = sum over survivors t <= i of their chunks past
 // the first L0, max(0, ceil((nd_t - 1) / G) - L0), counting only survivors
-// with no invalid position found yet; computed by the whole block (each
-// thread a contiguous run of survivors, then a scan of the run totals by the
-// first wave).
+// with no invalid position in those first L0 chunks (positions 1 .. L0 G, the
+// ones phase 1 walked); computed by the whole block (each thread a contiguous
+// run of survivors, then a scan of the run totals by the first wave).  Every
+// block of phase 2 scans for itself while others already walk and lower
+// first_bad, so the test must not see phase 2's own results: those are
+// positions past L0 G, which the test ignores, so every block computes the
+// same offsets (a chunk behind a phase-2 invalid position is skipped when it
+// is walked).
 template <int G>
 __device__ __forceinline__ void scan_remaining_chunks(const int32_t *__restrict__ nd,
                                                       const int32_t *__restrict__ first_bad, int n, int64_t L0,
@@ -346,7 +351,7 @@ __device__ __forceinline__ void scan_remaining_chunks(const int32_t *__restrict_
     int sum = 0;
     for (int i = lo; i < hi; ++i) {
         const int64_t chunks = (nd[i] - 1 + G - 1) / G;
-        const int len = first_bad[i] == INT_MAX ? (int)max((int64_t)0, chunks - L0) : 0;
+        const int len = first_bad[i] > L0 * G ? (int)max((int64_t)0, chunks - L0) : 0;
         off[i + 1] = len;
         sum += len;
     }

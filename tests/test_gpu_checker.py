@@ -316,6 +316,33 @@ def test_check_motions_many_survivors(gpu, oracle, strategy, eshape, wshape, wpc
     gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8)
 
 
+@pytest.mark.parametrize("vblock,mblock,wpc,wshape", [(64, 64, 32, (7, 1)), (256, 1024, 32, (7, 1)),
+                                                      (1024, 64, 32, (7, 1)), (256, 256, 32, (8, 1)),
+                                                      (256, 256, 16, (7, 2))])
+def test_edge_major_walk_with_more_waves_than_resident(gpu, oracle, vblock, mblock, wpc, wshape):
+    """About 3,000 survivors (the walk's edge-major second phase) over a grid
+    with more waves than the CUs hold at once: blocks of phase 2 that start
+    late, after others have found invalid positions, must deal the same chunks
+    as the early ones (no chunk skipped)."""
+    print(f"edge-major walk blocks {vblock}/{mblock} wpc {wpc} shape {wshape}", flush=True)
+    gpu.set_tuning(motion_strategy=3, validity_lanes_per_state=4, validity_obstacle_parts=1,
+                   motion_lanes_per_state=wshape[0], motion_obstacle_parts=wshape[1], motion_waves_per_cu=wpc,
+                   validity_block=vblock, motion_block=mblock)
+    try:
+        env, oenv, _ = env_pair(gpu, oracle, 1002, 16, 16)
+        lvs = lvs_config()
+        for seed in (10, 11):
+            a, b = motion_case(gpu, seed, 5000, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
+            ref_v, ref_c = oracle.check_motions(oenv, a, b, lvs, threads=8)
+            A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
+            for _ in range(2):
+                v, c = gpu.check_motions(env, A, B, lvs, exact_count=True)
+                assert np.array_equal(v.cpu().numpy(), ref_v) and np.array_equal(c.cpu().numpy(), ref_c), seed
+                assert np.array_equal(gpu.check_motions(env, A, B, lvs).cpu().numpy(), ref_v), seed
+    finally:
+        gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8, validity_block=256, motion_block=256)
+
+
 def test_env_tuning_overrides_library_tuning(gpu, oracle):
     """An environment's own launch shapes take the place of the library's for
     its batches only, and clearing them restores the library's."""
