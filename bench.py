@@ -244,7 +244,7 @@ def prm_batch(oa, n_samples, cpu_samples, cpu_threads, run_cpu):
         env = oa.Env(s, b)
         # warm the pipeline and tune the motion launch shape on this roadmap's candidate edges
         build_roadmap(env, PRM_SEED, n_samples, n_streams=256, fetch=False, autotune=True)
-        tuning = oa.get_tuning()
+        tuning = {k: v for k, v in env.tuning().items() if k != "own"}  # the environment's own shapes
         reps = []
         for _ in range(3):
             t0 = time.perf_counter()

@@ -515,7 +515,8 @@ typedef struct {
 
 /* flags of ompl_amd_prm_build */
 #define OMPL_AMD_PRM_AUTOTUNE 1u /* run the motion autotuner on (up to 32,768 of) the candidate edges first and
-                                    install the winner, as ompl_amd_autotune_motions */
+                                    keep the winner as the environment's own launch shapes
+                                    (ompl_amd_env_set_tuning); the library's tuning is left as it was */
 #define OMPL_AMD_PRM_KSTAR 2u    /* KStarStrategy (geometric/planners/prm/ConnectionStrategy.h:125-157, PRM*):
                                     k = ceil((e + e/7) * ln(milestones)), capped at 63; the k argument is
                                     ignored */

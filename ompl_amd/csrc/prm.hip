@@ -339,7 +339,13 @@ void build(PrmRoadmap &R, ompl_amd_env env, uint32_t seed, int n_streams, int64_
         hipLaunchKernelGGL(edge_endpoints_kernel, dim3(blocks_for(nt)), dim3(256), 0, s, R.keys, nt, R.nodes, teu,
                            tev, ta, tb);
         OA_HIP(hipStreamSynchronize(s));
-        check(ompl_amd_autotune_motions(env, ta, tb, nt, lvs, factor, 3, nullptr, nullptr));
+        // the winner becomes the environment's own launch shapes; the
+        // library's tuning (other environments') is left as it was
+        ompl_amd_tuning saved, best;
+        check(ompl_amd_get_tuning(&saved));
+        check(ompl_amd_autotune_motions(env, ta, tb, nt, lvs, factor, 3, &best, nullptr));
+        check(ompl_amd_set_tuning(&saved));
+        check(ompl_amd_env_set_tuning(env, &best));
     }
     // 5. motion checks
     t0 = Clock::now();

@@ -57,7 +57,8 @@ def build_roadmap(env, seed: int, n_samples: int, k: int = DEFAULT_NEAREST_NEIGH
     """Builds the roadmap on the device.  lvs is the longest valid segment in
     radians (the motion resolution); samples come from n_streams generators
     seeded seed, seed + 1, ...  autotune runs the motion autotuner on the
-    candidate edges first (the winner stays installed).  kstar uses PRM*'s
+    candidate edges first (the winner becomes the environment's own launch
+    shapes; the library's tuning is left as it was).  kstar uses PRM*'s
     KStarStrategy, k = ceil((e + e/7) ln(milestones)) (ConnectionStrategy.h:
     125-157), instead of k.  With fetch=False only the statistics are
     returned."""

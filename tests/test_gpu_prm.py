@@ -107,6 +107,23 @@ def test_prm_components_against_scipy(gpu):
     assert keys == set(map(tuple, g.edges.tolist()))
 
 
+def test_prm_autotune_keeps_winner_on_environment(gpu, oracle):
+    """The autotune flag tunes the motion checks on the roadmap's candidate
+    edges and keeps the winner as the environment's own shapes: the library's
+    tuning is unchanged, and the roadmap still equals the oracle's."""
+    from ompl_amd.prm import build_roadmap
+
+    s, b = gpu.workload.obstacles(1000, 24, 24)
+    env = gpu.Env(s, b)
+    before = gpu.get_tuning()
+    g = build_roadmap(env, 5, 3000, k=10, lvs=0.01, n_streams=16, autotune=True)
+    assert gpu.get_tuning() == before
+    assert env.tuning()["own"] is True
+    o = oracle.prm_batch(oracle.OracleEnv(s, b), 5, 3000, 10, 0.01, 1, 16, threads=8)
+    assert_same(g, o)
+    env.close()
+
+
 def test_prm_argument_errors(gpu):
     from ompl_amd._lib import OmplAmdError
     from ompl_amd.prm import build_roadmap
