@@ -10,9 +10,9 @@ exit.  Inputs are resident in HBM before the timed region.
 
 Reported (one JSON line):
   value            batched edge-checks/s (edges validated per second), the K
-                   steps issued round-robin on --streams HIP streams (2):
-                   independent batches, each fully checked; the one-stream
-                   figure is reported beside it
+                   steps issued round-robin on --streams HIP streams (4):
+                   independent batches, each fully checked; the one- and
+                   two-stream figures are reported beside it
   plain            the same edges through the one-launch-per-call path
   roofline         dominant kernel: algorithmic FP32 work in the reference order
                    / its average duration (HIP events on its launch stream)
@@ -84,6 +84,54 @@ def cpu_threads_default():
 
 def percentile(xs, p):
     return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
+
+
+def concurrent_streams(oa, torch, env, A, B, lvs, want, candidates=12, k=24):
+    """Up to `want` HIP streams (default priority) whose batches overlap.
+
+    HIP spreads a process's streams over a few hardware queues
+    (GPU_MAX_HW_QUEUES, 4 unless set) by a least-used rule whose ties the
+    program cannot see, and has no call that names a stream's queue; two
+    streams on one queue run their kernels one after the other.  A pair is
+    taken to overlap when k steps issued alternately on the two finish in
+    under 0.8x the time of k steps on one (about 0.57x when they overlap,
+    1.0x when they share a queue).  Candidates are kept greedily while each
+    overlaps with every stream kept so far.  Returns (streams, pair checks)."""
+    null = torch.cuda.current_stream()
+    pool = [torch.cuda.Stream() for _ in range(candidates)]
+    outs = [torch.empty(A.shape[1], dtype=torch.uint8, device=A.device) for _ in range(2)]
+    for s in pool:  # each stream's survivor workspace is allocated on its first call
+        oa.check_motions(env, A, B, lvs, out=outs[0], stream=s.cuda_stream)
+    torch.cuda.synchronize()
+
+    def timed(streams):
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(null)
+        for s in streams:
+            s.wait_event(ev0)
+        for i in range(k):
+            j = i % len(streams)
+            oa.check_motions(env, A, B, lvs, out=outs[j], stream=streams[j].cuda_stream)
+        for s in streams:
+            null.wait_stream(s)
+        ev1.record(null)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1)
+
+    one = min(timed([pool[0]]) for _ in range(2))
+    kept, checks = [pool[0]], 0
+    for c in pool[1:]:
+        if len(kept) == want:
+            break
+        ok = True
+        for s in kept:
+            checks += 1
+            if timed([s, c]) >= 0.8 * one:
+                ok = False
+                break
+        if ok:
+            kept.append(c)
+    return kept, checks
 
 
 def served_stream(oa, n_requests, qps, n_envs, plain_workers, seed=8, max_iters=2000, log_path=None,
@@ -723,8 +771,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--streams", type=int, default=2,
-                    help="HIP streams the timed steps are issued on, round-robin (independent batches)")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="HIP streams the timed steps are issued on, round-robin (independent batches); streams "
+                         "whose batches do not overlap (one hardware queue) are left out")
     ap.add_argument("--plain-edges", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -806,30 +855,27 @@ def main(argv=None):
         oa.check_motions(env, A, B, lvs, out=out)
     torch.cuda.synchronize()
 
+    # ---- the streams the timed steps are issued on: ones whose batches overlap
+    picked, pair_checks = concurrent_streams(oa, torch, env, A, B, lvs, max(1, args.streams))
+
     # ---- timed region: K batched steps, issued round-robin on S HIP streams
     #      (independent batches: each stream has its own output and survivor
     #      workspace), between events on the launch stream that waits for all
     def run_steps(n_streams, k):
-        side = [torch.cuda.Stream() for _ in range(n_streams - 1)]  # default priority
-        outs = [out] + [torch.empty_like(out) for _ in side]
+        use = picked[:n_streams]
+        handles = [s.cuda_stream for s in use]
+        outs = [out] + [torch.empty_like(out) for _ in use[1:]]
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        for j, sd in enumerate(side):  # warm the side streams' workspaces
-            with torch.cuda.stream(sd):
-                oa.check_motions(env, A, B, lvs, out=outs[1 + j])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record(stream)
-        for sd in side:
-            sd.wait_event(ev0)
+        for s in use:
+            s.wait_event(ev0)
         for i in range(k):
-            j = i % n_streams
-            if j == 0:
-                oa.check_motions(env, A, B, lvs, out=outs[0])
-            else:
-                with torch.cuda.stream(side[j - 1]):
-                    oa.check_motions(env, A, B, lvs, out=outs[j])
-        for sd in side:
-            stream.wait_stream(sd)
+            j = i % len(use)
+            oa.check_motions(env, A, B, lvs, out=outs[j], stream=handles[j])
+        for s in use:
+            stream.wait_stream(s)
         ev1.record(stream)
         torch.cuda.synchronize()
         wall_s = time.perf_counter() - t0
@@ -838,11 +884,12 @@ def main(argv=None):
 
     step_ms = []
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    gpu_ms, wall, streams_same = run_steps(args.streams, args.steps)
+    gpu_ms, wall, streams_same = run_steps(len(picked), args.steps)
     ms_per_step = gpu_ms / args.steps
     value = N_EDGES * args.steps / (gpu_ms / 1e3)
     valid = out.cpu().numpy().copy()
-    single_ms, _, _ = run_steps(1, args.steps) if args.streams > 1 else (gpu_ms, wall, True)
+    single_ms, _, _ = run_steps(1, args.steps) if len(picked) > 1 else (gpu_ms, wall, True)
+    pair_ms, _, _ = run_steps(2, args.steps) if len(picked) > 2 else (gpu_ms, wall, True)
 
     # ---- per-step latency (synchronous steps) for p50/p99
     for _ in range(min(args.steps, 200)):
@@ -919,17 +966,19 @@ def main(argv=None):
     # two graphs (each its own workspace and output) replayed alternately on
     # two streams, as the eager two-stream pipeline issues its batches
     gs = [oa.MotionGraph(env, A, B, lvs) for _ in range(2)]
-    side = torch.cuda.Stream()  # default priority
-    handles = (stream.cuda_stream, side.cuda_stream)
+    pair = (picked * 2)[:2]
+    handles = tuple(s.cuda_stream for s in pair)
     for _ in range(3):
         gs[0].launch(handles[0])
         gs[1].launch(handles[1])
     torch.cuda.synchronize()
     ev0.record(stream)
-    side.wait_event(ev0)
+    for s in pair:
+        s.wait_event(ev0)
     for i in range(args.steps):
         gs[i % 2].launch(handles[i % 2])
-    stream.wait_stream(side)
+    for s in pair:
+        stream.wait_stream(s)
     ev1.record(stream)
     torch.cuda.synchronize()
     graph_2s_ms = ev0.elapsed_time(ev1) / args.steps
@@ -1072,9 +1121,13 @@ def main(argv=None):
         "data": "synthetic (seeded std::mt19937 workload)",
         "config": {"workload": "uneven motion validation: 7-DOF arm, 128 spheres + 128 boxes, 16384 edges "
                                "lognormal(-1,1) length, 0.01 rad, bisection + first-collision exit",
-                   "edges": N_EDGES, "obstacles": N_SPHERES + N_BOXES, "tuning": t, "streams": args.streams},
+                   "edges": N_EDGES, "obstacles": N_SPHERES + N_BOXES, "tuning": t, "streams": len(picked)},
+        "streams_asked": args.streams,
+        "stream_pair_checks": pair_checks,
         "one_stream": {"edge_checks_per_s": N_EDGES * args.steps / (single_ms / 1e3),
                        "ms_per_step": single_ms / args.steps},
+        "two_streams": {"edge_checks_per_s": N_EDGES * args.steps / (pair_ms / 1e3),
+                        "ms_per_step": pair_ms / args.steps},
         "streams_same_result": streams_same,
         "kernels": kernels,
         "valid_fraction": float(valid.mean()),
@@ -1102,7 +1155,7 @@ def main(argv=None):
                      "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
         "roofline_s2_pass": roof_s2,
         # the whole step: the batch's reference-order work over the timed step
-        # (both kernels, two streams overlapping)
+        # (both kernels, the streams overlapping)
         "roofline_step": {"achieved": flops_step / (ms_per_step * 1e-3) / 1e12,
                           "frac": flops_step / (ms_per_step * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
                           "algorithmic_flops_per_step": flops_step, "ms_per_step": ms_per_step},
