@@ -850,33 +850,31 @@ def main(argv=None):
         _, tuned_us = oa.autotune_motions(env, A, B, lvs, reps=20)
     t = oa.get_tuning()
 
-    # ---- warmup
-    for _ in range(args.warmup):
-        oa.check_motions(env, A, B, lvs, out=out)
-    torch.cuda.synchronize()
-
     # ---- the streams the timed steps are issued on: ones whose batches overlap
     picked, pair_checks = concurrent_streams(oa, torch, env, A, B, lvs, max(1, args.streams))
 
-    # ---- timed region: K batched steps, issued round-robin on S HIP streams
-    #      (independent batches: each stream has its own output and survivor
-    #      workspace), between events on the launch stream that waits for all
-    def run_steps(n_streams, k):
+    # ---- W untimed warmup steps, then the timed region: K batched steps,
+    #      issued round-robin on S HIP streams (independent batches: each
+    #      stream has its own output and survivor workspace), between events
+    #      on the first stream, which waits for the others at the end
+    def run_steps(n_streams, k, warm):
         use = picked[:n_streams]
         handles = [s.cuda_stream for s in use]
         outs = [out] + [torch.empty_like(out) for _ in use[1:]]
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for i in range(warm):
+            oa.check_motions(env, A, B, lvs, out=outs[i % len(use)], stream=handles[i % len(use)])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ev0.record(stream)
-        for s in use:
+        ev0.record(use[0])
+        for s in use[1:]:
             s.wait_event(ev0)
         for i in range(k):
             j = i % len(use)
             oa.check_motions(env, A, B, lvs, out=outs[j], stream=handles[j])
-        for s in use:
-            stream.wait_stream(s)
-        ev1.record(stream)
+        for s in use[1:]:
+            use[0].wait_stream(s)
+        ev1.record(use[0])
         torch.cuda.synchronize()
         wall_s = time.perf_counter() - t0
         same = all(np.array_equal(o.cpu().numpy(), outs[0].cpu().numpy()) for o in outs[1:])
@@ -884,12 +882,12 @@ def main(argv=None):
 
     step_ms = []
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    gpu_ms, wall, streams_same = run_steps(len(picked), args.steps)
+    gpu_ms, wall, streams_same = run_steps(len(picked), args.steps, args.warmup)
     ms_per_step = gpu_ms / args.steps
     value = N_EDGES * args.steps / (gpu_ms / 1e3)
     valid = out.cpu().numpy().copy()
-    single_ms, _, _ = run_steps(1, args.steps) if len(picked) > 1 else (gpu_ms, wall, True)
-    pair_ms, _, _ = run_steps(2, args.steps) if len(picked) > 2 else (gpu_ms, wall, True)
+    single_ms, _, _ = run_steps(1, args.steps, args.warmup) if len(picked) > 1 else (gpu_ms, wall, True)
+    pair_ms, _, _ = run_steps(2, args.steps, args.warmup) if len(picked) > 2 else (gpu_ms, wall, True)
 
     # ---- per-step latency (synchronous steps) for p50/p99
     for _ in range(min(args.steps, 200)):
