@@ -961,28 +961,33 @@ def main(argv=None):
     graph2_ms = ev0.elapsed_time(ev1) / (2 * reps)
     graph2_same = all(bool(np.array_equal(o.cpu().numpy(), valid)) for o in g2.out)
     g2.close()
-    # two graphs (each its own workspace and output) replayed alternately on
-    # two streams, as the eager two-stream pipeline issues its batches
-    gs = [oa.MotionGraph(env, A, B, lvs) for _ in range(2)]
-    pair = (picked * 2)[:2]
-    handles = tuple(s.cuda_stream for s in pair)
-    for _ in range(3):
-        gs[0].launch(handles[0])
-        gs[1].launch(handles[1])
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for s in pair:
-        s.wait_event(ev0)
-    for i in range(args.steps):
-        gs[i % 2].launch(handles[i % 2])
-    for s in pair:
-        stream.wait_stream(s)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    graph_2s_ms = ev0.elapsed_time(ev1) / args.steps
-    graph_2s_same = all(bool(np.array_equal(g_.out.cpu().numpy(), valid)) for g_ in gs)
-    for g_ in gs:
-        g_.close()
+    # graphs (each its own workspace and output) replayed round-robin on the
+    # picked streams, as the eager pipeline issues its batches (time per batch)
+    def graphs_on_streams(n_streams):
+        use = (picked * n_streams)[:n_streams]
+        gs = [oa.MotionGraph(env, A, B, lvs) for _ in use]
+        handles = [s.cuda_stream for s in use]
+        for _ in range(3):
+            for g_, h in zip(gs, handles):
+                g_.launch(h)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(use[0])
+        for s in use[1:]:
+            s.wait_event(e0)
+        for i in range(args.steps):
+            gs[i % len(gs)].launch(handles[i % len(gs)])
+        for s in use[1:]:
+            use[0].wait_stream(s)
+        e1.record(use[0])
+        torch.cuda.synchronize()
+        same = all(bool(np.array_equal(g_.out.cpu().numpy(), valid)) for g_ in gs)
+        for g_ in gs:
+            g_.close()
+        return e0.elapsed_time(e1) / args.steps, same
+
+    graph_2s_ms, graph_2s_same = graphs_on_streams(2)
+    graph_4s_ms, graph_4s_same = graphs_on_streams(len(picked))
 
     # ---- per-kernel durations: HIP events recorded by the library on the
     #      launch stream around every launch of the same K steps
@@ -1137,6 +1142,10 @@ def main(argv=None):
                   "two_streams": {"ms_per_step": graph_2s_ms, "edge_checks_per_s": N_EDGES / (graph_2s_ms * 1e-3),
                                   "same_result": graph_2s_same,
                                   "note": "two graphs replayed alternately on two streams; time per batch"},
+                  "all_streams": {"streams": len(picked), "ms_per_step": graph_4s_ms,
+                                  "edge_checks_per_s": N_EDGES / (graph_4s_ms * 1e-3), "same_result": graph_4s_same,
+                                  "note": "one graph per stream, replayed round-robin on the streams the timed "
+                                          "steps use; time per batch"},
                   "two_batch_group": {"ms_per_step": graph2_ms, "edge_checks_per_s": N_EDGES / (graph2_ms * 1e-3),
                                       "same_result": graph2_same,
                                       "note": "two independent batches on forked streams in one graph; "
