@@ -852,6 +852,13 @@ def main(argv=None):
 
     # ---- the streams the timed steps are issued on: ones whose batches overlap
     picked, pair_checks = concurrent_streams(oa, torch, env, A, B, lvs, max(1, args.streams))
+    # ---- the launch shape for batches on those streams (the one-batch-at-a-time
+    #      shape above keeps more persistent walk waves than is best when the
+    #      streams' batches share the device)
+    t_one = t
+    streams_us = None
+    if not args.tuning and len(picked) > 1:
+        t, streams_us = oa.autotune_motions_streams(env, A, B, lvs, picked)
 
     # ---- W untimed warmup steps, then the timed region: K batched steps,
     #      issued round-robin on S HIP streams (independent batches: each
@@ -886,8 +893,11 @@ def main(argv=None):
     ms_per_step = gpu_ms / args.steps
     value = N_EDGES * args.steps / (gpu_ms / 1e3)
     valid = out.cpu().numpy().copy()
-    single_ms, _, _ = run_steps(1, args.steps, args.warmup) if len(picked) > 1 else (gpu_ms, wall, True)
     pair_ms, _, _ = run_steps(2, args.steps, args.warmup) if len(picked) > 2 else (gpu_ms, wall, True)
+    # one stream with the shape tuned for one stream
+    oa.set_tuning(**t_one)
+    single_ms, _, _ = run_steps(1, args.steps, args.warmup) if len(picked) > 1 else (gpu_ms, wall, True)
+    oa.set_tuning(**t)
 
     # ---- per-step latency (synchronous steps) for p50/p99
     for _ in range(min(args.steps, 200)):
@@ -991,18 +1001,22 @@ def main(argv=None):
 
     # ---- per-kernel durations: HIP events recorded by the library on the
     #      launch stream around every launch of the same K steps
-    oa.profile.reset()
-    oa.profile.enable(True)
-    for _ in range(args.steps):
-        oa.check_motions(env, A, B, lvs, out=out)
-    torch.cuda.synchronize()
-    kernels = {}
-    for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk"):
-        n, ms = oa.profile.query(name)
-        if n:
-            kernels[name] = {"launches": n, "avg_us": 1e3 * ms / n}
-    oa.profile.enable(False)
-    oa.profile.reset()
+    def kernel_times():
+        oa.profile.reset()
+        oa.profile.enable(True)
+        for _ in range(args.steps):
+            oa.check_motions(env, A, B, lvs, out=out)
+        torch.cuda.synchronize()
+        per = {}
+        for name in ("motion_fused", "motion_endpoint", "motion_walk", "motion_chunk"):
+            n, ms = oa.profile.query(name)
+            if n:
+                per[name] = {"launches": n, "avg_us": 1e3 * ms / n}
+        oa.profile.enable(False)
+        oa.profile.reset()
+        return per
+
+    kernels = kernel_times()
 
     # ---- algorithmic work in the reference order, for the roofline
     work = oa.motion_work(env, A, B, lvs)
@@ -1028,6 +1042,19 @@ def main(argv=None):
         roof_s2 = {"kernel": "motion_endpoint", "achieved": a2, "frac": a2 / PEAK_FP32_TFLOPS,
                    "kernel_avg_us": kernels["motion_endpoint"]["avg_us"], "algorithmic_flops_per_launch": flops_end,
                    "traffic": hbm_traffic("motion_endpoint", t)[0]}
+
+    # the rest (plain path, served stream, PRM, rows) with the one-stream shape
+    oa.set_tuning(**t_one)
+    # the same kernels with the one-stream shape (the shape for one batch at a time)
+    roof_one = None
+    if t_one != t:
+        k1 = kernel_times()
+        roof_one = {"tuning": t_one}
+        for kname, fl in (("motion_chunk", flops_step - flops_end), ("motion_endpoint", flops_end)):
+            if kname in k1:
+                a1 = fl / (k1[kname]["avg_us"] * 1e-6) / 1e12
+                roof_one[kname] = {"kernel_avg_us": k1[kname]["avg_us"], "achieved": a1,
+                                   "frac": a1 / PEAK_FP32_TFLOPS, "traffic": hbm_traffic(kname, t_one)[0]}
 
     # ---- plain one-launch-per-call path on a prefix of the same edges
     P = min(args.plain_edges, N_EDGES)
@@ -1128,7 +1155,7 @@ def main(argv=None):
         "streams_asked": args.streams,
         "stream_pair_checks": pair_checks,
         "one_stream": {"edge_checks_per_s": N_EDGES * args.steps / (single_ms / 1e3),
-                       "ms_per_step": single_ms / args.steps},
+                       "ms_per_step": single_ms / args.steps, "tuning": t_one},
         "two_streams": {"edge_checks_per_s": N_EDGES * args.steps / (pair_ms / 1e3),
                         "ms_per_step": pair_ms / args.steps},
         "streams_same_result": streams_same,
@@ -1161,6 +1188,7 @@ def main(argv=None):
                      "dispatches": 2 if dom == "motion_chunk" else 1,
                      "compute": "FP32 VALU (compute-bound; MI355X FP32 vector peak = FP32 matrix peak)"},
         "roofline_s2_pass": roof_s2,
+        "roofline_one_stream_shape": roof_one,
         # the whole step: the batch's reference-order work over the timed step
         # (both kernels, the streams overlapping)
         "roofline_step": {"achieved": flops_step / (ms_per_step * 1e-3) / 1e12,
@@ -1169,6 +1197,7 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "cpu_one_thread": cpu_all,
         "autotuned_us_per_batch": tuned_us,
+        "autotuned_streams_us_per_batch": streams_us,
         "parity_bit_exact": parity,
         "served_stream": stream_res,
         "prm_batch": prm_res,

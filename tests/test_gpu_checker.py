@@ -294,7 +294,8 @@ def test_check_motions_sparse_env_deep_walks(gpu, oracle, strategy, eshape, wsha
 
 @pytest.mark.parametrize("strategy,eshape,wshape,wpc", [(3, (8, 2), (8, 1), 8), (3, (4, 1), (7, 1), 32),
                                                          (3, (7, 4), (7, 9), 8), (4, (8, 2), (8, 4), 16),
-                                                         (1, (8, 1), (1, 1), 16), (2, (8, 1), (8, 1), 8)])
+                                                         (1, (8, 1), (1, 1), 16), (2, (8, 1), (8, 1), 8),
+                                                         (3, (7, 4), (7, 4), 2), (3, (7, 2), (7, 9), 4)])
 def test_check_motions_many_survivors(gpu, oracle, strategy, eshape, wshape, wpc):
     """16,384 edges in a 16 + 16 obstacle scene: about 10,000 edges pass s2,
     more than the chunk walk numbers edge-major, so its chunk-major form walks
@@ -341,6 +342,38 @@ def test_edge_major_walk_with_more_waves_than_resident(gpu, oracle, vblock, mblo
                 assert np.array_equal(gpu.check_motions(env, A, B, lvs).cpu().numpy(), ref_v), seed
     finally:
         gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8, validity_block=256, motion_block=256)
+
+
+def test_autotune_for_streams_keeps_results(gpu, oracle):
+    """The stream-pipeline autotuner tries every shape of its grid on batches
+    issued round-robin on two streams, installs one of them, and the batches
+    on either stream still equal the oracle; an empty stream list is rejected
+    and leaves the library's tuning as it was."""
+    import torch
+
+    env, oenv, _ = env_pair(gpu, oracle, 1002, 24, 24)
+    a, b = motion_case(gpu, 12, 2048, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
+    lvs = lvs_config()
+    ref_v, _ = oracle.check_motions(oenv, a, b, lvs)
+    A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]  # default priority
+    try:
+        t, us = gpu.autotune_motions_streams(env, A, B, lvs, streams, steps=6)
+        grid = gpu.STREAM_TUNING_GRID
+        assert us > 0 and t == gpu.get_tuning() and t["motion_strategy"] == 3
+        assert (t["validity_lanes_per_state"], t["validity_obstacle_parts"]) in grid["s2"]
+        assert (t["motion_lanes_per_state"], t["motion_obstacle_parts"]) in grid["walk"]
+        assert t["motion_waves_per_cu"] in grid["waves_per_cu"]
+        for s in streams:
+            v = gpu.check_motions(env, A, B, lvs, stream=s.cuda_stream)
+            torch.cuda.synchronize()
+            assert np.array_equal(v.cpu().numpy(), ref_v)
+        before = gpu.set_tuning(**DEFAULT)
+        with pytest.raises(AssertionError):
+            gpu.autotune_motions_streams(env, A, B, lvs, [])
+        assert gpu.get_tuning() == before
+    finally:
+        gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8)
 
 
 def test_env_tuning_overrides_library_tuning(gpu, oracle):
