@@ -334,7 +334,7 @@ def hbm_traffic(kernel: str, tuning: dict):
     except (OSError, ValueError):
         return None, None
     keys = ("validity_lanes_per_state", "validity_obstacle_parts", "motion_lanes_per_state",
-            "motion_obstacle_parts", "motion_strategy", "validity_shared_fk")
+            "motion_obstacle_parts", "motion_waves_per_cu", "motion_strategy", "validity_shared_fk")
     # the kernels that run for this tuning (shared FK exists for (8, parts > 1) only)
     lw, ow, le, oe = (tuning["motion_lanes_per_state"], tuning["motion_obstacle_parts"],
                       tuning["validity_lanes_per_state"], tuning["validity_obstacle_parts"])
