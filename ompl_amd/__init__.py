@@ -153,10 +153,14 @@ def autotune_motions_streams(env: "Env", a_soa, b_soa, lvs: float, streams, fact
     batches round-robin on the streams, timed by events on the first; the
     three fastest are timed again over 2 x `steps` and the fastest of those is
     kept.  Results do not depend on the shape.  The library's tuning before
-    the call is the starting point for the fields the grid leaves alone."""
+    the call is the starting point for the fields the grid leaves alone.  An
+    environment with its own shapes ignores the library's, so it is refused."""
     torch = _torch()
     streams = list(streams)
     assert streams, "at least one stream"
+    if env.tuning()["own"]:
+        raise ValueError("the environment has its own launch shapes, which take the place of the library's; "
+                         "clear them with env.set_tuning(None) first")
     n = a_soa.shape[1]
     outs = [torch.empty(n, dtype=torch.uint8, device=a_soa.device) for _ in streams]
     handles = [s.cuda_stream for s in streams]

@@ -347,8 +347,9 @@ def test_edge_major_walk_with_more_waves_than_resident(gpu, oracle, vblock, mblo
 def test_autotune_for_streams_keeps_results(gpu, oracle):
     """The stream-pipeline autotuner tries every shape of its grid on batches
     issued round-robin on two streams, installs one of them, and the batches
-    on either stream still equal the oracle; an empty stream list is rejected
-    and leaves the library's tuning as it was."""
+    on either stream still equal the oracle; an empty stream list, or an
+    environment with its own shapes, is rejected and leaves the library's
+    tuning as it was."""
     import torch
 
     env, oenv, _ = env_pair(gpu, oracle, 1002, 24, 24)
@@ -371,6 +372,10 @@ def test_autotune_for_streams_keeps_results(gpu, oracle):
         before = gpu.set_tuning(**DEFAULT)
         with pytest.raises(AssertionError):
             gpu.autotune_motions_streams(env, A, B, lvs, [])
+        env.set_tuning(motion_waves_per_cu=16)  # the environment's own shapes: refused
+        with pytest.raises(ValueError):
+            gpu.autotune_motions_streams(env, A, B, lvs, streams)
+        env.set_tuning(None)
         assert gpu.get_tuning() == before
     finally:
         gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8)
