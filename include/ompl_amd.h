@@ -44,7 +44,7 @@ int ompl_amd_init(int device);
 const char *ompl_amd_last_error(void);
 /* major*10000 + minor*100 + patch; bumped with every change of a signature or
  * struct layout, so a binding can refuse a library built from other sources. */
-#define OMPL_AMD_VERSION 312
+#define OMPL_AMD_VERSION 313
 int ompl_amd_version(void);
 
 /* Limits of this build and of the device it runs on. */
@@ -229,6 +229,21 @@ int ompl_amd_set_tuning(const ompl_amd_tuning *t);
 int ompl_amd_autotune_motions(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
                               double longest_valid_segment, unsigned factor, int reps, ompl_amd_tuning *best,
                               double *best_us);
+/* Autotuner for batches issued round-robin on several streams (1 to 16 HIP
+ * streams whose batches overlap, e.g. one per planner thread): the shape that
+ * is fastest for one batch at a time keeps more persistent walk waves than is
+ * best when the streams' batches share the device.  Each candidate (s2 pass
+ * shapes (7,1) (7,2) (7,4) (7,9) (8,4) (8,8) x walk shapes (7,2) (7,4) (7,9)
+ * (8,8) x 2/4/8/16 walk waves per CU, the chunk-parallel walk) runs `steps`
+ * batches round-robin on the streams, timed by HIP events on streams[0]; the
+ * three fastest run again over 2 x steps and the fastest is installed as the
+ * current tuning and returned with its time per batch.  Fields outside the
+ * grid keep the current tuning's values.  Refused for an environment with its
+ * own shapes (ompl_amd_env_set_tuning), which would ignore the candidates.
+ * No counterpart in the reference. */
+int ompl_amd_autotune_motions_streams(ompl_amd_env env, const double *a_soa_dev, const double *b_soa_dev, int64_t n,
+                                      double longest_valid_segment, unsigned factor, void *const *streams,
+                                      int n_streams, int steps, ompl_amd_tuning *best, double *best_us);
 /* Launch shapes of one environment: the batched checks on `env` (validity,
  * motions, last-valid motions, captured motion graphs at capture, the
  * no-allocation motion check) use `t` instead of the library's current tuning;

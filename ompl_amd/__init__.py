@@ -132,78 +132,26 @@ def autotune_motions(env: "Env", a_soa, b_soa, lvs: float, factor: int = 1, reps
     return t.as_dict(), us.value
 
 
-# Launch shapes tried for batches issued on several streams: the s2 pass's and
-# the walk's (lanes per state, obstacle parts) and the walk's waves per CU.
-STREAM_TUNING_GRID = {
-    "s2": ((7, 1), (7, 2), (7, 4), (7, 9), (8, 4), (8, 8)),
-    "walk": ((7, 2), (7, 4), (7, 9), (8, 8)),
-    "waves_per_cu": (2, 4, 8, 16),
-}
-
-
 def autotune_motions_streams(env: "Env", a_soa, b_soa, lvs: float, streams, factor: int = 1, steps: int = 200):
-    """Launch shape for batches issued round-robin on `streams` (torch streams
-    whose batches overlap); installs the fastest and returns (tuning dict,
-    microseconds per batch).
+    """Runs the library's autotuner for batches issued round-robin on
+    `streams` (torch streams whose batches overlap) and installs the fastest
+    shape; returns (tuning dict, microseconds per batch).
 
     The shape that is fastest for one batch at a time is not the fastest when
     batches of several streams share the device: fewer persistent walk waves
-    leave compute units to the other streams' s2 passes.  Each candidate of
-    STREAM_TUNING_GRID (strategy 3, the chunk-parallel walk) runs `steps`
-    batches round-robin on the streams, timed by events on the first; the
-    three fastest are timed again over 2 x `steps` and the fastest of those is
-    kept.  Results do not depend on the shape.  The library's tuning before
-    the call is the starting point for the fields the grid leaves alone.  An
-    environment with its own shapes ignores the library's, so it is refused."""
-    torch = _torch()
+    leave compute units to the other streams' s2 passes.  The library times
+    each shape of its grid (s2 pass x walk shape x walk waves per CU, the
+    chunk-parallel walk) over `steps` batches on the streams, and the three
+    fastest again over 2 x `steps`.  Results do not depend on the shape.  An
+    environment with its own shapes is refused (they would take the place of
+    every candidate)."""
     streams = list(streams)
-    assert streams, "at least one stream"
-    if env.tuning()["own"]:
-        raise ValueError("the environment has its own launch shapes, which take the place of the library's; "
-                         "clear them with env.set_tuning(None) first")
-    n = a_soa.shape[1]
-    outs = [torch.empty(n, dtype=torch.uint8, device=a_soa.device) for _ in streams]
-    handles = [s.cuda_stream for s in streams]
-
-    def per_batch(k):
-        for i in range(2 * len(handles)):  # warm: workspaces sized, kernels loaded
-            check_motions(env, a_soa, b_soa, lvs, factor, out=outs[i % len(handles)], stream=handles[i % len(handles)])
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(streams[0])
-        for s in streams[1:]:
-            s.wait_event(e0)
-        for i in range(k):
-            j = i % len(handles)
-            check_motions(env, a_soa, b_soa, lvs, factor, out=outs[j], stream=handles[j])
-        for s in streams[1:]:
-            streams[0].wait_stream(s)
-        e1.record(streams[0])
-        torch.cuda.synchronize()
-        return 1e3 * e0.elapsed_time(e1) / k
-
-    base = get_tuning()
-    timed = []
-    try:
-        for (le, oe) in STREAM_TUNING_GRID["s2"]:
-            for (lw, ow) in STREAM_TUNING_GRID["walk"]:
-                for wpc in STREAM_TUNING_GRID["waves_per_cu"]:
-                    cand = dict(base, validity_lanes_per_state=le, validity_obstacle_parts=oe,
-                                motion_lanes_per_state=lw, motion_obstacle_parts=ow, motion_waves_per_cu=wpc,
-                                motion_strategy=3, validity_shared_fk=0)
-                    set_tuning(**cand)
-                    timed.append((per_batch(steps), cand))
-        timed.sort(key=lambda x: x[0])
-        final = []
-        for _, cand in timed[:3]:
-            set_tuning(**cand)
-            final.append((per_batch(2 * steps), cand))
-        us, best = min(final, key=lambda x: x[0])
-    except BaseException:
-        set_tuning(**base)
-        raise
-    set_tuning(**best)
-    return get_tuning(), us
+    handles = (C.c_void_p * max(1, len(streams)))(*[s.cuda_stream for s in streams])
+    t = Tuning()
+    us = C.c_double()
+    call("ompl_amd_autotune_motions_streams", env.handle, _ptr(a_soa), _ptr(b_soa), a_soa.shape[1], float(lvs),
+         int(factor), handles, len(streams), int(steps), C.byref(t), C.byref(us))
+    return t.as_dict(), us.value
 
 
 def longest_valid_segment(lo, hi, resolution: float) -> float:

@@ -14,7 +14,7 @@ LIB_PATH = _PKG / "libompl_amd.so"
 
 _lib = None
 # the library's ompl_amd_version() these signatures were written for (OMPL_AMD_VERSION)
-ABI_VERSION = 312
+ABI_VERSION = 313
 
 c_int_p = C.POINTER(C.c_int)
 c_i64_p = C.POINTER(C.c_int64)
@@ -59,6 +59,8 @@ SIGNATURES = {
     "ompl_amd_env_get_tuning": (C.c_int, [vp, vp, C.POINTER(C.c_int)]),
     "ompl_amd_autotune_motions": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, C.c_int, vp,
                                             C.POINTER(C.c_double)]),
+    "ompl_amd_autotune_motions_streams": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp, C.c_int,
+                                                   C.c_int, vp, vp]),
     "ompl_amd_motion_work": (C.c_int, [vp, vp, vp, C.c_int64, C.c_double, C.c_uint, vp]),
     "ompl_amd_validity_work": (C.c_int, [vp, vp, C.c_int64, vp]),
     "ompl_amd_profile_enable": (C.c_int, [C.c_int]),

@@ -618,6 +618,121 @@ extern "C" int ompl_amd_autotune_motions(ompl_amd_env e, const double *a, const 
     });
 }
 
+namespace {
+
+// Device buffers and events of one stream-pipeline autotune, released on
+// every exit.
+struct PipelineScratch {
+    uint8_t *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> join;
+    ~PipelineScratch()
+    {
+        if (out) (void)hipFree(out);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (hipEvent_t e : join) (void)hipEventDestroy(e);
+    }
+};
+
+// Launch shapes tried for batches on several streams: the s2 pass's and the
+// walk's (lanes per state, obstacle parts) and the walk's waves per CU, all
+// with the chunk-parallel walk (strategy 3).
+std::vector<Tuning> stream_candidates(const Tuning &base)
+{
+    const int s2[][2] = {{7, 1}, {7, 2}, {7, 4}, {7, 9}, {8, 4}, {8, 8}};
+    const int walk[][2] = {{7, 2}, {7, 4}, {7, 9}, {8, 8}};
+    std::vector<Tuning> c;
+    for (const auto &e : s2)
+        for (const auto &w : walk)
+            for (int wpc : {2, 4, 8, 16}) {
+                Tuning t = base;
+                t.validity_lanes_per_state = e[0];
+                t.validity_obstacle_parts = e[1];
+                t.motion_lanes_per_state = w[0];
+                t.motion_obstacle_parts = w[1];
+                t.motion_waves_per_cu = wpc;
+                t.motion_strategy = 3;
+                t.validity_shared_fk = 0;
+                c.push_back(t);
+            }
+    return c;
+}
+
+} // namespace
+
+extern "C" int ompl_amd_autotune_motions_streams(ompl_amd_env e, const double *a, const double *b, int64_t n,
+                                                 double lvs, unsigned factor, void *const *streams, int n_streams,
+                                                 int steps, ompl_amd_tuning *best, double *best_us)
+{
+    return guarded([&] {
+        const Env &env = env_of(e);
+        OA_REQUIRE(n > 0 && a && b && steps >= 1, OMPL_AMD_ERR_ARG, "bad autotune arguments");
+        OA_REQUIRE(streams && n_streams >= 1 && n_streams <= 16, OMPL_AMD_ERR_ARG, "1 to 16 streams");
+        {
+            std::lock_guard<std::mutex> lk(e->tuning_mutex);
+            OA_REQUIRE(!e->has_tuning, OMPL_AMD_ERR_ARG,
+                       "the environment has its own launch shapes, which take the place of the library's");
+        }
+        std::vector<hipStream_t> st((size_t)n_streams);
+        for (int j = 0; j < n_streams; ++j) st[(size_t)j] = resolve_stream(streams[j]);
+        PipelineScratch S;
+        OA_HIP(hipMalloc(&S.out, (size_t)n * (size_t)n_streams));
+        OA_HIP(hipEventCreate(&S.e0));
+        OA_HIP(hipEventCreate(&S.e1));
+        for (int j = 0; j < n_streams; ++j) {
+            hipEvent_t ev = nullptr;
+            OA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            S.join.push_back(ev);
+        }
+        // k batches round-robin on the streams, between events on the first,
+        // which waits for the others at the end; microseconds per batch
+        auto per_batch = [&](const Tuning &t, int k) {
+            for (int i = 0; i < 2 * n_streams; ++i) { // warm: workspaces sized, kernels loaded
+                const int j = i % n_streams;
+                check_motions_tuned(t, env, a, b, n, lvs, factor, S.out + (size_t)j * (size_t)n, nullptr,
+                                    st[(size_t)j]);
+            }
+            for (hipStream_t s : st) OA_HIP(hipStreamSynchronize(s));
+            OA_HIP(hipEventRecord(S.e0, st[0]));
+            for (int j = 1; j < n_streams; ++j) OA_HIP(hipStreamWaitEvent(st[(size_t)j], S.e0, 0));
+            for (int i = 0; i < k; ++i) {
+                const int j = i % n_streams;
+                check_motions_tuned(t, env, a, b, n, lvs, factor, S.out + (size_t)j * (size_t)n, nullptr,
+                                    st[(size_t)j]);
+            }
+            for (int j = 1; j < n_streams; ++j) {
+                OA_HIP(hipEventRecord(S.join[(size_t)j], st[(size_t)j]));
+                OA_HIP(hipStreamWaitEvent(st[0], S.join[(size_t)j], 0));
+            }
+            OA_HIP(hipEventRecord(S.e1, st[0]));
+            OA_HIP(hipEventSynchronize(S.e1));
+            float ms = 0.0f;
+            OA_HIP(hipEventElapsedTime(&ms, S.e0, S.e1));
+            return 1e3 * (double)ms / k;
+        };
+        std::vector<std::pair<double, Tuning>> timed;
+        for (const Tuning &t : stream_candidates(get_tuning())) timed.emplace_back(per_batch(t, steps), t);
+        std::stable_sort(timed.begin(), timed.end(),
+                         [](const std::pair<double, Tuning> &x, const std::pair<double, Tuning> &y) {
+                             return x.first < y.first;
+                         });
+        // the three fastest again over twice the batches
+        double win_us = INFINITY;
+        Tuning win = timed.front().second;
+        for (size_t i = 0; i < timed.size() && i < 3; ++i) {
+            const double us = per_batch(timed[i].second, 2 * steps);
+            if (us < win_us) {
+                win_us = us;
+                win = timed[i].second;
+            }
+        }
+        set_tuning(win);
+        if (best) *best = tuning_to_c(win);
+        if (best_us) *best_us = win_us;
+    });
+}
+
 extern "C" {
 
 int ompl_amd_motion_graph_create(ompl_amd_env e, const double *a, const double *b, int64_t n, double lvs,

@@ -352,6 +352,8 @@ def test_autotune_for_streams_keeps_results(gpu, oracle):
     tuning as it was."""
     import torch
 
+    from ompl_amd._lib import OmplAmdError
+
     env, oenv, _ = env_pair(gpu, oracle, 1002, 24, 24)
     a, b = motion_case(gpu, 12, 2048, mu=-0.5, sigma=1.0, lo=0.05, hi=5.0)
     lvs = lvs_config()
@@ -359,26 +361,28 @@ def test_autotune_for_streams_keeps_results(gpu, oracle):
     A, B = to_dev(a.T.copy()), to_dev(b.T.copy())
     streams = [torch.cuda.Stream(), torch.cuda.Stream()]  # default priority
     try:
+        gpu.set_tuning(**DEFAULT, validity_block=512)
         t, us = gpu.autotune_motions_streams(env, A, B, lvs, streams, steps=6)
-        grid = gpu.STREAM_TUNING_GRID
-        assert us > 0 and t == gpu.get_tuning() and t["motion_strategy"] == 3
-        assert (t["validity_lanes_per_state"], t["validity_obstacle_parts"]) in grid["s2"]
-        assert (t["motion_lanes_per_state"], t["motion_obstacle_parts"]) in grid["walk"]
-        assert t["motion_waves_per_cu"] in grid["waves_per_cu"]
+        assert us > 0 and t == gpu.get_tuning() and t["motion_strategy"] == 3 and t["validity_shared_fk"] == 0
+        assert (t["validity_lanes_per_state"], t["validity_obstacle_parts"]) in \
+            {(7, 1), (7, 2), (7, 4), (7, 9), (8, 4), (8, 8)}
+        assert (t["motion_lanes_per_state"], t["motion_obstacle_parts"]) in {(7, 2), (7, 4), (7, 9), (8, 8)}
+        assert t["motion_waves_per_cu"] in {2, 4, 8, 16}
+        assert t["validity_block"] == 512  # fields outside the grid keep the tuning's values
         for s in streams:
             v = gpu.check_motions(env, A, B, lvs, stream=s.cuda_stream)
             torch.cuda.synchronize()
             assert np.array_equal(v.cpu().numpy(), ref_v)
-        before = gpu.set_tuning(**DEFAULT)
-        with pytest.raises(AssertionError):
+        before = gpu.set_tuning(**DEFAULT, validity_block=256)
+        with pytest.raises(OmplAmdError):
             gpu.autotune_motions_streams(env, A, B, lvs, [])
         env.set_tuning(motion_waves_per_cu=16)  # the environment's own shapes: refused
-        with pytest.raises(ValueError):
+        with pytest.raises(OmplAmdError):
             gpu.autotune_motions_streams(env, A, B, lvs, streams)
         env.set_tuning(None)
         assert gpu.get_tuning() == before
     finally:
-        gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8)
+        gpu.set_tuning(**DEFAULT, motion_waves_per_cu=8, validity_block=256)
 
 
 def test_env_tuning_overrides_library_tuning(gpu, oracle):
