@@ -10,13 +10,16 @@ exit.  Inputs are resident in HBM before the timed region.
 
 Reported (one JSON line):
   value            batched edge-checks/s (edges validated per second), the K
-                   steps issued round-robin on --streams HIP streams (4):
-                   independent batches, each fully checked; the one- and
-                   two-stream figures are reported beside it
+                   steps issued round-robin on --streams HIP streams (4, kept
+                   only if their batches overlap), with the launch shape tuned
+                   for those streams: independent batches, each fully checked;
+                   the one- and two-stream figures are reported beside it
   plain            the same edges through the one-launch-per-call path
   roofline         dominant kernel: algorithmic FP32 work in the reference order
-                   / its average duration (HIP events on its launch stream)
-  cpu_baseline     the CPU restatement (single thread) on a bounded sample
+                   / its average duration (HIP events on its launch stream),
+                   for the headline's shape and for the one-stream shape
+  cpu_baseline     the CPU restatement on the host's 16-thread share, on a
+                   bounded sample (one thread beside it)
   parity           batched results bit-exact vs the CPU restatement
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
